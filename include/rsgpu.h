@@ -1,0 +1,106 @@
+/*
+ * rsgpu.h - C ABI of the MI355X linear-solve and stamping core.
+ *
+ * Drop-in boundary for RustSpice's `trait LinearSolver`
+ * (crates/sim-core/src/solver.rs:141-166). A Rust `impl LinearSolver for
+ * GpuSolver` (shown in INTEGRATION.md) forwards each trait method to the
+ * function named beside it. Slices become pointer + length; every entry point
+ * returns an rs_status that maps one-to-one onto `SolverError`
+ * (solver.rs:76-92). Handles are single-owner and not thread-safe, matching
+ * the trait's `&mut self` methods; a handle may move between threads (`Send`).
+ *
+ * All numerics are FP64. Indices are int64 on the API and int32 on the device
+ * (checked at analyze time).
+ */
+#ifndef RSGPU_H
+#define RSGPU_H
+
+#include <stdint.h>
+
+#include "rs_circuit.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: SolverError variants (solver.rs:76-92) ---------------- */
+typedef enum rs_status {
+    RS_OK = 0,
+    RS_ERR_ANALYZE_FAILED = 1,   /* SolverError::AnalyzeFailed           */
+    RS_ERR_FACTOR_FAILED = 2,    /* SolverError::FactorFailed            */
+    RS_ERR_SOLVE_FAILED = 3,     /* SolverError::SolveFailed             */
+    RS_ERR_SINGULAR = 4,         /* SolverError::SingularMatrix{pivot}   */
+    RS_ERR_ILL_CONDITIONED = 5,  /* SolverError::IllConditioned{rcond}   */
+    RS_ERR_INVALID_MATRIX = 6,   /* SolverError::InvalidMatrix{reason}   */
+    RS_ERR_DEVICE = 7            /* HIP runtime error (no reference twin) */
+} rs_status;
+
+/* ---- solver kinds: SolverType variants this library implements -------------
+ * (solver.rs:121-139, factory solver.rs:169-200). Both share the GPU kernels;
+ * they differ only in host symbolic choices, exactly as the reference solvers
+ * do. */
+typedef enum rs_solver_kind {
+    RS_SOLVER_NATIVEKLU = 0, /* NativeKluSolver: BTF + per-block AMD, threshold
+                                pivoting on the first factor, pivots reused by
+                                refactors (native_klu.rs)                        */
+    RS_SOLVER_SPARSELU = 1   /* SparseLuSolver: AMD, no pivoting, 1e-14 pivot
+                                perturbation (sparse_lu.rs)                      */
+} rs_solver_kind;
+
+typedef struct rs_solver rs_solver;
+
+/* NativeKluStats (native_klu.rs:584-594) plus GPU counters. */
+typedef struct rs_solver_stats {
+    int64_t factor_count;    /* full (pivoting) factors                     */
+    int64_t refactor_count;  /* refactors with reused pivots                */
+    int64_t nnz_l, nnz_u, nblocks, noffdiag;
+    double rcond;            /* min|u_ii| / max|u_ii| (native_klu.rs:1406)  */
+    int64_t parallel_factor; /* 1 when the last factor ran on the GPU       */
+    int64_t n_factor_levels, max_level_width, n_lsolve_levels, n_usolve_levels;
+    double refactor_flops, solve_flops; /* algorithmic work on the pattern  */
+    int64_t plan_bytes;      /* device bytes of the index plan               */
+} rs_solver_stats;
+
+/* create_solver(kind, n) (solver.rs:169-200) */
+rs_solver *rs_solver_create(int32_t kind, int64_t n);
+void rs_solver_destroy(rs_solver *h);
+/* LinearSolver::prepare (native_klu.rs:1604-1610): a new n resets the pattern */
+void rs_prepare(rs_solver *h, int64_t n);
+/* LinearSolver::analyze (native_klu.rs:1612-1630): cached while (ap, ai)
+ * are unchanged; ap_len must be n + 1 */
+int32_t rs_analyze(rs_solver *h, const int64_t *ap, int64_t ap_len, const int64_t *ai,
+                   int64_t ai_len);
+/* LinearSolver::factor (native_klu.rs:1632-1639): re-analyzes on a pattern
+ * change; the first factor of a pattern picks pivots (host), later factors
+ * of the same pattern refactor on the GPU with those pivots. ax is the
+ * unmerged CSC value array (duplicates summed in order). */
+int32_t rs_factor(rs_solver *h, const int64_t *ap, int64_t ap_len, const int64_t *ai,
+                  int64_t ai_len, const double *ax, int64_t ax_len);
+/* LinearSolver::solve (native_klu.rs:1641-1657): rhs is overwritten with x */
+int32_t rs_solve(rs_solver *h, double *rhs, int64_t n);
+/* LinearSolver::reset_pattern (native_klu.rs:1661-1672) */
+void rs_reset_pattern(rs_solver *h);
+/* LinearSolver::name / set_parallel_threads (solver.rs:158-165). The thread
+ * count is recorded and ignored: the GPU schedule is fixed by the pattern. */
+const char *rs_solver_name(const rs_solver *h);
+void rs_set_parallel_threads(rs_solver *h, int64_t threads);
+/* NativeKluSolver::set_pivot_tolerance / set_ordering / set_btf
+ * (native_klu.rs:664-681) */
+void rs_set_config(rs_solver *h, double pivot_tol, int32_t ordering, int32_t use_btf);
+int32_t rs_get_stats(const rs_solver *h, rs_solver_stats *out);
+/* Text of the last error (SolverError Display, solver.rs:94-117). */
+int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);
+
+/* ---- device-resident hot path (no reference counterpart: the reference has
+ * no device boundary). The pattern must be the one last factored; pointers
+ * are device memory; `stream` is a hipStream_t (NULL = default stream). ---- */
+int32_t rs_refactor_device(rs_solver *h, const double *ax_dev, void *stream);
+int32_t rs_solve_device(rs_solver *h, const double *rhs_dev, double *x_dev, void *stream);
+/* Copies the current factor values to host arrays sized by rs_get_stats
+ * (lx[nnz_l], ux[nnz_u], udiag[n]); position-space order. */
+int32_t rs_get_factor_values(rs_solver *h, double *lx, double *ux, double *udiag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+/*
+ * rsgpu_host.h - host-only entry points of librsgpu.so: the integer analysis
+ * and the first (pivoting) factor that the GPU solver runs on the CPU before
+ * handing refactors and solves to the device. They allocate no device memory,
+ * so they can be exercised on any machine.
+ *
+ * Reference counterparts:
+ *   rs_host_amd_order     - amd::amd_order (crates/sim-core/src/amd.rs:143-161)
+ *   rs_host_btf           - btf::btf_decompose (crates/sim-core/src/btf.rs:143-175)
+ *   rs_host_factor_create - NativeKluSolver::do_analyze + the first do_factor
+ *                           (native_klu.rs:704-761, 1020-1148); kind 1 gives
+ *                           SparseLuSolver's no-pivot factor (sparse_lu.rs:302-537)
+ */
+#ifndef RSGPU_HOST_H
+#define RSGPU_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void rs_host_amd_order(int64_t n, const int64_t *ap, const int64_t *ai, int64_t *perm,
+                       int64_t *inv_perm);
+/* returns num_blocks; block_ptr needs n + 1 entries */
+int64_t rs_host_btf(int64_t n, const int64_t *ap, const int64_t *ai, int64_t *row_perm,
+                    int64_t *col_perm, int64_t *block_ptr, int64_t *structural_rank);
+
+typedef struct rs_host_factor rs_host_factor;
+rs_host_factor *rs_host_factor_create(int32_t kind, int64_t n, const int64_t *ap,
+                                      const int64_t *ai, const double *ax, double pivot_tol,
+                                      int32_t ordering, int32_t use_btf);
+void rs_host_factor_destroy(rs_host_factor *h);
+int64_t rs_host_factor_nblocks(const rs_host_factor *h);
+void rs_host_factor_block_sizes(const rs_host_factor *h, int64_t b, int64_t *n, int64_t *nl,
+                                int64_t *nu);
+void rs_host_factor_block_get(const rs_host_factor *h, int64_t b, int64_t *col_perm,
+                              int64_t *row_perm, int64_t *lcp, int64_t *lri, int64_t *ucp,
+                              int64_t *uri, double *lx, double *ux, double *udiag);
+/* out[8] = {factor levels, max level width, L-solve levels, U-solve levels,
+ *           scatter runs, off-diagonal entries, nnz(L), nnz(U)} */
+void rs_host_factor_plan_stats(const rs_host_factor *h, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

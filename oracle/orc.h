@@ -1,0 +1,134 @@
+/*
+ * orc.h - CPU oracle: a plain-C restatement of RustSpice's DC/TRAN hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
+ * calls this code; it exists so tests, the smoke check and the CPU baseline
+ * leg of the benchmark can compare the GPU path against the reference's
+ * arithmetic. Every function follows one reference file, cited at its
+ * definition; the restatement keeps the reference's loop order, tie-breaks
+ * and quirks (unmerged CSC, ground kept in the matrix, 1e-14 pivot clamps,
+ * first-factor threshold pivoting inside the static pattern, BE sign, etc.).
+ *
+ * Parity pinning (what checks each part) is listed in oracle/README.md.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include <stdint.h>
+#include "../include/rs_circuit.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- AMD (crates/sim-core/src/amd.rs:143-411) -------------------------- */
+void orc_amd_order(int64_t n, const int64_t *ap, const int64_t *ai,
+                   int64_t *perm, int64_t *inv_perm);
+
+/* ---- BTF (crates/sim-core/src/btf.rs:143-550) -------------------------- */
+typedef struct orc_btf {
+    int64_t n;
+    int64_t *row_perm, *col_perm, *row_perm_inv, *col_perm_inv;
+    int64_t *block_ptr; /* num_blocks + 1 */
+    int64_t num_blocks, structural_rank, num_singletons, max_block_size;
+} orc_btf;
+void orc_btf_decompose(int64_t n, const int64_t *ap, const int64_t *ai, orc_btf *out);
+void orc_btf_free(orc_btf *b);
+
+/* ---- NativeKLU (crates/sim-core/src/native_klu.rs) --------------------- */
+typedef struct orc_klu orc_klu;
+orc_klu *orc_klu_new(int64_t n);
+void orc_klu_free(orc_klu *s);
+void orc_klu_prepare(orc_klu *s, int64_t n);
+int orc_klu_analyze(orc_klu *s, int64_t ap_len, const int64_t *ap, int64_t ai_len,
+                    const int64_t *ai);
+int orc_klu_factor(orc_klu *s, int64_t ap_len, const int64_t *ap, int64_t ai_len,
+                   const int64_t *ai, const double *ax);
+int orc_klu_solve(orc_klu *s, int64_t n, double *rhs);
+void orc_klu_reset_pattern(orc_klu *s);
+void orc_klu_set_config(orc_klu *s, double pivot_tol, int ordering, int use_btf);
+/* stats: factor_count, refactor_count, nnz_l, nnz_u, nblocks, noffdiag */
+void orc_klu_stats(const orc_klu *s, int64_t out[6], double *rcond);
+/* symbolic/numeric of block b (for bit-exact comparisons) */
+int64_t orc_klu_block_n(const orc_klu *s, int64_t b);
+void orc_klu_block_sizes(const orc_klu *s, int64_t b, int64_t *nl, int64_t *nu);
+void orc_klu_block_get(const orc_klu *s, int64_t b, int64_t *col_perm, int64_t *row_perm,
+                       int64_t *l_col_ptr, int64_t *l_row_idx, int64_t *u_col_ptr,
+                       int64_t *u_row_idx, double *l_values, double *u_values,
+                       double *u_diag);
+/* BTF of the last analyze (nblocks==1 -> no BTF) */
+int orc_klu_btf_get(const orc_klu *s, int64_t *row_perm, int64_t *col_perm,
+                    int64_t *block_ptr);
+
+/* ---- MNA assembly + stamps (mna.rs, stamp.rs, bsim/evaluate.rs) -------- */
+enum { ORC_MODE_DC = 0, ORC_MODE_TRAN = 1 };
+enum { ORC_BE = 0, ORC_TRAP = 1 };
+
+typedef struct orc_tstate {
+    /* keyed by device index (the reference keys by unique instance name) */
+    double *cap_v;  /* cap_voltage */
+    double *cap_i;  /* cap_current */
+    uint8_t *has_v, *has_i;
+    int method;
+} orc_tstate;
+
+typedef struct orc_mna {
+    int64_t n, nnz;
+    int64_t *ap, *ai;
+    double *ax, *rhs;
+} orc_mna;
+
+/* Builds the unmerged CSC + RHS exactly as the engine closure does
+ * (engine.rs:262-283 for DC, :485-501 for TRAN). x has x_len entries. */
+void orc_build_mna(const rs_device *dev, int64_t ndev, int64_t node_count,
+                   const double *x, int64_t x_len, int mode, double t, double dt,
+                   const orc_tstate *st, double gmin, double source_scale,
+                   orc_mna *out);
+void orc_mna_free(orc_mna *m);
+
+/* level-1 MOSFET (bsim/evaluate.rs:285-356 via bsim/mod.rs:441-455):
+ * out = {ids, gm, gds, gmbs, ieq} */
+void orc_level1(const rs_device *d, double vd, double vg, double vs, double vb,
+                double out[5]);
+double orc_pulse_eval(const double p[7], double t);
+
+/* ---- Newton + engine (newton.rs, engine.rs, analysis.rs, waveform.rs) -- */
+typedef struct orc_newton_result {
+    int32_t converged, iterations, reason; /* reason: 0 conv, 1 maxiters, 2 solver */
+    double final_norm;
+} orc_newton_result;
+
+/* DC operating point (engine.rs:249-311). x_out gets n entries on convergence
+ * (the full vector incl. aux, newton.rs:98), *x_len its length. */
+int orc_dc_op(const rs_device *dev, int64_t ndev, int64_t node_count,
+              const rs_newton_opts *o, orc_klu *solver, double *x_out,
+              int64_t x_cap, int64_t *x_len, orc_newton_result *res);
+
+typedef struct orc_tran_result {
+    int32_t status;       /* 0 converged, 2 failed (RunStatus) */
+    int32_t accepted, rejected, newton_iters_total, factor_calls;
+    int32_t dc_iters;
+    int64_t npoints;      /* stored points incl. t=tstart */
+} orc_tran_result;
+
+/* TRAN (engine.rs:333-632). times[npoints], sols[npoints*x_stride] */
+int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count,
+             const rs_newton_opts *o, double tstep, double tstop, double tstart,
+             double tmax, orc_klu *solver, int64_t max_points, double *times,
+             double *sols, int64_t x_stride, orc_tran_result *res);
+
+/* Fixed-step trapezoidal transient (no reference counterpart): one TRAP
+ * Newton solve per step of dt, state updated as on an accepted step. */
+int orc_tran_fixed(const rs_device *dev, int64_t ndev, int64_t node_count,
+                   const rs_newton_opts *o, double dt, int64_t nsteps,
+                   orc_klu *solver, double *x_final, int64_t x_cap,
+                   int64_t *x_len, int32_t *newton_iters_total);
+
+/* analysis.rs:147-200: out = {error_norm, max_node, accept, factor, raw_lte} */
+void orc_lte_milne(const double *xb, const double *xt, int64_t n, double abs_tol,
+                   double rel_tol, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

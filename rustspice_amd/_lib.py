@@ -1,0 +1,96 @@
+"""ctypes binding of librsgpu.so (include/rsgpu.h, include/rsgpu_host.h).
+
+The library is built in-tree by ``make -C rustspice_amd/csrc`` (or
+``__graft_entry__.build()``). Importing the GPU entry points without it
+raises: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librsgpu.so")
+_LIB = None
+
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+#: symbols declared in include/rsgpu.h and include/rsgpu_host.h
+EXPORTS = [
+    "rs_solver_create", "rs_solver_destroy", "rs_prepare", "rs_analyze", "rs_factor",
+    "rs_solve", "rs_reset_pattern", "rs_solver_name", "rs_set_parallel_threads",
+    "rs_set_config", "rs_get_stats", "rs_last_error", "rs_refactor_device",
+    "rs_solve_device", "rs_get_factor_values",
+    "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
+    "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
+    "rs_host_factor_plan_stats",
+]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [
+        ("factor_count", C.c_int64), ("refactor_count", C.c_int64), ("nnz_l", C.c_int64),
+        ("nnz_u", C.c_int64), ("nblocks", C.c_int64), ("noffdiag", C.c_int64),
+        ("rcond", C.c_double), ("parallel_factor", C.c_int64),
+        ("n_factor_levels", C.c_int64), ("max_level_width", C.c_int64),
+        ("n_lsolve_levels", C.c_int64), ("n_usolve_levels", C.c_int64),
+        ("refactor_flops", C.c_double), ("solve_flops", C.c_double), ("plan_bytes", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Loads librsgpu.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C rustspice_amd/csrc` "
+            "(the GPU solver has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    L.rs_solver_create.restype = vp
+    L.rs_solver_create.argtypes = [C.c_int32, C.c_int64]
+    L.rs_solver_destroy.argtypes = [vp]
+    L.rs_prepare.argtypes = [vp, C.c_int64]
+    L.rs_analyze.restype = C.c_int32
+    L.rs_analyze.argtypes = [vp, i64p, C.c_int64, i64p, C.c_int64]
+    L.rs_factor.restype = C.c_int32
+    L.rs_factor.argtypes = [vp, i64p, C.c_int64, i64p, C.c_int64, f64p, C.c_int64]
+    L.rs_solve.restype = C.c_int32
+    L.rs_solve.argtypes = [vp, f64p, C.c_int64]
+    L.rs_reset_pattern.argtypes = [vp]
+    L.rs_solver_name.restype = C.c_char_p
+    L.rs_solver_name.argtypes = [vp]
+    L.rs_set_parallel_threads.argtypes = [vp, C.c_int64]
+    L.rs_set_config.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
+    L.rs_get_stats.restype = C.c_int32
+    L.rs_get_stats.argtypes = [vp, C.POINTER(SolverStats)]
+    L.rs_last_error.restype = C.c_int32
+    L.rs_last_error.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.rs_refactor_device.restype = C.c_int32
+    L.rs_refactor_device.argtypes = [vp, vp, vp]
+    L.rs_solve_device.restype = C.c_int32
+    L.rs_solve_device.argtypes = [vp, vp, vp, vp]
+    L.rs_get_factor_values.restype = C.c_int32
+    L.rs_get_factor_values.argtypes = [vp, f64p, f64p, f64p]
+    L.rs_host_amd_order.argtypes = [C.c_int64, i64p, i64p, i64p, i64p]
+    L.rs_host_btf.restype = C.c_int64
+    L.rs_host_btf.argtypes = [C.c_int64, i64p, i64p, i64p, i64p, i64p, C.POINTER(C.c_int64)]
+    L.rs_host_factor_create.restype = vp
+    L.rs_host_factor_create.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, C.c_double,
+                                        C.c_int32, C.c_int32]
+    L.rs_host_factor_destroy.argtypes = [vp]
+    L.rs_host_factor_nblocks.restype = C.c_int64
+    L.rs_host_factor_nblocks.argtypes = [vp]
+    L.rs_host_factor_block_sizes.argtypes = [vp, C.c_int64] + [C.POINTER(C.c_int64)] * 3
+    L.rs_host_factor_block_get.argtypes = [vp, C.c_int64] + [vp] * 9
+    L.rs_host_factor_plan_stats.argtypes = [vp, i64p]
+    _LIB = L
+    return L

@@ -1,0 +1,140 @@
+// hostapi.cpp - C entry points for the host-side integer analysis (AMD, BTF,
+// L/U patterns) and the first pivoting factor, so they can be checked on a
+// machine without a GPU. They touch no device memory.
+#include <cstring>
+#include <vector>
+
+#include "../../include/rsgpu_host.h"
+#include "plan.hpp"
+
+using namespace rsgpu;
+
+struct rs_host_factor {
+    HostFactor f;
+    PlanHost p;
+};
+
+extern "C" {
+
+void rs_host_amd_order(int64_t n, const int64_t* ap, const int64_t* ai, int64_t* perm,
+                       int64_t* inv_perm) {
+    AmdResult r = amd_order(n, ap, ai);
+    if (n) {
+        std::memcpy(perm, r.perm.data(), n * sizeof(int64_t));
+        std::memcpy(inv_perm, r.inv_perm.data(), n * sizeof(int64_t));
+    }
+}
+
+int64_t rs_host_btf(int64_t n, const int64_t* ap, const int64_t* ai, int64_t* row_perm,
+                    int64_t* col_perm, int64_t* block_ptr, int64_t* structural_rank) {
+    Btf b = btf_decompose(n, ap, ai);
+    if (n) {
+        std::memcpy(row_perm, b.row_perm.data(), n * sizeof(int64_t));
+        std::memcpy(col_perm, b.col_perm.data(), n * sizeof(int64_t));
+    }
+    std::memcpy(block_ptr, b.block_ptr.data(), b.block_ptr.size() * sizeof(int64_t));
+    if (structural_rank) *structural_rank = b.structural_rank;
+    return b.num_blocks;
+}
+
+rs_host_factor* rs_host_factor_create(int32_t kind, int64_t n, const int64_t* ap,
+                                      const int64_t* ai, const double* ax, double pivot_tol,
+                                      int32_t ordering, int32_t use_btf) {
+    rs_host_factor* h = new rs_host_factor();
+    HostFactor& f = h->f;
+    f.n = n;
+    const bool natural = ordering == 3;
+    const bool sparselu = kind == 1;
+    const double tol = sparselu ? 0.0 : pivot_tol;
+    if (!sparselu && use_btf && n >= 4) {
+        f.btf = btf_decompose(n, ap, ai);
+        f.has_btf = f.btf.num_blocks >= 2 && f.btf.max_block_size < n;
+    }
+    if (f.has_btf) {
+        const i64 nb = f.btf.num_blocks;
+        f.sym.resize(nb);
+        f.num.resize(nb);
+        for (i64 b = 0; b < nb; b++) {
+            i64 bs = f.btf.block_ptr[b], be = f.btf.block_ptr[b + 1], sz = be - bs;
+            std::vector<i64> bap(sz + 1, 0), bai;
+            std::vector<double> bax;
+            for (i64 lc = 0; lc < sz; lc++) {
+                i64 gco = f.btf.col_perm[bs + lc];
+                for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
+                    i64 gro = ai[q];
+                    if (gro >= n) continue;
+                    i64 grn = f.btf.row_perm_inv[gro];
+                    if (grn >= bs && grn < be) {
+                        bai.push_back(grn - bs);
+                        bax.push_back(ax[q]);
+                    }
+                }
+                bap[lc + 1] = (i64)bai.size();
+            }
+            block_symbolic(sz, bap.data(), bai.data(), natural, f.sym[b]);
+            factor_block_host(f.sym[b], bap.data(), bai.data(), bax.data(), nullptr, false, tol,
+                              f.num[b]);
+        }
+    } else {
+        f.sym.resize(1);
+        f.num.resize(1);
+        block_symbolic(n, ap, ai, natural, f.sym[0]);
+        factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0]);
+    }
+    build_plan(f, ap, ai, h->p);
+    return h;
+}
+
+void rs_host_factor_destroy(rs_host_factor* h) { delete h; }
+
+int64_t rs_host_factor_nblocks(const rs_host_factor* h) { return (int64_t)h->f.sym.size(); }
+
+void rs_host_factor_block_sizes(const rs_host_factor* h, int64_t b, int64_t* n, int64_t* nl,
+                                int64_t* nu) {
+    const BlockSymbolic& s = h->f.sym[b];
+    *n = s.n;
+    *nl = (int64_t)s.lri.size();
+    *nu = (int64_t)s.uri.size();
+}
+
+void rs_host_factor_block_get(const rs_host_factor* h, int64_t b, int64_t* col_perm,
+                              int64_t* row_perm, int64_t* lcp, int64_t* lri, int64_t* ucp,
+                              int64_t* uri, double* lx, double* ux, double* udiag) {
+    const BlockSymbolic& s = h->f.sym[b];
+    const BlockNumeric& m = h->f.num[b];
+    const i64 n = s.n;
+    auto cp = [](int64_t* dst, const std::vector<i64>& v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(int64_t));
+    };
+    auto cpd = [](double* dst, const std::vector<double>& v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+    };
+    cp(col_perm, s.col_perm);
+    cp(row_perm, m.row_perm);
+    cp(lcp, s.lcp);
+    cp(lri, s.lri);
+    cp(ucp, s.ucp);
+    cp(uri, s.uri);
+    if (n == 1) {
+        if (ux) ux[0] = m.ux[0];
+        if (udiag) udiag[0] = m.udiag[0];
+        return;
+    }
+    cpd(lx, m.lx);
+    cpd(ux, m.ux);
+    cpd(udiag, m.udiag);
+}
+
+void rs_host_factor_plan_stats(const rs_host_factor* h, int64_t* out) {
+    const PlanHost& p = h->p;
+    out[0] = p.nflev;
+    out[1] = p.max_flev_width;
+    out[2] = p.n_lsl;
+    out[3] = p.n_usl;
+    out[4] = p.nsc;
+    out[5] = p.nod;
+    out[6] = p.nl;
+    out[7] = p.nu;
+}
+
+}  // extern "C"

@@ -1,0 +1,336 @@
+// plan.cpp - builds the flat device plan of a factorization; see plan.hpp.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace rsgpu {
+
+namespace {
+
+void group_by_level(const std::vector<int32_t>& lvl, const std::vector<int32_t>& items,
+                    std::vector<int32_t>& ptr, std::vector<int32_t>& out) {
+    int32_t nlev = 0;
+    for (int32_t it : items) nlev = std::max(nlev, lvl[it] + 1);
+    std::vector<int32_t> cnt(nlev + 1, 0);
+    for (int32_t it : items) cnt[lvl[it] + 1]++;
+    for (int32_t l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+    size_t base = out.size();
+    out.resize(base + items.size());
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int32_t it : items) out[base + fill[lvl[it]]++] = it;
+    int32_t pbase = ptr.empty() ? 0 : ptr.back();
+    if (ptr.empty()) ptr.push_back(0);
+    for (int32_t l = 0; l < nlev; l++) ptr.push_back(pbase + cnt[l + 1]);
+    (void)pbase;
+}
+
+int32_t chk32(i64 v) {
+    if (v < 0 || v > INT32_MAX) throw std::runtime_error("plan index exceeds int32 range");
+    return (int32_t)v;
+}
+
+}  // namespace
+
+void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p) {
+    const i64 n = f.n;
+    p = PlanHost();
+    p.n = chk32(n);
+    const i64 nb = (i64)f.sym.size();
+    p.nblk = (int32_t)nb;
+    std::vector<i64> bstart(nb + 1, 0);
+    if (f.has_btf) {
+        for (i64 b = 0; b <= nb; b++) bstart[b] = f.btf.block_ptr[b];
+    } else {
+        bstart[1] = n;
+    }
+    for (i64 b = 0; b <= nb; b++) p.blk_ptr.push_back(chk32(bstart[b]));
+    // block of each position
+    std::vector<int32_t> blk_of(n, 0);
+    for (i64 b = 0; b < nb; b++)
+        for (i64 g = bstart[b]; g < bstart[b + 1]; g++) blk_of[g] = (int32_t)b;
+
+    // 1. position-space L/U CSC
+    p.lcp.assign(n + 1, 0);
+    p.ucp.assign(n + 1, 0);
+    p.colflag.assign(n, 0);
+    for (i64 b = 0; b < nb; b++) {
+        const BlockSymbolic& s = f.sym[b];
+        const i64 bs = bstart[b];
+        const bool singleton = (s.n == 1);
+        for (i64 k = 0; k < s.n; k++) {
+            i64 g = bs + k;
+            if (singleton) {
+                p.colflag[g] = COL_SINGLETON;
+                p.uri.push_back(chk32(g));  // diagonal only
+            } else {
+                for (i64 q = s.lcp[k]; q < s.lcp[k + 1]; q++) p.lri.push_back(chk32(bs + s.lri[q]));
+                for (i64 q = s.ucp[k]; q < s.ucp[k + 1]; q++) p.uri.push_back(chk32(bs + s.uri[q]));
+            }
+            p.lcp[g + 1] = (int32_t)p.lri.size();
+            p.ucp[g + 1] = (int32_t)p.uri.size();
+        }
+    }
+    p.nl = (int32_t)p.lri.size();
+    p.nu = (int32_t)p.uri.size();
+
+    // 2. factor level schedule: column g waits for every j in U(:,g), j < g
+    std::vector<int32_t> flev(n, 0), cols(n);
+    for (i64 g = 0; g < n; g++) {
+        int32_t lv = 0;
+        for (int32_t q = p.ucp[g]; q < p.ucp[g + 1] - 1; q++) lv = std::max(lv, flev[p.uri[q]] + 1);
+        flev[g] = lv;
+        cols[g] = (int32_t)g;
+    }
+    group_by_level(flev, cols, p.flev_ptr, p.flev_cols);
+    p.nflev = (int32_t)p.flev_ptr.size() - 1;
+    for (int32_t l = 0; l < p.nflev; l++)
+        p.max_flev_width = std::max(p.max_flev_width, p.flev_ptr[l + 1] - p.flev_ptr[l]);
+
+    // 3. scatter map: runs of duplicate (row, col) entries -> target position,
+    //    dropping targets outside the column's pattern (their values are
+    //    cleared unread by the reference).
+    std::vector<int32_t> inpat(n, -1);
+    p.sc_ptr.assign(n + 1, 0);
+    for (i64 b = 0; b < nb; b++) {
+        const BlockSymbolic& s = f.sym[b];
+        const BlockNumeric& m = f.num[b];
+        const i64 bs = bstart[b], be = bstart[b + 1];
+        for (i64 k = 0; k < s.n; k++) {
+            const i64 g = bs + k;
+            for (int32_t q = p.lcp[g]; q < p.lcp[g + 1]; q++) inpat[p.lri[q]] = (int32_t)g;
+            for (int32_t q = p.ucp[g]; q < p.ucp[g + 1]; q++) inpat[p.uri[q]] = (int32_t)g;
+            i64 oc;  // original column
+            if (f.has_btf) {
+                i64 lc = s.col_perm_inv[k];
+                oc = f.btf.col_perm[bs + lc];
+            } else {
+                oc = s.col_perm_inv[k];
+            }
+            i64 q = ap[oc];
+            while (q < ap[oc + 1]) {
+                i64 orow = ai[q];
+                i64 q2 = q + 1;
+                while (q2 < ap[oc + 1] && ai[q2] == orow) q2++;
+                if (orow < n) {
+                    i64 lr = -1;
+                    if (f.has_btf) {
+                        i64 grn = f.btf.row_perm_inv[orow];
+                        if (grn >= bs && grn < be) lr = grn - bs;
+                    } else {
+                        lr = orow;
+                    }
+                    if (lr >= 0) {
+                        i64 tgt = bs + m.row_perm_inv[s.col_perm[lr]];
+                        if (inpat[tgt] == (int32_t)g) {
+                            p.sc_src.push_back(chk32(q));
+                            p.sc_cnt.push_back((int32_t)(q2 - q));
+                            p.sc_dst.push_back((int32_t)tgt);
+                        }
+                    }
+                }
+                q = q2;
+            }
+            p.sc_ptr[g + 1] = (int32_t)p.sc_src.size();
+        }
+    }
+    p.nsc = (int32_t)p.sc_src.size();
+
+    // 4. solve permutations
+    p.perm_in.assign(n, 0);
+    p.perm_out.assign(n, 0);
+    for (i64 i = 0; i < n; i++) {
+        if (f.has_btf) {
+            i64 r = f.btf.row_perm_inv[i];
+            i64 b = blk_of[r];
+            const BlockSymbolic& s = f.sym[b];
+            const BlockNumeric& m = f.num[b];
+            i64 bs = bstart[b];
+            p.perm_in[i] = (int32_t)(bs + m.row_perm_inv[s.col_perm[r - bs]]);
+            i64 c = f.btf.col_perm_inv[i];
+            i64 b2 = blk_of[c];
+            p.perm_out[i] = (int32_t)(bstart[b2] + f.sym[b2].col_perm[c - bstart[b2]]);
+        } else {
+            const BlockSymbolic& s = f.sym[0];
+            const BlockNumeric& m = f.num[0];
+            p.perm_in[i] = (int32_t)m.row_perm_inv[s.col_perm[i]];
+            p.perm_out[i] = (int32_t)s.col_perm[i];
+        }
+    }
+
+    // 5. row-oriented L (cols ascending) and U without diagonal (cols descending)
+    p.lrp.assign(n + 1, 0);
+    p.urp.assign(n + 1, 0);
+    for (int32_t q = 0; q < p.nl; q++) p.lrp[p.lri[q] + 1]++;
+    for (i64 g = 0; g < n; g++)
+        for (int32_t q = p.ucp[g]; q < p.ucp[g + 1] - 1; q++) p.urp[p.uri[q] + 1]++;
+    for (i64 i = 0; i < n; i++) {
+        p.lrp[i + 1] += p.lrp[i];
+        p.urp[i + 1] += p.urp[i];
+    }
+    p.lrc.resize(p.lrp[n]);
+    p.lrv.resize(p.lrp[n]);
+    p.urc.resize(p.urp[n]);
+    p.urv.resize(p.urp[n]);
+    {
+        std::vector<int32_t> fl(p.lrp.begin(), p.lrp.end() - 1);
+        for (i64 g = 0; g < n; g++)
+            for (int32_t q = p.lcp[g]; q < p.lcp[g + 1]; q++) {
+                int32_t r = p.lri[q];
+                p.lrc[fl[r]] = (int32_t)g;
+                p.lrv[fl[r]++] = q;
+            }
+        std::vector<int32_t> fu(p.urp.begin(), p.urp.end() - 1);
+        for (i64 g = n - 1; g >= 0; g--)
+            for (int32_t q = p.ucp[g]; q < p.ucp[g + 1] - 1; q++) {
+                int32_t r = p.uri[q];
+                p.urc[fu[r]] = (int32_t)g;
+                p.urv[fu[r]++] = q;
+            }
+    }
+
+    // 6. per-block level schedules of the two triangular solves
+    std::vector<int32_t> llev(n, 0), ulev(n, 0), rows;
+    for (i64 b = 0; b < nb; b++) {
+        const i64 bs = bstart[b], be = bstart[b + 1];
+        rows.clear();
+        for (i64 r = bs; r < be; r++) {
+            int32_t lv = 0;
+            for (int32_t q = p.lrp[r]; q < p.lrp[r + 1]; q++) lv = std::max(lv, llev[p.lrc[q]] + 1);
+            llev[r] = lv;
+            rows.push_back((int32_t)r);
+        }
+        p.lsl_blk.push_back(p.lsl_ptr.empty() ? 0 : (int32_t)p.lsl_ptr.size() - 1);
+        group_by_level(llev, rows, p.lsl_ptr, p.lsl_rows);
+        rows.clear();
+        for (i64 r = be - 1; r >= bs; r--) {
+            int32_t lv = 0;
+            for (int32_t q = p.urp[r]; q < p.urp[r + 1]; q++) lv = std::max(lv, ulev[p.urc[q]] + 1);
+            ulev[r] = lv;
+        }
+        for (i64 r = bs; r < be; r++) rows.push_back((int32_t)r);
+        p.usl_blk.push_back(p.usl_ptr.empty() ? 0 : (int32_t)p.usl_ptr.size() - 1);
+        group_by_level(ulev, rows, p.usl_ptr, p.usl_rows);
+    }
+    p.lsl_blk.push_back((int32_t)p.lsl_ptr.size() - 1);
+    p.usl_blk.push_back((int32_t)p.usl_ptr.size() - 1);
+    p.n_lsl = (int32_t)p.lsl_ptr.size() - 1;
+    p.n_usl = (int32_t)p.usl_ptr.size() - 1;
+
+    // 7. off-diagonal BTF entries per row position, reference order
+    p.odp.assign(n + 1, 0);
+    p.blk_has_od.assign(nb, 0);
+    if (f.has_btf) {
+        struct OD { int32_t pos, ax, src; };
+        std::vector<OD> ods;
+        for (i64 br = 0; br < nb; br++) {
+            const i64 rs = bstart[br], re = bstart[br + 1];
+            for (i64 bc = br + 1; bc < nb; bc++) {
+                const i64 cs = bstart[bc], ce = bstart[bc + 1];
+                for (i64 lc = 0; lc < ce - cs; lc++) {
+                    i64 gco = f.btf.col_perm[cs + lc];
+                    int32_t src = (int32_t)(cs + f.sym[bc].col_perm[lc]);
+                    for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
+                        i64 gro = ai[q];
+                        if (gro >= n) continue;
+                        i64 grn = f.btf.row_perm_inv[gro];
+                        if (grn >= rs && grn < re) {
+                            int32_t pos = (int32_t)(rs + f.num[br].row_perm_inv[f.sym[br].col_perm[grn - rs]]);
+                            ods.push_back({pos, chk32(q), src});
+                            p.blk_has_od[br] = 1;
+                        }
+                    }
+                }
+            }
+        }
+        for (auto& o : ods) p.odp[o.pos + 1]++;
+        for (i64 i = 0; i < n; i++) p.odp[i + 1] += p.odp[i];
+        p.odax.resize(ods.size());
+        p.odsrc.resize(ods.size());
+        std::vector<int32_t> fo(p.odp.begin(), p.odp.end() - 1);
+        for (auto& o : ods) {  // stable: keeps reference order per row
+            p.odax[fo[o.pos]] = o.ax;
+            p.odsrc[fo[o.pos]++] = o.src;
+        }
+    }
+    p.nod = (int32_t)p.odax.size();
+
+    // 8. algorithmic work
+    double ff = 0;
+    for (i64 g = 0; g < n; g++) {
+        for (int32_t q = p.ucp[g]; q < p.ucp[g + 1] - 1; q++) {
+            int32_t j = p.uri[q];
+            ff += 2.0 * (p.lcp[j + 1] - p.lcp[j]);
+        }
+        ff += p.lcp[g + 1] - p.lcp[g];
+    }
+    p.refactor_flops = ff;
+    p.solve_flops = 2.0 * ((double)p.nl + (double)p.nu - (double)n) + (double)n;
+}
+
+void gather_values(const HostFactor& f, std::vector<double>& lx, std::vector<double>& ux,
+                   std::vector<double>& ud) {
+    lx.clear();
+    ux.clear();
+    ud.clear();
+    for (size_t b = 0; b < f.sym.size(); b++) {
+        const BlockNumeric& m = f.num[b];
+        if (f.sym[b].n == 1) {
+            ux.push_back(m.udiag[0]);
+            ud.push_back(m.udiag[0]);
+            continue;
+        }
+        lx.insert(lx.end(), m.lx.begin(), m.lx.end());
+        ux.insert(ux.end(), m.ux.begin(), m.ux.end());
+        ud.insert(ud.end(), m.udiag.begin(), m.udiag.end());
+    }
+}
+
+void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
+    auto put = [&](const std::vector<int32_t>& v) -> int32_t {
+        size_t off = pool.size();
+        if (off + v.size() > (size_t)INT32_MAX) throw std::runtime_error("plan pool too large");
+        pool.insert(pool.end(), v.begin(), v.end());
+        pool.push_back(0);  // keeps every offset valid for empty arrays
+        return (int32_t)off;
+    };
+    d.n = p.n;
+    d.nblk = p.nblk;
+    d.nflev = p.nflev;
+    d.nl = p.nl;
+    d.nu = p.nu;
+    d.nsc = p.nsc;
+    d.nod = p.nod;
+    d.o_lcp = put(p.lcp);
+    d.o_lri = put(p.lri);
+    d.o_ucp = put(p.ucp);
+    d.o_uri = put(p.uri);
+    d.o_colflag = put(p.colflag);
+    d.o_flev_ptr = put(p.flev_ptr);
+    d.o_flev_cols = put(p.flev_cols);
+    d.o_sc_ptr = put(p.sc_ptr);
+    d.o_sc_src = put(p.sc_src);
+    d.o_sc_cnt = put(p.sc_cnt);
+    d.o_sc_dst = put(p.sc_dst);
+    d.o_perm_in = put(p.perm_in);
+    d.o_perm_out = put(p.perm_out);
+    d.o_lrp = put(p.lrp);
+    d.o_lrc = put(p.lrc);
+    d.o_lrv = put(p.lrv);
+    d.o_urp = put(p.urp);
+    d.o_urc = put(p.urc);
+    d.o_urv = put(p.urv);
+    d.o_blk_ptr = put(p.blk_ptr);
+    d.o_lsl_blk = put(p.lsl_blk);
+    d.o_lsl_ptr = put(p.lsl_ptr);
+    d.o_lsl_rows = put(p.lsl_rows);
+    d.o_usl_blk = put(p.usl_blk);
+    d.o_usl_ptr = put(p.usl_ptr);
+    d.o_usl_rows = put(p.usl_rows);
+    d.o_odp = put(p.odp);
+    d.o_odax = put(p.odax);
+    d.o_odsrc = put(p.odsrc);
+    d.o_blk_has_od = put(p.blk_has_od);
+}
+
+}  // namespace rsgpu

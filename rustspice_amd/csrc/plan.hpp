@@ -1,0 +1,72 @@
+// plan.hpp - flat device plan of one factorization (pattern, pivots,
+// scatter map, level schedules) and its builder.
+//
+// Coordinates: a "position" g in [0, n) is block_start + pivot step inside a
+// BTF diagonal block (the reference's per-block permuted index). L and U of
+// all blocks live in one CSC over positions; off-diagonal BTF entries are
+// kept unmerged, per row, in the reference's solve order
+// (native_klu.rs:1501-1523).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "symbolic.hpp"
+
+namespace rsgpu {
+
+// Offsets (in elements) into a shared int32 pool and double pools. The same
+// struct is read by the kernels, so it is plain data.
+struct PlanDev {
+    int32_t n, nblk, nflev, flags;
+    int32_t nl, nu, nsc, nod;
+    int32_t o_lcp, o_lri, o_ucp, o_uri, o_colflag, o_flev_ptr, o_flev_cols;
+    int32_t o_sc_ptr, o_sc_src, o_sc_cnt, o_sc_dst;
+    int32_t o_perm_in, o_perm_out;
+    int32_t o_lrp, o_lrc, o_lrv, o_urp, o_urc, o_urv;
+    int32_t o_blk_ptr, o_lsl_blk, o_lsl_ptr, o_lsl_rows, o_usl_blk, o_usl_ptr, o_usl_rows;
+    int32_t o_odp, o_odax, o_odsrc, o_blk_has_od;
+    int32_t pad0;
+    int64_t o_lx, o_ux, o_ud;  // value pool
+    int64_t o_ax;              // ax pool (unmerged CSC values as handed in)
+    int64_t o_vec;             // rhs/x pool (n entries)
+    int64_t o_work;            // factor work vectors, one per wavefront (global path)
+    int64_t o_tvec;            // solve vector (global path)
+};
+
+enum { COL_SINGLETON = 1 };
+
+struct HostFactor {
+    i64 n = 0;
+    bool has_btf = false;
+    Btf btf;
+    std::vector<BlockSymbolic> sym;
+    std::vector<BlockNumeric> num;
+    i64 nnz_l = 0, nnz_u = 0, noffdiag = 0;
+};
+
+// Host-side plan arrays (int32) before upload.
+struct PlanHost {
+    int32_t n = 0, nblk = 0, nflev = 0, nl = 0, nu = 0, nsc = 0, nod = 0;
+    std::vector<int32_t> lcp, lri, ucp, uri, colflag, flev_ptr, flev_cols;
+    std::vector<int32_t> sc_ptr, sc_src, sc_cnt, sc_dst;
+    std::vector<int32_t> perm_in, perm_out;
+    std::vector<int32_t> lrp, lrc, lrv, urp, urc, urv;
+    std::vector<int32_t> blk_ptr, lsl_blk, lsl_ptr, lsl_rows, usl_blk, usl_ptr, usl_rows;
+    std::vector<int32_t> odp, odax, odsrc, blk_has_od;
+    // level statistics
+    int32_t max_flev_width = 0, n_lsl = 0, n_usl = 0;
+    // algorithmic work of one refactor / one solve (counted on the pattern)
+    double refactor_flops = 0, solve_flops = 0;
+};
+
+void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p);
+
+// Gather L/U values of all blocks into position-space arrays (lx[nl], ux[nu],
+// ud[n]) matching PlanHost's lri/uri order.
+void gather_values(const HostFactor& f, std::vector<double>& lx, std::vector<double>& ux,
+                   std::vector<double>& ud);
+
+// Append one plan's int arrays to a pool and fill the int offsets of d.
+void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
+
+}  // namespace rsgpu

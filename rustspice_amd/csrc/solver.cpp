@@ -1,0 +1,485 @@
+// solver.cpp - rs_solver: the LinearSolver state machine of NativeKluSolver
+// (crates/sim-core/src/native_klu.rs:1602-1682) over the GPU kernels.
+//
+// Host: pattern cache, BTF + AMD + L/U pattern, first factor with threshold
+// pivoting (its pivot choices are sequential). Device: every refactor and
+// every solve of an unchanged pattern.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rsgpu.h"
+#include "plan.hpp"
+
+namespace rsgpu {
+hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
+                           double* vals, double* wpool, int max_n, bool lds, int threads,
+                           hipStream_t stream);
+hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
+                        const double* vals, const double* rhs, double* x, double* wpool, int max_n,
+                        bool lds, int threads, hipStream_t stream);
+size_t device_max_lds();
+}  // namespace rsgpu
+
+using namespace rsgpu;
+
+namespace {
+enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
+constexpr int kThreads = 256;
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, (cnt ? cnt : 1) * sizeof(T));
+        if (e == hipSuccess) n = cnt;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+}  // namespace
+
+struct rs_solver {
+    int kind = RS_SOLVER_NATIVEKLU;
+    i64 n = 0;
+    double pivot_tol = 0.001;
+    int ordering = 0;
+    bool use_btf = true;
+    i64 btf_min_blocks = 2;
+    int state = ST_EMPTY;
+    std::vector<i64> last_ap, last_ai;
+    HostFactor hf;
+    PlanHost ph;
+    PlanDev pd{};
+    bool plan_on_device = false;
+    i64 factor_count = 0, refactor_count = 0;
+    double rcond = 0.0;
+    bool last_parallel = false;
+    i64 threads_hint = 0;
+    std::string err;
+    size_t plan_bytes = 0;
+    bool lds_refactor = false, lds_solve = false;
+    DevBuf<PlanDev> d_desc;
+    DevBuf<int32_t> d_ip;
+    DevBuf<double> d_vals, d_ax, d_rhs, d_x, d_work;
+    std::vector<double> h_ud;
+};
+
+namespace {
+
+int32_t fail(rs_solver* h, int32_t code, const std::string& msg) {
+    h->err = msg;
+    return code;
+}
+
+int32_t hip_fail(rs_solver* h, hipError_t e, const char* what) {
+    h->err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+    return RS_ERR_DEVICE;
+}
+
+bool pattern_matches(const rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail) {
+    if ((i64)h->last_ap.size() != apl || (i64)h->last_ai.size() != ail) return false;
+    if (apl && std::memcmp(h->last_ap.data(), ap, apl * sizeof(i64))) return false;
+    if (ail && std::memcmp(h->last_ai.data(), ai, ail * sizeof(i64))) return false;
+    return true;
+}
+
+void reset(rs_solver* h) {
+    h->state = ST_EMPTY;
+    h->hf = HostFactor();
+    h->ph = PlanHost();
+    h->plan_on_device = false;
+    h->last_ap.clear();
+    h->last_ai.clear();
+    h->factor_count = h->refactor_count = 0;
+    h->rcond = 0.0;
+}
+
+// do_analyze (native_klu.rs:704-761)
+int32_t do_analyze(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail) {
+    const i64 n = h->n;
+    for (i64 c = 0; c < n; c++)
+        if (ap[c + 1] < ap[c] || ap[c + 1] > ail)
+            return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: column pointers are not monotone");
+    for (i64 q = 0; q < ap[n]; q++)
+        if (ai[q] < 0) return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: negative row index");
+    HostFactor& f = h->hf;
+    f = HostFactor();
+    f.n = n;
+    const bool natural = h->ordering == 3;
+    bool use_btf = h->kind == RS_SOLVER_NATIVEKLU && h->use_btf && n >= 4;
+    if (use_btf) {
+        f.btf = btf_decompose(n, ap, ai);
+        f.has_btf = f.btf.num_blocks >= h->btf_min_blocks && f.btf.max_block_size < n;
+    }
+    if (f.has_btf) {
+        f.sym.resize(f.btf.num_blocks);
+        std::vector<i64> bap, bai;
+        for (i64 b = 0; b < f.btf.num_blocks; b++) {
+            i64 bs = f.btf.block_ptr[b], be = f.btf.block_ptr[b + 1], sz = be - bs;
+            bap.assign(sz + 1, 0);
+            bai.clear();
+            for (i64 lc = 0; lc < sz; lc++) {  // extract_block_pattern (native_klu.rs:764-797)
+                i64 gco = f.btf.col_perm[bs + lc];
+                for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
+                    i64 gro = ai[q];
+                    if (gro >= n) continue;
+                    i64 grn = f.btf.row_perm_inv[gro];
+                    if (grn >= bs && grn < be) bai.push_back(grn - bs);
+                }
+                bap[lc + 1] = (i64)bai.size();
+            }
+            block_symbolic(sz, bap.data(), bai.data(), natural, f.sym[b]);
+        }
+        i64 od = 0;  // noffdiag: entries above the block diagonal
+        for (i64 c = 0; c < n; c++) {
+            i64 cpos = f.btf.col_perm_inv[c];
+            for (i64 q = ap[c]; q < ap[c + 1]; q++) {
+                if (ai[q] >= n) continue;
+                i64 rpos = f.btf.row_perm_inv[ai[q]];
+                i64 bc = std::upper_bound(f.btf.block_ptr.begin(), f.btf.block_ptr.end(), cpos) -
+                         f.btf.block_ptr.begin() - 1;
+                if (rpos < f.btf.block_ptr[bc]) od++;
+            }
+        }
+        f.noffdiag = od;
+    } else {
+        f.sym.resize(1);
+        block_symbolic(n, ap, ai, natural, f.sym[0]);
+    }
+    for (auto& s : f.sym) {
+        f.nnz_l += (i64)s.lri.size();
+        f.nnz_u += (i64)s.uri.size();
+    }
+    h->state = ST_ANALYZED;
+    h->plan_on_device = false;
+    h->last_ap.assign(ap, ap + apl);
+    h->last_ai.assign(ai, ai + ail);
+    return RS_OK;
+}
+
+void compute_rcond(rs_solver* h, const double* ud, i64 n) {
+    double mn = 1.7976931348623157e308, mx = 0.0;
+    for (i64 i = 0; i < n; i++) {
+        double a = std::fabs(ud[i]);
+        if (a < mn) mn = a;
+        if (a > mx) mx = a;
+    }
+    h->rcond = mx > 0.0 ? mn / mx : 0.0;
+}
+
+int32_t upload_plan(rs_solver* h, i64 nnz) {
+    std::vector<int32_t> pool;
+    PlanDev d{};
+    append_plan(h->ph, pool, d);
+    const i64 n = h->n;
+    d.o_lx = 0;
+    d.o_ux = h->ph.nl + 1;
+    d.o_ud = d.o_ux + h->ph.nu + 1;
+    d.o_ax = 0;
+    d.o_vec = 0;
+    d.o_work = 0;
+    d.o_tvec = (int64_t)(kThreads / 64) * n;
+    size_t maxlds = device_max_lds();
+    h->lds_refactor = (size_t)(kThreads / 64) * n * sizeof(double) <= maxlds;
+    h->lds_solve = (size_t)n * sizeof(double) <= maxlds;
+    hipError_t e;
+    if ((e = h->d_desc.ensure(1)) != hipSuccess) return hip_fail(h, e, "hipMalloc(desc)");
+    if ((e = h->d_ip.ensure(pool.size())) != hipSuccess) return hip_fail(h, e, "hipMalloc(plan)");
+    if ((e = h->d_vals.ensure(d.o_ud + n + 1)) != hipSuccess) return hip_fail(h, e, "hipMalloc(vals)");
+    if ((e = h->d_ax.ensure(nnz)) != hipSuccess) return hip_fail(h, e, "hipMalloc(ax)");
+    if ((e = h->d_rhs.ensure(n)) != hipSuccess) return hip_fail(h, e, "hipMalloc(rhs)");
+    if ((e = h->d_x.ensure(n)) != hipSuccess) return hip_fail(h, e, "hipMalloc(x)");
+    size_t wn = (size_t)(kThreads / 64 + 1) * n;
+    bool new_work = h->d_work.n < wn;
+    if ((e = h->d_work.ensure(wn)) != hipSuccess) return hip_fail(h, e, "hipMalloc(work)");
+    (void)new_work;
+    {
+        if ((e = hipMemset(h->d_work.p, 0, wn * sizeof(double))) != hipSuccess)
+            return hip_fail(h, e, "hipMemset(work)");
+    }
+    if ((e = hipMemcpy(h->d_desc.p, &d, sizeof(d), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(desc)");
+    if ((e = hipMemcpy(h->d_ip.p, pool.data(), pool.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(plan)");
+    h->pd = d;
+    h->plan_bytes = pool.size() * sizeof(int32_t);
+    h->plan_on_device = true;
+    return RS_OK;
+}
+
+int32_t upload_values(rs_solver* h) {
+    std::vector<double> lx, ux, ud;
+    gather_values(h->hf, lx, ux, ud);
+    hipError_t e;
+    if (!lx.empty() &&
+        (e = hipMemcpy(h->d_vals.p + h->pd.o_lx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(lx)");
+    if ((e = hipMemcpy(h->d_vals.p + h->pd.o_ux, ux.data(), ux.size() * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(ux)");
+    if ((e = hipMemcpy(h->d_vals.p + h->pd.o_ud, ud.data(), ud.size() * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(ud)");
+    compute_rcond(h, ud.data(), (i64)ud.size());
+    return RS_OK;
+}
+
+// Full factor on the host (first factor of a pattern: pivots chosen here).
+int32_t host_full_factor(rs_solver* h, const i64* ap, const i64* ai, const double* ax) {
+    HostFactor& f = h->hf;
+    const i64 n = h->n;
+    const double tol = h->kind == RS_SOLVER_SPARSELU ? 0.0 : h->pivot_tol;
+    f.num.assign(f.sym.size(), BlockNumeric());
+    if (f.has_btf) {
+        std::vector<i64> bap, bai;
+        std::vector<double> bax;
+        for (size_t b = 0; b < f.sym.size(); b++) {  // extract_block_values (native_klu.rs:1151-1188)
+            i64 bs = f.btf.block_ptr[b], be = f.btf.block_ptr[b + 1], sz = be - bs;
+            bap.assign(sz + 1, 0);
+            bai.clear();
+            bax.clear();
+            for (i64 lc = 0; lc < sz; lc++) {
+                i64 gco = f.btf.col_perm[bs + lc];
+                for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
+                    i64 gro = ai[q];
+                    if (gro >= n) continue;
+                    i64 grn = f.btf.row_perm_inv[gro];
+                    if (grn >= bs && grn < be) {
+                        bai.push_back(grn - bs);
+                        bax.push_back(ax[q]);
+                    }
+                }
+                bap[lc + 1] = (i64)bai.size();
+            }
+            factor_block_host(f.sym[b], bap.data(), bai.data(), bax.data(), nullptr, false, tol, f.num[b]);
+        }
+    } else {
+        factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0]);
+    }
+    try {
+        build_plan(f, ap, ai, h->ph);
+    } catch (const std::exception& ex) {
+        return fail(h, RS_ERR_ANALYZE_FAILED, std::string("Symbolic analysis failed: ") + ex.what());
+    }
+    int32_t rc = upload_plan(h, ap[n]);
+    if (rc) return rc;
+    return upload_values(h);
+}
+
+int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool sync_rcond) {
+    hipError_t e = launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
+                                   (int)h->n, h->lds_refactor, kThreads, s);
+    if (e != hipSuccess) return hip_fail(h, e, "k_refactor launch");
+    if (sync_rcond) {
+        h->h_ud.resize(h->n);
+        if ((e = hipMemcpyAsync(h->h_ud.data(), h->d_vals.p + h->pd.o_ud, h->n * 8,
+                                hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return hip_fail(h, e, "hipMemcpy(udiag)");
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(h, e, "refactor");
+        compute_rcond(h, h->h_ud.data(), h->n);
+    }
+    return RS_OK;
+}
+
+}  // namespace
+
+namespace rsgpu {
+size_t device_max_lds() {
+    static size_t v = 0;
+    if (!v) {
+        int dev = 0, val = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+            val > 0)
+            v = (size_t)val;
+        else
+            v = 65536;
+    }
+    return v;
+}
+}  // namespace rsgpu
+
+extern "C" {
+
+rs_solver* rs_solver_create(int32_t kind, int64_t n) {
+    if (kind != RS_SOLVER_NATIVEKLU && kind != RS_SOLVER_SPARSELU) return nullptr;
+    rs_solver* h = new rs_solver();
+    h->kind = kind;
+    h->n = n;
+    if (kind == RS_SOLVER_SPARSELU) {
+        h->pivot_tol = 0.0;
+        h->use_btf = false;
+    }
+    return h;
+}
+
+void rs_solver_destroy(rs_solver* h) {
+    if (!h) return;
+    h->d_desc.release();
+    h->d_ip.release();
+    h->d_vals.release();
+    h->d_ax.release();
+    h->d_rhs.release();
+    h->d_x.release();
+    h->d_work.release();
+    delete h;
+}
+
+void rs_prepare(rs_solver* h, int64_t n) {
+    if (n != h->n) {
+        reset(h);
+        h->n = n;
+    }
+}
+
+int32_t rs_analyze(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t* ai,
+                   int64_t ai_len) {
+    if (h->state != ST_EMPTY && pattern_matches(h, ap, ap_len, ai, ai_len)) return RS_OK;
+    if (ap_len != h->n + 1) {
+        char b[160];
+        std::snprintf(b, sizeof b, "Invalid matrix: Column pointer length %lld != expected %lld",
+                      (long long)ap_len, (long long)(h->n + 1));
+        return fail(h, RS_ERR_INVALID_MATRIX, b);
+    }
+    return do_analyze(h, ap, ap_len, ai, ai_len);
+}
+
+int32_t rs_factor(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t* ai,
+                  int64_t ai_len, const double* ax, int64_t ax_len) {
+    if (h->state == ST_EMPTY || !pattern_matches(h, ap, ap_len, ai, ai_len)) {
+        if (ap_len != h->n + 1) {
+            char b[160];
+            std::snprintf(b, sizeof b, "Invalid matrix: Column pointer length %lld != expected %lld",
+                          (long long)ap_len, (long long)(h->n + 1));
+            return fail(h, RS_ERR_INVALID_MATRIX, b);
+        }
+        int32_t rc = do_analyze(h, ap, ap_len, ai, ai_len);
+        if (rc) return rc;
+    }
+    if (ax_len < ap[h->n]) return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: ax shorter than nnz");
+    const bool pattern_same = h->state == ST_FACTORED;
+    if (pattern_same && h->plan_on_device) {
+        hipError_t e = hipMemcpy(h->d_ax.p, ax, ap[h->n] * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
+        int32_t rc = gpu_refactor(h, h->d_ax.p, nullptr, true);
+        if (rc) return rc;
+        h->refactor_count++;
+        h->last_parallel = true;
+    } else {
+        int32_t rc = host_full_factor(h, ap, ai, ax);
+        if (rc) return rc;
+        h->factor_count++;
+        h->last_parallel = false;
+    }
+    h->state = ST_FACTORED;
+    return RS_OK;
+}
+
+int32_t rs_solve(rs_solver* h, double* rhs, int64_t n) {
+    if (h->state != ST_FACTORED) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    if (n != h->n) {
+        char b[160];
+        std::snprintf(b, sizeof b, "Invalid matrix: RHS length %lld != matrix dimension %lld",
+                      (long long)n, (long long)h->n);
+        return fail(h, RS_ERR_INVALID_MATRIX, b);
+    }
+    hipError_t e = hipMemcpy(h->d_rhs.p, rhs, n * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(rhs)");
+    e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, h->d_rhs.p, h->d_x.p,
+                     h->d_work.p, (int)n, h->lds_solve, kThreads, nullptr);
+    if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
+    e = hipMemcpy(rhs, h->d_x.p, n * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(x)");
+    return RS_OK;
+}
+
+void rs_reset_pattern(rs_solver* h) { reset(h); }
+
+const char* rs_solver_name(const rs_solver* h) {
+    return h->kind == RS_SOLVER_SPARSELU ? "SparseLU-MI355X" : "NativeKLU-MI355X";
+}
+
+void rs_set_parallel_threads(rs_solver* h, int64_t t) { h->threads_hint = t; }
+
+void rs_set_config(rs_solver* h, double pivot_tol, int32_t ordering, int32_t use_btf) {
+    if (h->kind == RS_SOLVER_SPARSELU) return;
+    h->pivot_tol = pivot_tol;
+    h->ordering = ordering;
+    h->use_btf = use_btf != 0;
+}
+
+int32_t rs_get_stats(const rs_solver* h, rs_solver_stats* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->factor_count = h->factor_count;
+    o->refactor_count = h->refactor_count;
+    o->nnz_l = h->hf.nnz_l;
+    o->nnz_u = h->hf.nnz_u;
+    o->nblocks = h->state == ST_EMPTY ? 0 : (i64)h->hf.sym.size();
+    o->noffdiag = h->hf.noffdiag;
+    o->rcond = h->rcond;
+    o->parallel_factor = h->last_parallel;
+    o->n_factor_levels = h->ph.nflev;
+    o->max_level_width = h->ph.max_flev_width;
+    o->n_lsolve_levels = h->ph.n_lsl;
+    o->n_usolve_levels = h->ph.n_usl;
+    o->refactor_flops = h->ph.refactor_flops;
+    o->solve_flops = h->ph.solve_flops;
+    o->plan_bytes = (int64_t)h->plan_bytes;
+    return RS_OK;
+}
+
+int32_t rs_last_error(const rs_solver* h, char* buf, int64_t len) {
+    if (len > 0) {
+        std::snprintf(buf, (size_t)len, "%s", h->err.c_str());
+    }
+    return (int32_t)h->err.size();
+}
+
+int32_t rs_refactor_device(rs_solver* h, const double* ax_dev, void* stream) {
+    if (h->state != ST_FACTORED || !h->plan_on_device)
+        return fail(h, RS_ERR_FACTOR_FAILED, "Numerical factorization failed: no factored pattern");
+    int32_t rc = gpu_refactor(h, ax_dev, (hipStream_t)stream, false);
+    if (rc == RS_OK) {
+        h->refactor_count++;
+        h->last_parallel = true;
+    }
+    return rc;
+}
+
+int32_t rs_solve_device(rs_solver* h, const double* rhs_dev, double* x_dev, void* stream) {
+    if (h->state != ST_FACTORED || !h->plan_on_device)
+        return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    hipError_t e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev,
+                                h->d_work.p, (int)h->n, h->lds_solve, kThreads, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
+    return RS_OK;
+}
+
+int32_t rs_get_factor_values(rs_solver* h, double* lx, double* ux, double* udiag) {
+    if (h->state != ST_FACTORED || !h->plan_on_device)
+        return fail(h, RS_ERR_FACTOR_FAILED, "Numerical factorization failed: not factored");
+    hipError_t e;
+    if (h->ph.nl && (e = hipMemcpy(lx, h->d_vals.p + h->pd.o_lx, h->ph.nl * 8, hipMemcpyDeviceToHost)))
+        return hip_fail(h, e, "hipMemcpy(lx)");
+    if ((e = hipMemcpy(ux, h->d_vals.p + h->pd.o_ux, h->ph.nu * 8, hipMemcpyDeviceToHost)))
+        return hip_fail(h, e, "hipMemcpy(ux)");
+    if ((e = hipMemcpy(udiag, h->d_vals.p + h->pd.o_ud, h->n * 8, hipMemcpyDeviceToHost)))
+        return hip_fail(h, e, "hipMemcpy(ud)");
+    return RS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+"""Seeded synthetic circuits for the five benchmark/parity workloads.
+
+Each generator returns a :class:`Circuit`: a numpy record array of
+``rs_device`` (``include/rs_circuit.h``) plus the node count.  The records are
+what RustSpice's stamp loop would see after parsing a netlist once
+(crates/sim-core/src/stamp.rs:94-553): numbers are stored as the doubles the
+reference's ``parse_number_with_suffix`` (stamp.rs:721-747) yields for the
+netlist text, e.g. ``10p`` is ``10 * 1e-12``.
+
+Randomness uses the reference's 64-bit LCG (tests/bbd_benchmark_tests.rs:290-301:
+``state = state * 6364136223846793005 + 1442695040888963407`` wrapping) with
+``u = (state >> 11) * 2**-53`` and Box-Muller normals; the seed of workload k
+is ``0x52530000 + k``.
+
+Node numbering follows the reference's NodeTable (circuit.rs:31-57): ground is
+0 and nodes are numbered by first appearance; every generator lists devices so
+that the first appearance order equals the ids documented below.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+RS_DEV_R, RS_DEV_C, RS_DEV_V, RS_DEV_I, RS_DEV_D, RS_DEV_M = 1, 2, 3, 4, 5, 6
+RS_F_PMOS, RS_F_WAVEFORM = 1, 2
+
+DEVICE_DTYPE = np.dtype(
+    [
+        ("kind", np.int32),
+        ("flags", np.int32),
+        ("node", np.int32, (4,)),
+        ("value", np.float64),
+        ("p", np.float64, (8,)),
+    ],
+    align=True,
+)
+assert DEVICE_DTYPE.itemsize == 96
+
+#: EPSILON_OX of crates/sim-devices/src/bsim/params.rs:10
+EPSILON_OX = 3.9 * 8.854e-12
+TOX_DEFAULT = 1.5e-8  # params.rs:344
+
+
+def sfx(mantissa: float, mult: float) -> float:
+    """Value of a SPICE token ``<mantissa><suffix>`` as the reference parses it."""
+    return float(mantissa) * mult
+
+
+P, N, U, F = 1e-12, 1e-9, 1e-6, 1e-15
+
+
+class Lcg:
+    """tests/bbd_benchmark_tests.rs:295-298 LCG."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state * 6364136223846793005 + 1442695040888963407) & self.MASK
+        return self.state
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+
+    def normal(self) -> float:
+        u1 = 1.0 - self.uniform()  # (0, 1]
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def generate_rhs(n: int, seed: int) -> np.ndarray:
+    """bbd_benchmark_tests.rs:290-301 generate_rhs."""
+    out = np.empty(n, dtype=np.float64)
+    st = seed & Lcg.MASK
+    for i in range(n):
+        st = (st * 6364136223846793005 + 1442695040888963407) & Lcg.MASK
+        out[i] = float(st >> 33) / float(1 << 31) - 1.0
+    return out
+
+
+@dataclass
+class Circuit:
+    devices: np.ndarray  # DEVICE_DTYPE records
+    node_count: int
+    name: str = ""
+
+    @property
+    def ndev(self) -> int:
+        return int(self.devices.shape[0])
+
+
+class _Builder:
+    def __init__(self):
+        self.rows = []
+
+    def add(self, kind, nodes, value=0.0, p=None, flags=0):
+        node = list(nodes) + [0] * (4 - len(nodes))
+        pp = list(p or []) + [0.0] * (8 - len(p or []))
+        self.rows.append((kind, flags, node, float(value), pp))
+
+    def build(self, node_count, name):
+        arr = np.zeros(len(self.rows), dtype=DEVICE_DTYPE)
+        for i, (k, f, nd, v, pp) in enumerate(self.rows):
+            arr[i]["kind"] = k
+            arr[i]["flags"] = f
+            arr[i]["node"] = nd
+            arr[i]["value"] = v
+            arr[i]["p"] = pp
+        return Circuit(arr, node_count, name)
+
+
+def mos_params(vth0: float, u0: float, w: float, l: float, tox: float = TOX_DEFAULT):
+    """p[] slots of a level-1 MOSFET (rs_mos_param)."""
+    return [1.0, vth0, u0, tox, w, l]
+
+
+def u0_from_kp(kp: float, tox: float = TOX_DEFAULT) -> float:
+    """Level 1 in the reference ignores KP (bsim/mod.rs:441-455) and uses
+    beta = u0 * 1e-4 * cox; a netlist that wants a given KP sets u0 to this."""
+    return kp / (1e-4 * (EPSILON_OX / tox))
+
+
+def pulse_params(v1, v2, td, tr, tf, pw, per):
+    return [v1, v2, td, tr, tf, pw, per]
+
+
+# --------------------------------------------------------------------------
+# C1: 64x64 resistor mesh, DC OP
+# --------------------------------------------------------------------------
+def resistor_mesh(nx: int = 64, ny: int = 64, seed: int = 0x52530000 + 1) -> Circuit:
+    """Mesh of nx*ny nodes; node (i, j) has id 1 + i*nx + j.
+
+    Device order: 1 Gohm from every node to ground (row-major, which fixes the
+    node numbering), the 1 V source at (0,0), horizontal then vertical mesh
+    resistors with R = 100 * 100**u, and 1 ohm from the far corner to ground.
+    """
+    rng = Lcg(seed)
+    b = _Builder()
+    nid = lambda i, j: 1 + i * nx + j  # noqa: E731
+    for i in range(ny):
+        for j in range(nx):
+            b.add(RS_DEV_R, (nid(i, j), 0), sfx(1, 1e9))
+    b.add(RS_DEV_V, (nid(0, 0), 0), 1.0)
+    for i in range(ny):
+        for j in range(nx - 1):
+            b.add(RS_DEV_R, (nid(i, j), nid(i, j + 1)), 100.0 * 100.0 ** rng.uniform())
+    for i in range(ny - 1):
+        for j in range(nx):
+            b.add(RS_DEV_R, (nid(i, j), nid(i + 1, j)), 100.0 * 100.0 ** rng.uniform())
+    b.add(RS_DEV_R, (nid(ny - 1, nx - 1), 0), 1.0)
+    return b.build(1 + nx * ny, f"mesh{nx}x{ny}")
+
+
+# --------------------------------------------------------------------------
+# C2 / C4: CMOS inverter chains with level-1 MOSFETs
+# --------------------------------------------------------------------------
+NMOS_KP, PMOS_KP = 200e-6, 100e-6
+
+
+def inverter_chain(
+    stages: int = 500,
+    vin_pulse: bool = True,
+    vin_dc: float = 0.0,
+    vto_n=None,
+    vto_p=None,
+    u0_scale_n=None,
+    u0_scale_p=None,
+    load_c: float = sfx(5, F),
+) -> Circuit:
+    """Chain of `stages` inverters. Nodes: 0 ground, 1 vdd, 2 in, 3+i out_i.
+
+    Devices: Vdd (1 V DC), Vin (PULSE(0 1 0 10p 10p 990p 2n) or DC vin_dc),
+    then per stage: NMOS (d=out, g=prev, s=0, b=0), PMOS (d=out, g=prev,
+    s=vdd, b=vdd, type=pmos), C load to ground. NMOS W/L = 1u/0.1u,
+    PMOS W/L = 2u/0.1u, VTO = +/-0.4 V, KP mapped to u0 (u0_from_kp).
+    Optional per-stage arrays perturb VTO and scale u0 (Monte-Carlo).
+    """
+    b = _Builder()
+    b.add(RS_DEV_V, (1, 0), 1.0)
+    if vin_pulse:
+        b.add(RS_DEV_V, (2, 0), 0.0,
+              pulse_params(0.0, 1.0, 0.0, sfx(10, P), sfx(10, P), sfx(990, P), sfx(2, N)),
+              flags=RS_F_WAVEFORM)
+    else:
+        b.add(RS_DEV_V, (2, 0), vin_dc)
+    u0n, u0p = u0_from_kp(NMOS_KP), u0_from_kp(PMOS_KP)
+    wn, wp, l = sfx(1, U), sfx(2, U), sfx(0.1, U)
+    for i in range(stages):
+        out, prev = 3 + i, (2 if i == 0 else 2 + i)
+        vn = 0.4 if vto_n is None else float(vto_n[i])
+        vp = -0.4 if vto_p is None else float(vto_p[i])
+        sn = 1.0 if u0_scale_n is None else float(u0_scale_n[i])
+        sp = 1.0 if u0_scale_p is None else float(u0_scale_p[i])
+        b.add(RS_DEV_M, (out, prev, 0, 0), 0.0, mos_params(vn, u0n * sn, wn, l))
+        b.add(RS_DEV_M, (out, prev, 1, 1), 0.0, mos_params(vp, u0p * sp, wp, l), flags=RS_F_PMOS)
+        b.add(RS_DEV_C, (out, 0), load_c)
+    return b.build(3 + stages, f"inv{stages}")
+
+
+#: .tran 5p 10n 0 5p of the 500-stage chain workload
+C2_TRAN = dict(tstep=sfx(5, P), tstop=sfx(10, N), tstart=0.0, tmax=sfx(5, P))
+
+
+def monte_carlo_chains(count: int = 4096, seed: int = 0x52530000 + 4, lo: int = 50,
+                       hi: int = 400):
+    """Monte-Carlo inverter chains: stages ~ U[lo, hi], VTO += N(0, 20 mV),
+    u0 *= N(1, 0.05) per MOSFET, DC input at 0 V. Draw order per instance:
+    stage count, then per stage (NMOS dVTO, NMOS u0 scale, PMOS dVTO,
+    PMOS u0 scale)."""
+    rng = Lcg(seed)
+    out = []
+    for _ in range(count):
+        stages = lo + int(rng.uniform() * (hi - lo + 1))
+        vn, vp, sn, sp = [], [], [], []
+        for _s in range(stages):
+            vn.append(0.4 + 0.02 * rng.normal())
+            sn.append(1.0 + 0.05 * rng.normal())
+            vp.append(-0.4 + 0.02 * rng.normal())
+            sp.append(1.0 + 0.05 * rng.normal())
+        out.append(inverter_chain(stages, vin_pulse=False, vin_dc=0.0, vto_n=vn, vto_p=vp,
+                                  u0_scale_n=sn, u0_scale_p=sp))
+    return out
+
+
+def rc_ladder(n: int = 20, r: float = 1e3, c: float = 1e-12, vin: float = 1.0) -> Circuit:
+    """Small RC ladder (V source, n series R, C from each tap to ground)."""
+    b = _Builder()
+    b.add(RS_DEV_V, (1, 0), vin)
+    for i in range(n):
+        b.add(RS_DEV_R, (1 + i, 2 + i), r)
+        b.add(RS_DEV_C, (2 + i, 0), c)
+    return b.build(2 + n, f"rc_ladder{n}")
+
+
+def diode_circuit(v: float = 1.0, r: float = 1e3) -> Circuit:
+    """V - R - diode to ground (is=1e-14, n=1)."""
+    b = _Builder()
+    b.add(RS_DEV_V, (1, 0), v)
+    b.add(RS_DEV_R, (1, 2), r)
+    b.add(RS_DEV_D, (2, 0), 0.0, [1e-14, 1.0])
+    return b.build(3, "diode")

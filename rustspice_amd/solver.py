@@ -1,0 +1,188 @@
+"""Python mirror of RustSpice's ``LinearSolver`` trait over librsgpu.so.
+
+``GpuSolver`` has the trait's methods (crates/sim-core/src/solver.rs:141-166)
+with the same argument meaning: ``prepare(n)``, ``analyze(ap, ai)``,
+``factor(ap, ai, ax)``, ``solve(rhs)`` (in place), ``reset_pattern()``,
+``name()``, ``set_parallel_threads(t)``. Errors raise :class:`SolverError`
+carrying the ``SolverError`` variant (solver.rs:76-92).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+NATIVEKLU, SPARSELU = 0, 1
+
+_ERR_NAMES = {
+    1: "AnalyzeFailed", 2: "FactorFailed", 3: "SolveFailed", 4: "SingularMatrix",
+    5: "IllConditioned", 6: "InvalidMatrix", 7: "DeviceError",
+}
+
+
+class SolverError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.variant = _ERR_NAMES.get(code, f"Unknown({code})")
+        super().__init__(f"{self.variant}: {message}")
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class GpuSolver:
+    """``create_solver(SolverType::NativeKlu | SparseLu, n)`` on the MI355X."""
+
+    def __init__(self, n: int, kind: int = NATIVEKLU):
+        self._L = _lib.lib()
+        self.h = self._L.rs_solver_create(kind, n)
+        if not self.h:
+            raise ValueError(f"unknown solver kind {kind}")
+        self.kind = kind
+        self.n = n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.rs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            buf = C.create_string_buffer(512)
+            self._L.rs_last_error(self.h, buf, 512)
+            raise SolverError(rc, buf.value.decode(errors="replace"))
+
+    # --- LinearSolver ---------------------------------------------------------
+    def prepare(self, n: int):
+        self._L.rs_prepare(self.h, n)
+        self.n = n
+
+    def analyze(self, ap, ai):
+        ap, ai = _i64(ap), _i64(ai)
+        self._check(self._L.rs_analyze(self.h, ap, len(ap), ai, len(ai)))
+
+    def factor(self, ap, ai, ax):
+        ap, ai, ax = _i64(ap), _i64(ai), _f64(ax)
+        self._check(self._L.rs_factor(self.h, ap, len(ap), ai, len(ai), ax, len(ax)))
+
+    def solve(self, rhs: np.ndarray):
+        """Overwrites ``rhs`` (float64, contiguous) with the solution."""
+        if rhs.dtype != np.float64 or not rhs.flags.c_contiguous:
+            raise TypeError("rhs must be a contiguous float64 array")
+        self._check(self._L.rs_solve(self.h, rhs, len(rhs)))
+
+    def reset_pattern(self):
+        self._L.rs_reset_pattern(self.h)
+
+    def name(self) -> str:
+        return self._L.rs_solver_name(self.h).decode()
+
+    def set_parallel_threads(self, threads: int):
+        self._L.rs_set_parallel_threads(self.h, threads)
+
+    # --- NativeKluSolver extras ------------------------------------------------
+    def set_config(self, pivot_tol=0.001, ordering=0, use_btf=True):
+        self._L.rs_set_config(self.h, pivot_tol, ordering, int(use_btf))
+
+    def stats(self) -> dict:
+        s = _lib.SolverStats()
+        self._L.rs_get_stats(self.h, C.byref(s))
+        return s.as_dict()
+
+    def rcond(self) -> float:
+        return self.stats()["rcond"]
+
+    # --- device-resident hot path (torch tensors on cuda) ---------------------
+    def refactor_device(self, ax_dev, stream=None):
+        self._check(self._L.rs_refactor_device(self.h, C.c_void_p(ax_dev.data_ptr()),
+                                               C.c_void_p(_stream_ptr(stream))))
+
+    def solve_device(self, rhs_dev, x_dev, stream=None):
+        self._check(self._L.rs_solve_device(self.h, C.c_void_p(rhs_dev.data_ptr()),
+                                            C.c_void_p(x_dev.data_ptr()),
+                                            C.c_void_p(_stream_ptr(stream))))
+
+    def factor_values(self):
+        st = self.stats()
+        n = st["nnz_l"], st["nnz_u"]
+        lx = np.zeros(max(n[0], 1))
+        ux = np.zeros(max(n[1], 1))
+        ud = np.zeros(max(self.n, 1))
+        self._check(self._L.rs_get_factor_values(self.h, lx, ux, ud))
+        return lx[: n[0]], ux[: n[1]], ud[: self.n]
+
+
+def _stream_ptr(stream) -> int:
+    if stream is None:
+        return 0
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+# --- host-side analysis (no device memory) -------------------------------------
+def host_amd_order(n, ap, ai):
+    perm, inv = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64)
+    _lib.lib().rs_host_amd_order(n, _i64(ap), _i64(ai), perm, inv)
+    return perm[:n], inv[:n]
+
+
+def host_btf(n, ap, ai):
+    rp, cp, bp = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64), np.zeros(n + 1, np.int64)
+    rank = C.c_int64()
+    nb = _lib.lib().rs_host_btf(n, _i64(ap), _i64(ai), rp, cp, bp, C.byref(rank))
+    return dict(row_perm=rp[:n], col_perm=cp[:n], block_ptr=bp[: nb + 1], num_blocks=int(nb),
+                structural_rank=rank.value)
+
+
+class HostFactor:
+    """First (pivoting) factor as computed on the host before the GPU takes over."""
+
+    def __init__(self, n, ap, ai, ax, kind=NATIVEKLU, pivot_tol=0.001, ordering=0, use_btf=True):
+        self._L = _lib.lib()
+        self.h = self._L.rs_host_factor_create(kind, n, _i64(ap), _i64(ai), _f64(ax), pivot_tol,
+                                               ordering, int(use_btf))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.rs_host_factor_destroy(self.h)
+            self.h = None
+
+    def nblocks(self):
+        return int(self._L.rs_host_factor_nblocks(self.h))
+
+    def block(self, b=0):
+        n, nl, nu = C.c_int64(), C.c_int64(), C.c_int64()
+        self._L.rs_host_factor_block_sizes(self.h, b, C.byref(n), C.byref(nl), C.byref(nu))
+        n, nl, nu = n.value, nl.value, nu.value
+        d = dict(col_perm=np.zeros(n, np.int64), row_perm=np.zeros(n, np.int64),
+                 l_col_ptr=np.zeros(n + 1, np.int64), l_row_idx=np.zeros(max(nl, 1), np.int64),
+                 u_col_ptr=np.zeros(n + 1, np.int64), u_row_idx=np.zeros(max(nu, 1), np.int64),
+                 l_values=np.zeros(max(nl, 1)), u_values=np.zeros(max(nu, 1)), u_diag=np.zeros(n))
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self._L.rs_host_factor_block_get(self.h, b, *[ptr(d[k]) for k in
+                                         ["col_perm", "row_perm", "l_col_ptr", "l_row_idx",
+                                          "u_col_ptr", "u_row_idx", "l_values", "u_values",
+                                          "u_diag"]])
+        for k, m in (("l_row_idx", nl), ("l_values", nl), ("u_row_idx", nu), ("u_values", nu)):
+            d[k] = d[k][:m]
+        d["n"] = n
+        return d
+
+    def plan_stats(self):
+        out = np.zeros(8, np.int64)
+        self._L.rs_host_factor_plan_stats(self.h, out)
+        keys = ["factor_levels", "max_level_width", "lsolve_levels", "usolve_levels",
+                "scatter_runs", "offdiag", "nnz_l", "nnz_u"]
+        return dict(zip(keys, map(int, out)))

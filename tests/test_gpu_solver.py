@@ -1,0 +1,166 @@
+"""GPU parity of the LinearSolver drop-in (librsgpu.so on the MI355X) against
+the CPU oracle's NativeKluSolver / SparseLuSolver restatement.
+
+The bar is bit-identical results: the GPU refactor applies each column's
+updates in the reference's order and the solve is row-oriented in the
+reference's order, with no FMA contraction.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rustspice_amd import netlists as NL
+from rustspice_amd import solver as S
+from tests.helpers import random_csc, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(n, kind):
+    o = O.NativeKlu(n)
+    if kind == S.SPARSELU:
+        o.set_config(0.0, 0, False)
+    return o
+
+
+def _check_cycle(ap, ai, ax_list, rhs_list, n, kind=S.NATIVEKLU):
+    """factor/solve a sequence of value sets on the same pattern with both
+    solvers; every solution and every factor must agree bit for bit."""
+    g = S.GpuSolver(n, kind)
+    o = _oracle(n, kind)
+    g.prepare(n)
+    o.prepare(n)
+    for it, (ax, b) in enumerate(zip(ax_list, rhs_list)):
+        g.analyze(ap, ai)
+        o.analyze(ap, ai)
+        g.factor(ap, ai, ax)
+        assert o.factor(ap, ai, ax) == 0
+        xg = b.copy()
+        xo = b.copy()
+        g.solve(xg)
+        assert o.solve(xo) == 0
+        assert same_bits(xg, xo), f"solve mismatch at call {it}: max diff {np.nanmax(np.abs(xg - xo))}"
+        sg, so = g.stats(), o.stats()
+        assert sg["factor_count"] == so["factor_count"]
+        assert sg["refactor_count"] == so["refactor_count"]
+        assert sg["nblocks"] == so["nblocks"]
+        assert sg["nnz_l"] == so["nnz_l"] and sg["nnz_u"] == so["nnz_u"]
+        assert sg["rcond"] == so["rcond"] or (np.isnan(sg["rcond"]) and np.isnan(so["rcond"]))
+    g.close()
+    return xg
+
+
+def test_known_answer_2x2(gpu):
+    # native_klu_tests.rs:33-49: x = 2, y = 5/3
+    ap, ai, ax = [0, 2, 3], [0, 1, 1], [2.0, 1.0, 3.0]
+    g = S.GpuSolver(2)
+    g.analyze(ap, ai)
+    g.factor(ap, ai, ax)
+    x = np.array([4.0, 7.0])
+    g.solve(x)
+    assert abs(x[0] - 2.0) < 1e-12 and abs(x[1] - 5.0 / 3.0) < 1e-12
+
+
+def test_known_answer_tridiagonal(gpu):
+    # klu_tests.rs:84-100 (3x3 tridiagonal [[4,-1,0],[-1,4,-1],[0,-1,4]], b=[0,1,0])
+    ap = np.array([0, 2, 5, 7])
+    ai = np.array([0, 1, 0, 1, 2, 1, 2])
+    ax = np.array([4.0, -1.0, -1.0, 4.0, -1.0, -1.0, 4.0])
+    g = S.GpuSolver(3)
+    g.factor(ap, ai, ax)
+    x = np.array([0.0, 1.0, 0.0])
+    g.solve(x)
+    exact = np.linalg.solve(np.array([[4, -1, 0], [-1, 4, -1], [0, -1, 4.0]]), [0, 1, 0.0])
+    assert np.allclose(x, exact, rtol=0, atol=1e-14)
+
+
+def test_refactor_counts_and_pattern_change(gpu):
+    # native_klu_tests.rs:202-239 and :404-429
+    ap1, ai1, ax1 = [0, 1, 2, 3], [0, 1, 2], [2.0, 3.0, 4.0]
+    g = S.GpuSolver(3)
+    g.factor(ap1, ai1, ax1)
+    g.factor(ap1, ai1, [4.0, 6.0, 8.0])
+    st = g.stats()
+    assert st["factor_count"] == 1 and st["refactor_count"] == 1
+    x = np.array([4.0, 9.0, 8.0])
+    g.solve(x)
+    assert np.allclose(x, [1.0, 1.5, 1.0])
+    ap2, ai2 = [0, 2, 5, 7], [0, 1, 0, 1, 2, 1, 2]
+    ax2 = [4.0, 1.0, 1.0, 5.0, 2.0, 2.0, 6.0]
+    g.factor(ap2, ai2, ax2)
+    assert g.stats()["factor_count"] == 2
+    b = np.array([5.0, 14.0, 14.0])
+    x = b.copy()
+    g.solve(x)
+    A = np.array([[4, 1, 0], [1, 5, 2], [0, 2, 6.0]])
+    assert np.abs(A @ x - b).max() < 1e-10
+
+
+def test_errors(gpu):
+    g = S.GpuSolver(3)
+    with pytest.raises(S.SolverError) as e:
+        g.solve(np.zeros(3))
+    assert e.value.variant == "SolveFailed"
+    with pytest.raises(S.SolverError) as e:
+        g.analyze([0, 1, 2], [0, 1])
+    assert e.value.variant == "InvalidMatrix"
+    g.factor([0, 1, 2, 3], [0, 1, 2], [1.0, 2.0, 3.0])
+    with pytest.raises(S.SolverError) as e:
+        g.solve(np.zeros(4))
+    assert e.value.variant == "InvalidMatrix"
+
+
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+def test_mesh_dc_matrix_bit_exact(gpu, kind):
+    """64x64 resistor mesh MNA (n=4098): first factor (host pivots) and three
+    GPU refactors with perturbed values, each solved on the GPU."""
+    c = NL.resistor_mesh()
+    ap, ai, ax, rhs, n = O.build_mna(c)
+    assert n == 4098 and len(ai) == 48649
+    rs = np.random.RandomState(7)
+    axs = [ax] + [ax * (1.0 + 0.01 * rs.randn(len(ax))) for _ in range(3)]
+    rhss = [rhs] + [NL.generate_rhs(n, 42 + i) for i in range(3)]
+    _check_cycle(ap, ai, axs, rhss, n, kind)
+
+
+def test_inverter_chain_btf_matrix_bit_exact(gpu):
+    """500-stage chain DC matrix: structurally singular (the PULSE input is not
+    stamped in DC), so BTF puts a trailing singleton with a 1e-14 pivot and an
+    off-diagonal block in the solve."""
+    c = NL.inverter_chain(500)
+    rs = np.random.RandomState(3)
+    ap, ai, ax0, rhs0, n = O.build_mna(c)
+    axs, rhss = [ax0], [rhs0]
+    for _ in range(3):
+        x = rs.rand(c.node_count)
+        ap2, ai2, ax, rhs, _ = O.build_mna(c, x=x)
+        assert np.array_equal(ap, ap2) and np.array_equal(ai, ai2)
+        axs.append(ax)
+        rhss.append(rhs)
+    _check_cycle(ap, ai, axs, rhss, n)
+
+
+def test_random_patterns_bit_exact(gpu):
+    rs = np.random.RandomState(11)
+    for trial in range(40):
+        n = int(rs.randint(1, 80))
+        ap, ai = random_csc(rs, n, rs.rand() * 0.25)
+        axs = [rs.randn(len(ai)) for _ in range(3)]
+        rhss = [rs.randn(n) for _ in range(3)]
+        for kind in (S.NATIVEKLU, S.SPARSELU):
+            _check_cycle(ap, ai, axs, rhss, n, kind)
+
+
+def test_gpu_factor_values_match_oracle_refactor(gpu):
+    c = NL.resistor_mesh(16, 16)
+    ap, ai, ax, rhs, n = O.build_mna(c)
+    ax2 = ax * 1.5
+    g = S.GpuSolver(n)
+    g.factor(ap, ai, ax)
+    g.factor(ap, ai, ax2)
+    lx, ux, ud = g.factor_values()
+    o = O.NativeKlu(n)
+    o.factor(ap, ai, ax)
+    o.factor(ap, ai, ax2)
+    b = o.block(0)
+    assert same_bits(lx, b["l_values"]) and same_bits(ux, b["u_values"]) and same_bits(ud, b["u_diag"])
