@@ -100,6 +100,55 @@ int32_t rs_solve_device(rs_solver *h, const double *rhs_dev, double *x_dev, void
  * (lx[nnz_l], ux[nnz_u], udiag[n]); position-space order. */
 int32_t rs_get_factor_values(rs_solver *h, double *lx, double *ux, double *udiag);
 
+/* ---- circuit-level boundary: DeviceStamp (stamp.rs:94-121) for a whole
+ * instance list, and the Newton drivers of engine.rs / newton.rs ---------- */
+typedef struct rs_circuit rs_circuit;
+
+enum rs_stamp_mode { RS_MODE_DC = 0, RS_MODE_TRAN_BE = 1, RS_MODE_TRAN_TRAP = 2 };
+
+/* NewtonResult (newton.rs:50-57); reason: 0 Converged, 1 MaxIters, 2 SolverFailure */
+typedef struct rs_newton_result {
+    int32_t converged, iterations, reason, pad;
+    double final_norm;
+} rs_newton_result;
+
+/* TRAN summary (engine.rs:603-631); status: 0 Converged, 2 Failed */
+typedef struct rs_tran_result {
+    int32_t status, accepted, rejected, newton_iters_total, factor_calls, dc_iters;
+    int64_t npoints;
+} rs_tran_result;
+
+/* Copies the instance list to the device (node numbers as in rs_circuit.h). */
+rs_circuit *rs_circuit_create(const rs_device *dev, int64_t ndev, int64_t node_count);
+void rs_circuit_destroy(rs_circuit *c);
+int32_t rs_circuit_last_error(const rs_circuit *c, char *buf, int64_t len);
+/* Unmerged CSC pattern the stamp loop produces in `mode` (the engine closure
+ * of engine.rs:262-283 / :485-501): call with ap = ai = NULL for the sizes. */
+int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap,
+                           int64_t *ai);
+/* TransientState per device (stamp.rs:699-718), device-index keyed. */
+int32_t rs_circuit_set_cap_state(rs_circuit *c, const double *cap_v, const double *cap_i,
+                                 const uint8_t *has_v, const uint8_t *has_i);
+/* One build(x) of the engine closure on the device: ax (unmerged, nnz) and
+ * rhs (n) exactly as SparseBuilder::finalize and the RHS vector hold them. */
+int32_t rs_stamp(rs_circuit *c, int32_t mode, const double *x, int64_t x_len, double t, double dt,
+                 double gmin, double source_scale, double *ax_out, double *rhs_out);
+/* Engine::run_dc_result (engine.rs:249-311) with run_newton_with_stepping. */
+int32_t rs_dc_op(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double *x_out, int64_t x_cap,
+                 int64_t *x_len, rs_newton_result *res);
+/* Engine::run_tran_result_with_params (engine.rs:333-632): BE + TRAP solves,
+ * Milne LTE, PI step control, PULSE breakpoints. Stores up to max_points
+ * (time, x[x_stride]) pairs. */
+int32_t rs_tran(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double tstep, double tstop,
+                double tstart, double tmax, int64_t max_points, double *times, double *sols,
+                int64_t x_stride, rs_tran_result *res);
+/* Fixed-step trapezoidal transient (no reference counterpart): DC OP, then
+ * one TRAP Newton solve per step of dt with the accepted-step state update.
+ * status: 0 ok, 1 DC OP failed, 2 a step failed. */
+int32_t rs_tran_fixed(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double dt,
+                      int64_t nsteps, double *x_final, int64_t x_cap, int64_t *x_len,
+                      int32_t *newton_iters, int32_t *status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,8 @@
 
 #include <stdint.h>
 
+#include "rs_circuit.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -40,6 +42,12 @@ void rs_host_factor_block_get(const rs_host_factor *h, int64_t b, int64_t *col_p
 /* out[8] = {factor levels, max level width, L-solve levels, U-solve levels,
  *           scatter runs, off-diagonal entries, nnz(L), nnz(U)} */
 void rs_host_factor_plan_stats(const rs_host_factor *h, int64_t *out);
+
+/* Unmerged CSC pattern of one stamp pass (mode 0 DC, else TRAN) as
+ * SparseBuilder::finalize produces it (mna.rs:320-347); ap/ai may be NULL to
+ * query n and nnz. Returns 0, or 6 for an unsupported device kind. */
+int32_t rs_host_stamp_pattern(const rs_device *dev, int64_t ndev, int64_t node_count,
+                              int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap, int64_t *ai);
 
 #ifdef __cplusplus
 }

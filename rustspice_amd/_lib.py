@@ -27,7 +27,9 @@ EXPORTS = [
     "rs_solve_device", "rs_get_factor_values",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
-    "rs_host_factor_plan_stats",
+    "rs_host_factor_plan_stats", "rs_host_stamp_pattern",
+    "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
+    "rs_circuit_set_cap_state", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
 ]
 
 
@@ -92,5 +94,8 @@ def lib():
     L.rs_host_factor_block_sizes.argtypes = [vp, C.c_int64] + [C.POINTER(C.c_int64)] * 3
     L.rs_host_factor_block_get.argtypes = [vp, C.c_int64] + [vp] * 9
     L.rs_host_factor_plan_stats.argtypes = [vp, i64p]
+    L.rs_host_stamp_pattern.restype = C.c_int32
+    L.rs_host_stamp_pattern.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32,
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64), vp, vp]
     _LIB = L
     return L

@@ -1,0 +1,617 @@
+// engine.cpp - rs_circuit: device-resident stamping plus the DC/TRAN Newton
+// drivers of crates/sim-core/src/engine.rs over the GPU solver.
+//
+// Per Newton iteration (newton.rs:65-117) everything on the device: stamp
+// evaluation and assembly into the unmerged CSC, refactor (or the first,
+// pivoting factor of a new pattern on the host), solve. The host keeps the
+// control flow that decides convergence, damping, LTE acceptance and the next
+// time step (newton.rs:93-107, engine.rs:463-602, analysis.rs:147-742,
+// waveform.rs:459-645) on the same doubles the reference computes, so
+// iteration counts and accepted steps follow the reference exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rsgpu.h"
+#include "solver_impl.hpp"
+#include "stamp.hpp"
+
+namespace rsgpu {
+hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, StampArgs a,
+                       const double* cap_v, const double* cap_i, const uint8_t* has_v,
+                       const uint8_t* has_i, double* sv, hipStream_t s);
+hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
+                           const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
+                           hipStream_t s);
+hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
+                            int full, double* cap_v, double* cap_i, uint8_t* has_v,
+                            uint8_t* has_i, hipStream_t s);
+}  // namespace rsgpu
+
+using namespace rsgpu;
+
+namespace {
+struct DevPattern {
+    bool built = false;
+    StampPattern p;
+    DevBuf<int32_t> ax_src, rhs_ptr, rhs_src;
+};
+}  // namespace
+
+struct rs_circuit {
+    std::vector<rs_device> dev;
+    i64 node_count = 0;
+    DevBuf<rs_device> d_dev;
+    DevPattern pat[2];  // 0: DC, 1: TRAN (BE and TRAP share the pattern)
+    DevBuf<double> d_sv, d_ax, d_rhs, d_x, d_xnew, d_cap_v, d_cap_i;
+    DevBuf<uint8_t> d_has_v, d_has_i;
+    std::string err;
+    int64_t stamp_calls = 0, newton_iters = 0, factor_calls = 0;
+};
+
+namespace {
+
+int32_t cfail(rs_circuit* c, int32_t code, const std::string& m) {
+    c->err = m;
+    return code;
+}
+int32_t chip(rs_circuit* c, hipError_t e, const char* what) {
+    c->err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+    return RS_ERR_DEVICE;
+}
+int32_t from_solver(rs_circuit* c, rs_solver* s, int32_t rc) {
+    c->err = s->err;
+    return rc;
+}
+
+int32_t get_pattern(rs_circuit* c, int mode, DevPattern*& out) {
+    int k = mode == MODE_DC ? 0 : 1;
+    DevPattern& dp = c->pat[k];
+    if (!dp.built) {
+        try {
+            build_stamp_pattern(c->dev.data(), (i64)c->dev.size(), c->node_count,
+                                k == 0 ? MODE_DC : MODE_TRAN_BE, dp.p);
+        } catch (const std::exception& ex) {
+            return cfail(c, RS_ERR_INVALID_MATRIX, std::string("Invalid matrix: ") + ex.what());
+        }
+        hipError_t e;
+        if ((e = dp.ax_src.ensure(dp.p.ax_src.size())) ||
+            (e = dp.rhs_ptr.ensure(dp.p.rhs_ptr.size())) ||
+            (e = dp.rhs_src.ensure(dp.p.rhs_src.size())))
+            return chip(c, e, "hipMalloc(pattern)");
+        if ((e = hipMemcpy(dp.ax_src.p, dp.p.ax_src.data(), dp.p.ax_src.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(dp.rhs_ptr.p, dp.p.rhs_ptr.data(), dp.p.rhs_ptr.size() * 4, hipMemcpyHostToDevice)) ||
+            (!dp.p.rhs_src.empty() &&
+             (e = hipMemcpy(dp.rhs_src.p, dp.p.rhs_src.data(), dp.p.rhs_src.size() * 4, hipMemcpyHostToDevice))))
+            return chip(c, e, "hipMemcpy(pattern)");
+        size_t nmax = (size_t)dp.p.n;
+        if ((e = c->d_ax.ensure(std::max(c->d_ax.n, dp.p.ai.size()))) ||
+            (e = c->d_rhs.ensure(std::max(c->d_rhs.n, nmax))) ||
+            (e = c->d_x.ensure(std::max(c->d_x.n, nmax))) ||
+            (e = c->d_xnew.ensure(std::max(c->d_xnew.n, nmax))))
+            return chip(c, e, "hipMalloc(vectors)");
+        dp.built = true;
+    }
+    out = &dp;
+    return RS_OK;
+}
+
+// Stamp on the device from a device-resident x.
+int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t, double dt,
+                  double gmin, double scale, DevPattern** pout, hipStream_t s) {
+    DevPattern* dp;
+    int32_t rc = get_pattern(c, mode, dp);
+    if (rc) return rc;
+    StampArgs a{t, dt, gmin, scale, mode, 0};
+    hipError_t e = launch_eval(c->d_dev.p, (int)c->dev.size(), x_dev, (int)xl, a, c->d_cap_v.p,
+                               c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s);
+    if (e) return chip(c, e, "k_eval");
+    e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p,
+                        (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, s);
+    if (e) return chip(c, e, "k_assemble");
+    c->stamp_calls++;
+    if (pout) *pout = dp;
+    return RS_OK;
+}
+
+double norm2(const double* v, i64 n) {
+    double s = 0.0;
+    for (i64 i = 0; i < n; i++) s += v[i] * v[i];
+    return std::sqrt(s);
+}
+
+// run_newton (newton.rs:65-117)
+int32_t run_newton(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::vector<double>& x,
+                   int mode, double t, double dt, double gmin, double scale,
+                   rs_newton_result* r) {
+    hipStream_t s = nullptr;
+    r->converged = 0;
+    r->iterations = 0;
+    r->reason = 1;
+    r->final_norm = 0.0;
+    double damping = o->damping;
+    double prev = 1.7976931348623157e308;
+    std::vector<double> xn, dx;
+    for (int it = 0; it < o->max_iters; it++) {
+        r->iterations = it + 1;
+        c->newton_iters++;
+        hipError_t e = hipMemcpyAsync(c->d_x.p, x.data(), x.size() * 8, hipMemcpyHostToDevice, s);
+        if (e) return chip(c, e, "hipMemcpy(x)");
+        DevPattern* dp;
+        int32_t rc = stamp_dev(c, mode, c->d_x.p, (i64)x.size(), t, dt, gmin, scale, &dp, s);
+        if (rc) return rc;
+        const i64 n = dp->p.n;
+        rs_prepare(sol, n);
+        rc = solver_factor_dev(sol, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
+                               c->d_ax.p, s);
+        c->factor_calls++;
+        if (rc == RS_OK) rc = solver_solve_dev(sol, c->d_rhs.p, c->d_xnew.p, s);
+        if (rc != RS_OK) {
+            if (rc == RS_ERR_DEVICE) return from_solver(c, sol, rc);
+            r->reason = 2;  // SolverFailure
+            c->err = sol->err;
+            break;
+        }
+        xn.resize(n);
+        e = hipMemcpyAsync(xn.data(), c->d_xnew.p, n * 8, hipMemcpyDeviceToHost, s);
+        if (!e) e = hipStreamSynchronize(s);
+        if (e) return chip(c, e, "hipMemcpy(x_new)");
+        const i64 nd = std::min<i64>(n, (i64)x.size());
+        dx.resize(nd);
+        for (i64 i = 0; i < nd; i++) dx[i] = xn[i] - x[i];
+        const double dxn = norm2(dx.data(), nd);
+        r->final_norm = norm2(xn.data(), n);
+        bool conv = true;
+        for (i64 i = 0; i < nd; i++)
+            if (!(std::fabs(dx[i]) <= o->abs_tol + o->rel_tol * std::fabs(xn[i]))) {
+                conv = false;
+                break;
+            }
+        if (conv) {
+            x = xn;
+            r->converged = 1;
+            r->reason = 0;
+            break;
+        }
+        if (dxn > prev && damping > o->damping_min) {
+            double d2 = damping * 0.5;
+            damping = d2 >= o->damping_min ? d2 : o->damping_min;
+        }
+        prev = dxn;
+        for (i64 i = 0; i < nd; i++) x[i] = x[i] + damping * (xn[i] - x[i]);
+    }
+    return RS_OK;
+}
+
+// run_newton_with_stepping (newton.rs:142-185)
+int32_t run_newton_stepping(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o,
+                            std::vector<double>& x, int mode, double t, double dt,
+                            rs_newton_result* out, int32_t* iters_total) {
+    double gstart = o->gmin * 1e3;
+    if (gstart < 1e-6) gstart = 1e-6;
+    rs_newton_result last{0, 0, 1, 0, 0.0};
+    for (int gi = 0; gi <= o->gmin_steps; gi++) {
+        double gmin = o->gmin_steps == 0
+                          ? o->gmin
+                          : gstart + (o->gmin - gstart) * ((double)gi / (double)o->gmin_steps);
+        for (int si = 0; si <= o->source_steps; si++) {
+            double scale = o->source_steps == 0 ? 1.0 : (double)si / (double)o->source_steps;
+            rs_newton_result r;
+            int32_t rc = run_newton(c, sol, o, x, mode, t, dt, gmin, scale, &r);
+            if (rc) return rc;
+            if (iters_total) *iters_total += r.iterations;
+            if (r.converged) {
+                *out = r;
+                return RS_OK;
+            }
+            last = r;
+        }
+    }
+    *out = last;
+    return RS_OK;
+}
+
+double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+struct Breakpoints {  // BreakpointManager (waveform.rs:380-645)
+    std::vector<double> t;
+    int settle = 0, steps = 5;
+    double factor = 0.1;
+    void add(double time) {
+        if (time < 0.0) return;
+        for (double v : t)
+            if (v == time) return;
+        t.insert(std::upper_bound(t.begin(), t.end(), time), time);
+    }
+    void pulse(const double* p, double tstop) {
+        double td = p[RS_PP_TD], tr = p[RS_PP_TR], tf = p[RS_PP_TF], pw = p[RS_PP_PW], per = p[RS_PP_PER];
+        double tt = td;
+        int64_t cnt = 0;
+        if (td > 0.0) add(td);
+        while (tt <= tstop && cnt < 10000) {
+            add(tt);
+            double th = tt + tr;
+            if (th > tstop) break;
+            add(th);
+            double tfa = th + pw;
+            if (tfa > tstop) break;
+            add(tfa);
+            double tl = tfa + tf;
+            if (tl > tstop) break;
+            add(tl);
+            if (per > 0.0) {
+                tt = td + per * (double)(cnt + 1);
+                cnt++;
+            } else {
+                break;
+            }
+        }
+    }
+    double limit_dt(double tc, double pdt, double mm) const {
+        for (double b : t)
+            if (b > tc) {
+                double ttb = b - tc;
+                if (pdt >= ttb) return ttb >= mm ? ttb : mm;
+                if (pdt > 0.9 * ttb) return ttb;
+                double rem = ttb - pdt;
+                if (rem < 0.1 * pdt && rem < mm * 10.0) return ttb;
+                break;
+            }
+        return pdt;
+    }
+    void update(double t0, double t1) {
+        for (double b : t)
+            if (b > t0 && b <= t1) {
+                settle = steps;
+                break;
+            }
+        if (settle > 0) settle--;
+    }
+};
+
+void lte_milne(const std::vector<double>& xb, const std::vector<double>& xt, double abs_tol,
+               double rel_tol, double out[5]) {  // analysis.rs:147-200
+    double mr = 0.0, raw = 0.0;
+    i64 mn = 0, n = (i64)std::min(xb.size(), xt.size());
+    for (i64 i = 0; i < n; i++) {
+        double be = xb[i], tr = xt[i];
+        if (std::fabs(be) < 1e-30 && std::fabs(tr) < 1e-30) continue;
+        double lte = std::fabs(tr - be) / 3.0;
+        double ab = std::fabs(be), at = std::fabs(tr);
+        double tol = abs_tol + rel_tol * (ab >= at ? ab : at);
+        double ratio = tol > 0.0 ? lte / tol : 0.0;
+        if (ratio > mr) {
+            mr = ratio;
+            mn = i;
+            raw = lte;
+        }
+    }
+    double f = 2.0;
+    if (mr > 1e-10) f = clampd(std::pow(1.0 / mr, 0.5) * 0.9, 0.1, 2.0);
+    out[0] = mr;
+    out[1] = (double)mn;
+    out[2] = mr <= 1.0 ? 1.0 : 0.0;
+    out[3] = f;
+    out[4] = raw;
+}
+
+int32_t reset_cap_state(rs_circuit* c) {
+    hipError_t e;
+    size_t nd = c->dev.size();
+    if ((e = hipMemset(c->d_has_v.p, 0, nd)) || (e = hipMemset(c->d_has_i.p, 0, nd)) ||
+        (e = hipMemset(c->d_cap_v.p, 0, nd * 8)) || (e = hipMemset(c->d_cap_i.p, 0, nd * 8)))
+        return chip(c, e, "hipMemset(state)");
+    return RS_OK;
+}
+
+// Capacitor history update from a host x (uploaded to d_x).
+int32_t cap_update(rs_circuit* c, const std::vector<double>& x, double dt, int full,
+                   double* cv, double* ci, uint8_t* hv, uint8_t* hi) {
+    hipError_t e = hipMemcpy(c->d_x.p, x.data(), x.size() * 8, hipMemcpyHostToDevice);
+    if (e) return chip(c, e, "hipMemcpy(x)");
+    e = launch_cap_state(c->d_dev.p, (int)c->dev.size(), c->d_x.p, (int)x.size(), dt, full, cv, ci,
+                         hv, hi, nullptr);
+    if (e) return chip(c, e, "k_cap_state");
+    return RS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_count) {
+    rs_circuit* c = new rs_circuit();
+    c->dev.assign(dev, dev + ndev);
+    c->node_count = node_count;
+    hipError_t e;
+    size_t nd = (size_t)ndev;
+    if ((e = c->d_dev.ensure(nd)) || (e = c->d_sv.ensure(nd * kSlotsPerDevice)) ||
+        (e = c->d_cap_v.ensure(nd)) || (e = c->d_cap_i.ensure(nd)) || (e = c->d_has_v.ensure(nd)) ||
+        (e = c->d_has_i.ensure(nd))) {
+        c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return c;
+    }
+    if (nd && (e = hipMemcpy(c->d_dev.p, dev, nd * sizeof(rs_device), hipMemcpyHostToDevice)))
+        c->err = std::string("hipMemcpy: ") + hipGetErrorString(e);
+    else if (nd && (e = hipMemset(c->d_sv.p, 0, nd * kSlotsPerDevice * 8)))
+        c->err = std::string("hipMemset: ") + hipGetErrorString(e);
+    else
+        reset_cap_state(c);
+    return c;
+}
+
+void rs_circuit_destroy(rs_circuit* c) {
+    if (!c) return;
+    for (auto& p : c->pat) {
+        p.ax_src.release();
+        p.rhs_ptr.release();
+        p.rhs_src.release();
+    }
+    c->d_dev.release();
+    c->d_sv.release();
+    c->d_ax.release();
+    c->d_rhs.release();
+    c->d_x.release();
+    c->d_xnew.release();
+    c->d_cap_v.release();
+    c->d_cap_i.release();
+    c->d_has_v.release();
+    c->d_has_i.release();
+    delete c;
+}
+
+int32_t rs_circuit_last_error(const rs_circuit* c, char* buf, int64_t len) {
+    if (len > 0) std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
+    return (int32_t)c->err.size();
+}
+
+int32_t rs_circuit_pattern(rs_circuit* c, int32_t mode, int64_t* n, int64_t* nnz, int64_t* ap,
+                           int64_t* ai) {
+    DevPattern* dp;
+    int32_t rc = get_pattern(c, mode, dp);
+    if (rc) return rc;
+    if (n) *n = dp->p.n;
+    if (nnz) *nnz = (int64_t)dp->p.ai.size();
+    if (ap) std::memcpy(ap, dp->p.ap.data(), dp->p.ap.size() * 8);
+    if (ai && !dp->p.ai.empty()) std::memcpy(ai, dp->p.ai.data(), dp->p.ai.size() * 8);
+    return RS_OK;
+}
+
+int32_t rs_circuit_set_cap_state(rs_circuit* c, const double* cap_v, const double* cap_i,
+                                 const uint8_t* has_v, const uint8_t* has_i) {
+    size_t nd = c->dev.size();
+    hipError_t e;
+    if ((e = hipMemcpy(c->d_cap_v.p, cap_v, nd * 8, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(c->d_cap_i.p, cap_i, nd * 8, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(c->d_has_v.p, has_v, nd, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(c->d_has_i.p, has_i, nd, hipMemcpyHostToDevice)))
+        return chip(c, e, "hipMemcpy(state)");
+    return RS_OK;
+}
+
+int32_t rs_stamp(rs_circuit* c, int32_t mode, const double* x, int64_t x_len, double t, double dt,
+                 double gmin, double source_scale, double* ax_out, double* rhs_out) {
+    DevPattern* dp;
+    int32_t rc = get_pattern(c, mode, dp);
+    if (rc) return rc;
+    hipError_t e = c->d_x.ensure(std::max<size_t>(c->d_x.n, (size_t)x_len));
+    if (e) return chip(c, e, "hipMalloc(x)");
+    e = hipMemcpy(c->d_x.p, x, x_len * 8, hipMemcpyHostToDevice);
+    if (e) return chip(c, e, "hipMemcpy(x)");
+    rc = stamp_dev(c, mode, c->d_x.p, x_len, t, dt, gmin, source_scale, &dp, nullptr);
+    if (rc) return rc;
+    if ((e = hipMemcpy(ax_out, c->d_ax.p, dp->p.ai.size() * 8, hipMemcpyDeviceToHost)) ||
+        (e = hipMemcpy(rhs_out, c->d_rhs.p, dp->p.n * 8, hipMemcpyDeviceToHost)))
+        return chip(c, e, "hipMemcpy(out)");
+    return RS_OK;
+}
+
+int32_t rs_dc_op(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double* x_out,
+                 int64_t x_cap, int64_t* x_len, rs_newton_result* res) {
+    std::vector<double> x(c->node_count, 0.0);
+    rs_prepare(sol, c->node_count);  // engine.rs:254
+    rs_newton_result r{};
+    int32_t rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &r, nullptr);
+    if (rc) return rc;
+    *res = r;
+    if (x_out) std::memcpy(x_out, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
+    if (x_len) *x_len = r.converged ? (int64_t)x.size() : 0;
+    return RS_OK;
+}
+
+int32_t rs_tran(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double tstep, double tstop,
+                double tstart, double tmax, int64_t max_points, double* times, double* sols,
+                int64_t x_stride, rs_tran_result* res) {
+    std::memset(res, 0, sizeof(*res));
+    const i64 nd = (i64)c->dev.size();
+    std::vector<double> x(c->node_count, 0.0);
+    int32_t rc = reset_cap_state(c);
+    if (rc) return rc;
+    // BE and TRAP keep separate histories (engine.rs:351-355)
+    DevBuf<double> bv, bi;
+    DevBuf<uint8_t> bhv, bhi;
+    hipError_t e;
+    if ((e = bv.ensure(nd)) || (e = bi.ensure(nd)) || (e = bhv.ensure(nd)) || (e = bhi.ensure(nd)))
+        return chip(c, e, "hipMalloc(state)");
+    if ((e = hipMemset(bhv.p, 0, nd)) || (e = hipMemset(bhi.p, 0, nd)) ||
+        (e = hipMemset(bv.p, 0, nd * 8)) || (e = hipMemset(bi.p, 0, nd * 8)))
+        return chip(c, e, "hipMemset(state)");
+    rs_prepare(sol, c->node_count);
+    const double min_dt = std::max(tstep * 1e-6, 1e-15);
+    const double max_dt = std::min(tmax, tstop / 10.0);
+    const double abs_tol = o->abs_tol, rel_tol = o->rel_tol;
+    const double k_p = 0.7 / 2.0, k_i = 0.4 / 2.0;
+    bool has_prev = false;
+    double prev_err = 0.0;
+    int ctl_consec = 0;
+    Breakpoints bp;
+    for (const auto& d : c->dev)
+        if ((d.kind == RS_DEV_V || d.kind == RS_DEV_I) && (d.flags & RS_F_WAVEFORM))
+            bp.pulse(d.p, tstop);
+    double t = tstart, dt = std::min(tstep, max_dt);
+    int consec = 0;
+    int32_t iters = 0;
+    rs_newton_result dcr{};
+    int64_t fc0 = c->factor_calls;
+    rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &dcr, &iters);
+    if (rc) return rc;
+    res->dc_iters = dcr.iterations;
+    if (!dcr.converged) {
+        res->status = 2;
+        res->factor_calls = (int32_t)(c->factor_calls - fc0);
+        return RS_OK;
+    }
+    int64_t np = 0;
+    auto store = [&](double tt) {
+        if (np < max_points) {
+            if (times) times[np] = tt;
+            if (sols)
+                for (i64 q = 0; q < x_stride; q++) sols[np * x_stride + q] = q < (i64)x.size() ? x[q] : 0.0;
+        }
+        np++;
+    };
+    store(tstart);
+    // update_transient_state into both histories (engine.rs:448-449)
+    if ((rc = cap_update(c, x, 0.0, 0, bv.p, bi.p, bhv.p, bhi.p)) ||
+        (rc = cap_update(c, x, 0.0, 0, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)))
+        return rc;
+    int status = 0;
+    while (t < tstop) {
+        dt = bp.limit_dt(t, dt, min_dt);
+        if (bp.settle > 0) dt = std::max(dt * bp.factor, min_dt);
+        if (t + dt > tstop) dt = tstop - t;
+        const double tt = t + dt;
+        // BE solve on the BE history
+        std::swap(bv.p, c->d_cap_v.p);
+        std::swap(bi.p, c->d_cap_i.p);
+        std::swap(bhv.p, c->d_has_v.p);
+        std::swap(bhi.p, c->d_has_i.p);
+        std::vector<double> xb = x;
+        rs_newton_result rb{};
+        rc = run_newton_stepping(c, sol, o, xb, MODE_TRAN_BE, tt, dt, &rb, &iters);
+        std::swap(bv.p, c->d_cap_v.p);
+        std::swap(bi.p, c->d_cap_i.p);
+        std::swap(bhv.p, c->d_has_v.p);
+        std::swap(bhi.p, c->d_has_i.p);
+        if (rc) return rc;
+        if (!rb.converged) {
+            consec++;
+            res->rejected++;
+            if (consec >= 10) {
+                status = 2;
+                break;
+            }
+            dt = std::max(dt * 0.25, min_dt);
+            continue;
+        }
+        std::vector<double> xt = x;
+        rs_newton_result rt{};
+        rc = run_newton_stepping(c, sol, o, xt, MODE_TRAN_TRAP, tt, dt, &rt, &iters);
+        if (rc) return rc;
+        if (!rt.converged) {
+            consec++;
+            res->rejected++;
+            if (consec >= 10) {
+                status = 2;
+                break;
+            }
+            dt = std::max(dt * 0.25, min_dt);
+            continue;
+        }
+        double lte[5];
+        lte_milne(xb, xt, abs_tol, rel_tol, lte);
+        const double err = lte[0];
+        const bool accept = lte[2] != 0.0 && !(err > 1.0);
+        double dt_new;
+        if (accept) {  // AdaptiveStepController::process_lte (analysis.rs:720-742)
+            double en = err > 1e-10 ? err : 1e-10;
+            double pf = std::pow(1.0 / en, k_p);
+            double inf = 1.0;
+            if (has_prev) inf = std::pow(1.0 / (prev_err > 1e-10 ? prev_err : 1e-10), k_i);
+            double fac = clampd(pf * inf * 0.9, 0.1, 2.0);
+            has_prev = true;
+            prev_err = err;
+            ctl_consec = 0;
+            dt_new = clampd(dt * fac, min_dt, max_dt);
+        } else {
+            ctl_consec++;
+            if (ctl_consec >= 3)
+                dt_new = ctl_consec <= 5 ? dt * 0.25 : (ctl_consec <= 10 ? dt * 0.1 : min_dt);
+            else
+                dt_new = clampd(dt * lte[3], min_dt, max_dt);
+        }
+        if (accept) {
+            const double tn = t + dt;
+            bp.update(t, tn);
+            x = xt;
+            if ((rc = cap_update(c, x, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)) ||
+                (rc = cap_update(c, x, dt, 0, bv.p, bi.p, bhv.p, bhi.p)))
+                return rc;
+            store(tn);
+            t = tn;
+            res->accepted++;
+            consec = 0;
+            dt = clampd(dt_new, min_dt, max_dt);
+        } else {
+            res->rejected++;
+            consec++;
+            if (consec >= 10) {
+                status = 2;
+                break;
+            }
+            dt = clampd(dt_new, min_dt, max_dt);
+        }
+    }
+    res->status = status;
+    res->npoints = np;
+    res->newton_iters_total = iters;
+    res->factor_calls = (int32_t)(c->factor_calls - fc0);
+    bv.release();
+    bi.release();
+    bhv.release();
+    bhi.release();
+    return RS_OK;
+}
+
+int32_t rs_tran_fixed(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double dt,
+                      int64_t nsteps, double* x_final, int64_t x_cap, int64_t* x_len,
+                      int32_t* newton_iters, int32_t* status_out) {
+    std::vector<double> x(c->node_count, 0.0);
+    int32_t rc = reset_cap_state(c);
+    if (rc) return rc;
+    rs_prepare(sol, c->node_count);
+    int32_t it = 0;
+    rs_newton_result r{};
+    rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &r, &it);
+    if (rc) return rc;
+    int32_t status = r.converged ? 0 : 1;
+    if (!status) {
+        rc = cap_update(c, x, 0.0, 0, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p);
+        if (rc) return rc;
+        double t = 0.0;
+        for (int64_t k = 0; k < nsteps; k++) {
+            double tt = t + dt;
+            rc = run_newton_stepping(c, sol, o, x, MODE_TRAN_TRAP, tt, dt, &r, &it);
+            if (rc) return rc;
+            if (!r.converged) {
+                status = 2;
+                break;
+            }
+            rc = cap_update(c, x, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p);
+            if (rc) return rc;
+            t = tt;
+        }
+    }
+    if (x_final) std::memcpy(x_final, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
+    if (x_len) *x_len = (int64_t)x.size();
+    if (newton_iters) *newton_iters = it;
+    if (status_out) *status_out = status;
+    return RS_OK;
+}
+
+}  // extern "C"

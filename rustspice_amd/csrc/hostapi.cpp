@@ -6,6 +6,7 @@
 
 #include "../../include/rsgpu_host.h"
 #include "plan.hpp"
+#include "stamp.hpp"
 
 using namespace rsgpu;
 
@@ -135,6 +136,21 @@ void rs_host_factor_plan_stats(const rs_host_factor* h, int64_t* out) {
     out[5] = p.nod;
     out[6] = p.nl;
     out[7] = p.nu;
+}
+
+int32_t rs_host_stamp_pattern(const rs_device* dev, int64_t ndev, int64_t node_count,
+                              int32_t mode, int64_t* n, int64_t* nnz, int64_t* ap, int64_t* ai) {
+    StampPattern p;
+    try {
+        build_stamp_pattern(dev, ndev, node_count, mode == 0 ? MODE_DC : MODE_TRAN_BE, p);
+    } catch (...) {
+        return 6;
+    }
+    if (n) *n = p.n;
+    if (nnz) *nnz = (int64_t)p.ai.size();
+    if (ap) std::memcpy(ap, p.ap.data(), p.ap.size() * sizeof(int64_t));
+    if (ai && !p.ai.empty()) std::memcpy(ai, p.ai.data(), p.ai.size() * sizeof(int64_t));
+    return 0;
 }
 
 }  // extern "C"

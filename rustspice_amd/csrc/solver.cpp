@@ -15,69 +15,10 @@
 
 #include "../../include/rsgpu.h"
 #include "plan.hpp"
+#include "solver_impl.hpp"
 
-namespace rsgpu {
-hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
-                           double* vals, double* wpool, int max_n, bool lds, int threads,
-                           hipStream_t stream);
-hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
-                        const double* vals, const double* rhs, double* x, double* wpool, int max_n,
-                        bool lds, int threads, hipStream_t stream);
-size_t device_max_lds();
-}  // namespace rsgpu
 
 using namespace rsgpu;
-
-namespace {
-enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
-constexpr int kThreads = 256;
-
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t ensure(size_t cnt) {
-        if (cnt <= n && p) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        hipError_t e = hipMalloc((void**)&p, (cnt ? cnt : 1) * sizeof(T));
-        if (e == hipSuccess) n = cnt;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-}  // namespace
-
-struct rs_solver {
-    int kind = RS_SOLVER_NATIVEKLU;
-    i64 n = 0;
-    double pivot_tol = 0.001;
-    int ordering = 0;
-    bool use_btf = true;
-    i64 btf_min_blocks = 2;
-    int state = ST_EMPTY;
-    std::vector<i64> last_ap, last_ai;
-    HostFactor hf;
-    PlanHost ph;
-    PlanDev pd{};
-    bool plan_on_device = false;
-    i64 factor_count = 0, refactor_count = 0;
-    double rcond = 0.0;
-    bool last_parallel = false;
-    i64 threads_hint = 0;
-    std::string err;
-    size_t plan_bytes = 0;
-    bool lds_refactor = false, lds_solve = false;
-    DevBuf<PlanDev> d_desc;
-    DevBuf<int32_t> d_ip;
-    DevBuf<double> d_vals, d_ax, d_rhs, d_x, d_work;
-    std::vector<double> h_ud;
-};
 
 namespace {
 
@@ -277,11 +218,20 @@ int32_t host_full_factor(rs_solver* h, const i64* ap, const i64* ai, const doubl
     }
     int32_t rc = upload_plan(h, ap[n]);
     if (rc) return rc;
+    // the solve reads BTF off-diagonal entries from the factored ax
+    hipError_t e = hipMemcpy(h->d_ax.p, ax, ap[n] * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
     return upload_values(h);
 }
 
 int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool sync_rcond) {
-    hipError_t e = launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
+    hipError_t e;
+    if (h->ph.nod > 0 && ax_dev != h->d_ax.p) {  // off-diagonal values used by the solve
+        e = hipMemcpyAsync(h->d_ax.p, ax_dev, h->last_ap.back() * sizeof(double),
+                           hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
+    }
+    e = launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
                                    (int)h->n, h->lds_refactor, kThreads, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_refactor launch");
     if (sync_rcond) {
@@ -291,6 +241,9 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
             return hip_fail(h, e, "hipMemcpy(udiag)");
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(h, e, "refactor");
         compute_rcond(h, h->h_ud.data(), h->n);
+        h->rcond_stale = false;
+    } else {
+        h->rcond_stale = true;
     }
     return RS_OK;
 }
@@ -311,6 +264,43 @@ size_t device_max_lds() {
     }
     return v;
 }
+
+int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail,
+                          const double* ax_dev, hipStream_t s) {
+    if (h->state == ST_EMPTY || !pattern_matches(h, ap, apl, ai, ail)) {
+        if (apl != h->n + 1) return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: bad ap length");
+        int32_t rc = do_analyze(h, ap, apl, ai, ail);
+        if (rc) return rc;
+    }
+    if (h->state == ST_FACTORED && h->plan_on_device) {
+        int32_t rc = gpu_refactor(h, ax_dev, s, false);
+        if (rc) return rc;
+        h->refactor_count++;
+        h->last_parallel = true;
+    } else {
+        std::vector<double> ax(ap[h->n]);
+        hipError_t e = hipMemcpyAsync(ax.data(), ax_dev, ax.size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
+        int32_t rc = host_full_factor(h, ap, ai, ax.data());
+        if (rc) return rc;
+        h->factor_count++;
+        h->last_parallel = false;
+    }
+    h->state = ST_FACTORED;
+    return RS_OK;
+}
+
+int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
+    if (h->state != ST_FACTORED || !h->plan_on_device)
+        return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    hipError_t e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev,
+                                h->d_work.p, (int)h->n, h->lds_solve, kThreads, s);
+    if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
+    return RS_OK;
+}
+
 }  // namespace rsgpu
 
 extern "C" {
@@ -422,7 +412,16 @@ void rs_set_config(rs_solver* h, double pivot_tol, int32_t ordering, int32_t use
     h->use_btf = use_btf != 0;
 }
 
-int32_t rs_get_stats(const rs_solver* h, rs_solver_stats* o) {
+int32_t rs_get_stats(const rs_solver* hc, rs_solver_stats* o) {
+    rs_solver* h = const_cast<rs_solver*>(hc);
+    if (h->rcond_stale && h->plan_on_device) {  // refactors on the device defer rcond
+        h->h_ud.resize(h->n);
+        if (hipMemcpy(h->h_ud.data(), h->d_vals.p + h->pd.o_ud, h->n * 8, hipMemcpyDeviceToHost) ==
+            hipSuccess) {
+            compute_rcond(h, h->h_ud.data(), h->n);
+            h->rcond_stale = false;
+        }
+    }
     std::memset(o, 0, sizeof(*o));
     o->factor_count = h->factor_count;
     o->refactor_count = h->refactor_count;

@@ -1,0 +1,80 @@
+// solver_impl.hpp - internal state of rs_solver shared by the C ABI
+// (solver.cpp) and the Newton/TRAN drivers (engine.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rsgpu.h"
+#include "plan.hpp"
+
+namespace rsgpu {
+hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
+                           double* vals, double* wpool, int max_n, bool lds, int threads,
+                           hipStream_t stream);
+hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
+                        const double* vals, const double* rhs, double* x, double* wpool, int max_n,
+                        bool lds, int threads, hipStream_t stream);
+size_t device_max_lds();
+
+enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
+constexpr int kThreads = 256;
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, (cnt ? cnt : 1) * sizeof(T));
+        if (e == hipSuccess) n = cnt;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Factor a pattern whose values are already in device memory: refactor on
+// the GPU when the pattern is the factored one, else a host first factor.
+int32_t solver_factor_dev(struct ::rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail,
+                          const double* ax_dev, hipStream_t s);
+int32_t solver_solve_dev(struct ::rs_solver* h, const double* rhs_dev, double* x_dev,
+                         hipStream_t s);
+}  // namespace rsgpu
+
+using rsgpu::i64;
+
+struct rs_solver {
+    int kind = RS_SOLVER_NATIVEKLU;
+    i64 n = 0;
+    double pivot_tol = 0.001;
+    int ordering = 0;
+    bool use_btf = true;
+    i64 btf_min_blocks = 2;
+    int state = rsgpu::ST_EMPTY;
+    std::vector<i64> last_ap, last_ai;
+    rsgpu::HostFactor hf;
+    rsgpu::PlanHost ph;
+    rsgpu::PlanDev pd{};
+    bool plan_on_device = false;
+    i64 factor_count = 0, refactor_count = 0;
+    double rcond = 0.0;
+    bool last_parallel = false;
+    i64 threads_hint = 0;
+    std::string err;
+    size_t plan_bytes = 0;
+    bool lds_refactor = false, lds_solve = false;
+    rsgpu::DevBuf<rsgpu::PlanDev> d_desc;
+    rsgpu::DevBuf<int32_t> d_ip;
+    rsgpu::DevBuf<double> d_vals, d_ax, d_rhs, d_x, d_work;
+    std::vector<double> h_ud;
+    bool rcond_stale = false;
+};
+

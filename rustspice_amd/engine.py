@@ -1,0 +1,172 @@
+"""Circuit-level front end: device stamping and the DC/TRAN Newton drivers.
+
+Mirrors ``Engine`` (crates/sim-core/src/engine.rs) for the hot path:
+``dc_op`` is ``run_dc_result`` (engine.rs:249-311), ``tran`` is
+``run_tran_result_with_params`` (engine.rs:333-632), ``stamp`` is one call of
+the engine's MNA build closure (engine.rs:262-283). The solver argument is a
+:class:`rustspice_amd.solver.GpuSolver` (the LinearSolver drop-in).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .solver import GpuSolver, SolverError
+
+MODE_DC, MODE_TRAN_BE, MODE_TRAN_TRAP = 0, 1, 2
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [
+        ("max_iters", C.c_int32), ("gmin_steps", C.c_int32), ("source_steps", C.c_int32),
+        ("pad", C.c_int32), ("abs_tol", C.c_double), ("rel_tol", C.c_double),
+        ("gmin", C.c_double), ("damping", C.c_double), ("damping_min", C.c_double),
+    ]
+
+
+def default_opts(**kw) -> NewtonOpts:
+    """newton_config_from_options (engine.rs:987-999) over the option defaults
+    of options.rs:79-145: itl1=100, vntol=1e-6, reltol=1e-3, gmin=1e-12."""
+    o = NewtonOpts(100, 0, 0, 0, 1e-6, 1e-3, 1e-12, 1.0, 0.1)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class NewtonResult(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("reason", C.c_int32),
+                ("pad", C.c_int32), ("final_norm", C.c_double)]
+
+
+class TranResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("accepted", C.c_int32), ("rejected", C.c_int32),
+                ("newton_iters_total", C.c_int32), ("factor_calls", C.c_int32),
+                ("dc_iters", C.c_int32), ("npoints", C.c_int64)]
+
+
+def _bind():
+    L = _lib.lib()
+    if getattr(L, "_engine_bound", False):
+        return L
+    vp = C.c_void_p
+    L.rs_circuit_create.restype = vp
+    L.rs_circuit_create.argtypes = [vp, C.c_int64, C.c_int64]
+    L.rs_circuit_destroy.argtypes = [vp]
+    L.rs_circuit_last_error.restype = C.c_int32
+    L.rs_circuit_last_error.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.rs_circuit_pattern.restype = C.c_int32
+    L.rs_circuit_pattern.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     vp, vp]
+    L.rs_circuit_set_cap_state.restype = C.c_int32
+    L.rs_circuit_set_cap_state.argtypes = [vp] * 5
+    L.rs_stamp.restype = C.c_int32
+    L.rs_stamp.argtypes = [vp, C.c_int32, vp, C.c_int64, C.c_double, C.c_double, C.c_double,
+                           C.c_double, vp, vp]
+    L.rs_dc_op.restype = C.c_int32
+    L.rs_dc_op.argtypes = [vp, vp, C.POINTER(NewtonOpts), vp, C.c_int64, C.POINTER(C.c_int64),
+                           C.POINTER(NewtonResult)]
+    L.rs_tran.restype = C.c_int32
+    L.rs_tran.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_int64, vp, vp, C.c_int64, C.POINTER(TranResult)]
+    L.rs_tran_fixed.restype = C.c_int32
+    L.rs_tran_fixed.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, vp,
+                                C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32)]
+    L._engine_bound = True
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class GpuCircuit:
+    """A device-resident instance list (include/rs_circuit.h records)."""
+
+    def __init__(self, circuit):
+        self._L = _bind()
+        self.circuit = circuit
+        self._dev = np.ascontiguousarray(circuit.devices)
+        self.h = self._L.rs_circuit_create(_ptr(self._dev), circuit.ndev, circuit.node_count)
+        self._check(0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.rs_circuit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        buf = C.create_string_buffer(512)
+        self._L.rs_circuit_last_error(self.h, buf, 512)
+        if rc != 0 or (buf.value and rc == 0 and b"hipMalloc" in buf.value):
+            raise SolverError(rc or 7, buf.value.decode(errors="replace"))
+
+    def pattern(self, mode=MODE_DC):
+        n, nnz = C.c_int64(), C.c_int64()
+        self._check(self._L.rs_circuit_pattern(self.h, mode, C.byref(n), C.byref(nnz), None, None))
+        ap = np.zeros(n.value + 1, np.int64)
+        ai = np.zeros(max(nnz.value, 1), np.int64)
+        self._check(self._L.rs_circuit_pattern(self.h, mode, None, None, _ptr(ap), _ptr(ai)))
+        return ap, ai[: nnz.value]
+
+    def set_cap_state(self, cap_v, cap_i, has_v, has_i):
+        arrs = [np.ascontiguousarray(cap_v, np.float64), np.ascontiguousarray(cap_i, np.float64),
+                np.ascontiguousarray(has_v, np.uint8), np.ascontiguousarray(has_i, np.uint8)]
+        self._check(self._L.rs_circuit_set_cap_state(self.h, *[_ptr(a) for a in arrs]))
+
+    def stamp(self, x=None, mode=MODE_DC, t=0.0, dt=0.0, gmin=1e-12, source_scale=1.0):
+        """One MNA build: returns (ap, ai, ax, rhs, n) like the engine closure."""
+        ap, ai = self.pattern(mode)
+        n = len(ap) - 1
+        x = np.ascontiguousarray(np.zeros(self.circuit.node_count) if x is None else x, np.float64)
+        ax = np.zeros(max(len(ai), 1))
+        rhs = np.zeros(max(n, 1))
+        self._check(self._L.rs_stamp(self.h, mode, _ptr(x), len(x), t, dt, gmin, source_scale,
+                                     _ptr(ax), _ptr(rhs)))
+        return ap, ai, ax[: len(ai)], rhs[:n], n
+
+    def dc_op(self, solver: GpuSolver, opts=None):
+        """Returns (x or None, NewtonResult)."""
+        opts = opts or default_opts()
+        cap = self.circuit.node_count + self.circuit.ndev + 8
+        x = np.zeros(cap)
+        xl = C.c_int64()
+        r = NewtonResult()
+        self._check(self._L.rs_dc_op(self.h, solver.h, C.byref(opts), _ptr(x), cap, C.byref(xl),
+                                     C.byref(r)))
+        return (x[: xl.value].copy() if r.converged else None), r
+
+    def tran(self, solver: GpuSolver, tstep, tstop, tstart=0.0, tmax=None, opts=None,
+             max_points=100000, store=True):
+        opts = opts or default_opts()
+        tmax = tstop if tmax is None else tmax
+        stride = self.circuit.node_count + self.circuit.ndev + 8
+        times = np.zeros(max_points) if store else None
+        sols = np.zeros(max_points * stride) if store else None
+        r = TranResult()
+        self._check(self._L.rs_tran(self.h, solver.h, C.byref(opts), tstep, tstop, tstart, tmax,
+                                    max_points, _ptr(times) if store else None,
+                                    _ptr(sols) if store else None, stride, C.byref(r)))
+        out = {k: getattr(r, k) for k, _ in TranResult._fields_}
+        if store:
+            k = min(int(r.npoints), max_points)
+            out["times"] = times[:k].copy()
+            out["sols"] = sols[: k * stride].reshape(k, stride).copy()
+        return out
+
+    def tran_fixed(self, solver: GpuSolver, dt, nsteps, opts=None):
+        opts = opts or default_opts()
+        cap = self.circuit.node_count + self.circuit.ndev + 8
+        x = np.zeros(cap)
+        xl, it, st = C.c_int64(), C.c_int32(), C.c_int32()
+        self._check(self._L.rs_tran_fixed(self.h, solver.h, C.byref(opts), dt, nsteps, _ptr(x), cap,
+                                          C.byref(xl), C.byref(it), C.byref(st)))
+        return st.value, x[: xl.value].copy(), it.value

@@ -186,3 +186,19 @@ class HostFactor:
         keys = ["factor_levels", "max_level_width", "lsolve_levels", "usolve_levels",
                 "scatter_runs", "offdiag", "nnz_l", "nnz_u"]
         return dict(zip(keys, map(int, out)))
+
+
+def host_stamp_pattern(circuit, mode=0):
+    """Unmerged CSC pattern the stamp loop produces (host replay, no device)."""
+    L = _lib.lib()
+    dev = np.ascontiguousarray(circuit.devices)
+    n, nnz = C.c_int64(), C.c_int64()
+    rc = L.rs_host_stamp_pattern(dev.ctypes.data_as(C.c_void_p), circuit.ndev, circuit.node_count,
+                                 mode, C.byref(n), C.byref(nnz), None, None)
+    if rc:
+        raise SolverError(rc, "unsupported device")
+    ap = np.zeros(n.value + 1, np.int64)
+    ai = np.zeros(max(nnz.value, 1), np.int64)
+    L.rs_host_stamp_pattern(dev.ctypes.data_as(C.c_void_p), circuit.ndev, circuit.node_count, mode,
+                            None, None, ap.ctypes.data_as(C.c_void_p), ai.ctypes.data_as(C.c_void_p))
+    return ap, ai[: nnz.value]
