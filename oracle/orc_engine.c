@@ -253,6 +253,7 @@ int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_ne
     res->dc_iters = dcr.iterations;
     if (!dcr.converged) {
         res->status = 2; res->npoints = 0;
+        res->factor_calls = fcalls; res->newton_iters_total = iters_total;
         free(x); tstate_free(&sbe); tstate_free(&str); free(bp.t);
         return 1;
     }

@@ -30,6 +30,7 @@ EXPORTS = [
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern",
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
     "rs_circuit_set_cap_state", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
+    "rs_profile_enable", "rs_profile_get",
 ]
 
 
@@ -97,5 +98,22 @@ def lib():
     L.rs_host_stamp_pattern.restype = C.c_int32
     L.rs_host_stamp_pattern.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32,
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64), vp, vp]
+    L.rs_profile_enable.argtypes = [C.c_int32]
+    L.rs_profile_get.argtypes = [f64p, i64p]
     _LIB = L
     return L
+
+
+PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble")
+
+
+def profile_enable(on: bool = True):
+    lib().rs_profile_enable(int(on))
+
+
+def profile_get() -> dict:
+    ms = np.zeros(4)
+    cnt = np.zeros(4, np.int64)
+    lib().rs_profile_get(ms, cnt)
+    return {k: dict(ms_total=float(m), count=int(c), avg_us=(1e3 * m / c if c else 0.0))
+            for k, m, c in zip(PROFILE_KERNELS, ms, cnt)}

@@ -20,6 +20,7 @@
 #include "../../include/rsgpu.h"
 #include "solver_impl.hpp"
 #include "stamp.hpp"
+#include "profile.hpp"
 
 namespace rsgpu {
 hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, StampArgs a,
@@ -108,11 +109,15 @@ int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t
     int32_t rc = get_pattern(c, mode, dp);
     if (rc) return rc;
     StampArgs a{t, dt, gmin, scale, mode, 0};
+    int ps = prof_begin(PK_EVAL, s);
     hipError_t e = launch_eval(c->d_dev.p, (int)c->dev.size(), x_dev, (int)xl, a, c->d_cap_v.p,
                                c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s);
+    prof_end(ps, s);
     if (e) return chip(c, e, "k_eval");
+    ps = prof_begin(PK_ASSEMBLE, s);
     e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p,
                         (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, s);
+    prof_end(ps, s);
     if (e) return chip(c, e, "k_assemble");
     c->stamp_calls++;
     if (pout) *pout = dp;

@@ -16,6 +16,7 @@
 #include "../../include/rsgpu.h"
 #include "plan.hpp"
 #include "solver_impl.hpp"
+#include "profile.hpp"
 
 
 using namespace rsgpu;
@@ -231,8 +232,10 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
                            hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
     }
+    int ps = prof_begin(PK_REFACTOR, s);
     e = launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
-                                   (int)h->n, h->lds_refactor, kThreads, s);
+                        (int)h->n, h->lds_refactor, kThreads, s);
+    prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_refactor launch");
     if (sync_rcond) {
         h->h_ud.resize(h->n);
@@ -295,8 +298,10 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
 int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
     if (h->state != ST_FACTORED || !h->plan_on_device)
         return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    int ps = prof_begin(PK_SOLVE, s);
     hipError_t e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev,
                                 h->d_work.p, (int)h->n, h->lds_solve, kThreads, s);
+    prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
     return RS_OK;
 }
@@ -460,12 +465,7 @@ int32_t rs_refactor_device(rs_solver* h, const double* ax_dev, void* stream) {
 }
 
 int32_t rs_solve_device(rs_solver* h, const double* rhs_dev, double* x_dev, void* stream) {
-    if (h->state != ST_FACTORED || !h->plan_on_device)
-        return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
-    hipError_t e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev,
-                                h->d_work.p, (int)h->n, h->lds_solve, kThreads, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
-    return RS_OK;
+    return solver_solve_dev(h, rhs_dev, x_dev, (hipStream_t)stream);
 }
 
 int32_t rs_get_factor_values(rs_solver* h, double* lx, double* ux, double* udiag) {
