@@ -62,6 +62,55 @@ def cpu_baseline(circ, tran_args, budget_s, gpu_res):
         dict(status_match=parity, oracle_dc_final_norm=ro.final_norm, oracle_dc_iters=ro.iterations)
 
 
+def batched_dc_ops(args, E, NL, S, torch):
+    """4,096 Monte-Carlo inverter chains (stages ~ U[50,400], VTO += N(0,20mV),
+    u0 *= N(1,0.05)), every DC OP in one persistent launch; compared with
+    looping the plain GPU backend (rs_dc_op per circuit) on a sample."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    circs = NL.monte_carlo_chains(args.batch)
+    b = E.GpuBatch(circs)
+    setup_s = time.perf_counter() - t0
+    info = b.info()
+    b.dc_op()  # warm-up
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.batch_steps):
+        b.dc_op()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = ev0.elapsed_time(ev1) / args.batch_steps
+    out = b.results()
+    iters = np.array([r.iterations for _, r in out])
+    conv = int(sum(r.converged for _, r in out))
+    dcops = args.batch * args.batch_steps / wall
+    # looping the plain single-circuit GPU backend on a sample of instances
+    k = min(args.loop_sample, len(circs))
+    idx = np.linspace(0, len(circs) - 1, k).astype(int)
+    gcs = [(E.GpuCircuit(circs[i]), S.GpuSolver(circs[i].node_count)) for i in idx]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    same = True
+    for (g, s), i in zip(gcs, idx):
+        _, r = g.dc_op(s)
+        same &= (r.iterations == out[i][1].iterations and r.final_norm == out[i][1].final_norm)
+    torch.cuda.synchronize()
+    loop_per = (time.perf_counter() - t0) / k
+    return dict(
+        workload=f"{args.batch} Monte-Carlo level-1 inverter chains, stages U[50,400], DC OP each "
+                 "(reference semantics: NativeKLU, no gmin/source stepping)",
+        dc_ops_per_s=dcops, refactor_solve_per_s=float(iters.sum()) * args.batch_steps / wall,
+        ms_per_batch=1e3 * wall / args.batch_steps, kernel_ms_per_batch=kern_ms,
+        instances=args.batch, patterns=info["patterns"], grid=info["grid"], nmax=info["nmax"],
+        converged=conv, newton_iters_total=int(iters.sum()), setup_s=setup_s,
+        loop_plain_backend_dc_ops_per_s=1.0 / loop_per, loop_sample=k,
+        speedup_vs_loop_plain=dcops * loop_per, loop_results_identical=bool(same))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +118,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=4096, help="Monte-Carlo instances (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--loop-sample", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -138,6 +190,8 @@ def main():
         tran_status=res[-1]["status"], dc_iters_per_step=res[-1]["dc_iters"],
         refactor_solve_calls=calls, kernels=kern, roofline=roof,
     )
+    if args.batch > 0:
+        line["batched"] = batched_dc_ops(args, E, NL, S, torch)
     if not args.no_cpu:
         cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])
         xg, rg = g.dc_op(s)

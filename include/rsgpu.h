@@ -149,6 +149,25 @@ int32_t rs_tran_fixed(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, doub
                       int64_t nsteps, double *x_final, int64_t x_cap, int64_t *x_len,
                       int32_t *newton_iters, int32_t *status);
 
+/* ---- batched DC operating points (no reference counterpart: the reference
+ * runs one Engine per circuit). Instance k owns devices
+ * dev[dev_ptr[k] .. dev_ptr[k+1]) over node_count[k] nodes. One launch runs
+ * Engine::run_dc_result (engine.rs:249-311) for every instance with its own
+ * fresh solver state: the same Newton iterations, pivots and results as one
+ * rs_dc_op per instance. kind is an rs_solver_kind. ----------------------- */
+typedef struct rs_batch rs_batch;
+rs_batch *rs_batch_create(const rs_device *dev, const int64_t *dev_ptr, const int64_t *node_count,
+                          int64_t ninst, int32_t kind);
+void rs_batch_destroy(rs_batch *b);
+int32_t rs_batch_last_error(const rs_batch *b, char *buf, int64_t len);
+/* out[4] = {instances, distinct patterns, largest n, persistent grid size} */
+int32_t rs_batch_info(const rs_batch *b, int64_t *out);
+/* Enqueues the batched Newton solve on `stream` (hipStream_t, NULL = default). */
+int32_t rs_batch_dc_op(rs_batch *b, const rs_newton_opts *o, void *stream);
+/* Copies results: res[k] (pad = number of pivoting factors), x of converged
+ * instances at x_out + x_ptr[k] (x_ptr has ninst + 1 entries). */
+int32_t rs_batch_results(rs_batch *b, rs_newton_result *res, double *x_out, int64_t *x_ptr);
+
 /* ---- measurement (no reference counterpart) -------------------------------
  * Kernel timing with HIP events recorded on each kernel's launch stream.
  * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly. Enabling

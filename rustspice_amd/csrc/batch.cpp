@@ -1,0 +1,313 @@
+// batch.cpp - rs_batch: host preparation and launch of the batched DC kernel
+// (see batch.hpp for the execution model).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rsgpu.h"
+#include "batch.hpp"
+#include "plan.hpp"
+#include "solver_impl.hpp"
+#include "stamp.hpp"
+
+namespace rsgpu {
+hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
+                             int* counter, const int32_t* ip, const rs_device* devs, double* scr,
+                             BResult* res, const BOpts& o, int nmax, int grid, hipStream_t s);
+int batch_blocks_per_cu(int nmax);
+}  // namespace rsgpu
+
+using namespace rsgpu;
+
+struct rs_batch {
+    int kind = RS_SOLVER_NATIVEKLU;
+    int64_t ninst = 0;
+    int nmax = 0, npat = 0;
+    std::vector<BInst> inst;
+    std::vector<int64_t> x_off;   // output x offset per instance
+    std::vector<int32_t> n_of;    // matrix size per instance
+    DevBuf<BPat> d_pats;
+    DevBuf<BInst> d_inst;
+    DevBuf<int32_t> d_order, d_ip;
+    DevBuf<int> d_counter;
+    DevBuf<rs_device> d_dev;
+    DevBuf<double> d_scr;
+    DevBuf<BResult> d_res;
+    std::string err;
+    int grid = 0;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+int32_t bfail(rs_batch* b, int32_t code, const std::string& m) {
+    b->err = m;
+    return code;
+}
+int32_t bhip(rs_batch* b, hipError_t e, const char* what) {
+    b->err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+    return RS_ERR_DEVICE;
+}
+
+// Pattern-level analysis: symbolic factor with identity pivots so that the
+// plan's scatter targets, solve permutation and off-diagonal row keys are the
+// pre-pivot positions the kernel maps through its own RPI.
+void analyze_pattern(const StampPattern& sp, int kind, PlanHost& ph) {
+    HostFactor f;
+    const i64 n = sp.n;
+    f.n = n;
+    const i64* ap = sp.ap.data();
+    const i64* ai = sp.ai.data();
+    if (kind == RS_SOLVER_NATIVEKLU && n >= 4) {
+        f.btf = btf_decompose(n, ap, ai);
+        f.has_btf = f.btf.num_blocks >= 2 && f.btf.max_block_size < n;
+    }
+    if (f.has_btf) {
+        f.sym.resize(f.btf.num_blocks);
+        for (i64 b = 0; b < f.btf.num_blocks; b++) {
+            i64 bs = f.btf.block_ptr[b], be = f.btf.block_ptr[b + 1], sz = be - bs;
+            std::vector<i64> bap(sz + 1, 0), bai;
+            for (i64 lc = 0; lc < sz; lc++) {
+                i64 gco = f.btf.col_perm[bs + lc];
+                for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
+                    i64 gro = ai[q];
+                    if (gro >= n) continue;
+                    i64 grn = f.btf.row_perm_inv[gro];
+                    if (grn >= bs && grn < be) bai.push_back(grn - bs);
+                }
+                bap[lc + 1] = (i64)bai.size();
+            }
+            block_symbolic(sz, bap.data(), bai.data(), false, f.sym[b]);
+        }
+    } else {
+        f.sym.resize(1);
+        block_symbolic(n, ap, ai, false, f.sym[0]);
+    }
+    f.num.resize(f.sym.size());
+    for (size_t b = 0; b < f.sym.size(); b++) {
+        i64 m = f.sym[b].n;
+        f.num[b].row_perm.resize(m);
+        f.num[b].row_perm_inv.resize(m);
+        for (i64 i = 0; i < m; i++) f.num[b].row_perm[i] = f.num[b].row_perm_inv[i] = i;
+    }
+    build_plan(f, ap, ai, ph);
+}
+
+int32_t put(std::vector<int32_t>& pool, const std::vector<int32_t>& v) {
+    int32_t off = (int32_t)pool.size();
+    pool.insert(pool.end(), v.begin(), v.end());
+    pool.push_back(0);
+    return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const int64_t* node_count,
+                          int64_t ninst, int32_t kind) {
+    rs_batch* b = new rs_batch();
+    b->kind = kind;
+    b->ninst = ninst;
+    std::vector<int32_t> pool;
+    std::vector<BPat> pats;
+    std::map<std::vector<int64_t>, int> pat_id;  // key: node_count, n, ap, ai, maps
+    b->inst.resize(ninst);
+    b->x_off.resize(ninst + 1);
+    b->n_of.resize(ninst);
+    int64_t scr = 0;
+    try {
+        for (int64_t k = 0; k < ninst; k++) {
+            const rs_device* d = dev + dev_ptr[k];
+            const int64_t nd = dev_ptr[k + 1] - dev_ptr[k];
+            StampPattern sp;
+            build_stamp_pattern(d, nd, node_count[k], MODE_DC, sp);
+            std::vector<int64_t> key;
+            key.reserve(4 + sp.ap.size() + 2 * sp.ai.size() + sp.rhs_ptr.size() + sp.rhs_src.size());
+            key.push_back(node_count[k]);
+            key.push_back(sp.n);
+            key.push_back(nd);
+            key.insert(key.end(), sp.ap.begin(), sp.ap.end());
+            key.insert(key.end(), sp.ai.begin(), sp.ai.end());
+            key.insert(key.end(), sp.ax_src.begin(), sp.ax_src.end());
+            key.insert(key.end(), sp.rhs_ptr.begin(), sp.rhs_ptr.end());
+            key.insert(key.end(), sp.rhs_src.begin(), sp.rhs_src.end());
+            auto it = pat_id.find(key);
+            int pid;
+            if (it == pat_id.end()) {
+                PlanHost ph;
+                analyze_pattern(sp, kind, ph);
+                BPat P{};
+                P.n = (int32_t)sp.n;
+                P.node_count = (int32_t)node_count[k];
+                P.nnz = (int32_t)sp.ai.size();
+                P.nblk = ph.nblk;
+                P.nl = ph.nl;
+                P.nu = ph.nu;
+                std::vector<int32_t> axs(sp.ax_src.begin(), sp.ax_src.end());
+                P.o_ax_src = put(pool, axs);
+                P.o_rhs_ptr = put(pool, sp.rhs_ptr);
+                P.o_rhs_src = put(pool, sp.rhs_src);
+                P.o_sc_ptr = put(pool, ph.sc_ptr);
+                P.o_sc_src = put(pool, ph.sc_src);
+                P.o_sc_cnt = put(pool, ph.sc_cnt);
+                P.o_sc_pre = put(pool, ph.sc_dst);
+                P.o_lcp = put(pool, ph.lcp);
+                P.o_lri = put(pool, ph.lri);
+                P.o_ucp = put(pool, ph.ucp);
+                P.o_uri = put(pool, ph.uri);
+                P.o_colflag = put(pool, ph.colflag);
+                P.o_pin = put(pool, ph.perm_in);
+                P.o_pout = put(pool, ph.perm_out);
+                P.o_lrp = put(pool, ph.lrp);
+                P.o_lrc = put(pool, ph.lrc);
+                P.o_lrv = put(pool, ph.lrv);
+                P.o_urp = put(pool, ph.urp);
+                P.o_urc = put(pool, ph.urc);
+                P.o_urv = put(pool, ph.urv);
+                P.o_blk_ptr = put(pool, ph.blk_ptr);
+                P.o_lsl_blk = put(pool, ph.lsl_blk);
+                P.o_lsl_ptr = put(pool, ph.lsl_ptr);
+                P.o_lsl_rows = put(pool, ph.lsl_rows);
+                P.o_usl_blk = put(pool, ph.usl_blk);
+                P.o_usl_ptr = put(pool, ph.usl_ptr);
+                P.o_usl_rows = put(pool, ph.usl_rows);
+                P.o_odp = put(pool, ph.odp);
+                P.o_odax = put(pool, ph.odax);
+                P.o_odsrc = put(pool, ph.odsrc);
+                P.o_blk_has_od = put(pool, ph.blk_has_od);
+                if (pool.size() > (size_t)INT32_MAX) throw std::runtime_error("pattern pool too large");
+                pid = (int)pats.size();
+                pats.push_back(P);
+                pat_id.emplace(std::move(key), pid);
+            } else {
+                pid = it->second;
+            }
+            const BPat& P = pats[pid];
+            BInst& I = b->inst[k];
+            I.pat = pid;
+            I.ndev = (int32_t)nd;
+            I.o_dev = dev_ptr[k];
+            I.o_sv = scr;
+            scr += nd * kSlotsPerDevice;
+            I.o_ax = scr;
+            scr += P.nnz;
+            I.o_lx = scr;
+            scr += P.nl;
+            I.o_ux = scr;
+            scr += P.nu;
+            I.o_ud = scr;
+            scr += P.n;
+            I.o_x = scr;
+            scr += P.n;
+            b->n_of[k] = P.n;
+            b->nmax = std::max(b->nmax, P.n);
+        }
+    } catch (const std::exception& ex) {
+        b->err = std::string("Invalid matrix: ") + ex.what();
+        return b;
+    }
+    b->npat = (int)pats.size();
+    // queue order: largest instances first (longest-processing-time first)
+    std::vector<int32_t> order(ninst);
+    for (int64_t k = 0; k < ninst; k++) order[k] = (int32_t)k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return b->n_of[x] > b->n_of[y]; });
+    hipError_t e;
+    int64_t ndev_total = dev_ptr[ninst];
+    if ((e = b->d_pats.ensure(pats.size())) || (e = b->d_inst.ensure(ninst)) ||
+        (e = b->d_order.ensure(ninst)) || (e = b->d_ip.ensure(pool.size())) ||
+        (e = b->d_counter.ensure(1)) || (e = b->d_dev.ensure(ndev_total)) ||
+        (e = b->d_scr.ensure(scr)) || (e = b->d_res.ensure(ninst))) {
+        bhip(b, e, "hipMalloc(batch)");
+        return b;
+    }
+    if ((e = hipMemcpy(b->d_pats.p, pats.data(), pats.size() * sizeof(BPat), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(b->d_inst.p, b->inst.data(), ninst * sizeof(BInst), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(b->d_order.p, order.data(), ninst * sizeof(int32_t), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(b->d_ip.p, pool.data(), pool.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(b->d_dev.p, dev, ndev_total * sizeof(rs_device), hipMemcpyHostToDevice)) ||
+        (e = hipMemset(b->d_scr.p, 0, scr * sizeof(double)))) {
+        bhip(b, e, "hipMemcpy(batch)");
+        return b;
+    }
+    int dev_id = 0, cus = 0;
+    (void)hipGetDevice(&dev_id);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+    int per_cu = batch_blocks_per_cu(b->nmax);
+    b->grid = (int)std::min<int64_t>(ninst, (int64_t)std::max(cus, 1) * per_cu);
+    if (b->grid < 1) b->grid = 1;
+    return b;
+}
+
+void rs_batch_destroy(rs_batch* b) {
+    if (!b) return;
+    b->d_pats.release();
+    b->d_inst.release();
+    b->d_order.release();
+    b->d_ip.release();
+    b->d_counter.release();
+    b->d_dev.release();
+    b->d_scr.release();
+    b->d_res.release();
+    delete b;
+}
+
+int32_t rs_batch_last_error(const rs_batch* b, char* buf, int64_t len) {
+    if (len > 0) std::snprintf(buf, (size_t)len, "%s", b->err.c_str());
+    return (int32_t)b->err.size();
+}
+
+int32_t rs_batch_info(const rs_batch* b, int64_t* out) {
+    out[0] = b->ninst;
+    out[1] = b->npat;
+    out[2] = b->nmax;
+    out[3] = b->grid;
+    return b->err.empty() ? RS_OK : RS_ERR_DEVICE;
+}
+
+int32_t rs_batch_dc_op(rs_batch* b, const rs_newton_opts* o, void* stream) {
+    if (!b->err.empty()) return RS_ERR_DEVICE;
+    BOpts bo{o->max_iters, o->gmin_steps, o->source_steps, 0,
+             o->abs_tol, o->rel_tol, o->gmin, o->damping, o->damping_min,
+             b->kind == RS_SOLVER_SPARSELU ? 0.0 : 0.001};
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemsetAsync(b->d_counter.p, 0, sizeof(int), s);
+    if (e) return bhip(b, e, "hipMemset(counter)");
+    e = launch_batch_dcop(b->d_pats.p, b->d_inst.p, b->d_order.p, (int)b->ninst, b->d_counter.p,
+                          b->d_ip.p, b->d_dev.p, b->d_scr.p, b->d_res.p, bo, b->nmax, b->grid, s);
+    if (e) return bhip(b, e, "k_batch_dcop");
+    return RS_OK;
+}
+
+int32_t rs_batch_results(rs_batch* b, rs_newton_result* res, double* x_out, int64_t* x_ptr) {
+    std::vector<BResult> r(b->ninst);
+    hipError_t e = hipMemcpy(r.data(), b->d_res.p, b->ninst * sizeof(BResult), hipMemcpyDeviceToHost);
+    if (e) return bhip(b, e, "hipMemcpy(results)");
+    int64_t off = 0;
+    for (int64_t k = 0; k < b->ninst; k++) {
+        if (res) {
+            res[k].converged = r[k].converged;
+            res[k].iterations = r[k].iterations;
+            res[k].reason = r[k].reason;
+            res[k].pad = r[k].full_factors;
+            res[k].final_norm = r[k].final_norm;
+        }
+        if (x_ptr) x_ptr[k] = off;
+        if (x_out && r[k].converged) {
+            e = hipMemcpy(x_out + off, b->d_scr.p + b->inst[k].o_x, b->n_of[k] * sizeof(double),
+                          hipMemcpyDeviceToHost);
+            if (e) return bhip(b, e, "hipMemcpy(x)");
+        }
+        off += b->n_of[k];
+    }
+    if (x_ptr) x_ptr[b->ninst] = off;
+    return RS_OK;
+}
+
+}  // extern "C"

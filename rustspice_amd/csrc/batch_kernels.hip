@@ -1,0 +1,322 @@
+// batch_kernels.hip - persistent batched DC operating point kernel.
+//
+// One wavefront (workgroup of 64) owns one instance at a time and runs its
+// whole Newton loop (newton.rs:65-185 inside engine.rs:249-311): device
+// evaluation and assembly (stamp.rs, mna.rs), the first numeric factor with
+// threshold pivoting inside the static pattern and later refactors with the
+// reused pivots (native_klu.rs:1194-1382), the BTF/L/U solve
+// (native_klu.rs:1442-1551), the convergence test, damping and x update.
+// The work vector, the right-hand side, x, x_new and the pivot arrays live in
+// LDS; stamp values and L/U values in the instance's global scratch. Every
+// floating-point operation is the reference's, in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "batch.hpp"
+#include "device_eval.hpp"
+
+namespace {
+
+using namespace rsgpu;
+
+constexpr int kW = 64;
+
+__device__ __forceinline__ void bsync() { __syncthreads(); }
+
+__global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats,
+                                                   const BInst* __restrict__ insts,
+                                                   const int32_t* __restrict__ order, int ninst,
+                                                   int* __restrict__ counter,
+                                                   const int32_t* __restrict__ ip,
+                                                   const rs_device* __restrict__ devs,
+                                                   double* __restrict__ scr,
+                                                   BResult* __restrict__ res, BOpts o, int nmax) {
+    extern __shared__ double lds[];
+    double* W = lds;
+    double* T = W + nmax;
+    double* X = T + nmax;
+    double* XN = X + nmax;
+    int* RPI = reinterpret_cast<int*>(XN + nmax);
+    int* RP = RPI + nmax;
+    __shared__ int s_qi;
+    __shared__ double s_dxn, s_fn;
+    const int lane = threadIdx.x;
+
+    for (;;) {
+        if (lane == 0) s_qi = atomicAdd(counter, 1);
+        bsync();
+        const int qi = s_qi;
+        bsync();
+        if (qi >= ninst) break;
+        const int ii = order[qi];
+        const BInst I = insts[ii];
+        const BPat P = pats[I.pat];
+        const int n = P.n, nc = P.node_count;
+        const rs_device* dev = devs + I.o_dev;
+        double* sv = scr + I.o_sv;
+        double* ax = scr + I.o_ax;
+        double* lx = scr + I.o_lx;
+        double* ux = scr + I.o_ux;
+        double* ud = scr + I.o_ud;
+        const int32_t* ax_src = ip + P.o_ax_src;
+        const int32_t* rhs_ptr = ip + P.o_rhs_ptr;
+        const int32_t* rhs_src = ip + P.o_rhs_src;
+        const int32_t* sc_ptr = ip + P.o_sc_ptr;
+        const int32_t* sc_src = ip + P.o_sc_src;
+        const int32_t* sc_cnt = ip + P.o_sc_cnt;
+        const int32_t* sc_pre = ip + P.o_sc_pre;
+        const int32_t* lcp = ip + P.o_lcp;
+        const int32_t* lri = ip + P.o_lri;
+        const int32_t* ucp = ip + P.o_ucp;
+        const int32_t* uri = ip + P.o_uri;
+        const int32_t* colflag = ip + P.o_colflag;
+        const int32_t* pin = ip + P.o_pin;
+        const int32_t* pout = ip + P.o_pout;
+        const int32_t* lrp = ip + P.o_lrp;
+        const int32_t* lrc = ip + P.o_lrc;
+        const int32_t* lrv = ip + P.o_lrv;
+        const int32_t* urp = ip + P.o_urp;
+        const int32_t* urc = ip + P.o_urc;
+        const int32_t* urv = ip + P.o_urv;
+        const int32_t* blk_ptr = ip + P.o_blk_ptr;
+        const int32_t* lsl_blk = ip + P.o_lsl_blk;
+        const int32_t* lsl_ptr = ip + P.o_lsl_ptr;
+        const int32_t* lsl_rows = ip + P.o_lsl_rows;
+        const int32_t* usl_blk = ip + P.o_usl_blk;
+        const int32_t* usl_ptr = ip + P.o_usl_ptr;
+        const int32_t* usl_rows = ip + P.o_usl_rows;
+        const int32_t* odp = ip + P.o_odp;
+        const int32_t* odax = ip + P.o_odax;
+        const int32_t* odsrc = ip + P.o_odsrc;
+        const int32_t* has_od = ip + P.o_blk_has_od;
+
+        for (int i = lane; i < n; i += kW) {
+            W[i] = 0.0;
+            X[i] = 0.0;
+        }
+        int xl = nc;  // the Newton x has node_count entries until convergence
+        bool factored = false;
+        BResult fin{0, 0, 1, 0, 0.0};
+        bsync();
+
+        const double gstart = fmax(o.gmin * 1e3, 1e-6);
+        for (int gi = 0; gi <= o.gmin_steps && !fin.converged; ++gi) {
+            const double gmin = o.gmin_steps == 0
+                                    ? o.gmin
+                                    : gstart + (o.gmin - gstart) * ((double)gi / (double)o.gmin_steps);
+            for (int si = 0; si <= o.source_steps && !fin.converged; ++si) {
+                const double scale = o.source_steps == 0 ? 1.0 : (double)si / (double)o.source_steps;
+                // ---- run_newton ----
+                BResult r{0, 0, 1, fin.full_factors, 0.0};
+                double damping = o.damping;
+                double prev = 1.7976931348623157e308;
+                for (int it = 0; it < o.max_iters; ++it) {
+                    r.iterations = it + 1;
+                    StampArgs a{0.0, 0.0, gmin, scale, MODE_DC, 0};
+                    for (int d = lane; d < I.ndev; d += kW) {
+                        const rs_device v = dev[d];
+                        eval_device(v, d, X, xl, a, nullptr, nullptr, nullptr, nullptr,
+                                    sv + (size_t)d * kSlotsPerDevice);
+                    }
+                    bsync();
+                    for (int q = lane; q < P.nnz; q += kW) {
+                        const int s = ax_src[q];
+                        ax[q] = s < 0 ? 1.0 : sv[s];
+                    }
+                    for (int i = lane; i < n; i += kW) {
+                        double acc = 0.0;
+                        for (int k = rhs_ptr[i]; k < rhs_ptr[i + 1]; ++k) acc += sv[rhs_src[k]];
+                        XN[i] = acc;  // rhs until the solve overwrites it
+                    }
+                    if (!factored)
+                        for (int i = lane; i < n; i += kW) {
+                            RPI[i] = i;
+                            RP[i] = i;
+                        }
+                    bsync();
+                    // ---- factor (first: with pivot search; later: reuse) ----
+                    for (int g = 0; g < n; ++g) {
+                        for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW) {
+                            double v = 0.0;
+                            const int src = sc_src[s], cnt = sc_cnt[s];
+                            for (int q = 0; q < cnt; ++q) v += ax[src + q];
+                            W[RPI[sc_pre[s]]] = v;
+                        }
+                        bsync();
+                        const int us = ucp[g], ue = ucp[g + 1] - 1;
+                        for (int u = us; u < ue; ++u) {
+                            const int j = uri[u];
+                            const double ujk = W[j];
+                            if (ujk != 0.0) {
+                                for (int l = lcp[j] + lane; l < lcp[j + 1]; l += kW) {
+                                    const int row = lri[l];
+                                    W[row] = W[row] - lx[l] * ujk;
+                                }
+                                bsync();
+                            }
+                        }
+                        const bool single = (colflag[g] & 1) != 0;
+                        const int ls = lcp[g], le = lcp[g + 1];
+                        if (!factored && !single && le > ls) {
+                            const double dabs = fabs(W[g]);
+                            double bv = -1.0;
+                            int bl = INT_MAX;
+                            for (int l = ls + lane; l < le; l += kW) {
+                                const double v = fabs(W[lri[l]]);
+                                if (v > dabs && v > bv) {
+                                    bv = v;
+                                    bl = l;
+                                }
+                            }
+                            for (int off = kW / 2; off > 0; off >>= 1) {
+                                const double ov = __shfl_xor(bv, off);
+                                const int ol = __shfl_xor(bl, off);
+                                if (ov > bv || (ov == bv && ol < bl)) {
+                                    bv = ov;
+                                    bl = ol;
+                                }
+                            }
+                            if (bl != INT_MAX && dabs < o.pivot_tol * bv) {
+                                if (lane == 0) {
+                                    const int mrow = lri[bl];
+                                    const double t = W[g];
+                                    W[g] = W[mrow];
+                                    W[mrow] = t;
+                                    const int okk = RP[g], om = RP[mrow];
+                                    RP[g] = om;
+                                    RP[mrow] = okk;
+                                    RPI[okk] = mrow;
+                                    RPI[om] = g;
+                                }
+                            }
+                            bsync();
+                        }
+                        for (int u = us + lane; u < ue; u += kW) ux[u] = W[uri[u]];
+                        double diag = W[g];
+                        if (single) {
+                            if (fabs(diag) < 1e-300) diag = 1e-14;
+                        } else if (fabs(diag) < 1e-14) {
+                            diag = diag >= 0.0 ? 1e-14 : -1e-14;
+                        }
+                        if (lane == 0) {
+                            ux[ue] = diag;
+                            ud[g] = diag;
+                        }
+                        for (int l = ls + lane; l < le; l += kW) lx[l] = W[lri[l]] / diag;
+                        bsync();
+                        for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW) W[RPI[sc_pre[s]]] = 0.0;
+                        for (int l = ls + lane; l < le; l += kW) W[lri[l]] = 0.0;
+                        for (int u = us + lane; u < ue; u += kW) W[uri[u]] = 0.0;
+                        if (lane == 0) W[g] = 0.0;
+                        bsync();
+                    }
+                    if (!factored) r.full_factors++;
+                    factored = true;
+                    // ---- solve ----
+                    for (int i = lane; i < n; i += kW) T[RPI[pin[i]]] = XN[i];
+                    bsync();
+                    for (int b = P.nblk - 1; b >= 0; --b) {
+                        if (has_od[b]) {
+                            for (int p = blk_ptr[b] + lane; p < blk_ptr[b + 1]; p += kW) {
+                                const int e0 = odp[p], e1 = odp[p + 1];
+                                if (e0 == e1) continue;
+                                const int pos = RPI[p];
+                                double t = T[pos];
+                                for (int e = e0; e < e1; ++e) t = t - ax[odax[e]] * T[odsrc[e]];
+                                T[pos] = t;
+                            }
+                            bsync();
+                        }
+                        for (int lev = lsl_blk[b]; lev < lsl_blk[b + 1]; ++lev) {
+                            for (int q = lsl_ptr[lev] + lane; q < lsl_ptr[lev + 1]; q += kW) {
+                                const int rr = lsl_rows[q];
+                                const int e0 = lrp[rr], e1 = lrp[rr + 1];
+                                if (e0 == e1) continue;
+                                double t = T[rr];
+                                for (int e = e0; e < e1; ++e) t = t - lx[lrv[e]] * T[lrc[e]];
+                                T[rr] = t;
+                            }
+                            bsync();
+                        }
+                        for (int lev = usl_blk[b]; lev < usl_blk[b + 1]; ++lev) {
+                            for (int q = usl_ptr[lev] + lane; q < usl_ptr[lev + 1]; q += kW) {
+                                const int rr = usl_rows[q];
+                                double t = T[rr];
+                                for (int e = urp[rr]; e < urp[rr + 1]; ++e) t = t - ux[urv[e]] * T[urc[e]];
+                                T[rr] = t / ud[rr];
+                            }
+                            bsync();
+                        }
+                    }
+                    for (int i = lane; i < n; i += kW) XN[i] = T[pout[i]];
+                    bsync();
+                    // ---- convergence, damping, update (newton.rs:93-107) ----
+                    const int nd = n < xl ? n : xl;
+                    int ok = 1;
+                    for (int i = lane; i < nd; i += kW) {
+                        const double dxi = XN[i] - X[i];
+                        if (!(fabs(dxi) <= o.abs_tol + o.rel_tol * fabs(XN[i]))) ok = 0;
+                    }
+                    const int conv = __syncthreads_and(ok);
+                    if (lane == 0) {
+                        double sd = 0.0, sf = 0.0;
+                        for (int i = 0; i < nd; ++i) {
+                            const double dxi = XN[i] - X[i];
+                            sd += dxi * dxi;
+                        }
+                        for (int i = 0; i < n; ++i) sf += XN[i] * XN[i];
+                        s_dxn = sqrt(sd);
+                        s_fn = sqrt(sf);
+                    }
+                    bsync();
+                    const double dxn = s_dxn;
+                    r.final_norm = s_fn;
+                    if (conv) {
+                        for (int i = lane; i < n; i += kW) X[i] = XN[i];
+                        xl = n;
+                        r.converged = 1;
+                        r.reason = 0;
+                        bsync();
+                        break;
+                    }
+                    if (dxn > prev && damping > o.damping_min) {
+                        const double d2 = damping * 0.5;
+                        damping = d2 >= o.damping_min ? d2 : o.damping_min;
+                    }
+                    prev = dxn;
+                    for (int i = lane; i < nd; i += kW) X[i] = X[i] + damping * (XN[i] - X[i]);
+                    bsync();
+                }
+                fin = r;
+            }
+        }
+        if (lane == 0) res[ii] = fin;
+        if (fin.converged)
+            for (int i = lane; i < n; i += kW) scr[I.o_x + i] = X[i];
+        bsync();
+    }
+}
+
+}  // namespace
+
+namespace rsgpu {
+
+hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
+                             int* counter, const int32_t* ip, const rs_device* devs, double* scr,
+                             BResult* res, const BOpts& o, int nmax, int grid, hipStream_t s) {
+    size_t shm = (size_t)nmax * (4 * sizeof(double) + 2 * sizeof(int));
+    hipLaunchKernelGGL(k_batch_dcop, dim3(grid), dim3(kW), shm, s, pats, insts, order, ninst,
+                       counter, ip, devs, scr, res, o, nmax);
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int nmax) {
+    size_t shm = (size_t)nmax * (4 * sizeof(double) + 2 * sizeof(int));
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop, kW, shm) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
+}
+
+}  // namespace rsgpu

@@ -142,6 +142,13 @@ int32_t run_newton(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::
     double damping = o->damping;
     double prev = 1.7976931348623157e308;
     std::vector<double> xn, dx;
+    {
+        DevPattern* dp0;  // allocates the pattern's device vectors before x is uploaded
+        int32_t rc0 = get_pattern(c, mode, dp0);
+        if (rc0) return rc0;
+        hipError_t e0 = c->d_x.ensure(std::max(c->d_x.n, x.size()));
+        if (e0) return chip(c, e0, "hipMalloc(x)");
+    }
     for (int it = 0; it < o->max_iters; it++) {
         r->iterations = it + 1;
         c->newton_iters++;
@@ -317,7 +324,9 @@ int32_t reset_cap_state(rs_circuit* c) {
 // Capacitor history update from a host x (uploaded to d_x).
 int32_t cap_update(rs_circuit* c, const std::vector<double>& x, double dt, int full,
                    double* cv, double* ci, uint8_t* hv, uint8_t* hi) {
-    hipError_t e = hipMemcpy(c->d_x.p, x.data(), x.size() * 8, hipMemcpyHostToDevice);
+    hipError_t e = c->d_x.ensure(std::max(c->d_x.n, x.size()));
+    if (e) return chip(c, e, "hipMalloc(x)");
+    e = hipMemcpy(c->d_x.p, x.data(), x.size() * 8, hipMemcpyHostToDevice);
     if (e) return chip(c, e, "hipMemcpy(x)");
     e = launch_cap_state(c->d_dev.p, (int)c->dev.size(), c->d_x.p, (int)x.size(), dt, full, cv, ci,
                          hv, hi, nullptr);

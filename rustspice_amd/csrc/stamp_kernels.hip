@@ -11,34 +11,14 @@
 // k_cap_state: capacitor history after an accepted step (stamp.rs:572-668).
 #include <hip/hip_runtime.h>
 
+#include "device_eval.hpp"
 #include "stamp.hpp"
 
 namespace {
 
 using rsgpu::kSlotsPerDevice;
 using rsgpu::StampArgs;
-
-__device__ __forceinline__ double xget(const double* x, int xl, int i) {
-    return (i >= 0 && i < xl) ? x[i] : 0.0;
-}
-
-__device__ __forceinline__ double fmax_rust(double a, double b) {
-    // f64::max: returns the non-NaN operand
-    return (a >= b || b != b) ? a : b;
-}
-
-__device__ double pulse_eval(const double* p, double t) {
-    const double v1 = p[0], v2 = p[1], td = p[2], tr = p[3], tf = p[4], pw = p[5], per = p[6];
-    if (t < td) return v1;
-    const double trl = per > 0.0 ? fmod(t - td, per) : t - td;
-    if (trl < tr) return v1 + (v2 - v1) * trl / tr;
-    if (trl < tr + pw) return v2;
-    if (trl < tr + pw + tf) {
-        const double tfall = trl - tr - pw;
-        return v2 - (v2 - v1) * tfall / tf;
-    }
-    return v1;
-}
+using rsgpu::xget;
 
 __global__ __launch_bounds__(256) void k_eval(const rs_device* __restrict__ dev, int ndev,
                                               const double* __restrict__ x, int xl, StampArgs a,
@@ -50,97 +30,7 @@ __global__ __launch_bounds__(256) void k_eval(const rs_device* __restrict__ dev,
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndev) return;
     const rs_device v = dev[d];
-    double* o = sv + (size_t)d * kSlotsPerDevice;
-    const bool dc = a.mode == rsgpu::MODE_DC;
-    switch (v.kind) {
-    case RS_DEV_R: {
-        const double g = 1.0 / v.value;
-        o[0] = g; o[1] = g; o[2] = -g; o[3] = -g;
-        break;
-    }
-    case RS_DEV_I:
-    case RS_DEV_V: {
-        double val;
-        if (dc) val = v.value;  // waveform sources are not stamped in DC
-        else val = (v.flags & RS_F_WAVEFORM) ? pulse_eval(v.p, a.t) : v.value;
-        val = val * a.source_scale;
-        if (v.kind == RS_DEV_I) {
-            o[0] = -val; o[1] = val;
-        } else {
-            o[0] = 1.0; o[1] = -1.0; o[2] = 1.0; o[3] = -1.0; o[4] = val;
-        }
-        break;
-    }
-    case RS_DEV_C: {
-        if (dc) break;
-        const double c = v.value;
-        const double vprev = has_v[d] ? cap_v[d] : 0.0;
-        if (a.mode == rsgpu::MODE_TRAN_BE) {
-            const double g = c / a.dt;
-            const double ieq = g * vprev;
-            o[0] = g; o[1] = g; o[2] = -g; o[3] = -g; o[4] = -ieq; o[5] = ieq;
-        } else {
-            const double g = 2.0 * c / a.dt;
-            const double iprev = has_i[d] ? cap_i[d] : 0.0;
-            const double ieq = g * vprev + iprev;
-            o[0] = g; o[1] = g; o[2] = -g; o[3] = -g; o[4] = ieq; o[5] = -ieq;
-        }
-        break;
-    }
-    case RS_DEV_D: {
-        const double gmin = a.gmin > 0.0 ? a.gmin : 1e-12;
-        const double vt = 0.02585 * v.p[1];
-        const double vd = xget(x, xl, v.node[0]) - xget(x, xl, v.node[1]);
-        const double e = exp(vd / vt);
-        const double id = v.p[0] * (e - 1.0);
-        const double gd = (v.p[0] / vt) * e;
-        const double g = fmax_rust(gd, gmin);
-        const double ieq = id - gd * vd;
-        o[0] = g; o[1] = g; o[2] = -g; o[3] = -g; o[4] = -ieq; o[5] = ieq;
-        break;
-    }
-    case RS_DEV_M: {
-        const double gmin = a.gmin > 0.0 ? a.gmin : 1e-12;
-        const double GMIN = 1e-12;
-        const double EPS_OX = 3.9 * 8.854e-12;
-        const bool pm = (v.flags & RS_F_PMOS) != 0;
-        const double vd = xget(x, xl, v.node[0]), vg = xget(x, xl, v.node[1]);
-        const double vs = xget(x, xl, v.node[2]);
-        const double cox = EPS_OX / v.p[RS_MP_TOX];
-        const double beta = v.p[RS_MP_U0] * 1e-4 * cox;
-        double vdi, vgi, vsi, sign;
-        if (pm) { vdi = -vs; vgi = -vg; vsi = -vd; sign = -1.0; }
-        else { vdi = vd; vgi = vg; vsi = vs; sign = 1.0; }
-        double vgs = vgi - vsi, vds = vdi - vsi;
-        if (vds < 0.0) { vds = -vds; vgs = vgi - vdi; }
-        const double vth0 = v.p[RS_MP_VTH0];
-        const double vth = pm ? -fabs(vth0) : fabs(vth0);
-        const double beta_eff = beta * v.p[RS_MP_W] / v.p[RS_MP_L];
-        const double lambda = 0.02;
-        double ids, gm, gds;
-        if (vgs <= vth) {
-            ids = 0.0; gm = 0.0; gds = GMIN;
-        } else if (vds < vgs - vth) {
-            ids = beta_eff * ((vgs - vth) * vds - 0.5 * vds * vds);
-            gm = beta_eff * vds;
-            gds = fmax_rust(beta_eff * ((vgs - vth) - vds), GMIN);
-        } else {
-            const double vov = vgs - vth;
-            ids = 0.5 * beta_eff * (vov * vov) * (1.0 + lambda * vds);
-            gm = beta_eff * (vgs - vth) * (1.0 + lambda * vds);
-            gds = fmax_rust(0.5 * beta_eff * (vov * vov) * lambda, GMIN);
-        }
-        const double ids_s = ids * sign;
-        const double ieq = ids_s - gm * (vg - vs) - gds * (vd - vs);
-        const double gdsm = fmax_rust(gds, gmin);
-        o[0] = gdsm; o[1] = gdsm; o[2] = -gdsm; o[3] = -gdsm;
-        o[4] = gm; o[5] = -gm; o[6] = -gm; o[7] = gm;
-        o[8] = -ieq; o[9] = ieq;
-        break;
-    }
-    default:
-        break;
-    }
+    rsgpu::eval_device(v, d, x, xl, a, cap_v, cap_i, has_v, has_i, sv + (size_t)d * kSlotsPerDevice);
 }
 
 __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax_src, int nnz,

@@ -74,6 +74,17 @@ def _bind():
     L.rs_tran_fixed.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, vp,
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32)]
+    L.rs_batch_create.restype = vp
+    L.rs_batch_create.argtypes = [vp, vp, vp, C.c_int64, C.c_int32]
+    L.rs_batch_destroy.argtypes = [vp]
+    L.rs_batch_last_error.restype = C.c_int32
+    L.rs_batch_last_error.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.rs_batch_info.restype = C.c_int32
+    L.rs_batch_info.argtypes = [vp, vp]
+    L.rs_batch_dc_op.restype = C.c_int32
+    L.rs_batch_dc_op.argtypes = [vp, C.POINTER(NewtonOpts), vp]
+    L.rs_batch_results.restype = C.c_int32
+    L.rs_batch_results.argtypes = [vp, vp, vp, vp]
     L._engine_bound = True
     return L
 
@@ -170,3 +181,59 @@ class GpuCircuit:
         self._check(self._L.rs_tran_fixed(self.h, solver.h, C.byref(opts), dt, nsteps, _ptr(x), cap,
                                           C.byref(xl), C.byref(it), C.byref(st)))
         return st.value, x[: xl.value].copy(), it.value
+
+
+class GpuBatch:
+    """Many independent DC operating points solved in one persistent launch
+    (one wavefront per instance, work queue in descending size order)."""
+
+    def __init__(self, circuits, kind=0):
+        self._L = _bind()
+        self.circuits = list(circuits)
+        devs = [np.ascontiguousarray(c.devices) for c in self.circuits]
+        self._dev = np.ascontiguousarray(np.concatenate(devs))
+        self._ptr = np.zeros(len(devs) + 1, np.int64)
+        self._ptr[1:] = np.cumsum([len(d) for d in devs])
+        self._nc = np.array([c.node_count for c in self.circuits], np.int64)
+        self.h = self._L.rs_batch_create(_ptr(self._dev), _ptr(self._ptr), _ptr(self._nc),
+                                         len(self.circuits), kind)
+        self._check(0)
+
+    def _check(self, rc):
+        buf = C.create_string_buffer(512)
+        self._L.rs_batch_last_error(self.h, buf, 512)
+        if rc != 0 or buf.value:
+            raise SolverError(rc or 7, buf.value.decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.rs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(4, np.int64)
+        self._L.rs_batch_info(self.h, _ptr(out))
+        return dict(instances=int(out[0]), patterns=int(out[1]), nmax=int(out[2]), grid=int(out[3]))
+
+    def dc_op(self, opts=None, stream=None):
+        """Enqueues the batched solve (asynchronous on `stream`)."""
+        opts = opts or default_opts()
+        self._opts = opts
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        self._check(self._L.rs_batch_dc_op(self.h, C.byref(opts), C.c_void_p(s)))
+
+    def results(self):
+        """List of (x or None, NewtonResult) per instance (x: full MNA vector)."""
+        k = len(self.circuits)
+        res = (NewtonResult * k)()
+        xp = np.zeros(k + 1, np.int64)
+        self._check(self._L.rs_batch_results(self.h, C.cast(res, C.c_void_p), None, _ptr(xp)))
+        x = np.zeros(max(int(xp[-1]), 1))
+        self._check(self._L.rs_batch_results(self.h, C.cast(res, C.c_void_p), _ptr(x), _ptr(xp)))
+        return [((x[xp[i]:xp[i + 1]].copy() if res[i].converged else None), res[i]) for i in range(k)]

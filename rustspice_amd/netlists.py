@@ -205,24 +205,96 @@ def inverter_chain(
 C2_TRAN = dict(tstep=sfx(5, P), tstop=sfx(10, N), tstart=0.0, tmax=sfx(5, P))
 
 
+class LcgStream:
+    """The same LCG as :class:`Lcg`, drawn in vectorised blocks with a
+    precomputed jump table (uint64 arithmetic wraps modulo 2**64)."""
+
+    BLOCK = 1 << 16
+    _A = None
+    _C = None
+
+    def __init__(self, seed: int):
+        if LcgStream._A is None:
+            a, c = 6364136223846793005, 1442695040888963407
+            A = np.empty(self.BLOCK, np.uint64)
+            Cc = np.empty(self.BLOCK, np.uint64)
+            pa, pc = 1, 0
+            for j in range(self.BLOCK):
+                pa = (pa * a) & Lcg.MASK
+                pc = (pc * a + c) & Lcg.MASK
+                A[j], Cc[j] = pa, pc
+            LcgStream._A, LcgStream._C = A, Cc
+        self.state = np.uint64(seed & Lcg.MASK)
+        self.buf = np.zeros(0)
+        self.pos = 0
+
+    def _refill(self):
+        with np.errstate(over="ignore"):
+            st = LcgStream._A * self.state + LcgStream._C
+        self.state = st[-1]
+        u = (st >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        self.buf = np.concatenate([self.buf[self.pos:], u])
+        self.pos = 0
+
+    def uniforms(self, k: int) -> np.ndarray:
+        while len(self.buf) - self.pos < k:
+            self._refill()
+        out = self.buf[self.pos:self.pos + k]
+        self.pos += k
+        return out
+
+    def normals(self, k: int) -> np.ndarray:
+        """Box-Muller, one normal per (u1, u2) pair taken in stream order."""
+        u = self.uniforms(2 * k).reshape(k, 2)
+        return np.sqrt(-2.0 * np.log(1.0 - u[:, 0])) * np.cos(2.0 * np.pi * u[:, 1])
+
+
+def _chain_records(stages, vin_dc, vn, sn, vp, sp, load_c=sfx(5, F), with_caps=True):
+    """Vectorised inverter_chain(vin_pulse=False) records (same order)."""
+    per = 3 if with_caps else 2
+    arr = np.zeros(2 + per * stages, dtype=DEVICE_DTYPE)
+    arr[0]["kind"], arr[0]["node"], arr[0]["value"] = RS_DEV_V, (1, 0, 0, 0), 1.0
+    arr[1]["kind"], arr[1]["node"], arr[1]["value"] = RS_DEV_V, (2, 0, 0, 0), vin_dc
+    i = np.arange(stages)
+    out = 3 + i
+    prev = np.where(i == 0, 2, 2 + i)
+    mn = arr[2::per][:stages]
+    mp = arr[3::per][:stages]
+    u0n, u0p = u0_from_kp(NMOS_KP), u0_from_kp(PMOS_KP)
+    wn, wp, l = sfx(1, U), sfx(2, U), sfx(0.1, U)
+    mn["kind"] = RS_DEV_M
+    mn["node"] = np.stack([out, prev, np.zeros_like(i), np.zeros_like(i)], 1)
+    mn["p"][:, 0], mn["p"][:, 1], mn["p"][:, 2] = 1.0, vn, u0n * sn
+    mn["p"][:, 3], mn["p"][:, 4], mn["p"][:, 5] = TOX_DEFAULT, wn, l
+    mp["kind"], mp["flags"] = RS_DEV_M, RS_F_PMOS
+    mp["node"] = np.stack([out, prev, np.ones_like(i), np.ones_like(i)], 1)
+    mp["p"][:, 0], mp["p"][:, 1], mp["p"][:, 2] = 1.0, vp, u0p * sp
+    mp["p"][:, 3], mp["p"][:, 4], mp["p"][:, 5] = TOX_DEFAULT, wp, l
+    arr[2::per][:stages] = mn
+    arr[3::per][:stages] = mp
+    if with_caps:
+        cc = arr[4::per][:stages]
+        cc["kind"] = RS_DEV_C
+        cc["node"] = np.stack([out, np.zeros_like(i), np.zeros_like(i), np.zeros_like(i)], 1)
+        cc["value"] = load_c
+        arr[4::per][:stages] = cc
+    return arr
+
+
 def monte_carlo_chains(count: int = 4096, seed: int = 0x52530000 + 4, lo: int = 50,
                        hi: int = 400):
     """Monte-Carlo inverter chains: stages ~ U[lo, hi], VTO += N(0, 20 mV),
     u0 *= N(1, 0.05) per MOSFET, DC input at 0 V. Draw order per instance:
-    stage count, then per stage (NMOS dVTO, NMOS u0 scale, PMOS dVTO,
-    PMOS u0 scale)."""
-    rng = Lcg(seed)
+    one uniform for the stage count, then per stage four normals (NMOS dVTO,
+    NMOS u0 scale, PMOS dVTO, PMOS u0 scale), each from two uniforms."""
+    rng = LcgStream(seed)
     out = []
     for _ in range(count):
-        stages = lo + int(rng.uniform() * (hi - lo + 1))
-        vn, vp, sn, sp = [], [], [], []
-        for _s in range(stages):
-            vn.append(0.4 + 0.02 * rng.normal())
-            sn.append(1.0 + 0.05 * rng.normal())
-            vp.append(-0.4 + 0.02 * rng.normal())
-            sp.append(1.0 + 0.05 * rng.normal())
-        out.append(inverter_chain(stages, vin_pulse=False, vin_dc=0.0, vto_n=vn, vto_p=vp,
-                                  u0_scale_n=sn, u0_scale_p=sp))
+        stages = lo + int(rng.uniforms(1)[0] * (hi - lo + 1))
+        z = rng.normals(4 * stages).reshape(stages, 4)
+        recs = _chain_records(stages, 0.0, 0.4 + 0.02 * z[:, 0], 1.0 + 0.05 * z[:, 1],
+                              -0.4 + 0.02 * z[:, 2], 1.0 + 0.05 * z[:, 3])
+        out.append(Circuit(recs, 3 + stages, f"mc_inv{stages}"))
     return out
 
 
