@@ -3,10 +3,17 @@ import numpy as np
 
 
 def same_bits(a, b):
-    """Bit-identical float64 arrays (NaN payloads compared as bits)."""
+    """Bit-identical float64 arrays. NaNs count as equal whatever their sign
+    and payload: x86 and the GPU produce different default NaN encodings, and
+    IEEE 754 leaves the payload of an invalid operation unspecified."""
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
-    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a[~na].view(np.int64), b[~nb].view(np.int64))
 
 
 def random_csc(rs, n, dens, sym=None, diag=None, dup=None):
