@@ -126,6 +126,12 @@ int32_t rs_circuit_last_error(const rs_circuit *c, char *buf, int64_t len);
  * of engine.rs:262-283 / :485-501): call with ap = ai = NULL for the sizes. */
 int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap,
                            int64_t *ai);
+/* 1 (default): a DC OP whose solver starts from scratch (the circuit has aux
+ * rows and the solver its default configuration, no stepping) runs as one
+ * device-resident launch, with results identical to the host-driven loop;
+ * afterwards the solver keeps its counts but no factorization. 0: every
+ * Newton iteration is driven from the host. */
+void rs_circuit_set_device_newton(rs_circuit *c, int32_t on);
 /* TransientState per device (stamp.rs:699-718), device-index keyed. */
 int32_t rs_circuit_set_cap_state(rs_circuit *c, const double *cap_v, const double *cap_i,
                                  const uint8_t *has_v, const uint8_t *has_i);
@@ -160,7 +166,8 @@ rs_batch *rs_batch_create(const rs_device *dev, const int64_t *dev_ptr, const in
                           int64_t ninst, int32_t kind);
 void rs_batch_destroy(rs_batch *b);
 int32_t rs_batch_last_error(const rs_batch *b, char *buf, int64_t len);
-/* out[4] = {instances, distinct patterns, largest n, persistent grid size} */
+/* out[5] = {instances, distinct patterns, largest n, persistent grid size,
+ *           doubles of L/U values kept in LDS per instance (0: in HBM)} */
 int32_t rs_batch_info(const rs_batch *b, int64_t *out);
 /* Enqueues the batched Newton solve on `stream` (hipStream_t, NULL = default). */
 int32_t rs_batch_dc_op(rs_batch *b, const rs_newton_opts *o, void *stream);
@@ -170,8 +177,9 @@ int32_t rs_batch_results(rs_batch *b, rs_newton_result *res, double *x_out, int6
 
 /* ---- measurement (no reference counterpart) -------------------------------
  * Kernel timing with HIP events recorded on each kernel's launch stream.
- * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly. Enabling
- * (or disabling) clears the totals. */
+ * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly,
+ * 4 batched Newton (k_batch_dcop). ms_total and counts hold 5 entries.
+ * Enabling (or disabling) clears the totals. */
 void rs_profile_enable(int32_t on);
 void rs_profile_get(double *ms_total, int64_t *counts);
 

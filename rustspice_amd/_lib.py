@@ -32,7 +32,7 @@ EXPORTS = [
     "rs_circuit_set_cap_state", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
     "rs_profile_enable", "rs_profile_get",
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_info",
-    "rs_batch_dc_op", "rs_batch_results",
+    "rs_batch_dc_op", "rs_batch_results", "rs_circuit_set_device_newton",
 ]
 
 
@@ -106,7 +106,7 @@ def lib():
     return L
 
 
-PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble")
+PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble", "batch")
 
 
 def profile_enable(on: bool = True):
@@ -114,8 +114,8 @@ def profile_enable(on: bool = True):
 
 
 def profile_get() -> dict:
-    ms = np.zeros(4)
-    cnt = np.zeros(4, np.int64)
+    ms = np.zeros(len(PROFILE_KERNELS))
+    cnt = np.zeros(len(PROFILE_KERNELS), np.int64)
     lib().rs_profile_get(ms, cnt)
     return {k: dict(ms_total=float(m), count=int(c), avg_us=(1e3 * m / c if c else 0.0))
             for k, m, c in zip(PROFILE_KERNELS, ms, cnt)}

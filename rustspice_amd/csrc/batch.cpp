@@ -11,6 +11,7 @@
 
 #include "../../include/rsgpu.h"
 #include "batch.hpp"
+#include "profile.hpp"
 #include "plan.hpp"
 #include "solver_impl.hpp"
 #include "stamp.hpp"
@@ -18,8 +19,9 @@
 namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
                              int* counter, const int32_t* ip, const rs_device* devs, double* scr,
-                             BResult* res, const BOpts& o, int nmax, int grid, hipStream_t s);
-int batch_blocks_per_cu(int nmax);
+                             BResult* res, const BOpts& o, int nmax, int lumax, int grid,
+                             hipStream_t s);
+int batch_blocks_per_cu(int nmax, int lumax);
 }  // namespace rsgpu
 
 using namespace rsgpu;
@@ -27,7 +29,8 @@ using namespace rsgpu;
 struct rs_batch {
     int kind = RS_SOLVER_NATIVEKLU;
     int64_t ninst = 0;
-    int nmax = 0, npat = 0;
+    int nmax = 0, npat = 0, lumax = 0;
+    bool vals_lds = false;
     std::vector<BInst> inst;
     std::vector<int64_t> x_off;   // output x offset per instance
     std::vector<int32_t> n_of;    // matrix size per instance
@@ -181,6 +184,25 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
                 P.o_odax = put(pool, ph.odax);
                 P.o_odsrc = put(pool, ph.odsrc);
                 P.o_blk_has_od = put(pool, ph.blk_has_od);
+                {  // update program (pairs ordered as the left-looking loop visits them)
+                    std::vector<int32_t> up_ptr(ph.n + 1, 0), up_j, up_row, up_l;
+                    for (int32_t g = 0; g < ph.n; g++) {
+                        for (int32_t u = ph.ucp[g]; u < ph.ucp[g + 1] - 1; u++) {
+                            const int32_t j = ph.uri[u];
+                            for (int32_t l = ph.lcp[j]; l < ph.lcp[j + 1]; l++) {
+                                up_j.push_back(j);
+                                up_row.push_back(ph.lri[l]);
+                                up_l.push_back(l);
+                            }
+                        }
+                        up_ptr[g + 1] = (int32_t)up_j.size();
+                    }
+                    P.o_up_ptr = put(pool, up_ptr);
+                    P.o_up_j = put(pool, up_j);
+                    P.o_up_row = put(pool, up_row);
+                    P.o_up_l = put(pool, up_l);
+                }
+                P.nrhs = (int32_t)sp.rhs_src.size();
                 if (pool.size() > (size_t)INT32_MAX) throw std::runtime_error("pattern pool too large");
                 pid = (int)pats.size();
                 pats.push_back(P);
@@ -205,8 +227,11 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
             scr += P.n;
             I.o_x = scr;
             scr += P.n;
+            I.o_rv = scr;
+            scr += P.nrhs;
             b->n_of[k] = P.n;
             b->nmax = std::max(b->nmax, P.n);
+            b->lumax = std::max(b->lumax, P.nl + P.nu + P.n);
         }
     } catch (const std::exception& ex) {
         b->err = std::string("Invalid matrix: ") + ex.what();
@@ -239,7 +264,13 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     int dev_id = 0, cus = 0;
     (void)hipGetDevice(&dev_id);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-    int per_cu = batch_blocks_per_cu(b->nmax);
+    // L/U values in LDS when they are small: faster per instance, fewer
+    // instances resident per CU; chosen by the larger resident throughput
+    // estimate (waves per CU are capped at 16).
+    int per_cu_g = std::min(16, batch_blocks_per_cu(b->nmax, 0));
+    int per_cu_l = std::min(16, batch_blocks_per_cu(b->nmax, b->lumax));
+    b->vals_lds = per_cu_l >= 4 && b->lumax <= 8192;
+    int per_cu = b->vals_lds ? per_cu_l : per_cu_g;
     b->grid = (int)std::min<int64_t>(ninst, (int64_t)std::max(cus, 1) * per_cu);
     if (b->grid < 1) b->grid = 1;
     return b;
@@ -268,6 +299,7 @@ int32_t rs_batch_info(const rs_batch* b, int64_t* out) {
     out[1] = b->npat;
     out[2] = b->nmax;
     out[3] = b->grid;
+    out[4] = b->vals_lds ? b->lumax : 0;
     return b->err.empty() ? RS_OK : RS_ERR_DEVICE;
 }
 
@@ -279,8 +311,11 @@ int32_t rs_batch_dc_op(rs_batch* b, const rs_newton_opts* o, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipMemsetAsync(b->d_counter.p, 0, sizeof(int), s);
     if (e) return bhip(b, e, "hipMemset(counter)");
+    const int ps = prof_begin(PK_BATCH, s);
     e = launch_batch_dcop(b->d_pats.p, b->d_inst.p, b->d_order.p, (int)b->ninst, b->d_counter.p,
-                          b->d_ip.p, b->d_dev.p, b->d_scr.p, b->d_res.p, bo, b->nmax, b->grid, s);
+                          b->d_ip.p, b->d_dev.p, b->d_scr.p, b->d_res.p, bo, b->nmax,
+                          b->vals_lds ? b->lumax : 0, b->grid, s);
+    prof_end(ps, s);
     if (e) return bhip(b, e, "k_batch_dcop");
     return RS_OK;
 }

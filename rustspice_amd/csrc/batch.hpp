@@ -30,12 +30,16 @@ struct BPat {
     int32_t o_pin, o_pout, o_lrp, o_lrc, o_lrv, o_urp, o_urc, o_urv;
     int32_t o_blk_ptr, o_lsl_blk, o_lsl_ptr, o_lsl_rows, o_usl_blk, o_usl_ptr, o_usl_rows;
     int32_t o_odp, o_odax, o_odsrc, o_blk_has_od;
+    // per-column update program: for each U(j,g), j ascending, every L(:,j)
+    // entry as (j, row, index into L values)
+    int32_t o_up_ptr, o_up_j, o_up_row, o_up_l;
+    int32_t nrhs, pad2;
 };
 
 struct BInst {
     int32_t pat, ndev;
     int64_t o_dev;  // first device record
-    int64_t o_sv, o_ax, o_lx, o_ux, o_ud, o_x;
+    int64_t o_sv, o_ax, o_lx, o_ux, o_ud, o_x, o_rv;
 };
 
 struct BResult {

@@ -1,14 +1,20 @@
 // batch_kernels.hip - persistent batched DC operating point kernel.
 //
-// One wavefront (workgroup of 64) owns one instance at a time and runs its
+// One wavefront (a workgroup of 64) owns one instance at a time and runs its
 // whole Newton loop (newton.rs:65-185 inside engine.rs:249-311): device
 // evaluation and assembly (stamp.rs, mna.rs), the first numeric factor with
 // threshold pivoting inside the static pattern and later refactors with the
 // reused pivots (native_klu.rs:1194-1382), the BTF/L/U solve
 // (native_klu.rs:1442-1551), the convergence test, damping and x update.
-// The work vector, the right-hand side, x, x_new and the pivot arrays live in
-// LDS; stamp values and L/U values in the instance's global scratch. Every
-// floating-point operation is the reference's, in the reference's order.
+// Every floating-point operation is the reference's, in the reference's order.
+//
+// Latency is what bounds one instance, so each factor column runs from a
+// precomputed column program: the (j, row, L index) pairs of its left-looking
+// updates are loaded for up to 64 pairs at once, then applied group by group
+// in ascending j with only LDS traffic. Work vector, x, x_new and the pivot
+// arrays always live in LDS; the L/U values too when they fit (lumax > 0).
+// The workgroup is one wavefront, whose LDS and global accesses execute in
+// program order, so steps are separated by compiler fences only.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -22,7 +28,19 @@ using namespace rsgpu;
 
 constexpr int kW = 64;
 
-__device__ __forceinline__ void bsync() { __syncthreads(); }
+// LDS exchange between the lanes of this wavefront: LDS instructions of one
+// wavefront execute in program order, so a compiler fence is enough.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Global-memory exchange between lanes: workgroup-scope fence (waits for the
+// wavefront's outstanding vector memory operations).
+__device__ __forceinline__ void gsync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
 
 __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats,
                                                    const BInst* __restrict__ insts,
@@ -31,13 +49,14 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                                    const int32_t* __restrict__ ip,
                                                    const rs_device* __restrict__ devs,
                                                    double* __restrict__ scr,
-                                                   BResult* __restrict__ res, BOpts o, int nmax) {
+                                                   BResult* __restrict__ res, BOpts o, int nmax,
+                                                   int lumax) {
     extern __shared__ double lds[];
-    double* W = lds;
-    double* T = W + nmax;
-    double* X = T + nmax;
-    double* XN = X + nmax;
-    int* RPI = reinterpret_cast<int*>(XN + nmax);
+    double* W = lds;            // factor work vector; also the solve vector
+    double* X = W + nmax;
+    double* XN = X + nmax;      // rhs, then x_new
+    double* VL = XN + nmax;     // L/U/diag values when lumax > 0
+    int* RPI = reinterpret_cast<int*>(VL + lumax);
     int* RP = RPI + nmax;
     __shared__ int s_qi;
     __shared__ double s_dxn, s_fn;
@@ -45,9 +64,9 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
 
     for (;;) {
         if (lane == 0) s_qi = atomicAdd(counter, 1);
-        bsync();
+        __syncthreads();
         const int qi = s_qi;
-        bsync();
+        __syncthreads();
         if (qi >= ninst) break;
         const int ii = order[qi];
         const BInst I = insts[ii];
@@ -56,9 +75,10 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         const rs_device* dev = devs + I.o_dev;
         double* sv = scr + I.o_sv;
         double* ax = scr + I.o_ax;
-        double* lx = scr + I.o_lx;
-        double* ux = scr + I.o_ux;
-        double* ud = scr + I.o_ud;
+        double* rv = scr + I.o_rv;
+        double* lx = lumax ? VL : scr + I.o_lx;
+        double* ux = lumax ? VL + P.nl : scr + I.o_ux;
+        double* ud = lumax ? VL + P.nl + P.nu : scr + I.o_ud;
         const int32_t* ax_src = ip + P.o_ax_src;
         const int32_t* rhs_ptr = ip + P.o_rhs_ptr;
         const int32_t* rhs_src = ip + P.o_rhs_src;
@@ -71,6 +91,10 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         const int32_t* ucp = ip + P.o_ucp;
         const int32_t* uri = ip + P.o_uri;
         const int32_t* colflag = ip + P.o_colflag;
+        const int32_t* up_ptr = ip + P.o_up_ptr;
+        const int32_t* up_j = ip + P.o_up_j;
+        const int32_t* up_row = ip + P.o_up_row;
+        const int32_t* up_l = ip + P.o_up_l;
         const int32_t* pin = ip + P.o_pin;
         const int32_t* pout = ip + P.o_pout;
         const int32_t* lrp = ip + P.o_lrp;
@@ -98,7 +122,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         int xl = nc;  // the Newton x has node_count entries until convergence
         bool factored = false;
         BResult fin{0, 0, 1, 0, 0.0};
-        bsync();
+        wsync();
 
         const double gstart = fmax(o.gmin * 1e3, 1e-6);
         for (int gi = 0; gi <= o.gmin_steps && !fin.converged; ++gi) {
@@ -107,34 +131,36 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                     : gstart + (o.gmin - gstart) * ((double)gi / (double)o.gmin_steps);
             for (int si = 0; si <= o.source_steps && !fin.converged; ++si) {
                 const double scale = o.source_steps == 0 ? 1.0 : (double)si / (double)o.source_steps;
-                // ---- run_newton ----
                 BResult r{0, 0, 1, fin.full_factors, 0.0};
                 double damping = o.damping;
                 double prev = 1.7976931348623157e308;
                 for (int it = 0; it < o.max_iters; ++it) {
                     r.iterations = it + 1;
+                    // ---- stamp: evaluate, gather ax and the RHS contributions ----
                     StampArgs a{0.0, 0.0, gmin, scale, MODE_DC, 0};
                     for (int d = lane; d < I.ndev; d += kW) {
                         const rs_device v = dev[d];
                         eval_device(v, d, X, xl, a, nullptr, nullptr, nullptr, nullptr,
                                     sv + (size_t)d * kSlotsPerDevice);
                     }
-                    bsync();
+                    gsync();
                     for (int q = lane; q < P.nnz; q += kW) {
                         const int s = ax_src[q];
                         ax[q] = s < 0 ? 1.0 : sv[s];
                     }
-                    for (int i = lane; i < n; i += kW) {
-                        double acc = 0.0;
-                        for (int k = rhs_ptr[i]; k < rhs_ptr[i + 1]; ++k) acc += sv[rhs_src[k]];
-                        XN[i] = acc;  // rhs until the solve overwrites it
-                    }
+                    for (int k = lane; k < P.nrhs; k += kW) rv[k] = sv[rhs_src[k]];
                     if (!factored)
                         for (int i = lane; i < n; i += kW) {
                             RPI[i] = i;
                             RP[i] = i;
                         }
-                    bsync();
+                    gsync();
+                    for (int i = lane; i < n; i += kW) {
+                        double acc = 0.0;  // 0.0 + contributions in insertion order
+                        const int k1 = rhs_ptr[i + 1];
+                        for (int k = rhs_ptr[i]; k < k1; ++k) acc += rv[k];
+                        XN[i] = acc;
+                    }
                     // ---- factor (first: with pivot search; later: reuse) ----
                     for (int g = 0; g < n; ++g) {
                         for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW) {
@@ -143,21 +169,29 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                             for (int q = 0; q < cnt; ++q) v += ax[src + q];
                             W[RPI[sc_pre[s]]] = v;
                         }
-                        bsync();
-                        const int us = ucp[g], ue = ucp[g + 1] - 1;
-                        for (int u = us; u < ue; ++u) {
-                            const int j = uri[u];
-                            const double ujk = W[j];
-                            if (ujk != 0.0) {
-                                for (int l = lcp[j] + lane; l < lcp[j + 1]; l += kW) {
-                                    const int row = lri[l];
-                                    W[row] = W[row] - lx[l] * ujk;
-                                }
-                                bsync();
+                        wsync();
+                        const int p1 = up_ptr[g + 1];
+                        for (int base = up_ptr[g]; base < p1; base += kW) {
+                            const int idx = base + lane;
+                            const bool act = idx < p1;
+                            const int jj = act ? up_j[idx] : INT_MAX;
+                            const int rr = act ? up_row[idx] : 0;
+                            const double lv = act ? lx[up_l[idx]] : 0.0;
+                            const int cend = (p1 - base) < kW ? (p1 - base) : kW;
+                            int cur = 0;
+                            while (cur < cend) {  // groups of equal j, ascending
+                                const int jcur = __shfl(jj, cur);
+                                const double u = W[jcur];
+                                const bool mine = act && jj == jcur;
+                                const unsigned long long m = __ballot(mine);
+                                if (u != 0.0 && mine) W[rr] = W[rr] - lv * u;
+                                wsync();
+                                cur += __popcll(m);
                             }
                         }
                         const bool single = (colflag[g] & 1) != 0;
                         const int ls = lcp[g], le = lcp[g + 1];
+                        const int us = ucp[g], ue = ucp[g + 1] - 1;
                         if (!factored && !single && le > ls) {
                             const double dabs = fabs(W[g]);
                             double bv = -1.0;
@@ -177,20 +211,18 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                     bl = ol;
                                 }
                             }
-                            if (bl != INT_MAX && dabs < o.pivot_tol * bv) {
-                                if (lane == 0) {
-                                    const int mrow = lri[bl];
-                                    const double t = W[g];
-                                    W[g] = W[mrow];
-                                    W[mrow] = t;
-                                    const int okk = RP[g], om = RP[mrow];
-                                    RP[g] = om;
-                                    RP[mrow] = okk;
-                                    RPI[okk] = mrow;
-                                    RPI[om] = g;
-                                }
+                            if (bl != INT_MAX && dabs < o.pivot_tol * bv && lane == 0) {
+                                const int mrow = lri[bl];
+                                const double t = W[g];
+                                W[g] = W[mrow];
+                                W[mrow] = t;
+                                const int okk = RP[g], om = RP[mrow];
+                                RP[g] = om;
+                                RP[mrow] = okk;
+                                RPI[okk] = mrow;
+                                RPI[om] = g;
                             }
-                            bsync();
+                            wsync();
                         }
                         for (int u = us + lane; u < ue; u += kW) ux[u] = W[uri[u]];
                         double diag = W[g];
@@ -204,18 +236,20 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                             ud[g] = diag;
                         }
                         for (int l = ls + lane; l < le; l += kW) lx[l] = W[lri[l]] / diag;
-                        bsync();
+                        wsync();
                         for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW) W[RPI[sc_pre[s]]] = 0.0;
                         for (int l = ls + lane; l < le; l += kW) W[lri[l]] = 0.0;
                         for (int u = us + lane; u < ue; u += kW) W[uri[u]] = 0.0;
                         if (lane == 0) W[g] = 0.0;
-                        bsync();
+                        if (lumax) wsync();
+                        else gsync();  // L/U values of this column are read by later columns
                     }
                     if (!factored) r.full_factors++;
                     factored = true;
-                    // ---- solve ----
+                    // ---- solve, in W (all zero now; re-zeroed afterwards) ----
+                    double* T = W;
                     for (int i = lane; i < n; i += kW) T[RPI[pin[i]]] = XN[i];
-                    bsync();
+                    wsync();
                     for (int b = P.nblk - 1; b >= 0; --b) {
                         if (has_od[b]) {
                             for (int p = blk_ptr[b] + lane; p < blk_ptr[b + 1]; p += kW) {
@@ -226,7 +260,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                 for (int e = e0; e < e1; ++e) t = t - ax[odax[e]] * T[odsrc[e]];
                                 T[pos] = t;
                             }
-                            bsync();
+                            wsync();
                         }
                         for (int lev = lsl_blk[b]; lev < lsl_blk[b + 1]; ++lev) {
                             for (int q = lsl_ptr[lev] + lane; q < lsl_ptr[lev + 1]; q += kW) {
@@ -237,7 +271,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                 for (int e = e0; e < e1; ++e) t = t - lx[lrv[e]] * T[lrc[e]];
                                 T[rr] = t;
                             }
-                            bsync();
+                            wsync();
                         }
                         for (int lev = usl_blk[b]; lev < usl_blk[b + 1]; ++lev) {
                             for (int q = usl_ptr[lev] + lane; q < usl_ptr[lev + 1]; q += kW) {
@@ -246,11 +280,12 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                 for (int e = urp[rr]; e < urp[rr + 1]; ++e) t = t - ux[urv[e]] * T[urc[e]];
                                 T[rr] = t / ud[rr];
                             }
-                            bsync();
+                            wsync();
                         }
                     }
                     for (int i = lane; i < n; i += kW) XN[i] = T[pout[i]];
-                    bsync();
+                    wsync();
+                    for (int i = lane; i < n; i += kW) W[i] = 0.0;
                     // ---- convergence, damping, update (newton.rs:93-107) ----
                     const int nd = n < xl ? n : xl;
                     int ok = 1;
@@ -258,7 +293,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         const double dxi = XN[i] - X[i];
                         if (!(fabs(dxi) <= o.abs_tol + o.rel_tol * fabs(XN[i]))) ok = 0;
                     }
-                    const int conv = __syncthreads_and(ok);
+                    const bool conv = __ballot(ok == 0) == 0ull;
                     if (lane == 0) {
                         double sd = 0.0, sf = 0.0;
                         for (int i = 0; i < nd; ++i) {
@@ -269,7 +304,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         s_dxn = sqrt(sd);
                         s_fn = sqrt(sf);
                     }
-                    bsync();
+                    wsync();
                     const double dxn = s_dxn;
                     r.final_norm = s_fn;
                     if (conv) {
@@ -277,7 +312,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         xl = n;
                         r.converged = 1;
                         r.reason = 0;
-                        bsync();
+                        wsync();
                         break;
                     }
                     if (dxn > prev && damping > o.damping_min) {
@@ -286,7 +321,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     }
                     prev = dxn;
                     for (int i = lane; i < nd; i += kW) X[i] = X[i] + damping * (XN[i] - X[i]);
-                    bsync();
+                    wsync();
                 }
                 fin = r;
             }
@@ -294,8 +329,17 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         if (lane == 0) res[ii] = fin;
         if (fin.converged)
             for (int i = lane; i < n; i += kW) scr[I.o_x + i] = X[i];
-        bsync();
+        if (lumax) {  // leave the factor in global scratch as well
+            for (int l = lane; l < P.nl; l += kW) scr[I.o_lx + l] = lx[l];
+            for (int u = lane; u < P.nu; u += kW) scr[I.o_ux + u] = ux[u];
+            for (int i = lane; i < n; i += kW) scr[I.o_ud + i] = ud[i];
+        }
+        __syncthreads();
     }
+}
+
+size_t lds_bytes(int nmax, int lumax) {
+    return (size_t)nmax * (3 * sizeof(double) + 2 * sizeof(int)) + (size_t)lumax * sizeof(double);
 }
 
 }  // namespace
@@ -304,17 +348,18 @@ namespace rsgpu {
 
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
                              int* counter, const int32_t* ip, const rs_device* devs, double* scr,
-                             BResult* res, const BOpts& o, int nmax, int grid, hipStream_t s) {
-    size_t shm = (size_t)nmax * (4 * sizeof(double) + 2 * sizeof(int));
-    hipLaunchKernelGGL(k_batch_dcop, dim3(grid), dim3(kW), shm, s, pats, insts, order, ninst,
-                       counter, ip, devs, scr, res, o, nmax);
+                             BResult* res, const BOpts& o, int nmax, int lumax, int grid,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_dcop, dim3(grid), dim3(kW), lds_bytes(nmax, lumax), s, pats, insts,
+                       order, ninst, counter, ip, devs, scr, res, o, nmax, lumax);
     return hipGetLastError();
 }
 
-int batch_blocks_per_cu(int nmax) {
-    size_t shm = (size_t)nmax * (4 * sizeof(double) + 2 * sizeof(int));
+int batch_blocks_per_cu(int nmax, int lumax) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop, kW, shm) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop, kW, lds_bytes(nmax, lumax)) !=
+            hipSuccess ||
+        nb < 1)
         nb = 1;
     return nb;
 }

@@ -36,6 +36,15 @@ hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int
 
 using namespace rsgpu;
 
+extern "C" {
+rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const int64_t* node_count,
+                          int64_t ninst, int32_t kind);
+void rs_batch_destroy(rs_batch* b);
+int32_t rs_batch_dc_op(rs_batch* b, const rs_newton_opts* o, void* stream);
+int32_t rs_batch_results(rs_batch* b, rs_newton_result* res, double* x_out, int64_t* x_ptr);
+int32_t rs_batch_last_error(const rs_batch* b, char* buf, int64_t len);
+}
+
 namespace {
 struct DevPattern {
     bool built = false;
@@ -53,6 +62,9 @@ struct rs_circuit {
     DevBuf<uint8_t> d_has_v, d_has_i;
     std::string err;
     int64_t stamp_calls = 0, newton_iters = 0, factor_calls = 0;
+    bool device_newton = true;  // whole DC OP in one launch when equivalent
+    rs_batch* dcb[2] = {nullptr, nullptr};  // one-instance batch per solver kind
+    int64_t device_dc_runs = 0;
 };
 
 namespace {
@@ -312,6 +324,66 @@ void lte_milne(const std::vector<double>& xb, const std::vector<double>& xt, dou
     out[4] = raw;
 }
 
+// DC OP as one device-resident launch (the batched Newton kernel on a single
+// instance). Equivalent to run_newton_with_stepping on the host path when the
+// solver starts from scratch: the circuit has aux rows (so prepare(n) resets
+// the solver, native_klu.rs:1604-1610) and the solver uses the default
+// configuration. Afterwards the handle keeps the reference's factor/refactor
+// counts but no factorization; callers that continue on the same pattern go
+// through the host path (see can_run_device_dc).
+constexpr int64_t kDeviceDcMaxN = 2048;
+
+bool can_run_device_dc(rs_circuit* c, rs_solver* sol) {
+    if (!c->device_newton) return false;
+    DevPattern* dp;
+    if (get_pattern(c, MODE_DC, dp)) return false;
+    if (dp->p.n == c->node_count) return false;
+    // one wavefront per instance: large meshes run faster on the level-scheduled
+    // multi-wave kernels, and W/X/XN/RPI/RP must fit in LDS
+    if (dp->p.n > kDeviceDcMaxN) return false;
+    if (sol->kind == RS_SOLVER_NATIVEKLU &&
+        !(sol->pivot_tol == 0.001 && sol->ordering == 0 && sol->use_btf))
+        return false;
+    return true;
+}
+
+int32_t device_dc(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::vector<double>& x,
+                  rs_newton_result* r, int32_t* iters_total) {
+    const int k = sol->kind == RS_SOLVER_SPARSELU ? 1 : 0;
+    if (!c->dcb[k]) {
+        int64_t ptr[2] = {0, (int64_t)c->dev.size()};
+        int64_t nc = c->node_count;
+        c->dcb[k] = rs_batch_create(c->dev.data(), ptr, &nc, 1, sol->kind);
+        char buf[256];
+        if (rs_batch_last_error(c->dcb[k], buf, sizeof buf) > 0) return cfail(c, RS_ERR_DEVICE, buf);
+    }
+    int32_t rc = rs_batch_dc_op(c->dcb[k], o, nullptr);
+    DevPattern* dp;
+    get_pattern(c, MODE_DC, dp);
+    std::vector<double> xo(dp->p.n);
+    int64_t xp[2];
+    if (!rc) rc = rs_batch_results(c->dcb[k], r, xo.data(), xp);
+    if (rc) {
+        char buf[256];
+        rs_batch_last_error(c->dcb[k], buf, sizeof buf);
+        return cfail(c, rc, buf);
+    }
+    const int32_t full = r->pad;
+    r->pad = 0;
+    if (r->converged) x = xo;
+    // iterations of earlier stepping passes are not reported by the kernel;
+    // the reference engine runs one pass (gmin_steps = source_steps = 0)
+    if (iters_total) *iters_total += r->iterations;
+    c->newton_iters += r->iterations;
+    c->factor_calls += r->iterations;
+    c->device_dc_runs++;
+    rs_prepare(sol, dp->p.n);
+    rs_reset_pattern(sol);
+    sol->factor_count = full;
+    sol->refactor_count = r->iterations - full;
+    return RS_OK;
+}
+
 int32_t reset_cap_state(rs_circuit* c) {
     hipError_t e;
     size_t nd = c->dev.size();
@@ -361,6 +433,8 @@ rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_c
 
 void rs_circuit_destroy(rs_circuit* c) {
     if (!c) return;
+    for (auto*& b : c->dcb)
+        if (b) rs_batch_destroy(b);
     for (auto& p : c->pat) {
         p.ax_src.release();
         p.rhs_ptr.release();
@@ -378,6 +452,8 @@ void rs_circuit_destroy(rs_circuit* c) {
     c->d_has_i.release();
     delete c;
 }
+
+void rs_circuit_set_device_newton(rs_circuit* c, int32_t on) { c->device_newton = on != 0; }
 
 int32_t rs_circuit_last_error(const rs_circuit* c, char* buf, int64_t len) {
     if (len > 0) std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
@@ -430,7 +506,9 @@ int32_t rs_dc_op(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double*
     std::vector<double> x(c->node_count, 0.0);
     rs_prepare(sol, c->node_count);  // engine.rs:254
     rs_newton_result r{};
-    int32_t rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &r, nullptr);
+    int32_t rc = (can_run_device_dc(c, sol) && o->gmin_steps == 0 && o->source_steps == 0)
+                     ? device_dc(c, sol, o, x, &r, nullptr)
+                     : run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &r, nullptr);
     if (rc) return rc;
     *res = r;
     if (x_out) std::memcpy(x_out, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
@@ -472,7 +550,14 @@ int32_t rs_tran(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double t
     int32_t iters = 0;
     rs_newton_result dcr{};
     int64_t fc0 = c->factor_calls;
-    rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &dcr, &iters);
+    bool dev_dc = can_run_device_dc(c, sol) && o->gmin_steps == 0 && o->source_steps == 0;
+    if (dev_dc) {  // the TRAN pattern must differ from DC's, else TRAN would refactor on DC pivots
+        DevPattern *pd, *pt;
+        if ((rc = get_pattern(c, MODE_DC, pd)) || (rc = get_pattern(c, MODE_TRAN_BE, pt))) return rc;
+        dev_dc = !(pd->p.ap == pt->p.ap && pd->p.ai == pt->p.ai);
+    }
+    rc = dev_dc ? device_dc(c, sol, o, x, &dcr, &iters)
+                : run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &dcr, &iters);
     if (rc) return rc;
     res->dc_iters = dcr.iterations;
     if (!dcr.converged) {

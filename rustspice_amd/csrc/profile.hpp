@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 
 namespace rsgpu {
-enum ProfKernel { PK_REFACTOR = 0, PK_SOLVE = 1, PK_EVAL = 2, PK_ASSEMBLE = 3, PK_COUNT = 4 };
+enum ProfKernel { PK_REFACTOR = 0, PK_SOLVE = 1, PK_EVAL = 2, PK_ASSEMBLE = 3, PK_BATCH = 4, PK_COUNT = 5 };
 bool prof_on();
 // Returns an event pair slot to record around a launch, or -1 when disabled.
 int prof_begin(int kernel, hipStream_t s);

@@ -74,6 +74,7 @@ def _bind():
     L.rs_tran_fixed.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, vp,
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32)]
+    L.rs_circuit_set_device_newton.argtypes = [vp, C.c_int32]
     L.rs_batch_create.restype = vp
     L.rs_batch_create.argtypes = [vp, vp, vp, C.c_int64, C.c_int32]
     L.rs_batch_destroy.argtypes = [vp]
@@ -119,6 +120,10 @@ class GpuCircuit:
         self._L.rs_circuit_last_error(self.h, buf, 512)
         if rc != 0 or (buf.value and rc == 0 and b"hipMalloc" in buf.value):
             raise SolverError(rc or 7, buf.value.decode(errors="replace"))
+
+    def set_device_newton(self, on: bool):
+        """True: a fresh DC OP runs as one device launch (same results)."""
+        self._L.rs_circuit_set_device_newton(self.h, int(on))
 
     def pattern(self, mode=MODE_DC):
         n, nnz = C.c_int64(), C.c_int64()
@@ -217,9 +222,10 @@ class GpuBatch:
             pass
 
     def info(self):
-        out = np.zeros(4, np.int64)
+        out = np.zeros(5, np.int64)
         self._L.rs_batch_info(self.h, _ptr(out))
-        return dict(instances=int(out[0]), patterns=int(out[1]), nmax=int(out[2]), grid=int(out[3]))
+        return dict(instances=int(out[0]), patterns=int(out[1]), nmax=int(out[2]), grid=int(out[3]),
+                    lds_values=int(out[4]))
 
     def dc_op(self, opts=None, stream=None):
         """Enqueues the batched solve (asynchronous on `stream`)."""

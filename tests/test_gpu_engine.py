@@ -131,3 +131,45 @@ def test_tran_fixed_step(gpu):
     st, xg, itg = E.GpuCircuit(c).tran_fixed(S.GpuSolver(c.node_count), 1e-11, 50)
     assert rco == st == 0 and ito == itg
     assert same_bits(xg, xo)
+
+
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+@pytest.mark.parametrize("which", ["mesh", "divider", "chain", "mc", "diode"])
+def test_device_dc_equals_host_driven(gpu, kind, which):
+    """The one-launch DC OP gives the host-driven Newton loop's result bit for
+    bit, and leaves the same factor/refactor counts on the solver."""
+    if which == "divider":
+        b = NL._Builder()
+        b.add(NL.RS_DEV_V, (1, 0), 1.0)
+        b.add(NL.RS_DEV_R, (1, 2), 1e3)
+        b.add(NL.RS_DEV_R, (2, 0), 2e3)
+        c = b.build(3, "divider")
+    else:
+        c = {"mesh": lambda: NL.resistor_mesh(20, 20), "chain": lambda: NL.inverter_chain(60),
+             "mc": lambda: NL.monte_carlo_chains(2)[1], "diode": NL.diode_circuit}[which]()
+    out = []
+    for dev in (True, False):
+        g = E.GpuCircuit(c)
+        g.set_device_newton(dev)
+        s = S.GpuSolver(c.node_count, kind)
+        x, r = g.dc_op(s)
+        st = s.stats()
+        out.append((x, r, st["factor_count"], st["refactor_count"]))
+    (xd, rd, fd, rfd), (xh, rh, fh, rfh) = out
+    assert (rd.converged, rd.iterations, rd.reason) == (rh.converged, rh.iterations, rh.reason)
+    assert same_bits([rd.final_norm], [rh.final_norm])
+    assert same_bits(xd, xh)
+    assert (fd, rfd) == (fh, rfh)
+
+
+def test_device_dc_then_tran_equals_host_driven(gpu):
+    c, args = NL.rc_ladder(10), dict(tstep=1e-10, tstop=2e-9, tmax=1e-10)
+    res = []
+    for dev in (True, False):
+        g = E.GpuCircuit(c)
+        g.set_device_newton(dev)
+        res.append(g.tran(S.GpuSolver(c.node_count), **args))
+    for k in ("status", "accepted", "rejected", "dc_iters", "npoints", "newton_iters_total",
+              "factor_calls"):
+        assert res[0][k] == res[1][k], k
+    assert same_bits(res[0]["sols"], res[1]["sols"])
