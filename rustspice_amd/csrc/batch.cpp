@@ -22,6 +22,7 @@ hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t
                              BResult* res, const BOpts& o, int nmax, int lumax, int grid,
                              hipStream_t s);
 int batch_blocks_per_cu(int nmax, int lumax);
+size_t batch_lds_bytes(int nmax, int lumax);
 }  // namespace rsgpu
 
 using namespace rsgpu;
@@ -264,12 +265,23 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     int dev_id = 0, cus = 0;
     (void)hipGetDevice(&dev_id);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-    // L/U values in LDS when they are small: faster per instance, fewer
-    // instances resident per CU; chosen by the larger resident throughput
-    // estimate (waves per CU are capped at 16).
+    // L/U values in LDS when they fit: faster per instance, fewer instances
+    // resident per CU. Kept when every instance still gets a resident
+    // wavefront, or when at least 4 fit per CU (waves per CU capped at 16).
+    int lds_max = 0;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id);
+    const size_t kStatic = 64;  // s_qi, s_dxn, s_fn
+    if (batch_lds_bytes(b->nmax, 0) + kStatic > (size_t)lds_max) {
+        char m[160];
+        std::snprintf(m, sizeof m, "instance of %d unknowns exceeds the %d bytes of LDS per workgroup",
+                      b->nmax, lds_max);
+        b->err = m;
+        return b;
+    }
+    const bool lds_fit = batch_lds_bytes(b->nmax, b->lumax) + kStatic <= (size_t)lds_max;
     int per_cu_g = std::min(16, batch_blocks_per_cu(b->nmax, 0));
-    int per_cu_l = std::min(16, batch_blocks_per_cu(b->nmax, b->lumax));
-    b->vals_lds = per_cu_l >= 4 && b->lumax <= 8192;
+    int per_cu_l = lds_fit ? std::min(16, batch_blocks_per_cu(b->nmax, b->lumax)) : 0;
+    b->vals_lds = lds_fit && (per_cu_l >= 4 || ninst <= (int64_t)std::max(cus, 1) * per_cu_l);
     int per_cu = b->vals_lds ? per_cu_l : per_cu_g;
     b->grid = (int)std::min<int64_t>(ninst, (int64_t)std::max(cus, 1) * per_cu);
     if (b->grid < 1) b->grid = 1;

@@ -355,6 +355,8 @@ hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t
     return hipGetLastError();
 }
 
+size_t batch_lds_bytes(int nmax, int lumax) { return lds_bytes(nmax, lumax); }
+
 int batch_blocks_per_cu(int nmax, int lumax) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop, kW, lds_bytes(nmax, lumax)) !=
