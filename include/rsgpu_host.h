@@ -43,6 +43,14 @@ void rs_host_factor_block_get(const rs_host_factor *h, int64_t b, int64_t *col_p
  *           scatter runs, off-diagonal entries, nnz(L), nnz(U)} */
 void rs_host_factor_plan_stats(const rs_host_factor *h, int64_t *out);
 
+/* Step-program check on the host (no reference counterpart): the first
+ * factor (pivots) on ax0, then the refactor on ax1 and the solve of rhs,
+ * executed by the host model of the GPU step programs. x equals
+ * NativeKluSolver factor(ax0); factor(ax1); solve(rhs) bit for bit.
+ * stats[4] = {refactor steps, refactor ops, solve steps, solve ops}. */
+int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t *ap, const int64_t *ai,
+                            const double *ax0, const double *ax1, const double *rhs, double *x,
+                            int64_t *stats);
 /* Unmerged CSC pattern of one stamp pass (mode 0 DC, else TRAN) as
  * SparseBuilder::finalize produces it (mna.rs:320-347); ap/ai may be NULL to
  * query n and nnz. Returns 0, or 6 for an unsupported device kind. */

@@ -11,16 +11,17 @@
 
 #include "../../include/rsgpu.h"
 #include "batch.hpp"
-#include "profile.hpp"
 #include "plan.hpp"
+#include "profile.hpp"
+#include "sp.hpp"
 #include "solver_impl.hpp"
 #include "stamp.hpp"
 
 namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
-                             int* counter, const int32_t* ip, const rs_device* devs, double* scr,
-                             BResult* res, const BOpts& o, int nmax, int lumax, int grid,
-                             hipStream_t s);
+                             int* counter, const int32_t* ip, const uint64_t* prog,
+                             const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
+                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s);
 int batch_blocks_per_cu(int nmax, int lumax);
 size_t batch_lds_bytes(int nmax, int lumax);
 }  // namespace rsgpu
@@ -37,7 +38,8 @@ struct rs_batch {
     std::vector<int32_t> n_of;    // matrix size per instance
     DevBuf<BPat> d_pats;
     DevBuf<BInst> d_inst;
-    DevBuf<int32_t> d_order, d_ip;
+    DevBuf<int32_t> d_order, d_ip, d_iscr;
+    DevBuf<uint64_t> d_prog;
     DevBuf<int> d_counter;
     DevBuf<rs_device> d_dev;
     DevBuf<double> d_scr;
@@ -58,50 +60,6 @@ int32_t bhip(rs_batch* b, hipError_t e, const char* what) {
     return RS_ERR_DEVICE;
 }
 
-// Pattern-level analysis: symbolic factor with identity pivots so that the
-// plan's scatter targets, solve permutation and off-diagonal row keys are the
-// pre-pivot positions the kernel maps through its own RPI.
-void analyze_pattern(const StampPattern& sp, int kind, PlanHost& ph) {
-    HostFactor f;
-    const i64 n = sp.n;
-    f.n = n;
-    const i64* ap = sp.ap.data();
-    const i64* ai = sp.ai.data();
-    if (kind == RS_SOLVER_NATIVEKLU && n >= 4) {
-        f.btf = btf_decompose(n, ap, ai);
-        f.has_btf = f.btf.num_blocks >= 2 && f.btf.max_block_size < n;
-    }
-    if (f.has_btf) {
-        f.sym.resize(f.btf.num_blocks);
-        for (i64 b = 0; b < f.btf.num_blocks; b++) {
-            i64 bs = f.btf.block_ptr[b], be = f.btf.block_ptr[b + 1], sz = be - bs;
-            std::vector<i64> bap(sz + 1, 0), bai;
-            for (i64 lc = 0; lc < sz; lc++) {
-                i64 gco = f.btf.col_perm[bs + lc];
-                for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
-                    i64 gro = ai[q];
-                    if (gro >= n) continue;
-                    i64 grn = f.btf.row_perm_inv[gro];
-                    if (grn >= bs && grn < be) bai.push_back(grn - bs);
-                }
-                bap[lc + 1] = (i64)bai.size();
-            }
-            block_symbolic(sz, bap.data(), bai.data(), false, f.sym[b]);
-        }
-    } else {
-        f.sym.resize(1);
-        block_symbolic(n, ap, ai, false, f.sym[0]);
-    }
-    f.num.resize(f.sym.size());
-    for (size_t b = 0; b < f.sym.size(); b++) {
-        i64 m = f.sym[b].n;
-        f.num[b].row_perm.resize(m);
-        f.num[b].row_perm_inv.resize(m);
-        for (i64 i = 0; i < m; i++) f.num[b].row_perm[i] = f.num[b].row_perm_inv[i] = i;
-    }
-    build_plan(f, ap, ai, ph);
-}
-
 int32_t put(std::vector<int32_t>& pool, const std::vector<int32_t>& v) {
     int32_t off = (int32_t)pool.size();
     pool.insert(pool.end(), v.begin(), v.end());
@@ -119,12 +77,14 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     b->kind = kind;
     b->ninst = ninst;
     std::vector<int32_t> pool;
+    std::vector<uint64_t> prog;
     std::vector<BPat> pats;
     std::map<std::vector<int64_t>, int> pat_id;  // key: node_count, n, ap, ai, maps
     b->inst.resize(ninst);
     b->x_off.resize(ninst + 1);
     b->n_of.resize(ninst);
-    int64_t scr = 0;
+    int64_t scr = 0, iscr = 0;
+    std::vector<int32_t> ph_nsc;  // scatter runs per pattern
     try {
         for (int64_t k = 0; k < ninst; k++) {
             const rs_device* d = dev + dev_ptr[k];
@@ -145,7 +105,7 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
             int pid;
             if (it == pat_id.end()) {
                 PlanHost ph;
-                analyze_pattern(sp, kind, ph);
+                build_identity_plan(sp.n, sp.ap.data(), sp.ai.data(), kind, ph);
                 BPat P{};
                 P.n = (int32_t)sp.n;
                 P.node_count = (int32_t)node_count[k];
@@ -204,9 +164,33 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
                     P.o_up_l = put(pool, up_l);
                 }
                 P.nrhs = (int32_t)sp.rhs_src.size();
+                {  // column of every scatter run
+                    std::vector<int32_t> col(ph.sc_ptr[ph.n]);
+                    for (int32_t g = 0; g < ph.n; g++)
+                        for (int32_t q = ph.sc_ptr[g]; q < ph.sc_ptr[g + 1]; q++) col[q] = g;
+                    P.o_sc_col = put(pool, col);
+                }
+                {  // step programs of the refactor and the solve
+                    SpProgram fac, sol;
+                    std::string why;
+                    const SpLayout L = sp_layout(ph);
+                    P.sp_ok = build_sp_programs(ph, fac, sol, &why) ? 1 : 0;
+                    P.t_base = L.t_base;
+                    P.od_base = L.od_base;
+                    P.nslots = L.nslots;
+                    if (P.sp_ok) {
+                        P.o_spf = (int64_t)prog.size();
+                        prog.insert(prog.end(), fac.words.begin(), fac.words.end());
+                        P.spf_chunks = fac.nsteps_pad / kSpChunk - 1;
+                        P.o_sps = (int64_t)prog.size();
+                        prog.insert(prog.end(), sol.words.begin(), sol.words.end());
+                        P.sps_chunks = sol.nsteps_pad / kSpChunk - 1;
+                    }
+                }
                 if (pool.size() > (size_t)INT32_MAX) throw std::runtime_error("pattern pool too large");
                 pid = (int)pats.size();
                 pats.push_back(P);
+                ph_nsc.push_back(ph.sc_ptr[ph.n]);
                 pat_id.emplace(std::move(key), pid);
             } else {
                 pid = it->second;
@@ -230,9 +214,11 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
             scr += P.n;
             I.o_rv = scr;
             scr += P.nrhs;
+            I.o_ss = iscr;
+            iscr += ph_nsc[pid];
             b->n_of[k] = P.n;
             b->nmax = std::max(b->nmax, P.n);
-            b->lumax = std::max(b->lumax, P.nl + P.nu + P.n);
+            b->lumax = std::max(b->lumax, P.nslots);
         }
     } catch (const std::exception& ex) {
         b->err = std::string("Invalid matrix: ") + ex.what();
@@ -249,7 +235,9 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     if ((e = b->d_pats.ensure(pats.size())) || (e = b->d_inst.ensure(ninst)) ||
         (e = b->d_order.ensure(ninst)) || (e = b->d_ip.ensure(pool.size())) ||
         (e = b->d_counter.ensure(1)) || (e = b->d_dev.ensure(ndev_total)) ||
-        (e = b->d_scr.ensure(scr)) || (e = b->d_res.ensure(ninst))) {
+        (e = b->d_scr.ensure(scr)) || (e = b->d_res.ensure(ninst)) ||
+        (e = b->d_iscr.ensure(std::max<int64_t>(iscr, 1))) ||
+        (e = b->d_prog.ensure(std::max<size_t>(prog.size(), 1)))) {
         bhip(b, e, "hipMalloc(batch)");
         return b;
     }
@@ -258,7 +246,8 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
         (e = hipMemcpy(b->d_order.p, order.data(), ninst * sizeof(int32_t), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(b->d_ip.p, pool.data(), pool.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(b->d_dev.p, dev, ndev_total * sizeof(rs_device), hipMemcpyHostToDevice)) ||
-        (e = hipMemset(b->d_scr.p, 0, scr * sizeof(double)))) {
+        (e = hipMemset(b->d_scr.p, 0, scr * sizeof(double))) ||
+        (e = hipMemcpy(b->d_prog.p, prog.data(), prog.size() * sizeof(uint64_t), hipMemcpyHostToDevice))) {
         bhip(b, e, "hipMemcpy(batch)");
         return b;
     }
@@ -298,6 +287,8 @@ void rs_batch_destroy(rs_batch* b) {
     b->d_dev.release();
     b->d_scr.release();
     b->d_res.release();
+    b->d_iscr.release();
+    b->d_prog.release();
     delete b;
 }
 
@@ -325,7 +316,7 @@ int32_t rs_batch_dc_op(rs_batch* b, const rs_newton_opts* o, void* stream) {
     if (e) return bhip(b, e, "hipMemset(counter)");
     const int ps = prof_begin(PK_BATCH, s);
     e = launch_batch_dcop(b->d_pats.p, b->d_inst.p, b->d_order.p, (int)b->ninst, b->d_counter.p,
-                          b->d_ip.p, b->d_dev.p, b->d_scr.p, b->d_res.p, bo, b->nmax,
+                          b->d_ip.p, b->d_prog.p, b->d_dev.p, b->d_scr.p, b->d_iscr.p, b->d_res.p, bo, b->nmax,
                           b->vals_lds ? b->lumax : 0, b->grid, s);
     prof_end(ps, s);
     if (e) return bhip(b, e, "k_batch_dcop");

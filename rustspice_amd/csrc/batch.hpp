@@ -33,13 +33,17 @@ struct BPat {
     // per-column update program: for each U(j,g), j ascending, every L(:,j)
     // entry as (j, row, index into L values)
     int32_t o_up_ptr, o_up_j, o_up_row, o_up_l;
-    int32_t nrhs, pad2;
+    int32_t nrhs, o_sc_col;
+    // step programs (sp.hpp): word offsets into the program pool, chunk counts
+    int64_t o_spf, o_sps;
+    int32_t spf_chunks, sps_chunks, sp_ok, t_base, od_base, nslots;
 };
 
 struct BInst {
     int32_t pat, ndev;
     int64_t o_dev;  // first device record
     int64_t o_sv, o_ax, o_lx, o_ux, o_ud, o_x, o_rv;
+    int64_t o_ss;  // scatter slots for the step-program refactor (int scratch)
 };
 
 struct BResult {

@@ -21,6 +21,7 @@
 
 #include "batch.hpp"
 #include "device_eval.hpp"
+#include "sp.hpp"
 
 namespace {
 
@@ -42,13 +43,90 @@ __device__ __forceinline__ void gsync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// ---- step programs (sp.hpp) --------------------------------------------------
+__device__ __forceinline__ void sp_fetch(uint64_t w, const double* V, const int* RPI, int t_base,
+                                         int& da, double& va, double& vb, double& vc) {
+    const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+    int d = (int)(lo & 0xFFFFu);
+    if (((hi >> 16) & 15u) == SP_ODSUB) d = t_base + RPI[d];
+    da = d;
+    va = V[d];
+    vb = V[lo >> 16];
+    vc = V[hi & 0xFFFFu];
+}
+
+__device__ __forceinline__ void sp_exec(uint64_t w, int da, double va, double vb, double vc,
+                                        double& reg, double* V) {
+    const uint32_t hi = (uint32_t)(w >> 32);
+    const uint32_t kind = (hi >> 16) & 15u, f = (hi >> 20) & 7u;
+    if (kind == SP_NOP) return;
+    const double a = (f & kSpFwdA) ? reg : va;
+    const double b = (f & kSpFwdB) ? reg : vb;
+    const double c = (f & kSpFwdC) ? reg : vc;
+    double r = a;
+    if (kind == SP_SUB || kind == SP_ODSUB || (kind == SP_UPD && c != 0.0)) r = a - b * c;
+    if (kind == SP_DIV) r = a / b;
+    if (kind == SP_DIAG && fabs(a) < 1e-14) r = a >= 0.0 ? 1e-14 : -1e-14;
+    if (kind == SP_DIAG1 && fabs(a) < 1e-300) r = 1e-14;
+    V[da] = r;
+    reg = r;
+}
+
+// Runs one program on the 64 lanes of this wavefront. Op words stream from
+// global memory one chunk ahead; the operands of step s+1 are read from LDS
+// before step s writes (the scheduling contract of sp.hpp).
+__device__ void sp_run(const uint64_t* __restrict__ prog, int nchunks, double* V, const int* RPI,
+                       int t_base, int lane) {
+    uint64_t q[kSpChunk];
+#pragma unroll
+    for (int k = 0; k < kSpChunk; ++k) q[k] = prog[k * kW + lane];
+    double reg = 0.0, va, vb, vc;
+    int da;
+    uint64_t w = q[0];
+    sp_fetch(w, V, RPI, t_base, da, va, vb, vc);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        uint64_t qn[kSpChunk];
+        const uint64_t* pn = prog + (size_t)(ch + 1) * kSpChunk * kW + lane;
+#pragma unroll
+        for (int k = 0; k < kSpChunk; ++k) qn[k] = pn[k * kW];
+#pragma unroll
+        for (int k = 0; k < kSpChunk; ++k) {
+            const uint64_t wn = k + 1 < kSpChunk ? q[k + 1] : qn[0];
+            int dn;
+            double na, nb, nc;
+            sp_fetch(wn, V, RPI, t_base, dn, na, nb, nc);
+            sp_exec(w, da, va, vb, vc, reg, V);
+            w = wn;
+            da = dn;
+            va = na;
+            vb = nb;
+            vc = nc;
+        }
+#pragma unroll
+        for (int k = 0; k < kSpChunk; ++k) q[k] = qn[k];
+    }
+}
+
+// Index of key in the ascending a[lo, hi), or -1.
+__device__ __forceinline__ int find_sorted(const int32_t* a, int lo, int hi, int key) {
+    const int end = hi;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < end && a[lo] == key) ? lo : -1;
+}
+
 __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats,
                                                    const BInst* __restrict__ insts,
                                                    const int32_t* __restrict__ order, int ninst,
                                                    int* __restrict__ counter,
                                                    const int32_t* __restrict__ ip,
+                                                   const uint64_t* __restrict__ prog,
                                                    const rs_device* __restrict__ devs,
                                                    double* __restrict__ scr,
+                                                   int32_t* __restrict__ iscr,
                                                    BResult* __restrict__ res, BOpts o, int nmax,
                                                    int lumax) {
     extern __shared__ double lds[];
@@ -114,6 +192,10 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         const int32_t* odax = ip + P.o_odax;
         const int32_t* odsrc = ip + P.o_odsrc;
         const int32_t* has_od = ip + P.o_blk_has_od;
+        const int32_t* sc_col = ip + P.o_sc_col;
+        int32_t* ss = iscr + I.o_ss;
+        const bool sp = lumax > 0 && P.sp_ok;  // step programs (values in LDS)
+        const int nsc = sc_ptr[n];
 
         for (int i = lane; i < n; i += kW) {
             W[i] = 0.0;
@@ -162,6 +244,21 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         XN[i] = acc;
                     }
                     // ---- factor (first: with pivot search; later: reuse) ----
+                    if (sp && factored) {  // refactor as a step program
+                        for (int i = lane; i < P.nl + P.nu; i += kW) VL[i] = 0.0;
+                        wsync();
+                        for (int s = lane; s < nsc; s += kW) {
+                            const int t = ss[s];
+                            if (t < 0) continue;
+                            double v = 0.0;
+                            const int src = sc_src[s], cnt = sc_cnt[s];
+                            for (int q = 0; q < cnt; ++q) v += ax[src + q];
+                            VL[t] = v;
+                        }
+                        wsync();
+                        sp_run(prog + P.o_spf, P.spf_chunks, VL, RPI, P.t_base, lane);
+                        wsync();
+                    } else
                     for (int g = 0; g < n; ++g) {
                         for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW) {
                             double v = 0.0;
@@ -244,8 +341,36 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         if (lumax) wsync();
                         else gsync();  // L/U values of this column are read by later columns
                     }
-                    if (!factored) r.full_factors++;
+                    if (!factored) {
+                        r.full_factors++;
+                        if (sp) {  // scatter slots under the pivots just found
+                            for (int s = lane; s < nsc; s += kW) {
+                                const int g = sc_col[s], pos = RPI[sc_pre[s]];
+                                const int ue = ucp[g + 1] - 1;
+                                int slot = -1;
+                                if (pos == g) {
+                                    slot = P.nl + ue;
+                                } else if (pos < g) {
+                                    const int q = find_sorted(uri, ucp[g], ue, pos);
+                                    slot = q < 0 ? -1 : P.nl + q;
+                                } else {
+                                    slot = find_sorted(lri, lcp[g], lcp[g + 1], pos);
+                                }
+                                ss[s] = slot;
+                            }
+                            gsync();
+                        }
+                    }
                     factored = true;
+                    if (sp) {  // ---- solve as a step program ----
+                        for (int e = lane; e < P.nslots - P.od_base; e += kW) VL[P.od_base + e] = ax[odax[e]];
+                        for (int i = lane; i < n; i += kW) VL[P.t_base + RPI[pin[i]]] = XN[i];
+                        wsync();
+                        sp_run(prog + P.o_sps, P.sps_chunks, VL, RPI, P.t_base, lane);
+                        wsync();
+                        for (int i = lane; i < n; i += kW) XN[i] = VL[P.t_base + pout[i]];
+                        wsync();
+                    } else {
                     // ---- solve, in W (all zero now; re-zeroed afterwards) ----
                     double* T = W;
                     for (int i = lane; i < n; i += kW) T[RPI[pin[i]]] = XN[i];
@@ -286,6 +411,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     for (int i = lane; i < n; i += kW) XN[i] = T[pout[i]];
                     wsync();
                     for (int i = lane; i < n; i += kW) W[i] = 0.0;
+                    }
                     // ---- convergence, damping, update (newton.rs:93-107) ----
                     const int nd = n < xl ? n : xl;
                     int ok = 1;
@@ -332,7 +458,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         if (lumax) {  // leave the factor in global scratch as well
             for (int l = lane; l < P.nl; l += kW) scr[I.o_lx + l] = lx[l];
             for (int u = lane; u < P.nu; u += kW) scr[I.o_ux + u] = ux[u];
-            for (int i = lane; i < n; i += kW) scr[I.o_ud + i] = ud[i];
+            for (int i = lane; i < n; i += kW) scr[I.o_ud + i] = ux[ucp[i + 1] - 1];
         }
         __syncthreads();
     }
@@ -347,11 +473,11 @@ size_t lds_bytes(int nmax, int lumax) {
 namespace rsgpu {
 
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
-                             int* counter, const int32_t* ip, const rs_device* devs, double* scr,
-                             BResult* res, const BOpts& o, int nmax, int lumax, int grid,
-                             hipStream_t s) {
+                             int* counter, const int32_t* ip, const uint64_t* prog,
+                             const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
+                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_dcop, dim3(grid), dim3(kW), lds_bytes(nmax, lumax), s, pats, insts,
-                       order, ninst, counter, ip, devs, scr, res, o, nmax, lumax);
+                       order, ninst, counter, ip, prog, devs, scr, iscr, res, o, nmax, lumax);
     return hipGetLastError();
 }
 

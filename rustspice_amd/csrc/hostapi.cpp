@@ -1,11 +1,16 @@
 // hostapi.cpp - C entry points for the host-side integer analysis (AMD, BTF,
 // L/U patterns) and the first pivoting factor, so they can be checked on a
 // machine without a GPU. They touch no device memory.
+#include <algorithm>
 #include <cstring>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
+#include "../../include/rsgpu.h"
 #include "../../include/rsgpu_host.h"
 #include "plan.hpp"
+#include "sp.hpp"
 #include "stamp.hpp"
 
 using namespace rsgpu;
@@ -151,6 +156,55 @@ int32_t rs_host_stamp_pattern(const rs_device* dev, int64_t ndev, int64_t node_c
     if (ap) std::memcpy(ap, p.ap.data(), p.ap.size() * sizeof(int64_t));
     if (ai && !p.ai.empty()) std::memcpy(ai, p.ai.data(), p.ai.size() * sizeof(int64_t));
     return 0;
+}
+
+int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const int64_t* ai,
+                            const double* ax0, const double* ax1, const double* rhs, double* x,
+                            int64_t* stats) {
+    rs_host_factor* h = nullptr;
+    try {
+        h = rs_host_factor_create(kind, n, ap, ai, ax0, 0.001, 0, 1);
+        PlanHost ph;
+        build_identity_plan(n, ap, ai, kind, ph);
+        SpProgram fac, sol;
+        std::string why;
+        if (!build_sp_programs(ph, fac, sol, &why)) {
+            delete h;
+            return RS_ERR_INVALID_MATRIX;
+        }
+        const SpLayout L = sp_layout(ph);
+        std::vector<int32_t> rpi(n);
+        for (int32_t b = 0; b < ph.nblk; b++) {
+            const int32_t bs = ph.blk_ptr[b], be = ph.blk_ptr[b + 1];
+            for (int32_t i = 0; i < be - bs; i++)
+                rpi[bs + i] = bs + (int32_t)h->f.num[b].row_perm_inv[i];
+        }
+        std::vector<int32_t> ss;
+        sp_scatter_slots(ph, rpi.data(), ss);
+        std::vector<double> v(std::max<int32_t>(L.nslots, 1), 0.0);
+        for (int32_t s = 0; s < (int32_t)ss.size(); s++) {
+            if (ss[s] < 0) continue;
+            double acc = 0.0;
+            for (int32_t q = 0; q < ph.sc_cnt[s]; q++) acc += ax1[ph.sc_src[s] + q];
+            v[ss[s]] = acc;
+        }
+        sp_emulate(fac, L, v.data(), rpi.data());
+        for (int64_t i = 0; i < n; i++) v[L.t_base + rpi[ph.perm_in[i]]] = rhs[i];
+        for (int32_t e = 0; e < ph.nod; e++) v[L.od_base + e] = ax1[ph.odax[e]];
+        sp_emulate(sol, L, v.data(), rpi.data());
+        for (int64_t i = 0; i < n; i++) x[i] = v[L.t_base + ph.perm_out[i]];
+        if (stats) {
+            stats[0] = fac.nsteps;
+            stats[1] = fac.nops;
+            stats[2] = sol.nsteps;
+            stats[3] = sol.nops;
+        }
+        delete h;
+        return RS_OK;
+    } catch (const std::exception&) {
+        delete h;
+        return RS_ERR_INVALID_MATRIX;
+    }
 }
 
 }  // extern "C"

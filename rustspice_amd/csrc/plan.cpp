@@ -333,4 +333,46 @@ void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
     d.o_blk_has_od = put(p.blk_has_od);
 }
 
+// Pattern-level analysis: symbolic factor with identity pivots so that the
+// plan's scatter targets, solve permutation and off-diagonal row keys are the
+// pre-pivot positions the kernel maps through its own RPI.
+void build_identity_plan(i64 n, const i64* ap, const i64* ai, int kind, PlanHost& ph) {
+    HostFactor f;
+    f.n = n;
+    if (kind == 0 && n >= 4) {
+        f.btf = btf_decompose(n, ap, ai);
+        f.has_btf = f.btf.num_blocks >= 2 && f.btf.max_block_size < n;
+    }
+    if (f.has_btf) {
+        f.sym.resize(f.btf.num_blocks);
+        for (i64 b = 0; b < f.btf.num_blocks; b++) {
+            i64 bs = f.btf.block_ptr[b], be = f.btf.block_ptr[b + 1], sz = be - bs;
+            std::vector<i64> bap(sz + 1, 0), bai;
+            for (i64 lc = 0; lc < sz; lc++) {
+                i64 gco = f.btf.col_perm[bs + lc];
+                for (i64 q = ap[gco]; q < ap[gco + 1]; q++) {
+                    i64 gro = ai[q];
+                    if (gro >= n) continue;
+                    i64 grn = f.btf.row_perm_inv[gro];
+                    if (grn >= bs && grn < be) bai.push_back(grn - bs);
+                }
+                bap[lc + 1] = (i64)bai.size();
+            }
+            block_symbolic(sz, bap.data(), bai.data(), false, f.sym[b]);
+        }
+    } else {
+        f.sym.resize(1);
+        block_symbolic(n, ap, ai, false, f.sym[0]);
+    }
+    f.num.resize(f.sym.size());
+    for (size_t b = 0; b < f.sym.size(); b++) {
+        i64 m = f.sym[b].n;
+        f.num[b].row_perm.resize(m);
+        f.num[b].row_perm_inv.resize(m);
+        for (i64 i = 0; i < m; i++) f.num[b].row_perm[i] = f.num[b].row_perm_inv[i] = i;
+    }
+    build_plan(f, ap, ai, ph);
+}
+
+
 }  // namespace rsgpu

@@ -66,6 +66,12 @@ void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p);
 void gather_values(const HostFactor& f, std::vector<double>& lx, std::vector<double>& ux,
                    std::vector<double>& ud);
 
+// Plan of a pattern factored with identity pivots (kind 0: NativeKLU with
+// BTF, 1: SparseLU): its scatter targets, solve permutation and off-diagonal
+// row keys are pre-pivot positions, to be mapped through the pivots found
+// later (row_perm_inv per block, as positions).
+void build_identity_plan(i64 n, const i64* ap, const i64* ai, int kind, PlanHost& ph);
+
 // Append one plan's int arrays to a pool and fill the int offsets of d.
 void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
 

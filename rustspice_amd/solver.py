@@ -202,3 +202,16 @@ def host_stamp_pattern(circuit, mode=0):
     L.rs_host_stamp_pattern(dev.ctypes.data_as(C.c_void_p), circuit.ndev, circuit.node_count, mode,
                             None, None, ap.ctypes.data_as(C.c_void_p), ai.ctypes.data_as(C.c_void_p))
     return ap, ai[: nnz.value]
+
+
+def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU):
+    """Host model of the GPU step programs: first factor on ax0 (pivots),
+    refactor on ax1, solve rhs. Returns (x, stats)."""
+    L = _lib.lib()
+    x = np.zeros(n)
+    st = np.zeros(4, np.int64)
+    rc = L.rs_host_sp_selftest(kind, n, _i64(ap), _i64(ai), _f64(ax0), _f64(ax1), _f64(rhs), x, st)
+    if rc:
+        raise SolverError(rc, "step program")
+    keys = ["refactor_steps", "refactor_ops", "solve_steps", "solve_ops"]
+    return x, dict(zip(keys, map(int, st)))
