@@ -171,6 +171,12 @@ def main():
     st = s.stats()
     ap_dc, ai_dc = g.pattern(E.MODE_DC)
     n_dc = len(ap_dc) - 1
+    # L/U sizes of the DC matrix from the host analysis (the device-resident
+    # DC OP leaves no factorization on the solver handle)
+    _, _, ax0, _, _ = g.stamp(np.zeros(circ.node_count), mode=E.MODE_DC)
+    ps = S.HostFactor(n_dc, ap_dc, ai_dc, ax0).plan_stats()
+    st.update(nnz_l=ps["nnz_l"], nnz_u=ps["nnz_u"], n_factor_levels=ps["factor_levels"],
+              n_lsolve_levels=ps["lsolve_levels"], n_usolve_levels=ps["usolve_levels"])
     b_ref, b_sol = _algorithmic_bytes(len(ai_dc), st["nnz_l"], st["nnz_u"], n_dc)
     kern = {}
     # k_batch_dcop runs dc_iters refactor+solve iterations per launch

@@ -175,6 +175,14 @@ int32_t rs_batch_dc_op(rs_batch *b, const rs_newton_opts *o, void *stream);
  * instances at x_out + x_ptr[k] (x_ptr has ninst + 1 entries). */
 int32_t rs_batch_results(rs_batch *b, rs_newton_result *res, double *x_out, int64_t *x_ptr);
 
+/* Per-phase shader-clock cycles summed over the wavefronts of later
+ * rs_batch_dc_op calls (0 device evaluation, 1 ax/RHS gathers, 2 RHS sums,
+ * 3 first factor, 4 refactor, 5 scatter-slot map, 6 solve, 7 Newton update
+ * and bookkeeping). on != 0 clears and enables; out has 8 entries. No
+ * reference counterpart: measurement only. */
+int32_t rs_batch_phase_profile(rs_batch *b, int32_t on);
+int32_t rs_batch_phase_cycles(rs_batch *b, int64_t *out);
+
 /* ---- measurement (no reference counterpart) -------------------------------
  * Kernel timing with HIP events recorded on each kernel's launch stream.
  * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly,

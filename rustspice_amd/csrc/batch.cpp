@@ -21,7 +21,8 @@ namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
                              int* counter, const int32_t* ip, const uint64_t* prog,
                              const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
-                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s);
+                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s,
+                             unsigned long long* phase_cyc);
 int batch_blocks_per_cu(int nmax, int lumax);
 size_t batch_lds_bytes(int nmax, int lumax);
 }  // namespace rsgpu
@@ -40,6 +41,8 @@ struct rs_batch {
     DevBuf<BInst> d_inst;
     DevBuf<int32_t> d_order, d_ip, d_iscr;
     DevBuf<uint64_t> d_prog;
+    DevBuf<unsigned long long> d_phase;  // per-phase cycle totals when profiling
+    bool phase_on = false;
     DevBuf<int> d_counter;
     DevBuf<rs_device> d_dev;
     DevBuf<double> d_scr;
@@ -58,6 +61,14 @@ int32_t bfail(rs_batch* b, int32_t code, const std::string& m) {
 int32_t bhip(rs_batch* b, hipError_t e, const char* what) {
     b->err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
     return RS_ERR_DEVICE;
+}
+
+int32_t put16(std::vector<int32_t>& pool, const std::vector<int32_t>& v) {  // 16-byte aligned
+    while (pool.size() % 4) pool.push_back(0);
+    int32_t off = (int32_t)pool.size();
+    pool.insert(pool.end(), v.begin(), v.end());
+    pool.push_back(0);
+    return off;
 }
 
 int32_t put(std::vector<int32_t>& pool, const std::vector<int32_t>& v) {
@@ -164,6 +175,36 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
                     P.o_up_l = put(pool, up_l);
                 }
                 P.nrhs = (int32_t)sp.rhs_src.size();
+                {  // gather records (see BPat)
+                    const int32_t nsc = ph.sc_ptr[ph.n];
+                    std::vector<int32_t> rec(4 * (size_t)std::max(nsc, 1), 0), lrun, rrec(4 * (size_t)ph.n, 0), lrow;
+                    for (int32_t q = 0; q < nsc; q++) {
+                        const int32_t cnt = ph.sc_cnt[q], src = ph.sc_src[q];
+                        rec[4 * q] = cnt;
+                        if (cnt <= 3) {
+                            for (int32_t k = 0; k < cnt; k++) rec[4 * q + 1 + k] = axs[src + k];
+                        } else {
+                            rec[4 * q + 1] = src;
+                            lrun.push_back(q);
+                        }
+                    }
+                    for (int32_t i = 0; i < ph.n; i++) {
+                        const int32_t k0 = sp.rhs_ptr[i], cnt = sp.rhs_ptr[i + 1] - k0;
+                        rrec[4 * i] = cnt;
+                        if (cnt <= 3) {
+                            for (int32_t k = 0; k < cnt; k++) rrec[4 * i + 1 + k] = sp.rhs_src[k0 + k];
+                        } else {
+                            rrec[4 * i + 1] = k0;
+                            lrow.push_back(i);
+                        }
+                    }
+                    P.o_screc = put16(pool, rec);
+                    P.o_lrun = put(pool, lrun);
+                    P.n_lrun = (int32_t)lrun.size();
+                    P.o_rhsrec = put16(pool, rrec);
+                    P.o_lrow = put(pool, lrow);
+                    P.n_lrow = (int32_t)lrow.size();
+                }
                 {  // column of every scatter run
                     std::vector<int32_t> col(ph.sc_ptr[ph.n]);
                     for (int32_t g = 0; g < ph.n; g++)
@@ -181,10 +222,10 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
                     if (P.sp_ok) {
                         P.o_spf = (int64_t)prog.size();
                         prog.insert(prog.end(), fac.words.begin(), fac.words.end());
-                        P.spf_chunks = fac.nsteps_pad / kSpChunk - 1;
+                        P.spf_chunks = fac.nsteps_pad / kSpChunk - 2;
                         P.o_sps = (int64_t)prog.size();
                         prog.insert(prog.end(), sol.words.begin(), sol.words.end());
-                        P.sps_chunks = sol.nsteps_pad / kSpChunk - 1;
+                        P.sps_chunks = sol.nsteps_pad / kSpChunk - 2;
                     }
                 }
                 if (pool.size() > (size_t)INT32_MAX) throw std::runtime_error("pattern pool too large");
@@ -289,6 +330,7 @@ void rs_batch_destroy(rs_batch* b) {
     b->d_res.release();
     b->d_iscr.release();
     b->d_prog.release();
+    b->d_phase.release();
     delete b;
 }
 
@@ -317,10 +359,25 @@ int32_t rs_batch_dc_op(rs_batch* b, const rs_newton_opts* o, void* stream) {
     const int ps = prof_begin(PK_BATCH, s);
     e = launch_batch_dcop(b->d_pats.p, b->d_inst.p, b->d_order.p, (int)b->ninst, b->d_counter.p,
                           b->d_ip.p, b->d_prog.p, b->d_dev.p, b->d_scr.p, b->d_iscr.p, b->d_res.p, bo, b->nmax,
-                          b->vals_lds ? b->lumax : 0, b->grid, s);
+                          b->vals_lds ? b->lumax : 0, b->grid, s, b->phase_on ? b->d_phase.p : nullptr);
     prof_end(ps, s);
     if (e) return bhip(b, e, "k_batch_dcop");
     return RS_OK;
+}
+
+int32_t rs_batch_phase_profile(rs_batch* b, int32_t on) {
+    b->phase_on = on != 0;
+    if (!b->phase_on) return RS_OK;
+    hipError_t e;
+    if ((e = b->d_phase.ensure(8)) || (e = hipMemset(b->d_phase.p, 0, 8 * sizeof(unsigned long long))))
+        return bhip(b, e, "phase profile");
+    return RS_OK;
+}
+
+int32_t rs_batch_phase_cycles(rs_batch* b, int64_t* out) {
+    if (!b->d_phase.p) return RS_ERR_DEVICE;
+    hipError_t e = hipMemcpy(out, b->d_phase.p, 8 * sizeof(int64_t), hipMemcpyDeviceToHost);
+    return e ? bhip(b, e, "phase cycles") : RS_OK;
 }
 
 int32_t rs_batch_results(rs_batch* b, rs_newton_result* res, double* x_out, int64_t* x_ptr) {

@@ -37,6 +37,10 @@ struct BPat {
     // step programs (sp.hpp): word offsets into the program pool, chunk counts
     int64_t o_spf, o_sps;
     int32_t spf_chunks, sps_chunks, sp_ok, t_base, od_base, nslots;
+    // gather records (int4, 16-byte aligned): per scatter run and per RHS row
+    // {count, slot0, slot1, slot2} when count <= 3, else {count, first, 0, 0}
+    // with the row/run listed for the cooperative long-sum path
+    int32_t o_screc, o_lrun, n_lrun, o_rhsrec, o_lrow, n_lrow;
 };
 
 struct BInst {

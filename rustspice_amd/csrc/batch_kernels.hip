@@ -28,6 +28,7 @@ namespace {
 using namespace rsgpu;
 
 constexpr int kW = 64;
+constexpr int kPhases = 8;  // eval, gather, rhs, factor, refactor, slots, solve, newton
 
 // LDS exchange between the lanes of this wavefront: LDS instructions of one
 // wavefront execute in program order, so a compiler fence is enough.
@@ -44,67 +45,114 @@ __device__ __forceinline__ void gsync() {
 }
 
 // ---- step programs (sp.hpp) --------------------------------------------------
-__device__ __forceinline__ void sp_fetch(uint64_t w, const double* V, const int* RPI, int t_base,
-                                         int& da, double& va, double& vb, double& vc) {
+// One step: every lane reads its operands, then writes its result. Kinds that
+// need more than a multiply-subtract take a wavefront-uniform branch, so steps
+// without them do not pay for the division or the clamps.
+__device__ __forceinline__ void sp_step(uint64_t w, double* V, const int* RPI, int t_base) {
     const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+    const uint32_t kind = hi >> 16;
     int d = (int)(lo & 0xFFFFu);
-    if (((hi >> 16) & 15u) == SP_ODSUB) d = t_base + RPI[d];
-    da = d;
-    va = V[d];
-    vb = V[lo >> 16];
-    vc = V[hi & 0xFFFFu];
-}
-
-__device__ __forceinline__ void sp_exec(uint64_t w, int da, double va, double vb, double vc,
-                                        double& reg, double* V) {
-    const uint32_t hi = (uint32_t)(w >> 32);
-    const uint32_t kind = (hi >> 16) & 15u, f = (hi >> 20) & 7u;
-    if (kind == SP_NOP) return;
-    const double a = (f & kSpFwdA) ? reg : va;
-    const double b = (f & kSpFwdB) ? reg : vb;
-    const double c = (f & kSpFwdC) ? reg : vc;
-    double r = a;
-    if (kind == SP_SUB || kind == SP_ODSUB || (kind == SP_UPD && c != 0.0)) r = a - b * c;
-    if (kind == SP_DIV) r = a / b;
-    if (kind == SP_DIAG && fabs(a) < 1e-14) r = a >= 0.0 ? 1e-14 : -1e-14;
-    if (kind == SP_DIAG1 && fabs(a) < 1e-300) r = 1e-14;
-    V[da] = r;
-    reg = r;
+    if (__ballot(kind == SP_ODSUB)) {
+        if (kind == SP_ODSUB) d = t_base + RPI[d];
+    }
+    const double a = V[d];
+    const double b = V[lo >> 16];
+    const double c = V[hi & 0xFFFFu];
+    double r = (kind == SP_UPD && c == 0.0) ? a : a - b * c;
+    if (__ballot(kind >= SP_DIV)) {
+        if (kind == SP_DIV) r = a / b;
+        if (kind == SP_DIAG) r = fabs(a) < 1e-14 ? (a >= 0.0 ? 1e-14 : -1e-14) : a;
+        if (kind == SP_DIAG1) r = fabs(a) < 1e-300 ? 1e-14 : a;
+    }
+    if (kind != SP_NOP) V[d] = r;
 }
 
 // Runs one program on the 64 lanes of this wavefront. Op words stream from
-// global memory one chunk ahead; the operands of step s+1 are read from LDS
-// before step s writes (the scheduling contract of sp.hpp).
-__device__ void sp_run(const uint64_t* __restrict__ prog, int nchunks, double* V, const int* RPI,
+// global memory through two register buffers: while one chunk of steps runs,
+// the next chunk's loads are in flight. Programs are padded to an even number
+// of chunks plus one chunk of no-ops, so the loop needs no bounds checks.
+__device__ __forceinline__ void sp_run(const uint64_t* __restrict__ prog, int nchunks, double* V, const int* RPI,
                        int t_base, int lane) {
-    uint64_t q[kSpChunk];
+    uint64_t qa[kSpChunk], qb[kSpChunk];
+    const uint64_t* p = prog + lane;
 #pragma unroll
-    for (int k = 0; k < kSpChunk; ++k) q[k] = prog[k * kW + lane];
-    double reg = 0.0, va, vb, vc;
-    int da;
-    uint64_t w = q[0];
-    sp_fetch(w, V, RPI, t_base, da, va, vb, vc);
-    for (int ch = 0; ch < nchunks; ++ch) {
-        uint64_t qn[kSpChunk];
-        const uint64_t* pn = prog + (size_t)(ch + 1) * kSpChunk * kW + lane;
+    for (int k = 0; k < kSpChunk; ++k) qa[k] = p[k * kW];
+    // Buffer loads are issued a whole chunk of steps before their first use.
+    // The next chunk's loads go out only after the current chunk's first step,
+    // so at the loop head the only loads in flight are the ones that step needs
+    // (otherwise the compiler's wait at the head also covers the new loads).
+    // The empty asm statements keep the loads where they are written.
+    for (int ch = 0; ch < nchunks; ch += 2) {
+        sp_step(qa[0], V, RPI, t_base);
+        asm volatile("" ::: "memory");
+        p += kSpChunk * kW;
 #pragma unroll
-        for (int k = 0; k < kSpChunk; ++k) qn[k] = pn[k * kW];
+        for (int k = 0; k < kSpChunk; ++k) qb[k] = p[k * kW];
+        asm volatile("" ::: "memory");
 #pragma unroll
-        for (int k = 0; k < kSpChunk; ++k) {
-            const uint64_t wn = k + 1 < kSpChunk ? q[k + 1] : qn[0];
-            int dn;
-            double na, nb, nc;
-            sp_fetch(wn, V, RPI, t_base, dn, na, nb, nc);
-            sp_exec(w, da, va, vb, vc, reg, V);
-            w = wn;
-            da = dn;
-            va = na;
-            vb = nb;
-            vc = nc;
+        for (int k = 1; k < kSpChunk; ++k) sp_step(qa[k], V, RPI, t_base);
+        asm volatile("" ::: "memory");
+        p += kSpChunk * kW;
+#pragma unroll
+        for (int k = 0; k < kSpChunk; ++k) qa[k] = p[k * kW];
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int k = 0; k < kSpChunk; ++k) sp_step(qb[k], V, RPI, t_base);
+        asm volatile("" ::: "memory");
+    }
+}
+
+// ---- gathers from the device slot values --------------------------------------
+__device__ __forceinline__ double sv_at(const double* sv, int i) { return i < 0 ? 1.0 : sv[i]; }
+
+// Ordered sum 0.0 + v0 + v1 + v2 of a short gather record (count <= 3);
+// records of long sums contribute nothing here.
+__device__ __forceinline__ double rec_sum(const int4 r, const double* sv) {
+    const int c = r.x <= 3 ? r.x : 0;
+    const double v0 = c > 0 ? sv_at(sv, r.y) : 0.0;
+    const double v1 = c > 1 ? sv_at(sv, r.z) : 0.0;
+    const double v2 = c > 2 ? sv_at(sv, r.w) : 0.0;
+    double acc = 0.0;
+    if (c > 0) acc += v0;
+    if (c > 1) acc += v1;
+    if (c > 2) acc += v2;
+    return acc;
+}
+
+constexpr int kStage = 4 * kW;  // doubles of LDS staging for long sums
+
+// Ordered sum 0.0 + sv[idx[0]] + ... + sv[idx[cnt-1]] of a long list: all
+// lanes load a chunk into LDS, lane 0 adds it in order. Valid on lane 0.
+__device__ double coop_sum(const int32_t* idx, int cnt, const double* sv, double* S, int lane) {
+    double acc = 0.0;
+    for (int b = 0; b < cnt; b += kStage) {
+        const int m = cnt - b < kStage ? cnt - b : kStage;
+        int id[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = j * kW + lane;
+            id[j] = q < m ? idx[b + q] : 0;
         }
 #pragma unroll
-        for (int k = 0; k < kSpChunk; ++k) q[k] = qn[k];
+        for (int j = 0; j < 4; ++j) {
+            const int q = j * kW + lane;
+            if (q < m) S[q] = sv_at(sv, id[j]);
+        }
+        wsync();
+        if (lane == 0) {  // reads batched 16 at a time; the adds stay in order
+            int q = 0;
+            for (; q + 16 <= m; q += 16) {
+                double t[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) t[j] = S[q + j];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc += t[j];
+            }
+            for (; q < m; ++q) acc += S[q];
+        }
+        wsync();
     }
+    return acc;
 }
 
 // Index of key in the ascending a[lo, hi), or -1.
@@ -128,41 +176,51 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                                    double* __restrict__ scr,
                                                    int32_t* __restrict__ iscr,
                                                    BResult* __restrict__ res, BOpts o, int nmax,
-                                                   int lumax) {
+                                                   int lumax,
+                                                   unsigned long long* __restrict__ phase_cyc) {
     extern __shared__ double lds[];
     double* W = lds;            // factor work vector; also the solve vector
     double* X = W + nmax;
     double* XN = X + nmax;      // rhs, then x_new
-    double* VL = XN + nmax;     // L/U/diag values when lumax > 0
+    double* S = XN + nmax;      // staging of long ordered sums
+    double* VL = S + kStage;    // L/U/diag values when lumax > 0
     int* RPI = reinterpret_cast<int*>(VL + lumax);
     int* RP = RPI + nmax;
     __shared__ int s_qi;
     __shared__ double s_dxn, s_fn;
     const int lane = threadIdx.x;
+    // optional per-phase cycle accounting (measurement only)
+    unsigned long long pc[kPhases] = {};
+    unsigned long long t_last = 0;
+#define PHASE(k)                                                  \
+    if (phase_cyc) {                                              \
+        const unsigned long long t_now = __builtin_amdgcn_s_memtime(); \
+        pc[k] += t_now - t_last;                                  \
+        t_last = t_now;                                           \
+    }
 
     for (;;) {
         if (lane == 0) s_qi = atomicAdd(counter, 1);
         __syncthreads();
         const int qi = s_qi;
         __syncthreads();
-        if (qi >= ninst) break;
+        if (qi >= ninst) {
+            if (phase_cyc && lane == 0)
+                for (int k = 0; k < kPhases; ++k) atomicAdd(phase_cyc + k, pc[k]);
+            break;
+        }
         const int ii = order[qi];
         const BInst I = insts[ii];
         const BPat P = pats[I.pat];
         const int n = P.n, nc = P.node_count;
         const rs_device* dev = devs + I.o_dev;
         double* sv = scr + I.o_sv;
-        double* ax = scr + I.o_ax;
-        double* rv = scr + I.o_rv;
         double* lx = lumax ? VL : scr + I.o_lx;
         double* ux = lumax ? VL + P.nl : scr + I.o_ux;
         double* ud = lumax ? VL + P.nl + P.nu : scr + I.o_ud;
         const int32_t* ax_src = ip + P.o_ax_src;
-        const int32_t* rhs_ptr = ip + P.o_rhs_ptr;
         const int32_t* rhs_src = ip + P.o_rhs_src;
         const int32_t* sc_ptr = ip + P.o_sc_ptr;
-        const int32_t* sc_src = ip + P.o_sc_src;
-        const int32_t* sc_cnt = ip + P.o_sc_cnt;
         const int32_t* sc_pre = ip + P.o_sc_pre;
         const int32_t* lcp = ip + P.o_lcp;
         const int32_t* lri = ip + P.o_lri;
@@ -193,6 +251,11 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         const int32_t* odsrc = ip + P.o_odsrc;
         const int32_t* has_od = ip + P.o_blk_has_od;
         const int32_t* sc_col = ip + P.o_sc_col;
+        const int4* screc = reinterpret_cast<const int4*>(ip + P.o_screc);
+        const int4* rhsrec = reinterpret_cast<const int4*>(ip + P.o_rhsrec);
+        const int32_t* lrun = ip + P.o_lrun;
+        const int32_t* lrow = ip + P.o_lrow;
+        double* rsum = scr + I.o_ax;  // run sums for the column-by-column factor
         int32_t* ss = iscr + I.o_ss;
         const bool sp = lumax > 0 && P.sp_ok;  // step programs (values in LDS)
         const int nsc = sc_ptr[n];
@@ -205,6 +268,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         bool factored = false;
         BResult fin{0, 0, 1, 0, 0.0};
         wsync();
+        if (phase_cyc) t_last = __builtin_amdgcn_s_memtime();
 
         const double gstart = fmax(o.gmin * 1e3, 1e-6);
         for (int gi = 0; gi <= o.gmin_steps && !fin.converged; ++gi) {
@@ -218,6 +282,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                 double prev = 1.7976931348623157e308;
                 for (int it = 0; it < o.max_iters; ++it) {
                     r.iterations = it + 1;
+                    PHASE(7)
                     // ---- stamp: evaluate, gather ax and the RHS contributions ----
                     StampArgs a{0.0, 0.0, gmin, scale, MODE_DC, 0};
                     for (int d = lane; d < I.ndev; d += kW) {
@@ -226,46 +291,81 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                     sv + (size_t)d * kSlotsPerDevice);
                     }
                     gsync();
-                    for (int q = lane; q < P.nnz; q += kW) {
-                        const int s = ax_src[q];
-                        ax[q] = s < 0 ? 1.0 : sv[s];
-                    }
-                    for (int k = lane; k < P.nrhs; k += kW) rv[k] = sv[rhs_src[k]];
+                    PHASE(0)
                     if (!factored)
                         for (int i = lane; i < n; i += kW) {
                             RPI[i] = i;
                             RP[i] = i;
                         }
-                    gsync();
-                    for (int i = lane; i < n; i += kW) {
-                        double acc = 0.0;  // 0.0 + contributions in insertion order
-                        const int k1 = rhs_ptr[i + 1];
-                        for (int k = rhs_ptr[i]; k < k1; ++k) acc += rv[k];
-                        XN[i] = acc;
+                    // RHS rows: short ones from their records, long ones cooperatively
+                    for (int base = 0; base < n; base += 4 * kW) {
+                        int4 rr[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = base + j * kW + lane;
+                            rr[j] = i < n ? rhsrec[i] : make_int4(0, 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = base + j * kW + lane;
+                            const double v = rec_sum(rr[j], sv);
+                            if (i < n && rr[j].x <= 3) XN[i] = v;
+                        }
                     }
-                    // ---- factor (first: with pivot search; later: reuse) ----
-                    if (sp && factored) {  // refactor as a step program
+                    for (int j = 0; j < P.n_lrow; ++j) {
+                        const int i = lrow[j];
+                        const int4 rr = rhsrec[i];
+                        const double v = coop_sum(rhs_src + rr.y, rr.x, sv, S, lane);
+                        if (lane == 0) XN[i] = v;
+                    }
+                    PHASE(1)
+                    // scatter-run sums: into the L/U slots for a step-program
+                    // refactor, else into rsum for the column-by-column factor
+                    const bool spr = sp && factored;
+                    if (spr) {
                         for (int i = lane; i < P.nl + P.nu; i += kW) VL[i] = 0.0;
                         wsync();
-                        for (int s = lane; s < nsc; s += kW) {
-                            const int t = ss[s];
-                            if (t < 0) continue;
-                            double v = 0.0;
-                            const int src = sc_src[s], cnt = sc_cnt[s];
-                            for (int q = 0; q < cnt; ++q) v += ax[src + q];
-                            VL[t] = v;
+                    }
+                    for (int base = 0; base < nsc; base += 4 * kW) {
+                        int4 rr[4];
+                        int tt[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int q = base + j * kW + lane;
+                            rr[j] = q < nsc ? screc[q] : make_int4(0, 0, 0, 0);
+                            tt[j] = (spr && q < nsc) ? ss[q] : -1;
                         }
-                        wsync();
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int q = base + j * kW + lane;
+                            const double v = rec_sum(rr[j], sv);
+                            if (q < nsc && rr[j].x <= 3) {
+                                if (!spr) rsum[q] = v;
+                                else if (tt[j] >= 0) VL[tt[j]] = v;
+                            }
+                        }
+                    }
+                    for (int j = 0; j < P.n_lrun; ++j) {
+                        const int q = lrun[j];
+                        const int4 rr = screc[q];
+                        const double v = coop_sum(ax_src + rr.y, rr.x, sv, S, lane);
+                        if (lane == 0) {
+                            if (!spr) rsum[q] = v;
+                            else if (ss[q] >= 0) VL[ss[q]] = v;
+                        }
+                    }
+                    if (spr) wsync();
+                    else gsync();
+                    PHASE(2)
+                    // ---- factor (first: with pivot search; later: reuse) ----
+                    if (spr) {  // refactor as a step program
                         sp_run(prog + P.o_spf, P.spf_chunks, VL, RPI, P.t_base, lane);
                         wsync();
+                        PHASE(4)
                     } else
                     for (int g = 0; g < n; ++g) {
-                        for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW) {
-                            double v = 0.0;
-                            const int src = sc_src[s], cnt = sc_cnt[s];
-                            for (int q = 0; q < cnt; ++q) v += ax[src + q];
-                            W[RPI[sc_pre[s]]] = v;
-                        }
+                        for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW)
+                            W[RPI[sc_pre[s]]] = rsum[s];
                         wsync();
                         const int p1 = up_ptr[g + 1];
                         for (int base = up_ptr[g]; base < p1; base += kW) {
@@ -342,6 +442,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         else gsync();  // L/U values of this column are read by later columns
                     }
                     if (!factored) {
+                        PHASE(3)
                         r.full_factors++;
                         if (sp) {  // scatter slots under the pivots just found
                             for (int s = lane; s < nsc; s += kW) {
@@ -359,11 +460,13 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                 ss[s] = slot;
                             }
                             gsync();
+                            PHASE(5)
                         }
                     }
                     factored = true;
                     if (sp) {  // ---- solve as a step program ----
-                        for (int e = lane; e < P.nslots - P.od_base; e += kW) VL[P.od_base + e] = ax[odax[e]];
+                        for (int e = lane; e < P.nslots - P.od_base; e += kW)
+                            VL[P.od_base + e] = sv_at(sv, ax_src[odax[e]]);
                         for (int i = lane; i < n; i += kW) VL[P.t_base + RPI[pin[i]]] = XN[i];
                         wsync();
                         sp_run(prog + P.o_sps, P.sps_chunks, VL, RPI, P.t_base, lane);
@@ -382,7 +485,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                 if (e0 == e1) continue;
                                 const int pos = RPI[p];
                                 double t = T[pos];
-                                for (int e = e0; e < e1; ++e) t = t - ax[odax[e]] * T[odsrc[e]];
+                                for (int e = e0; e < e1; ++e) t = t - sv_at(sv, ax_src[odax[e]]) * T[odsrc[e]];
                                 T[pos] = t;
                             }
                             wsync();
@@ -412,6 +515,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     wsync();
                     for (int i = lane; i < n; i += kW) W[i] = 0.0;
                     }
+                    PHASE(6)
                     // ---- convergence, damping, update (newton.rs:93-107) ----
                     const int nd = n < xl ? n : xl;
                     int ok = 1;
@@ -421,12 +525,32 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     }
                     const bool conv = __ballot(ok == 0) == 0ull;
                     if (lane == 0) {
+                        // two ordered sums (newton.rs:98-101), reads batched 8 at a time
                         double sd = 0.0, sf = 0.0;
-                        for (int i = 0; i < nd; ++i) {
-                            const double dxi = XN[i] - X[i];
-                            sd += dxi * dxi;
+                        int i = 0;
+                        for (; i + 8 <= n; i += 8) {
+                            double xn[8], xo[8];
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                xn[j] = XN[i + j];
+                                xo[j] = X[i + j];
+                            }
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                if (i + j < nd) {
+                                    const double dxi = xn[j] - xo[j];
+                                    sd += dxi * dxi;
+                                }
+                                sf += xn[j] * xn[j];
+                            }
                         }
-                        for (int i = 0; i < n; ++i) sf += XN[i] * XN[i];
+                        for (; i < n; ++i) {
+                            if (i < nd) {
+                                const double dxi = XN[i] - X[i];
+                                sd += dxi * dxi;
+                            }
+                            sf += XN[i] * XN[i];
+                        }
                         s_dxn = sqrt(sd);
                         s_fn = sqrt(sf);
                     }
@@ -452,6 +576,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                 fin = r;
             }
         }
+        PHASE(7)
         if (lane == 0) res[ii] = fin;
         if (fin.converged)
             for (int i = lane; i < n; i += kW) scr[I.o_x + i] = X[i];
@@ -465,7 +590,8 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
 }
 
 size_t lds_bytes(int nmax, int lumax) {
-    return (size_t)nmax * (3 * sizeof(double) + 2 * sizeof(int)) + (size_t)lumax * sizeof(double);
+    return (size_t)nmax * (3 * sizeof(double) + 2 * sizeof(int)) +
+           (size_t)(lumax + kStage) * sizeof(double);
 }
 
 }  // namespace
@@ -475,9 +601,11 @@ namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
                              int* counter, const int32_t* ip, const uint64_t* prog,
                              const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
-                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s) {
+                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s,
+                             unsigned long long* phase_cyc) {
     hipLaunchKernelGGL(k_batch_dcop, dim3(grid), dim3(kW), lds_bytes(nmax, lumax), s, pats, insts,
-                       order, ninst, counter, ip, prog, devs, scr, iscr, res, o, nmax, lumax);
+                       order, ninst, counter, ip, prog, devs, scr, iscr, res, o, nmax, lumax,
+                       phase_cyc);
     return hipGetLastError();
 }
 

@@ -78,88 +78,39 @@ void schedule(const std::vector<SeqOp>& ops, const std::vector<std::vector<int32
     for (int32_t i = m - 1; i >= 0; i--)
         for (int32_t s : succ[i]) prio[i] = std::max(prio[i], prio[s] + 1);
 
-    std::vector<int32_t> step(m, -1), lane(m, -1), npend(m), fwd(m, 0);
-    std::vector<int32_t> ready, placed;
+    std::vector<int32_t> npend(m), ready, placed, keep;
     for (int32_t i = 0; i < m; i++) {
         npend[i] = (int32_t)preds[i].size();
         if (!npend[i]) ready.push_back(i);
     }
-    std::vector<std::vector<std::pair<int32_t, int32_t>>> rows;  // per step: (lane, op)
+    std::vector<std::vector<int32_t>> rows;  // per step: ops, lane = position
     int32_t done = 0;
-    for (int32_t s = 0; done < m; s++) {
+    while (done < m) {
         std::sort(ready.begin(), ready.end(), [&](int32_t x, int32_t y) {
             return prio[x] != prio[y] ? prio[x] > prio[y] : x < y;
         });
-        bool used[kSpLanes] = {};
-        int nfree = kSpLanes;
-        placed.clear();
-        std::vector<int32_t> keep;
-        for (int32_t v : ready) {
-            if (!nfree) {
-                keep.push_back(v);
-                continue;
-            }
-            int forced = -1;
-            uint32_t f = 0;
-            bool ok = true;
-            for (auto& p : preds[v]) {
-                const int32_t d = s - step[p.op];
-                if (p.role == R_WAR) continue;  // reader ran at an earlier step
-                if (d >= 2) continue;
-                if (p.role & (R_BAR | R_WAR)) {
-                    ok = false;
-                    break;
-                }
-                const int l = lane[p.op];
-                if (forced >= 0 && forced != l) {
-                    ok = false;
-                    break;
-                }
-                forced = l;
-                if (p.role & R_A) f |= kSpFwdA;
-                if (p.role & R_B) f |= kSpFwdB;
-                if (p.role & R_C) f |= kSpFwdC;
-            }
-            int l = -1;
-            if (ok) {
-                if (forced >= 0) {
-                    if (!used[forced]) l = forced;
-                } else {
-                    for (int q = 0; q < kSpLanes; q++)
-                        if (!used[q]) {
-                            l = q;
-                            break;
-                        }
-                }
-            }
-            if (l < 0) {
-                keep.push_back(v);
-                continue;
-            }
-            used[l] = true;
-            nfree--;
-            step[v] = s;
-            lane[v] = l;
-            fwd[v] = (int32_t)f;
-            placed.push_back(v);
-        }
+        const size_t take = std::min<size_t>(ready.size(), kSpLanes);
+        placed.assign(ready.begin(), ready.begin() + take);
+        keep.assign(ready.begin() + take, ready.end());
         for (int32_t v : placed)
             for (int32_t w : succ[v])
                 if (--npend[w] == 0) keep.push_back(w);
         ready.swap(keep);
         done += (int32_t)placed.size();
-        rows.emplace_back();
-        for (int32_t v : placed) rows.back().push_back({lane[v], v});
+        rows.push_back(placed);
     }
     out.nsteps = (int32_t)rows.size();
     out.nops = m;
-    const int32_t padded = (out.nsteps + kSpChunk - 1) / kSpChunk * kSpChunk + kSpChunk;
+    // whole chunks, an even number of them, plus two chunks of no-ops that the
+    // kernel's double-buffered loop reads ahead
+    const int32_t chunks = ((out.nsteps + kSpChunk - 1) / kSpChunk + 1) / 2 * 2;
+    const int32_t padded = (chunks + 2) * kSpChunk;
     out.nsteps_pad = padded;
     out.words.assign((size_t)padded * kSpLanes, 0ull);
     for (int32_t s = 0; s < out.nsteps; s++)
-        for (auto& lv : rows[s]) {
-            const SeqOp& o = ops[lv.second];
-            out.words[(size_t)s * kSpLanes + lv.first] = sp_word(o.kind, o.d, o.b, o.c, (uint32_t)fwd[lv.second]);
+        for (size_t l = 0; l < rows[s].size(); l++) {
+            const SeqOp& o = ops[rows[s][l]];
+            out.words[(size_t)s * kSpLanes + l] = sp_word(o.kind, o.d, o.b, o.c);
         }
 }
 
@@ -283,60 +234,44 @@ void sp_scatter_slots(const PlanHost& p, const int32_t* rpi, std::vector<int32_t
 }
 
 void sp_emulate(const SpProgram& prog, const SpLayout& L, double* v, const int32_t* rpi) {
-    struct Lane {
-        double reg = 0.0, a = 0.0, b = 0.0, c = 0.0;
-        uint32_t da = 0;
-    } ln[kSpLanes];
-    auto fetch = [&](int s) {  // operand reads of step s
-        for (int l = 0; l < kSpLanes; l++) {
-            const uint64_t w = prog.words[(size_t)s * kSpLanes + l];
-            const uint32_t kind = (uint32_t)(w >> 48) & 15u;
-            const uint32_t d = (uint32_t)w & 0xFFFFu;
-            ln[l].da = kind == SP_ODSUB ? (uint32_t)(L.t_base + rpi[d]) : d;
-            ln[l].a = v[ln[l].da];
-            ln[l].b = v[(w >> 16) & 0xFFFFu];
-            ln[l].c = v[(w >> 32) & 0xFFFFu];
-        }
-    };
-    Lane nx[kSpLanes];
-    fetch(0);
+    uint32_t da[kSpLanes];
+    double a[kSpLanes], b[kSpLanes], c[kSpLanes];
     for (int s = 0; s < prog.nsteps; s++) {
-        Lane cur[kSpLanes];
-        for (int l = 0; l < kSpLanes; l++) cur[l] = ln[l];
-        fetch(s + 1);  // before the writes of step s
-        for (int l = 0; l < kSpLanes; l++) nx[l] = ln[l];
-        for (int l = 0; l < kSpLanes; l++) {
-            const uint64_t w = prog.words[(size_t)s * kSpLanes + l];
-            const uint32_t kind = (uint32_t)(w >> 48) & 15u, f = (uint32_t)(w >> 52) & 7u;
+        const uint64_t* w = &prog.words[(size_t)s * kSpLanes];
+        for (int l = 0; l < kSpLanes; l++) {  // all reads of the step
+            const uint32_t kind = (uint32_t)(w[l] >> 48) & 15u;
+            const uint32_t d = (uint32_t)w[l] & 0xFFFFu;
+            da[l] = kind == SP_ODSUB ? (uint32_t)(L.t_base + rpi[d]) : d;
+            a[l] = v[da[l]];
+            b[l] = v[(w[l] >> 16) & 0xFFFFu];
+            c[l] = v[(w[l] >> 32) & 0xFFFFu];
+        }
+        for (int l = 0; l < kSpLanes; l++) {  // then all writes
+            const uint32_t kind = (uint32_t)(w[l] >> 48) & 15u;
             if (kind == SP_NOP) continue;
-            const double a = (f & kSpFwdA) ? cur[l].reg : cur[l].a;
-            const double b = (f & kSpFwdB) ? cur[l].reg : cur[l].b;
-            const double c = (f & kSpFwdC) ? cur[l].reg : cur[l].c;
-            double r = a;
+            double r = a[l];
             switch (kind) {
                 case SP_UPD:
-                    if (c != 0.0) r = a - b * c;
+                    if (c[l] != 0.0) r = a[l] - b[l] * c[l];
                     break;
                 case SP_SUB:
                 case SP_ODSUB:
-                    r = a - b * c;
+                    r = a[l] - b[l] * c[l];
                     break;
                 case SP_DIV:
-                    r = a / b;
+                    r = a[l] / b[l];
                     break;
                 case SP_DIAG:
-                    if (std::fabs(a) < 1e-14) r = a >= 0.0 ? 1e-14 : -1e-14;
+                    if (std::fabs(a[l]) < 1e-14) r = a[l] >= 0.0 ? 1e-14 : -1e-14;
                     break;
                 case SP_DIAG1:
-                    if (std::fabs(a) < 1e-300) r = 1e-14;
+                    if (std::fabs(a[l]) < 1e-300) r = 1e-14;
                     break;
                 default:
                     break;
             }
-            v[cur[l].da] = r;
-            nx[l].reg = r;
+            v[da[l]] = r;
         }
-        for (int l = 0; l < kSpLanes; l++) ln[l] = nx[l];
     }
 }
 

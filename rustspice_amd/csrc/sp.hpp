@@ -15,11 +15,9 @@
 // bit-identical; only independent operations are reordered.
 //
 // Execution model (what the scheduler guarantees and the kernel relies on):
-// step s runs at most one op per lane; a lane's operands for step s+1 are read
-// from LDS before the writes of step s. So an op may read a slot written by
-// another lane at step <= s-2, or the value its own lane produced at step s-1
-// through a register (forwarding flags), never a slot written at s-1 by
-// another lane.
+// step s runs at most one op per lane; all lanes read their operands from LDS,
+// then all lanes write. The LDS operations of one wavefront execute in program
+// order, so an op may read any slot written at an earlier step, by any lane.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -33,19 +31,16 @@ enum SpKind : uint32_t {
     SP_NOP = 0,
     SP_UPD = 1,
     SP_SUB = 2,
-    SP_DIV = 3,
-    SP_DIAG = 4,
-    SP_DIAG1 = 5,
-    SP_ODSUB = 6,
+    SP_ODSUB = 3,
+    SP_DIV = 4,  // kinds >= SP_DIV need the slow path
+    SP_DIAG = 5,
+    SP_DIAG1 = 6,
 };
 constexpr int kSpLanes = 64;
 constexpr int kSpChunk = 16;  // steps per prefetched chunk in the kernel
-constexpr uint32_t kSpFwdA = 1, kSpFwdB = 2, kSpFwdC = 4;
-
-// op word: dst[0:16) b[16:32) c[32:48) kind[48:52) fwd[52:55)
-inline uint64_t sp_word(uint32_t kind, uint32_t d, uint32_t b, uint32_t c, uint32_t fwd) {
-    return (uint64_t)d | ((uint64_t)b << 16) | ((uint64_t)c << 32) | ((uint64_t)kind << 48) |
-           ((uint64_t)fwd << 52);
+// op word: dst[0:16) b[16:32) c[32:48) kind[48:52)
+inline uint64_t sp_word(uint32_t kind, uint32_t d, uint32_t b, uint32_t c) {
+    return (uint64_t)d | ((uint64_t)b << 16) | ((uint64_t)c << 32) | ((uint64_t)kind << 48);
 }
 
 // Value slots: lx [0,nl) | ux [nl,nl+nu) | T [t_base, t_base+n) | off-diagonal ax [od_base, +nod)
@@ -55,7 +50,7 @@ struct SpLayout {
 
 struct SpProgram {
     int32_t nsteps = 0;     // scheduled steps
-    int32_t nsteps_pad = 0; // rounded up to kSpChunk, plus one chunk of NOPs
+    int32_t nsteps_pad = 0; // an even number of whole chunks, plus two chunks of NOPs
     int64_t nops = 0;
     std::vector<uint64_t> words;  // nsteps_pad * 64, step-major
 };

@@ -84,6 +84,10 @@ def _bind():
     L.rs_batch_info.argtypes = [vp, vp]
     L.rs_batch_dc_op.restype = C.c_int32
     L.rs_batch_dc_op.argtypes = [vp, C.POINTER(NewtonOpts), vp]
+    L.rs_batch_phase_profile.restype = C.c_int32
+    L.rs_batch_phase_profile.argtypes = [vp, C.c_int32]
+    L.rs_batch_phase_cycles.restype = C.c_int32
+    L.rs_batch_phase_cycles.argtypes = [vp, vp]
     L.rs_batch_results.restype = C.c_int32
     L.rs_batch_results.argtypes = [vp, vp, vp, vp]
     L._engine_bound = True
@@ -226,6 +230,17 @@ class GpuBatch:
         self._L.rs_batch_info(self.h, _ptr(out))
         return dict(instances=int(out[0]), patterns=int(out[1]), nmax=int(out[2]), grid=int(out[3]),
                     lds_values=int(out[4]))
+
+    PHASES = ("eval", "gather", "rhs", "factor", "refactor", "slots", "solve", "newton")
+
+    def phase_profile(self, on=True):
+        """Per-phase shader-clock accounting of later dc_op calls (measurement)."""
+        self._check(self._L.rs_batch_phase_profile(self.h, int(on)))
+
+    def phase_cycles(self):
+        out = np.zeros(8, np.int64)
+        self._check(self._L.rs_batch_phase_cycles(self.h, _ptr(out)))
+        return dict(zip(self.PHASES, map(int, out)))
 
     def dc_op(self, opts=None, stream=None):
         """Enqueues the batched solve (asynchronous on `stream`)."""
