@@ -45,6 +45,18 @@ def _algorithmic_bytes(nnz_a, nnz_l, nnz_u, n):
     return refactor, solve
 
 
+def _measured_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (scaled by a plain copy's counters), or None."""
+    path = os.path.join(ROOT, "profiles", "k_batch_dcop_chain500_traffic.md")
+    if kernel != "k_batch_dcop" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(circ, tran_args, budget_s, gpu_res):
     """Oracle (C restatement) timed on one host core: same circuit, same
     reference semantics. Returns the baseline dict and a parity verdict."""
@@ -132,7 +144,19 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--loop-sample", type=int, default=16)
     ap.add_argument("--host-steps", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_mib:
+        import torch
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the solver runs one matrix per device; see DESIGN.md for sharding)", file=sys.stderr)
@@ -209,7 +233,7 @@ def main():
                                         hres[-1]["dc_iters"] == res[-1]["dc_iters"] and
                                         hres[-1]["factor_calls"] == res[-1]["factor_calls"]))
     roof = dict(bound="hbm", achieved=kern[dom]["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=kern[dom]["gbs"] / HBM_PEAK_GBS, traffic=None, kernel=kname,
+                frac=kern[dom]["gbs"] / HBM_PEAK_GBS, traffic=_measured_traffic(kname), kernel=kname,
                 bytes_per_launch=kern[dom]["bytes"], avg_launch_us=kern[dom]["avg_us"])
 
     line = dict(
