@@ -85,6 +85,7 @@ def lib():
         L.orc_klu_solve.argtypes = [vp, C.c_int64, f64p]
         L.orc_klu_reset_pattern.argtypes = [vp]
         L.orc_klu_set_config.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.orc_klu_set_sparselu.argtypes = [vp]
         L.orc_klu_stats.argtypes = [vp, i64p, C.POINTER(C.c_double)]
         L.orc_klu_block_n.restype = C.c_int64
         L.orc_klu_block_n.argtypes = [vp, C.c_int64]
@@ -153,6 +154,10 @@ class NativeKlu:
 
     def set_config(self, pivot_tol=0.001, ordering=0, use_btf=True):
         lib().orc_klu_set_config(self.h, pivot_tol, ordering, int(use_btf))
+
+    def set_sparselu(self):
+        """SparseLuSolver semantics (sparse_lu.rs): no pivoting, no BTF."""
+        lib().orc_klu_set_sparselu(self.h)
 
     def prepare(self, n):
         lib().orc_klu_prepare(self.h, n)

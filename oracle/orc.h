@@ -47,6 +47,7 @@ int orc_klu_factor(orc_klu *s, int64_t ap_len, const int64_t *ap, int64_t ai_len
 int orc_klu_solve(orc_klu *s, int64_t n, double *rhs);
 void orc_klu_reset_pattern(orc_klu *s);
 void orc_klu_set_config(orc_klu *s, double pivot_tol, int ordering, int use_btf);
+void orc_klu_set_sparselu(orc_klu *s); /* sparse_lu.rs semantics */
 /* stats: factor_count, refactor_count, nnz_l, nnz_u, nblocks, noffdiag */
 void orc_klu_stats(const orc_klu *s, int64_t out[6], double *rcond);
 /* symbolic/numeric of block b (for bit-exact comparisons) */

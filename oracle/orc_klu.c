@@ -34,6 +34,7 @@ struct orc_klu {
     int ordering, use_btf;
     int64_t btf_min_blocks;
     int has_btf;
+    int sparselu; /* SparseLuSolver semantics: no pivoting, no BTF, no singleton rule */
     orc_btf btf;
     int64_t nblk;
     symf *sym;
@@ -75,6 +76,13 @@ orc_klu *orc_klu_new(int64_t n) {
     s->state = ST_EMPTY;
     s->work = (double *)orc_xcalloc((size_t)(n ? n : 1), sizeof(double));
     return s;
+}
+
+/* SparseLuSolver (sparse_lu.rs:302-618) is this solver with pivot_tol 0, no
+ * BTF and the plain 1e-14 clamp also for a 1x1 matrix (:493-499; it has no
+ * singleton fast path). */
+void orc_klu_set_sparselu(orc_klu *s) {
+    s->pivot_tol = 0.0; s->ordering = 0; s->use_btf = 0; s->sparselu = 1;
 }
 
 void orc_klu_set_config(orc_klu *s, double pivot_tol, int ordering, int use_btf) {
@@ -338,7 +346,7 @@ static void factor_block(const orc_klu *s, const symf *sym, const int64_t *ap, c
         out->row_perm = orc_xmalloc(8); out->row_perm_inv = orc_xmalloc(8);
         return;
     }
-    if (n == 1) {
+    if (n == 1 && !s->sparselu) {
         int64_t oc = sym->col_perm_inv[0];
         double diag = 0.0;
         for (int64_t idx = ap[oc]; idx < ap[oc + 1]; idx++) {

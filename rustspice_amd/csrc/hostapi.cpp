@@ -85,7 +85,8 @@ rs_host_factor* rs_host_factor_create(int32_t kind, int64_t n, const int64_t* ap
         f.sym.resize(1);
         f.num.resize(1);
         block_symbolic(n, ap, ai, natural, f.sym[0]);
-        factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0]);
+        f.sparselu = sparselu;
+        factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0], f.sparselu);
     }
     build_plan(f, ap, ai, h->p);
     return h;

@@ -57,7 +57,7 @@ void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p) 
     for (i64 b = 0; b < nb; b++) {
         const BlockSymbolic& s = f.sym[b];
         const i64 bs = bstart[b];
-        const bool singleton = (s.n == 1);
+        const bool singleton = (s.n == 1) && !f.sparselu;
         for (i64 k = 0; k < s.n; k++) {
             i64 g = bs + k;
             if (singleton) {
@@ -339,6 +339,7 @@ void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
 void build_identity_plan(i64 n, const i64* ap, const i64* ai, int kind, PlanHost& ph) {
     HostFactor f;
     f.n = n;
+    f.sparselu = kind == 1;
     if (kind == 0 && n >= 4) {
         f.btf = btf_decompose(n, ap, ai);
         f.has_btf = f.btf.num_blocks >= 2 && f.btf.max_block_size < n;

@@ -38,6 +38,7 @@ enum { COL_SINGLETON = 1 };
 struct HostFactor {
     i64 n = 0;
     bool has_btf = false;
+    bool sparselu = false;  // SparseLuSolver: no singleton rule for a 1x1 matrix
     Btf btf;
     std::vector<BlockSymbolic> sym;
     std::vector<BlockNumeric> num;

@@ -210,7 +210,8 @@ int32_t host_full_factor(rs_solver* h, const i64* ap, const i64* ai, const doubl
             factor_block_host(f.sym[b], bap.data(), bai.data(), bax.data(), nullptr, false, tol, f.num[b]);
         }
     } else {
-        factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0]);
+        f.sparselu = h->kind == RS_SOLVER_SPARSELU;
+        factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0], f.sparselu);
     }
     try {
         build_plan(f, ap, ai, h->ph);

@@ -366,11 +366,11 @@ void block_symbolic(i64 n, const i64* ap, const i64* ai, bool natural, BlockSymb
 // ---------------------------------------------------------------------------
 void factor_block_host(const BlockSymbolic& s, const i64* ap, const i64* ai, const double* ax,
                        const BlockNumeric* old, bool refactor, double pivot_tol,
-                       BlockNumeric& out) {
+                       BlockNumeric& out, bool sparselu) {
     const i64 n = s.n;
     out = BlockNumeric();
     if (n == 0) return;
-    if (n == 1) {
+    if (n == 1 && !sparselu) {
         i64 oc = s.col_perm_inv[0];
         double diag = 0.0;
         for (i64 q = ap[oc]; q < ap[oc + 1]; q++) {

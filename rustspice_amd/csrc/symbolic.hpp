@@ -41,9 +41,10 @@ struct BlockNumeric {
     std::vector<i64> row_perm, row_perm_inv;
 };
 // Reference factor_block (native_klu.rs:1194-1382) on one block given as CSC.
-// refactor=true reuses old's pivots (no pivot search).
+// refactor=true reuses old's pivots (no pivot search). sparselu: SparseLuSolver
+// semantics for a 1x1 matrix (plain 1e-14 clamp, no singleton rule).
 void factor_block_host(const BlockSymbolic& s, const i64* ap, const i64* ai, const double* ax,
                        const BlockNumeric* old, bool refactor, double pivot_tol,
-                       BlockNumeric& out);
+                       BlockNumeric& out, bool sparselu = false);
 
 }  // namespace rsgpu

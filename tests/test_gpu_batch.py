@@ -40,7 +40,7 @@ def test_batch_matches_oracle_per_instance(gpu, kind):
     for c, (xg, rg) in zip(circs, out):
         o = O.NativeKlu(c.node_count)
         if kind == S.SPARSELU:
-            o.set_config(0.0, 0, False)
+            o.set_sparselu()
         xo, ro = O.dc_op(c, solver=o)
         assert (rg.converged, rg.iterations, rg.reason) == (ro.converged, ro.iterations, ro.reason), c.name
         assert same_bits([rg.final_norm], [ro.final_norm]), c.name

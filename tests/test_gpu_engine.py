@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def _oracle_solver(n, kind):
     o = O.NativeKlu(n)
     if kind == S.SPARSELU:
-        o.set_config(0.0, 0, False)
+        o.set_sparselu()
     return o
 
 

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 def _oracle(n, kind):
     o = O.NativeKlu(n)
     if kind == S.SPARSELU:
-        o.set_config(0.0, 0, False)
+        o.set_sparselu()
     return o
 
 
@@ -164,3 +164,16 @@ def test_gpu_factor_values_match_oracle_refactor(gpu):
     o.factor(ap, ai, ax2)
     b = o.block(0)
     assert same_bits(lx, b["l_values"]) and same_bits(ux, b["u_values"]) and same_bits(ud, b["u_diag"])
+
+
+def test_sparselu_one_by_one_clamp(gpu):
+    """SparseLU semantics for a 1x1 matrix: the plain 1e-14 clamp
+    (sparse_lu.rs:493-499), not NativeKLU's singleton rule."""
+    ap, ai = np.array([0, 1], np.int64), np.array([0], np.int64)
+    for kind, want in ((S.SPARSELU, 1.0 / 1e-14), (S.NATIVEKLU, 1.0 / 1e-20)):
+        s = S.GpuSolver(1, kind)
+        s.factor(ap, ai, np.array([1e-20]))
+        s.factor(ap, ai, np.array([1e-20]))  # second call: GPU refactor
+        x = np.array([1.0])
+        s.solve(x)
+        assert x[0] == want, (kind, x[0])

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 def _cmp_factor(ap, ai, ax, n, kind):
     o = O.NativeKlu(n)
     if kind == S.SPARSELU:
-        o.set_config(0.0, 0, False)
+        o.set_sparselu()
     assert o.factor(ap, ai, ax) == 0
     h = S.HostFactor(n, ap, ai, ax, kind=kind)
     assert h.nblocks() == o.stats()["nblocks"]

@@ -474,7 +474,7 @@ def test_divider_dc_op():  # dc_basic_end_to_end.rs:9-52: V(in)=1, V(out)=2/3
     c = b.build(3, "divider")
     # SparseLU: a linear circuit converges on the 2nd iteration (Q4)
     s = O.NativeKlu(4)
-    s.set_config(0.0, 0, False)
+    s.set_sparselu()
     x, res = O.dc_op(c, solver=s)
     assert res.converged and res.iterations == 2
     assert abs(x[1] - 1.0) < 1e-9 and abs(x[2] - 2.0 / 3.0) < 1e-9
@@ -503,3 +503,23 @@ def test_chain_dc_op_hits_max_iters():
     solver): every Newton iteration is still a full refactor+solve."""
     x, res = O.dc_op(NL.inverter_chain(20, vin_pulse=False))
     assert not res.converged and res.iterations == 100 and res.reason == 1
+
+
+def test_sparselu_has_no_singleton_rule():
+    """sparse_lu.rs:493-499 clamps |d| < 1e-14 to +-1e-14 for every pivot, a
+    1x1 matrix included; NativeKLU's 1x1 fast path (native_klu.rs:1215-1237)
+    only replaces |d| < 1e-300."""
+    ap, ai = np.array([0, 1], np.int64), np.array([0], np.int64)
+    for d, klu_x, slu_x in ((1e-20, 1.0 / 1e-20, 1.0 / 1e-14), (-1e-20, 1.0 / -1e-20, 1.0 / -1e-14),
+                            (0.0, 1.0 / 1e-14, 1.0 / 1e-14), (-1e-310, 1.0 / 1e-14, 1.0 / -1e-14)):
+        k = O.NativeKlu(1)
+        k.factor(ap, ai, np.array([d]))
+        r = np.array([1.0])
+        k.solve(r)
+        assert r[0] == klu_x, (d, r[0])
+        s = O.NativeKlu(1)
+        s.set_sparselu()
+        s.factor(ap, ai, np.array([d]))
+        r = np.array([1.0])
+        s.solve(r)
+        assert r[0] == slu_x, (d, r[0])

@@ -18,7 +18,7 @@ from tests.helpers import random_csc, same_bits
 def _oracle(ap, ai, ax0, ax1, rhs, n, kind):
     o = O.NativeKlu(n)
     if kind == S.SPARSELU:
-        o.set_config(0.0, 0, False)
+        o.set_sparselu()
     o.factor(ap, ai, ax0)
     o.factor(ap, ai, ax1)
     r = rhs.copy()
