@@ -57,6 +57,12 @@ class TState(C.Structure):
                 ("method", C.c_int)]
 
 
+class BbdDecomp(C.Structure):
+    _fields_ = [("n", C.c_int64), ("num_blocks", C.c_int64), ("border_start", C.c_int64),
+                ("border_size", C.c_int64), ("perm", C.POINTER(C.c_int64)),
+                ("inv_perm", C.POINTER(C.c_int64)), ("block_ptr", C.POINTER(C.c_int64))]
+
+
 class Btf(C.Structure):
     _fields_ = [("n", C.c_int64), ("row_perm", C.POINTER(C.c_int64)),
                 ("col_perm", C.POINTER(C.c_int64)), ("row_perm_inv", C.POINTER(C.c_int64)),
@@ -108,6 +114,22 @@ def lib():
                                      C.c_int64, vp, f64p, C.c_int64, C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32)]
         L.orc_lte_milne.argtypes = [f64p, f64p, C.c_int64, C.c_double, C.c_double, f64p]
+        L.orc_bbd_partition.restype = C.c_int64
+        L.orc_bbd_partition.argtypes = [C.c_int64, i64p, i64p, C.c_int64, i64p]
+        L.orc_bbd_decompose.argtypes = [C.c_int64, i64p, i64p, C.c_int64, C.POINTER(BbdDecomp)]
+        L.orc_bbd_decomp_free.argtypes = [C.POINTER(BbdDecomp)]
+        L.orc_bbd_new.restype = vp
+        L.orc_bbd_new.argtypes = [C.c_int64, C.c_int64]
+        L.orc_bbd_free.argtypes = [vp]
+        L.orc_bbd_prepare.argtypes = [vp, C.c_int64]
+        L.orc_bbd_reset_pattern.argtypes = [vp]
+        L.orc_bbd_analyze.argtypes = [vp, C.c_int64, i64p, C.c_int64, i64p]
+        L.orc_bbd_factor.argtypes = [vp, C.c_int64, i64p, C.c_int64, i64p, f64p]
+        L.orc_bbd_solve.argtypes = [vp, C.c_int64, f64p]
+        L.orc_bbd_info.argtypes = [vp, i64p]
+        L.orc_dense_build.argtypes = [C.c_int64, C.c_int64, i64p, i64p, f64p, f64p]
+        L.orc_dense_factor.argtypes = [C.c_int64, f64p, i64p, C.c_int]
+        L.orc_dense_solve.argtypes = [C.c_int64, f64p, i64p, f64p, C.c_int]
         _LIB = L
     return _LIB
 
@@ -214,6 +236,78 @@ class NativeKlu:
         if nb == 0:
             return None
         return dict(row_perm=rp, col_perm=cp, block_ptr=bp[: nb + 1], num_blocks=nb)
+
+
+def bbd_partition(n, ap, ai, num_blocks):
+    """GreedyBisectionPartitioner::partition (bbd.rs:103-171): (assignment
+    with -1 for border nodes, block count)."""
+    a = np.zeros(max(n, 1), np.int64)
+    nb = lib().orc_bbd_partition(n, _i64(ap), _i64(ai), num_blocks, a)
+    return a[:n], int(nb)
+
+
+def bbd_decompose(n, ap, ai, num_blocks):
+    """bbd_decompose (bbd.rs:355-485) as a dict."""
+    d = BbdDecomp()
+    lib().orc_bbd_decompose(n, _i64(ap), _i64(ai), num_blocks, C.byref(d))
+    nb = d.num_blocks
+    out = dict(num_blocks=int(nb), border_start=int(d.border_start), border_size=int(d.border_size),
+               perm=np.array(d.perm[:n], np.int64), inv_perm=np.array(d.inv_perm[:n], np.int64),
+               block_ptr=np.array(d.block_ptr[: nb + 1], np.int64))
+    lib().orc_bbd_decomp_free(C.byref(d))
+    return out
+
+
+class Bbd:
+    """Oracle BbdSolver (bbd_solver.rs) with GreedyBisectionPartitioner."""
+
+    def __init__(self, n, num_blocks=2):
+        self.h = lib().orc_bbd_new(n, num_blocks)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_bbd_free(self.h)
+            self.h = None
+
+    def prepare(self, n):
+        lib().orc_bbd_prepare(self.h, n)
+
+    def analyze(self, ap, ai):
+        ap, ai = _i64(ap), _i64(ai)
+        return lib().orc_bbd_analyze(self.h, len(ap), ap, len(ai), ai)
+
+    def factor(self, ap, ai, ax):
+        ap, ai = _i64(ap), _i64(ai)
+        return lib().orc_bbd_factor(self.h, len(ap), ap, len(ai), ai, _f64(ax))
+
+    def solve(self, rhs):
+        assert rhs.dtype == np.float64 and rhs.flags.c_contiguous
+        return lib().orc_bbd_solve(self.h, len(rhs), rhs)
+
+    def reset_pattern(self):
+        lib().orc_bbd_reset_pattern(self.h)
+
+    def info(self):
+        out = np.zeros(4, np.int64)
+        lib().orc_bbd_info(self.h, out)
+        return dict(blocks=int(out[0]), border=int(out[1]), schur=int(out[2]),
+                    factor_count=int(out[3]))
+
+
+def dense_factor_solve(n, ap, ai, ax, rhs, schur=False):
+    """DenseSolver (solver.rs:595-724) factor + solve, or DenseSchurSolver's
+    checks with schur=True. Returns (status, x)."""
+    lu = np.zeros(max(n * n, 1))
+    piv = np.zeros(max(n, 1), np.int64)
+    L = lib()
+    ap = _i64(ap)
+    rc = L.orc_dense_build(n, len(ap), ap, _i64(ai), _f64(ax), lu)
+    if rc == 0:
+        rc = L.orc_dense_factor(n, lu, piv, int(schur))
+    x = np.array(rhs, np.float64)
+    if rc == 0:
+        rc = L.orc_dense_solve(n, lu, piv, x, int(schur))
+    return rc, x
 
 
 def _devptr(circ):

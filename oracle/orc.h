@@ -61,6 +61,36 @@ void orc_klu_block_get(const orc_klu *s, int64_t b, int64_t *col_perm, int64_t *
 int orc_klu_btf_get(const orc_klu *s, int64_t *row_perm, int64_t *col_perm,
                     int64_t *block_ptr);
 
+/* ---- BBD (bbd.rs), BbdSolver (bbd_solver.rs), DenseSolver (solver.rs) -- */
+#define ORC_DENSE_SCHUR_MAX 64 /* DENSE_SCHUR_THRESHOLD, bbd_solver.rs:85 */
+typedef struct orc_bbd_decomp {
+    int64_t n, num_blocks, border_start, border_size;
+    int64_t *perm, *inv_perm, *block_ptr; /* perm[new] = old */
+} orc_bbd_decomp;
+/* assignment: block id, or -1 for border; returns the block count */
+int64_t orc_bbd_partition(int64_t n, const int64_t *ap, const int64_t *ai, int64_t num_blocks,
+                          int64_t *assign);
+void orc_bbd_decompose(int64_t n, const int64_t *ap, const int64_t *ai, int64_t num_blocks,
+                       orc_bbd_decomp *d);
+void orc_bbd_decomp_free(orc_bbd_decomp *d);
+typedef struct orc_bbd orc_bbd;
+orc_bbd *orc_bbd_new(int64_t n, int64_t num_blocks);
+void orc_bbd_free(orc_bbd *s);
+void orc_bbd_prepare(orc_bbd *s, int64_t n);
+void orc_bbd_reset_pattern(orc_bbd *s);
+int orc_bbd_analyze(orc_bbd *s, int64_t apl, const int64_t *ap, int64_t ail, const int64_t *ai);
+int orc_bbd_factor(orc_bbd *s, int64_t apl, const int64_t *ap, int64_t ail, const int64_t *ai,
+                   const double *ax);
+int orc_bbd_solve(orc_bbd *s, int64_t n, double *rhs);
+int orc_bbd_is_bbd(const orc_bbd *s);
+/* out[4] = {blocks, border size, schur kind (1 dense, 0 sparse, -1 none), factor count} */
+int orc_bbd_info(const orc_bbd *s, int64_t *out);
+/* dense LU with partial pivoting; schur != 0: DenseSchurSolver checks (1e-30) */
+int orc_dense_build(int64_t n, int64_t apl, const int64_t *ap, const int64_t *ai, const double *ax,
+                    double *lu);
+int orc_dense_factor(int64_t n, double *lu, int64_t *piv, int schur);
+int orc_dense_solve(int64_t n, const double *lu, const int64_t *piv, double *rhs, int schur);
+
 /* ---- MNA assembly + stamps (mna.rs, stamp.rs, bsim/evaluate.rs) -------- */
 enum { ORC_MODE_DC = 0, ORC_MODE_TRAN = 1 };
 enum { ORC_BE = 0, ORC_TRAP = 1 };

@@ -1,0 +1,200 @@
+"""Pins of the oracle's BBD restatement (bbd.rs, bbd_solver.rs) and of its
+DenseSolver (solver.rs:595-724).
+
+Known answers are the reference's own tests (bbd.rs:500-697,
+bbd_solver.rs:1094-1416); the decomposition of the coupling case is derived
+by hand below; solutions are checked against numpy/scipy solves of the same
+matrices; the ground-node collapse (Q7) is checked on an MNA matrix.
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from oracle import oracle as O
+from rustspice_amd import netlists as NL
+from tests.helpers import csc_dense, same_bits
+
+
+def _residual(n, ap, ai, ax, x, b):
+    A = csc_dense(ap, ai, ax, n)
+    r = b - A @ x
+    return np.linalg.norm(r) / max(np.linalg.norm(b), 1e-300)
+
+
+def _bbd_solve(n, ap, ai, ax, b, nb=2):
+    s = O.Bbd(n, nb)
+    s.prepare(n)
+    assert s.analyze(ap, ai) == 0
+    assert s.factor(ap, ai, ax) == 0
+    x = np.array(b, np.float64)
+    assert s.solve(x) == 0
+    return x, s
+
+
+# ---- bbd.rs tests --------------------------------------------------------
+def test_partition_block_diagonal():  # bbd.rs:515-535
+    ap, ai = [0, 2, 4, 6, 8], [0, 1, 0, 1, 2, 3, 2, 3]
+    a, nb = O.bbd_partition(4, ap, ai, 2)
+    assert nb == 2 and a[0] == a[1] and a[2] == a[3] and a[0] != a[2] and -1 not in a
+
+
+def test_decompose_block_diagonal_and_boundaries():  # bbd.rs:538-561, 671-696
+    ap, ai = [0, 2, 4, 6, 8], [0, 1, 0, 1, 2, 3, 2, 3]
+    d = O.bbd_decompose(4, ap, ai, 2)
+    assert d["num_blocks"] == 2 and d["border_size"] == 0
+    assert list(d["block_ptr"]) == [0, 2, 4] and d["border_start"] == 4
+    assert all(d["inv_perm"][d["perm"][k]] == k for k in range(4))
+
+
+def test_decompose_with_coupling_by_hand():  # bbd.rs:564-593
+    """adj 0-2, 2-4; BFS from node 1 (first minimum degree 0) gives levels
+    [1],[0] | [2],[3],[4] (disconnected nodes as singletons, half = 2), so
+    A={0,1}, B={2,3,4}; 0 and 2 face each other across the cut. Blocks {1},
+    {3,4}, border {0,2}."""
+    ap, ai = [0, 2, 3, 6, 7, 9], [0, 2, 1, 0, 2, 4, 3, 2, 4]
+    d = O.bbd_decompose(5, ap, ai, 2)
+    assert list(d["perm"]) == [1, 3, 4, 0, 2]
+    assert list(d["block_ptr"]) == [0, 1, 3] and d["border_start"] == 3 and d["border_size"] == 2
+
+
+def test_decompose_ladder_empty_single():  # bbd.rs:596-668
+    n = 6
+    cols = [[c - 1] * (c > 0) + [c] + [c + 1] * (c < n - 1) for c in range(n)]
+    ap = np.cumsum([0] + [len(c) for c in cols])
+    ai = np.concatenate(cols)
+    d = O.bbd_decompose(n, ap, ai, 2)
+    assert sorted(d["perm"]) == list(range(n)) and d["num_blocks"] >= 1
+    d = O.bbd_decompose(0, [0], [], 2)
+    assert d["num_blocks"] == 1 and d["border_size"] == 0
+    d = O.bbd_decompose(1, [0, 1], [0], 2)
+    assert d["num_blocks"] == 1 and list(d["perm"]) == [0]
+
+
+# ---- bbd_solver.rs tests -------------------------------------------------
+def test_solver_known_answers():  # bbd_solver.rs:1137-1170
+    x, s = _bbd_solve(3, [0, 1, 2, 3], [0, 1, 2], [2.0, 3.0, 4.0], [4.0, 9.0, 8.0])
+    assert np.allclose(x, [2, 3, 2], atol=1e-10) and s.info()["blocks"] == 1
+    x, _ = _bbd_solve(2, [0, 2, 4], [0, 1, 0, 1], [3.0, 1.0, 1.0, 2.0], [9.0, 8.0])
+    assert np.allclose(x, [2, 3], atol=1e-9)
+
+
+def test_solver_bordered_and_larger_match_dense():  # bbd_solver.rs:1231-1364
+    n = 5
+    ap = [0, 2, 5, 7, 10, 13]
+    ai = [0, 1, 0, 1, 4, 2, 3, 2, 3, 4, 1, 3, 4]
+    ax = [4.0, 1.0, 1.0, 4.0, 1.0, 4.0, 1.0, 1.0, 4.0, 1.0, 1.0, 1.0, 3.0]
+    b = np.array([5.0, 10.0, 5.0, 10.0, 8.0])
+    x, s = _bbd_solve(n, ap, ai, ax, b)
+    assert s.info()["blocks"] == 2 and s.info()["schur"] == 1  # dense Schur path taken
+    rc, xd = O.dense_factor_solve(n, ap, ai, ax, b)
+    assert rc == 0 and np.allclose(x, xd, atol=1e-8) and _residual(n, ap, ai, ax, x, b) < 1e-10
+    n = 8
+    ap = [0, 2, 5, 8, 10, 13, 16, 19, 22]
+    ai = [0, 1, 0, 1, 2, 1, 2, 6, 3, 4, 3, 4, 5, 4, 5, 7, 2, 6, 7, 5, 6, 7]
+    ax = [2.0, -1.0, -1.0, 2.0, -1.0, -1.0, 2.0, 1.0, 2.0, -1.0, -1.0, 2.0, -1.0, -1.0, 2.0,
+          1.0, 1.0, 3.0, 0.5, 1.0, 0.5, 3.0]
+    b = np.array([1.0, 2.0, 3.0, 1.0, 2.0, 3.0, 4.0, 4.0])
+    x, _ = _bbd_solve(n, ap, ai, ax, b)
+    rc, xd = O.dense_factor_solve(n, ap, ai, ax, b)
+    assert rc == 0 and np.allclose(x, xd, atol=1e-8)
+
+
+def test_solver_repeated_factor_solve():  # bbd_solver.rs:1366-1394
+    ap, ai = [0, 2, 4, 6, 8], [0, 1, 0, 1, 2, 3, 2, 3]
+    s = O.Bbd(4, 2)
+    s.prepare(4)
+    s.analyze(ap, ai)
+    for ax, b in (([2.0, 1.0, 1.0, 3.0, 4.0, 1.0, 1.0, 2.0], [7.0, 11.0, 17.0, 6.0]),
+                  ([5.0, 2.0, 2.0, 6.0, 3.0, 1.0, 1.0, 4.0], [1.0, 2.0, 3.0, 4.0])):
+        assert s.factor(ap, ai, ax) == 0
+        x = np.array(b)
+        s.solve(x)
+        assert _residual(4, ap, ai, ax, x, np.array(b)) < 1e-10
+    assert s.info()["factor_count"] == 2
+
+
+def _grid(k, rs, ground=False):
+    """k x k conductance grid (+ diagonal loading), optionally with node 0 as
+    a ground row/column touching every node (the MNA shape of Q7)."""
+    n = k * k + (1 if ground else 0)
+    off = 1 if ground else 0
+    trip = {}
+    def add(i, j, v):
+        trip.setdefault(j, []).append((i, v))
+    for r in range(k):
+        for c in range(k):
+            i = off + r * k + c
+            add(i, i, 4.0 + rs.rand())
+            for (rr, cc) in ((r + 1, c), (r, c + 1)):
+                if rr < k and cc < k:
+                    j = off + rr * k + cc
+                    g = -0.5 - rs.rand()
+                    add(i, j, g)
+                    add(j, i, g)
+            if ground:
+                add(0, i, -1e-3)
+                add(i, 0, -1e-3)
+    if ground:
+        add(0, 0, 1.0 + 1e-3 * k * k)
+    ap, ai, ax = [0], [], []
+    for j in range(n):
+        ents = sorted(trip.get(j, []), key=lambda e: e[0])
+        ai += [e[0] for e in ents]
+        ax += [e[1] for e in ents]
+        ap.append(len(ai))
+    return n, np.array(ap, np.int64), np.array(ai, np.int64), np.array(ax)
+
+
+def test_grid_dense_and_sparse_schur_match_scipy():
+    rs = np.random.RandomState(3)
+    for k, nb, want in ((12, 2, 1), (40, 2, 0), (24, 8, None)):
+        n, ap, ai, ax = _grid(k, rs)
+        b = rs.randn(n)
+        x, s = _bbd_solve(n, ap, ai, ax, b, nb)
+        info = s.info()
+        assert info["blocks"] >= 2, info
+        if want is not None:
+            assert info["schur"] == want, info  # 1: dense (border <= 64), 0: sparse
+        xt = spl.spsolve(sp.csc_matrix((ax, ai, ap), shape=(n, n)), b)
+        assert np.abs(x - xt).max() <= 1e-10 * np.abs(xt).max()
+
+
+def test_ground_collapses_to_sparselu_bit_exact():
+    """Q7: with a ground row touching every node the BFS bisection collapses,
+    the decomposition is trivial and BbdSolver is exactly SparseLU."""
+    rs = np.random.RandomState(5)
+    n, ap, ai, ax = _grid(10, rs, ground=True)
+    b = rs.randn(n)
+    x, s = _bbd_solve(n, ap, ai, ax, b, 64)
+    assert s.info()["blocks"] == 1 and s.info()["schur"] == -1
+    k = O.NativeKlu(n)
+    k.set_sparselu()
+    k.factor(ap, ai, ax)
+    y = b.copy()
+    k.solve(y)
+    assert same_bits(x, y)
+    c = NL.resistor_mesh(16, 16)
+    ap, ai, ax, rhs, n = O.build_mna(c)
+    x, s = _bbd_solve(n, ap, ai, ax, rhs, 64)
+    assert s.info()["blocks"] == 1
+
+
+# ---- DenseSolver (solver.rs:595-724) -------------------------------------
+def test_dense_solver_matches_numpy_and_fails_like_reference():
+    rs = np.random.RandomState(9)
+    for n in (1, 2, 5, 17, 50):
+        A = rs.randn(n, n) + n * np.eye(n) * rs.rand()
+        ap = np.arange(0, n * n + 1, n)
+        ai = np.tile(np.arange(n), n)
+        ax = A.T.ravel().copy()
+        b = rs.randn(n)
+        rc, x = O.dense_factor_solve(n, ap, ai, ax, b)
+        assert rc == 0 and np.allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+    # a zero column: DenseSolver fails with FactorFailed, DenseSchur with SingularMatrix
+    ap, ai, ax = [0, 2, 2], [0, 1], [1.0, 2.0]
+    assert O.dense_factor_solve(2, ap, ai, ax, [1.0, 1.0])[0] == 2
+    assert O.dense_factor_solve(2, ap, ai, ax, [1.0, 1.0], schur=True)[0] == 4
+    # 1e-31 pivot: DenseSolver proceeds, DenseSchur reports it singular (< 1e-30)
+    rc, x = O.dense_factor_solve(1, [0, 1], [0], [1e-31], [1.0])
+    assert rc == 0 and x[0] == 1e31
+    assert O.dense_factor_solve(1, [0, 1], [0], [1e-31], [1.0], schur=True)[0] == 4
