@@ -43,8 +43,15 @@ typedef enum rs_solver_kind {
     RS_SOLVER_NATIVEKLU = 0, /* NativeKluSolver: BTF + per-block AMD, threshold
                                 pivoting on the first factor, pivots reused by
                                 refactors (native_klu.rs)                        */
-    RS_SOLVER_SPARSELU = 1   /* SparseLuSolver: AMD, no pivoting, 1e-14 pivot
+    RS_SOLVER_SPARSELU = 1,  /* SparseLuSolver: AMD, no pivoting, 1e-14 pivot
                                 perturbation (sparse_lu.rs)                      */
+    RS_SOLVER_BBD = 2,       /* BbdSolver with GreedyBisectionPartitioner: SparseLU
+                                diagonal blocks, dense (border <= 64) or SparseLU
+                                Schur complement, SparseLU fallback for a trivial
+                                decomposition (bbd.rs, bbd_solver.rs). Default 4
+                                target blocks (SolverType::Bbd, solver.rs:186)    */
+    RS_SOLVER_DENSE = 3      /* DenseSolver: row-major LU with partial pivoting
+                                (solver.rs:595-724)                              */
 } rs_solver_kind;
 
 typedef struct rs_solver rs_solver;
@@ -88,6 +95,12 @@ void rs_set_parallel_threads(rs_solver *h, int64_t threads);
  * (native_klu.rs:664-681) */
 void rs_set_config(rs_solver *h, double pivot_tol, int32_t ordering, int32_t use_btf);
 int32_t rs_get_stats(const rs_solver *h, rs_solver_stats *out);
+/* BbdSolver::new(n, partitioner, num_blocks) target block count (max(2, .));
+ * takes effect at the next analyze. */
+void rs_set_bbd_blocks(rs_solver *h, int64_t num_blocks);
+/* out[4] = {diagonal blocks (1: SparseLU fallback), border size, Schur kind
+ * (1 dense, 0 sparse, -1 none), factor count} of an RS_SOLVER_BBD solver. */
+int32_t rs_bbd_info(const rs_solver *h, int64_t *out);
 /* Text of the last error (SolverError Display, solver.rs:94-117). */
 int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);
 

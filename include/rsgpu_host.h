@@ -43,6 +43,11 @@ void rs_host_factor_block_get(const rs_host_factor *h, int64_t b, int64_t *col_p
  *           scatter runs, off-diagonal entries, nnz(L), nnz(U)} */
 void rs_host_factor_plan_stats(const rs_host_factor *h, int64_t *out);
 
+/* bbd_decompose (crates/sim-core/src/bbd.rs:355-485) with
+ * GreedyBisectionPartitioner: perm[new] = old (n), block_ptr (blocks + 1
+ * entries, at most num_blocks + 1), border_start; returns the block count. */
+int64_t rs_host_bbd_decompose(int64_t n, const int64_t *ap, const int64_t *ai, int64_t num_blocks,
+                              int64_t *perm, int64_t *block_ptr, int64_t *border_start);
 /* Step-program check on the host (no reference counterpart): the first
  * factor (pivots) on ax0, then the refactor on ax1 and the solve of rhs,
  * executed by the host model of the GPU step programs. x equals

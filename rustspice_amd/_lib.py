@@ -28,6 +28,7 @@ EXPORTS = [
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest",
+    "rs_host_bbd_decompose", "rs_set_bbd_blocks", "rs_bbd_info",
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
     "rs_circuit_set_cap_state", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
     "rs_profile_enable", "rs_profile_get",
@@ -101,6 +102,12 @@ def lib():
     L.rs_host_stamp_pattern.restype = C.c_int32
     L.rs_host_stamp_pattern.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32,
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64), vp, vp]
+    L.rs_host_bbd_decompose.restype = C.c_int64
+    L.rs_host_bbd_decompose.argtypes = [C.c_int64, i64p, i64p, C.c_int64, i64p, i64p,
+                                        C.POINTER(C.c_int64)]
+    L.rs_set_bbd_blocks.argtypes = [C.c_void_p, C.c_int64]
+    L.rs_bbd_info.restype = C.c_int32
+    L.rs_bbd_info.argtypes = [C.c_void_p, i64p]
     L.rs_host_sp_selftest.restype = C.c_int32
     L.rs_host_sp_selftest.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, f64p, f64p, f64p, i64p]
     L.rs_profile_enable.argtypes = [C.c_int32]

@@ -9,6 +9,7 @@
 
 #include "../../include/rsgpu.h"
 #include "../../include/rsgpu_host.h"
+#include "bbd.hpp"
 #include "plan.hpp"
 #include "sp.hpp"
 #include "stamp.hpp"
@@ -206,6 +207,16 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
         delete h;
         return RS_ERR_INVALID_MATRIX;
     }
+}
+
+int64_t rs_host_bbd_decompose(int64_t n, const int64_t* ap, const int64_t* ai, int64_t num_blocks,
+                              int64_t* perm, int64_t* block_ptr, int64_t* border_start) {
+    BbdDecomp d;
+    bbd_decompose(n, ap, ai, num_blocks, d);
+    if (n) std::memcpy(perm, d.perm.data(), n * sizeof(int64_t));
+    std::memcpy(block_ptr, d.block_ptr.data(), d.block_ptr.size() * sizeof(int64_t));
+    if (border_start) *border_start = d.border_start;
+    return d.num_blocks;
 }
 
 }  // extern "C"

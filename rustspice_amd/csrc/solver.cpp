@@ -15,6 +15,7 @@
 
 #include "../../include/rsgpu.h"
 #include "plan.hpp"
+#include "bbd.hpp"
 #include "solver_impl.hpp"
 #include "profile.hpp"
 
@@ -312,10 +313,12 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
 extern "C" {
 
 rs_solver* rs_solver_create(int32_t kind, int64_t n) {
-    if (kind != RS_SOLVER_NATIVEKLU && kind != RS_SOLVER_SPARSELU) return nullptr;
+    if (kind < RS_SOLVER_NATIVEKLU || kind > RS_SOLVER_DENSE) return nullptr;
     rs_solver* h = new rs_solver();
     h->kind = kind;
     h->n = n;
+    if (kind == RS_SOLVER_BBD) h->bbd = bbd_create();
+    if (kind == RS_SOLVER_DENSE) h->dense = dense_create();
     if (kind == RS_SOLVER_SPARSELU) {
         h->pivot_tol = 0.0;
         h->use_btf = false;
@@ -325,6 +328,8 @@ rs_solver* rs_solver_create(int32_t kind, int64_t n) {
 
 void rs_solver_destroy(rs_solver* h) {
     if (!h) return;
+    bbd_destroy(h->bbd);
+    dense_destroy(h->dense);
     h->d_desc.release();
     h->d_ip.release();
     h->d_vals.release();
@@ -337,6 +342,7 @@ void rs_solver_destroy(rs_solver* h) {
 
 void rs_prepare(rs_solver* h, int64_t n) {
     if (n != h->n) {
+        if (h->bbd) bbd_reset(h);
         reset(h);
         h->n = n;
     }
@@ -344,6 +350,8 @@ void rs_prepare(rs_solver* h, int64_t n) {
 
 int32_t rs_analyze(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t* ai,
                    int64_t ai_len) {
+    if (h->bbd) return bbd_analyze(h, ap, ap_len, ai, ai_len);
+    if (h->dense) return RS_OK;  // DenseSolver::analyze is a no-op (solver.rs:683-685)
     if (h->state != ST_EMPTY && pattern_matches(h, ap, ap_len, ai, ai_len)) return RS_OK;
     if (ap_len != h->n + 1) {
         char b[160];
@@ -356,6 +364,12 @@ int32_t rs_analyze(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_
 
 int32_t rs_factor(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t* ai,
                   int64_t ai_len, const double* ax, int64_t ax_len) {
+    if (h->bbd || h->dense) {
+        if (ap_len == h->n + 1 && ax_len < ap[h->n])
+            return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: ax shorter than nnz");
+        return h->bbd ? bbd_factor(h, ap, ap_len, ai, ai_len, ax)
+                      : dense_factor(h, ap, ap_len, ai, ai_len, ax);
+    }
     if (h->state == ST_EMPTY || !pattern_matches(h, ap, ap_len, ai, ai_len)) {
         if (ap_len != h->n + 1) {
             char b[160];
@@ -386,6 +400,11 @@ int32_t rs_factor(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t
 }
 
 int32_t rs_solve(rs_solver* h, double* rhs, int64_t n) {
+    if (h->dense) {  // DenseSolver checks the length first (solver.rs:689-692)
+        if (n != h->n) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+        return dense_solve(h, rhs);
+    }
+    if (h->bbd && h->state != ST_FACTORED) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
     if (h->state != ST_FACTORED) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
     if (n != h->n) {
         char b[160];
@@ -393,6 +412,7 @@ int32_t rs_solve(rs_solver* h, double* rhs, int64_t n) {
                       (long long)n, (long long)h->n);
         return fail(h, RS_ERR_INVALID_MATRIX, b);
     }
+    if (h->bbd) return bbd_solve(h, rhs);
     hipError_t e = hipMemcpy(h->d_rhs.p, rhs, n * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(rhs)");
     e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, h->d_rhs.p, h->d_x.p,
@@ -403,10 +423,28 @@ int32_t rs_solve(rs_solver* h, double* rhs, int64_t n) {
     return RS_OK;
 }
 
-void rs_reset_pattern(rs_solver* h) { reset(h); }
+void rs_reset_pattern(rs_solver* h) {
+    if (h->bbd) bbd_reset(h);
+    reset(h);
+}
 
 const char* rs_solver_name(const rs_solver* h) {
+    if (h->kind == RS_SOLVER_BBD) {  // bbd_solver.rs:1091-1098
+        int64_t info[4];
+        bbd_info(h, info);
+        return info[0] > 1 ? "BBD-MI355X" : "SparseLU (BBD fallback)-MI355X";
+    }
+    if (h->kind == RS_SOLVER_DENSE) return "Dense-MI355X";
     return h->kind == RS_SOLVER_SPARSELU ? "SparseLU-MI355X" : "NativeKLU-MI355X";
+}
+
+void rs_set_bbd_blocks(rs_solver* h, int64_t num_blocks) {
+    if (h->bbd) bbd_set_target(h, num_blocks);
+}
+
+int32_t rs_bbd_info(const rs_solver* h, int64_t* out) {
+    if (!h->bbd) return RS_ERR_INVALID_MATRIX;
+    return bbd_info(h, out);
 }
 
 void rs_set_parallel_threads(rs_solver* h, int64_t t) { h->threads_hint = t; }

@@ -18,6 +18,9 @@ hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const
                         bool lds, int threads, hipStream_t stream);
 size_t device_max_lds();
 
+struct BbdImpl;
+struct DenseImpl;
+
 enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
 constexpr int kThreads = 256;
 
@@ -57,6 +60,8 @@ using rsgpu::i64;
 
 struct rs_solver {
     int kind = RS_SOLVER_NATIVEKLU;
+    rsgpu::BbdImpl* bbd = nullptr;      // kind RS_SOLVER_BBD
+    rsgpu::DenseImpl* dense = nullptr;  // kind RS_SOLVER_DENSE
     i64 n = 0;
     double pivot_tol = 0.001;
     int ordering = 0;

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 
-NATIVEKLU, SPARSELU = 0, 1
+NATIVEKLU, SPARSELU, BBD, DENSE = 0, 1, 2, 3
 
 _ERR_NAMES = {
     1: "AnalyzeFailed", 2: "FactorFailed", 3: "SolveFailed", 4: "SingularMatrix",
@@ -38,7 +38,8 @@ def _f64(a):
 
 
 class GpuSolver:
-    """``create_solver(SolverType::NativeKlu | SparseLu, n)`` on the MI355X."""
+    """``create_solver(SolverType::NativeKlu | SparseLu | Bbd | Dense, n)`` on
+    the MI355X (solver.rs:169-200)."""
 
     def __init__(self, n: int, kind: int = NATIVEKLU):
         self._L = _lib.lib()
@@ -86,6 +87,16 @@ class GpuSolver:
 
     def reset_pattern(self):
         self._L.rs_reset_pattern(self.h)
+
+    def set_bbd_blocks(self, num_blocks: int):
+        """BbdSolver::new(.., num_blocks): target diagonal blocks (BBD kind)."""
+        self._L.rs_set_bbd_blocks(self.h, num_blocks)
+
+    def bbd_info(self):
+        out = np.zeros(4, np.int64)
+        self._check(self._L.rs_bbd_info(self.h, out))
+        return dict(blocks=int(out[0]), border=int(out[1]), schur=int(out[2]),
+                    factor_count=int(out[3]))
 
     def name(self) -> str:
         return self._L.rs_solver_name(self.h).decode()
@@ -215,3 +226,14 @@ def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU):
         raise SolverError(rc, "step program")
     keys = ["refactor_steps", "refactor_ops", "solve_steps", "solve_ops"]
     return x, dict(zip(keys, map(int, st)))
+
+
+def host_bbd_decompose(n, ap, ai, num_blocks):
+    """The library's bbd_decompose (host integer work) as a dict."""
+    L = _lib.lib()
+    perm = np.zeros(max(n, 1), np.int64)
+    bp = np.zeros(num_blocks + 2, np.int64)
+    b0 = C.c_int64()
+    nb = L.rs_host_bbd_decompose(n, _i64(ap), _i64(ai), num_blocks, perm, bp, C.byref(b0))
+    return dict(num_blocks=int(nb), perm=perm[:n], block_ptr=bp[: nb + 1], border_start=b0.value,
+                border_size=n - b0.value if nb > 1 else 0)
