@@ -139,6 +139,16 @@ int32_t rs_circuit_last_error(const rs_circuit *c, char *buf, int64_t len);
  * of engine.rs:262-283 / :485-501): call with ap = ai = NULL for the sizes. */
 int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap,
                            int64_t *ai);
+/* DC sensitivities by the adjoint method (no reference counterpart).
+ * x: a converged DC solution (x_len >= node_count, as returned by rs_dc_op);
+ * probe: the unknown whose sensitivity is wanted (a node index). Solves
+ * J(x)^T lambda = e_probe with J the exact Jacobian of the stamped DC
+ * currents (SparseLU factor on the GPU, transposed triangular solves reusing
+ * it) and writes dx[probe]/dp for every device into dp[ndev]: resistors
+ * d/dR, V and I sources d/dvalue, MOSFETs d/dW, diodes d/dIs, capacitors 0.
+ * lambda (n entries) is optional. gmin: the value the DC OP used. */
+int32_t rs_dc_adjoint(rs_circuit *c, const double *x, int64_t x_len, int64_t probe, double gmin,
+                      double *dp, double *lambda);
 /* 1 (default): a DC OP whose solver starts from scratch (the circuit has aux
  * rows and the solver its default configuration, no stepping) runs as one
  * device-resident launch, with results identical to the host-driven loop;

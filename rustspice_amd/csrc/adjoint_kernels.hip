@@ -1,0 +1,147 @@
+// adjoint_kernels.hip - DC sensitivities by the adjoint method (no reference
+// counterpart; the forward path it differentiates is engine.rs's DC OP).
+//
+// At a converged operating point F(x*, p) = 0, with J = dF/dx(x*):
+//   J^T lambda = e_probe,   dV_probe/dp = -lambda^T dF/dp.
+// k_solve_t solves J^T lambda = e with the stored L/U of J (a single-block
+// plan: J = Pout U^-1 L^-1 Pin inverted, so J^-T = Pin^T L^-T U^-T Pout^T);
+// k_adjoint_contract forms dF/dp per device in closed form and contracts it
+// with lambda, one device per thread.
+#include <hip/hip_runtime.h>
+
+#include "../../include/rs_circuit.h"
+#include "device_eval.hpp"
+#include "plan.hpp"
+
+namespace {
+
+constexpr int kT = 256, kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One workgroup. U^T is lower triangular: column k of U gives row k of U^T,
+// so y_k = (t_k - sum_j U(j,k) y_j) / U(k,k) in the refactor's column levels
+// (column k waits for every j in U(:,k)). L^T is unit upper: z_k = y_k -
+// sum_i L(i,k) z_i over the L^T levels (column k waits for every row of
+// L(:,k)). One wavefront per column, lanes over its entries.
+__global__ __launch_bounds__(kT) void k_solve_t(const rsgpu::PlanDev* __restrict__ pd,
+                                               const int32_t* __restrict__ ip,
+                                               const double* __restrict__ vals,
+                                               const double* __restrict__ e,
+                                               const int32_t* __restrict__ lt_ptr,
+                                               const int32_t* __restrict__ lt_cols, int n_lt,
+                                               double* __restrict__ T, double* __restrict__ lam) {
+    const rsgpu::PlanDev d = *pd;
+    const int n = d.n, lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const int32_t* perm_in = ip + d.o_perm_in;
+    const int32_t* perm_out = ip + d.o_perm_out;
+    const int32_t* lcp = ip + d.o_lcp;
+    const int32_t* lri = ip + d.o_lri;
+    const int32_t* ucp = ip + d.o_ucp;
+    const int32_t* uri = ip + d.o_uri;
+    const int32_t* flev_ptr = ip + d.o_flev_ptr;
+    const int32_t* flev_cols = ip + d.o_flev_cols;
+    const double* lx = vals + d.o_lx;
+    const double* ux = vals + d.o_ux;
+    const double* ud = vals + d.o_ud;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) T[perm_out[i]] = e[i];
+    __syncthreads();
+    for (int lev = 0; lev < d.nflev; ++lev) {
+        for (int q = flev_ptr[lev] + wid; q < flev_ptr[lev + 1]; q += nw) {
+            const int k = flev_cols[q];
+            double s = 0.0;
+            for (int u = ucp[k] + lane; u < ucp[k + 1] - 1; u += kWave) s += ux[u] * T[uri[u]];
+            s = wave_sum(s);
+            if (lane == 0) T[k] = (T[k] - s) / ud[k];
+        }
+        __syncthreads();
+    }
+    for (int lev = 0; lev < n_lt; ++lev) {
+        for (int q = lt_ptr[lev] + wid; q < lt_ptr[lev + 1]; q += nw) {
+            const int k = lt_cols[q];
+            double s = 0.0;
+            for (int l = lcp[k] + lane; l < lcp[k + 1]; l += kWave) s += lx[l] * T[lri[l]];
+            s = wave_sum(s);
+            if (lane == 0) T[k] = T[k] - s;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lam[i] = T[perm_in[i]];
+}
+
+__device__ __forceinline__ double at(const double* v, int xl, int i) { return i < xl ? v[i] : 0.0; }
+
+// dV_probe/dp per device: R -> d/dR, V and I sources -> d/dvalue,
+// MOSFET -> d/dW (level 1: the stamped current is linear in W), diode -> d/dIs,
+// capacitor -> 0 (open in DC).
+__global__ void k_adjoint_contract(const rs_device* __restrict__ dev, int ndev,
+                                   const int32_t* __restrict__ aux_of, const double* __restrict__ x,
+                                   int xl, const double* __restrict__ lam, double gmin,
+                                   double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ndev) return;
+    const rs_device v = dev[k];
+    const int a = v.node[0], b = v.node[1];
+    double r = 0.0;
+    switch (v.kind) {
+    case RS_DEV_R: {
+        const double vab = at(x, xl, a) - at(x, xl, b);
+        r = vab * (lam[a] - lam[b]) / (v.value * v.value);
+        break;
+    }
+    case RS_DEV_V:
+        r = aux_of[k] >= 0 ? lam[aux_of[k]] : 0.0;
+        break;
+    case RS_DEV_I:
+        r = -(lam[a] - lam[b]);
+        break;
+    case RS_DEV_D: {  // F_a += id(Is), id linear in Is
+        const double vt = 0.02585 * v.p[1];
+        const double vd = at(x, xl, a) - at(x, xl, b);
+        const double id_per_is = exp(vd / vt) - 1.0;
+        r = -(lam[a] - lam[b]) * id_per_is;
+        break;
+    }
+    case RS_DEV_M: {  // F_d += ids_s, F_s -= ids_s
+        // The Newton stamp at x* reproduces the current: gds (vd - vs) +
+        // gm (vg - vs) + ieq = ids_s (gds is never below the gmin floor here).
+        double o[rsgpu::kSlotsPerDevice];
+        const rsgpu::StampArgs sa{0.0, 0.0, gmin, 1.0, rsgpu::MODE_DC, 0};
+        rsgpu::eval_device(v, k, x, xl, sa, nullptr, nullptr, nullptr, nullptr, o);
+        const double vd = at(x, xl, v.node[0]), vg = at(x, xl, v.node[1]), vs = at(x, xl, v.node[2]);
+        const double ids = o[0] * (vd - vs) + o[4] * (vg - vs) + o[9];
+        r = -(lam[v.node[0]] - lam[v.node[2]]) * ids / v.p[RS_MP_W];
+        break;
+    }
+    default:
+        break;
+    }
+    out[k] = r;
+}
+
+}  // namespace
+
+namespace rsgpu {
+
+hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* vals, const double* e,
+                          const int32_t* lt_ptr, const int32_t* lt_cols, int n_lt, double* T,
+                          double* lam, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_t, dim3(1), dim3(kT), 0, s, pd, ip, vals, e, lt_ptr, lt_cols, n_lt,
+                       T, lam);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_contract(const rs_device* dev, int ndev, const int32_t* aux_of,
+                                   const double* x, int xl, const double* lam, double gmin,
+                                   double* out, hipStream_t s) {
+    if (ndev <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint_contract, dim3((ndev + kT - 1) / kT), dim3(kT), 0, s, dev, ndev,
+                       aux_of, x, xl, lam, gmin, out);
+    return hipGetLastError();
+}
+
+}  // namespace rsgpu

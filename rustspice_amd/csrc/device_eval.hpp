@@ -122,8 +122,20 @@ __device__ __forceinline__ void eval_device(const rs_device& v, int d, const dou
         const double ids_s = ids * sign;
         const double ieq = ids_s - gm * (vg - vs) - gds * (vd - vs);
         const double gdsm = fmax_rust(gds, gmin);
-        o[0] = gdsm; o[1] = gdsm; o[2] = -gdsm; o[3] = -gdsm;
-        o[4] = gm; o[5] = -gm; o[6] = -gm; o[7] = gm;
+        if (a.flags & rsgpu::kStampJacobian) {
+            // d(ids_s)/d(vd, vg, vs) of the current above: internal-frame
+            // partials (reversed when the internal vds was negative), mapped
+            // back through the PMOS flip (vd, vg, vs) -> (-vs, -vg, -vd).
+            const bool rev = (vdi - vsi) < 0.0;
+            const double fd = rev ? -gm - gds : gds, fg = gm, fs = rev ? gds : -gm - gds;
+            const double jd = pm ? fs : fd, jg = fg, js = pm ? fd : fs;
+            const double lk = gdsm - gds;  // the gmin floor stamped on (d, s)
+            o[0] = jd + lk; o[1] = -js + lk; o[2] = js - lk; o[3] = -jd - lk;
+            o[4] = jg; o[5] = 0.0; o[6] = -jg; o[7] = 0.0;
+        } else {
+            o[0] = gdsm; o[1] = gdsm; o[2] = -gdsm; o[3] = -gdsm;
+            o[4] = gm; o[5] = -gm; o[6] = -gm; o[7] = gm;
+        }
         o[8] = -ieq; o[9] = ieq;
         break;
     }

@@ -32,6 +32,12 @@ hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_pt
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
                             uint8_t* has_i, hipStream_t s);
+hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* vals, const double* e,
+                          const int32_t* lt_ptr, const int32_t* lt_cols, int n_lt, double* T,
+                          double* lam, hipStream_t s);
+hipError_t launch_adjoint_contract(const rs_device* dev, int ndev, const int32_t* aux_of,
+                                   const double* x, int xl, const double* lam, double gmin,
+                                   double* out, hipStream_t s);
 }  // namespace rsgpu
 
 using namespace rsgpu;
@@ -65,6 +71,12 @@ struct rs_circuit {
     bool device_newton = true;  // whole DC OP in one launch when equivalent
     rs_batch* dcb[2] = {nullptr, nullptr};  // one-instance batch per solver kind
     int64_t device_dc_runs = 0;
+    // adjoint: SparseLU factor of the exact Jacobian, L^T levels, vectors
+    rs_solver* adj = nullptr;
+    int64_t adj_factors = -1;  // adj factor_count when the L^T levels were built
+    int n_lt = 0;
+    DevBuf<int32_t> d_aux, d_ltp, d_ltc;
+    DevBuf<double> d_e, d_t, d_lam, d_dp;
 };
 
 namespace {
@@ -116,11 +128,11 @@ int32_t get_pattern(rs_circuit* c, int mode, DevPattern*& out) {
 
 // Stamp on the device from a device-resident x.
 int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t, double dt,
-                  double gmin, double scale, DevPattern** pout, hipStream_t s) {
+                  double gmin, double scale, DevPattern** pout, hipStream_t s, int32_t flags = 0) {
     DevPattern* dp;
     int32_t rc = get_pattern(c, mode, dp);
     if (rc) return rc;
-    StampArgs a{t, dt, gmin, scale, mode, 0};
+    StampArgs a{t, dt, gmin, scale, mode, flags};
     int ps = prof_begin(PK_EVAL, s);
     hipError_t e = launch_eval(c->d_dev.p, (int)c->dev.size(), x_dev, (int)xl, a, c->d_cap_v.p,
                                c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s);
@@ -433,6 +445,7 @@ rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_c
 
 void rs_circuit_destroy(rs_circuit* c) {
     if (!c) return;
+    if (c->adj) rs_solver_destroy(c->adj);
     for (auto*& b : c->dcb)
         if (b) rs_batch_destroy(b);
     for (auto& p : c->pat) {
@@ -513,6 +526,71 @@ int32_t rs_dc_op(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double*
     *res = r;
     if (x_out) std::memcpy(x_out, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
     if (x_len) *x_len = r.converged ? (int64_t)x.size() : 0;
+    return RS_OK;
+}
+
+int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t probe, double gmin,
+                      double* dp_out, double* lambda_out) {
+    DevPattern* dp;
+    int32_t rc = get_pattern(c, MODE_DC, dp);
+    if (rc) return rc;
+    const i64 n = dp->p.n, nd = (i64)c->dev.size();
+    if (probe < 0 || probe >= n) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: probe out of range");
+    if (x_len < c->node_count) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: x shorter than node_count");
+    const i64 xl = std::min<i64>(x_len, n);
+    hipStream_t s = nullptr;
+    hipError_t e;
+    if ((e = hipMemcpy(c->d_x.p, x, xl * sizeof(double), hipMemcpyHostToDevice)))
+        return chip(c, e, "hipMemcpy(x)");
+    // J(x*): the exact derivative of the stamped currents (MOSFETs differ from the Newton matrix)
+    if ((rc = stamp_dev(c, MODE_DC, c->d_x.p, xl, 0.0, 0.0, gmin, 1.0, nullptr, s, kStampJacobian)))
+        return rc;
+    if (!c->adj) c->adj = rs_solver_create(RS_SOLVER_SPARSELU, n);
+    rs_prepare(c->adj, n);
+    if ((rc = solver_factor_dev(c->adj, dp->p.ap.data(), n + 1, dp->p.ai.data(),
+                                (i64)dp->p.ai.size(), c->d_ax.p, s)))
+        return from_solver(c, c->adj, rc);
+    rs_solver* a = c->adj;
+    if (a->pd.nblk != 1) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint needs a single-block factor");
+    if (c->adj_factors != a->factor_count || !c->d_ltp.p) {  // L^T levels of this factorization
+        const PlanHost& ph = a->ph;
+        std::vector<int32_t> lev(n, 0), ptr, cols;
+        int32_t nlev = 0;
+        for (i64 k = n - 1; k >= 0; k--) {
+            int32_t lv = 0;
+            for (int32_t q = ph.lcp[k]; q < ph.lcp[k + 1]; q++) lv = std::max(lv, lev[ph.lri[q]] + 1);
+            lev[k] = lv;
+            nlev = std::max(nlev, lv + 1);
+        }
+        ptr.assign(nlev + 1, 0);
+        for (i64 k = 0; k < n; k++) ptr[lev[k] + 1]++;
+        for (int32_t l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
+        cols.resize(n);
+        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+        for (i64 k = 0; k < n; k++) cols[fill[lev[k]]++] = (int32_t)k;
+        std::vector<int32_t> aux(nd);
+        for (i64 k = 0; k < nd; k++) aux[k] = dp->p.aux_of[k];
+        if ((e = c->d_ltp.ensure(ptr.size())) || (e = c->d_ltc.ensure(std::max<size_t>(cols.size(), 1))) ||
+            (e = c->d_aux.ensure(std::max<i64>(nd, 1))) ||
+            (e = hipMemcpy(c->d_ltp.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice)) ||
+            (!cols.empty() && (e = hipMemcpy(c->d_ltc.p, cols.data(), cols.size() * 4, hipMemcpyHostToDevice))) ||
+            (nd && (e = hipMemcpy(c->d_aux.p, aux.data(), nd * 4, hipMemcpyHostToDevice))))
+            return chip(c, e, "adjoint levels");
+        c->n_lt = nlev;
+        c->adj_factors = a->factor_count;
+    }
+    std::vector<double> ev(n, 0.0);
+    ev[probe] = 1.0;
+    if ((e = c->d_e.ensure(n)) || (e = c->d_t.ensure(n)) || (e = c->d_lam.ensure(n)) ||
+        (e = c->d_dp.ensure(std::max<i64>(nd, 1))) ||
+        (e = hipMemcpy(c->d_e.p, ev.data(), n * sizeof(double), hipMemcpyHostToDevice)) ||
+        (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_e.p, c->d_ltp.p, c->d_ltc.p,
+                            c->n_lt, c->d_t.p, c->d_lam.p, s)) ||
+        (e = launch_adjoint_contract(c->d_dev.p, (int)nd, c->d_aux.p, c->d_x.p, (int)xl, c->d_lam.p,
+                                     gmin, c->d_dp.p, s)) ||
+        (nd && (e = hipMemcpy(dp_out, c->d_dp.p, nd * sizeof(double), hipMemcpyDeviceToHost))) ||
+        (lambda_out && (e = hipMemcpy(lambda_out, c->d_lam.p, n * sizeof(double), hipMemcpyDeviceToHost))))
+        return chip(c, e, "adjoint");
     return RS_OK;
 }
 

@@ -42,11 +42,14 @@ struct StampPattern {
 void build_stamp_pattern(const rs_device* dev, i64 ndev, i64 node_count, int mode,
                          StampPattern& out);
 
-// Device-side stamp parameters per call.
+// Device-side stamp parameters per call. flags & kStampJacobian: MOSFET
+// matrix values are the exact partial derivatives of the stamped current
+// (used by the adjoint), not the Newton linearisation of stamp.rs.
+constexpr int32_t kStampJacobian = 1;
 struct StampArgs {
     double t, dt, gmin, source_scale;
     int32_t mode;
-    int32_t pad;
+    int32_t flags;
 };
 
 }  // namespace rsgpu

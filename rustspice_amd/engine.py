@@ -75,6 +75,8 @@ def _bind():
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32)]
     L.rs_circuit_set_device_newton.argtypes = [vp, C.c_int32]
+    L.rs_dc_adjoint.restype = C.c_int32
+    L.rs_dc_adjoint.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_double, vp, vp]
     L.rs_batch_create.restype = vp
     L.rs_batch_create.argtypes = [vp, vp, vp, C.c_int64, C.c_int32]
     L.rs_batch_destroy.argtypes = [vp]
@@ -124,6 +126,17 @@ class GpuCircuit:
         self._L.rs_circuit_last_error(self.h, buf, 512)
         if rc != 0 or (buf.value and rc == 0 and b"hipMalloc" in buf.value):
             raise SolverError(rc or 7, buf.value.decode(errors="replace"))
+
+    def dc_adjoint(self, x, probe: int, gmin: float = 1e-12):
+        """dx[probe]/dp for every device at the DC solution x (adjoint method):
+        returns (per-device sensitivities, lambda)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        ap, _ = self.pattern(MODE_DC)
+        n = len(ap) - 1
+        dp = np.zeros(max(self.circuit.ndev, 1))
+        lam = np.zeros(n)
+        self._check(self._L.rs_dc_adjoint(self.h, _ptr(x), len(x), probe, gmin, _ptr(dp), _ptr(lam)))
+        return dp[: self.circuit.ndev], lam
 
     def set_device_newton(self, on: bool):
         """True: a fresh DC OP runs as one device launch (same results)."""
