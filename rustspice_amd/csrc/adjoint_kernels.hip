@@ -75,6 +75,47 @@ __global__ __launch_bounds__(kT) void k_solve_t(const rsgpu::PlanDev* __restrict
 
 __device__ __forceinline__ double at(const double* v, int xl, int i) { return i < xl ? v[i] : 0.0; }
 
+// r = e - J^T lam with J the unmerged CSC (one column per thread): the
+// residual of the transposed system for iterative refinement.
+__global__ void k_resid_t(const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
+                          const double* __restrict__ ax, const double* __restrict__ e,
+                          const double* __restrict__ lam, double* __restrict__ r, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double s = 0.0;
+    for (int q = ap[j]; q < ap[j + 1]; ++q)
+        if (ai[q] < n) s += ax[q] * lam[ai[q]];
+    r[j] = e[j] - s;
+}
+
+// lam += delta; out[0] = max |delta|, out[1] = max |lam| (one workgroup)
+__global__ __launch_bounds__(kT) void k_refine_update(double* __restrict__ lam,
+                                                      const double* __restrict__ delta, int n,
+                                                      double* __restrict__ out) {
+    __shared__ double sd[kT], sl[kT];
+    double md = 0.0, ml = 0.0;
+    for (int i = threadIdx.x; i < n; i += kT) {
+        const double v = lam[i] + delta[i];
+        lam[i] = v;
+        md = fmax(md, fabs(delta[i]));
+        ml = fmax(ml, fabs(v));
+    }
+    sd[threadIdx.x] = md;
+    sl[threadIdx.x] = ml;
+    __syncthreads();
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + w]);
+            sl[threadIdx.x] = fmax(sl[threadIdx.x], sl[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sd[0];
+        out[1] = sl[0];
+    }
+}
+
 // dV_probe/dp per device: R -> d/dR, V and I sources -> d/dvalue,
 // MOSFET -> d/dW (level 1: the stamped current is linear in W), diode -> d/dIs,
 // capacitor -> 0 (open in DC).
@@ -132,6 +173,17 @@ hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* va
                           double* lam, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_t, dim3(1), dim3(kT), 0, s, pd, ip, vals, e, lt_ptr, lt_cols, n_lt,
                        T, lam);
+    return hipGetLastError();
+}
+
+hipError_t launch_resid_t(const int32_t* ap, const int32_t* ai, const double* ax, const double* e,
+                          const double* lam, double* r, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_resid_t, dim3((n + kT - 1) / kT), dim3(kT), 0, s, ap, ai, ax, e, lam, r, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_update(double* lam, const double* delta, int n, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_refine_update, dim3(1), dim3(kT), 0, s, lam, delta, n, out);
     return hipGetLastError();
 }
 

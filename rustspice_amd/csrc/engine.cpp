@@ -38,6 +38,9 @@ hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* va
 hipError_t launch_adjoint_contract(const rs_device* dev, int ndev, const int32_t* aux_of,
                                    const double* x, int xl, const double* lam, double gmin,
                                    double* out, hipStream_t s);
+hipError_t launch_resid_t(const int32_t* ap, const int32_t* ai, const double* ax, const double* e,
+                          const double* lam, double* r, int n, hipStream_t s);
+hipError_t launch_refine_update(double* lam, const double* delta, int n, double* out, hipStream_t s);
 }  // namespace rsgpu
 
 using namespace rsgpu;
@@ -75,8 +78,9 @@ struct rs_circuit {
     rs_solver* adj = nullptr;
     int64_t adj_factors = -1;  // adj factor_count when the L^T levels were built
     int n_lt = 0;
-    DevBuf<int32_t> d_aux, d_ltp, d_ltc;
-    DevBuf<double> d_e, d_t, d_lam, d_dp;
+    DevBuf<int32_t> d_aux, d_ltp, d_ltc, d_jap, d_jai;
+    DevBuf<double> d_e, d_t, d_lam, d_dp, d_r, d_del, d_nrm;
+    int32_t adj_refine = 0;  // refinement steps of the last rs_dc_adjoint
 };
 
 namespace {
@@ -576,17 +580,42 @@ int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t pro
             (!cols.empty() && (e = hipMemcpy(c->d_ltc.p, cols.data(), cols.size() * 4, hipMemcpyHostToDevice))) ||
             (nd && (e = hipMemcpy(c->d_aux.p, aux.data(), nd * 4, hipMemcpyHostToDevice))))
             return chip(c, e, "adjoint levels");
+        std::vector<int32_t> jap(n + 1), jai(std::max<size_t>(dp->p.ai.size(), 1));
+        for (i64 i = 0; i <= n; i++) jap[i] = (int32_t)dp->p.ap[i];
+        for (size_t q = 0; q < dp->p.ai.size(); q++) jai[q] = (int32_t)dp->p.ai[q];
+        if ((e = c->d_jap.ensure(n + 1)) || (e = c->d_jai.ensure(jai.size())) ||
+            (e = hipMemcpy(c->d_jap.p, jap.data(), jap.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(c->d_jai.p, jai.data(), jai.size() * 4, hipMemcpyHostToDevice)))
+            return chip(c, e, "adjoint pattern");
         c->n_lt = nlev;
         c->adj_factors = a->factor_count;
     }
     std::vector<double> ev(n, 0.0);
     ev[probe] = 1.0;
     if ((e = c->d_e.ensure(n)) || (e = c->d_t.ensure(n)) || (e = c->d_lam.ensure(n)) ||
+        (e = c->d_r.ensure(n)) || (e = c->d_del.ensure(n)) || (e = c->d_nrm.ensure(2)) ||
         (e = c->d_dp.ensure(std::max<i64>(nd, 1))) ||
         (e = hipMemcpy(c->d_e.p, ev.data(), n * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_e.p, c->d_ltp.p, c->d_ltc.p,
-                            c->n_lt, c->d_t.p, c->d_lam.p, s)) ||
-        (e = launch_adjoint_contract(c->d_dev.p, (int)nd, c->d_aux.p, c->d_x.p, (int)xl, c->d_lam.p,
+                            c->n_lt, c->d_t.p, c->d_lam.p, s)))
+        return chip(c, e, "adjoint solve");
+    // Iterative refinement against the exact J^T: SparseLU's 1e-14 clamp of a
+    // structurally zero MNA pivot factors J plus a rank-one term, which the
+    // transposed solve alone does not undo.
+    c->adj_refine = 0;
+    for (int it = 0; it < 12; ++it) {
+        double nrm[2];
+        if ((e = launch_resid_t(c->d_jap.p, c->d_jai.p, c->d_ax.p, c->d_e.p, c->d_lam.p, c->d_r.p,
+                                (int)n, s)) ||
+            (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_r.p, c->d_ltp.p, c->d_ltc.p,
+                                c->n_lt, c->d_t.p, c->d_del.p, s)) ||
+            (e = launch_refine_update(c->d_lam.p, c->d_del.p, (int)n, c->d_nrm.p, s)) ||
+            (e = hipMemcpy(nrm, c->d_nrm.p, sizeof nrm, hipMemcpyDeviceToHost)))
+            return chip(c, e, "adjoint refinement");
+        c->adj_refine = it + 1;
+        if (!(nrm[0] > 1e-15 * nrm[1])) break;  // also stops on NaN
+    }
+    if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, c->d_aux.p, c->d_x.p, (int)xl, c->d_lam.p,
                                      gmin, c->d_dp.p, s)) ||
         (nd && (e = hipMemcpy(dp_out, c->d_dp.p, nd * sizeof(double), hipMemcpyDeviceToHost))) ||
         (lambda_out && (e = hipMemcpy(lambda_out, c->d_lam.p, n * sizeof(double), hipMemcpyDeviceToHost))))

@@ -1,10 +1,15 @@
 """DC adjoint sensitivities (no reference counterpart) against central finite
-differences of the forward DC operating point and a dense FP64 CPU adjoint.
+differences of the DC operating point and a dense FP64 CPU adjoint.
 
-The forward solutions come from the oracle's DC OP (the reference Newton loop,
-SparseLU, tight tolerances so the finite differences resolve 1e-6). The
-transposed solve is checked against numpy's dense solve of J^T lambda = e on
-circuits whose Newton matrix is their Jacobian (R, V, I, diode, NMOS forward).
+The adjoint differentiates the exact root of F(x, p) = A(x) x - rhs(x), the
+reference's stamped currents (stamp.rs). The operating points for the finite
+differences are therefore polished to that root: the oracle's DC OP (the
+reference Newton loop with SparseLU) first, then dense Newton steps with a
+numerical Jacobian of the oracle's F. The polish matters: with a voltage
+source whose aux pivot SparseLU clamps to 1e-14 (sparse_lu.rs:493-499), the
+reference loop stops at a point whose node voltages are off by ~1e-3 V (the
+inverter case below), so finite differences of the unpolished solver measure
+that solver, not the circuit.
 """
 import numpy as np
 import pytest
@@ -49,7 +54,18 @@ def _pmos_cs():
     return b.build(4, "pmos_cs")
 
 
-CIRCS = {"ladder": lambda: NL.rc_ladder(12), "mesh": lambda: NL.resistor_mesh(6, 6),
+def _ladder(stages=12):
+    """V - R - R - ... with a shunt R and C to ground at every node (DC current flows)."""
+    b = NL._Builder()
+    b.add(NL.RS_DEV_V, (1, 0), 1.0)
+    for k in range(stages):
+        b.add(NL.RS_DEV_R, (1 + k, 2 + k), 100.0 * (1 + k))
+        b.add(NL.RS_DEV_R, (2 + k, 0), 1e4 / (1 + 0.1 * k))
+        b.add(NL.RS_DEV_C, (2 + k, 0), 1e-12)
+    return b.build(stages + 2, "ladder")
+
+
+CIRCS = {"ladder": _ladder, "mesh": lambda: NL.resistor_mesh(6, 6),
          "diode": NL.diode_circuit, "cs_amp": _cs_amp, "inv1": _inv1, "pmos_cs": _pmos_cs}
 
 
@@ -66,6 +82,32 @@ def _dc(c):
     return x
 
 
+def _root(c, x=None):
+    """Exact root of the oracle's F(x) = A(x) x - rhs(x): dense Newton with a
+    central-difference Jacobian, from the reference DC OP."""
+    x = _dc(c) if x is None else x.copy()
+    nc = c.node_count
+
+    def F(v):
+        ap, ai, ax, rhs, n = O.build_mna(c, x=v[:nc], mode=0)
+        return csc_dense(ap, ai, ax, n) @ v - rhs
+
+    for _ in range(40):
+        n = len(x)
+        J = np.zeros((n, n))
+        for j in range(n):
+            h = 1e-7 * max(1.0, abs(x[j]))
+            xp, xm = x.copy(), x.copy()
+            xp[j] += h
+            xm[j] -= h
+            J[:, j] = (F(xp) - F(xm)) / (2.0 * h)
+        dx = np.linalg.solve(J, -F(x))
+        x = x + dx
+        if np.abs(dx).max() <= 1e-15 * max(1.0, np.abs(x).max()):
+            break
+    return x
+
+
 def _param(c, k):
     """(getter, setter) of the differentiated parameter of device k."""
     d = c.devices[k]
@@ -79,7 +121,7 @@ def _param(c, k):
 @pytest.mark.parametrize("name", sorted(CIRCS))
 def test_adjoint_matches_central_differences(gpu, name):
     c = CIRCS[name]()
-    x = _dc(c)
+    x = _root(c)
     probe = c.node_count // 2 if name == "mesh" else c.node_count - 1
     g = E.GpuCircuit(c)
     dp, lam = g.dc_adjoint(x, probe)
@@ -93,9 +135,9 @@ def test_adjoint_matches_central_differences(gpu, name):
         p0 = get()
         h = h_rel * abs(p0) if p0 != 0.0 else h_rel
         put(p0 + h)
-        xp = _dc(c)
+        xp = _root(c, x)
         put(p0 - h)
-        xm = _dc(c)
+        xm = _root(c, x)
         put(p0)
         fd = (xp[probe] - xm[probe]) / (2.0 * h)
         # resolution of the difference quotient: solution round-off over the step
@@ -108,7 +150,7 @@ def test_adjoint_matches_central_differences(gpu, name):
 @pytest.mark.parametrize("name", ["ladder", "mesh", "diode", "cs_amp"])
 def test_lambda_matches_dense_cpu_adjoint(gpu, name):
     c = CIRCS[name]()
-    x = _dc(c)
+    x = _root(c)
     probe = c.node_count // 2 if name == "mesh" else c.node_count - 1
     ap, ai, ax, rhs, n = O.build_mna(c, x=x[: c.node_count], mode=0)
     A = csc_dense(ap, ai, ax, n)
