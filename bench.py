@@ -133,6 +133,45 @@ def batched_dc_ops(args, E, NL, S, torch):
         speedup_vs_loop_plain=dcops * loop_per, loop_results_identical=bool(same))
 
 
+def adjoint_cost(E, NL, S, torch, reps=5):
+    """DC adjoint (dV/dp for every device) on the 64x64 resistor mesh against
+    the forward DC OP through the same backend (SparseLU, GPU)."""
+    c = NL.resistor_mesh(64, 64)
+    g = E.GpuCircuit(c)
+    s = S.GpuSolver(c.node_count, S.SPARSELU)
+    x, r = g.dc_op(s)
+    probe = c.node_count // 2
+    g.dc_adjoint(x, probe)  # first call: host analysis of the Jacobian pattern
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.dc_op(s)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dp, _ = g.dc_adjoint(x, probe)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / reps
+    # one forward refactor + solve of the same matrix (GPU refactor path)
+    ap, ai, ax, rhs, n = g.stamp(x)
+    m = S.GpuSolver(n, S.SPARSELU)
+    m.factor(ap, ai, ax)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.factor(ap, ai, ax)
+        b = rhs.copy()
+        m.solve(b)
+    torch.cuda.synchronize()
+    t_rs = (time.perf_counter() - t0) / reps
+    return dict(workload="64x64 resistor mesh (n=4,098), dV(center)/dp for all 8,066 devices",
+                forward_dc_op_ms=1e3 * t_fwd, forward_refactor_solve_ms=1e3 * t_rs,
+                adjoint_ms=1e3 * t_adj, adjoint_over_dc_op=t_adj / t_fwd,
+                adjoint_over_refactor_solve=t_adj / t_rs, newton_iters=r.iterations,
+                sensitivities=int(len(dp)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +183,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--loop-sample", type=int, default=16)
     ap.add_argument("--host-steps", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -253,6 +293,8 @@ def main():
     )
     if args.batch > 0:
         line["batched"] = batched_dc_ops(args, E, NL, S, torch)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_cost(E, NL, S, torch)
     if not args.no_cpu:
         cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])
         xg, rg = g.dc_op(s)
