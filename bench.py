@@ -165,7 +165,7 @@ def adjoint_cost(E, NL, S, torch, reps=5):
         m.solve(b)
     torch.cuda.synchronize()
     t_rs = (time.perf_counter() - t0) / reps
-    return dict(workload="64x64 resistor mesh (n=4,098), dV(center)/dp for all 8,066 devices",
+    return dict(workload="64x64 resistor mesh (n=4,098), dV(center)/dp for all 12,162 devices (R, V, gmin R)",
                 forward_dc_op_ms=1e3 * t_fwd, forward_refactor_solve_ms=1e3 * t_rs,
                 adjoint_ms=1e3 * t_adj, adjoint_over_dc_op=t_adj / t_fwd,
                 adjoint_over_refactor_solve=t_adj / t_rs, newton_iters=r.iterations,
