@@ -171,6 +171,29 @@ int32_t rs_dc_op(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double *x
 int32_t rs_tran(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double tstep, double tstop,
                 double tstart, double tmax, int64_t max_points, double *times, double *sols,
                 int64_t x_stride, rs_tran_result *res);
+/* DC sweep summary (RunResult of engine.rs:684-820); status: 0 Converged,
+ * 1 MaxIters, 2 Failed (solver failure), 3 source index out of range
+ * ("DC sweep source not found"). */
+typedef struct rs_sweep_result {
+    int32_t status, iterations, factor_calls, pad;
+    int64_t npoints;  /* sweep values (RunResult.sweep_values)               */
+    int64_t nsolved;  /* points with a solution (RunResult.sweep_solutions)  */
+    double fail_value; /* sweep value that failed to converge (status 1, 2)  */
+} rs_sweep_result;
+/* Sweep values of Engine::run_dc_sweep_result (engine.rs:715-741): points
+ * start + i*step up to stop, plus stop itself when the last point falls more
+ * than half a step short. Returns the count; writes up to cap values. */
+int64_t rs_dc_sweep_points(double start, double stop, double step, double *values, int64_t cap);
+/* Engine::run_dc_sweep_result (engine.rs:684-820): for each sweep value, sets
+ * device `source`'s value (a PULSE source becomes a plain DC source, as the
+ * reference replaces the value text) and runs run_newton_with_stepping from
+ * the previous point's solution on one solver, prepared once. Stops at the
+ * first point that does not converge. The device keeps the last swept value,
+ * as the reference's instance does. Stores up to max_points values and
+ * solutions (x[x_stride], the full MNA vector of each converged point). */
+int32_t rs_dc_sweep(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, int64_t source,
+                    double start, double stop, double step, int64_t max_points, double *values,
+                    double *sols, int64_t x_stride, rs_sweep_result *res);
 /* Fixed-step trapezoidal transient (no reference counterpart): DC OP, then
  * one TRAP Newton solve per step of dt with the accepted-step state update.
  * status: 0 ok, 1 DC OP failed, 2 a step failed. */

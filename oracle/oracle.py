@@ -45,6 +45,11 @@ class TranResult(C.Structure):
                 ("dc_iters", C.c_int32), ("npoints", C.c_int64)]
 
 
+class SweepResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("npoints", C.c_int64),
+                ("nsolved", C.c_int64)]
+
+
 class Mna(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("ap", C.POINTER(C.c_int64)),
                 ("ai", C.POINTER(C.c_int64)), ("ax", C.POINTER(C.c_double)),
@@ -114,6 +119,11 @@ def lib():
                                      C.c_int64, vp, f64p, C.c_int64, C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32)]
         L.orc_lte_milne.argtypes = [f64p, f64p, C.c_int64, C.c_double, C.c_double, f64p]
+        L.orc_dc_sweep_points.restype = C.c_int64
+        L.orc_dc_sweep_points.argtypes = [C.c_double, C.c_double, C.c_double, vp, C.c_int64]
+        L.orc_dc_sweep.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(NewtonOpts), C.c_int64,
+                                   C.c_double, C.c_double, C.c_double, vp, C.c_int64, vp,
+                                   C.c_int64, C.POINTER(SweepResult)]
         L.orc_bbd_partition.restype = C.c_int64
         L.orc_bbd_partition.argtypes = [C.c_int64, i64p, i64p, C.c_int64, i64p]
         L.orc_bbd_decompose.argtypes = [C.c_int64, i64p, i64p, C.c_int64, C.POINTER(BbdDecomp)]
@@ -403,6 +413,29 @@ def tran_fixed(circ, dt, nsteps, opts=None, solver=None):
     rc = lib().orc_tran_fixed(_devptr(circ), circ.ndev, circ.node_count, C.byref(opts), dt, nsteps,
                               s.h, x, cap, C.byref(xl), C.byref(it))
     return rc, x[: xl.value].copy(), it.value
+
+
+def dc_sweep_points(start, stop, step):
+    n = lib().orc_dc_sweep_points(start, stop, step, None, 0)
+    v = np.zeros(max(n, 1))
+    lib().orc_dc_sweep_points(start, stop, step, v.ctypes.data_as(C.c_void_p), n)
+    return v[:n].copy()
+
+
+def dc_sweep(circ, source, start, stop, step, opts=None, solver=None):
+    """engine.rs:684-820 on device index `source`: dict(status, iterations,
+    values, sols[nsolved, stride])."""
+    opts = opts or default_opts()
+    s = solver or NativeKlu(circ.node_count)
+    values = dc_sweep_points(start, stop, step)
+    stride = circ.node_count + circ.ndev + 8
+    sols = np.zeros((max(len(values), 1), stride))
+    res = SweepResult()
+    lib().orc_dc_sweep(_devptr(circ), circ.ndev, circ.node_count, C.byref(opts), source, start,
+                       stop, step, s.h, len(values), sols.ctypes.data_as(C.c_void_p), stride,
+                       C.byref(res))
+    return dict(status=res.status, iterations=res.iterations, values=values,
+                nsolved=int(res.nsolved), sols=sols[: res.nsolved].copy())
 
 
 def lte_milne(xb, xt, abs_tol=1e-6, rel_tol=1e-3):

@@ -155,6 +155,21 @@ int orc_tran_fixed(const rs_device *dev, int64_t ndev, int64_t node_count,
                    orc_klu *solver, double *x_final, int64_t x_cap,
                    int64_t *x_len, int32_t *newton_iters_total);
 
+/* Sweep points of engine.rs:715-741; returns the count (values may be NULL). */
+int64_t orc_dc_sweep_points(double start, double stop, double step, double *values,
+                            int64_t cap);
+typedef struct {
+    int32_t status;       /* 0 Converged, 1 MaxIters, 2 Failed, 3 no such source */
+    int32_t iterations;   /* sum of the returned pass's iterations per point */
+    int64_t npoints, nsolved;
+} orc_sweep_result;
+/* Engine::run_dc_sweep_result (engine.rs:684-820) on device index `source`:
+ * sols[nsolved * x_stride]. */
+int orc_dc_sweep(const rs_device *dev, int64_t ndev, int64_t node_count,
+                 const rs_newton_opts *o, int64_t source, double start, double stop,
+                 double step, orc_klu *solver, int64_t max_points, double *sols,
+                 int64_t x_stride, orc_sweep_result *res);
+
 /* analysis.rs:147-200: out = {error_norm, max_node, accept, factor, raw_lte} */
 void orc_lte_milne(const double *xb, const double *xt, int64_t n, double abs_tol,
                    double rel_tol, double out[5]);

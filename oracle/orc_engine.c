@@ -392,3 +392,66 @@ int orc_tran_fixed(const rs_device *dev, int64_t ndev, int64_t node_count, const
     free(x); tstate_free(&str);
     return rc;
 }
+
+/* engine.rs:715-741: n = floor(|range/step|) + 1 points start + i*step, then
+ * stop appended when the last point misses it by more than half a step (and
+ * fewer than 10000 points were generated). A step below 1e-15 gives {start}. */
+int64_t orc_dc_sweep_points(double start, double stop, double step, double *values,
+                            int64_t cap) {
+    double st = stop >= start ? fabs(step) : -fabs(step);
+    int64_t cnt = 0;
+    double last = start;
+    if (fabs(st) < 1e-15) {
+        if (values && cap > 0) values[0] = start;
+        return 1;
+    }
+    int64_t np = (int64_t)floor(fabs((stop - start) / st)) + 1;
+    for (int64_t i = 0; i < np; i++) {
+        last = start + (double)i * st;
+        if (values && cnt < cap) values[cnt] = last;
+        cnt++;
+    }
+    if (fabs(last - stop) > 1e-12 && cnt < 10000 && fabs(last - stop) / fabs(st) > 0.5) {
+        if (values && cnt < cap) values[cnt] = stop;
+        cnt++;
+    }
+    return cnt;
+}
+
+/* engine.rs:684-820: one solver prepared with node_count, x carried from
+ * point to point, the swept value written into the instance (a PULSE source
+ * parses as a plain number afterwards), stop at the first failure. */
+int orc_dc_sweep(const rs_device *dev, int64_t ndev, int64_t node_count,
+                 const rs_newton_opts *o, int64_t source, double start, double stop,
+                 double step, orc_klu *solver, int64_t max_points, double *sols,
+                 int64_t x_stride, orc_sweep_result *res) {
+    memset(res, 0, sizeof(*res));
+    if (source < 0 || source >= ndev) { res->status = 3; return 3; }
+    int64_t np = orc_dc_sweep_points(start, stop, step, NULL, 0);
+    double *pts = (double *)orc_xmalloc((size_t)np * sizeof(double));
+    orc_dc_sweep_points(start, stop, step, pts, np);
+    res->npoints = np;
+    rs_device *d = (rs_device *)orc_xmalloc((size_t)(ndev ? ndev : 1) * sizeof(rs_device));
+    memcpy(d, dev, (size_t)ndev * sizeof(rs_device));
+    d[source].flags &= ~RS_F_WAVEFORM;
+    int64_t xl = node_count;
+    double *x = (double *)orc_xcalloc((size_t)(xl ? xl : 1), sizeof(double));
+    orc_klu_prepare(solver, node_count);
+    build_ctx bc = {d, ndev, node_count, ORC_MODE_DC, 0.0, 0.0, NULL};
+    for (int64_t k = 0; k < np; k++) {
+        d[source].value = pts[k];
+        orc_newton_result r = run_newton_stepping(o, &x, &xl, &bc, solver, NULL, NULL);
+        res->iterations += r.iterations;
+        if (r.converged) {
+            if (sols && res->nsolved < max_points)
+                for (int64_t q = 0; q < x_stride; q++)
+                    sols[res->nsolved * x_stride + q] = q < xl ? x[q] : 0.0;
+            res->nsolved++;
+        } else {
+            res->status = r.reason == 2 ? 2 : 1;
+            break;
+        }
+    }
+    free(x); free(d); free(pts);
+    return res->status;
+}

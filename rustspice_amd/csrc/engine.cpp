@@ -784,6 +784,81 @@ int32_t rs_tran(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double t
     return RS_OK;
 }
 
+int64_t rs_dc_sweep_points(double start, double stop, double step, double* values, int64_t cap) {
+    std::vector<double> v;
+    const double st = stop >= start ? std::fabs(step) : -std::fabs(step);
+    if (std::fabs(st) < 1e-15) {
+        v.push_back(start);
+    } else {
+        const double npt = std::floor(std::fabs((stop - start) / st)) + 1.0;
+        if (!(npt >= 1.0 && npt <= 1e8)) return -1;  // non-finite or absurd ranges
+        const int64_t np = (int64_t)npt;
+        v.reserve((size_t)np + 1);
+        for (int64_t i = 0; i < np; i++) v.push_back(start + (double)i * st);
+        const double last = v.back();
+        if (std::fabs(last - stop) > 1e-12 && v.size() < 10000 &&
+            std::fabs(last - stop) / std::fabs(st) > 0.5)
+            v.push_back(stop);
+    }
+    if (values)
+        for (int64_t i = 0; i < (int64_t)v.size() && i < cap; i++) values[i] = v[i];
+    return (int64_t)v.size();
+}
+
+int32_t rs_dc_sweep(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, int64_t source,
+                    double start, double stop, double step, int64_t max_points, double* values,
+                    double* sols, int64_t x_stride, rs_sweep_result* res) {
+    std::memset(res, 0, sizeof(*res));
+    if (source < 0 || source >= (int64_t)c->dev.size()) {
+        res->status = 3;
+        return cfail(c, RS_OK, "DC sweep source not found");
+    }
+    const int64_t np = rs_dc_sweep_points(start, stop, step, nullptr, 0);
+    if (np < 0) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: DC sweep range is not finite");
+    std::vector<double> pts((size_t)np);
+    rs_dc_sweep_points(start, stop, step, pts.data(), np);
+    res->npoints = np;
+    if (values)
+        for (int64_t i = 0; i < np && i < max_points; i++) values[i] = pts[i];
+    rs_device& d = c->dev[source];
+    if (d.flags & RS_F_WAVEFORM) {  // the DC pattern changes (a PULSE source is not stamped in DC)
+        d.flags &= ~RS_F_WAVEFORM;
+        c->pat[0].built = false;
+        c->pat[1].built = false;
+        for (auto*& b : c->dcb)
+            if (b) {
+                rs_batch_destroy(b);
+                b = nullptr;
+            }
+    }
+    std::vector<double> x(c->node_count, 0.0);
+    rs_prepare(sol, c->node_count);
+    const int64_t fc0 = c->factor_calls;
+    int32_t iters = 0, rc = RS_OK;
+    for (int64_t k = 0; k < np; k++) {
+        d.value = pts[k];
+        hipError_t e = hipMemcpy(c->d_dev.p + source, &d, sizeof(rs_device), hipMemcpyHostToDevice);
+        if (e) return chip(c, e, "hipMemcpy(device)");
+        rs_newton_result r{};
+        rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &r, nullptr);
+        if (rc) return rc;
+        iters += r.iterations;  // the returned pass only (engine.rs:771)
+        if (r.converged) {
+            if (sols && res->nsolved < max_points)
+                for (int64_t q = 0; q < x_stride; q++)
+                    sols[res->nsolved * x_stride + q] = q < (int64_t)x.size() ? x[q] : 0.0;
+            res->nsolved++;
+        } else {
+            res->status = r.reason == 2 ? 2 : 1;
+            res->fail_value = pts[k];
+            break;
+        }
+    }
+    res->iterations = iters;
+    res->factor_calls = (int32_t)(c->factor_calls - fc0);
+    return RS_OK;
+}
+
 int32_t rs_tran_fixed(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double dt,
                       int64_t nsteps, double* x_final, int64_t x_cap, int64_t* x_len,
                       int32_t* newton_iters, int32_t* status_out) {

@@ -46,6 +46,12 @@ class TranResult(C.Structure):
                 ("dc_iters", C.c_int32), ("npoints", C.c_int64)]
 
 
+class SweepResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("factor_calls", C.c_int32),
+                ("pad", C.c_int32), ("npoints", C.c_int64), ("nsolved", C.c_int64),
+                ("fail_value", C.c_double)]
+
+
 def _bind():
     L = _lib.lib()
     if getattr(L, "_engine_bound", False):
@@ -75,6 +81,11 @@ def _bind():
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32)]
     L.rs_circuit_set_device_newton.argtypes = [vp, C.c_int32]
+    L.rs_dc_sweep_points.restype = C.c_int64
+    L.rs_dc_sweep_points.argtypes = [C.c_double, C.c_double, C.c_double, vp, C.c_int64]
+    L.rs_dc_sweep.restype = C.c_int32
+    L.rs_dc_sweep.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_int64, C.c_double, C.c_double,
+                              C.c_double, C.c_int64, vp, vp, C.c_int64, C.POINTER(SweepResult)]
     L.rs_dc_adjoint.restype = C.c_int32
     L.rs_dc_adjoint.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_double, vp, vp]
     L.rs_batch_create.restype = vp
@@ -194,6 +205,25 @@ class GpuCircuit:
             out["times"] = times[:k].copy()
             out["sols"] = sols[: k * stride].reshape(k, stride).copy()
         return out
+
+    def dc_sweep(self, solver: GpuSolver, source: int, start, stop, step, opts=None):
+        """Engine::run_dc_sweep_result (engine.rs:684-820) over the value of
+        device `source`. Returns dict(status, iterations, factor_calls,
+        values, sols[nsolved, stride], fail_value); status 0 Converged,
+        1 MaxIters, 2 Failed, 3 source not found."""
+        opts = opts or default_opts()
+        n = self._L.rs_dc_sweep_points(start, stop, step, None, 0)
+        if n < 0:
+            raise ValueError("DC sweep range is not finite")
+        stride = self.circuit.node_count + self.circuit.ndev + 8
+        values = np.zeros(max(n, 1))
+        sols = np.zeros((max(n, 1), stride))
+        res = SweepResult()
+        self._check(self._L.rs_dc_sweep(self.h, solver.h, C.byref(opts), source, start, stop, step,
+                                        n, _ptr(values), _ptr(sols), stride, C.byref(res)))
+        return dict(status=res.status, iterations=res.iterations, factor_calls=res.factor_calls,
+                    values=values[: res.npoints].copy(), sols=sols[: res.nsolved].copy(),
+                    fail_value=res.fail_value)
 
     def tran_fixed(self, solver: GpuSolver, dt, nsteps, opts=None):
         opts = opts or default_opts()
