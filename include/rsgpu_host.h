@@ -52,7 +52,8 @@ int64_t rs_host_bbd_decompose(int64_t n, const int64_t *ap, const int64_t *ai, i
  * factor (pivots) on ax0, then the refactor on ax1 and the solve of rhs,
  * executed by the host model of the GPU step programs. x equals
  * NativeKluSolver factor(ax0); factor(ax1); solve(rhs) bit for bit.
- * stats[4] = {refactor steps, refactor ops, solve steps, solve ops}. */
+ * stats[8] = {refactor steps, refactor ops, solve steps, solve ops, refactor
+ * slow chunks, solve slow chunks, refactor forwarded ops, solve forwarded ops}. */
 int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t *ap, const int64_t *ai,
                             const double *ax0, const double *ax1, const double *rhs, double *x,
                             int64_t *stats);

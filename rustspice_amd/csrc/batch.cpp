@@ -19,12 +19,13 @@
 
 namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
-                             int* counter, const int32_t* ip, const uint64_t* prog,
+                             int* counter, const int32_t* ip, const SpOpWord* prog,
                              const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
                              const BOpts& o, int nmax, int lumax, int grid, hipStream_t s,
                              unsigned long long* phase_cyc);
 int batch_blocks_per_cu(int nmax, int lumax);
 size_t batch_lds_bytes(int nmax, int lumax);
+size_t batch_static_lds();
 }  // namespace rsgpu
 
 using namespace rsgpu;
@@ -40,7 +41,7 @@ struct rs_batch {
     DevBuf<BPat> d_pats;
     DevBuf<BInst> d_inst;
     DevBuf<int32_t> d_order, d_ip, d_iscr;
-    DevBuf<uint64_t> d_prog;
+    DevBuf<SpOpWord> d_prog;
     DevBuf<unsigned long long> d_phase;  // per-phase cycle totals when profiling
     bool phase_on = false;
     DevBuf<int> d_counter;
@@ -88,7 +89,7 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     b->kind = kind;
     b->ninst = ninst;
     std::vector<int32_t> pool;
-    std::vector<uint64_t> prog;
+    std::vector<SpOpWord> prog;
     std::vector<BPat> pats;
     std::map<std::vector<int64_t>, int> pat_id;  // key: node_count, n, ap, ai, maps
     b->inst.resize(ninst);
@@ -259,7 +260,7 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
             iscr += ph_nsc[pid];
             b->n_of[k] = P.n;
             b->nmax = std::max(b->nmax, P.n);
-            b->lumax = std::max(b->lumax, P.nslots);
+            b->lumax = std::max(b->lumax, P.nslots + 1);  // + the dummy slot
         }
     } catch (const std::exception& ex) {
         b->err = std::string("Invalid matrix: ") + ex.what();
@@ -288,7 +289,7 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
         (e = hipMemcpy(b->d_ip.p, pool.data(), pool.size() * sizeof(int32_t), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(b->d_dev.p, dev, ndev_total * sizeof(rs_device), hipMemcpyHostToDevice)) ||
         (e = hipMemset(b->d_scr.p, 0, scr * sizeof(double))) ||
-        (e = hipMemcpy(b->d_prog.p, prog.data(), prog.size() * sizeof(uint64_t), hipMemcpyHostToDevice))) {
+        (e = hipMemcpy(b->d_prog.p, prog.data(), prog.size() * sizeof(SpOpWord), hipMemcpyHostToDevice))) {
         bhip(b, e, "hipMemcpy(batch)");
         return b;
     }
@@ -300,7 +301,7 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     // wavefront, or when at least 4 fit per CU (waves per CU capped at 16).
     int lds_max = 0;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id);
-    const size_t kStatic = 64;  // s_qi, s_dxn, s_fn
+    const size_t kStatic = 0;  // the kernel has no static LDS
     if (batch_lds_bytes(b->nmax, 0) + kStatic > (size_t)lds_max) {
         char m[160];
         std::snprintf(m, sizeof m, "instance of %d unknowns exceeds the %d bytes of LDS per workgroup",
@@ -311,7 +312,9 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     const bool lds_fit = batch_lds_bytes(b->nmax, b->lumax) + kStatic <= (size_t)lds_max;
     int per_cu_g = std::min(16, batch_blocks_per_cu(b->nmax, 0));
     int per_cu_l = lds_fit ? std::min(16, batch_blocks_per_cu(b->nmax, b->lumax)) : 0;
-    b->vals_lds = lds_fit && (per_cu_l >= 4 || ninst <= (int64_t)std::max(cus, 1) * per_cu_l);
+    // (step programs need the values in LDS and address them absolutely)
+    b->vals_lds = lds_fit && batch_static_lds() == 0 &&
+                  (per_cu_l >= 4 || ninst <= (int64_t)std::max(cus, 1) * per_cu_l);
     int per_cu = b->vals_lds ? per_cu_l : per_cu_g;
     b->grid = (int)std::min<int64_t>(ninst, (int64_t)std::max(cus, 1) * per_cu);
     if (b->grid < 1) b->grid = 1;

@@ -34,7 +34,7 @@ struct BPat {
     // entry as (j, row, index into L values)
     int32_t o_up_ptr, o_up_j, o_up_row, o_up_l;
     int32_t nrhs, o_sc_col;
-    // step programs (sp.hpp): word offsets into the program pool, chunk counts
+    // step programs (sp.hpp): op-word offsets into the program pool, chunk counts
     int64_t o_spf, o_sps;
     int32_t spf_chunks, sps_chunks, sp_ok, t_base, od_base, nslots;
     // gather records (int4, 16-byte aligned): per scatter run and per RHS row

@@ -45,60 +45,117 @@ __device__ __forceinline__ void gsync() {
 }
 
 // ---- step programs (sp.hpp) --------------------------------------------------
-// One step: every lane reads its operands, then writes its result. Kinds that
-// need more than a multiply-subtract take a wavefront-uniform branch, so steps
-// without them do not pay for the division or the clamps.
-__device__ __forceinline__ void sp_step(uint64_t w, double* V, const int* RPI, int t_base) {
-    const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
-    const uint32_t kind = hi >> 16;
-    int d = (int)(lo & 0xFFFFu);
-    if (__ballot(kind == SP_ODSUB)) {
-        if (kind == SP_ODSUB) d = t_base + RPI[d];
-    }
-    const double a = V[d];
-    const double b = V[lo >> 16];
-    const double c = V[hi & 0xFFFFu];
-    double r = (kind == SP_UPD && c == 0.0) ? a : a - b * c;
-    if (__ballot(kind >= SP_DIV)) {
-        if (kind == SP_DIV) r = a / b;
-        if (kind == SP_DIAG) r = fabs(a) < 1e-14 ? (a >= 0.0 ? 1e-14 : -1e-14) : a;
-        if (kind == SP_DIAG1) r = fabs(a) < 1e-300 ? 1e-14 : a;
-    }
-    if (kind != SP_NOP) V[d] = r;
+// Operands of one step, read two steps ahead of its execution; d is the
+// destination's LDS byte address.
+struct SpIn {
+    double a, b, c;
+    uint32_t d;
+};
+
+// Step-program slots are addressed by their LDS byte address: the slots start
+// the kernel's dynamic LDS, and the kernel has no static LDS (checked on the
+// host before step programs are enabled), so slot byte offset = LDS address.
+typedef __attribute__((address_space(3))) double lds_double;
+__device__ __forceinline__ lds_double& vat(uint32_t b8) {
+    return *reinterpret_cast<lds_double*>(static_cast<size_t>(b8));
 }
 
-// Runs one program on the 64 lanes of this wavefront. Op words stream from
-// global memory through two register buffers: while one chunk of steps runs,
-// the next chunk's loads are in flight. Programs are padded to an even number
-// of chunks plus one chunk of no-ops, so the loop needs no bounds checks.
-__device__ __forceinline__ void sp_run(const uint64_t* __restrict__ prog, int nchunks, double* V, const int* RPI,
-                       int t_base, int lane) {
-    uint64_t qa[kSpChunk], qb[kSpChunk];
-    const uint64_t* p = prog + lane;
+// Issues the LDS reads of one step (w = {d8, b8, c8, flags}). An off-diagonal
+// update's destination is the pivot position of its row (wavefront-uniform
+// branch; kOd programs only).
+template <bool kOd>
+__device__ __forceinline__ void sp_fetch(const uint4& w, const int* RPI, int t_base, SpIn& f) {
+    uint32_t d = w.x;
+    if (kOd) {
+        const uint32_t kind = w.w & 7u;
+        if (__ballot(kind == SP_ODSUB)) {
+            if (kind == SP_ODSUB) d = (uint32_t)(t_base + RPI[d >> 3]) << 3;
+        }
+    }
+    f.d = d;
+    f.a = vat(d);
+    f.b = vat(w.y);
+    f.c = vat(w.z);
+}
+
+// Executes one step. Operand a (flag word < 0) or c (flag word > 0xFFFF) may
+// be the lane's previous result r. The plain op is UPD in a refactor program
+// (kUpd: skip when c == 0) and SUB in a solve program; kinds >= SP_DIV occur
+// in slow chunks only (kSlow), whose steps take a wavefront-uniform branch
+// for them. Every lane writes (idle lanes to the dummy slot).
+template <bool kUpd, bool kSlow>
+__device__ __forceinline__ void sp_exec(const uint4& w, const SpIn& f, double& r) {
+    const int32_t fl = (int32_t)w.w;
+    const double a = fl < 0 ? r : f.a;
+    const double c = fl > 0xFFFF ? r : f.c;
+    double x = a - f.b * c;
+    if (kUpd) x = c == 0.0 ? a : x;
+    if (kSlow) {
+        if (__ballot((fl & 7) >= (int32_t)SP_DIV)) {
+            const int kind = fl & 7;
+            if (kind == SP_DIV) x = a / c;
+            if (kind == SP_DIAG) x = fabs(a) < 1e-14 ? (a >= 0.0 ? 1e-14 : -1e-14) : a;
+            if (kind == SP_DIAG1) x = fabs(a) < 1e-300 ? 1e-14 : a;
+        }
+    }
+    r = x;
+    vat(f.d) = x;
+}
+
+// One chunk of kSpChunk steps from cur; steps 10 and 11 read ahead into nxt,
+// whose loads (the chunk after cur) go out after the first step. The empty
+// asm statements keep the buffer loads where they are written, so at the
+// chunk head only the loads that step needs are in flight.
+template <bool kUpd, bool kOd, bool kSlow>
+__device__ __forceinline__ void sp_chunk(const uint4 (&cur)[kSpChunk], uint4 (&nxt)[kSpChunk],
+                                         const uint4*& p, SpIn (&f)[3], double& r,
+                                         const int* RPI, int t_base) {
+    sp_fetch<kOd>(cur[2], RPI, t_base, f[2]);
+    sp_exec<kUpd, kSlow>(cur[0], f[0], r);
+    asm volatile("" ::: "memory");
+    p += kSpChunk * kW;
+#pragma unroll
+    for (int k = 0; k < kSpChunk; ++k) nxt[k] = p[k * kW];
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int k = 1; k < kSpChunk; ++k) {
+        sp_fetch<kOd>(k + 2 < kSpChunk ? cur[k + 2] : nxt[k + 2 - kSpChunk], RPI, t_base,
+                      f[(k + 2) % 3]);
+        sp_exec<kUpd, kSlow>(cur[k], f[k % 3], r);
+    }
+    asm volatile("" ::: "memory");
+}
+
+// Runs one program on the 64 lanes of this wavefront. Step s reads its
+// operands while step s - 2 executes (the scheduler keeps every LDS operand
+// kSpLag steps behind its producer, or forwards it), so the LDS latency
+// overlaps two steps of work. Op words stream from global memory through two
+// register buffers of kSpChunk steps; programs are padded to an even number
+// of chunks plus two chunks of idle ops, so the loop needs no bounds checks.
+// kSpChunk is a multiple of 3, which keeps the rotation of the three operand
+// sets in step with the loop. A chunk's kind (slow or not) is uniform: one
+// scalar branch per chunk.
+static_assert(kSpChunk % 3 == 0, "operand rotation");
+template <bool kUpd, bool kOd>
+__device__ __forceinline__ void sp_run(const uint4* __restrict__ prog, int nchunks,
+                                       const int* RPI, int t_base, int lane) {
+    uint4 qa[kSpChunk], qb[kSpChunk];
+    SpIn f[3];
+    double r = 0.0;
+    const uint4* p = prog + lane;
 #pragma unroll
     for (int k = 0; k < kSpChunk; ++k) qa[k] = p[k * kW];
-    // Buffer loads are issued a whole chunk of steps before their first use.
-    // The next chunk's loads go out only after the current chunk's first step,
-    // so at the loop head the only loads in flight are the ones that step needs
-    // (otherwise the compiler's wait at the head also covers the new loads).
-    // The empty asm statements keep the loads where they are written.
+    sp_fetch<kOd>(qa[0], RPI, t_base, f[0]);
+    sp_fetch<kOd>(qa[1], RPI, t_base, f[1]);
     for (int ch = 0; ch < nchunks; ch += 2) {
-        sp_step(qa[0], V, RPI, t_base);
-        asm volatile("" ::: "memory");
-        p += kSpChunk * kW;
-#pragma unroll
-        for (int k = 0; k < kSpChunk; ++k) qb[k] = p[k * kW];
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int k = 1; k < kSpChunk; ++k) sp_step(qa[k], V, RPI, t_base);
-        asm volatile("" ::: "memory");
-        p += kSpChunk * kW;
-#pragma unroll
-        for (int k = 0; k < kSpChunk; ++k) qa[k] = p[k * kW];
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int k = 0; k < kSpChunk; ++k) sp_step(qb[k], V, RPI, t_base);
-        asm volatile("" ::: "memory");
+        if (__builtin_amdgcn_readfirstlane(qa[0].w) & kSpSlowChunk)
+            sp_chunk<kUpd, kOd, true>(qa, qb, p, f, r, RPI, t_base);
+        else
+            sp_chunk<kUpd, kOd, false>(qa, qb, p, f, r, RPI, t_base);
+        if (__builtin_amdgcn_readfirstlane(qb[0].w) & kSpSlowChunk)
+            sp_chunk<kUpd, kOd, true>(qb, qa, p, f, r, RPI, t_base);
+        else
+            sp_chunk<kUpd, kOd, false>(qb, qa, p, f, r, RPI, t_base);
     }
 }
 
@@ -171,7 +228,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                                    const int32_t* __restrict__ order, int ninst,
                                                    int* __restrict__ counter,
                                                    const int32_t* __restrict__ ip,
-                                                   const uint64_t* __restrict__ prog,
+                                                   const SpOpWord* __restrict__ prog,
                                                    const rs_device* __restrict__ devs,
                                                    double* __restrict__ scr,
                                                    int32_t* __restrict__ iscr,
@@ -179,15 +236,13 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                                    int lumax,
                                                    unsigned long long* __restrict__ phase_cyc) {
     extern __shared__ double lds[];
-    double* W = lds;            // factor work vector; also the solve vector
+    double* VL = lds;           // L/U/diag values when lumax > 0 (step-program slots)
+    double* W = VL + lumax;     // factor work vector; also the solve vector
     double* X = W + nmax;
     double* XN = X + nmax;      // rhs, then x_new
     double* S = XN + nmax;      // staging of long ordered sums
-    double* VL = S + kStage;    // L/U/diag values when lumax > 0
-    int* RPI = reinterpret_cast<int*>(VL + lumax);
+    int* RPI = reinterpret_cast<int*>(S + kStage);
     int* RP = RPI + nmax;
-    __shared__ int s_qi;
-    __shared__ double s_dxn, s_fn;
     const int lane = threadIdx.x;
     // optional per-phase cycle accounting (measurement only)
     unsigned long long pc[kPhases] = {};
@@ -200,10 +255,9 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
     }
 
     for (;;) {
-        if (lane == 0) s_qi = atomicAdd(counter, 1);
-        __syncthreads();
-        const int qi = s_qi;
-        __syncthreads();
+        int qv = 0;
+        if (lane == 0) qv = atomicAdd(counter, 1);
+        const int qi = __shfl(qv, 0);
         if (qi >= ninst) {
             if (phase_cyc && lane == 0)
                 for (int k = 0; k < kPhases; ++k) atomicAdd(phase_cyc + k, pc[k]);
@@ -359,7 +413,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     PHASE(2)
                     // ---- factor (first: with pivot search; later: reuse) ----
                     if (spr) {  // refactor as a step program
-                        sp_run(prog + P.o_spf, P.spf_chunks, VL, RPI, P.t_base, lane);
+                        sp_run<true, false>(reinterpret_cast<const uint4*>(prog + P.o_spf), P.spf_chunks, RPI, P.t_base, lane);
                         wsync();
                         PHASE(4)
                     } else
@@ -469,7 +523,11 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                             VL[P.od_base + e] = sv_at(sv, ax_src[odax[e]]);
                         for (int i = lane; i < n; i += kW) VL[P.t_base + RPI[pin[i]]] = XN[i];
                         wsync();
-                        sp_run(prog + P.o_sps, P.sps_chunks, VL, RPI, P.t_base, lane);
+                        const uint4* sprog = reinterpret_cast<const uint4*>(prog + P.o_sps);
+                        if (P.nslots > P.od_base)
+                            sp_run<false, true>(sprog, P.sps_chunks, RPI, P.t_base, lane);
+                        else
+                            sp_run<false, false>(sprog, P.sps_chunks, RPI, P.t_base, lane);
                         wsync();
                         for (int i = lane; i < n; i += kW) XN[i] = VL[P.t_base + pout[i]];
                         wsync();
@@ -524,6 +582,7 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         if (!(fabs(dxi) <= o.abs_tol + o.rel_tol * fabs(XN[i]))) ok = 0;
                     }
                     const bool conv = __ballot(ok == 0) == 0ull;
+                    double dxn_l = 0.0, fn_l = 0.0;
                     if (lane == 0) {
                         // two ordered sums (newton.rs:98-101), reads batched 8 at a time
                         double sd = 0.0, sf = 0.0;
@@ -551,12 +610,11 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                             }
                             sf += XN[i] * XN[i];
                         }
-                        s_dxn = sqrt(sd);
-                        s_fn = sqrt(sf);
+                        dxn_l = sqrt(sd);
+                        fn_l = sqrt(sf);
                     }
-                    wsync();
-                    const double dxn = s_dxn;
-                    r.final_norm = s_fn;
+                    const double dxn = __shfl(dxn_l, 0);
+                    r.final_norm = __shfl(fn_l, 0);
                     if (conv) {
                         for (int i = lane; i < n; i += kW) X[i] = XN[i];
                         xl = n;
@@ -599,7 +657,7 @@ size_t lds_bytes(int nmax, int lumax) {
 namespace rsgpu {
 
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
-                             int* counter, const int32_t* ip, const uint64_t* prog,
+                             int* counter, const int32_t* ip, const SpOpWord* prog,
                              const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
                              const BOpts& o, int nmax, int lumax, int grid, hipStream_t s,
                              unsigned long long* phase_cyc) {
@@ -610,6 +668,15 @@ hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t
 }
 
 size_t batch_lds_bytes(int nmax, int lumax) { return lds_bytes(nmax, lumax); }
+
+// Static LDS of k_batch_dcop: step programs address their slots by absolute
+// LDS address, which is only right when this is 0.
+size_t batch_static_lds() {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_batch_dcop)) != hipSuccess)
+        return (size_t)-1;
+    return a.sharedSizeBytes;
+}
 
 int batch_blocks_per_cu(int nmax, int lumax) {
     int nb = 0;

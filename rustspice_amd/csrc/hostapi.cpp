@@ -183,7 +183,7 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
         }
         std::vector<int32_t> ss;
         sp_scatter_slots(ph, rpi.data(), ss);
-        std::vector<double> v(std::max<int32_t>(L.nslots, 1), 0.0);
+        std::vector<double> v((size_t)L.nslots + 1, 0.0);
         for (int32_t s = 0; s < (int32_t)ss.size(); s++) {
             if (ss[s] < 0) continue;
             double acc = 0.0;
@@ -200,6 +200,10 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
             stats[1] = fac.nops;
             stats[2] = sol.nsteps;
             stats[3] = sol.nops;
+            stats[4] = fac.nslow;
+            stats[5] = sol.nslow;
+            stats[6] = fac.nfwd;
+            stats[7] = sol.nfwd;
         }
         delete h;
         return RS_OK;

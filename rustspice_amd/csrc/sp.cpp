@@ -24,24 +24,35 @@ inline int reads_of(const SeqOp& o, uint32_t* slot, uint8_t* role, uint32_t pseu
     int k = 0;
     slot[k] = o.kind == SP_ODSUB ? pseudo_base + o.d : o.d;
     role[k++] = R_A;
-    if (o.kind == SP_UPD || o.kind == SP_SUB || o.kind == SP_ODSUB || o.kind == SP_DIV) {
+    if (o.kind == SP_UPD || o.kind == SP_SUB || o.kind == SP_ODSUB) {
         slot[k] = o.b;
         role[k++] = R_B;
     }
-    if (o.kind == SP_UPD || o.kind == SP_SUB || o.kind == SP_ODSUB) {
+    if (o.kind == SP_UPD || o.kind == SP_SUB || o.kind == SP_ODSUB || o.kind == SP_DIV) {
         slot[k] = o.c;
         role[k++] = R_C;
     }
     return k;
 }
 
-// List scheduling of a sequential op list onto 64 lockstep lanes.
-// extra[i] lists ops that must precede op i without forwarding (barriers).
+// Forwarding field for a dependency of the given roles: exactly one operand,
+// no barrier (a barrier orders against a write whose slot is not static).
+inline uint32_t fwd_of(uint8_t role) {
+    if (role & R_BAR) return SP_FWD_NONE;
+    switch (role & (R_A | R_B | R_C)) {
+        case R_A: return SP_FWD_A;
+        case R_C: return SP_FWD_C;
+        default: return SP_FWD_NONE;
+    }
+}
+
+// List scheduling of a sequential op list onto 64 lockstep lanes under the
+// read lag and forwarding rules of sp.hpp. extra[i] lists ops that must
+// precede op i without forwarding (barriers).
 void schedule(const std::vector<SeqOp>& ops, const std::vector<std::vector<int32_t>>& extra,
-              uint32_t nslots, uint32_t npseudo, SpProgram& out) {
+              uint32_t nslots, uint32_t npseudo, uint32_t idle_kind, SpProgram& out) {
     const int32_t m = (int32_t)ops.size();
     std::vector<std::vector<Pred>> preds(m);
-    std::vector<std::vector<int32_t>> succ(m);
     {
         const uint32_t total = nslots + npseudo;
         std::vector<int32_t> last_w(total, -1);
@@ -71,46 +82,144 @@ void schedule(const std::vector<SeqOp>& ops, const std::vector<std::vector<int32
             for (int32_t u : extra[i]) add(u, R_BAR);
         }
     }
+    std::vector<std::vector<Pred>> succ(m);
     for (int32_t i = 0; i < m; i++)
-        for (auto& p : preds[i]) succ[p.op].push_back(i);
-    // priority: longest path to a sink
+        for (auto& p : preds[i]) succ[p.op].push_back({i, p.role});
+    // priority: longest path to a sink; a forwardable edge costs one step, an
+    // LDS dependency kSpLag, an ordering-only (write-after-read) edge none
+    auto weight = [](uint8_t role) -> int32_t {
+        if (!(role & (R_A | R_B | R_C | R_BAR))) return 0;
+        return fwd_of(role) ? 1 : kSpLag;
+    };
     std::vector<int32_t> prio(m, 0);
     for (int32_t i = m - 1; i >= 0; i--)
-        for (int32_t s : succ[i]) prio[i] = std::max(prio[i], prio[s] + 1);
+        for (const Pred& s : succ[i]) prio[i] = std::max(prio[i], prio[s.op] + weight(s.role));
 
-    std::vector<int32_t> npend(m), ready, placed, keep;
+    std::vector<int32_t> npend(m), step(m, -1), lane(m, -1);
+    // per released op: earliest step without forwarding, and the one late
+    // producer it could forward from (step(p1) + 1, on lane(p1))
+    std::vector<int32_t> est(m, 0), p1(m, -1), fwd(m, 0);
+    std::vector<uint32_t> fkind(m, 0);
+    std::vector<int32_t> released, next;
+    auto release = [&](int32_t v) {
+        int32_t e = 0, best = -1, bestv = -1, second = 0;
+        for (const Pred& p : preds[v]) {
+            const int32_t t = (p.role & (R_A | R_B | R_C | R_BAR)) ? step[p.op] + kSpLag : step[p.op];
+            e = std::max(e, t);
+            if (t > bestv) {
+                second = std::max(second, bestv);
+                bestv = t;
+                best = p.op;
+            } else {
+                second = std::max(second, t);
+            }
+        }
+        est[v] = e;
+        p1[v] = -1;
+        if (best >= 0) {
+            uint8_t role = 0;
+            for (const Pred& p : preds[v])
+                if (p.op == best) role = p.role;
+            const uint32_t f = fwd_of(role);
+            if (f && second <= step[best] + 1) {
+                p1[v] = best;
+                fkind[v] = f;
+            }
+        }
+        released.push_back(v);
+    };
     for (int32_t i = 0; i < m; i++) {
         npend[i] = (int32_t)preds[i].size();
-        if (!npend[i]) ready.push_back(i);
+        if (!npend[i]) release(i);
     }
-    std::vector<std::vector<int32_t>> rows;  // per step: ops, lane = position
+    std::vector<std::vector<int32_t>> rows;  // per step: op on each lane (-1 none)
+    std::vector<int32_t> cf, cn;
     int32_t done = 0;
-    while (done < m) {
-        std::sort(ready.begin(), ready.end(), [&](int32_t x, int32_t y) {
-            return prio[x] != prio[y] ? prio[x] > prio[y] : x < y;
-        });
-        const size_t take = std::min<size_t>(ready.size(), kSpLanes);
-        placed.assign(ready.begin(), ready.begin() + take);
-        keep.assign(ready.begin() + take, ready.end());
-        for (int32_t v : placed)
-            for (int32_t w : succ[v])
-                if (--npend[w] == 0) keep.push_back(w);
-        ready.swap(keep);
+    int64_t nfwd = 0;
+    std::vector<uint8_t> slow_chunk;
+    auto is_slow = [&](int32_t v) { return ops[v].kind >= SP_DIV; };
+    for (int32_t s = 0; done < m; s++) {
+        std::vector<int32_t> row(kSpLanes, -1);
+        cf.clear();
+        cn.clear();
+        for (int32_t v : released) {
+            if (est[v] <= s) cn.push_back(v);
+            else if (p1[v] >= 0 && step[p1[v]] + 1 == s) cf.push_back(v);
+        }
+        // a chunk is slow when a slow op can start at its first step; fast
+        // chunks hold no slow ops
+        if (s % kSpChunk == 0) {
+            bool any = false;
+            for (int32_t v : cn) any = any || is_slow(v);
+            for (int32_t v : cf) any = any || is_slow(v);
+            slow_chunk.push_back(any ? 1 : 0);
+        }
+        if (!slow_chunk.back()) {
+            cn.erase(std::remove_if(cn.begin(), cn.end(), is_slow), cn.end());
+            cf.erase(std::remove_if(cf.begin(), cf.end(), is_slow), cf.end());
+        }
+        auto by_prio = [&](int32_t x, int32_t y) { return prio[x] != prio[y] ? prio[x] > prio[y] : x < y; };
+        std::sort(cf.begin(), cf.end(), by_prio);
+        std::sort(cn.begin(), cn.end(), by_prio);
+        std::vector<int32_t> placed;
+        for (int32_t v : cf) {
+            const int32_t l = lane[p1[v]];
+            if (row[l] >= 0) continue;
+            row[l] = v;
+            fwd[v] = (int32_t)fkind[v];
+            placed.push_back(v);
+        }
+        int32_t l = 0;
+        for (int32_t v : cn) {
+            while (l < kSpLanes && row[l] >= 0) l++;
+            if (l == kSpLanes) break;
+            row[l] = v;
+            fwd[v] = 0;
+            placed.push_back(v);
+        }
+        for (int32_t q = 0; q < kSpLanes; q++)
+            if (row[q] >= 0) {
+                step[row[q]] = s;
+                lane[row[q]] = q;
+            }
+        next.clear();
+        for (int32_t v : released)
+            if (step[v] < 0) next.push_back(v);
+        released.swap(next);
+        for (int32_t v : placed) {
+            if (fwd[v]) nfwd++;
+            for (const Pred& w : succ[v])
+                if (--npend[w.op] == 0) release(w.op);
+        }
         done += (int32_t)placed.size();
-        rows.push_back(placed);
+        rows.push_back(std::move(row));
     }
     out.nsteps = (int32_t)rows.size();
     out.nops = m;
-    // whole chunks, an even number of them, plus two chunks of no-ops that the
-    // kernel's double-buffered loop reads ahead
+    out.nfwd = nfwd;
+    // whole chunks, an even number of them (the kernel's loop runs two
+    // buffers), plus two chunks of idle ops that its read-ahead touches
     const int32_t chunks = ((out.nsteps + kSpChunk - 1) / kSpChunk + 1) / 2 * 2;
     const int32_t padded = (chunks + 2) * kSpChunk;
     out.nsteps_pad = padded;
-    out.words.assign((size_t)padded * kSpLanes, 0ull);
+    const uint32_t dummy = nslots;  // idle lanes: a SUB on the dummy slot
+    out.words.assign((size_t)padded * kSpLanes, sp_word(idle_kind, dummy, dummy, dummy));
     for (int32_t s = 0; s < out.nsteps; s++)
-        for (size_t l = 0; l < rows[s].size(); l++) {
-            const SeqOp& o = ops[rows[s][l]];
-            out.words[(size_t)s * kSpLanes + l] = sp_word(o.kind, o.d, o.b, o.c);
+        for (int32_t q = 0; q < kSpLanes; q++) {
+            const int32_t v = rows[s][q];
+            if (v < 0) continue;
+            const SeqOp& o = ops[v];
+            const bool rb = o.kind == SP_UPD || o.kind == SP_SUB || o.kind == SP_ODSUB;
+            const bool rc = rb || o.kind == SP_DIV;
+            out.words[(size_t)s * kSpLanes + q] =
+                sp_word(o.kind, o.d, rb ? o.b : dummy, rc ? o.c : dummy, (uint32_t)fwd[v]);
+        }
+    out.nslow = 0;
+    for (size_t c = 0; c < slow_chunk.size(); c++)
+        if (slow_chunk[c]) {
+            out.nslow++;
+            for (size_t i = c * kSpChunk * kSpLanes; i < (c + 1) * kSpChunk * kSpLanes; i++)
+                out.words[i].fl |= kSpSlowChunk;
         }
 }
 
@@ -136,8 +245,8 @@ SpLayout sp_layout(const PlanHost& p) {
 
 bool build_sp_programs(const PlanHost& p, SpProgram& fac, SpProgram& sol, std::string* why) {
     const SpLayout L = sp_layout(p);
-    if (L.nslots > 65535 || p.n > 65535) {
-        if (why) *why = "pattern has more than 65535 value slots";
+    if (L.nslots + 1 > kSpMaxSlots) {
+        if (why) *why = "pattern has more value slots than LDS can hold";
         return false;
     }
     const int32_t n = p.n;
@@ -164,13 +273,13 @@ bool build_sp_programs(const PlanHost& p, SpProgram& fac, SpProgram& sol, std::s
                 }
             }
             ops.push_back({SP_DIAG, dslot, 0, 0});
-            for (int32_t l = ls; l < le; l++) ops.push_back({SP_DIV, (uint32_t)l, dslot, 0});
+            for (int32_t l = ls; l < le; l++) ops.push_back({SP_DIV, (uint32_t)l, 0, dslot});
             for (int32_t u = us; u < ue; u++) pos2slot[p.uri[u]] = -1;
             pos2slot[k] = -1;
             for (int32_t l = ls; l < le; l++) pos2slot[p.lri[l]] = -1;
         }
         std::vector<std::vector<int32_t>> extra(ops.size());
-        schedule(ops, extra, (uint32_t)L.nslots, 0, fac);
+        schedule(ops, extra, (uint32_t)L.nslots, 0, SP_UPD, fac);
     }
     // ---- solve: blocks last to first; off-diagonal rows, L (cols ascending), U (cols descending)
     {
@@ -203,11 +312,11 @@ bool build_sp_programs(const PlanHost& p, SpProgram& fac, SpProgram& sol, std::s
                                    (uint32_t)(L.t_base + p.urc[e])});
                     extra.push_back(tails);
                 }
-                ops.push_back({SP_DIV, (uint32_t)(L.t_base + r), (uint32_t)(p.nl + p.ucp[r + 1] - 1), 0});
+                ops.push_back({SP_DIV, (uint32_t)(L.t_base + r), 0, (uint32_t)(p.nl + p.ucp[r + 1] - 1)});
                 extra.push_back(tails);
             }
         }
-        schedule(ops, extra, (uint32_t)L.nslots, (uint32_t)n, sol);
+        schedule(ops, extra, (uint32_t)L.nslots, (uint32_t)n, SP_SUB, sol);
     }
     return true;
 }
@@ -234,43 +343,53 @@ void sp_scatter_slots(const PlanHost& p, const int32_t* rpi, std::vector<int32_t
 }
 
 void sp_emulate(const SpProgram& prog, const SpLayout& L, double* v, const int32_t* rpi) {
-    uint32_t da[kSpLanes];
-    double a[kSpLanes], b[kSpLanes], c[kSpLanes];
-    for (int s = 0; s < prog.nsteps; s++) {
-        const uint64_t* w = &prog.words[(size_t)s * kSpLanes];
-        for (int l = 0; l < kSpLanes; l++) {  // all reads of the step
-            const uint32_t kind = (uint32_t)(w[l] >> 48) & 15u;
-            const uint32_t d = (uint32_t)w[l] & 0xFFFFu;
-            da[l] = kind == SP_ODSUB ? (uint32_t)(L.t_base + rpi[d]) : d;
-            a[l] = v[da[l]];
-            b[l] = v[(w[l] >> 16) & 0xFFFFu];
-            c[l] = v[(w[l] >> 32) & 0xFFFFu];
+    // operands of the steps s, s+1, s+2 in flight, as the kernel holds them
+    struct Fetched {
+        uint32_t d;
+        double a, b, c;
+    };
+    std::vector<Fetched> q[3];
+    for (auto& x : q) x.assign(kSpLanes, {0, 0.0, 0.0, 0.0});
+    double r[kSpLanes] = {};
+    auto at = [&](uint32_t b8) -> double& { return v[b8 / 8]; };
+    auto fetch = [&](int s, std::vector<Fetched>& f) {
+        if (s >= prog.nsteps_pad) return;
+        const SpOpWord* w = &prog.words[(size_t)s * kSpLanes];
+        for (int l = 0; l < kSpLanes; l++) {
+            f[l].d = sp_kind(w[l]) == SP_ODSUB ? (uint32_t)(L.t_base + rpi[w[l].d8 / 8]) * 8u : w[l].d8;
+            f[l].a = at(f[l].d);
+            f[l].b = at(w[l].b8);
+            f[l].c = at(w[l].c8);
         }
-        for (int l = 0; l < kSpLanes; l++) {  // then all writes
-            const uint32_t kind = (uint32_t)(w[l] >> 48) & 15u;
-            if (kind == SP_NOP) continue;
-            double r = a[l];
+    };
+    fetch(0, q[0]);
+    fetch(1, q[1]);
+    for (int s = 0; s < prog.nsteps; s++) {
+        fetch(s + 2, q[(s + 2) % 3]);  // issued before this step's writes
+        const SpOpWord* w = &prog.words[(size_t)s * kSpLanes];
+        const std::vector<Fetched>& f = q[s % 3];
+        for (int l = 0; l < kSpLanes; l++) {
+            const uint32_t kind = sp_kind(w[l]);
+            const uint32_t fw = sp_fwd(w[l]);
+            const double a = fw == SP_FWD_A ? r[l] : f[l].a;
+            const double b = f[l].b;
+            const double c = fw == SP_FWD_C ? r[l] : f[l].c;
+            double x = (kind == SP_UPD && c == 0.0) ? a : a - b * c;
             switch (kind) {
-                case SP_UPD:
-                    if (c[l] != 0.0) r = a[l] - b[l] * c[l];
-                    break;
-                case SP_SUB:
-                case SP_ODSUB:
-                    r = a[l] - b[l] * c[l];
-                    break;
                 case SP_DIV:
-                    r = a[l] / b[l];
+                    x = a / c;
                     break;
                 case SP_DIAG:
-                    if (std::fabs(a[l]) < 1e-14) r = a[l] >= 0.0 ? 1e-14 : -1e-14;
+                    x = std::fabs(a) < 1e-14 ? (a >= 0.0 ? 1e-14 : -1e-14) : a;
                     break;
                 case SP_DIAG1:
-                    if (std::fabs(a[l]) < 1e-300) r = 1e-14;
+                    x = std::fabs(a) < 1e-300 ? 1e-14 : a;
                     break;
                 default:
                     break;
             }
-            v[da[l]] = r;
+            r[l] = x;
+            at(f[l].d) = x;
         }
     }
 }

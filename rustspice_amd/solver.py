@@ -220,11 +220,12 @@ def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU):
     refactor on ax1, solve rhs. Returns (x, stats)."""
     L = _lib.lib()
     x = np.zeros(n)
-    st = np.zeros(4, np.int64)
+    st = np.zeros(8, np.int64)
     rc = L.rs_host_sp_selftest(kind, n, _i64(ap), _i64(ai), _f64(ax0), _f64(ax1), _f64(rhs), x, st)
     if rc:
         raise SolverError(rc, "step program")
-    keys = ["refactor_steps", "refactor_ops", "solve_steps", "solve_ops"]
+    keys = ["refactor_steps", "refactor_ops", "solve_steps", "solve_ops", "refactor_slow_chunks",
+            "solve_slow_chunks", "refactor_forwarded", "solve_forwarded"]
     return x, dict(zip(keys, map(int, st)))
 
 
