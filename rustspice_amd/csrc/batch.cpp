@@ -21,10 +21,10 @@ namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
                              int* counter, const int32_t* ip, const SpOpWord* prog,
                              const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
-                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s,
+                             const BOpts& o, int nmax, int lumax, int grid, int waves, hipStream_t s,
                              unsigned long long* phase_cyc);
-int batch_blocks_per_cu(int nmax, int lumax);
-size_t batch_lds_bytes(int nmax, int lumax);
+int batch_blocks_per_cu(int nmax, int lumax, int waves);
+size_t batch_lds_bytes(int nmax, int lumax, int waves);
 size_t batch_static_lds();
 }  // namespace rsgpu
 
@@ -50,6 +50,7 @@ struct rs_batch {
     DevBuf<BResult> d_res;
     std::string err;
     int grid = 0;
+    int waves = 1;  // wavefronts per instance
     double last_ms = 0.0;
 };
 
@@ -302,16 +303,19 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
     int lds_max = 0;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id);
     const size_t kStatic = 0;  // the kernel has no static LDS
-    if (batch_lds_bytes(b->nmax, 0) + kStatic > (size_t)lds_max) {
+    // Four wavefronts per instance when every instance has a CU of its own
+    // (the data-parallel phases spread over them), else one.
+    b->waves = ninst <= (int64_t)std::max(cus, 1) ? 4 : 1;
+    if (batch_lds_bytes(b->nmax, 0, b->waves) + kStatic > (size_t)lds_max) {
         char m[160];
         std::snprintf(m, sizeof m, "instance of %d unknowns exceeds the %d bytes of LDS per workgroup",
                       b->nmax, lds_max);
         b->err = m;
         return b;
     }
-    const bool lds_fit = batch_lds_bytes(b->nmax, b->lumax) + kStatic <= (size_t)lds_max;
-    int per_cu_g = std::min(16, batch_blocks_per_cu(b->nmax, 0));
-    int per_cu_l = lds_fit ? std::min(16, batch_blocks_per_cu(b->nmax, b->lumax)) : 0;
+    const bool lds_fit = batch_lds_bytes(b->nmax, b->lumax, b->waves) + kStatic <= (size_t)lds_max;
+    int per_cu_g = std::min(16, batch_blocks_per_cu(b->nmax, 0, b->waves));
+    int per_cu_l = lds_fit ? std::min(16, batch_blocks_per_cu(b->nmax, b->lumax, b->waves)) : 0;
     // (step programs need the values in LDS and address them absolutely)
     b->vals_lds = lds_fit && batch_static_lds() == 0 &&
                   (per_cu_l >= 4 || ninst <= (int64_t)std::max(cus, 1) * per_cu_l);
@@ -362,7 +366,8 @@ int32_t rs_batch_dc_op(rs_batch* b, const rs_newton_opts* o, void* stream) {
     const int ps = prof_begin(PK_BATCH, s);
     e = launch_batch_dcop(b->d_pats.p, b->d_inst.p, b->d_order.p, (int)b->ninst, b->d_counter.p,
                           b->d_ip.p, b->d_prog.p, b->d_dev.p, b->d_scr.p, b->d_iscr.p, b->d_res.p, bo, b->nmax,
-                          b->vals_lds ? b->lumax : 0, b->grid, s, b->phase_on ? b->d_phase.p : nullptr);
+                          b->vals_lds ? b->lumax : 0, b->grid, b->waves, s,
+                          b->phase_on ? b->d_phase.p : nullptr);
     prof_end(ps, s);
     if (e) return bhip(b, e, "k_batch_dcop");
     return RS_OK;

@@ -223,7 +223,24 @@ __device__ __forceinline__ int find_sorted(const int32_t* a, int lo, int hi, int
     return (lo < end && a[lo] == key) ? lo : -1;
 }
 
-__global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats,
+// Block-wide exchange for kWv wavefronts per instance (one: the wavefront
+// helpers above).
+template <int kWv>
+__device__ __forceinline__ void bsync() {
+    if (kWv == 1) wsync();
+    else __syncthreads();
+}
+template <int kWv>
+__device__ __forceinline__ void gbsync() {
+    if (kWv == 1) gsync();
+    else __syncthreads();
+}
+
+// kWv wavefronts per instance: the data-parallel phases (evaluation, gathers,
+// convergence test, vector updates) use all of them; the factor, the solve and
+// the ordered norms run on wavefront 0, whose step programs are latency-bound.
+template <int kWv>
+__global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict__ pats,
                                                    const BInst* __restrict__ insts,
                                                    const int32_t* __restrict__ order, int ninst,
                                                    int* __restrict__ counter,
@@ -240,10 +257,16 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
     double* W = VL + lumax;     // factor work vector; also the solve vector
     double* X = W + nmax;
     double* XN = X + nmax;      // rhs, then x_new
-    double* S = XN + nmax;      // staging of long ordered sums
-    int* RPI = reinterpret_cast<int*>(S + kStage);
+    double* S = XN + nmax;      // staging of long ordered sums, kStage per wavefront
+    double* SB = S + kStage * kWv;  // broadcast slots: 4 doubles, then 8 ints
+    int* SBi = reinterpret_cast<int*>(SB + 4);
+    // (kWv > 1) squares of dx and x_new for the ordered norms
+    double* Q = SB + 8;
+    int* RPI = reinterpret_cast<int*>(Q + (kWv > 1 ? 2 * nmax : 0));
     int* RP = RPI + nmax;
-    const int lane = threadIdx.x;
+    constexpr int kT = kW * kWv;
+    const int tid = threadIdx.x, lane = tid & (kW - 1), wave = tid / kW;
+    double* Sw = S + wave * kStage;
     // optional per-phase cycle accounting (measurement only)
     unsigned long long pc[kPhases] = {};
     unsigned long long t_last = 0;
@@ -255,11 +278,19 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
     }
 
     for (;;) {
-        int qv = 0;
-        if (lane == 0) qv = atomicAdd(counter, 1);
-        const int qi = __shfl(qv, 0);
+        int qi;
+        if (kWv == 1) {
+            int qv = 0;
+            if (lane == 0) qv = atomicAdd(counter, 1);
+            qi = __shfl(qv, 0);
+        } else {
+            if (tid == 0) SBi[0] = atomicAdd(counter, 1);
+            __syncthreads();
+            qi = SBi[0];
+            __syncthreads();
+        }
         if (qi >= ninst) {
-            if (phase_cyc && lane == 0)
+            if (phase_cyc && tid == 0)
                 for (int k = 0; k < kPhases; ++k) atomicAdd(phase_cyc + k, pc[k]);
             break;
         }
@@ -314,14 +345,14 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
         const bool sp = lumax > 0 && P.sp_ok;  // step programs (values in LDS)
         const int nsc = sc_ptr[n];
 
-        for (int i = lane; i < n; i += kW) {
+        for (int i = tid; i < n; i += kT) {
             W[i] = 0.0;
             X[i] = 0.0;
         }
         int xl = nc;  // the Newton x has node_count entries until convergence
         bool factored = false;
         BResult fin{0, 0, 1, 0, 0.0};
-        wsync();
+        bsync<kWv>();
         if (phase_cyc) t_last = __builtin_amdgcn_s_memtime();
 
         const double gstart = fmax(o.gmin * 1e3, 1e-6);
@@ -339,37 +370,49 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     PHASE(7)
                     // ---- stamp: evaluate, gather ax and the RHS contributions ----
                     StampArgs a{0.0, 0.0, gmin, scale, MODE_DC, 0};
-                    for (int d = lane; d < I.ndev; d += kW) {
-                        const rs_device v = dev[d];
-                        eval_device(v, d, X, xl, a, nullptr, nullptr, nullptr, nullptr,
-                                    sv + (size_t)d * kSlotsPerDevice);
+                    {  // device records loaded four at a time: one memory round trip per four
+                        int d = tid;
+                        for (; d + 3 * kT < I.ndev; d += 4 * kT) {
+                            rs_device v[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) v[j] = dev[d + j * kT];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                eval_device(v[j], d + j * kT, X, xl, a, nullptr, nullptr, nullptr,
+                                            nullptr, sv + (size_t)(d + j * kT) * kSlotsPerDevice);
+                        }
+                        for (; d < I.ndev; d += kT) {
+                            const rs_device v = dev[d];
+                            eval_device(v, d, X, xl, a, nullptr, nullptr, nullptr, nullptr,
+                                        sv + (size_t)d * kSlotsPerDevice);
+                        }
                     }
-                    gsync();
+                    gbsync<kWv>();
                     PHASE(0)
                     if (!factored)
-                        for (int i = lane; i < n; i += kW) {
+                        for (int i = tid; i < n; i += kT) {
                             RPI[i] = i;
                             RP[i] = i;
                         }
                     // RHS rows: short ones from their records, long ones cooperatively
-                    for (int base = 0; base < n; base += 4 * kW) {
+                    for (int base = 0; base < n; base += 4 * kT) {
                         int4 rr[4];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int i = base + j * kW + lane;
+                            const int i = base + j * kT + tid;
                             rr[j] = i < n ? rhsrec[i] : make_int4(0, 0, 0, 0);
                         }
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int i = base + j * kW + lane;
+                            const int i = base + j * kT + tid;
                             const double v = rec_sum(rr[j], sv);
                             if (i < n && rr[j].x <= 3) XN[i] = v;
                         }
                     }
-                    for (int j = 0; j < P.n_lrow; ++j) {
+                    for (int j = wave; j < P.n_lrow; j += kWv) {
                         const int i = lrow[j];
                         const int4 rr = rhsrec[i];
-                        const double v = coop_sum(rhs_src + rr.y, rr.x, sv, S, lane);
+                        const double v = coop_sum(rhs_src + rr.y, rr.x, sv, Sw, lane);
                         if (lane == 0) XN[i] = v;
                     }
                     PHASE(1)
@@ -377,21 +420,21 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     // refactor, else into rsum for the column-by-column factor
                     const bool spr = sp && factored;
                     if (spr) {
-                        for (int i = lane; i < P.nl + P.nu; i += kW) VL[i] = 0.0;
-                        wsync();
+                        for (int i = tid; i < P.nl + P.nu; i += kT) VL[i] = 0.0;
+                        bsync<kWv>();
                     }
-                    for (int base = 0; base < nsc; base += 4 * kW) {
+                    for (int base = 0; base < nsc; base += 4 * kT) {
                         int4 rr[4];
                         int tt[4];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int q = base + j * kW + lane;
+                            const int q = base + j * kT + tid;
                             rr[j] = q < nsc ? screc[q] : make_int4(0, 0, 0, 0);
                             tt[j] = (spr && q < nsc) ? ss[q] : -1;
                         }
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int q = base + j * kW + lane;
+                            const int q = base + j * kT + tid;
                             const double v = rec_sum(rr[j], sv);
                             if (q < nsc && rr[j].x <= 3) {
                                 if (!spr) rsum[q] = v;
@@ -399,24 +442,26 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                             }
                         }
                     }
-                    for (int j = 0; j < P.n_lrun; ++j) {
+                    for (int j = wave; j < P.n_lrun; j += kWv) {
                         const int q = lrun[j];
                         const int4 rr = screc[q];
-                        const double v = coop_sum(ax_src + rr.y, rr.x, sv, S, lane);
+                        const double v = coop_sum(ax_src + rr.y, rr.x, sv, Sw, lane);
                         if (lane == 0) {
                             if (!spr) rsum[q] = v;
                             else if (ss[q] >= 0) VL[ss[q]] = v;
                         }
                     }
-                    if (spr) wsync();
-                    else gsync();
+                    if (spr) bsync<kWv>();
+                    else gbsync<kWv>();
                     PHASE(2)
                     // ---- factor (first: with pivot search; later: reuse) ----
                     if (spr) {  // refactor as a step program
-                        sp_run<true, false>(reinterpret_cast<const uint4*>(prog + P.o_spf), P.spf_chunks, RPI, P.t_base, lane);
-                        wsync();
+                        if (wave == 0)
+                            sp_run<true, false>(reinterpret_cast<const uint4*>(prog + P.o_spf),
+                                                P.spf_chunks, RPI, P.t_base, lane);
+                        bsync<kWv>();
                         PHASE(4)
-                    } else
+                    } else if (wave == 0)
                     for (int g = 0; g < n; ++g) {
                         for (int s = sc_ptr[g] + lane; s < sc_ptr[g + 1]; s += kW)
                             W[RPI[sc_pre[s]]] = rsum[s];
@@ -495,11 +540,12 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         if (lumax) wsync();
                         else gsync();  // L/U values of this column are read by later columns
                     }
+                    if (!spr) gbsync<kWv>();  // the factor of wavefront 0 for all
                     if (!factored) {
                         PHASE(3)
                         r.full_factors++;
                         if (sp) {  // scatter slots under the pivots just found
-                            for (int s = lane; s < nsc; s += kW) {
+                            for (int s = tid; s < nsc; s += kT) {
                                 const int g = sc_col[s], pos = RPI[sc_pre[s]];
                                 const int ue = ucp[g + 1] - 1;
                                 int slot = -1;
@@ -513,25 +559,28 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                                 }
                                 ss[s] = slot;
                             }
-                            gsync();
+                            gbsync<kWv>();
                             PHASE(5)
                         }
                     }
                     factored = true;
                     if (sp) {  // ---- solve as a step program ----
-                        for (int e = lane; e < P.nslots - P.od_base; e += kW)
+                        for (int e = tid; e < P.nslots - P.od_base; e += kT)
                             VL[P.od_base + e] = sv_at(sv, ax_src[odax[e]]);
-                        for (int i = lane; i < n; i += kW) VL[P.t_base + RPI[pin[i]]] = XN[i];
-                        wsync();
-                        const uint4* sprog = reinterpret_cast<const uint4*>(prog + P.o_sps);
-                        if (P.nslots > P.od_base)
-                            sp_run<false, true>(sprog, P.sps_chunks, RPI, P.t_base, lane);
-                        else
-                            sp_run<false, false>(sprog, P.sps_chunks, RPI, P.t_base, lane);
-                        wsync();
-                        for (int i = lane; i < n; i += kW) XN[i] = VL[P.t_base + pout[i]];
-                        wsync();
+                        for (int i = tid; i < n; i += kT) VL[P.t_base + RPI[pin[i]]] = XN[i];
+                        bsync<kWv>();
+                        if (wave == 0) {
+                            const uint4* sprog = reinterpret_cast<const uint4*>(prog + P.o_sps);
+                            if (P.nslots > P.od_base)
+                                sp_run<false, true>(sprog, P.sps_chunks, RPI, P.t_base, lane);
+                            else
+                                sp_run<false, false>(sprog, P.sps_chunks, RPI, P.t_base, lane);
+                        }
+                        bsync<kWv>();
+                        for (int i = tid; i < n; i += kT) XN[i] = VL[P.t_base + pout[i]];
+                        bsync<kWv>();
                     } else {
+                    if (wave == 0) {
                     // ---- solve, in W (all zero now; re-zeroed afterwards) ----
                     double* T = W;
                     for (int i = lane; i < n; i += kW) T[RPI[pin[i]]] = XN[i];
@@ -573,17 +622,55 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                     wsync();
                     for (int i = lane; i < n; i += kW) W[i] = 0.0;
                     }
+                    bsync<kWv>();
+                    }
                     PHASE(6)
                     // ---- convergence, damping, update (newton.rs:93-107) ----
                     const int nd = n < xl ? n : xl;
                     int ok = 1;
-                    for (int i = lane; i < nd; i += kW) {
+                    for (int i = tid; i < nd; i += kT) {
                         const double dxi = XN[i] - X[i];
                         if (!(fabs(dxi) <= o.abs_tol + o.rel_tol * fabs(XN[i]))) ok = 0;
                     }
-                    const bool conv = __ballot(ok == 0) == 0ull;
+                    bool conv = __ballot(ok == 0) == 0ull;
+                    if (kWv > 1) {
+                        if (lane == 0) SBi[1 + wave] = conv ? 1 : 0;
+                        __syncthreads();
+                        for (int w = 0; w < kWv; ++w) conv = conv && SBi[1 + w] != 0;
+                    }
                     double dxn_l = 0.0, fn_l = 0.0;
-                    if (lane == 0) {
+                    if (kWv > 1) {
+                        // the squares in parallel; the two sums stay in order on one lane
+                        for (int i = tid; i < n; i += kT) {
+                            const double dxi = XN[i] - X[i];
+                            Q[i] = i < nd ? dxi * dxi : 0.0;
+                            Q[nmax + i] = XN[i] * XN[i];
+                        }
+                        __syncthreads();
+                        if (tid == 0) {
+                            double sd = 0.0, sf = 0.0;
+                            int i = 0;
+                            for (; i + 16 <= n; i += 16) {
+                                double qd[16], qf[16];
+#pragma unroll
+                                for (int j = 0; j < 16; ++j) {
+                                    qd[j] = Q[i + j];
+                                    qf[j] = Q[nmax + i + j];
+                                }
+#pragma unroll
+                                for (int j = 0; j < 16; ++j) {
+                                    if (i + j < nd) sd += qd[j];
+                                    sf += qf[j];
+                                }
+                            }
+                            for (; i < n; ++i) {
+                                if (i < nd) sd += Q[i];
+                                sf += Q[nmax + i];
+                            }
+                            dxn_l = sqrt(sd);
+                            fn_l = sqrt(sf);
+                        }
+                    } else if (tid == 0) {
                         // two ordered sums (newton.rs:98-101), reads batched 8 at a time
                         double sd = 0.0, sf = 0.0;
                         int i = 0;
@@ -613,14 +700,25 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         dxn_l = sqrt(sd);
                         fn_l = sqrt(sf);
                     }
-                    const double dxn = __shfl(dxn_l, 0);
-                    r.final_norm = __shfl(fn_l, 0);
+                    double dxn;
+                    if (kWv == 1) {
+                        dxn = __shfl(dxn_l, 0);
+                        r.final_norm = __shfl(fn_l, 0);
+                    } else {
+                        if (tid == 0) {
+                            SB[0] = dxn_l;
+                            SB[1] = fn_l;
+                        }
+                        __syncthreads();
+                        dxn = SB[0];
+                        r.final_norm = SB[1];
+                    }
                     if (conv) {
-                        for (int i = lane; i < n; i += kW) X[i] = XN[i];
+                        for (int i = tid; i < n; i += kT) X[i] = XN[i];
                         xl = n;
                         r.converged = 1;
                         r.reason = 0;
-                        wsync();
+                        bsync<kWv>();
                         break;
                     }
                     if (dxn > prev && damping > o.damping_min) {
@@ -628,28 +726,28 @@ __global__ __launch_bounds__(64) void k_batch_dcop(const BPat* __restrict__ pats
                         damping = d2 >= o.damping_min ? d2 : o.damping_min;
                     }
                     prev = dxn;
-                    for (int i = lane; i < nd; i += kW) X[i] = X[i] + damping * (XN[i] - X[i]);
-                    wsync();
+                    for (int i = tid; i < nd; i += kT) X[i] = X[i] + damping * (XN[i] - X[i]);
+                    bsync<kWv>();
                 }
                 fin = r;
             }
         }
         PHASE(7)
-        if (lane == 0) res[ii] = fin;
+        if (tid == 0) res[ii] = fin;
         if (fin.converged)
-            for (int i = lane; i < n; i += kW) scr[I.o_x + i] = X[i];
+            for (int i = tid; i < n; i += kT) scr[I.o_x + i] = X[i];
         if (lumax) {  // leave the factor in global scratch as well
-            for (int l = lane; l < P.nl; l += kW) scr[I.o_lx + l] = lx[l];
-            for (int u = lane; u < P.nu; u += kW) scr[I.o_ux + u] = ux[u];
-            for (int i = lane; i < n; i += kW) scr[I.o_ud + i] = ux[ucp[i + 1] - 1];
+            for (int l = tid; l < P.nl; l += kT) scr[I.o_lx + l] = lx[l];
+            for (int u = tid; u < P.nu; u += kT) scr[I.o_ux + u] = ux[u];
+            for (int i = tid; i < n; i += kT) scr[I.o_ud + i] = ux[ucp[i + 1] - 1];
         }
         __syncthreads();
     }
 }
 
-size_t lds_bytes(int nmax, int lumax) {
-    return (size_t)nmax * (3 * sizeof(double) + 2 * sizeof(int)) +
-           (size_t)(lumax + kStage) * sizeof(double);
+size_t lds_bytes(int nmax, int lumax, int waves) {
+    return (size_t)nmax * ((waves > 1 ? 5 : 3) * sizeof(double) + 2 * sizeof(int)) +
+           (size_t)(lumax + kStage * waves + 8) * sizeof(double);
 }
 
 }  // namespace
@@ -659,31 +757,42 @@ namespace rsgpu {
 hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t* order, int ninst,
                              int* counter, const int32_t* ip, const SpOpWord* prog,
                              const rs_device* devs, double* scr, int32_t* iscr, BResult* res,
-                             const BOpts& o, int nmax, int lumax, int grid, hipStream_t s,
+                             const BOpts& o, int nmax, int lumax, int grid, int waves, hipStream_t s,
                              unsigned long long* phase_cyc) {
-    hipLaunchKernelGGL(k_batch_dcop, dim3(grid), dim3(kW), lds_bytes(nmax, lumax), s, pats, insts,
-                       order, ninst, counter, ip, prog, devs, scr, iscr, res, o, nmax, lumax,
-                       phase_cyc);
+    if (waves == 4)
+        hipLaunchKernelGGL(k_batch_dcop<4>, dim3(grid), dim3(4 * kW), lds_bytes(nmax, lumax, 4), s,
+                           pats, insts, order, ninst, counter, ip, prog, devs, scr, iscr, res, o,
+                           nmax, lumax, phase_cyc);
+    else
+        hipLaunchKernelGGL(k_batch_dcop<1>, dim3(grid), dim3(kW), lds_bytes(nmax, lumax, 1), s,
+                           pats, insts, order, ninst, counter, ip, prog, devs, scr, iscr, res, o,
+                           nmax, lumax, phase_cyc);
     return hipGetLastError();
 }
 
-size_t batch_lds_bytes(int nmax, int lumax) { return lds_bytes(nmax, lumax); }
+size_t batch_lds_bytes(int nmax, int lumax, int waves) { return lds_bytes(nmax, lumax, waves); }
 
 // Static LDS of k_batch_dcop: step programs address their slots by absolute
 // LDS address, which is only right when this is 0.
 size_t batch_static_lds() {
-    hipFuncAttributes a{};
-    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_batch_dcop)) != hipSuccess)
-        return (size_t)-1;
-    return a.sharedSizeBytes;
+    size_t mx = 0;
+    for (const void* k : {reinterpret_cast<const void*>(k_batch_dcop<1>),
+                          reinterpret_cast<const void*>(k_batch_dcop<4>)}) {
+        hipFuncAttributes a{};
+        if (hipFuncGetAttributes(&a, k) != hipSuccess) return (size_t)-1;
+        mx = a.sharedSizeBytes > mx ? a.sharedSizeBytes : mx;
+    }
+    return mx;
 }
 
-int batch_blocks_per_cu(int nmax, int lumax) {
+int batch_blocks_per_cu(int nmax, int lumax, int waves) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop, kW, lds_bytes(nmax, lumax)) !=
-            hipSuccess ||
-        nb < 1)
-        nb = 1;
+    hipError_t e = waves == 4
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop<4>, 4 * kW,
+                                                                      lds_bytes(nmax, lumax, 4))
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_dcop<1>, kW,
+                                                                      lds_bytes(nmax, lumax, 1));
+    if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
 }
 

@@ -48,6 +48,22 @@ def test_batch_matches_oracle_per_instance(gpu, kind):
             assert same_bits(xg, xo), c.name
 
 
+def test_batch_one_wavefront_per_instance(gpu):
+    """More instances than CUs: one wavefront per instance (fewer than that
+    give each instance four). Both layouts must match the oracle."""
+    circs = NL.monte_carlo_chains(300, seed=21, lo=5, hi=30)
+    b = E.GpuBatch(circs, S.SPARSELU)
+    b.dc_op()
+    for c, (xg, rg) in zip(circs, b.results()):
+        o = O.NativeKlu(c.node_count)
+        o.set_sparselu()
+        xo, ro = O.dc_op(c, solver=o)
+        assert (rg.converged, rg.iterations, rg.reason) == (ro.converged, ro.iterations, ro.reason)
+        assert same_bits([rg.final_norm], [ro.final_norm])
+        if ro.converged:
+            assert same_bits(xg, xo)
+
+
 def test_batch_matches_single_circuit_backend(gpu):
     """The batched kernel and the plain GPU backend (rs_dc_op per circuit) give
     identical results."""
