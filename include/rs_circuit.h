@@ -42,15 +42,34 @@ enum rs_device_flags {
 };
 
 /* MOSFET parameter slots in rs_device.p (values after build_bsim_params,
- * crates/sim-devices/src/bsim/mod.rs:83-388, for the fields level 1 reads). */
+ * crates/sim-devices/src/bsim/mod.rs:83-388). Level 1 reads VTH0, U0, TOX,
+ * W and L; any other level (49 BSIM3, 54 BSIM4; others evaluate as BSIM3,
+ * bsim/mod.rs:430-472) reads its model parameters from the circuit's
+ * rs_bsim_model table entry p[RS_MP_MODEL], and W, L and TEMP. */
 enum rs_mos_param {
-    RS_MP_LEVEL = 0, /* model level; only 1 is evaluated on the device    */
-    RS_MP_VTH0 = 1,  /* vth0 / vto                                         */
+    RS_MP_LEVEL = 0, /* model level (`level`, default 49: stamp.rs:400)    */
+    RS_MP_VTH0 = 1,  /* vth0 / vto (level 1)                               */
     RS_MP_U0 = 2,    /* u0 [cm^2/Vs]; level 1 ignores KP (bsim/mod.rs:441) */
-    RS_MP_TOX = 3,   /* tox [m]                                            */
+    RS_MP_TOX = 3,   /* tox [m] (level 1)                                  */
     RS_MP_W = 4,     /* instance W [m]                                     */
-    RS_MP_L = 5      /* instance L [m]                                     */
+    RS_MP_L = 5,     /* instance L [m]                                     */
+    RS_MP_MODEL = 6, /* BSIM: index into the rs_bsim_model table           */
+    RS_MP_TEMP = 7   /* BSIM: instance `temp` [K] (default 300.15)         */
 };
+
+/* The BsimParams fields that evaluate_bsim_dc and evaluate_bsim4_dc read
+ * (crates/sim-devices/src/bsim/params.rs:13-303), after build_bsim_params,
+ * plus the instance's sa and sb (stamp.rs:419-421). pmos: type=pmos. */
+typedef struct rs_bsim_model {
+    double vth0, k1, k2, dvt0, dvt1, dvt2, eta0, dsub, drout, nlx, nfactor;
+    double u0, ua, ub, uc, vsat, pclm, tox, lint, wint, rdsw, prt;
+    double tnom, ute, kt1, kt1l, kt2;
+    double saref, sbref, ku0, kvth0, tku0;
+    double alpha0, alpha1, beta0, beta1;
+    double jtss, jtsd, vtss, vtsd, nsti;
+    double sa, sb;
+    int32_t pmos, pad;
+} rs_bsim_model;
 
 /* PULSE parameter slots in rs_device.p when RS_F_WAVEFORM is set
  * (waveform.rs:680-709 field order). */

@@ -135,6 +135,19 @@ typedef struct rs_tran_result {
 rs_circuit *rs_circuit_create(const rs_device *dev, int64_t ndev, int64_t node_count);
 void rs_circuit_destroy(rs_circuit *c);
 int32_t rs_circuit_last_error(const rs_circuit *c, char *buf, int64_t len);
+/* Model table of the circuit's BSIM MOSFETs (rs_device.p[RS_MP_LEVEL] != 1,
+ * indexed by p[RS_MP_MODEL]); copied to the device. Circuits with BSIM
+ * devices stamp on the host-driven Newton path: the reference stamps the
+ * gmbs, substrate and gate-current groups only above current thresholds
+ * (stamp.rs:441-509), so the pattern can change between iterations. */
+int32_t rs_circuit_set_bsim_models(rs_circuit *c, const rs_bsim_model *models, int64_t n);
+/* evaluate_bsim_dc / evaluate_bsim4_dc (bsim/evaluate.rs:77-279, 387-620;
+ * level 54 is BSIM4, any other level but 1 BSIM3, bsim/mod.rs:430-472) on
+ * the GPU for n bias points: bias[7*i..] = {w, l, vd, vg, vs, vb, temp},
+ * out[13*i..] = {ids, gm, gds, gmbs, ieq, region (0 cutoff, 1 linear,
+ * 2 saturation), vth, isub, gsub, igs, igd, gigs, gigd}. Host arrays. */
+int32_t rs_mos_eval(const rs_bsim_model *models, int64_t nmodels, const int32_t *model,
+                    const int32_t *level, const double *bias, int64_t n, double *out);
 /* Unmerged CSC pattern the stamp loop produces in `mode` (the engine closure
  * of engine.rs:262-283 / :485-501): call with ap = ai = NULL for the sizes. */
 int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap,
@@ -159,9 +172,13 @@ void rs_circuit_set_device_newton(rs_circuit *c, int32_t on);
 int32_t rs_circuit_set_cap_state(rs_circuit *c, const double *cap_v, const double *cap_i,
                                  const uint8_t *has_v, const uint8_t *has_i);
 /* One build(x) of the engine closure on the device: ax (unmerged, nnz) and
- * rhs (n) exactly as SparseBuilder::finalize and the RHS vector hold them. */
+ * rhs (n) exactly as SparseBuilder::finalize and the RHS vector hold them.
+ * *nnz and *n receive the sizes; when they exceed ax_cap / rhs_cap nothing is
+ * copied and RS_ERR_INVALID_MATRIX is returned (with BSIM devices the
+ * pattern depends on x). rs_circuit_pattern afterwards gives this pattern. */
 int32_t rs_stamp(rs_circuit *c, int32_t mode, const double *x, int64_t x_len, double t, double dt,
-                 double gmin, double source_scale, double *ax_out, double *rhs_out);
+                 double gmin, double source_scale, double *ax_out, int64_t ax_cap,
+                 double *rhs_out, int64_t rhs_cap, int64_t *nnz, int64_t *n);
 /* Engine::run_dc_result (engine.rs:249-311) with run_newton_with_stepping. */
 int32_t rs_dc_op(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double *x_out, int64_t x_cap,
                  int64_t *x_len, rs_newton_result *res);

@@ -119,6 +119,9 @@ def lib():
                                      C.c_int64, vp, f64p, C.c_int64, C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32)]
         L.orc_lte_milne.argtypes = [f64p, f64p, C.c_int64, C.c_double, C.c_double, f64p]
+        L.orc_set_bsim_models.argtypes = [vp, C.c_int64]
+        L.orc_bsim_eval.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, f64p]
         L.orc_dc_sweep_points.restype = C.c_int64
         L.orc_dc_sweep_points.argtypes = [C.c_double, C.c_double, C.c_double, vp, C.c_int64]
         L.orc_dc_sweep.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(NewtonOpts), C.c_int64,
@@ -320,8 +323,35 @@ def dense_factor_solve(n, ap, ai, ax, rhs, schur=False):
     return rc, x
 
 
+_MODELS_REF = None
+
+
+def set_bsim_models(models):
+    """The BSIM model table orc_build_mna uses (kept alive here)."""
+    global _MODELS_REF
+    if models is None or len(models) == 0:
+        _MODELS_REF = None
+        lib().orc_set_bsim_models(None, 0)
+        return
+    _MODELS_REF = np.ascontiguousarray(models)
+    lib().orc_set_bsim_models(_MODELS_REF.ctypes.data_as(C.c_void_p), len(_MODELS_REF))
+
+
 def _devptr(circ):
+    set_bsim_models(getattr(circ, "models", None))
     return circ.devices.ctypes.data_as(C.c_void_p)
+
+
+def bsim_eval(model, level, w, l, vd, vg, vs, vb, temp=300.15):
+    """evaluate_bsim_dc / evaluate_bsim4_dc of one bias point: dict."""
+    m = np.ascontiguousarray(np.array(model))
+    out = np.zeros(13)
+    lib().orc_bsim_eval(m.ctypes.data_as(C.c_void_p), int(level), w, l, vd, vg, vs, vb, temp, out)
+    keys = ["ids", "gm", "gds", "gmbs", "ieq", "region", "vth", "isub", "gsub", "igs", "igd",
+            "gigs", "gigd"]
+    r = dict(zip(keys, map(float, out)))
+    r["region"] = int(r["region"])
+    return r
 
 
 def build_mna(circ, x=None, mode=0, t=0.0, dt=0.0, tstate=None, gmin=1e-12,

@@ -119,6 +119,13 @@ void orc_mna_free(orc_mna *m);
 
 /* level-1 MOSFET (bsim/evaluate.rs:285-356 via bsim/mod.rs:441-455):
  * out = {ids, gm, gds, gmbs, ieq} */
+/* BSIM model table used by orc_build_mna for MOSFETs of level != 1 */
+void orc_set_bsim_models(const rs_bsim_model *m, int64_t n);
+const rs_bsim_model *orc_bsim_model(int64_t i);
+/* evaluate_bsim_dc / evaluate_bsim4_dc (level 54): out[13] = {ids, gm, gds,
+ * gmbs, ieq, region, vth, isub, gsub, igs, igd, gigs, gigd} */
+void orc_bsim_eval(const rs_bsim_model *m, int level, double w, double l, double vd, double vg,
+                   double vs, double vb, double temp, double out[13]);
 void orc_level1(const rs_device *d, double vd, double vg, double vs, double vb,
                 double out[5]);
 double orc_pulse_eval(const double p[7], double t);

@@ -234,11 +234,20 @@ void orc_build_mna(const rs_device *dev, int64_t ndev, int64_t node_count, const
             ctx_add_rhs(&b, a, -ieq); ctx_add_rhs(&b, c, ieq);
             break;
         }
-        case RS_DEV_M: { /* stamp.rs:389-553, level-1 branch */
+        case RS_DEV_M: { /* stamp.rs:389-553 */
             int64_t dr = d->node[0], ga = d->node[1], so = d->node[2], bu = d->node[3];
             double gmin = b.gmin > 0.0 ? b.gmin : 1e-12;
-            double o[5];
-            orc_level1(d, xget(x, xl, dr), xget(x, xl, ga), xget(x, xl, so), xget(x, xl, bu), o);
+            double vd = xget(x, xl, dr), vg = xget(x, xl, ga), vs = xget(x, xl, so), vb = xget(x, xl, bu);
+            int level = (int)d->p[RS_MP_LEVEL];
+            double o[13] = {0};
+            if (level == 1) {
+                orc_level1(d, vd, vg, vs, vb, o);
+            } else {
+                const rs_bsim_model *mm = orc_bsim_model((int64_t)d->p[RS_MP_MODEL]);
+                if (!mm) break; /* no model: the caller did not provide one */
+                orc_bsim_eval(mm, level, d->p[RS_MP_W], d->p[RS_MP_L], vd, vg, vs, vb,
+                              d->p[RS_MP_TEMP], o);
+            }
             double gm = o[1], gds = o[2] >= gmin ? o[2] : gmin, gmbs = o[3], ieq = o[4];
             ctx_add(&b, dr, dr, gds); ctx_add(&b, so, so, gds);
             ctx_add(&b, dr, so, -gds); ctx_add(&b, so, dr, -gds);
@@ -249,6 +258,27 @@ void orc_build_mna(const rs_device *dev, int64_t ndev, int64_t node_count, const
                 ctx_add(&b, so, bu, -gmbs); ctx_add(&b, so, so, gmbs);
             }
             ctx_add_rhs(&b, dr, -ieq); ctx_add_rhs(&b, so, ieq);
+            if (level == 54) { /* BSIM4 substrate and gate currents (stamp.rs:466-509) */
+                double isub = o[7], gsub = o[8], igs = o[9], igd = o[10], gigs = o[11], gigd = o[12];
+                if (fabs(isub) > gmin && gsub > gmin * 0.01) {
+                    ctx_add(&b, dr, dr, gsub); ctx_add(&b, bu, bu, gsub);
+                    ctx_add(&b, dr, bu, -gsub); ctx_add(&b, bu, dr, -gsub);
+                    double isub_eq = isub - gsub * (vd - vb);
+                    ctx_add_rhs(&b, dr, -isub_eq); ctx_add_rhs(&b, bu, isub_eq);
+                }
+                if (fabs(igs) > gmin && gigs > gmin * 0.01) {
+                    ctx_add(&b, ga, ga, gigs); ctx_add(&b, so, so, gigs);
+                    ctx_add(&b, ga, so, -gigs); ctx_add(&b, so, ga, -gigs);
+                    double igs_eq = igs - gigs * (vg - vs);
+                    ctx_add_rhs(&b, ga, -igs_eq); ctx_add_rhs(&b, so, igs_eq);
+                }
+                if (fabs(igd) > gmin && gigd > gmin * 0.01) {
+                    ctx_add(&b, ga, ga, gigd); ctx_add(&b, dr, dr, gigd);
+                    ctx_add(&b, ga, dr, -gigd); ctx_add(&b, dr, ga, -gigd);
+                    double igd_eq = igd - gigd * (vg - vd);
+                    ctx_add_rhs(&b, ga, -igd_eq); ctx_add_rhs(&b, dr, igd_eq);
+                }
+            }
             break;
         }
         default: break;

@@ -102,6 +102,10 @@ rs_batch* rs_batch_create(const rs_device* dev, const int64_t* dev_ptr, const in
         for (int64_t k = 0; k < ninst; k++) {
             const rs_device* d = dev + dev_ptr[k];
             const int64_t nd = dev_ptr[k + 1] - dev_ptr[k];
+            for (int64_t i = 0; i < nd; i++)
+                if (is_bsim(d[i]))
+                    throw std::runtime_error("BSIM devices (MOSFET level != 1) are not supported by "
+                                             "the batched kernel; use the circuit path");
             StampPattern sp;
             build_stamp_pattern(d, nd, node_count[k], MODE_DC, sp);
             std::vector<int64_t> key;

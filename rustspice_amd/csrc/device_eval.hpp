@@ -6,6 +6,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "bsim_eval.hpp"
 #include "stamp.hpp"
 
 namespace rsgpu {
@@ -30,6 +31,41 @@ __device__ __forceinline__ double pulse_eval(const double* p, double t) {
         return v2 - (v2 - v1) * tfall / tf;
     }
     return v1;
+}
+
+// BSIM MOSFET (level != 1): the stamp.rs:389-553 values into its kBsimSlots
+// block, and the mask of the conditional groups that are stamped.
+__device__ __forceinline__ uint8_t eval_bsim(const rs_device& v, const double* __restrict__ x,
+                                             int xl, const StampArgs& a, double* __restrict__ o) {
+    const double gmin = a.gmin > 0.0 ? a.gmin : 1e-12;
+    const int level = (int)v.p[RS_MP_LEVEL];
+    const rs_bsim_model& m = a.models[(int)v.p[RS_MP_MODEL]];
+    const double vd = xget(x, xl, v.node[0]), vg = xget(x, xl, v.node[1]);
+    const double vs = xget(x, xl, v.node[2]), vb = xget(x, xl, v.node[3]);
+    const BsimOut r = bsim_eval(m, level, v.p[RS_MP_W], v.p[RS_MP_L], vd, vg, vs, vb,
+                                v.p[RS_MP_TEMP]);
+    const double gm = r.gm, gds = r.gds >= gmin ? r.gds : gmin, gmbs = r.gmbs;
+    uint8_t mask = 0;
+    o[0] = gds; o[1] = gds; o[2] = -gds; o[3] = -gds;
+    o[4] = gm; o[5] = -gm; o[6] = -gm; o[7] = gm;
+    if (fabs(gmbs) > gmin * 0.01) mask |= 1;
+    o[8] = gmbs; o[9] = -gmbs; o[10] = -gmbs; o[11] = gmbs;
+    o[12] = -r.ieq; o[13] = r.ieq;
+    if (level == 54) {
+        if (fabs(r.isub) > gmin && r.gsub > gmin * 0.01) mask |= 2;
+        const double isub_eq = r.isub - r.gsub * (vd - vb);
+        o[14] = r.gsub; o[15] = r.gsub; o[16] = -r.gsub; o[17] = -r.gsub;
+        o[18] = -isub_eq; o[19] = isub_eq;
+        if (fabs(r.igs) > gmin && r.gigs > gmin * 0.01) mask |= 4;
+        const double igs_eq = r.igs - r.gigs * (vg - vs);
+        o[20] = r.gigs; o[21] = r.gigs; o[22] = -r.gigs; o[23] = -r.gigs;
+        o[24] = -igs_eq; o[25] = igs_eq;
+        if (fabs(r.igd) > gmin && r.gigd > gmin * 0.01) mask |= 8;
+        const double igd_eq = r.igd - r.gigd * (vg - vd);
+        o[26] = r.gigd; o[27] = r.gigd; o[28] = -r.gigd; o[29] = -r.gigd;
+        o[30] = -igd_eq; o[31] = igd_eq;
+    }
+    return mask;
 }
 
 // Evaluates device v (index d for its capacitor history) into o[0..9].

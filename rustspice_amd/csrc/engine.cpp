@@ -25,7 +25,10 @@
 namespace rsgpu {
 hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, StampArgs a,
                        const double* cap_v, const double* cap_i, const uint8_t* has_v,
-                       const uint8_t* has_i, double* sv, hipStream_t s);
+                       const uint8_t* has_i, double* sv, hipStream_t s, const int32_t* sbase,
+                       uint8_t* gmask);
+hipError_t launch_mos_eval(const rs_bsim_model* models, const int32_t* mi, const int32_t* lv,
+                           const double* in, int64_t n, double* out, hipStream_t s);
 hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
                            const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
                            hipStream_t s);
@@ -57,6 +60,7 @@ int32_t rs_batch_last_error(const rs_batch* b, char* buf, int64_t len);
 namespace {
 struct DevPattern {
     bool built = false;
+    std::vector<uint8_t> gmask;  // BSIM group masks the pattern was built for
     StampPattern p;
     DevBuf<int32_t> ax_src, rhs_ptr, rhs_src;
 };
@@ -81,6 +85,15 @@ struct rs_circuit {
     DevBuf<int32_t> d_aux, d_ltp, d_ltc, d_jap, d_jai;
     DevBuf<double> d_e, d_t, d_lam, d_dp, d_r, d_del, d_nrm;
     int32_t adj_refine = 0;  // refinement steps of the last rs_dc_adjoint
+    // BSIM devices: slot bases, model table, group masks of the last evaluation
+    int64_t nbsim = 0;
+    std::vector<int32_t> sbase;
+    std::vector<uint8_t> gmask;
+    DevBuf<int32_t> d_sbase;
+    DevBuf<uint8_t> d_gmask;
+    DevBuf<rs_bsim_model> d_models;
+    int64_t nmodels = 0;
+    int64_t pattern_switches = 0;  // patterns rebuilt because a BSIM group switched
 };
 
 namespace {
@@ -101,10 +114,17 @@ int32_t from_solver(rs_circuit* c, rs_solver* s, int32_t rc) {
 int32_t get_pattern(rs_circuit* c, int mode, DevPattern*& out) {
     int k = mode == MODE_DC ? 0 : 1;
     DevPattern& dp = c->pat[k];
+    if (dp.built && c->nbsim && dp.gmask != c->gmask) {  // a BSIM group switched
+        dp.built = false;
+        c->pattern_switches++;
+    }
     if (!dp.built) {
         try {
             build_stamp_pattern(c->dev.data(), (i64)c->dev.size(), c->node_count,
-                                k == 0 ? MODE_DC : MODE_TRAN_BE, dp.p);
+                                k == 0 ? MODE_DC : MODE_TRAN_BE, dp.p,
+                                c->nbsim ? c->sbase.data() : nullptr,
+                                c->nbsim ? c->gmask.data() : nullptr);
+            dp.gmask = c->gmask;
         } catch (const std::exception& ex) {
             return cfail(c, RS_ERR_INVALID_MATRIX, std::string("Invalid matrix: ") + ex.what());
         }
@@ -134,14 +154,24 @@ int32_t get_pattern(rs_circuit* c, int mode, DevPattern*& out) {
 int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t, double dt,
                   double gmin, double scale, DevPattern** pout, hipStream_t s, int32_t flags = 0) {
     DevPattern* dp;
-    int32_t rc = get_pattern(c, mode, dp);
-    if (rc) return rc;
-    StampArgs a{t, dt, gmin, scale, mode, flags};
+    int32_t rc;
+    if (c->nbsim && c->nmodels == 0)
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: BSIM devices without a model table");
+    if (c->nbsim && (flags & kStampJacobian))
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: no exact Jacobian for BSIM devices");
+    StampArgs a{t, dt, gmin, scale, mode, flags, c->d_models.p};
     int ps = prof_begin(PK_EVAL, s);
     hipError_t e = launch_eval(c->d_dev.p, (int)c->dev.size(), x_dev, (int)xl, a, c->d_cap_v.p,
-                               c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s);
+                               c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s,
+                               c->nbsim ? c->d_sbase.p : nullptr, c->d_gmask.p);
     prof_end(ps, s);
     if (e) return chip(c, e, "k_eval");
+    if (c->nbsim) {  // the stamp pattern depends on which BSIM groups are stamped
+        e = hipMemcpyAsync(c->gmask.data(), c->d_gmask.p, c->gmask.size(), hipMemcpyDeviceToHost, s);
+        if (!e) e = hipStreamSynchronize(s);
+        if (e) return chip(c, e, "hipMemcpy(group masks)");
+    }
+    if ((rc = get_pattern(c, mode, dp))) return rc;
     ps = prof_begin(PK_ASSEMBLE, s);
     e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p,
                         (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, s);
@@ -350,7 +380,7 @@ void lte_milne(const std::vector<double>& xb, const std::vector<double>& xt, dou
 constexpr int64_t kDeviceDcMaxN = 2048;
 
 bool can_run_device_dc(rs_circuit* c, rs_solver* sol) {
-    if (!c->device_newton) return false;
+    if (!c->device_newton || c->nbsim) return false;  // BSIM patterns change with the bias
     DevPattern* dp;
     if (get_pattern(c, MODE_DC, dp)) return false;
     if (dp->p.n == c->node_count) return false;
@@ -432,7 +462,23 @@ rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_c
     c->node_count = node_count;
     hipError_t e;
     size_t nd = (size_t)ndev;
-    if ((e = c->d_dev.ensure(nd)) || (e = c->d_sv.ensure(nd * kSlotsPerDevice)) ||
+    i64 nslots = nd * kSlotsPerDevice;
+    try {
+        nslots = stamp_slot_bases(dev, ndev, c->sbase);
+    } catch (const std::exception& ex) {
+        c->err = ex.what();
+        return c;
+    }
+    for (i64 d = 0; d < ndev; d++) c->nbsim += is_bsim(dev[d]) ? 1 : 0;
+    c->gmask.assign(c->nbsim ? nd : 0, 0);
+    if (c->nbsim &&
+        ((e = c->d_sbase.ensure(nd)) || (e = c->d_gmask.ensure(nd)) ||
+         (e = hipMemcpy(c->d_sbase.p, c->sbase.data(), nd * 4, hipMemcpyHostToDevice)) ||
+         (e = hipMemset(c->d_gmask.p, 0, nd)))) {
+        c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return c;
+    }
+    if ((e = c->d_dev.ensure(nd)) || (e = c->d_sv.ensure((size_t)nslots)) ||
         (e = c->d_cap_v.ensure(nd)) || (e = c->d_cap_i.ensure(nd)) || (e = c->d_has_v.ensure(nd)) ||
         (e = c->d_has_i.ensure(nd))) {
         c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
@@ -440,7 +486,7 @@ rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_c
     }
     if (nd && (e = hipMemcpy(c->d_dev.p, dev, nd * sizeof(rs_device), hipMemcpyHostToDevice)))
         c->err = std::string("hipMemcpy: ") + hipGetErrorString(e);
-    else if (nd && (e = hipMemset(c->d_sv.p, 0, nd * kSlotsPerDevice * 8)))
+    else if (nd && (e = hipMemset(c->d_sv.p, 0, (size_t)nslots * 8)))
         c->err = std::string("hipMemset: ") + hipGetErrorString(e);
     else
         reset_cap_state(c);
@@ -467,10 +513,45 @@ void rs_circuit_destroy(rs_circuit* c) {
     c->d_cap_i.release();
     c->d_has_v.release();
     c->d_has_i.release();
+    c->d_sbase.release();
+    c->d_gmask.release();
+    c->d_models.release();
     delete c;
 }
 
 void rs_circuit_set_device_newton(rs_circuit* c, int32_t on) { c->device_newton = on != 0; }
+
+int32_t rs_mos_eval(const rs_bsim_model* models, int64_t nmodels, const int32_t* model,
+                    const int32_t* level, const double* bias, int64_t n, double* out) {
+    if (n <= 0) return RS_OK;
+    for (int64_t i = 0; i < n; i++)
+        if (model[i] < 0 || model[i] >= nmodels || level[i] == 1) return RS_ERR_INVALID_MATRIX;
+    DevBuf<rs_bsim_model> dm;
+    DevBuf<int32_t> dmi, dlv;
+    DevBuf<double> din, dout;
+    hipError_t e;
+    if ((e = dm.ensure((size_t)nmodels)) || (e = dmi.ensure((size_t)n)) || (e = dlv.ensure((size_t)n)) ||
+        (e = din.ensure((size_t)n * 7)) || (e = dout.ensure((size_t)n * 13)) ||
+        (e = hipMemcpy(dm.p, models, (size_t)nmodels * sizeof(rs_bsim_model), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(dmi.p, model, (size_t)n * 4, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(dlv.p, level, (size_t)n * 4, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(din.p, bias, (size_t)n * 7 * 8, hipMemcpyHostToDevice)) ||
+        (e = launch_mos_eval(dm.p, dmi.p, dlv.p, din.p, n, dout.p, nullptr)) ||
+        (e = hipMemcpy(out, dout.p, (size_t)n * 13 * 8, hipMemcpyDeviceToHost)))
+        return RS_ERR_DEVICE;
+    return RS_OK;
+}
+
+int32_t rs_circuit_set_bsim_models(rs_circuit* c, const rs_bsim_model* models, int64_t n) {
+    for (const auto& d : c->dev)
+        if (is_bsim(d) && !(d.p[RS_MP_MODEL] >= 0.0 && d.p[RS_MP_MODEL] < (double)n))
+            return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: BSIM model index out of range");
+    hipError_t e = c->d_models.ensure((size_t)std::max<int64_t>(n, 1));
+    if (!e && n) e = hipMemcpy(c->d_models.p, models, (size_t)n * sizeof(rs_bsim_model), hipMemcpyHostToDevice);
+    if (e) return chip(c, e, "hipMemcpy(models)");
+    c->nmodels = n;
+    return RS_OK;
+}
 
 int32_t rs_circuit_last_error(const rs_circuit* c, char* buf, int64_t len) {
     if (len > 0) std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
@@ -502,7 +583,8 @@ int32_t rs_circuit_set_cap_state(rs_circuit* c, const double* cap_v, const doubl
 }
 
 int32_t rs_stamp(rs_circuit* c, int32_t mode, const double* x, int64_t x_len, double t, double dt,
-                 double gmin, double source_scale, double* ax_out, double* rhs_out) {
+                 double gmin, double source_scale, double* ax_out, int64_t ax_cap,
+                 double* rhs_out, int64_t rhs_cap, int64_t* nnz_out, int64_t* n_out) {
     DevPattern* dp;
     int32_t rc = get_pattern(c, mode, dp);
     if (rc) return rc;
@@ -512,6 +594,10 @@ int32_t rs_stamp(rs_circuit* c, int32_t mode, const double* x, int64_t x_len, do
     if (e) return chip(c, e, "hipMemcpy(x)");
     rc = stamp_dev(c, mode, c->d_x.p, x_len, t, dt, gmin, source_scale, &dp, nullptr);
     if (rc) return rc;
+    if (nnz_out) *nnz_out = (int64_t)dp->p.ai.size();
+    if (n_out) *n_out = dp->p.n;
+    if ((int64_t)dp->p.ai.size() > ax_cap || dp->p.n > rhs_cap)
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: stamp output buffers too small");
     if ((e = hipMemcpy(ax_out, c->d_ax.p, dp->p.ai.size() * 8, hipMemcpyDeviceToHost)) ||
         (e = hipMemcpy(rhs_out, c->d_rhs.p, dp->p.n * 8, hipMemcpyDeviceToHost)))
         return chip(c, e, "hipMemcpy(out)");
@@ -535,6 +621,7 @@ int32_t rs_dc_op(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double*
 
 int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t probe, double gmin,
                       double* dp_out, double* lambda_out) {
+    if (c->nbsim) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: the adjoint has no BSIM derivatives");
     DevPattern* dp;
     int32_t rc = get_pattern(c, MODE_DC, dp);
     if (rc) return rc;

@@ -13,8 +13,23 @@ struct Ins {
 };
 }  // namespace
 
+i64 stamp_slot_bases(const rs_device* dev, i64 ndev, std::vector<int32_t>& sbase) {
+    sbase.resize(ndev);
+    i64 next = ndev * kSlotsPerDevice;
+    for (i64 d = 0; d < ndev; d++) {
+        if (is_bsim(dev[d])) {
+            sbase[d] = (int32_t)next;
+            next += kBsimSlots;
+        } else {
+            sbase[d] = (int32_t)(d * kSlotsPerDevice);
+        }
+        if (next > INT32_MAX) throw std::runtime_error("too many devices");
+    }
+    return next;
+}
+
 void build_stamp_pattern(const rs_device* dev, i64 ndev, i64 node_count, int mode,
-                         StampPattern& out) {
+                         StampPattern& out, const int32_t* sbase, const uint8_t* gmask) {
     out = StampPattern();
     out.node_count = node_count;
     out.aux_of.assign(ndev, -1);
@@ -31,8 +46,8 @@ void build_stamp_pattern(const rs_device* dev, i64 ndev, i64 node_count, int mod
     const bool dc = mode == MODE_DC;
     for (i64 d = 0; d < ndev; d++) {
         const rs_device& v = dev[d];
-        const int32_t b = (int32_t)(d * kSlotsPerDevice);
-        if ((i64)b + kSlotsPerDevice > INT32_MAX) throw std::runtime_error("too many devices");
+        if (d * kSlotsPerDevice + kSlotsPerDevice > INT32_MAX) throw std::runtime_error("too many devices");
+        const int32_t b = sbase ? sbase[d] : (int32_t)(d * kSlotsPerDevice);
         const i64 a = v.node[0], c = v.node[1];
         switch (v.kind) {
         case RS_DEV_R:  // stamp.rs:242-259
@@ -58,11 +73,30 @@ void build_stamp_pattern(const rs_device* dev, i64 ndev, i64 node_count, int mod
             add_rhs(k, b + 4);
             break;
         }
-        case RS_DEV_M: {  // stamp.rs:389-553 level-1 branch (gmbs = 0: no bulk stamps)
-            const i64 dr = v.node[0], ga = v.node[1], so = v.node[2];
+        case RS_DEV_M: {  // stamp.rs:389-553
+            const i64 dr = v.node[0], ga = v.node[1], so = v.node[2], bu = v.node[3];
             add(dr, dr, b + 0); add(so, so, b + 1); add(dr, so, b + 2); add(so, dr, b + 3);
             add(dr, ga, b + 4); add(dr, so, b + 5); add(so, ga, b + 6); add(so, so, b + 7);
-            add_rhs(dr, b + 8); add_rhs(so, b + 9);
+            if (!is_bsim(v)) {  // level 1: gmbs = 0, no bulk stamps
+                add_rhs(dr, b + 8); add_rhs(so, b + 9);
+                break;
+            }
+            if (!sbase) throw std::runtime_error("BSIM device without a slot table");
+            const uint8_t gm = gmask ? gmask[d] : 0;
+            if (gm & 1) { add(dr, bu, b + 8); add(dr, so, b + 9); add(so, bu, b + 10); add(so, so, b + 11); }
+            add_rhs(dr, b + 12); add_rhs(so, b + 13);
+            if (gm & 2) {
+                add(dr, dr, b + 14); add(bu, bu, b + 15); add(dr, bu, b + 16); add(bu, dr, b + 17);
+                add_rhs(dr, b + 18); add_rhs(bu, b + 19);
+            }
+            if (gm & 4) {
+                add(ga, ga, b + 20); add(so, so, b + 21); add(ga, so, b + 22); add(so, ga, b + 23);
+                add_rhs(ga, b + 24); add_rhs(so, b + 25);
+            }
+            if (gm & 8) {
+                add(ga, ga, b + 26); add(dr, dr, b + 27); add(ga, dr, b + 28); add(dr, ga, b + 29);
+                add_rhs(ga, b + 30); add_rhs(dr, b + 31);
+            }
             break;
         }
         default:

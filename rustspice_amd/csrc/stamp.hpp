@@ -12,6 +12,8 @@
 // its value (ax is the reference's unmerged array, bit for bit), one thread
 // per row sums its RHS contributions from 0.0 in insertion order.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <vector>
 
@@ -26,6 +28,17 @@ enum StampMode { MODE_DC = 0, MODE_TRAN_BE = 1, MODE_TRAN_TRAP = 2 };
 // Fixed per-device slot block in the stamp-value array: matrix values first,
 // then RHS values.
 constexpr int kSlotsPerDevice = 10;
+// BSIM MOSFETs (level != 1) use a block of their own after all fixed blocks
+// (the circuit's slot base table points there), in stamp.rs:389-553 order:
+//   0-3 gds, 4-7 gm, 8-11 gmbs, 12-13 RHS (drain, source),
+//   BSIM4 only: 14-17 gsub, 18-19 RHS (drain, bulk), 20-23 gigs, 24-25 RHS
+//   (gate, source), 26-29 gigd, 30-31 RHS (gate, drain).
+// The gmbs and BSIM4 groups are stamped only when their currents exceed the
+// reference's thresholds; the evaluation reports which in a group mask (bit 0
+// gmbs, 1 substrate, 2 gate-source, 3 gate-drain), and the pattern of a
+// Newton iteration is the one built for the mask of every BSIM device.
+constexpr int kBsimSlots = 32;
+__host__ __device__ inline bool is_bsim(const rs_device& v) { return v.kind == RS_DEV_M && v.p[RS_MP_LEVEL] != 1.0; }
 
 struct StampPattern {
     i64 n = 0;          // matrix size: node_count + aux rows of this mode
@@ -38,9 +51,15 @@ struct StampPattern {
 };
 
 // Builds the pattern of one mode. DC skips capacitors and waveform sources
-// (stamp.rs:137, 261-300); TRAN stamps every device.
+// (stamp.rs:137, 261-300); TRAN stamps every device. sbase: slot base of
+// every device (NULL: d * kSlotsPerDevice, no BSIM devices); gmask: group
+// mask of every device (BSIM devices only; NULL: all groups off).
 void build_stamp_pattern(const rs_device* dev, i64 ndev, i64 node_count, int mode,
-                         StampPattern& out);
+                         StampPattern& out, const int32_t* sbase = nullptr,
+                         const uint8_t* gmask = nullptr);
+// Slot bases: fixed blocks first, then one kBsimSlots block per BSIM device
+// in device order. Returns the total slot count.
+i64 stamp_slot_bases(const rs_device* dev, i64 ndev, std::vector<int32_t>& sbase);
 
 // Device-side stamp parameters per call. flags & kStampJacobian: MOSFET
 // matrix values are the exact partial derivatives of the stamped current
@@ -50,6 +69,7 @@ struct StampArgs {
     double t, dt, gmin, source_scale;
     int32_t mode;
     int32_t flags;
+    const rs_bsim_model* models;  // BSIM model table (device memory) or NULL
 };
 
 }  // namespace rsgpu

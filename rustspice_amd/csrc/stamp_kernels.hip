@@ -26,11 +26,34 @@ __global__ __launch_bounds__(256) void k_eval(const rs_device* __restrict__ dev,
                                               const double* __restrict__ cap_i,
                                               const uint8_t* __restrict__ has_v,
                                               const uint8_t* __restrict__ has_i,
-                                              double* __restrict__ sv) {
+                                              double* __restrict__ sv,
+                                              const int32_t* __restrict__ sbase,
+                                              uint8_t* __restrict__ gmask) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndev) return;
     const rs_device v = dev[d];
+    if (sbase && rsgpu::is_bsim(v)) {
+        gmask[d] = rsgpu::eval_bsim(v, x, xl, a, sv + sbase[d]);
+        return;
+    }
     rsgpu::eval_device(v, d, x, xl, a, cap_v, cap_i, has_v, has_i, sv + (size_t)d * kSlotsPerDevice);
+}
+
+// Model evaluation for arrays of bias points (BSIM levels).
+__global__ __launch_bounds__(256) void k_mos_eval(const rs_bsim_model* __restrict__ models,
+                                                  const int32_t* __restrict__ mi,
+                                                  const int32_t* __restrict__ lv,
+                                                  const double* __restrict__ in, int64_t n,
+                                                  double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* q = in + 7 * i;  // w, l, vd, vg, vs, vb, temp
+    const rsgpu::BsimOut r = rsgpu::bsim_eval(models[mi[i]], lv[i], q[0], q[1], q[2], q[3], q[4],
+                                              q[5], q[6]);
+    double* o = out + 13 * i;
+    o[0] = r.ids; o[1] = r.gm; o[2] = r.gds; o[3] = r.gmbs; o[4] = r.ieq; o[5] = (double)r.region;
+    o[6] = r.vth; o[7] = r.isub; o[8] = r.gsub; o[9] = r.igs; o[10] = r.igd; o[11] = r.gigs;
+    o[12] = r.gigd;
 }
 
 __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax_src, int nnz,
@@ -75,12 +98,21 @@ __global__ __launch_bounds__(256) void k_cap_state(const rs_device* __restrict__
 
 namespace rsgpu {
 
+hipError_t launch_mos_eval(const rs_bsim_model* models, const int32_t* mi, const int32_t* lv,
+                           const double* in, int64_t n, double* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mos_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, models, mi,
+                       lv, in, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, StampArgs a,
                        const double* cap_v, const double* cap_i, const uint8_t* has_v,
-                       const uint8_t* has_i, double* sv, hipStream_t s) {
+                       const uint8_t* has_i, double* sv, hipStream_t s, const int32_t* sbase,
+                       uint8_t* gmask) {
     if (ndev <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_eval, dim3((ndev + 255) / 256), dim3(256), 0, s, dev, ndev, x, xl, a,
-                       cap_v, cap_i, has_v, has_i, sv);
+                       cap_v, cap_i, has_v, has_i, sv, sbase, gmask);
     return hipGetLastError();
 }
 

@@ -69,7 +69,8 @@ def _bind():
     L.rs_circuit_set_cap_state.argtypes = [vp] * 5
     L.rs_stamp.restype = C.c_int32
     L.rs_stamp.argtypes = [vp, C.c_int32, vp, C.c_int64, C.c_double, C.c_double, C.c_double,
-                           C.c_double, vp, vp]
+                           C.c_double, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64),
+                           C.POINTER(C.c_int64)]
     L.rs_dc_op.restype = C.c_int32
     L.rs_dc_op.argtypes = [vp, vp, C.POINTER(NewtonOpts), vp, C.c_int64, C.POINTER(C.c_int64),
                            C.POINTER(NewtonResult)]
@@ -81,6 +82,10 @@ def _bind():
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32)]
     L.rs_circuit_set_device_newton.argtypes = [vp, C.c_int32]
+    L.rs_circuit_set_bsim_models.restype = C.c_int32
+    L.rs_circuit_set_bsim_models.argtypes = [vp, vp, C.c_int64]
+    L.rs_mos_eval.restype = C.c_int32
+    L.rs_mos_eval.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int64, vp]
     L.rs_dc_sweep_points.restype = C.c_int64
     L.rs_dc_sweep_points.argtypes = [C.c_double, C.c_double, C.c_double, vp, C.c_int64]
     L.rs_dc_sweep.restype = C.c_int32
@@ -120,6 +125,11 @@ class GpuCircuit:
         self._dev = np.ascontiguousarray(circuit.devices)
         self.h = self._L.rs_circuit_create(_ptr(self._dev), circuit.ndev, circuit.node_count)
         self._check(0)
+        models = getattr(circuit, "models", None)
+        if models is not None and len(models):
+            self._models = np.ascontiguousarray(models)
+            self._check(self._L.rs_circuit_set_bsim_models(self.h, _ptr(self._models),
+                                                           len(self._models)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -167,15 +177,26 @@ class GpuCircuit:
         self._check(self._L.rs_circuit_set_cap_state(self.h, *[_ptr(a) for a in arrs]))
 
     def stamp(self, x=None, mode=MODE_DC, t=0.0, dt=0.0, gmin=1e-12, source_scale=1.0):
-        """One MNA build: returns (ap, ai, ax, rhs, n) like the engine closure."""
-        ap, ai = self.pattern(mode)
-        n = len(ap) - 1
+        """One MNA build: returns (ap, ai, ax, rhs, n) like the engine closure.
+        (With BSIM devices the pattern depends on x: it is read after the stamp.)"""
         x = np.ascontiguousarray(np.zeros(self.circuit.node_count) if x is None else x, np.float64)
-        ax = np.zeros(max(len(ai), 1))
-        rhs = np.zeros(max(n, 1))
-        self._check(self._L.rs_stamp(self.h, mode, _ptr(x), len(x), t, dt, gmin, source_scale,
-                                     _ptr(ax), _ptr(rhs)))
-        return ap, ai, ax[: len(ai)], rhs[:n], n
+        ap, ai = self.pattern(mode)
+        cap_ax, cap_n = len(ai), len(ap) - 1
+        nnz, n = C.c_int64(), C.c_int64()
+        for _ in range(2):
+            ax = np.zeros(max(cap_ax, 1))
+            rhs = np.zeros(max(cap_n, 1))
+            rc = self._L.rs_stamp(self.h, mode, _ptr(x), len(x), t, dt, gmin, source_scale,
+                                  _ptr(ax), cap_ax, _ptr(rhs), cap_n, C.byref(nnz), C.byref(n))
+            if rc == 0:
+                break
+            if nnz.value <= cap_ax and n.value <= cap_n:
+                self._check(rc)
+            cap_ax, cap_n = nnz.value, n.value
+        self._check(rc)
+        ap, ai = self.pattern(mode)
+        assert len(ai) == nnz.value and len(ap) - 1 == n.value
+        return ap, ai, ax[: nnz.value], rhs[: n.value], n.value
 
     def dc_op(self, solver: GpuSolver, opts=None):
         """Returns (x or None, NewtonResult)."""
@@ -233,6 +254,22 @@ class GpuCircuit:
         self._check(self._L.rs_tran_fixed(self.h, solver.h, C.byref(opts), dt, nsteps, _ptr(x), cap,
                                           C.byref(xl), C.byref(it), C.byref(st)))
         return st.value, x[: xl.value].copy(), it.value
+
+
+def mos_eval(models, model_idx, level, bias):
+    """rs_mos_eval: BSIM evaluation of n bias points on the GPU.
+    bias: (n, 7) = w, l, vd, vg, vs, vb, temp. Returns (n, 13)."""
+    L = _bind()
+    models = np.ascontiguousarray(models)
+    mi = np.ascontiguousarray(model_idx, dtype=np.int32)
+    lv = np.ascontiguousarray(level, dtype=np.int32)
+    b = np.ascontiguousarray(bias, dtype=np.float64)
+    n = len(mi)
+    out = np.zeros((n, 13))
+    rc = L.rs_mos_eval(_ptr(models), len(models), _ptr(mi), _ptr(lv), _ptr(b), n, _ptr(out))
+    if rc:
+        raise SolverError(rc, "rs_mos_eval")
+    return out
 
 
 class GpuBatch:

@@ -87,6 +87,7 @@ class Circuit:
     devices: np.ndarray  # DEVICE_DTYPE records
     node_count: int
     name: str = ""
+    models: np.ndarray | None = None  # BSIM_MODEL_DTYPE records for MOSFETs of level != 1
 
     @property
     def ndev(self) -> int:
@@ -102,7 +103,7 @@ class _Builder:
         pp = list(p or []) + [0.0] * (8 - len(p or []))
         self.rows.append((kind, flags, node, float(value), pp))
 
-    def build(self, node_count, name):
+    def build(self, node_count, name, models=None):
         arr = np.zeros(len(self.rows), dtype=DEVICE_DTYPE)
         for i, (k, f, nd, v, pp) in enumerate(self.rows):
             arr[i]["kind"] = k
@@ -110,12 +111,61 @@ class _Builder:
             arr[i]["node"] = nd
             arr[i]["value"] = v
             arr[i]["p"] = pp
-        return Circuit(arr, node_count, name)
+        if models is not None:
+            models = np.ascontiguousarray(np.array(list(models), dtype=BSIM_MODEL_DTYPE))
+        return Circuit(arr, node_count, name, models)
 
 
 def mos_params(vth0: float, u0: float, w: float, l: float, tox: float = TOX_DEFAULT):
     """p[] slots of a level-1 MOSFET (rs_mos_param)."""
     return [1.0, vth0, u0, tox, w, l]
+
+
+#: rs_bsim_model (include/rs_circuit.h): the BsimParams fields evaluate_bsim_dc
+#: and evaluate_bsim4_dc read, plus the instance's sa and sb
+BSIM_FIELDS = ("vth0 k1 k2 dvt0 dvt1 dvt2 eta0 dsub drout nlx nfactor "
+               "u0 ua ub uc vsat pclm tox lint wint rdsw prt "
+               "tnom ute kt1 kt1l kt2 saref sbref ku0 kvth0 tku0 "
+               "alpha0 alpha1 beta0 beta1 jtss jtsd vtss vtsd nsti sa sb").split()
+BSIM_MODEL_DTYPE = np.dtype([(f, np.float64) for f in BSIM_FIELDS] +
+                            [("pmos", np.int32), ("pad", np.int32)], align=True)
+assert BSIM_MODEL_DTYPE.itemsize == 43 * 8 + 8
+
+#: BsimParams::nmos_default (crates/sim-devices/src/bsim/params.rs:309-440)
+BSIM_NMOS_DEFAULT = dict(
+    vth0=0.7, k1=0.5, k2=0.0, dvt0=2.2, dvt1=0.53, dvt2=-0.032, eta0=0.08, dsub=0.56,
+    drout=0.56, nlx=1.74e-7, nfactor=1.0, u0=500.0, ua=2.25e-9, ub=5.87e-19, uc=-4.65e-11,
+    vsat=1.5e5, pclm=1.3, tox=1.5e-8, lint=0.0, wint=0.0, rdsw=0.0, prt=0.0, tnom=300.15,
+    ute=-1.5, kt1=-0.11, kt1l=0.0, kt2=0.022, saref=1e-6, sbref=1e-6, ku0=0.0, kvth0=0.0,
+    tku0=0.0, alpha0=0.0, alpha1=0.0, beta0=30.0, beta1=0.0, jtss=0.0, jtsd=0.0, vtss=10.0,
+    vtsd=10.0, nsti=1.0, sa=0.0, sb=0.0)
+#: BsimParams::pmos_default (params.rs:443-455): the fields it changes
+BSIM_PMOS_CHANGES = dict(vth0=-0.7, u0=150.0, ute=-1.0, kt1=-0.08)
+
+
+def bsim_model(pmos: bool = False, tnom_c: float | None = None, **overrides) -> np.ndarray:
+    """One rs_bsim_model record: the reference's defaults for the type, then the
+    given instance/model parameters as build_bsim_params (bsim/mod.rs:83-388)
+    applies them. `tnom_c` is the netlist's TNOM in Celsius (+273.15)."""
+    rec = np.zeros((), dtype=BSIM_MODEL_DTYPE)
+    vals = dict(BSIM_NMOS_DEFAULT)
+    if pmos:
+        vals.update(BSIM_PMOS_CHANGES)
+    if tnom_c is not None:
+        vals["tnom"] = tnom_c + 273.15
+    for k, v in overrides.items():
+        if k not in vals:
+            raise KeyError(k)
+        vals[k] = float(v)
+    for k, v in vals.items():
+        rec[k] = v
+    rec["pmos"] = 1 if pmos else 0
+    return rec
+
+
+def bsim_params(level: int, model: int, w: float, l: float, temp: float = 300.15):
+    """p[] slots of a BSIM MOSFET (rs_mos_param): level 49 BSIM3, 54 BSIM4."""
+    return [float(level), 0.0, 0.0, 0.0, w, l, float(model), temp]
 
 
 def u0_from_kp(kp: float, tox: float = TOX_DEFAULT) -> float:
