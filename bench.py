@@ -157,6 +157,9 @@ def adjoint_cost(E, NL, S, torch, reps=5):
     ap, ai, ax, rhs, n = g.stamp(x)
     m = S.GpuSolver(n, S.SPARSELU)
     m.factor(ap, ai, ax)
+    m.factor(ap, ai, ax)  # the first refactor times the launch shapes
+    b = rhs.copy()
+    m.solve(b)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -170,6 +173,49 @@ def adjoint_cost(E, NL, S, torch, reps=5):
                 adjoint_ms=1e3 * t_adj, adjoint_over_dc_op=t_adj / t_fwd,
                 adjoint_over_refactor_solve=t_adj / t_rs, newton_iters=r.iterations,
                 sensitivities=int(len(dp)))
+
+
+def mesh_refactor_solve(E, NL, S, torch, np, reps=10):
+    """Refactor + solve of the 64x64 mesh DC matrix (n=4,098), NativeKLU
+    semantics, inputs resident in HBM, with the autotuned launch shapes."""
+    c = NL.resistor_mesh(64, 64)
+    g = E.GpuCircuit(c)
+    ap, ai, ax, rhs, n = g.stamp(np.zeros(c.node_count))
+    s = S.GpuSolver(n, S.NATIVEKLU)
+    s.factor(ap, ai, ax)
+    d_ax = torch.tensor(ax, dtype=torch.float64, device="cuda")
+    d_b = torch.tensor(rhs, dtype=torch.float64, device="cuda")
+    d_x = torch.empty_like(d_b)
+    s.refactor_device(d_ax)  # the first refactor and solve time the launch shapes
+    s.solve_device(d_b, d_x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.refactor_device(d_ax)
+        s.solve_device(d_b, d_x)
+    torch.cuda.synchronize()
+    t_gpu = (time.perf_counter() - t0) / reps
+    res = dict(workload="64x64 resistor mesh DC matrix (n=4,098), refactor + solve, NativeKLU semantics",
+               gpu_ms=1e3 * t_gpu, launch_shape=s.launch_shape())
+    return res, (ap, ai, ax, rhs, n, d_x.cpu().numpy())
+
+
+def cpu_baseline_mesh(mesh, np, reps=10):
+    """The oracle's refactor + solve of the same mesh matrix on one host core,
+    and whether the GPU result is bit-identical to it."""
+    from oracle import oracle as O
+
+    ap, ai, ax, rhs, n, xg = mesh
+    o = O.NativeKlu(n)
+    o.factor(ap, ai, ax)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.factor(ap, ai, ax)
+        bb = rhs.copy()
+        o.solve(bb)
+    t_cpu = (time.perf_counter() - t0) / reps
+    return dict(cpu_oracle_ms=1e3 * t_cpu, cores=1, kind="port",
+                bit_identical=bool(np.array_equal(xg.view(np.int64), bb.view(np.int64))))
 
 
 def main():
@@ -295,6 +341,7 @@ def main():
         line["batched"] = batched_dc_ops(args, E, NL, S, torch)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_cost(E, NL, S, torch)
+        line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
     if not args.no_cpu:
         cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])
         xg, rg = g.dc_op(s)
@@ -303,6 +350,8 @@ def main():
                                            rg.iterations == par["oracle_dc_iters"])
         line["cpu_baseline"] = cb
         line["parity"] = par
+        if not args.no_adjoint:
+            line["mesh_refactor_solve"]["cpu_baseline"] = cpu_baseline_mesh(mesh, np)
     print(json.dumps(line))
 
 

@@ -30,7 +30,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_refactor(const PlanDev* __restrict__ descs,
+__global__ __launch_bounds__(1024) void k_refactor(const PlanDev* __restrict__ descs,
                                                   const int32_t* __restrict__ ip,
                                                   const double* __restrict__ axpool,
                                                   double* __restrict__ vpool,
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_refactor(const PlanDev* __restrict__ de
 }
 
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_solve(const PlanDev* __restrict__ descs,
+__global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ descs,
                                                const int32_t* __restrict__ ip,
                                                const double* __restrict__ axpool,
                                                const double* __restrict__ vpool,

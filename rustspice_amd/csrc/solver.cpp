@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -125,6 +127,36 @@ void compute_rcond(rs_solver* h, const double* ud, i64 n) {
     h->rcond = mx > 0.0 ? mn / mx : 0.0;
 }
 
+// Tuned launch shapes per pattern (FNV-1a of n, ap, ai, the solver
+// semantics and the factor's size and level count), shared by every solver of the process: a pattern re-analyzed
+// after prepare() or met by another handle is not timed again.
+struct TunedShapes {
+    int ref_threads = 0, sol_threads = 0;
+    bool ref_lds = false, sol_lds = false;
+    float ref_us = 0.0f, sol_us = 0.0f;
+    bool have_ref = false, have_sol = false;
+};
+std::mutex g_tune_mu;
+std::map<uint64_t, TunedShapes> g_tuned;
+
+uint64_t pattern_key(const rs_solver* h) {
+    uint64_t x = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        for (int b = 0; b < 8; b++) {
+            x ^= (v >> (8 * b)) & 0xFFu;
+            x *= 1099511628211ull;
+        }
+    };
+    mix((uint64_t)h->n);
+    mix((uint64_t)h->kind);
+    mix((uint64_t)h->ph.nl);  // the pivots chosen by the first factor shape the plan
+    mix((uint64_t)h->ph.nu);
+    mix((uint64_t)h->ph.nflev);
+    for (i64 v : h->last_ap) mix((uint64_t)v);
+    for (i64 v : h->last_ai) mix((uint64_t)v);
+    return x;
+}
+
 int32_t upload_plan(rs_solver* h, i64 nnz) {
     std::vector<int32_t> pool;
     PlanDev d{};
@@ -136,10 +168,34 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
     d.o_ax = 0;
     d.o_vec = 0;
     d.o_work = 0;
-    d.o_tvec = (int64_t)(kThreads / 64) * n;
+    d.o_tvec = (int64_t)(kMaxThreads / 64) * n;  // work vectors for up to 16 wavefronts
     size_t maxlds = device_max_lds();
-    h->lds_refactor = (size_t)(kThreads / 64) * n * sizeof(double) <= maxlds;
-    h->lds_solve = (size_t)n * sizeof(double) <= maxlds;
+    if (!h->forced) {
+        h->ref_threads = h->sol_threads = kThreads;
+        h->lds_refactor = (size_t)(kThreads / 64) * n * sizeof(double) <= maxlds;
+        h->lds_solve = (size_t)n * sizeof(double) <= maxlds;
+    }
+    h->tuned_ref = h->tuned_sol = false;
+    h->tune_key = pattern_key(h);
+    if (!h->forced) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(h->tune_key);
+        if (it != g_tuned.end()) {
+            const TunedShapes& t = it->second;
+            if (t.have_ref) {
+                h->ref_threads = t.ref_threads;
+                h->lds_refactor = t.ref_lds;
+                h->tune_ref_us = t.ref_us;
+                h->tuned_ref = true;
+            }
+            if (t.have_sol) {
+                h->sol_threads = t.sol_threads;
+                h->lds_solve = t.sol_lds;
+                h->tune_sol_us = t.sol_us;
+                h->tuned_sol = true;
+            }
+        }
+    }
     hipError_t e;
     if ((e = h->d_desc.ensure(1)) != hipSuccess) return hip_fail(h, e, "hipMalloc(desc)");
     if ((e = h->d_ip.ensure(pool.size())) != hipSuccess) return hip_fail(h, e, "hipMalloc(plan)");
@@ -147,7 +203,7 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
     if ((e = h->d_ax.ensure(nnz)) != hipSuccess) return hip_fail(h, e, "hipMalloc(ax)");
     if ((e = h->d_rhs.ensure(n)) != hipSuccess) return hip_fail(h, e, "hipMalloc(rhs)");
     if ((e = h->d_x.ensure(n)) != hipSuccess) return hip_fail(h, e, "hipMalloc(x)");
-    size_t wn = (size_t)(kThreads / 64 + 1) * n;
+    size_t wn = (size_t)(kMaxThreads / 64 + 1) * n;
     bool new_work = h->d_work.n < wn;
     if ((e = h->d_work.ensure(wn)) != hipSuccess) return hip_fail(h, e, "hipMalloc(work)");
     (void)new_work;
@@ -227,6 +283,81 @@ int32_t host_full_factor(rs_solver* h, const i64* ap, const i64* ai, const doubl
     return upload_values(h);
 }
 
+// Launch-shape candidates: 1 to 16 wavefronts, work vectors in LDS when they
+// fit. Every shape gives bit-identical values (a column or a row is always
+// handled by one wavefront or one thread with a fixed order), so the choice
+// is a pure timing decision.
+struct Shape {
+    int threads;
+    bool lds;
+};
+std::vector<Shape> refactor_shapes(i64 n) {
+    std::vector<Shape> v;
+    const size_t maxlds = device_max_lds();
+    for (int t = 64; t <= kMaxThreads; t *= 2) {
+        if ((size_t)(t / 64) * n * sizeof(double) <= maxlds) v.push_back({t, true});
+        v.push_back({t, false});
+    }
+    return v;
+}
+std::vector<Shape> solve_shapes(i64 n) {
+    std::vector<Shape> v;
+    const bool fits = (size_t)n * sizeof(double) <= device_max_lds();
+    for (int t = 64; t <= kMaxThreads; t *= 2) {
+        if (fits) v.push_back({t, true});
+        v.push_back({t, false});
+    }
+    return v;
+}
+
+// Mean time of `reps` launches of f on stream s (after one warm-up), in us.
+template <class F>
+float time_launches(F&& f, hipStream_t s, int reps, hipError_t& err) {
+    hipEvent_t a, b;
+    float ms = -1.0f;
+    if ((err = hipEventCreate(&a)) != hipSuccess) return ms;
+    if ((err = hipEventCreate(&b)) != hipSuccess) { (void)hipEventDestroy(a); return ms; }
+    if ((err = f()) == hipSuccess && (err = hipEventRecord(a, s)) == hipSuccess) {
+        for (int r = 0; r < reps && err == hipSuccess; r++) err = f();
+        if (err == hipSuccess && (err = hipEventRecord(b, s)) == hipSuccess &&
+            (err = hipEventSynchronize(b)) == hipSuccess)
+            err = hipEventElapsedTime(&ms, a, b);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return err == hipSuccess ? ms * 1000.0f / reps : -1.0f;
+}
+
+int32_t tune_refactor(rs_solver* h, const double* ax_dev, hipStream_t s) {
+    float best = -1.0f;
+    Shape pick{h->ref_threads, h->lds_refactor};
+    for (const Shape& sh : refactor_shapes(h->n)) {
+        hipError_t e;
+        const float us = time_launches(
+            [&] {
+                return launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
+                                       (int)h->n, sh.lds, sh.threads, s);
+            },
+            s, 3, e);
+        if (e != hipSuccess) return hip_fail(h, e, "k_refactor (autotune)");
+        if (us >= 0.0f && (best < 0.0f || us < best)) {
+            best = us;
+            pick = sh;
+        }
+    }
+    h->ref_threads = pick.threads;
+    h->lds_refactor = pick.lds;
+    h->tune_ref_us = best;
+    h->tuned_ref = true;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TunedShapes& t = g_tuned[h->tune_key];
+    t.ref_threads = pick.threads;
+    t.ref_lds = pick.lds;
+    t.ref_us = best;
+    t.have_ref = true;
+    return RS_OK;
+}
+
 int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool sync_rcond) {
     hipError_t e;
     if (h->ph.nod > 0 && ax_dev != h->d_ax.p) {  // off-diagonal values used by the solve
@@ -234,9 +365,13 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
                            hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
     }
+    if (h->autotune && !h->forced && !h->tuned_ref) {
+        const int32_t rc = tune_refactor(h, ax_dev, s);
+        if (rc) return rc;
+    }
     int ps = prof_begin(PK_REFACTOR, s);
     e = launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
-                        (int)h->n, h->lds_refactor, kThreads, s);
+                        (int)h->n, h->lds_refactor, h->ref_threads, s);
     prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_refactor launch");
     if (sync_rcond) {
@@ -300,9 +435,38 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
 int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
     if (h->state != ST_FACTORED || !h->plan_on_device)
         return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    if (h->autotune && !h->forced && !h->tuned_sol) {  // time the shapes into scratch
+        hipError_t e = h->d_tune.ensure((size_t)h->n);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMalloc(tune)");
+        float best = -1.0f;
+        Shape pick{h->sol_threads, h->lds_solve};
+        for (const Shape& sh : solve_shapes(h->n)) {
+            const float us = time_launches(
+                [&] {
+                    return launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev,
+                                        h->d_tune.p, h->d_work.p, (int)h->n, sh.lds, sh.threads, s);
+                },
+                s, 3, e);
+            if (e != hipSuccess) return hip_fail(h, e, "k_solve (autotune)");
+            if (us >= 0.0f && (best < 0.0f || us < best)) {
+                best = us;
+                pick = sh;
+            }
+        }
+        h->sol_threads = pick.threads;
+        h->lds_solve = pick.lds;
+        h->tune_sol_us = best;
+        h->tuned_sol = true;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        TunedShapes& t = g_tuned[h->tune_key];
+        t.sol_threads = pick.threads;
+        t.sol_lds = pick.lds;
+        t.sol_us = best;
+        t.have_sol = true;
+    }
     int ps = prof_begin(PK_SOLVE, s);
     hipError_t e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev,
-                                h->d_work.p, (int)h->n, h->lds_solve, kThreads, s);
+                                h->d_work.p, (int)h->n, h->lds_solve, h->sol_threads, s);
     prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
     return RS_OK;
@@ -337,6 +501,7 @@ void rs_solver_destroy(rs_solver* h) {
     h->d_rhs.release();
     h->d_x.release();
     h->d_work.release();
+    h->d_tune.release();
     delete h;
 }
 
@@ -415,9 +580,8 @@ int32_t rs_solve(rs_solver* h, double* rhs, int64_t n) {
     if (h->bbd) return bbd_solve(h, rhs);
     hipError_t e = hipMemcpy(h->d_rhs.p, rhs, n * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(rhs)");
-    e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, h->d_rhs.p, h->d_x.p,
-                     h->d_work.p, (int)n, h->lds_solve, kThreads, nullptr);
-    if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
+    const int32_t rc = solver_solve_dev(h, h->d_rhs.p, h->d_x.p, nullptr);
+    if (rc) return rc;
     e = hipMemcpy(rhs, h->d_x.p, n * 8, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(x)");
     return RS_OK;
@@ -448,6 +612,38 @@ int32_t rs_bbd_info(const rs_solver* h, int64_t* out) {
 }
 
 void rs_set_parallel_threads(rs_solver* h, int64_t t) { h->threads_hint = t; }
+
+void rs_set_autotune(rs_solver* h, int32_t on) {
+    h->autotune = on != 0;
+    h->forced = false;
+}
+
+int32_t rs_set_launch_shape(rs_solver* h, int32_t ref_threads, int32_t ref_lds, int32_t sol_threads,
+                            int32_t sol_lds) {
+    auto ok = [](int t) { return t >= 64 && t <= kMaxThreads && (t & (t - 1)) == 0; };
+    if (!ok(ref_threads) || !ok(sol_threads))
+        return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: launch threads must be 64..1024, a power of 2");
+    const size_t maxlds = device_max_lds();
+    if (h->n > 0 && ((ref_lds && (size_t)(ref_threads / 64) * h->n * sizeof(double) > maxlds) ||
+                     (sol_lds && (size_t)h->n * sizeof(double) > maxlds)))
+        return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: work vectors do not fit in LDS");
+    h->ref_threads = ref_threads;
+    h->lds_refactor = ref_lds != 0;
+    h->sol_threads = sol_threads;
+    h->lds_solve = sol_lds != 0;
+    h->forced = true;
+    return RS_OK;
+}
+
+int32_t rs_launch_shape(const rs_solver* h, int64_t* out) {
+    out[0] = h->ref_threads;
+    out[1] = h->lds_refactor ? 1 : 0;
+    out[2] = h->sol_threads;
+    out[3] = h->lds_solve ? 1 : 0;
+    out[4] = (int64_t)(h->tune_ref_us * 1000.0f);
+    out[5] = (int64_t)(h->tune_sol_us * 1000.0f);
+    return (h->tuned_ref ? 1 : 0) | (h->tuned_sol ? 2 : 0) | (h->forced ? 4 : 0);
+}
 
 void rs_set_config(rs_solver* h, double pivot_tol, int32_t ordering, int32_t use_btf) {
     if (h->kind == RS_SOLVER_SPARSELU) return;

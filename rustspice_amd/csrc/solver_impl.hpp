@@ -23,6 +23,7 @@ struct DenseImpl;
 
 enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
 constexpr int kThreads = 256;
+constexpr int kMaxThreads = 1024;  // largest launch the autotuner tries (16 wavefronts)
 
 template <class T>
 struct DevBuf {
@@ -80,6 +81,13 @@ struct rs_solver {
     std::string err;
     size_t plan_bytes = 0;
     bool lds_refactor = false, lds_solve = false;
+    // launch shapes of k_refactor / k_solve: chosen by timing every candidate
+    // on the first device refactor and solve of a pattern (autotune), or set
+    int ref_threads = rsgpu::kThreads, sol_threads = rsgpu::kThreads;
+    bool autotune = true, tuned_ref = false, tuned_sol = false, forced = false;
+    float tune_ref_us = 0.0f, tune_sol_us = 0.0f;  // best times found
+    uint64_t tune_key = 0;                         // pattern key of the tuning cache
+    rsgpu::DevBuf<double> d_tune;                  // solve output while tuning
     rsgpu::DevBuf<rsgpu::PlanDev> d_desc;
     rsgpu::DevBuf<int32_t> d_ip;
     rsgpu::DevBuf<double> d_vals, d_ax, d_rhs, d_x, d_work;

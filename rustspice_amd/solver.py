@@ -108,6 +108,22 @@ class GpuSolver:
     def set_config(self, pivot_tol=0.001, ordering=0, use_btf=True):
         self._L.rs_set_config(self.h, pivot_tol, ordering, int(use_btf))
 
+    def set_autotune(self, on=True):
+        """Time the kernel launch shapes on the first refactor/solve of each
+        pattern and keep the fastest (default on)."""
+        self._L.rs_set_autotune(self.h, int(bool(on)))
+
+    def set_launch_shape(self, ref_threads, ref_lds, sol_threads, sol_lds):
+        self._check(self._L.rs_set_launch_shape(self.h, ref_threads, int(ref_lds), sol_threads,
+                                                int(sol_lds)))
+
+    def launch_shape(self) -> dict:
+        out = (C.c_int64 * 6)()
+        flags = self._L.rs_launch_shape(self.h, out)
+        return dict(ref_threads=out[0], ref_lds=bool(out[1]), sol_threads=out[2],
+                    sol_lds=bool(out[3]), ref_us=out[4] / 1000.0, sol_us=out[5] / 1000.0,
+                    tuned_ref=bool(flags & 1), tuned_sol=bool(flags & 2), forced=bool(flags & 4))
+
     def stats(self) -> dict:
         s = _lib.SolverStats()
         self._L.rs_get_stats(self.h, C.byref(s))

@@ -177,3 +177,67 @@ def test_sparselu_one_by_one_clamp(gpu):
         x = np.array([1.0])
         s.solve(x)
         assert x[0] == want, (kind, x[0])
+
+
+def _refactor_solve(g, ap, ai, ax, ax2, rhs):
+    g.factor(ap, ai, ax)
+    g.factor(ap, ai, ax2)  # a device refactor on the pivots of the first factor
+    x = rhs.copy()
+    g.solve(x)
+    return g.factor_values(), x
+
+
+@pytest.mark.parametrize("case", ["mesh64", "chain"])
+def test_every_launch_shape_is_bit_identical(gpu, case):
+    """k_refactor / k_solve give the reference's values for every launch shape
+    (1 to 16 wavefronts, work vectors in LDS or in global memory)."""
+    c = NL.resistor_mesh() if case == "mesh64" else NL.inverter_chain(200)
+    x0 = np.random.RandomState(3).rand(c.node_count)
+    ap, ai, ax, rhs, n = O.build_mna(c, x=x0)
+    ax2 = ax * (1.0 + 0.25 * np.random.RandomState(4).rand(len(ax)))
+    o = O.NativeKlu(n)
+    o.factor(ap, ai, ax)
+    o.factor(ap, ai, ax2)
+    xo = rhs.copy()
+    o.solve(xo)
+    want = [o.block(b) for b in range(o.stats()["nblocks"])]
+    tried = 0
+    for t in (64, 128, 256, 512, 1024):
+        for lds in (True, False):
+            g = S.GpuSolver(n)
+            g.prepare(n)
+            g.analyze(ap, ai)
+            try:  # shapes whose work vectors do not fit in this device's LDS are refused
+                g.set_launch_shape(t, lds, t, lds)
+            except S.SolverError:
+                g.close()
+                continue
+            (lx, ux, ud), xg = _refactor_solve(g, ap, ai, ax, ax2, rhs)
+            assert same_bits(xg, xo), (t, lds)
+            if len(want) == 1:
+                assert same_bits(lx, want[0]["l_values"]) and same_bits(ud, want[0]["u_diag"]), (t, lds)
+            assert g.launch_shape()["forced"]
+            g.close()
+            tried += 1
+    assert tried >= 6
+
+
+def test_autotune_picks_a_shape_and_keeps_parity(gpu):
+    c = NL.resistor_mesh()
+    ap, ai, ax, rhs, n = O.build_mna(c)
+    ax2 = ax * 1.5
+    g = S.GpuSolver(n)
+    (_, _, _), xg = _refactor_solve(g, ap, ai, ax, ax2, rhs)
+    sh = g.launch_shape()
+    assert sh["tuned_ref"] and sh["tuned_sol"] and not sh["forced"]
+    assert sh["ref_threads"] in (64, 128, 256, 512, 1024) and sh["sol_threads"] in (64, 128, 256, 512, 1024)
+    assert sh["ref_us"] > 0.0 and sh["sol_us"] > 0.0
+    o = O.NativeKlu(n)
+    o.factor(ap, ai, ax)
+    o.factor(ap, ai, ax2)
+    xo = rhs.copy()
+    o.solve(xo)
+    assert same_bits(xg, xo)
+    # later refactors reuse the shape; a new pattern is tuned again
+    g.factor(ap, ai, ax)
+    assert g.launch_shape()["ref_threads"] == sh["ref_threads"]
