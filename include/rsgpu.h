@@ -103,15 +103,19 @@ void rs_set_bbd_blocks(rs_solver *h, int64_t num_blocks);
 int32_t rs_bbd_info(const rs_solver *h, int64_t *out);
 /* Launch shapes of the refactor and solve kernels (no reference
  * counterpart). By default the library times every shape (1 to 16
- * wavefronts; work vectors in LDS when they fit) on the first device refactor
- * and the first solve of each pattern and keeps the fastest; results are
- * bit-identical for every shape. rs_set_launch_shape fixes the shapes
- * instead (threads a power of 2 in 64..1024; lds only where it fits).
- * rs_launch_shape: out[6] = {refactor threads, refactor lds, solve threads,
+ * wavefronts; the column or the right-looking refactor schedule; work
+ * vectors or values in LDS when they fit) on the first device refactor and
+ * the first solve of each pattern, keeps the fastest and remembers it per
+ * pattern for the process; results are bit-identical for every shape.
+ * rs_set_launch_shape fixes the shapes instead: threads a power of 2 in
+ * 64..1024; ref_mode bit 0 LDS, bit 1 right-looking schedule (checked against
+ * the plan by the next factor when none is built yet, which then fails with
+ * RS_ERR_INVALID_MATRIX if it does not fit); sol_lds only where it fits.
+ * rs_launch_shape: out[6] = {refactor threads, refactor mode, solve threads,
  * solve lds, best refactor ns, best solve ns}; returns bit 0 refactor tuned,
  * bit 1 solve tuned, bit 2 shapes fixed by the caller. */
 void rs_set_autotune(rs_solver *h, int32_t on);
-int32_t rs_set_launch_shape(rs_solver *h, int32_t ref_threads, int32_t ref_lds, int32_t sol_threads,
+int32_t rs_set_launch_shape(rs_solver *h, int32_t ref_threads, int32_t ref_mode, int32_t sol_threads,
                             int32_t sol_lds);
 int32_t rs_launch_shape(const rs_solver *h, int64_t *out);
 /* Text of the last error (SolverError Display, solver.rs:94-117). */

@@ -53,9 +53,22 @@ int64_t rs_host_bbd_decompose(int64_t n, const int64_t *ap, const int64_t *ai, i
  * executed by the host model of the GPU step programs. x equals
  * NativeKluSolver factor(ax0); factor(ax1); solve(rhs) bit for bit.
  * stats[8] = {refactor steps, refactor ops, solve steps, solve ops, refactor
- * slow chunks, solve slow chunks, refactor forwarded ops, solve forwarded ops}. */
+ * slow chunks, solve slow chunks, refactor forwarded ops, solve forwarded ops}.
+ * kind | 0x100 builds the programs on the solver's own plan (positions after
+ * the first factor's pivots, as the single-matrix solver runs them) instead
+ * of the identity-pivot plan of the batched kernel. */
 int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t *ap, const int64_t *ai,
                             const double *ax0, const double *ax1, const double *rhs, double *x,
+                            int64_t *stats);
+/* Right-looking refactor schedule on the host (no reference counterpart):
+ * the first factor (pivots) on ax0, then the refactor on ax1 run phase by
+ * phase with each phase's independent work in reverse order. lx (nnz L),
+ * ux (nnz U) and ud (n) are in the solver's position order and equal
+ * NativeKluSolver factor(ax0); factor(ax1) bit for bit. Returns 6 when the
+ * plan has no right-looking schedule. stats[8] = {phases, tasks, updates,
+ * long tasks, column levels, nnz(L), nnz(U), value slots}. */
+int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t *ap, const int64_t *ai,
+                            const double *ax0, const double *ax1, double *lx, double *ux, double *ud,
                             int64_t *stats);
 /* Unmerged CSC pattern of one stamp pass (mode 0 DC, else TRAN) as
  * SparseBuilder::finalize produces it (mna.rs:320-347); ap/ai may be NULL to

@@ -750,6 +750,93 @@ size_t lds_bytes(int nmax, int lumax, int waves) {
            (size_t)(lumax + kStage * waves + 8) * sizeof(double);
 }
 
+// ---- one matrix by its step programs (single-matrix solver) ------------------
+// VL (dynamic LDS from address 0, as sp_run requires) holds the value slots:
+// lx | ux | T | off-diagonal ax | dummy. The refactor scatters A into the
+// slots (duplicates summed in CSC order from 0.0, fill entries 0), runs the
+// refactor program on wavefront 0 and writes the values back in the solver's
+// layout; the solve loads them, runs the solve program and writes x.
+__global__ __launch_bounds__(1024) void k_sp_refactor(const SpSolverDev* __restrict__ sdp,
+                                                     const PlanDev* __restrict__ pdp,
+                                                     const int32_t* __restrict__ ip,
+                                                     const int32_t* __restrict__ sip,
+                                                     const SpOpWord* __restrict__ words,
+                                                     const double* __restrict__ ax,
+                                                     double* __restrict__ vals) {
+    extern __shared__ double VL[];
+    const SpSolverDev sd = *sdp;
+    const PlanDev d = *pdp;
+    const int tid = threadIdx.x, bs = blockDim.x, lane = tid & (kW - 1), wave = tid / kW;
+    for (int a = tid; a <= sd.nslots; a += bs) VL[a] = 0.0;
+    __syncthreads();
+    const int32_t* sc_src = ip + d.o_sc_src;
+    const int32_t* sc_cnt = ip + d.o_sc_cnt;
+    const int32_t* ss = sip + sd.o_ss;
+    for (int s = tid; s < sd.nsc; s += bs) {
+        const int slot = ss[s];
+        if (slot < 0) continue;
+        double v = 0.0;
+        const int src = sc_src[s], cnt = sc_cnt[s];
+        for (int q = 0; q < cnt; ++q) v += ax[src + q];
+        VL[slot] = v;
+    }
+    __syncthreads();
+    if (wave == 0)
+        sp_run<true, false>(reinterpret_cast<const uint4*>(words + sd.o_fac), sd.fac_chunks,
+                            sip + sd.o_rpi, sd.t_base, lane);
+    __syncthreads();
+    double* lx = vals + d.o_lx;
+    double* ux = vals + d.o_ux;
+    double* ud = vals + d.o_ud;
+    const int32_t* ucp = ip + d.o_ucp;
+    for (int a = tid; a < sd.nl; a += bs) lx[a] = VL[a];
+    for (int a = tid; a < sd.nu; a += bs) ux[a] = VL[sd.nl + a];
+    for (int g = tid; g < sd.n; g += bs) ud[g] = VL[sd.nl + ucp[g + 1] - 1];
+    if (d.flags & PLAN_CSR) {
+        const int32_t* lrv = ip + d.o_lrv;
+        const int32_t* urv = ip + d.o_urv;
+        double* lxr = vals + d.o_lxr;
+        double* uxr = vals + d.o_uxr;
+        for (int e = tid; e < d.nl; e += bs) lxr[e] = VL[lrv[e]];
+        for (int e = tid; e < d.nu - d.n; e += bs) uxr[e] = VL[sd.nl + urv[e]];
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_sp_solve(const SpSolverDev* __restrict__ sdp,
+                                                  const PlanDev* __restrict__ pdp,
+                                                  const int32_t* __restrict__ ip,
+                                                  const int32_t* __restrict__ sip,
+                                                  const SpOpWord* __restrict__ words,
+                                                  const double* __restrict__ ax,
+                                                  const double* __restrict__ vals,
+                                                  const double* __restrict__ rhs,
+                                                  double* __restrict__ x) {
+    extern __shared__ double VL[];
+    const SpSolverDev sd = *sdp;
+    const PlanDev d = *pdp;
+    const int tid = threadIdx.x, bs = blockDim.x, lane = tid & (kW - 1), wave = tid / kW;
+    const double* lx = vals + d.o_lx;
+    const double* ux = vals + d.o_ux;
+    const int32_t* odax = ip + d.o_odax;
+    const int32_t* perm_in = ip + d.o_perm_in;
+    const int32_t* perm_out = ip + d.o_perm_out;
+    for (int a = tid; a < sd.nl; a += bs) VL[a] = lx[a];
+    for (int a = tid; a < sd.nu; a += bs) VL[sd.nl + a] = ux[a];
+    for (int e = tid; e < sd.nod; e += bs) VL[sd.od_base + e] = ax[odax[e]];
+    for (int i = tid; i < sd.n; i += bs) VL[sd.t_base + perm_in[i]] = rhs[i];
+    if (tid == 0) VL[sd.nslots] = 0.0;
+    __syncthreads();
+    if (wave == 0) {
+        const uint4* prog = reinterpret_cast<const uint4*>(words + sd.o_sol);
+        if (sd.nod > 0)
+            sp_run<false, true>(prog, sd.sol_chunks, sip + sd.o_rpi, sd.t_base, lane);
+        else
+            sp_run<false, false>(prog, sd.sol_chunks, sip + sd.o_rpi, sd.t_base, lane);
+    }
+    __syncthreads();
+    for (int i = tid; i < sd.n; i += bs) x[i] = VL[sd.t_base + perm_out[i]];
+}
+
 }  // namespace
 
 namespace rsgpu {
@@ -771,6 +858,36 @@ hipError_t launch_batch_dcop(const BPat* pats, const BInst* insts, const int32_t
 }
 
 size_t batch_lds_bytes(int nmax, int lumax, int waves) { return lds_bytes(nmax, lumax, waves); }
+
+hipError_t launch_sp_refactor(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
+                              const int32_t* sip, const SpOpWord* words, const double* ax,
+                              double* vals, int nslots, int threads, hipStream_t s) {
+    hipLaunchKernelGGL(k_sp_refactor, dim3(1), dim3(threads), (size_t)(nslots + 1) * sizeof(double),
+                       s, sd, pd, ip, sip, words, ax, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_sp_solve(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
+                           const int32_t* sip, const SpOpWord* words, const double* ax,
+                           const double* vals, const double* rhs, double* x, int nslots, int threads,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_sp_solve, dim3(1), dim3(threads), (size_t)(nslots + 1) * sizeof(double), s,
+                       sd, pd, ip, sip, words, ax, vals, rhs, x);
+    return hipGetLastError();
+}
+
+// Static LDS of the single-matrix step-program kernels (must be 0, as for
+// k_batch_dcop).
+size_t sp_solver_static_lds() {
+    size_t mx = 0;
+    for (const void* k : {reinterpret_cast<const void*>(k_sp_refactor),
+                          reinterpret_cast<const void*>(k_sp_solve)}) {
+        hipFuncAttributes a{};
+        if (hipFuncGetAttributes(&a, k) != hipSuccess) return (size_t)-1;
+        mx = a.sharedSizeBytes > mx ? a.sharedSizeBytes : mx;
+    }
+    return mx;
+}
 
 // Static LDS of k_batch_dcop: step programs address their slots by absolute
 // LDS address, which is only right when this is 0.

@@ -486,7 +486,7 @@ int32_t build_bbd(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* a
     for (int64_t k = 0; k < nb; k++) bstart[k] = c32(d.block_ptr[k]);
     const size_t maxlds = device_max_lds();
     b->lds_ref = (size_t)(kThreads / 64) * b->max_blk * sizeof(double) <= maxlds;
-    b->lds_sol = (size_t)b->max_blk * sizeof(double) <= maxlds;
+    b->lds_sol = ((size_t)b->max_blk + kThreads) * sizeof(double) <= maxlds;  // + ordered-sum staging
     hipError_t e;
     if ((e = put(b->d_fac, fac)) || (e = put(b->d_blk, blk)) || (e = put(b->d_pair, pairs)) ||
         (e = put(b->d_ip, pool)) || (e = put(b->d_bmap, bmap)) || (e = put(b->d_perm, perm)) ||

@@ -2,6 +2,7 @@
 // L/U patterns) and the first pivoting factor, so they can be checked on a
 // machine without a GPU. They touch no device memory.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -165,9 +166,16 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
                             int64_t* stats) {
     rs_host_factor* h = nullptr;
     try {
+        // kind bit 8: programs of the solver's own plan (positions after the
+        // pivots, RPI the identity) instead of the identity-pivot plan
+        const bool own = (kind & 0x100) != 0;
+        kind &= 0xFF;
         h = rs_host_factor_create(kind, n, ap, ai, ax0, 0.001, 0, 1);
         PlanHost ph;
-        build_identity_plan(n, ap, ai, kind, ph);
+        if (own)
+            ph = h->p;
+        else
+            build_identity_plan(n, ap, ai, kind, ph);
         SpProgram fac, sol;
         std::string why;
         if (!build_sp_programs(ph, fac, sol, &why)) {
@@ -179,7 +187,7 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
         for (int32_t b = 0; b < ph.nblk; b++) {
             const int32_t bs = ph.blk_ptr[b], be = ph.blk_ptr[b + 1];
             for (int32_t i = 0; i < be - bs; i++)
-                rpi[bs + i] = bs + (int32_t)h->f.num[b].row_perm_inv[i];
+                rpi[bs + i] = own ? bs + i : bs + (int32_t)h->f.num[b].row_perm_inv[i];
         }
         std::vector<int32_t> ss;
         sp_scatter_slots(ph, rpi.data(), ss);
@@ -204,6 +212,79 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
             stats[5] = sol.nslow;
             stats[6] = fac.nfwd;
             stats[7] = sol.nfwd;
+        }
+        delete h;
+        return RS_OK;
+    } catch (const std::exception&) {
+        delete h;
+        return RS_ERR_INVALID_MATRIX;
+    }
+}
+
+int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t* ap, const int64_t* ai,
+                            const double* ax0, const double* ax1, double* lx, double* ux, double* ud,
+                            int64_t* stats) {
+    rs_host_factor* h = nullptr;
+    try {
+        h = rs_host_factor_create(kind, n, ap, ai, ax0, 0.001, 0, 1);
+        const PlanHost& p = h->p;
+        RlHost r;
+        if (!build_rl_plan(p, r)) {
+            delete h;
+            return RS_ERR_INVALID_MATRIX;
+        }
+        // the schedule, with every phase's independent work run in reverse
+        // order: the values must not depend on the order inside a phase
+        std::vector<double> V((size_t)r.nv, 0.0);
+        const int32_t ux0 = p.nl + 1;
+        for (int32_t a = 0; a < r.nv; a++) {
+            double v = 0.0;
+            for (int32_t q = 0; q < r.ini_cnt[a]; q++) v += ax1[r.ini_src[a] + q];
+            V[a] = v;
+        }
+        auto clampd = [](double d, bool single) {
+            if (single) return std::fabs(d) < 1e-300 ? 1e-14 : d;
+            if (std::fabs(d) < 1e-14) return d >= 0.0 ? 1e-14 : -1e-14;
+            return d;
+        };
+        std::vector<double> udv(n, 0.0);
+        int64_t nlong = 0;
+        for (int32_t ph = 0; ph < r.nph; ph++) {
+            for (int32_t k = r.div_ptr[ph + 1] - 1; k >= r.div_ptr[ph]; k--)
+                V[r.div[2 * k]] = V[r.div[2 * k]] / clampd(V[r.div[2 * k + 1]], false);
+            for (int32_t k = r.piv_ptr[ph + 1] - 1; k >= r.piv_ptr[ph]; k--) {
+                const int32_t j = r.piv[k];
+                const int32_t a = ux0 + p.ucp[j + 1] - 1;
+                V[a] = clampd(V[a], p.colflag[j] & COL_SINGLETON);
+                udv[j] = V[a];
+            }
+            nlong += r.task_nlong[ph];
+            for (int32_t k = r.sgl_ptr[ph + 1] - 1; k >= r.sgl_ptr[ph]; k--) {
+                const int32_t* g = &r.sgl[4 * (size_t)k];
+                const double uu = V[g[2]];
+                if (uu != 0.0) V[g[0]] = V[g[0]] - V[g[1]] * uu;
+            }
+            for (int32_t k = r.task_ptr[ph + 1] - 1; k >= r.task_ptr[ph]; k--) {
+                double v = V[r.task_tgt[k]];
+                for (int32_t q = r.task_beg[k]; q < r.task_beg[k + 1]; q++) {
+                    const double uu = V[r.pair[2 * q + 1]];
+                    if (uu != 0.0) v = v - V[r.pair[2 * q]] * uu;
+                }
+                V[r.task_tgt[k]] = v;
+            }
+        }
+        if (p.nl) std::memcpy(lx, V.data(), p.nl * sizeof(double));
+        if (p.nu) std::memcpy(ux, V.data() + ux0, p.nu * sizeof(double));
+        if (n) std::memcpy(ud, udv.data(), n * sizeof(double));
+        if (stats) {
+            stats[0] = r.nph;
+            stats[1] = (int64_t)r.ntask + r.nsgl;
+            stats[2] = r.npair;
+            stats[3] = nlong;
+            stats[4] = p.nflev;
+            stats[5] = p.nl;
+            stats[6] = p.nu;
+            stats[7] = r.nv;
         }
         delete h;
         return RS_OK;

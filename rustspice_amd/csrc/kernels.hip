@@ -8,10 +8,16 @@
 //   ascending row order, each lane owning distinct rows of L(:,j), so every
 //   work[row] sees the reference's updates in the reference's order: results
 //   are bit-identical to the sequential loop (no FMA contraction).
+// k_refactor_rl: the same arithmetic scheduled right-looking (rlplan.cpp):
+//   pivots of one phase divide their L columns, then every entry updated in
+//   that phase applies its updates in the reference's order. The critical
+//   path is the number of phases instead of the sum of the column chains;
+//   the values live in LDS when they fit.
 // k_solve: P, L (unit), U, P^T sweeps of native_klu.rs:1442-1488 and the BTF
 //   block back-substitution of :1491-1551, row-oriented so each unknown is a
 //   sequential dot product in the reference's update order, rows of one
-//   dependency level in parallel.
+//   dependency level in parallel. Long rows take a wavefront: the products
+//   are formed in parallel and one lane subtracts them in order.
 #include <hip/hip_runtime.h>
 
 #include "plan.hpp"
@@ -21,12 +27,33 @@ using rsgpu::PlanDev;
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kSglBatch = 8;  // one-update tasks in flight per thread
 
 __device__ __forceinline__ void wave_sync() {
     // Orders this wave's LDS/global accesses before the following ones
     // (lanes exchange data through memory between steps).
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
+}
+
+// Row-ordered copies of L and U (without diagonal) for the solve, which then
+// reads each row's values contiguously instead of through lrv/urv.
+__device__ __forceinline__ void csr_copies(const PlanDev& d, const int32_t* __restrict__ ip,
+                                           double* __restrict__ vpool, const double* lx,
+                                           const double* ux) {
+    const int32_t* lrv = ip + d.o_lrv;
+    const int32_t* urv = ip + d.o_urv;
+    double* lxr = vpool + d.o_lxr;
+    double* uxr = vpool + d.o_uxr;
+    const int nlr = d.nl, nur = d.nu - d.n;
+    for (int e = threadIdx.x; e < nlr; e += blockDim.x) lxr[e] = lx[lrv[e]];
+    for (int e = threadIdx.x; e < nur; e += blockDim.x) uxr[e] = ux[urv[e]];
+}
+
+__device__ __forceinline__ double clamp_pivot(double diag, bool singleton) {
+    if (singleton) return fabs(diag) < 1e-300 ? 1e-14 : diag;
+    if (fabs(diag) < 1e-14) return diag >= 0.0 ? 1e-14 : -1e-14;
+    return diag;
 }
 
 template <bool kLds>
@@ -87,12 +114,7 @@ __global__ __launch_bounds__(1024) void k_refactor(const PlanDev* __restrict__ d
                 wave_sync();
             }
             for (int u = us + lane; u < ue; u += kWave) ux[u] = work[uri[u]];
-            double diag = work[g];
-            if (colflag[g] & rsgpu::COL_SINGLETON) {
-                if (fabs(diag) < 1e-300) diag = 1e-14;
-            } else if (fabs(diag) < 1e-14) {
-                diag = diag >= 0.0 ? 1e-14 : -1e-14;
-            }
+            const double diag = clamp_pivot(work[g], colflag[g] & rsgpu::COL_SINGLETON);
             if (lane == 0) {
                 ux[ue] = diag;
                 ud[g] = diag;
@@ -108,22 +130,182 @@ __global__ __launch_bounds__(1024) void k_refactor(const PlanDev* __restrict__ d
         }
         __syncthreads();
     }
+    if (d.flags & rsgpu::PLAN_CSR) csr_copies(d, ip, vpool, lx, ux);
 }
 
-template <bool kLds>
-__global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ descs,
-                                               const int32_t* __restrict__ ip,
-                                               const double* __restrict__ axpool,
-                                               const double* __restrict__ vpool,
-                                               const double* __restrict__ rhspool,
-                                               double* __restrict__ xpool,
-                                               double* __restrict__ wpool) {
+
+// Values V: lx | pad | ux | pad (indices of the schedule), in LDS (kLdsV) or
+// in the value pool; G is the value pool at lx (ud follows V there).
+template <bool kLdsV>
+__global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict__ descs,
+                                                     const int32_t* __restrict__ ip,
+                                                     const double* __restrict__ axpool,
+                                                     double* __restrict__ vpool) {
     extern __shared__ double smem[];
     const PlanDev d = descs[blockIdx.x];
-    const int n = d.n;
-    double* T = kLds ? smem : wpool + d.o_tvec;
-    const int32_t* perm_in = ip + d.o_perm_in;
-    const int32_t* perm_out = ip + d.o_perm_out;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    double* S = smem + wid * kWave;  // ordered-sum staging of this wavefront
+    double* G = vpool + d.o_lx;
+    double* V = kLdsV ? smem + nw * kWave : G;
+    const int nv = d.rl_nv;
+    const int ux0 = d.nl + 1, ud0 = d.nl + 1 + d.nu + 1;
+    const double* ax = axpool + d.o_ax;
+    const int32_t* ini_src = ip + d.o_rl_ini_src;
+    const int32_t* ini_cnt = ip + d.o_rl_ini_cnt;
+    const int32_t* div_ptr = ip + d.o_rl_div_ptr;
+    const int2* div = reinterpret_cast<const int2*>(ip + d.o_rl_div);
+    const int32_t* piv_ptr = ip + d.o_rl_piv_ptr;
+    const int32_t* piv = ip + d.o_rl_piv;
+    const int32_t* task_ptr = ip + d.o_rl_task_ptr;
+    const int32_t* task_nlong = ip + d.o_rl_task_nlong;
+    const int32_t* task_tgt = ip + d.o_rl_task_tgt;
+    const int32_t* task_beg = ip + d.o_rl_task_beg;
+    const int2* pair = reinterpret_cast<const int2*>(ip + d.o_rl_pair);
+    const int32_t* sgl_ptr = ip + d.o_rl_sgl_ptr;
+    const int4* sgl = reinterpret_cast<const int4*>(ip + d.o_rl_sgl);
+    const int32_t* ucp = ip + d.o_ucp;
+    const int32_t* colflag = ip + d.o_colflag;
+
+    // A scattered into every slot: duplicates summed in CSC order from 0.0
+    for (int a = tid; a < nv; a += bs) {
+        double v = 0.0;
+        const int c = ini_cnt[a], s0 = ini_src[a];
+        for (int q = 0; q < c; ++q) v += ax[s0 + q];
+        V[a] = v;
+    }
+    __syncthreads();
+    for (int p = 0; p < d.rl_nph; ++p) {
+        // (a) L columns of this phase's pivots over their clamped diagonals
+        for (int k = div_ptr[p] + tid; k < div_ptr[p + 1]; k += bs) {
+            const int2 dv = div[k];
+            V[dv.x] = V[dv.x] / clamp_pivot(V[dv.y], false);
+        }
+        __syncthreads();
+        // (b) the clamped diagonals, then the ordered updates of the phase
+        for (int k = piv_ptr[p] + tid; k < piv_ptr[p + 1]; k += bs) {
+            const int j = piv[k];
+            const int a = ux0 + ucp[j + 1] - 1;
+            const double diag = clamp_pivot(V[a], colflag[j] & rsgpu::COL_SINGLETON);
+            V[a] = diag;
+            G[ud0 + j] = diag;
+        }
+        const int t0 = task_ptr[p], tl = t0 + task_nlong[p], t1 = task_ptr[p + 1];
+        for (int k = t0 + wid; k < tl; k += nw) {
+            const int tgt = task_tgt[k], b = task_beg[k], e = task_beg[k + 1];
+            double v = V[tgt];
+            for (int base = b; base < e; base += kWave) {
+                const int q = base + lane;
+                double pr = 0.0;
+                bool act = false;
+                if (q < e) {
+                    const int2 pq = pair[q];
+                    const double uu = V[pq.y];
+                    if (uu != 0.0) {
+                        pr = V[pq.x] * uu;
+                        act = true;
+                    }
+                }
+                const unsigned long long m = __ballot(act);
+                S[lane] = pr;
+                wave_sync();
+                if (lane == 0) {
+                    const int c = min(kWave, e - base);
+                    for (int i = 0; i < c; ++i)
+                        if ((m >> i) & 1ull) v = v - S[i];
+                }
+                wave_sync();
+            }
+            if (lane == 0) V[tgt] = v;
+        }
+        // one-update tasks, kSglBatch per thread with every load issued first
+        const int g0 = sgl_ptr[p], g1 = sgl_ptr[p + 1];
+        for (int base = g0 + tid; base < g1; base += kSglBatch * bs) {
+            int4 rc[kSglBatch];
+#pragma unroll
+            for (int i = 0; i < kSglBatch; ++i) {
+                const int k = base + i * bs;
+                rc[i] = k < g1 ? sgl[k] : make_int4(-1, 0, 0, 0);
+            }
+            double vt[kSglBatch], vl[kSglBatch], vu[kSglBatch];
+#pragma unroll
+            for (int i = 0; i < kSglBatch; ++i) {
+                if (rc[i].x >= 0) {
+                    vt[i] = V[rc[i].x];
+                    vl[i] = V[rc[i].y];
+                    vu[i] = V[rc[i].z];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kSglBatch; ++i)
+                if (rc[i].x >= 0 && vu[i] != 0.0) V[rc[i].x] = vt[i] - vl[i] * vu[i];
+        }
+        for (int k = tl + tid; k < t1; k += bs) {
+            const int tgt = task_tgt[k], b = task_beg[k], e = task_beg[k + 1];
+            double v = V[tgt];
+            for (int q = b; q < e; ++q) {
+                const int2 pq = pair[q];
+                const double uu = V[pq.y];
+                if (uu != 0.0) v = v - V[pq.x] * uu;
+            }
+            V[tgt] = v;
+        }
+        __syncthreads();
+    }
+    if (kLdsV)
+        for (int a = tid; a < nv; a += bs) G[a] = V[a];
+    if (d.flags & rsgpu::PLAN_CSR) csr_copies(d, ip, vpool, V, V + ux0);
+}
+
+// One level of a row-oriented triangular sweep: T[r] = (T[r] - sum_e v_e * T[c_e]) (/ diag)
+// with the subtractions in entry order. Entry values come from the row-ordered
+// copy (kCsr) or through the position map. Long rows (the level's first
+// nlong) take a wavefront each.
+template <bool kCsr, bool kDiv>
+__device__ __forceinline__ void sweep_level(double* T, double* S, const int32_t* rows, int q0,
+                                            int nlong, int q1, const int32_t* __restrict__ rp,
+                                            const int32_t* __restrict__ rc,
+                                            const int32_t* __restrict__ rv,
+                                            const double* __restrict__ vals,
+                                            const double* __restrict__ ud) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (q1 - q0 <= 2 * nw) nlong = q1 - q0;  // a narrow level: every row takes a wavefront
+    for (int q = q0 + wid; q < q0 + nlong; q += nw) {
+        const int r = rows[q];
+        const int e0 = rp[r], e1 = rp[r + 1];
+        double t = T[r];
+        for (int base = e0; base < e1; base += kWave) {
+            const int e = base + lane;
+            double pr = 0.0;
+            if (e < e1) pr = vals[kCsr ? e : rv[e]] * T[rc[e]];
+            S[lane] = pr;
+            wave_sync();
+            if (lane == 0) {
+                const int c = min(kWave, e1 - base);
+                for (int i = 0; i < c; ++i) t = t - S[i];
+            }
+            wave_sync();
+        }
+        if (lane == 0) T[r] = kDiv ? t / ud[r] : t;
+    }
+    for (int q = q0 + nlong + threadIdx.x; q < q1; q += blockDim.x) {
+        const int r = rows[q];
+        const int e0 = rp[r], e1 = rp[r + 1];
+        if (!kDiv && e0 == e1) continue;
+        double t = T[r];
+        for (int e = e0; e < e1; ++e) t = t - vals[kCsr ? e : rv[e]] * T[rc[e]];
+        T[r] = kDiv ? t / ud[r] : t;
+    }
+}
+
+template <bool kCsr>
+__device__ __forceinline__ void solve_body(const PlanDev& d, const int32_t* __restrict__ ip,
+                                           const double* __restrict__ ax,
+                                           const double* __restrict__ vpool, double* T, double* S) {
     const int32_t* lrp = ip + d.o_lrp;
     const int32_t* lrc = ip + d.o_lrc;
     const int32_t* lrv = ip + d.o_lrv;
@@ -133,23 +315,19 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
     const int32_t* lsl_blk = ip + d.o_lsl_blk;
     const int32_t* lsl_ptr = ip + d.o_lsl_ptr;
     const int32_t* lsl_rows = ip + d.o_lsl_rows;
+    const int32_t* lsl_nlong = ip + d.o_lsl_nlong;
     const int32_t* usl_blk = ip + d.o_usl_blk;
     const int32_t* usl_ptr = ip + d.o_usl_ptr;
     const int32_t* usl_rows = ip + d.o_usl_rows;
+    const int32_t* usl_nlong = ip + d.o_usl_nlong;
     const int32_t* blk_ptr = ip + d.o_blk_ptr;
     const int32_t* odp = ip + d.o_odp;
     const int32_t* odax = ip + d.o_odax;
     const int32_t* odsrc = ip + d.o_odsrc;
     const int32_t* has_od = ip + d.o_blk_has_od;
-    const double* ax = axpool + d.o_ax;
-    const double* lx = vpool + d.o_lx;
-    const double* ux = vpool + d.o_ux;
+    const double* lv = kCsr ? vpool + d.o_lxr : vpool + d.o_lx;
+    const double* uv = kCsr ? vpool + d.o_uxr : vpool + d.o_ux;
     const double* ud = vpool + d.o_ud;
-    const double* rhs = rhspool + d.o_vec;
-    double* x = xpool + d.o_vec;
-
-    for (int i = threadIdx.x; i < n; i += blockDim.x) T[perm_in[i]] = rhs[i];
-    __syncthreads();
     for (int b = d.nblk - 1; b >= 0; --b) {
         if (has_od[b]) {
             for (int r = blk_ptr[b] + threadIdx.x; r < blk_ptr[b + 1]; r += blockDim.x) {
@@ -162,26 +340,46 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
             __syncthreads();
         }
         for (int lev = lsl_blk[b]; lev < lsl_blk[b + 1]; ++lev) {
-            for (int q = lsl_ptr[lev] + threadIdx.x; q < lsl_ptr[lev + 1]; q += blockDim.x) {
-                const int r = lsl_rows[q];
-                const int e0 = lrp[r], e1 = lrp[r + 1];
-                if (e0 == e1) continue;
-                double t = T[r];
-                for (int e = e0; e < e1; ++e) t = t - lx[lrv[e]] * T[lrc[e]];
-                T[r] = t;
-            }
+            sweep_level<kCsr, false>(T, S, lsl_rows, lsl_ptr[lev], lsl_nlong[lev], lsl_ptr[lev + 1],
+                                     lrp, lrc, lrv, lv, ud);
             __syncthreads();
         }
         for (int lev = usl_blk[b]; lev < usl_blk[b + 1]; ++lev) {
-            for (int q = usl_ptr[lev] + threadIdx.x; q < usl_ptr[lev + 1]; q += blockDim.x) {
-                const int r = usl_rows[q];
-                double t = T[r];
-                for (int e = urp[r]; e < urp[r + 1]; ++e) t = t - ux[urv[e]] * T[urc[e]];
-                T[r] = t / ud[r];
-            }
+            sweep_level<kCsr, true>(T, S, usl_rows, usl_ptr[lev], usl_nlong[lev], usl_ptr[lev + 1],
+                                    urp, urc, urv, uv, ud);
             __syncthreads();
         }
     }
+}
+
+// Shared memory: 64 doubles per wavefront (ordered sums of long rows), then
+// the solve vector when kLds.
+template <bool kLds>
+__global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ descs,
+                                               const int32_t* __restrict__ ip,
+                                               const double* __restrict__ axpool,
+                                               const double* __restrict__ vpool,
+                                               const double* __restrict__ rhspool,
+                                               double* __restrict__ xpool,
+                                               double* __restrict__ wpool) {
+    extern __shared__ double smem[];
+    const PlanDev d = descs[blockIdx.x];
+    const int n = d.n;
+    const int nw = blockDim.x / kWave;
+    double* S = smem + (threadIdx.x / kWave) * kWave;
+    double* T = kLds ? smem + nw * kWave : wpool + d.o_tvec;
+    const int32_t* perm_in = ip + d.o_perm_in;
+    const int32_t* perm_out = ip + d.o_perm_out;
+    const double* ax = axpool + d.o_ax;
+    const double* rhs = rhspool + d.o_vec;
+    double* x = xpool + d.o_vec;
+
+    for (int i = threadIdx.x; i < n; i += blockDim.x) T[perm_in[i]] = rhs[i];
+    __syncthreads();
+    if (d.flags & rsgpu::PLAN_CSR)
+        solve_body<true>(d, ip, ax, vpool, T, S);
+    else
+        solve_body<false>(d, ip, ax, vpool, T, S);
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T[perm_out[i]];
 }
 
@@ -189,9 +387,24 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
 
 namespace rsgpu {
 
+// mode: bit 0 LDS (work vectors, or the values of the right-looking
+// schedule), bit 1 right-looking schedule.
 hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
-                           double* vals, double* wpool, int max_n, bool lds, int threads,
-                           hipStream_t stream) {
+                           double* vals, double* wpool, int max_n, int mode, int threads,
+                           hipStream_t stream, int rl_nv) {
+    const bool lds = mode & 1;
+    if (mode & 2) {
+        size_t shm = (size_t)(threads / kWave) * kWave * sizeof(double);
+        if (lds) {
+            shm += (size_t)rl_nv * sizeof(double);
+            hipLaunchKernelGGL(k_refactor_rl<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip,
+                               ax, vals);
+        } else {
+            hipLaunchKernelGGL(k_refactor_rl<false>, dim3(nmat), dim3(threads), shm, stream, descs,
+                               ip, ax, vals);
+        }
+        return hipGetLastError();
+    }
     size_t shm = lds ? (size_t)(threads / kWave) * max_n * sizeof(double) : 0;
     if (lds)
         hipLaunchKernelGGL(k_refactor<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax,
@@ -205,12 +418,13 @@ hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, co
 hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
                         bool lds, int threads, hipStream_t stream) {
-    size_t shm = lds ? (size_t)max_n * sizeof(double) : 0;
+    size_t shm = (size_t)(threads / kWave) * kWave * sizeof(double) +
+                 (lds ? (size_t)max_n * sizeof(double) : 0);
     if (lds)
         hipLaunchKernelGGL(k_solve<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax, vals,
                            rhs, x, wpool);
     else
-        hipLaunchKernelGGL(k_solve<false>, dim3(nmat), dim3(threads), 0, stream, descs, ip, ax, vals,
+        hipLaunchKernelGGL(k_solve<false>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax, vals,
                            rhs, x, wpool);
     return hipGetLastError();
 }

@@ -216,6 +216,19 @@ void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p) 
     p.usl_blk.push_back((int32_t)p.usl_ptr.size() - 1);
     p.n_lsl = (int32_t)p.lsl_ptr.size() - 1;
     p.n_usl = (int32_t)p.usl_ptr.size() - 1;
+    // rows of a level are independent: long rows go first and are solved by a
+    // whole wavefront (products in parallel, the ordered sum on one lane)
+    auto long_first = [&](const std::vector<int32_t>& ptr, std::vector<int32_t>& rows,
+                          const std::vector<int32_t>& rp, std::vector<int32_t>& nlong) {
+        nlong.assign(ptr.size() ? ptr.size() - 1 : 0, 0);
+        for (size_t l = 0; l + 1 < ptr.size(); l++) {
+            auto b = rows.begin() + ptr[l], e = rows.begin() + ptr[l + 1];
+            auto mid = std::stable_partition(b, e, [&](int32_t r) { return rp[r + 1] - rp[r] >= kSolveLongRow; });
+            nlong[l] = (int32_t)(mid - b);
+        }
+    };
+    long_first(p.lsl_ptr, p.lsl_rows, p.lrp, p.lsl_nlong);
+    long_first(p.usl_ptr, p.usl_rows, p.urp, p.usl_nlong);
 
     // 7. off-diagonal BTF entries per row position, reference order
     p.odp.assign(n + 1, 0);
@@ -331,6 +344,8 @@ void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
     d.o_odax = put(p.odax);
     d.o_odsrc = put(p.odsrc);
     d.o_blk_has_od = put(p.blk_has_od);
+    d.o_lsl_nlong = put(p.lsl_nlong);
+    d.o_usl_nlong = put(p.usl_nlong);
 }
 
 // Pattern-level analysis: symbolic factor with identity pivots so that the

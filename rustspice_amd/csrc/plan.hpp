@@ -25,8 +25,15 @@ struct PlanDev {
     int32_t o_lrp, o_lrc, o_lrv, o_urp, o_urc, o_urv;
     int32_t o_blk_ptr, o_lsl_blk, o_lsl_ptr, o_lsl_rows, o_usl_blk, o_usl_ptr, o_usl_rows;
     int32_t o_odp, o_odax, o_odsrc, o_blk_has_od;
-    int32_t pad0;
+    int32_t o_lsl_nlong, o_usl_nlong;  // per solve level: its first rows are long (a wavefront each)
+    // right-looking refactor schedule (rl_nph == 0: none); value indices are
+    // relative to o_lx with lx | pad | ux | pad | ud contiguous
+    int32_t rl_nph, rl_nv;
+    int32_t o_rl_ini_src, o_rl_ini_cnt, o_rl_div_ptr, o_rl_div, o_rl_piv_ptr, o_rl_piv;
+    int32_t o_rl_task_ptr, o_rl_task_nlong, o_rl_task_tgt, o_rl_task_beg, o_rl_pair;
+    int32_t o_rl_sgl_ptr, o_rl_sgl;
     int64_t o_lx, o_ux, o_ud;  // value pool
+    int64_t o_lxr, o_uxr;      // row-ordered copies of lx / ux (flags & PLAN_CSR)
     int64_t o_ax;              // ax pool (unmerged CSC values as handed in)
     int64_t o_vec;             // rhs/x pool (n entries)
     int64_t o_work;            // factor work vectors, one per wavefront (global path)
@@ -34,6 +41,10 @@ struct PlanDev {
 };
 
 enum { COL_SINGLETON = 1 };
+enum { PLAN_CSR = 1 };
+
+// A solve row with at least this many off-diagonal entries is a long row.
+constexpr int32_t kSolveLongRow = 32;
 
 struct HostFactor {
     i64 n = 0;
@@ -53,6 +64,7 @@ struct PlanHost {
     std::vector<int32_t> perm_in, perm_out;
     std::vector<int32_t> lrp, lrc, lrv, urp, urc, urv;
     std::vector<int32_t> blk_ptr, lsl_blk, lsl_ptr, lsl_rows, usl_blk, usl_ptr, usl_rows;
+    std::vector<int32_t> lsl_nlong, usl_nlong;
     std::vector<int32_t> odp, odax, odsrc, blk_has_od;
     // level statistics
     int32_t max_flev_width = 0, n_lsl = 0, n_usl = 0;
@@ -75,5 +87,31 @@ void build_identity_plan(i64 n, const i64* ap, const i64* ai, int kind, PlanHost
 
 // Append one plan's int arrays to a pool and fill the int offsets of d.
 void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
+
+// Right-looking schedule of the static-pivot refactor. Pivot j runs in phase
+// lev(j) = 1 + max lev over the columns it waits for (U(:,j) and the columns
+// j' with L(j,j') != 0). Every update L(r,j)*U(j,g) of an entry (r,g) is
+// applied in phase max(lev of the entry's pivots so far, in the column's U
+// order), so each entry sees the reference's updates in the reference's order
+// while the entries of one phase are updated in parallel. Value indices are
+// relative to lx with lx | pad | ux | pad | ud contiguous.
+constexpr int32_t kRlLongTask = 24;  // a task with this many updates takes a wavefront
+
+struct RlHost {
+    int32_t nph = 0, nv = 0, ntask = 0, npair = 0;
+    std::vector<int32_t> ini_src, ini_cnt;  // per value slot: run of ax entries summed into it
+    std::vector<int32_t> div_ptr, div;      // per phase: (L slot, diagonal slot) pairs
+    std::vector<int32_t> piv_ptr, piv;      // per phase: pivot columns
+    std::vector<int32_t> task_ptr, task_nlong, task_tgt, task_beg;  // long tasks first per phase
+    std::vector<int32_t> pair;              // (L slot, U slot) pairs of each task, in order
+    std::vector<int32_t> sgl_ptr, sgl;      // per phase: one-update tasks as (target, L, U, 0)
+    int32_t nsgl = 0;
+};
+
+// false when the plan has no such schedule (a column's U order that is not
+// topological, or an update outside the column's pattern); the column kernel
+// then stays in use.
+bool build_rl_plan(const PlanHost& p, RlHost& r);
+void append_rl(const RlHost& r, std::vector<int32_t>& pool, PlanDev& d);
 
 }  // namespace rsgpu

@@ -132,7 +132,7 @@ void compute_rcond(rs_solver* h, const double* ud, i64 n) {
 // after prepare() or met by another handle is not timed again.
 struct TunedShapes {
     int ref_threads = 0, sol_threads = 0;
-    bool ref_lds = false, sol_lds = false;
+    int ref_mode = 0, sol_mode = 0;
     float ref_us = 0.0f, sol_us = 0.0f;
     bool have_ref = false, have_sol = false;
 };
@@ -157,6 +157,97 @@ uint64_t pattern_key(const rs_solver* h) {
     return x;
 }
 
+// Whether refactor mode m (bit 0 LDS, bit 1 right-looking, 4 step program)
+// can run with t threads on this plan.
+bool refactor_mode_fits(const rs_solver* h, int t, int m) {
+    const size_t maxlds = device_max_lds();
+    if (m == 4) return h->sp_ok;
+    if (m & 2) {
+        if (h->pd.rl_nph == 0 && h->n > 0) return false;
+        return !(m & 1) || ((size_t)h->pd.rl_nv + t) * sizeof(double) <= maxlds;
+    }
+    return !(m & 1) || (size_t)(t / 64) * h->n * sizeof(double) <= maxlds;
+}
+int default_refactor_mode(const rs_solver* h, int t) {
+    for (int m : {4, 3, 2, 1, 0})
+        if (refactor_mode_fits(h, t, m)) return m;
+    return 0;
+}
+// Solve mode m: 0 solve vector in global memory, 1 in LDS, 2 step program.
+bool solve_mode_fits(const rs_solver* h, int t, int m) {
+    if (m == 2) return h->sp_ok;
+    return m == 0 || ((size_t)h->n + t) * sizeof(double) <= device_max_lds();
+}
+int default_solve_mode(const rs_solver* h, int t) {
+    for (int m : {2, 1, 0})
+        if (solve_mode_fits(h, t, m)) return m;
+    return 0;
+}
+
+// Step programs of the solver's own plan (RPI = identity), when their slots
+// fit the LDS and the kernels address it absolutely.
+bool build_solver_sp(rs_solver* h) {
+    static const bool static_ok = sp_solver_static_lds() == 0;
+    const SpLayout L = sp_layout(h->ph);
+    if (!static_ok || L.nslots >= kSpMaxSlots || (size_t)(L.nslots + 1) * sizeof(double) > device_max_lds())
+        return false;
+    SpProgram fac, sol;
+    std::string why;
+    if (!build_sp_programs(h->ph, fac, sol, &why)) return false;
+    std::vector<int32_t> ss;
+    std::vector<int32_t> rpi(h->ph.n);
+    for (int32_t i = 0; i < h->ph.n; i++) rpi[i] = i;
+    sp_scatter_slots(h->ph, rpi.data(), ss);
+    SpSolverDev& sd = h->spd;
+    sd = SpSolverDev{};
+    sd.n = L.n;
+    sd.nl = L.nl;
+    sd.nu = L.nu;
+    sd.nod = L.nod;
+    sd.nslots = L.nslots;
+    sd.t_base = L.t_base;
+    sd.od_base = L.od_base;
+    sd.nsc = (int32_t)ss.size();
+    sd.fac_chunks = fac.nsteps_pad / kSpChunk - 2;
+    sd.sol_chunks = sol.nsteps_pad / kSpChunk - 2;
+    sd.o_fac = 0;
+    sd.o_sol = (int64_t)fac.words.size();
+    h->sp_words = std::move(fac.words);
+    h->sp_words.insert(h->sp_words.end(), sol.words.begin(), sol.words.end());
+    sd.o_ss = 0;
+    sd.o_rpi = (int32_t)ss.size();
+    h->sp_ints = std::move(ss);
+    h->sp_ints.insert(h->sp_ints.end(), rpi.begin(), rpi.end());
+    h->sp_ints.push_back(0);
+    return true;
+}
+
+// Largest right-looking schedule built (updates; 8 bytes each on the device,
+// about 70 ns each to build on the host).
+constexpr double kRlMaxUpdates = 32.0 * 1024 * 1024;
+
+uint64_t plan_key(const PlanHost& p) {
+    uint64_t x = 1469598103934665603ull;
+    auto mix = [&](const std::vector<int32_t>& v) {
+        x ^= v.size();
+        x *= 1099511628211ull;
+        for (int32_t a : v) {
+            x ^= (uint32_t)a;
+            x *= 1099511628211ull;
+        }
+    };
+    mix(p.lcp);
+    mix(p.lri);
+    mix(p.ucp);
+    mix(p.uri);
+    mix(p.colflag);
+    mix(p.sc_ptr);
+    mix(p.sc_src);
+    mix(p.sc_cnt);
+    mix(p.sc_dst);
+    return x;
+}
+
 int32_t upload_plan(rs_solver* h, i64 nnz) {
     std::vector<int32_t> pool;
     PlanDev d{};
@@ -169,11 +260,34 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
     d.o_vec = 0;
     d.o_work = 0;
     d.o_tvec = (int64_t)(kMaxThreads / 64) * n;  // work vectors for up to 16 wavefronts
-    size_t maxlds = device_max_lds();
+    d.flags = PLAN_CSR;
+    d.o_lxr = d.o_ud + n + 1;
+    d.o_uxr = d.o_lxr + h->ph.nl + 1;
+    const int64_t nvals = d.o_uxr + (h->ph.nu - n) + 1;
+    {
+        // the right-looking schedule and the step programs, rebuilt only when
+        // the plan changes (a DC OP re-analyzes the same pattern after prepare())
+        const uint64_t key = plan_key(h->ph);
+        if (key != h->rl_key || h->rl.nv == 0) {
+            h->rl_key = key;
+            h->rl_ok = h->ph.refactor_flops <= 2.0 * kRlMaxUpdates && build_rl_plan(h->ph, h->rl);
+            if (!h->rl_ok) h->rl = RlHost();
+        }
+        if (h->rl_ok) append_rl(h->rl, pool, d);
+        if (key != h->sp_key) {
+            h->sp_key = key;
+            h->sp_ok = build_solver_sp(h);
+            if (!h->sp_ok) {
+                h->sp_words.clear();
+                h->sp_ints.clear();
+            }
+        }
+    }
+    h->pd = d;  // read by the shape helpers below
     if (!h->forced) {
         h->ref_threads = h->sol_threads = kThreads;
-        h->lds_refactor = (size_t)(kThreads / 64) * n * sizeof(double) <= maxlds;
-        h->lds_solve = (size_t)n * sizeof(double) <= maxlds;
+        h->ref_mode = default_refactor_mode(h, kThreads);
+        h->sol_mode = default_solve_mode(h, kThreads);
     }
     h->tuned_ref = h->tuned_sol = false;
     h->tune_key = pattern_key(h);
@@ -184,22 +298,36 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
             const TunedShapes& t = it->second;
             if (t.have_ref) {
                 h->ref_threads = t.ref_threads;
-                h->lds_refactor = t.ref_lds;
+                h->ref_mode = t.ref_mode;
                 h->tune_ref_us = t.ref_us;
                 h->tuned_ref = true;
             }
             if (t.have_sol) {
                 h->sol_threads = t.sol_threads;
-                h->lds_solve = t.sol_lds;
+                h->sol_mode = t.sol_mode;
                 h->tune_sol_us = t.sol_us;
                 h->tuned_sol = true;
             }
         }
     }
+    if (h->forced && (!refactor_mode_fits(h, h->ref_threads, h->ref_mode) ||
+                      !solve_mode_fits(h, h->sol_threads, h->sol_mode)))
+        return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: the fixed launch shape does not fit this plan");
     hipError_t e;
+    if (h->sp_ok) {
+        if ((e = h->d_spd.ensure(1)) != hipSuccess || (e = h->d_spw.ensure(h->sp_words.size())) != hipSuccess ||
+            (e = h->d_spi.ensure(h->sp_ints.size())) != hipSuccess)
+            return hip_fail(h, e, "hipMalloc(step programs)");
+        if ((e = hipMemcpy(h->d_spd.p, &h->spd, sizeof(SpSolverDev), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->d_spw.p, h->sp_words.data(), h->sp_words.size() * sizeof(SpOpWord),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->d_spi.p, h->sp_ints.data(), h->sp_ints.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(h, e, "hipMemcpy(step programs)");
+    }
     if ((e = h->d_desc.ensure(1)) != hipSuccess) return hip_fail(h, e, "hipMalloc(desc)");
     if ((e = h->d_ip.ensure(pool.size())) != hipSuccess) return hip_fail(h, e, "hipMalloc(plan)");
-    if ((e = h->d_vals.ensure(d.o_ud + n + 1)) != hipSuccess) return hip_fail(h, e, "hipMalloc(vals)");
+    if ((e = h->d_vals.ensure(nvals)) != hipSuccess) return hip_fail(h, e, "hipMalloc(vals)");
     if ((e = h->d_ax.ensure(nnz)) != hipSuccess) return hip_fail(h, e, "hipMalloc(ax)");
     if ((e = h->d_rhs.ensure(n)) != hipSuccess) return hip_fail(h, e, "hipMalloc(rhs)");
     if ((e = h->d_x.ensure(n)) != hipSuccess) return hip_fail(h, e, "hipMalloc(x)");
@@ -233,6 +361,17 @@ int32_t upload_values(rs_solver* h) {
         return hip_fail(h, e, "hipMemcpy(ux)");
     if ((e = hipMemcpy(h->d_vals.p + h->pd.o_ud, ud.data(), ud.size() * 8, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(h, e, "hipMemcpy(ud)");
+    if (h->pd.flags & PLAN_CSR) {  // row-ordered copies read by the solve
+        std::vector<double> lxr(h->ph.lrv.size()), uxr(h->ph.urv.size());
+        for (size_t k = 0; k < lxr.size(); k++) lxr[k] = lx[h->ph.lrv[k]];
+        for (size_t k = 0; k < uxr.size(); k++) uxr[k] = ux[h->ph.urv[k]];
+        if (!lxr.empty() && (e = hipMemcpy(h->d_vals.p + h->pd.o_lxr, lxr.data(), lxr.size() * 8,
+                                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(h, e, "hipMemcpy(lxr)");
+        if (!uxr.empty() && (e = hipMemcpy(h->d_vals.p + h->pd.o_uxr, uxr.data(), uxr.size() * 8,
+                                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(h, e, "hipMemcpy(uxr)");
+    }
     compute_rcond(h, ud.data(), (i64)ud.size());
     return RS_OK;
 }
@@ -283,31 +422,45 @@ int32_t host_full_factor(rs_solver* h, const i64* ap, const i64* ai, const doubl
     return upload_values(h);
 }
 
-// Launch-shape candidates: 1 to 16 wavefronts, work vectors in LDS when they
-// fit. Every shape gives bit-identical values (a column or a row is always
-// handled by one wavefront or one thread with a fixed order), so the choice
-// is a pure timing decision.
+// Launch-shape candidates: 1 to 16 wavefronts; the column or the
+// right-looking refactor schedule; work vectors (column) or values
+// (right-looking) in LDS when they fit. Every shape gives bit-identical values
+// (each entry sees the same updates in the same order), so the choice is a
+// pure timing decision.
 struct Shape {
     int threads;
-    bool lds;
+    int mode;  // refactor_mode_fits / solve_mode_fits
 };
-std::vector<Shape> refactor_shapes(i64 n) {
+std::vector<Shape> refactor_shapes(const rs_solver* h) {
     std::vector<Shape> v;
-    const size_t maxlds = device_max_lds();
-    for (int t = 64; t <= kMaxThreads; t *= 2) {
-        if ((size_t)(t / 64) * n * sizeof(double) <= maxlds) v.push_back({t, true});
-        v.push_back({t, false});
-    }
+    for (int t = 64; t <= kMaxThreads; t *= 2)
+        for (int m : {4, 3, 2, 1, 0})
+            if (refactor_mode_fits(h, t, m)) v.push_back({t, m});
     return v;
 }
-std::vector<Shape> solve_shapes(i64 n) {
+std::vector<Shape> solve_shapes(const rs_solver* h) {
     std::vector<Shape> v;
-    const bool fits = (size_t)n * sizeof(double) <= device_max_lds();
-    for (int t = 64; t <= kMaxThreads; t *= 2) {
-        if (fits) v.push_back({t, true});
-        v.push_back({t, false});
-    }
+    for (int t = 64; t <= kMaxThreads; t *= 2)
+        for (int m : {2, 1, 0})
+            if (solve_mode_fits(h, t, m)) v.push_back({t, m});
     return v;
+}
+
+hipError_t refactor_launch(rs_solver* h, const double* ax_dev, int mode, int threads, hipStream_t s) {
+    if (mode == 4)
+        return launch_sp_refactor(h->d_spd.p, h->d_desc.p, h->d_ip.p, h->d_spi.p, h->d_spw.p, ax_dev,
+                                  h->d_vals.p, h->spd.nslots, threads, s);
+    return launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p, (int)h->n,
+                           mode, threads, s, h->pd.rl_nv);
+}
+
+hipError_t solve_launch(rs_solver* h, const double* rhs_dev, double* x_dev, int mode, int threads,
+                        hipStream_t s) {
+    if (mode == 2)
+        return launch_sp_solve(h->d_spd.p, h->d_desc.p, h->d_ip.p, h->d_spi.p, h->d_spw.p, h->d_ax.p,
+                               h->d_vals.p, rhs_dev, x_dev, h->spd.nslots, threads, s);
+    return launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev, h->d_work.p,
+                        (int)h->n, mode == 1, threads, s);
 }
 
 // Mean time of `reps` launches of f on stream s (after one warm-up), in us.
@@ -330,14 +483,11 @@ float time_launches(F&& f, hipStream_t s, int reps, hipError_t& err) {
 
 int32_t tune_refactor(rs_solver* h, const double* ax_dev, hipStream_t s) {
     float best = -1.0f;
-    Shape pick{h->ref_threads, h->lds_refactor};
-    for (const Shape& sh : refactor_shapes(h->n)) {
+    Shape pick{h->ref_threads, h->ref_mode};
+    for (const Shape& sh : refactor_shapes(h)) {
         hipError_t e;
         const float us = time_launches(
-            [&] {
-                return launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
-                                       (int)h->n, sh.lds, sh.threads, s);
-            },
+            [&] { return refactor_launch(h, ax_dev, sh.mode, sh.threads, s); },
             s, 3, e);
         if (e != hipSuccess) return hip_fail(h, e, "k_refactor (autotune)");
         if (us >= 0.0f && (best < 0.0f || us < best)) {
@@ -346,13 +496,13 @@ int32_t tune_refactor(rs_solver* h, const double* ax_dev, hipStream_t s) {
         }
     }
     h->ref_threads = pick.threads;
-    h->lds_refactor = pick.lds;
+    h->ref_mode = pick.mode;
     h->tune_ref_us = best;
     h->tuned_ref = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     TunedShapes& t = g_tuned[h->tune_key];
     t.ref_threads = pick.threads;
-    t.ref_lds = pick.lds;
+    t.ref_mode = pick.mode;
     t.ref_us = best;
     t.have_ref = true;
     return RS_OK;
@@ -370,8 +520,7 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
         if (rc) return rc;
     }
     int ps = prof_begin(PK_REFACTOR, s);
-    e = launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p,
-                        (int)h->n, h->lds_refactor, h->ref_threads, s);
+    e = refactor_launch(h, ax_dev, h->ref_mode, h->ref_threads, s);
     prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_refactor launch");
     if (sync_rcond) {
@@ -439,14 +588,10 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
         hipError_t e = h->d_tune.ensure((size_t)h->n);
         if (e != hipSuccess) return hip_fail(h, e, "hipMalloc(tune)");
         float best = -1.0f;
-        Shape pick{h->sol_threads, h->lds_solve};
-        for (const Shape& sh : solve_shapes(h->n)) {
+        Shape pick{h->sol_threads, h->sol_mode};
+        for (const Shape& sh : solve_shapes(h)) {
             const float us = time_launches(
-                [&] {
-                    return launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev,
-                                        h->d_tune.p, h->d_work.p, (int)h->n, sh.lds, sh.threads, s);
-                },
-                s, 3, e);
+                [&] { return solve_launch(h, rhs_dev, h->d_tune.p, sh.mode, sh.threads, s); }, s, 3, e);
             if (e != hipSuccess) return hip_fail(h, e, "k_solve (autotune)");
             if (us >= 0.0f && (best < 0.0f || us < best)) {
                 best = us;
@@ -454,19 +599,18 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
             }
         }
         h->sol_threads = pick.threads;
-        h->lds_solve = pick.lds;
+        h->sol_mode = pick.mode;
         h->tune_sol_us = best;
         h->tuned_sol = true;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         TunedShapes& t = g_tuned[h->tune_key];
         t.sol_threads = pick.threads;
-        t.sol_lds = pick.lds;
+        t.sol_mode = pick.mode;
         t.sol_us = best;
         t.have_sol = true;
     }
     int ps = prof_begin(PK_SOLVE, s);
-    hipError_t e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev,
-                                h->d_work.p, (int)h->n, h->lds_solve, h->sol_threads, s);
+    hipError_t e = solve_launch(h, rhs_dev, x_dev, h->sol_mode, h->sol_threads, s);
     prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
     return RS_OK;
@@ -618,28 +762,35 @@ void rs_set_autotune(rs_solver* h, int32_t on) {
     h->forced = false;
 }
 
-int32_t rs_set_launch_shape(rs_solver* h, int32_t ref_threads, int32_t ref_lds, int32_t sol_threads,
-                            int32_t sol_lds) {
+int32_t rs_set_launch_shape(rs_solver* h, int32_t ref_threads, int32_t ref_mode, int32_t sol_threads,
+                            int32_t sol_mode) {
     auto ok = [](int t) { return t >= 64 && t <= kMaxThreads && (t & (t - 1)) == 0; };
-    if (!ok(ref_threads) || !ok(sol_threads))
-        return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: launch threads must be 64..1024, a power of 2");
-    const size_t maxlds = device_max_lds();
-    if (h->n > 0 && ((ref_lds && (size_t)(ref_threads / 64) * h->n * sizeof(double) > maxlds) ||
-                     (sol_lds && (size_t)h->n * sizeof(double) > maxlds)))
-        return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: work vectors do not fit in LDS");
+    if (!ok(ref_threads) || !ok(sol_threads) || ref_mode < 0 || ref_mode > 4 || sol_mode < 0 ||
+        sol_mode > 2)
+        return fail(h, RS_ERR_INVALID_MATRIX,
+                    "Invalid matrix: launch threads must be 64..1024, a power of 2; refactor mode 0..4, "
+                    "solve mode 0..2");
+    // the right-looking schedule and the step programs are known once a
+    // factor has built the plan; until then their fit is checked by the next
+    // factor
+    const bool kref = h->plan_on_device || !(ref_mode & 6);
+    const bool ksol = h->plan_on_device || sol_mode != 2;
+    if (h->n > 0 && ((kref && !refactor_mode_fits(h, ref_threads, ref_mode)) ||
+                     (ksol && !solve_mode_fits(h, sol_threads, sol_mode))))
+        return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: launch shape does not fit this plan or the LDS");
     h->ref_threads = ref_threads;
-    h->lds_refactor = ref_lds != 0;
+    h->ref_mode = ref_mode;
     h->sol_threads = sol_threads;
-    h->lds_solve = sol_lds != 0;
+    h->sol_mode = sol_mode;
     h->forced = true;
     return RS_OK;
 }
 
 int32_t rs_launch_shape(const rs_solver* h, int64_t* out) {
     out[0] = h->ref_threads;
-    out[1] = h->lds_refactor ? 1 : 0;
+    out[1] = h->ref_mode;
     out[2] = h->sol_threads;
-    out[3] = h->lds_solve ? 1 : 0;
+    out[3] = h->sol_mode;
     out[4] = (int64_t)(h->tune_ref_us * 1000.0f);
     out[5] = (int64_t)(h->tune_sol_us * 1000.0f);
     return (h->tuned_ref ? 1 : 0) | (h->tuned_sol ? 2 : 0) | (h->forced ? 4 : 0);

@@ -8,15 +8,25 @@
 
 #include "../../include/rsgpu.h"
 #include "plan.hpp"
+#include "sp.hpp"
 
 namespace rsgpu {
+// mode: bit 0 LDS, bit 1 right-looking schedule (rl_nv value slots)
 hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
-                           double* vals, double* wpool, int max_n, bool lds, int threads,
-                           hipStream_t stream);
+                           double* vals, double* wpool, int max_n, int mode, int threads,
+                           hipStream_t stream, int rl_nv = 0);
 hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
                         bool lds, int threads, hipStream_t stream);
 size_t device_max_lds();
+hipError_t launch_sp_refactor(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
+                              const int32_t* sip, const SpOpWord* words, const double* ax,
+                              double* vals, int nslots, int threads, hipStream_t s);
+hipError_t launch_sp_solve(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
+                           const int32_t* sip, const SpOpWord* words, const double* ax,
+                           const double* vals, const double* rhs, double* x, int nslots, int threads,
+                           hipStream_t s);
+size_t sp_solver_static_lds();
 
 struct BbdImpl;
 struct DenseImpl;
@@ -80,13 +90,25 @@ struct rs_solver {
     i64 threads_hint = 0;
     std::string err;
     size_t plan_bytes = 0;
-    bool lds_refactor = false, lds_solve = false;
+    int ref_mode = 0;  // bit 0 LDS, bit 1 right-looking schedule, 4 step program
+    int sol_mode = 0;  // 0 solve vector in global memory, 1 in LDS, 2 step program
     // launch shapes of k_refactor / k_solve: chosen by timing every candidate
     // on the first device refactor and solve of a pattern (autotune), or set
     int ref_threads = rsgpu::kThreads, sol_threads = rsgpu::kThreads;
     bool autotune = true, tuned_ref = false, tuned_sol = false, forced = false;
     float tune_ref_us = 0.0f, tune_sol_us = 0.0f;  // best times found
     uint64_t tune_key = 0;                         // pattern key of the tuning cache
+    uint64_t rl_key = 0;                           // plan the cached right-looking schedule belongs to
+    bool rl_ok = false;
+    rsgpu::RlHost rl;
+    uint64_t sp_key = 0;                           // plan the cached step programs belong to
+    bool sp_ok = false;                            // step programs built and on the device
+    rsgpu::SpSolverDev spd{};
+    std::vector<rsgpu::SpOpWord> sp_words;
+    std::vector<int32_t> sp_ints;
+    rsgpu::DevBuf<rsgpu::SpSolverDev> d_spd;
+    rsgpu::DevBuf<rsgpu::SpOpWord> d_spw;
+    rsgpu::DevBuf<int32_t> d_spi;
     rsgpu::DevBuf<double> d_tune;                  // solve output while tuning
     rsgpu::DevBuf<rsgpu::PlanDev> d_desc;
     rsgpu::DevBuf<int32_t> d_ip;

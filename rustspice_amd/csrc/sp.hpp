@@ -86,6 +86,16 @@ struct SpProgram {
 
 SpLayout sp_layout(const PlanHost& p);
 
+// Step programs of one solver's own plan (positions after its pivots, so the
+// row map RPI is the identity), run by k_sp_refactor / k_sp_solve: offsets of
+// the op words in a word buffer and of the scatter slots (nsc) and RPI (n) in
+// an int buffer.
+struct SpSolverDev {
+    int32_t n, nl, nu, nod, nslots, t_base, od_base, nsc;
+    int32_t fac_chunks, sol_chunks, o_ss, o_rpi;
+    int64_t o_fac, o_sol;
+};
+
 // Builds the refactor and solve programs of a plan whose positions are
 // pre-pivot (the batched kernel's identity-pivot analysis). Returns false
 // with a reason when the slots exceed kSpMaxSlots.

@@ -113,15 +113,18 @@ class GpuSolver:
         pattern and keep the fastest (default on)."""
         self._L.rs_set_autotune(self.h, int(bool(on)))
 
-    def set_launch_shape(self, ref_threads, ref_lds, sol_threads, sol_lds):
-        self._check(self._L.rs_set_launch_shape(self.h, ref_threads, int(ref_lds), sol_threads,
-                                                int(sol_lds)))
+    def set_launch_shape(self, ref_threads, ref_mode, sol_threads, sol_mode):
+        """ref_mode: 0/1 column kernel (1: LDS work vectors), 2/3 right-looking
+        schedule (3: values in LDS), 4 step program; sol_mode: 0 solve vector
+        in global memory, 1 in LDS, 2 step program."""
+        self._check(self._L.rs_set_launch_shape(self.h, ref_threads, int(ref_mode), sol_threads,
+                                                int(sol_mode)))
 
     def launch_shape(self) -> dict:
         out = (C.c_int64 * 6)()
         flags = self._L.rs_launch_shape(self.h, out)
-        return dict(ref_threads=out[0], ref_lds=bool(out[1]), sol_threads=out[2],
-                    sol_lds=bool(out[3]), ref_us=out[4] / 1000.0, sol_us=out[5] / 1000.0,
+        return dict(ref_threads=out[0], ref_mode=int(out[1]), sol_threads=out[2],
+                    sol_mode=int(out[3]), ref_us=out[4] / 1000.0, sol_us=out[5] / 1000.0,
                     tuned_ref=bool(flags & 1), tuned_sol=bool(flags & 2), forced=bool(flags & 4))
 
     def stats(self) -> dict:
@@ -231,18 +234,38 @@ def host_stamp_pattern(circuit, mode=0):
     return ap, ai[: nnz.value]
 
 
-def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU):
+def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU, own_plan=False):
     """Host model of the GPU step programs: first factor on ax0 (pivots),
-    refactor on ax1, solve rhs. Returns (x, stats)."""
+    refactor on ax1, solve rhs. own_plan: the programs of the single-matrix
+    solver's plan (positions after the pivots) instead of the batched
+    kernel's identity-pivot plan. Returns (x, stats)."""
     L = _lib.lib()
     x = np.zeros(n)
     st = np.zeros(8, np.int64)
-    rc = L.rs_host_sp_selftest(kind, n, _i64(ap), _i64(ai), _f64(ax0), _f64(ax1), _f64(rhs), x, st)
+    k = kind | (0x100 if own_plan else 0)
+    rc = L.rs_host_sp_selftest(k, n, _i64(ap), _i64(ai), _f64(ax0), _f64(ax1), _f64(rhs), x, st)
     if rc:
         raise SolverError(rc, "step program")
     keys = ["refactor_steps", "refactor_ops", "solve_steps", "solve_ops", "refactor_slow_chunks",
             "solve_slow_chunks", "refactor_forwarded", "solve_forwarded"]
     return x, dict(zip(keys, map(int, st)))
+
+
+def host_rl_selftest(n, ap, ai, ax0, ax1, kind=NATIVEKLU):
+    """Host model of the right-looking refactor schedule: first factor on ax0
+    (pivots), refactor on ax1 phase by phase. Returns (lx, ux, ud, stats) in
+    the solver's position order."""
+    L = _lib.lib()
+    hf = HostFactor(n, ap, ai, ax0, kind)
+    st0 = hf.plan_stats()
+    lx, ux, ud = np.zeros(max(st0["nnz_l"], 1)), np.zeros(max(st0["nnz_u"], 1)), np.zeros(max(n, 1))
+    st = np.zeros(8, np.int64)
+    rc = L.rs_host_rl_selftest(kind, n, _i64(ap), _i64(ai), _f64(ax0), _f64(ax1), lx, ux, ud, st)
+    if rc:
+        raise SolverError(rc, "no right-looking schedule")
+    keys = ["phases", "tasks", "updates", "long_tasks", "column_levels", "nnz_l", "nnz_u", "slots"]
+    stats = dict(zip(keys, map(int, st)))
+    return lx[: stats["nnz_l"]], ux[: stats["nnz_u"]], ud[:n], stats
 
 
 def host_bbd_decompose(n, ap, ai, num_blocks):

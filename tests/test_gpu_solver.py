@@ -201,24 +201,33 @@ def test_every_launch_shape_is_bit_identical(gpu, case):
     xo = rhs.copy()
     o.solve(xo)
     want = [o.block(b) for b in range(o.stats()["nblocks"])]
-    tried = 0
+    tried, modes = 0, set()
     for t in (64, 128, 256, 512, 1024):
-        for lds in (True, False):
+        # refactor: 0/1 column kernel, 2/3 right-looking schedule, 4 step
+        # program; solve: 0 global, 1 LDS, 2 step program
+        for mode in (0, 1, 2, 3, 4):
+            smode = {0: 0, 1: 1, 2: 0, 3: 1, 4: 2}[mode]
             g = S.GpuSolver(n)
             g.prepare(n)
             g.analyze(ap, ai)
-            try:  # shapes whose work vectors do not fit in this device's LDS are refused
-                g.set_launch_shape(t, lds, t, lds)
-            except S.SolverError:
+            try:  # shapes that do not fit this plan or this device's LDS are refused
+                g.set_launch_shape(t, mode, t, smode)
+                (lx, ux, ud), xg = _refactor_solve(g, ap, ai, ax, ax2, rhs)
+            except S.SolverError as e:
+                assert e.variant == "InvalidMatrix", e
                 g.close()
                 continue
-            (lx, ux, ud), xg = _refactor_solve(g, ap, ai, ax, ax2, rhs)
-            assert same_bits(xg, xo), (t, lds)
+            assert same_bits(xg, xo), (t, mode)
             if len(want) == 1:
-                assert same_bits(lx, want[0]["l_values"]) and same_bits(ud, want[0]["u_diag"]), (t, lds)
-            assert g.launch_shape()["forced"]
+                assert same_bits(lx, want[0]["l_values"]) and same_bits(ud, want[0]["u_diag"]), (t, mode)
+            sh = g.launch_shape()
+            assert sh["forced"] and sh["ref_mode"] == mode and sh["sol_mode"] == smode
             g.close()
             tried += 1
+            modes.add(mode)
+    assert {0, 2} <= modes  # the column and the right-looking kernels both ran
+    if case == "chain":
+        assert 4 in modes  # the step programs fit the chain's LDS
     assert tried >= 6
 
 

@@ -47,6 +47,19 @@ def test_circuit_refactor_solve_bit_exact(name, kind):
     assert st["refactor_ops"] > 0 and st["solve_ops"] > 0
 
 
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+@pytest.mark.parametrize("name", sorted(CIRCUITS))
+def test_solver_plan_programs_bit_exact(name, kind):
+    """The single-matrix solver runs the programs of its own plan (positions
+    after the first factor's pivots, RPI the identity)."""
+    c = CIRCUITS[name]()
+    rs = np.random.RandomState(9)
+    ap, ai, ax0, _, n = O.build_mna(c, x=np.zeros(c.node_count), mode=0)
+    _, _, ax1, rhs, _ = O.build_mna(c, x=rs.rand(c.node_count) * 1.2 - 0.1, mode=0)
+    x, st = S.host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind, own_plan=True)
+    assert same_bits(x, _oracle(ap, ai, ax0, ax1, rhs, n, kind))
+
+
 def test_random_matrices_with_pivoting_and_btf():
     """Random patterns (duplicates, structural zeros on the diagonal, several
     BTF blocks with off-diagonal couplings) and values that make the first
@@ -62,8 +75,10 @@ def test_random_matrices_with_pivoting_and_btf():
         ax1 = ax0 * (1.0 + 0.1 * rs.randn(len(ai)))
         rhs = rs.randn(n)
         for kind in (S.NATIVEKLU, S.SPARSELU):
-            x, _ = S.host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind)
-            assert same_bits(x, _oracle(ap, ai, ax0, ax1, rhs, n, kind)), (trial, kind)
+            want = _oracle(ap, ai, ax0, ax1, rhs, n, kind)
+            for own in (False, True):
+                x, _ = S.host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind, own_plan=own)
+                assert same_bits(x, want), (trial, kind, own)
             checked += 1
     assert checked > 80
 
