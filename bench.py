@@ -200,6 +200,40 @@ def mesh_refactor_solve(E, NL, S, torch, np, reps=10):
     return res, (ap, ai, ax, rhs, n, d_x.cpu().numpy())
 
 
+def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
+    """Transient adjoint on the RC mesh driven by level-1 inverters (the
+    adjoint workload at nx x nx with nx*nx/32 taps): dV(centre at t =
+    nsteps*2 ps)/dp for every R, C and MOSFET W, checkpoint every `every`
+    steps, against one forward fixed-step TRAP transient through the same
+    plain GPU backend."""
+    import numpy as np
+
+    c = NL.rc_mesh_inverters(nx, nx)
+    probe = NL.rc_mesh_probe(nx, nx)
+    dt = NL.C5_TRAN["dt"]
+    g = E.GpuCircuit(c)
+    s = S.GpuSolver(c.node_count)
+    st, _, _ = g.tran_fixed(s, dt, 2)  # warm-up: patterns, first factors, launch shapes
+    t0 = time.perf_counter()
+    st, xf, it = g.tran_fixed(s, dt, nsteps)
+    t_fwd = time.perf_counter() - t0
+    g.tran_adjoint(s, dt, 2, probe, every)  # warm-up of the adjoint's solvers
+    t0 = time.perf_counter()
+    dp, xa, info = g.tran_adjoint(s, dt, nsteps, probe, every)
+    t_adj = time.perf_counter() - t0
+    kinds = [d["kind"] for d in c.devices]
+    return dict(workload=f"{nx}x{nx} RC mesh, {nx * nx // 32} inverter taps, {nsteps} TRAP steps of 2 ps, "
+                         f"checkpoint every {every}; dV(centre)/d(every R, C, W)",
+                n=int(len(xa)), forward_status=int(st), forward_ms=1e3 * t_fwd, forward_newton_iters=int(it),
+                adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd, adjoint_info=info,
+                sensitivities=dict(R=kinds.count(NL.RS_DEV_R), C=kinds.count(NL.RS_DEV_C),
+                                   W=kinds.count(NL.RS_DEV_M)),
+                probe_v=float(xa[probe]),
+                max_abs_dp=dict(R=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_R]).max()),
+                                C=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_C]).max()),
+                                W=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_M]).max())))
+
+
 def cpu_baseline_mesh(mesh, np, reps=10):
     """The oracle's refactor + solve of the same mesh matrix on one host core,
     and whether the GPU result is bit-identical to it."""
@@ -341,6 +375,7 @@ def main():
         line["batched"] = batched_dc_ops(args, E, NL, S, torch)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_cost(E, NL, S, torch)
+        line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
         line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
     if not args.no_cpu:
         cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])

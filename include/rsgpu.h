@@ -173,12 +173,38 @@ int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz
  * x: a converged DC solution (x_len >= node_count, as returned by rs_dc_op);
  * probe: the unknown whose sensitivity is wanted (a node index). Solves
  * J(x)^T lambda = e_probe with J the exact Jacobian of the stamped DC
- * currents (SparseLU factor on the GPU, transposed triangular solves reusing
+ * currents (SparseLU factor on the GPU, in AMD order with the unknowns that
+ * lack a diagonal entry, the sources' aux rows, ordered last so that no pivot
+ * is a structural zero; transposed triangular solves reusing
  * it) and writes dx[probe]/dp for every device into dp[ndev]: resistors
  * d/dR, V and I sources d/dvalue, MOSFETs d/dW, diodes d/dIs, capacitors 0.
  * lambda (n entries) is optional. gmin: the value the DC OP used. */
 int32_t rs_dc_adjoint(rs_circuit *c, const double *x, int64_t x_len, int64_t probe, double gmin,
                       double *dp, double *lambda);
+/* Transient sensitivities by the discrete adjoint (no reference
+ * counterpart). Runs rs_tran_fixed's transient (DC OP, then nsteps TRAP
+ * steps of dt, the accepted-step capacitor history of stamp.rs:615-668
+ * between steps) on solver s, keeping a checkpoint (x_k, x_{k-1}, capacitor
+ * history) every `every` steps; then, segment by segment from the last
+ * checkpoint, recomputes the segment's steps and solves the adjoint steps
+ * backward: J_k^T lambda_k = e_probe [k = nsteps] - sum over the two later
+ * steps of (dF_j/dx_k)^T lambda_j (their capacitor currents read v_k), with
+ * J_k the exact Jacobian of step k's stamped currents (SparseLU as above,
+ * transposed solves with refinement). dp[ndev] receives dx_N[probe]/dp:
+ * resistors d/dR, capacitors d/dC, DC V and I sources d/dvalue, MOSFETs d/dW,
+ * diodes d/dIs; PULSE sources 0. probe is a node index. The adjoint
+ * differentiates each step's exact root: after the reference Newton loop
+ * converges (within its solver's factorization error), Newton steps on the
+ * exact residual J dx = b - A x polish x before it is stored, in the forward
+ * and in every recomputation alike. x_final (x_cap) gets the polished x_N.
+ * info->status: 0 ok, 1 DC OP failed, 2 a step failed (no gradient). */
+typedef struct rs_tadj_info {
+    int32_t status, checkpoints, max_refine, polish_iters;
+    int64_t forward_iters, recompute_iters, recomputed_steps, adjoint_solves;
+} rs_tadj_info;
+int32_t rs_tran_adjoint(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double dt,
+                        int64_t nsteps, int64_t probe, int64_t every, double *dp,
+                        double *x_final, int64_t x_cap, int64_t *x_len, rs_tadj_info *info);
 /* 1 (default): a DC OP whose solver starts from scratch (the circuit has aux
  * rows and the solver its default configuration, no stepping) runs as one
  * device-resident launch, with results identical to the host-driven loop;

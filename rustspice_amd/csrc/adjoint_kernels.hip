@@ -75,6 +75,20 @@ __global__ __launch_bounds__(kT) void k_solve_t(const rsgpu::PlanDev* __restrict
 
 __device__ __forceinline__ double at(const double* v, int xl, int i) { return i < xl ? v[i] : 0.0; }
 
+// r = b - A x with A the unmerged CSC through its row map (rp: row pointers,
+// rq: CSC entry of each row entry, in column order), one row per thread;
+// x beyond xl reads 0. Deterministic: each row sums in a fixed order.
+__global__ void k_resid_n(const int32_t* __restrict__ rp, const int32_t* __restrict__ rc,
+                          const int32_t* __restrict__ rq, const double* __restrict__ ax,
+                          const double* __restrict__ b, const double* __restrict__ x, int xl,
+                          double* __restrict__ r, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) s += ax[rq[e]] * at(x, xl, rc[e]);
+    r[i] = b[i] - s;
+}
+
 // r = e - J^T lam with J the unmerged CSC (one column per thread): the
 // residual of the transposed system for iterative refinement.
 __global__ void k_resid_t(const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
@@ -119,9 +133,18 @@ __global__ __launch_bounds__(kT) void k_refine_update(double* __restrict__ lam,
 // dV_probe/dp per device: R -> d/dR, V and I sources -> d/dvalue,
 // MOSFET -> d/dW (level 1: the stamped current is linear in W), diode -> d/dIs,
 // capacitor -> 0 (open in DC).
+// -lambda^T dF/dp of one step, per device (one thread each), written or
+// added to out (acc). Sources scale with the source-stepping factor of the
+// solve; PULSE sources contribute nothing (their stamp does not use the
+// value, and DC does not stamp them). Capacitors: in a TRAP step k the
+// current is I = (2C/dt)(v_k - v_{k-1}) - i_{k-1}, with the stored current
+// i_{k-1} = C (v_{k-1} - v_{k-2}) / dt from the step before (0 after the DC
+// OP: xp2 null), so dI/dC = (2/dt)(v_k - v_{k-1}) - (v_{k-1} - v_{k-2})/dt.
 __global__ void k_adjoint_contract(const rs_device* __restrict__ dev, int ndev,
                                    const int32_t* __restrict__ aux_of, const double* __restrict__ x,
-                                   int xl, const double* __restrict__ lam, double gmin,
+                                   int xl, const double* __restrict__ lam, double gmin, double scale,
+                                   int mode, const double* __restrict__ xp,
+                                   const double* __restrict__ xp2, double dt, int acc,
                                    double* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ndev) return;
@@ -135,10 +158,18 @@ __global__ void k_adjoint_contract(const rs_device* __restrict__ dev, int ndev,
         break;
     }
     case RS_DEV_V:
-        r = aux_of[k] >= 0 ? lam[aux_of[k]] : 0.0;
+        if (!(v.flags & RS_F_WAVEFORM)) r = aux_of[k] >= 0 ? lam[aux_of[k]] * scale : 0.0;
         break;
     case RS_DEV_I:
-        r = -(lam[a] - lam[b]);
+        if (!(v.flags & RS_F_WAVEFORM)) r = -(lam[a] - lam[b]) * scale;
+        break;
+    case RS_DEV_C:
+        if (mode == rsgpu::MODE_TRAN_TRAP && xp) {
+            const double vk = x[a] - x[b], v1 = xp[a] - xp[b];
+            double didc = (2.0 / dt) * (vk - v1);
+            if (xp2) didc -= (v1 - (xp2[a] - xp2[b])) / dt;
+            r = -(lam[a] - lam[b]) * didc;
+        }
         break;
     case RS_DEV_D: {  // F_a += id(Is), id linear in Is
         const double vt = 0.02585 * v.p[1];
@@ -161,7 +192,36 @@ __global__ void k_adjoint_contract(const rs_device* __restrict__ dev, int ndev,
     default:
         break;
     }
-    out[k] = r;
+    out[k] = acc ? out[k] + r : r;
+}
+
+// Right-hand side of the adjoint equation of TRAP step k (or of the DC OP,
+// k = 0): rhs = e_probe [k = N] - sum_j (dF_j/dx_k)^T lam_j over the two
+// later steps, whose capacitor currents read v_k:
+//   dI_{k+1}/dv_k = -2C/dt - C/dt [k >= 1],   dI_{k+2}/dv_k = C/dt.
+// One thread per unknown; the capacitors of a node (cap_ptr/cap_dev, with
+// the node's sign in bit 31) are summed in a fixed order.
+__global__ void k_tadj_rhs(const rs_device* __restrict__ dev, const int32_t* __restrict__ cap_ptr,
+                           const int32_t* __restrict__ cap_dev, int node_count,
+                           const double* __restrict__ lam1, const double* __restrict__ lam2,
+                           double dt, int k_ge1, int probe, int n, double* __restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    if (i < node_count) {
+        for (int q = cap_ptr[i]; q < cap_ptr[i + 1]; ++q) {
+            const int w = cap_dev[q];
+            const rs_device& v = dev[w & 0x7FFFFFFF];
+            const int a = v.node[0], b = v.node[1];
+            const double cdt = v.value / dt;
+            double coef = 0.0;
+            if (lam1) coef += (2.0 * cdt + (k_ge1 ? cdt : 0.0)) * (lam1[a] - lam1[b]);
+            if (lam2) coef -= cdt * (lam2[a] - lam2[b]);
+            s += (w < 0) ? -coef : coef;
+        }
+    }
+    if (i == probe) s += 1.0;
+    rhs[i] = s;
 }
 
 }  // namespace
@@ -182,6 +242,13 @@ hipError_t launch_resid_t(const int32_t* ap, const int32_t* ai, const double* ax
     return hipGetLastError();
 }
 
+hipError_t launch_resid_n(const int32_t* rp, const int32_t* rc, const int32_t* rq, const double* ax,
+                          const double* b, const double* x, int xl, double* r, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resid_n, dim3((n + kT - 1) / kT), dim3(kT), 0, s, rp, rc, rq, ax, b, x, xl, r, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_refine_update(double* lam, const double* delta, int n, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_refine_update, dim3(1), dim3(kT), 0, s, lam, delta, n, out);
     return hipGetLastError();
@@ -189,10 +256,20 @@ hipError_t launch_refine_update(double* lam, const double* delta, int n, double*
 
 hipError_t launch_adjoint_contract(const rs_device* dev, int ndev, const int32_t* aux_of,
                                    const double* x, int xl, const double* lam, double gmin,
-                                   double* out, hipStream_t s) {
+                                   double scale, int mode, const double* xp, const double* xp2,
+                                   double dt, int acc, double* out, hipStream_t s) {
     if (ndev <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_adjoint_contract, dim3((ndev + kT - 1) / kT), dim3(kT), 0, s, dev, ndev,
-                       aux_of, x, xl, lam, gmin, out);
+                       aux_of, x, xl, lam, gmin, scale, mode, xp, xp2, dt, acc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_tadj_rhs(const rs_device* dev, const int32_t* cap_ptr, const int32_t* cap_dev,
+                           int node_count, const double* lam1, const double* lam2, double dt,
+                           int k_ge1, int probe, int n, double* rhs, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tadj_rhs, dim3((n + kT - 1) / kT), dim3(kT), 0, s, dev, cap_ptr, cap_dev,
+                       node_count, lam1, lam2, dt, k_ge1, probe, n, rhs);
     return hipGetLastError();
 }
 

@@ -40,10 +40,16 @@ hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* va
                           double* lam, hipStream_t s);
 hipError_t launch_adjoint_contract(const rs_device* dev, int ndev, const int32_t* aux_of,
                                    const double* x, int xl, const double* lam, double gmin,
-                                   double* out, hipStream_t s);
+                                   double scale, int mode, const double* xp, const double* xp2,
+                                   double dt, int acc, double* out, hipStream_t s);
+hipError_t launch_tadj_rhs(const rs_device* dev, const int32_t* cap_ptr, const int32_t* cap_dev,
+                           int node_count, const double* lam1, const double* lam2, double dt,
+                           int k_ge1, int probe, int n, double* rhs, hipStream_t s);
 hipError_t launch_resid_t(const int32_t* ap, const int32_t* ai, const double* ax, const double* e,
                           const double* lam, double* r, int n, hipStream_t s);
 hipError_t launch_refine_update(double* lam, const double* delta, int n, double* out, hipStream_t s);
+hipError_t launch_resid_n(const int32_t* rp, const int32_t* rc, const int32_t* rq, const double* ax,
+                          const double* b, const double* x, int xl, double* r, int n, hipStream_t s);
 }  // namespace rsgpu
 
 using namespace rsgpu;
@@ -78,13 +84,19 @@ struct rs_circuit {
     bool device_newton = true;  // whole DC OP in one launch when equivalent
     rs_batch* dcb[2] = {nullptr, nullptr};  // one-instance batch per solver kind
     int64_t device_dc_runs = 0;
-    // adjoint: SparseLU factor of the exact Jacobian, L^T levels, vectors
-    rs_solver* adj = nullptr;
-    int64_t adj_factors = -1;  // adj factor_count when the L^T levels were built
-    int n_lt = 0;
-    DevBuf<int32_t> d_aux, d_ltp, d_ltc, d_jap, d_jai;
+    // adjoint: per pattern (0 DC, 1 TRAN) a SparseLU factor of the exact
+    // Jacobian with its L^T levels; shared vectors
+    struct AdjCtx {
+        rs_solver* s = nullptr;
+        int64_t factors = -1;  // factor_count when the L^T levels were built
+        int n_lt = 0;
+        DevBuf<int32_t> aux, ltp, ltc, jap, jai;
+        bool have_rows = false;        // row map of the stamped pattern (polish residuals)
+        DevBuf<int32_t> rp, rc, rq;
+    } adj[2];
+    int64_t polish_iters = 0;
     DevBuf<double> d_e, d_t, d_lam, d_dp, d_r, d_del, d_nrm;
-    int32_t adj_refine = 0;  // refinement steps of the last rs_dc_adjoint
+    int32_t adj_refine = 0;  // refinement steps of the last adjoint solve (largest over a TRAN)
     // BSIM devices: slot bases, model table, group masks of the last evaluation
     int64_t nbsim = 0;
     std::vector<int32_t> sbase;
@@ -261,7 +273,8 @@ int32_t run_newton(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::
 // run_newton_with_stepping (newton.rs:142-185)
 int32_t run_newton_stepping(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o,
                             std::vector<double>& x, int mode, double t, double dt,
-                            rs_newton_result* out, int32_t* iters_total) {
+                            rs_newton_result* out, int32_t* iters_total, double* gmin_used = nullptr,
+                            double* scale_used = nullptr) {
     double gstart = o->gmin * 1e3;
     if (gstart < 1e-6) gstart = 1e-6;
     rs_newton_result last{0, 0, 1, 0, 0.0};
@@ -277,6 +290,8 @@ int32_t run_newton_stepping(rs_circuit* c, rs_solver* sol, const rs_newton_opts*
             if (iters_total) *iters_total += r.iterations;
             if (r.converged) {
                 *out = r;
+                if (gmin_used) *gmin_used = gmin;
+                if (scale_used) *scale_used = scale;
                 return RS_OK;
             }
             last = r;
@@ -452,6 +467,174 @@ int32_t cap_update(rs_circuit* c, const std::vector<double>& x, double dt, int f
     return RS_OK;
 }
 
+// Adjoint factor: the exact Jacobian of the stamped currents at x (mode DC
+// or TRAN) in c->d_ax, factored by the mode's SparseLU solver (a device
+// refactor when the pattern is unchanged), and the L^T levels and J's int32
+// pattern of that factorization.
+int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t, double dt,
+                   double gmin, bool fresh) {
+    DevPattern* dp;
+    int32_t rc;
+    if ((rc = stamp_dev(c, mode, x_dev, xl, t, dt, gmin, 1.0, &dp, nullptr, kStampJacobian)))
+        return rc;
+    const i64 n = dp->p.n, nd = (i64)c->dev.size();
+    rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
+    if (!A.s) {
+        // SparseLU in one block (the transposed solve needs one), AMD order
+        // with the unknowns that have no diagonal entry (sources' aux rows)
+        // last: their pivots then carry the Schur complement of the nodes
+        // instead of a structural zero clamped to 1e-14
+        A.s = rs_solver_create(RS_SOLVER_SPARSELU, n);
+        A.s->ordering = 4;
+    }
+    if (fresh || A.s->n != n) rs_prepare(A.s, n);
+    if ((rc = solver_factor_dev(A.s, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
+                                c->d_ax.p, nullptr)))
+        return from_solver(c, A.s, rc);
+    rs_solver* a = A.s;
+    if (a->pd.nblk != 1) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint needs a single-block factor");
+    hipError_t e;
+    if (A.factors != a->factor_count || !A.ltp.p) {  // L^T levels of this factorization
+        const PlanHost& ph = a->ph;
+        std::vector<int32_t> lev(n, 0), ptr, cols;
+        int32_t nlev = 0;
+        for (i64 k = n - 1; k >= 0; k--) {
+            int32_t lv = 0;
+            for (int32_t q = ph.lcp[k]; q < ph.lcp[k + 1]; q++) lv = std::max(lv, lev[ph.lri[q]] + 1);
+            lev[k] = lv;
+            nlev = std::max(nlev, lv + 1);
+        }
+        ptr.assign(nlev + 1, 0);
+        for (i64 k = 0; k < n; k++) ptr[lev[k] + 1]++;
+        for (int32_t l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
+        cols.resize(n);
+        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+        for (i64 k = 0; k < n; k++) cols[fill[lev[k]]++] = (int32_t)k;
+        std::vector<int32_t> aux(nd);
+        for (i64 k = 0; k < nd; k++) aux[k] = dp->p.aux_of[k];
+        if ((e = A.ltp.ensure(ptr.size())) || (e = A.ltc.ensure(std::max<size_t>(cols.size(), 1))) ||
+            (e = A.aux.ensure(std::max<i64>(nd, 1))) ||
+            (e = hipMemcpy(A.ltp.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice)) ||
+            (!cols.empty() && (e = hipMemcpy(A.ltc.p, cols.data(), cols.size() * 4, hipMemcpyHostToDevice))) ||
+            (nd && (e = hipMemcpy(A.aux.p, aux.data(), nd * 4, hipMemcpyHostToDevice))))
+            return chip(c, e, "adjoint levels");
+        std::vector<int32_t> jap(n + 1), jai(std::max<size_t>(dp->p.ai.size(), 1));
+        for (i64 i = 0; i <= n; i++) jap[i] = (int32_t)dp->p.ap[i];
+        for (size_t q = 0; q < dp->p.ai.size(); q++) jai[q] = (int32_t)dp->p.ai[q];
+        if ((e = A.jap.ensure(n + 1)) || (e = A.jai.ensure(jai.size())) ||
+            (e = hipMemcpy(A.jap.p, jap.data(), jap.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(A.jai.p, jai.data(), jai.size() * 4, hipMemcpyHostToDevice)))
+            return chip(c, e, "adjoint pattern");
+        A.n_lt = nlev;
+        A.factors = a->factor_count;
+    }
+    return RS_OK;
+}
+
+// J^T lam = rhs with the factor of the last adj_factor of this mode (J still
+// in c->d_ax), refined against the exact J^T (a pivot below SparseLU's 1e-14
+// clamp would otherwise leave its rank-one error in lambda).
+int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_dev) {
+    rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
+    rs_solver* a = A.s;
+    const i64 n = a->n;
+    hipStream_t s = nullptr;
+    hipError_t e;
+    if ((e = c->d_t.ensure(n)) || (e = c->d_r.ensure(n)) ||
+        (e = c->d_del.ensure(n)) || (e = c->d_nrm.ensure(2)) ||
+        (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, rhs_dev, A.ltp.p, A.ltc.p, A.n_lt,
+                            c->d_t.p, lam_dev, s)))
+        return chip(c, e, "adjoint solve");
+    double prev = 1.7976931348623157e308;
+    for (int it = 0; it < 30; ++it) {
+        double nrm[2];
+        if ((e = launch_resid_t(A.jap.p, A.jai.p, c->d_ax.p, rhs_dev, lam_dev, c->d_r.p, (int)n, s)) ||
+            (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_r.p, A.ltp.p, A.ltc.p, A.n_lt,
+                                c->d_t.p, c->d_del.p, s)) ||
+            (e = launch_refine_update(lam_dev, c->d_del.p, (int)n, c->d_nrm.p, s)) ||
+            (e = hipMemcpy(nrm, c->d_nrm.p, sizeof nrm, hipMemcpyDeviceToHost)))
+            return chip(c, e, "adjoint refinement");
+        c->adj_refine = std::max(c->adj_refine, it + 1);
+        // converged, or stalled at the round-off of the residual (also stops on NaN)
+        if (!(nrm[0] > 1e-15 * nrm[1]) || (nrm[0] <= 1e-10 * nrm[1] && nrm[0] >= 0.5 * prev)) break;
+        prev = nrm[0];
+    }
+    return RS_OK;
+}
+
+// Newton on the exact residual from a converged reference Newton point:
+// x <- x + dx with J(x) dx = b(x) - A(x) x (J the exact Jacobian, factor of
+// the mode's adjoint solver, one refinement step), until
+// max|dx| <= 1e-15 max(1, max|x|) or the steps stall at round-off. The reference Newton solves A x = b and
+// stops within its solver's factorization error; the adjoint differentiates
+// the exact root.
+int32_t polish(rs_circuit* c, std::vector<double>& x, int mode, double t, double dt, double gmin,
+               double scale) {
+    rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
+    hipError_t e;
+    int32_t rc;
+    double prev = 1.7976931348623157e308;
+    for (int it = 0; it < 16; ++it) {
+        const i64 xl = (i64)x.size();
+        if ((e = c->d_x.ensure(std::max(c->d_x.n, x.size()))) ||
+            (e = hipMemcpy(c->d_x.p, x.data(), xl * 8, hipMemcpyHostToDevice)))
+            return chip(c, e, "polish x");
+        DevPattern* dp;
+        if ((rc = stamp_dev(c, mode, c->d_x.p, xl, t, dt, gmin, scale, &dp, nullptr))) return rc;
+        const i64 n = dp->p.n;
+        if (!A.have_rows) {  // row map of the pattern, columns ascending within a row
+            std::vector<int32_t> rp(n + 1, 0), rcol(std::max<size_t>(dp->p.ai.size(), 1)), rq(rcol.size());
+            for (size_t q = 0; q < dp->p.ai.size(); q++)
+                if (dp->p.ai[q] < n) rp[dp->p.ai[q] + 1]++;
+            for (i64 i = 0; i < n; i++) rp[i + 1] += rp[i];
+            std::vector<int32_t> f(rp.begin(), rp.end() - 1);
+            for (i64 j = 0; j < n; j++)
+                for (i64 q = dp->p.ap[j]; q < dp->p.ap[j + 1]; q++)
+                    if (dp->p.ai[q] < n) {
+                        const int32_t at = f[dp->p.ai[q]]++;
+                        rcol[at] = (int32_t)j;
+                        rq[at] = (int32_t)q;
+                    }
+            if ((e = A.rp.ensure(n + 1)) || (e = A.rc.ensure(rcol.size())) || (e = A.rq.ensure(rq.size())) ||
+                (e = hipMemcpy(A.rp.p, rp.data(), (n + 1) * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(A.rc.p, rcol.data(), rcol.size() * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(A.rq.p, rq.data(), rq.size() * 4, hipMemcpyHostToDevice)))
+                return chip(c, e, "polish rows");
+            A.have_rows = true;
+        }
+        if ((e = c->d_r.ensure(n)) || (e = c->d_del.ensure(n)) || (e = c->d_t.ensure(n)) ||
+            (e = c->d_e.ensure(n)) || (e = c->d_nrm.ensure(2)) ||
+            (e = launch_resid_n(A.rp.p, A.rc.p, A.rq.p, c->d_ax.p, c->d_rhs.p, c->d_x.p, (int)xl, c->d_r.p,
+                                (int)n, nullptr)))
+            return chip(c, e, "polish residual");
+        if ((rc = adj_factor(c, mode, c->d_x.p, xl, t, dt, gmin, false))) return rc;
+        rs_solver* a = A.s;
+        double nrm[2];
+        if ((rc = solver_solve_dev(a, c->d_r.p, c->d_del.p, nullptr))) return from_solver(c, a, rc);
+        if ((e = launch_resid_n(A.rp.p, A.rc.p, A.rq.p, c->d_ax.p, c->d_r.p, c->d_del.p, (int)n, c->d_e.p,
+                                (int)n, nullptr)))
+            return chip(c, e, "polish refinement");
+        if ((rc = solver_solve_dev(a, c->d_e.p, c->d_t.p, nullptr))) return from_solver(c, a, rc);
+        if ((e = launch_refine_update(c->d_del.p, c->d_t.p, (int)n, c->d_nrm.p, nullptr)))
+            return chip(c, e, "polish update");
+        std::vector<double> dx(n);
+        if ((e = hipMemcpy(dx.data(), c->d_del.p, n * 8, hipMemcpyDeviceToHost))) return chip(c, e, "polish dx");
+        (void)nrm;
+        x.resize(n, 0.0);
+        double mdx = 0.0, mx = 1.0;
+        for (i64 i = 0; i < n; i++) {
+            x[i] += dx[i];
+            mdx = std::max(mdx, std::fabs(dx[i]));
+            mx = std::max(mx, std::fabs(x[i]));
+        }
+        c->polish_iters++;
+        // converged, or stalled at round-off (also stops on NaN)
+        if (!(mdx > 1e-15 * mx) || (mdx <= 1e-13 * mx && mdx >= prev)) break;
+        prev = mdx;
+    }
+    return RS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -495,7 +678,8 @@ rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_c
 
 void rs_circuit_destroy(rs_circuit* c) {
     if (!c) return;
-    if (c->adj) rs_solver_destroy(c->adj);
+    for (auto& a : c->adj)
+        if (a.s) rs_solver_destroy(a.s);
     for (auto*& b : c->dcb)
         if (b) rs_batch_destroy(b);
     for (auto& p : c->pat) {
@@ -633,77 +817,18 @@ int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t pro
     hipError_t e;
     if ((e = hipMemcpy(c->d_x.p, x, xl * sizeof(double), hipMemcpyHostToDevice)))
         return chip(c, e, "hipMemcpy(x)");
-    // J(x*): the exact derivative of the stamped currents (MOSFETs differ from the Newton matrix)
-    if ((rc = stamp_dev(c, MODE_DC, c->d_x.p, xl, 0.0, 0.0, gmin, 1.0, nullptr, s, kStampJacobian)))
+    c->adj_refine = 0;
+    if ((rc = adj_factor(c, MODE_DC, c->d_x.p, xl, 0.0, 0.0, gmin, true)))
         return rc;
-    if (!c->adj) c->adj = rs_solver_create(RS_SOLVER_SPARSELU, n);
-    rs_prepare(c->adj, n);
-    if ((rc = solver_factor_dev(c->adj, dp->p.ap.data(), n + 1, dp->p.ai.data(),
-                                (i64)dp->p.ai.size(), c->d_ax.p, s)))
-        return from_solver(c, c->adj, rc);
-    rs_solver* a = c->adj;
-    if (a->pd.nblk != 1) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint needs a single-block factor");
-    if (c->adj_factors != a->factor_count || !c->d_ltp.p) {  // L^T levels of this factorization
-        const PlanHost& ph = a->ph;
-        std::vector<int32_t> lev(n, 0), ptr, cols;
-        int32_t nlev = 0;
-        for (i64 k = n - 1; k >= 0; k--) {
-            int32_t lv = 0;
-            for (int32_t q = ph.lcp[k]; q < ph.lcp[k + 1]; q++) lv = std::max(lv, lev[ph.lri[q]] + 1);
-            lev[k] = lv;
-            nlev = std::max(nlev, lv + 1);
-        }
-        ptr.assign(nlev + 1, 0);
-        for (i64 k = 0; k < n; k++) ptr[lev[k] + 1]++;
-        for (int32_t l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
-        cols.resize(n);
-        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
-        for (i64 k = 0; k < n; k++) cols[fill[lev[k]]++] = (int32_t)k;
-        std::vector<int32_t> aux(nd);
-        for (i64 k = 0; k < nd; k++) aux[k] = dp->p.aux_of[k];
-        if ((e = c->d_ltp.ensure(ptr.size())) || (e = c->d_ltc.ensure(std::max<size_t>(cols.size(), 1))) ||
-            (e = c->d_aux.ensure(std::max<i64>(nd, 1))) ||
-            (e = hipMemcpy(c->d_ltp.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice)) ||
-            (!cols.empty() && (e = hipMemcpy(c->d_ltc.p, cols.data(), cols.size() * 4, hipMemcpyHostToDevice))) ||
-            (nd && (e = hipMemcpy(c->d_aux.p, aux.data(), nd * 4, hipMemcpyHostToDevice))))
-            return chip(c, e, "adjoint levels");
-        std::vector<int32_t> jap(n + 1), jai(std::max<size_t>(dp->p.ai.size(), 1));
-        for (i64 i = 0; i <= n; i++) jap[i] = (int32_t)dp->p.ap[i];
-        for (size_t q = 0; q < dp->p.ai.size(); q++) jai[q] = (int32_t)dp->p.ai[q];
-        if ((e = c->d_jap.ensure(n + 1)) || (e = c->d_jai.ensure(jai.size())) ||
-            (e = hipMemcpy(c->d_jap.p, jap.data(), jap.size() * 4, hipMemcpyHostToDevice)) ||
-            (e = hipMemcpy(c->d_jai.p, jai.data(), jai.size() * 4, hipMemcpyHostToDevice)))
-            return chip(c, e, "adjoint pattern");
-        c->n_lt = nlev;
-        c->adj_factors = a->factor_count;
-    }
     std::vector<double> ev(n, 0.0);
     ev[probe] = 1.0;
-    if ((e = c->d_e.ensure(n)) || (e = c->d_t.ensure(n)) || (e = c->d_lam.ensure(n)) ||
-        (e = c->d_r.ensure(n)) || (e = c->d_del.ensure(n)) || (e = c->d_nrm.ensure(2)) ||
-        (e = c->d_dp.ensure(std::max<i64>(nd, 1))) ||
-        (e = hipMemcpy(c->d_e.p, ev.data(), n * sizeof(double), hipMemcpyHostToDevice)) ||
-        (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_e.p, c->d_ltp.p, c->d_ltc.p,
-                            c->n_lt, c->d_t.p, c->d_lam.p, s)))
-        return chip(c, e, "adjoint solve");
-    // Iterative refinement against the exact J^T: SparseLU's 1e-14 clamp of a
-    // structurally zero MNA pivot factors J plus a rank-one term, which the
-    // transposed solve alone does not undo.
-    c->adj_refine = 0;
-    for (int it = 0; it < 12; ++it) {
-        double nrm[2];
-        if ((e = launch_resid_t(c->d_jap.p, c->d_jai.p, c->d_ax.p, c->d_e.p, c->d_lam.p, c->d_r.p,
-                                (int)n, s)) ||
-            (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_r.p, c->d_ltp.p, c->d_ltc.p,
-                                c->n_lt, c->d_t.p, c->d_del.p, s)) ||
-            (e = launch_refine_update(c->d_lam.p, c->d_del.p, (int)n, c->d_nrm.p, s)) ||
-            (e = hipMemcpy(nrm, c->d_nrm.p, sizeof nrm, hipMemcpyDeviceToHost)))
-            return chip(c, e, "adjoint refinement");
-        c->adj_refine = it + 1;
-        if (!(nrm[0] > 1e-15 * nrm[1])) break;  // also stops on NaN
-    }
-    if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, c->d_aux.p, c->d_x.p, (int)xl, c->d_lam.p,
-                                     gmin, c->d_dp.p, s)) ||
+    if ((e = c->d_e.ensure(n)) || (e = c->d_lam.ensure(n)) || (e = c->d_dp.ensure(std::max<i64>(nd, 1))) ||
+        (e = hipMemcpy(c->d_e.p, ev.data(), n * sizeof(double), hipMemcpyHostToDevice)))
+        return chip(c, e, "adjoint rhs");
+    if ((rc = adj_solve(c, MODE_DC, c->d_e.p, c->d_lam.p)))
+        return rc;
+    if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, c->adj[0].aux.p, c->d_x.p, (int)xl, c->d_lam.p,
+                                     gmin, 1.0, MODE_DC, nullptr, nullptr, 0.0, 0, c->d_dp.p, s)) ||
         (nd && (e = hipMemcpy(dp_out, c->d_dp.p, nd * sizeof(double), hipMemcpyDeviceToHost))) ||
         (lambda_out && (e = hipMemcpy(lambda_out, c->d_lam.p, n * sizeof(double), hipMemcpyDeviceToHost))))
         return chip(c, e, "adjoint");
@@ -979,6 +1104,176 @@ int32_t rs_tran_fixed(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, do
     if (x_len) *x_len = (int64_t)x.size();
     if (newton_iters) *newton_iters = it;
     if (status_out) *status_out = status;
+    return RS_OK;
+}
+
+int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double dt,
+                        int64_t nsteps, int64_t probe, int64_t every, double* dp_out,
+                        double* x_final, int64_t x_cap, int64_t* x_len, rs_tadj_info* info) {
+    std::memset(info, 0, sizeof(*info));
+    if (c->nbsim) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: the adjoint has no BSIM derivatives");
+    if (probe < 0 || probe >= c->node_count)
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: probe must be a node");
+    if (nsteps < 0 || !(dt > 0.0)) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: bad dt or step count");
+    if (every < 1) every = 1;
+    const i64 nd = (i64)c->dev.size(), nc = c->node_count;
+    hipError_t e;
+    int32_t rc;
+    // a checkpoint: x_k, x_{k-1} and the capacitor history after step k
+    struct Ckpt {
+        int64_t k;
+        std::vector<double> x, xm1, cv, ci;
+        std::vector<uint8_t> hv, hi;
+    };
+    std::vector<Ckpt> ck;
+    auto save = [&](int64_t k, const std::vector<double>& x, const std::vector<double>& xm1) -> int32_t {
+        Ckpt p{k, x, xm1, std::vector<double>(nd), std::vector<double>(nd), std::vector<uint8_t>(nd),
+               std::vector<uint8_t>(nd)};
+        if (nd && ((e = hipMemcpy(p.cv.data(), c->d_cap_v.p, nd * 8, hipMemcpyDeviceToHost)) ||
+                   (e = hipMemcpy(p.ci.data(), c->d_cap_i.p, nd * 8, hipMemcpyDeviceToHost)) ||
+                   (e = hipMemcpy(p.hv.data(), c->d_has_v.p, nd, hipMemcpyDeviceToHost)) ||
+                   (e = hipMemcpy(p.hi.data(), c->d_has_i.p, nd, hipMemcpyDeviceToHost))))
+            return chip(c, e, "checkpoint");
+        ck.push_back(std::move(p));
+        info->checkpoints++;
+        return RS_OK;
+    };
+    auto restore = [&](const Ckpt& p) -> int32_t {
+        if (nd && ((e = hipMemcpy(c->d_cap_v.p, p.cv.data(), nd * 8, hipMemcpyHostToDevice)) ||
+                   (e = hipMemcpy(c->d_cap_i.p, p.ci.data(), nd * 8, hipMemcpyHostToDevice)) ||
+                   (e = hipMemcpy(c->d_has_v.p, p.hv.data(), nd, hipMemcpyHostToDevice)) ||
+                   (e = hipMemcpy(c->d_has_i.p, p.hi.data(), nd, hipMemcpyHostToDevice))))
+            return chip(c, e, "restore checkpoint");
+        return RS_OK;
+    };
+    // ---- forward: rs_tran_fixed, keeping a checkpoint every `every` steps
+    std::vector<double> tk(nsteps + 1, 0.0), gm(nsteps + 1, o->gmin), sc(nsteps + 1, 1.0);
+    std::vector<double> x(nc, 0.0), xm1;
+    if ((rc = reset_cap_state(c))) return rc;
+    rs_prepare(sol, nc);
+    rs_newton_result r{};
+    int32_t it = 0;
+    c->polish_iters = 0;
+    if ((rc = run_newton_stepping(c, sol, o, x, MODE_DC, 0.0, 0.0, &r, &it, &gm[0], &sc[0]))) return rc;
+    if (!r.converged) {
+        info->status = 1;
+        info->forward_iters = it;
+        return RS_OK;
+    }
+    if ((rc = polish(c, x, MODE_DC, 0.0, 0.0, gm[0], sc[0]))) return rc;
+    if ((rc = cap_update(c, x, 0.0, 0, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p))) return rc;
+    if ((rc = save(0, x, {}))) return rc;
+    double t = 0.0;
+    for (int64_t k = 1; k <= nsteps; k++) {
+        const double tt = t + dt;
+        xm1 = x;
+        if ((rc = run_newton_stepping(c, sol, o, x, MODE_TRAN_TRAP, tt, dt, &r, &it, &gm[k], &sc[k])))
+            return rc;
+        if (!r.converged) {
+            info->status = 2;
+            info->forward_iters = it;
+            return RS_OK;
+        }
+        if ((rc = polish(c, x, MODE_TRAN_TRAP, tt, dt, gm[k], sc[k]))) return rc;
+        if ((rc = cap_update(c, x, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)))
+            return rc;
+        t = tt;
+        tk[k] = t;
+        if (k % every == 0 && k < nsteps && (rc = save(k, x, xm1))) return rc;
+    }
+    info->forward_iters = it;
+    if (x_final) std::memcpy(x_final, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
+    if (x_len) *x_len = (int64_t)x.size();
+    // ---- backward, segment by segment from the last checkpoint
+    // capacitors of each node, sign in bit 31 for the second terminal
+    std::vector<int32_t> cptr(nc + 1, 0), cdev;
+    for (i64 d = 0; d < nd; d++)
+        if (c->dev[d].kind == RS_DEV_C)
+            for (int tn = 0; tn < 2; tn++)
+                if (c->dev[d].node[tn] < nc) cptr[c->dev[d].node[tn] + 1]++;
+    for (i64 i = 0; i < nc; i++) cptr[i + 1] += cptr[i];
+    cdev.resize(std::max<int32_t>(cptr[nc], 1));
+    {
+        std::vector<int32_t> f(cptr.begin(), cptr.end() - 1);
+        for (i64 d = 0; d < nd; d++)
+            if (c->dev[d].kind == RS_DEV_C)
+                for (int tn = 0; tn < 2; tn++) {
+                    const int64_t nn = c->dev[d].node[tn];
+                    if (nn < nc) cdev[f[nn]++] = (int32_t)d | (tn == 1 ? (int32_t)0x80000000u : 0);
+                }
+    }
+    DevBuf<int32_t> d_cptr, d_cdev;
+    DevBuf<double> lamb[3], xk, xk1, xk2, dpa;
+    i64 nmax = (i64)x.size() + 8;
+    DevPattern* pdc;
+    if ((rc = get_pattern(c, MODE_DC, pdc))) return rc;
+    nmax = std::max<i64>(nmax, pdc->p.n);
+    if ((e = d_cptr.ensure(nc + 1)) || (e = d_cdev.ensure(cdev.size())) ||
+        (e = hipMemcpy(d_cptr.p, cptr.data(), (nc + 1) * 4, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(d_cdev.p, cdev.data(), cdev.size() * 4, hipMemcpyHostToDevice)) ||
+        (e = lamb[0].ensure(nmax)) || (e = lamb[1].ensure(nmax)) || (e = lamb[2].ensure(nmax)) ||
+        (e = xk.ensure(nmax)) || (e = xk1.ensure(nmax)) || (e = xk2.ensure(nmax)) ||
+        (e = dpa.ensure(std::max<i64>(nd, 1))) || (e = c->d_e.ensure(nmax)) ||
+        (e = hipMemset(dpa.p, 0, std::max<i64>(nd, 1) * 8)))
+        return chip(c, e, "adjoint buffers");
+    c->adj_refine = 0;
+    const double* l1 = nullptr;  // lambda_{k+1}
+    const double* l2 = nullptr;  // lambda_{k+2}
+    int cur = 0;
+    bool fresh[2] = {true, true};
+    auto up = [&](DevBuf<double>& b, const std::vector<double>& v) -> hipError_t {
+        return v.empty() ? hipSuccess : hipMemcpy(b.p, v.data(), v.size() * 8, hipMemcpyHostToDevice);
+    };
+    for (int64_t ci = (int64_t)ck.size() - 1; ci >= 0; ci--) {
+        const Ckpt& p = ck[ci];
+        const int64_t k0 = p.k, k1 = ci + 1 < (int64_t)ck.size() ? ck[ci + 1].k : nsteps;
+        std::vector<std::vector<double>> xs(k1 - k0 + 1);
+        xs[0] = p.x;
+        if ((rc = restore(p))) return rc;
+        std::vector<double> xr = p.x;
+        int32_t rit = 0;
+        for (int64_t k = k0 + 1; k <= k1; k++) {  // recompute the segment
+            if ((rc = run_newton_stepping(c, sol, o, xr, MODE_TRAN_TRAP, tk[k], dt, &r, &rit))) return rc;
+            if (!r.converged) return cfail(c, RS_ERR_SOLVE_FAILED, "recomputed step did not converge");
+            if ((rc = polish(c, xr, MODE_TRAN_TRAP, tk[k], dt, gm[k], sc[k]))) return rc;
+            if ((rc = cap_update(c, xr, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)))
+                return rc;
+            xs[k - k0] = xr;
+            info->recomputed_steps++;
+        }
+        info->recompute_iters += rit;
+        for (int64_t k = k1; k >= (k0 == 0 ? 0 : k0 + 1); k--) {
+            const int mode = k == 0 ? MODE_DC : MODE_TRAN_TRAP;
+            const std::vector<double>& xv = xs[k - k0];
+            const std::vector<double>* xv1 = k >= 1 ? (k - 1 >= k0 ? &xs[k - 1 - k0] : &p.xm1) : nullptr;
+            const std::vector<double>* xv2 =
+                k >= 2 ? (k - 2 >= k0 ? &xs[k - 2 - k0] : (k - 2 == k0 - 1 ? &p.xm1 : nullptr)) : nullptr;
+            if (k >= 2 && !xv2) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint checkpoint lacks x_{k-2}");
+            if ((e = up(xk, xv)) || (xv1 && (e = up(xk1, *xv1))) || (xv2 && (e = up(xk2, *xv2))))
+                return chip(c, e, "adjoint x");
+            if ((rc = adj_factor(c, mode, xk.p, (i64)xv.size(), tk[k], dt, gm[k], fresh[mode == MODE_DC ? 0 : 1])))
+                return rc;
+            fresh[mode == MODE_DC ? 0 : 1] = false;
+            rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
+            const i64 n = A.s->n;
+            if ((e = launch_tadj_rhs(c->d_dev.p, d_cptr.p, d_cdev.p, (int)nc, l1, l2, dt, k >= 1 ? 1 : 0,
+                                     k == nsteps ? (int)probe : -1, (int)n, c->d_e.p, nullptr)))
+                return chip(c, e, "adjoint rhs");
+            double* lam = lamb[cur].p;
+            if ((rc = adj_solve(c, mode, c->d_e.p, lam))) return rc;
+            if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, A.aux.p, xk.p, (int)xv.size(), lam, gm[k],
+                                             sc[k], mode, xv1 ? xk1.p : nullptr, xv2 ? xk2.p : nullptr, dt, 1,
+                                             dpa.p, nullptr)))
+                return chip(c, e, "adjoint contraction");
+            info->adjoint_solves++;
+            l2 = l1;
+            l1 = lam;
+            cur = (cur + 1) % 3;
+        }
+    }
+    if (nd && (e = hipMemcpy(dp_out, dpa.p, nd * 8, hipMemcpyDeviceToHost))) return chip(c, e, "adjoint dp");
+    info->max_refine = c->adj_refine;
+    info->polish_iters = (int32_t)c->polish_iters;
     return RS_OK;
 }
 

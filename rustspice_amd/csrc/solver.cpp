@@ -104,7 +104,7 @@ int32_t do_analyze(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail)
         f.noffdiag = od;
     } else {
         f.sym.resize(1);
-        block_symbolic(n, ap, ai, natural, f.sym[0]);
+        block_symbolic(n, ap, ai, natural, f.sym[0], h->ordering == 4);
     }
     for (auto& s : f.sym) {
         f.nnz_l += (i64)s.lri.size();

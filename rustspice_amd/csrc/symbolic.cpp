@@ -300,7 +300,8 @@ Btf btf_decompose(i64 n, const i64* ap, const i64* ai) {
 // the set reachable from its off-diagonal rows through the L columns of rows
 // < k; rows < k form U (ascending, then the diagonal), rows > k form L.
 // ---------------------------------------------------------------------------
-void block_symbolic(i64 n, const i64* ap, const i64* ai, bool natural, BlockSymbolic& s) {
+void block_symbolic(i64 n, const i64* ap, const i64* ai, bool natural, BlockSymbolic& s,
+                    bool zero_diag_last) {
     s.n = n;
     s.lcp.assign(n + 1, 0);
     s.ucp.assign(n + 1, 0);
@@ -315,6 +316,19 @@ void block_symbolic(i64 n, const i64* ap, const i64* ai, bool natural, BlockSymb
         AmdResult a = amd_order(n, ap, ai);
         s.col_perm = std::move(a.perm);
         s.col_perm_inv = std::move(a.inv_perm);
+    }
+    if (zero_diag_last) {  // unknowns without a diagonal entry after the others, order kept
+        std::vector<char> has(n, 0);
+        for (i64 c = 0; c < n; c++)
+            for (i64 q = ap[c]; q < ap[c + 1]; q++)
+                if (ai[q] == c) has[c] = 1;
+        std::vector<i64> inv;
+        inv.reserve(n);
+        for (int pass = 1; pass >= 0; pass--)
+            for (i64 k = 0; k < n; k++)
+                if (has[s.col_perm_inv[k]] == pass) inv.push_back(s.col_perm_inv[k]);
+        s.col_perm_inv = inv;
+        for (i64 k = 0; k < n; k++) s.col_perm[inv[k]] = k;
     }
     std::vector<i64> mark(n, -1), stack, pat, ur, lr;
     for (i64 k = 0; k < n; k++) {

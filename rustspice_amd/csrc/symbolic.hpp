@@ -34,7 +34,8 @@ struct BlockSymbolic {
     std::vector<i64> lcp, lri;  // L: strictly lower, rows ascending
     std::vector<i64> ucp, uri;  // U: rows ascending, diagonal last
 };
-void block_symbolic(i64 n, const i64* ap, const i64* ai, bool natural, BlockSymbolic& out);
+void block_symbolic(i64 n, const i64* ap, const i64* ai, bool natural, BlockSymbolic& s,
+                    bool zero_diag_last = false);
 
 struct BlockNumeric {
     std::vector<double> lx, ux, udiag;

@@ -35,6 +35,12 @@ def default_opts(**kw) -> NewtonOpts:
     return o
 
 
+class TranAdjointInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("checkpoints", C.c_int32), ("max_refine", C.c_int32),
+                ("polish_iters", C.c_int32), ("forward_iters", C.c_int64), ("recompute_iters", C.c_int64),
+                ("recomputed_steps", C.c_int64), ("adjoint_solves", C.c_int64)]
+
+
 class NewtonResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("reason", C.c_int32),
                 ("pad", C.c_int32), ("final_norm", C.c_double)]
@@ -91,6 +97,10 @@ def _bind():
     L.rs_dc_sweep.restype = C.c_int32
     L.rs_dc_sweep.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_int64, C.c_double, C.c_double,
                               C.c_double, C.c_int64, vp, vp, C.c_int64, C.POINTER(SweepResult)]
+    L.rs_tran_adjoint.restype = C.c_int32
+    L.rs_tran_adjoint.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, C.c_int64,
+                                  C.c_int64, vp, vp, C.c_int64, C.POINTER(C.c_int64),
+                                  C.POINTER(TranAdjointInfo)]
     L.rs_dc_adjoint.restype = C.c_int32
     L.rs_dc_adjoint.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_double, vp, vp]
     L.rs_batch_create.restype = vp
@@ -245,6 +255,21 @@ class GpuCircuit:
         return dict(status=res.status, iterations=res.iterations, factor_calls=res.factor_calls,
                     values=values[: res.npoints].copy(), sols=sols[: res.nsolved].copy(),
                     fail_value=res.fail_value)
+
+    def tran_adjoint(self, solver: GpuSolver, dt, nsteps, probe: int, every: int = 25, opts=None):
+        """rs_tran_adjoint: d x_N[probe] / dp for every device after the
+        fixed-step TRAP transient of tran_fixed, by the discrete adjoint with a
+        checkpoint every `every` steps. Returns (dp, x_N, info dict)."""
+        opts = opts or default_opts()
+        cap = self.circuit.node_count + self.circuit.ndev + 8
+        x = np.zeros(cap)
+        dp = np.zeros(max(self.circuit.ndev, 1))
+        xl = C.c_int64()
+        info = TranAdjointInfo()
+        self._check(self._L.rs_tran_adjoint(self.h, solver.h, C.byref(opts), dt, nsteps, probe, every,
+                                            _ptr(dp), _ptr(x), cap, C.byref(xl), C.byref(info)))
+        out = {k: getattr(info, k) for k, _ in TranAdjointInfo._fields_}
+        return dp[: self.circuit.ndev], x[: xl.value].copy(), out
 
     def tran_fixed(self, solver: GpuSolver, dt, nsteps, opts=None):
         opts = opts or default_opts()

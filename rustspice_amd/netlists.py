@@ -348,6 +348,63 @@ def monte_carlo_chains(count: int = 4096, seed: int = 0x52530000 + 4, lo: int = 
     return out
 
 
+def rc_mesh_inverters(nx: int = 256, ny: int = 256, taps: int | None = None,
+                      seed: int = 0x52530000 + 5, load: float | None = None) -> Circuit:
+    """RC mesh driven at seeded taps by level-1 inverters with a shared PULSE
+    input (the adjoint workload; 256x256 with 2,048 taps at full size).
+
+    Nodes: 0 ground, 1 vdd, 2 in, mesh node (i, j) = 3 + i*nx + j.
+    Devices: Vdd (1 V DC); Vin PULSE(0 1 0 10p 10p 990p 2n); 1 Gohm from in
+    to ground, which defines the input in the DC OP, where the PULSE source
+    is not stamped (stamp.rs DC rule); horizontal then vertical mesh
+    resistors R = 1 + 9u ohm; a capacitor C = (1 + 9u) fF from every mesh node
+    to ground (row-major); then per tap, in draw order, an NMOS (d=tap, g=in,
+    s=0, b=0) and a PMOS (d=tap, g=in, s=vdd, b=vdd) with the inverter-chain
+    sizes. Taps (default nx*ny/32) are distinct mesh nodes drawn by a partial
+    Fisher-Yates shuffle with the LCG. load: a resistor from every tap to
+    ground after the inverters (none by default).
+    """
+    rng = Lcg(seed)
+    taps = max(1, nx * ny // 32) if taps is None else taps
+    b = _Builder()
+    nid = lambda i, j: 3 + i * nx + j  # noqa: E731
+    b.add(RS_DEV_V, (1, 0), 1.0)
+    b.add(RS_DEV_V, (2, 0), 0.0,
+          pulse_params(0.0, 1.0, 0.0, sfx(10, P), sfx(10, P), sfx(990, P), sfx(2, N)),
+          flags=RS_F_WAVEFORM)
+    b.add(RS_DEV_R, (2, 0), sfx(1, 1e9))
+    for i in range(ny):
+        for j in range(nx - 1):
+            b.add(RS_DEV_R, (nid(i, j), nid(i, j + 1)), 1.0 + 9.0 * rng.uniform())
+    for i in range(ny - 1):
+        for j in range(nx):
+            b.add(RS_DEV_R, (nid(i, j), nid(i + 1, j)), 1.0 + 9.0 * rng.uniform())
+    for i in range(ny):
+        for j in range(nx):
+            b.add(RS_DEV_C, (nid(i, j), 0), (1.0 + 9.0 * rng.uniform()) * F)
+    pool = list(range(nx * ny))
+    u0n, u0p = u0_from_kp(NMOS_KP), u0_from_kp(PMOS_KP)
+    wn, wp, l = sfx(1, U), sfx(2, U), sfx(0.1, U)
+    for t in range(taps):
+        k = t + int(rng.uniform() * (len(pool) - t))
+        pool[t], pool[k] = pool[k], pool[t]
+        node = 3 + pool[t]
+        b.add(RS_DEV_M, (node, 2, 0, 0), 0.0, mos_params(0.4, u0n, wn, l))
+        b.add(RS_DEV_M, (node, 2, 1, 1), 0.0, mos_params(-0.4, u0p, wp, l), flags=RS_F_PMOS)
+    if load is not None:
+        for t in range(taps):
+            b.add(RS_DEV_R, (3 + pool[t], 0), load)
+    return b.build(3 + nx * ny, f"rcmesh{nx}x{ny}t{taps}")
+
+
+#: 500 TRAP steps of 2 ps, checkpoint every 25, probe at the mesh centre
+C5_TRAN = dict(dt=sfx(2, P), nsteps=500, every=25)
+
+
+def rc_mesh_probe(nx: int, ny: int) -> int:
+    return 3 + (ny // 2) * nx + nx // 2
+
+
 def rc_ladder(n: int = 20, r: float = 1e3, c: float = 1e-12, vin: float = 1.0) -> Circuit:
     """Small RC ladder (V source, n series R, C from each tap to ground)."""
     b = _Builder()

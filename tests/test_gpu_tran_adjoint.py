@@ -1,0 +1,119 @@
+"""Transient sensitivities by the discrete adjoint (rs_tran_adjoint; no
+reference counterpart) against the dense FP64 CPU adjoint and central
+finite differences of the exact-root forward (tests/tran_adjoint_ref.py,
+itself pinned by its own finite differences on the CPU).
+
+The workload is the RC mesh driven by level-1 inverters at seeded taps; the
+checked instances give each tap a load to ground, so every MOSFET stays off
+the level-1 kink at vds = 0 where a central-difference Jacobian is not the
+model's derivative.
+"""
+import numpy as np
+import pytest
+
+from rustspice_amd import engine as E
+from rustspice_amd import netlists as NL
+from rustspice_amd import solver as S
+from tests import tran_adjoint_ref as R
+
+pytestmark = pytest.mark.gpu
+
+DT = 2e-12
+
+
+def _gpu(c, nsteps, probe, every):
+    g = E.GpuCircuit(c)
+    s = S.GpuSolver(c.node_count)
+    dp, x, info = g.tran_adjoint(s, DT, nsteps, probe, every)
+    assert info["status"] == 0, info
+    return dp, x, info
+
+
+def test_forward_is_the_exact_root_trajectory(gpu):
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    probe = NL.rc_mesh_probe(4, 4)
+    _, x, info = _gpu(c, 12, probe, 5)
+    xs, _ = R.forward(c, DT, 12)
+    assert np.abs(x[: c.node_count] - xs[-1][: c.node_count]).max() <= 1e-12
+    assert info["checkpoints"] == 3 and info["recomputed_steps"] == 12
+    assert info["adjoint_solves"] == 13 and info["polish_iters"] > 0
+
+
+def test_matches_dense_cpu_adjoint(gpu):
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    probe = NL.rc_mesh_probe(4, 4)
+    N = 12
+    dp, _, _ = _gpu(c, N, probe, 5)
+    params = [(k, *R.param(c, k)) for k in range(c.ndev)
+              if c.devices[k]["kind"] in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M)]
+    ref, _, _ = R.adjoint(c, DT, N, probe, params)
+    for kind in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M):
+        idx = [i for i, (k, *_) in enumerate(params) if c.devices[k]["kind"] == kind]
+        want = ref[idx]
+        got = dp[[params[i][0] for i in idx]]
+        scale = np.abs(want).max()
+        assert scale > 0.0
+        # each sensitivity within 1e-6 of itself or of the kind's largest (the
+        # attainable accuracy of lambda is eps * cond(J), about 1e-7 here)
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-6 * scale), (kind, got, want)
+
+
+def test_matches_central_differences(gpu):
+    """h = 1e-6 relative on seeded parameters whose sensitivity is resolved
+    (well above the 1e-12 / h round-off floor of the difference quotient)."""
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    probe = NL.rc_mesh_probe(4, 4)
+    N = 12
+    dp, _, _ = _gpu(c, N, probe, 4)
+    rng = np.random.RandomState(5)
+    cand = [k for k in range(c.ndev) if c.devices[k]["kind"] in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M)]
+    checked = 0
+    for k in rng.permutation(cand)[:24]:
+        p0 = R.param(c, k)[0]()
+        fd = R.finite_difference(c, DT, N, probe, int(k), h_rel=1e-6)
+        floor = 1e-12 / (1e-6 * abs(p0))
+        if abs(fd) < 1e3 * floor:
+            continue
+        assert abs(dp[k] - fd) <= 1e-6 * abs(fd) + floor, (int(k), dp[k], fd)
+        checked += 1
+    assert checked >= 6
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(gpu):
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    probe = NL.rc_mesh_probe(4, 4)
+    runs = [_gpu(c, 10, probe, every) for every in (1, 3, 1000)]
+    for dp, x, info in runs[1:]:
+        assert np.array_equal(dp, runs[0][0]) and np.array_equal(x, runs[0][1])
+    assert [r[2]["checkpoints"] for r in runs] == [10, 4, 1]
+
+
+def test_unloaded_instance_matches_differences_of_the_gpu_forward(gpu):
+    """The workload's own construction (no tap loads: the DC OP holds the
+    PMOS at vds = 0, where the model's one-sided derivative is the one the
+    solution follows): central differences (h = 1e-6 relative) of the GPU's
+    exact-root forward itself, on an 8x8 instance over 20 steps."""
+    c = NL.rc_mesh_inverters(8, 8, 4)
+    probe = NL.rc_mesh_probe(8, 8)
+    N = 20
+    dp, _, _ = _gpu(c, N, probe, 7)
+    ks = [k for k in range(c.ndev) if c.devices[k]["kind"] == NL.RS_DEV_M][:4]
+    ks += [k for k in range(c.ndev) if c.devices[k]["kind"] == NL.RS_DEV_C][30:33]
+    ks += [0]  # Vdd
+    checked = 0
+    for k in ks:
+        get, put = R.param(c, k)
+        p0 = get()
+        h = 1e-6 * abs(p0)
+        put(p0 + h)
+        xp = _gpu(c, N, probe, N)[1][probe]
+        put(p0 - h)
+        xm = _gpu(c, N, probe, N)[1][probe]
+        put(p0)
+        fd = (xp - xm) / (2.0 * h)
+        floor = 1e-12 / h
+        if abs(fd) < 1e3 * floor:
+            continue
+        assert abs(dp[k] - fd) <= 1e-6 * abs(fd) + floor, (k, dp[k], fd)
+        checked += 1
+    assert checked >= 4

@@ -1,0 +1,51 @@
+"""The dense CPU transient adjoint (tests/tran_adjoint_ref.py), which checks
+rs_tran_adjoint on the GPU, pinned on its own: its forward against the
+oracle's fixed-step transient, and its adjoint gradient against central
+differences of its forward (two independent routes to the same number)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rustspice_amd import netlists as NL
+from tests import tran_adjoint_ref as R
+
+
+def test_forward_steps_are_exact_roots():
+    """Every stored x_k zeroes its step's stamped currents to round-off; the
+    oracle's fixed-step transient (the reference Newton loop, which stops
+    within its solver's factorization error) follows the same trajectory."""
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    dt, N = 2e-12, 12
+    xs, hist = R.forward(c, dt, N)
+    t = 0.0
+    for k in range(N + 1):
+        if k:
+            t += dt
+        F, _ = R._F(c, xs[k], R.DC if k == 0 else R.TRAN, t, dt if k else 0.0, hist[k])
+        assert np.abs(F).max() <= 1e-13, k
+    rc, x, _ = O.tran_fixed(c, dt, N)
+    assert rc == 0
+    assert np.abs(xs[-1][: c.node_count] - x[: c.node_count]).max() < 1e-2
+
+
+@pytest.mark.parametrize("kind", [NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M])
+def test_adjoint_matches_its_finite_differences(kind):
+    """Central differences of the exact-root forward resolve a sensitivity
+    down to about 1e-12 / h (the roots' round-off over the step), which is
+    the floor of the comparison; at least one parameter per kind lies far
+    above it."""
+    c = NL.rc_mesh_inverters(3, 3, 2, load=1e4)
+    dt, N = 2e-12, 6
+    probe = NL.rc_mesh_probe(3, 3)
+    ks = [k for k in range(c.ndev) if c.devices[k]["kind"] == kind]
+    ks = ks[:2] + ks[-2:]
+    params = [(k, *R.param(c, k)) for k in ks]
+    dp, lam, xs = R.adjoint(c, dt, N, probe, params)
+    resolved = 0
+    for k, g in zip(ks, dp):
+        h_rel = 1e-5
+        fd = R.finite_difference(c, dt, N, probe, k, h_rel=h_rel)
+        floor = 1e-12 / (h_rel * abs(R.param(c, k)[0]()))
+        assert abs(g - fd) <= 1e-5 * max(abs(fd), abs(g)) + floor, (kind, k, g, fd, floor)
+        resolved += abs(fd) > 1e3 * floor
+    assert resolved >= 1
