@@ -205,7 +205,9 @@ def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
     adjoint workload at nx x nx with nx*nx/32 taps): dV(centre at t =
     nsteps*2 ps)/dp for every R, C and MOSFET W, checkpoint every `every`
     steps, against one forward fixed-step TRAP transient through the same
-    plain GPU backend."""
+    plain GPU backend (adjoint_over_forward) and against the adjoint's own
+    forward pass, which solves every step to its exact root
+    (adjoint_over_exact_forward)."""
     import numpy as np
 
     c = NL.rc_mesh_inverters(nx, nx)
@@ -221,11 +223,14 @@ def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
     t0 = time.perf_counter()
     dp, xa, info = g.tran_adjoint(s, dt, nsteps, probe, every)
     t_adj = time.perf_counter() - t0
+    if info["status"] != 0:
+        return dict(workload=f"{nx}x{nx} RC mesh adjoint", adjoint_info=info, forward_status=int(st))
     kinds = [d["kind"] for d in c.devices]
     return dict(workload=f"{nx}x{nx} RC mesh, {nx * nx // 32} inverter taps, {nsteps} TRAP steps of 2 ps, "
                          f"checkpoint every {every}; dV(centre)/d(every R, C, W)",
                 n=int(len(xa)), forward_status=int(st), forward_ms=1e3 * t_fwd, forward_newton_iters=int(it),
-                adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd, adjoint_info=info,
+                adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd,
+                adjoint_over_exact_forward=t_adj / max(info["forward_ms"] * 1e-3, 1e-12), adjoint_info=info,
                 sensitivities=dict(R=kinds.count(NL.RS_DEV_R), C=kinds.count(NL.RS_DEV_C),
                                    W=kinds.count(NL.RS_DEV_M)),
                 probe_v=float(xa[probe]),

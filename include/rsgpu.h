@@ -193,14 +193,22 @@ int32_t rs_dc_adjoint(rs_circuit *c, const double *x, int64_t x_len, int64_t pro
  * transposed solves with refinement). dp[ndev] receives dx_N[probe]/dp:
  * resistors d/dR, capacitors d/dC, DC V and I sources d/dvalue, MOSFETs d/dW,
  * diodes d/dIs; PULSE sources 0. probe is a node index. The adjoint
- * differentiates each step's exact root: after the reference Newton loop
- * converges (within its solver's factorization error), Newton steps on the
- * exact residual J dx = b - A x polish x before it is stored, in the forward
- * and in every recomputation alike. x_final (x_cap) gets the polished x_N.
- * info->status: 0 ok, 1 DC OP failed, 2 a step failed (no gradient). */
+ * differentiates each step's exact root, which the reference Newton loop
+ * reaches only within its solver's factorization error: the DC OP runs the
+ * reference loop on s and then Newton on the exact Jacobian (a Jacobian
+ * stamp whose right-hand side is consistent with it, factored by SparseLU
+ * with zero-diagonal unknowns last); every TRAP step, in the forward and in
+ * each recomputation alike, runs that exact-Jacobian Newton from x_{k-1}
+ * (the reference loop first if it does not converge). x_final (x_cap) gets
+ * x_N. info: polish_iters = exact-Jacobian Newton iterations, forward_iters =
+ * reference Newton iterations; status 0 ok, 1 DC OP failed, 2 a step failed
+ * (no gradient). */
 typedef struct rs_tadj_info {
     int32_t status, checkpoints, max_refine, polish_iters;
     int64_t forward_iters, recompute_iters, recomputed_steps, adjoint_solves;
+    int64_t inexact_steps; /* steps (forward and recomputation) whose exact root was out of
+                              reach: the reference loop's point is kept there */
+    double forward_ms, recompute_ms, backward_ms; /* host wall clock of the three phases */
 } rs_tadj_info;
 int32_t rs_tran_adjoint(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double dt,
                         int64_t nsteps, int64_t probe, int64_t every, double *dp,

@@ -156,7 +156,7 @@ __device__ __forceinline__ void eval_device(const rs_device& v, int d, const dou
             gds = fmax_rust(0.5 * beta_eff * (vov * vov) * lambda, GMIN);
         }
         const double ids_s = ids * sign;
-        const double ieq = ids_s - gm * (vg - vs) - gds * (vd - vs);
+        double ieq = ids_s - gm * (vg - vs) - gds * (vd - vs);
         const double gdsm = fmax_rust(gds, gmin);
         if (a.flags & rsgpu::kStampJacobian) {
             // d(ids_s)/d(vd, vg, vs) of the current above: internal-frame
@@ -168,6 +168,10 @@ __device__ __forceinline__ void eval_device(const rs_device& v, int d, const dou
             const double lk = gdsm - gds;  // the gmin floor stamped on (d, s)
             o[0] = jd + lk; o[1] = -js + lk; o[2] = js - lk; o[3] = -jd - lk;
             o[4] = jg; o[5] = 0.0; o[6] = -jg; o[7] = 0.0;
+            // right-hand side consistent with these entries: J x - rhs equals
+            // the Newton stamp's current ids_s + (gdsm - gds)(vd - vs), so a
+            // solve of J x = rhs is a Newton step on the exact Jacobian
+            ieq = ids_s - (jd * vd + jg * vg + js * vs);
         } else {
             o[0] = gdsm; o[1] = gdsm; o[2] = -gdsm; o[3] = -gdsm;
             o[4] = gm; o[5] = -gm; o[6] = -gm; o[7] = gm;

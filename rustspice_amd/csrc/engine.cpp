@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -203,7 +204,7 @@ double norm2(const double* v, i64 n) {
 // run_newton (newton.rs:65-117)
 int32_t run_newton(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::vector<double>& x,
                    int mode, double t, double dt, double gmin, double scale,
-                   rs_newton_result* r) {
+                   rs_newton_result* r, int32_t stamp_flags = 0) {
     hipStream_t s = nullptr;
     r->converged = 0;
     r->iterations = 0;
@@ -225,7 +226,7 @@ int32_t run_newton(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::
         hipError_t e = hipMemcpyAsync(c->d_x.p, x.data(), x.size() * 8, hipMemcpyHostToDevice, s);
         if (e) return chip(c, e, "hipMemcpy(x)");
         DevPattern* dp;
-        int32_t rc = stamp_dev(c, mode, c->d_x.p, (i64)x.size(), t, dt, gmin, scale, &dp, s);
+        int32_t rc = stamp_dev(c, mode, c->d_x.p, (i64)x.size(), t, dt, gmin, scale, &dp, s, stamp_flags);
         if (rc) return rc;
         const i64 n = dp->p.n;
         rs_prepare(sol, n);
@@ -534,7 +535,7 @@ int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double 
 // J^T lam = rhs with the factor of the last adj_factor of this mode (J still
 // in c->d_ax), refined against the exact J^T (a pivot below SparseLU's 1e-14
 // clamp would otherwise leave its rank-one error in lambda).
-int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_dev) {
+int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_dev, int max_refine = 30) {
     rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
     rs_solver* a = A.s;
     const i64 n = a->n;
@@ -546,14 +547,19 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
                             c->d_t.p, lam_dev, s)))
         return chip(c, e, "adjoint solve");
     double prev = 1.7976931348623157e308;
-    for (int it = 0; it < 30; ++it) {
-        double nrm[2];
+    for (int it = 0; it < max_refine; ++it) {
+        double nrm[2] = {0.0, 1.0};
+        const bool last = it + 1 == max_refine && max_refine < 30;  // a fixed count: no read-back
         if ((e = launch_resid_t(A.jap.p, A.jai.p, c->d_ax.p, rhs_dev, lam_dev, c->d_r.p, (int)n, s)) ||
             (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_r.p, A.ltp.p, A.ltc.p, A.n_lt,
                                 c->d_t.p, c->d_del.p, s)) ||
             (e = launch_refine_update(lam_dev, c->d_del.p, (int)n, c->d_nrm.p, s)) ||
-            (e = hipMemcpy(nrm, c->d_nrm.p, sizeof nrm, hipMemcpyDeviceToHost)))
+            (!last && (e = hipMemcpy(nrm, c->d_nrm.p, sizeof nrm, hipMemcpyDeviceToHost))))
             return chip(c, e, "adjoint refinement");
+        if (last) {
+            c->adj_refine = std::max(c->adj_refine, it + 1);
+            break;
+        }
         c->adj_refine = std::max(c->adj_refine, it + 1);
         // converged, or stalled at the round-off of the residual (also stops on NaN)
         if (!(nrm[0] > 1e-15 * nrm[1]) || (nrm[0] <= 1e-10 * nrm[1] && nrm[0] >= 0.5 * prev)) break;
@@ -562,26 +568,35 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
     return RS_OK;
 }
 
-// Newton on the exact residual from a converged reference Newton point:
-// x <- x + dx with J(x) dx = b(x) - A(x) x (J the exact Jacobian, factor of
-// the mode's adjoint solver, one refinement step), until
-// max|dx| <= 1e-15 max(1, max|x|) or the steps stall at round-off. The reference Newton solves A x = b and
-// stops within its solver's factorization error; the adjoint differentiates
-// the exact root.
-int32_t polish(rs_circuit* c, std::vector<double>& x, int mode, double t, double dt, double gmin,
-               double scale) {
+// Newton on the exact Jacobian in correction form: the Jacobian stamp's
+// right-hand side is consistent with its entries (J x - rhs is the stamped
+// current), so dx = J^{-1} (rhs - J x) is a Newton step of the stamped
+// currents with their exact derivative; x converges quadratically to the
+// step's exact root, and the residual form keeps the solve's round-off
+// relative to the shrinking residual. The factor is the mode's adjoint solver
+// (SparseLU, zero-diagonal unknowns last: no clamped pivot). Stops when every
+// |dx| <= 1e-12 + 1e-10 |x| (the update then leaves an error of order
+// |dx|^2), or when the steps stall at round-off below 1e-8 |x|.
+int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, double dt,
+                     double gmin, double scale, bool* ok) {
     rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
     hipError_t e;
     int32_t rc;
+    *ok = false;
     double prev = 1.7976931348623157e308;
-    for (int it = 0; it < 16; ++it) {
+    for (int it = 0; it < 50; ++it) {
         const i64 xl = (i64)x.size();
         if ((e = c->d_x.ensure(std::max(c->d_x.n, x.size()))) ||
             (e = hipMemcpy(c->d_x.p, x.data(), xl * 8, hipMemcpyHostToDevice)))
-            return chip(c, e, "polish x");
+            return chip(c, e, "exact Newton x");
         DevPattern* dp;
-        if ((rc = stamp_dev(c, mode, c->d_x.p, xl, t, dt, gmin, scale, &dp, nullptr))) return rc;
+        if ((rc = stamp_dev(c, mode, c->d_x.p, xl, t, dt, gmin, scale, &dp, nullptr, kStampJacobian)))
+            return rc;
         const i64 n = dp->p.n;
+        if (!A.s) {
+            A.s = rs_solver_create(RS_SOLVER_SPARSELU, n);
+            A.s->ordering = 4;
+        }
         if (!A.have_rows) {  // row map of the pattern, columns ascending within a row
             std::vector<int32_t> rp(n + 1, 0), rcol(std::max<size_t>(dp->p.ai.size(), 1)), rq(rcol.size());
             for (size_t q = 0; q < dp->p.ai.size(); q++)
@@ -599,38 +614,36 @@ int32_t polish(rs_circuit* c, std::vector<double>& x, int mode, double t, double
                 (e = hipMemcpy(A.rp.p, rp.data(), (n + 1) * 4, hipMemcpyHostToDevice)) ||
                 (e = hipMemcpy(A.rc.p, rcol.data(), rcol.size() * 4, hipMemcpyHostToDevice)) ||
                 (e = hipMemcpy(A.rq.p, rq.data(), rq.size() * 4, hipMemcpyHostToDevice)))
-                return chip(c, e, "polish rows");
+                return chip(c, e, "exact Newton rows");
             A.have_rows = true;
         }
-        if ((e = c->d_r.ensure(n)) || (e = c->d_del.ensure(n)) || (e = c->d_t.ensure(n)) ||
-            (e = c->d_e.ensure(n)) || (e = c->d_nrm.ensure(2)) ||
+        if ((e = c->d_r.ensure(n)) || (e = c->d_del.ensure(n)) ||
             (e = launch_resid_n(A.rp.p, A.rc.p, A.rq.p, c->d_ax.p, c->d_rhs.p, c->d_x.p, (int)xl, c->d_r.p,
                                 (int)n, nullptr)))
-            return chip(c, e, "polish residual");
-        if ((rc = adj_factor(c, mode, c->d_x.p, xl, t, dt, gmin, false))) return rc;
-        rs_solver* a = A.s;
-        double nrm[2];
-        if ((rc = solver_solve_dev(a, c->d_r.p, c->d_del.p, nullptr))) return from_solver(c, a, rc);
-        if ((e = launch_resid_n(A.rp.p, A.rc.p, A.rq.p, c->d_ax.p, c->d_r.p, c->d_del.p, (int)n, c->d_e.p,
-                                (int)n, nullptr)))
-            return chip(c, e, "polish refinement");
-        if ((rc = solver_solve_dev(a, c->d_e.p, c->d_t.p, nullptr))) return from_solver(c, a, rc);
-        if ((e = launch_refine_update(c->d_del.p, c->d_t.p, (int)n, c->d_nrm.p, nullptr)))
-            return chip(c, e, "polish update");
+            return chip(c, e, "exact Newton residual");
+        rs_prepare(A.s, n);
+        if ((rc = solver_factor_dev(A.s, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
+                                    c->d_ax.p, nullptr)))
+            return from_solver(c, A.s, rc);
+        if ((rc = solver_solve_dev(A.s, c->d_r.p, c->d_del.p, nullptr))) return from_solver(c, A.s, rc);
         std::vector<double> dx(n);
-        if ((e = hipMemcpy(dx.data(), c->d_del.p, n * 8, hipMemcpyDeviceToHost))) return chip(c, e, "polish dx");
-        (void)nrm;
+        if ((e = hipMemcpy(dx.data(), c->d_del.p, n * 8, hipMemcpyDeviceToHost))) return chip(c, e, "exact Newton dx");
         x.resize(n, 0.0);
-        double mdx = 0.0, mx = 1.0;
+        bool conv = true;
+        double rel = 0.0;
         for (i64 i = 0; i < n; i++) {
             x[i] += dx[i];
-            mdx = std::max(mdx, std::fabs(dx[i]));
-            mx = std::max(mx, std::fabs(x[i]));
+            const double ad = std::fabs(dx[i]);
+            if (!(ad <= 1e-12 + 1e-10 * std::fabs(x[i]))) conv = false;
+            rel = std::max(rel, ad / (1e-12 + std::fabs(x[i])));
         }
         c->polish_iters++;
-        // converged, or stalled at round-off (also stops on NaN)
-        if (!(mdx > 1e-15 * mx) || (mdx <= 1e-13 * mx && mdx >= prev)) break;
-        prev = mdx;
+        if (!(rel == rel)) return RS_OK;  // NaN: not converged
+        if (conv || (rel <= 1e-8 && rel >= 0.5 * prev)) {
+            *ok = true;
+            return RS_OK;
+        }
+        prev = rel;
     }
     return RS_OK;
 }
@@ -1146,6 +1159,11 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
             return chip(c, e, "restore checkpoint");
         return RS_OK;
     };
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+    };
+    const clk::time_point t_fwd = clk::now();
     // ---- forward: rs_tran_fixed, keeping a checkpoint every `every` steps
     std::vector<double> tk(nsteps + 1, 0.0), gm(nsteps + 1, o->gmin), sc(nsteps + 1, 1.0);
     std::vector<double> x(nc, 0.0), xm1;
@@ -1160,21 +1178,56 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         info->forward_iters = it;
         return RS_OK;
     }
-    if ((rc = polish(c, x, MODE_DC, 0.0, 0.0, gm[0], sc[0]))) return rc;
+    bool ok = false;
+    if ((rc = exact_newton(c, x, MODE_DC, 0.0, 0.0, gm[0], sc[0], &ok))) return rc;
+    if (!ok) {
+        info->status = 1;
+        info->forward_iters = it;
+        return RS_OK;
+    }
     if ((rc = cap_update(c, x, 0.0, 0, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p))) return rc;
     if ((rc = save(0, x, {}))) return rc;
+    // one TRAP step to its exact root: exact-Jacobian Newton from x_{k-1};
+    // should that not converge, the reference loop (with its stepping) first.
+    // The recomputation runs the same function, so it reproduces x_k exactly.
+    int64_t inexact = 0;
+    auto tran_step = [&](std::vector<double>& xv, int64_t k, double tt, int32_t* iters,
+                         bool* conv) -> int32_t {
+        std::vector<double> x0 = xv;
+        bool okk = false;
+        int32_t rcc = exact_newton(c, xv, MODE_TRAN_TRAP, tt, dt, gm[k], sc[k], &okk);
+        if (rcc) return rcc;
+        if (!okk) {
+            xv = x0;
+            rs_newton_result rr{};
+            if ((rcc = run_newton_stepping(c, sol, o, xv, MODE_TRAN_TRAP, tt, dt, &rr, iters, &gm[k], &sc[k])))
+                return rcc;
+            if (!rr.converged) {
+                *conv = false;
+                return RS_OK;
+            }
+            const std::vector<double> xref = xv;
+            if ((rcc = exact_newton(c, xv, MODE_TRAN_TRAP, tt, dt, gm[k], sc[k], &okk))) return rcc;
+            if (!okk) {  // no exact root within reach (the level-1 current is not
+                xv = xref;  // smooth at vds = 0): the reference loop's point
+                inexact++;
+            }
+        }
+        *conv = true;
+        return RS_OK;
+    };
     double t = 0.0;
     for (int64_t k = 1; k <= nsteps; k++) {
         const double tt = t + dt;
         xm1 = x;
-        if ((rc = run_newton_stepping(c, sol, o, x, MODE_TRAN_TRAP, tt, dt, &r, &it, &gm[k], &sc[k])))
-            return rc;
-        if (!r.converged) {
+        if ((rc = tran_step(x, k, tt, &it, &ok))) return rc;
+        if (!ok) {
             info->status = 2;
             info->forward_iters = it;
+            info->polish_iters = (int32_t)c->polish_iters;
+            info->inexact_steps = inexact;
             return RS_OK;
         }
-        if ((rc = polish(c, x, MODE_TRAN_TRAP, tt, dt, gm[k], sc[k]))) return rc;
         if ((rc = cap_update(c, x, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)))
             return rc;
         t = tt;
@@ -1182,6 +1235,7 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         if (k % every == 0 && k < nsteps && (rc = save(k, x, xm1))) return rc;
     }
     info->forward_iters = it;
+    info->forward_ms = ms_since(t_fwd);
     if (x_final) std::memcpy(x_final, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
     if (x_len) *x_len = (int64_t)x.size();
     // ---- backward, segment by segment from the last checkpoint
@@ -1232,16 +1286,19 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         if ((rc = restore(p))) return rc;
         std::vector<double> xr = p.x;
         int32_t rit = 0;
+        const clk::time_point t_rec = clk::now();
         for (int64_t k = k0 + 1; k <= k1; k++) {  // recompute the segment
-            if ((rc = run_newton_stepping(c, sol, o, xr, MODE_TRAN_TRAP, tk[k], dt, &r, &rit))) return rc;
-            if (!r.converged) return cfail(c, RS_ERR_SOLVE_FAILED, "recomputed step did not converge");
-            if ((rc = polish(c, xr, MODE_TRAN_TRAP, tk[k], dt, gm[k], sc[k]))) return rc;
+            bool okr = false;
+            if ((rc = tran_step(xr, k, tk[k], &rit, &okr))) return rc;
+            if (!okr) return cfail(c, RS_ERR_SOLVE_FAILED, "recomputed step did not converge");
             if ((rc = cap_update(c, xr, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)))
                 return rc;
             xs[k - k0] = xr;
             info->recomputed_steps++;
         }
         info->recompute_iters += rit;
+        info->recompute_ms += ms_since(t_rec);
+        const clk::time_point t_bwd = clk::now();
         for (int64_t k = k1; k >= (k0 == 0 ? 0 : k0 + 1); k--) {
             const int mode = k == 0 ? MODE_DC : MODE_TRAN_TRAP;
             const std::vector<double>& xv = xs[k - k0];
@@ -1260,7 +1317,8 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
                                      k == nsteps ? (int)probe : -1, (int)n, c->d_e.p, nullptr)))
                 return chip(c, e, "adjoint rhs");
             double* lam = lamb[cur].p;
-            if ((rc = adj_solve(c, mode, c->d_e.p, lam))) return rc;
+            // the adjoint factor has no clamped pivot: one refinement step
+            if ((rc = adj_solve(c, mode, c->d_e.p, lam, 1))) return rc;
             if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, A.aux.p, xk.p, (int)xv.size(), lam, gm[k],
                                              sc[k], mode, xv1 ? xk1.p : nullptr, xv2 ? xk2.p : nullptr, dt, 1,
                                              dpa.p, nullptr)))
@@ -1270,10 +1328,13 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
             l1 = lam;
             cur = (cur + 1) % 3;
         }
+        if ((e = hipDeviceSynchronize())) return chip(c, e, "adjoint segment");
+        info->backward_ms += ms_since(t_bwd);
     }
     if (nd && (e = hipMemcpy(dp_out, dpa.p, nd * 8, hipMemcpyDeviceToHost))) return chip(c, e, "adjoint dp");
     info->max_refine = c->adj_refine;
     info->polish_iters = (int32_t)c->polish_iters;
+    info->inexact_steps = inexact;
     return RS_OK;
 }
 

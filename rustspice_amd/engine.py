@@ -38,7 +38,9 @@ def default_opts(**kw) -> NewtonOpts:
 class TranAdjointInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("checkpoints", C.c_int32), ("max_refine", C.c_int32),
                 ("polish_iters", C.c_int32), ("forward_iters", C.c_int64), ("recompute_iters", C.c_int64),
-                ("recomputed_steps", C.c_int64), ("adjoint_solves", C.c_int64)]
+                ("recomputed_steps", C.c_int64), ("adjoint_solves", C.c_int64),
+                ("inexact_steps", C.c_int64), ("forward_ms", C.c_double), ("recompute_ms", C.c_double),
+                ("backward_ms", C.c_double)]
 
 
 class NewtonResult(C.Structure):

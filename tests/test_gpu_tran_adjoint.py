@@ -25,7 +25,7 @@ def _gpu(c, nsteps, probe, every):
     g = E.GpuCircuit(c)
     s = S.GpuSolver(c.node_count)
     dp, x, info = g.tran_adjoint(s, DT, nsteps, probe, every)
-    assert info["status"] == 0, info
+    assert info["status"] == 0 and info["inexact_steps"] == 0, info
     return dp, x, info
 
 
