@@ -181,6 +181,18 @@ int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz
  * lambda (n entries) is optional. gmin: the value the DC OP used. */
 int32_t rs_dc_adjoint(rs_circuit *c, const double *x, int64_t x_len, int64_t probe, double gmin,
                       double *dp, double *lambda);
+/* 1: rs_tran_fixed runs its steps after the first TRAP step as captured
+ * hipGraphs (no reference counterpart): one graph per step holds two
+ * Newton iterations (device evaluation, assembly, refactor, solve and the
+ * Newton loop's convergence test and damping on the device, each kernel a
+ * no-op once the step has converged) and the accepted-step capacitor update;
+ * the host reads the step's state once per launch. Results are bit-identical
+ * to the host-driven loop. Used when the circuit has no BSIM devices, no
+ * gmin or source stepping is asked for and the solver is NativeKLU or
+ * SparseLU; otherwise the host-driven loop runs. 0 (default): host-driven.
+ * rs_circuit_graph_steps: steps run through graphs so far. */
+void rs_circuit_set_tran_graph(rs_circuit *c, int32_t on);
+int64_t rs_circuit_graph_steps(const rs_circuit *c);
 /* Transient sensitivities by the discrete adjoint (no reference
  * counterpart). Runs rs_tran_fixed's transient (DC OP, then nsteps TRAP
  * steps of dt, the accepted-step capacitor history of stamp.rs:615-668

@@ -762,8 +762,10 @@ __global__ __launch_bounds__(1024) void k_sp_refactor(const SpSolverDev* __restr
                                                      const int32_t* __restrict__ sip,
                                                      const SpOpWord* __restrict__ words,
                                                      const double* __restrict__ ax,
-                                                     double* __restrict__ vals) {
+                                                     double* __restrict__ vals,
+                                                     const int32_t* __restrict__ skip) {
     extern __shared__ double VL[];
+    if (skip && *skip) return;
     const SpSolverDev sd = *sdp;
     const PlanDev d = *pdp;
     const int tid = threadIdx.x, bs = blockDim.x, lane = tid & (kW - 1), wave = tid / kW;
@@ -810,8 +812,10 @@ __global__ __launch_bounds__(1024) void k_sp_solve(const SpSolverDev* __restrict
                                                   const double* __restrict__ ax,
                                                   const double* __restrict__ vals,
                                                   const double* __restrict__ rhs,
-                                                  double* __restrict__ x) {
+                                                  double* __restrict__ x,
+                                                  const int32_t* __restrict__ skip) {
     extern __shared__ double VL[];
+    if (skip && *skip) return;
     const SpSolverDev sd = *sdp;
     const PlanDev d = *pdp;
     const int tid = threadIdx.x, bs = blockDim.x, lane = tid & (kW - 1), wave = tid / kW;
@@ -861,18 +865,19 @@ size_t batch_lds_bytes(int nmax, int lumax, int waves) { return lds_bytes(nmax, 
 
 hipError_t launch_sp_refactor(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
                               const int32_t* sip, const SpOpWord* words, const double* ax,
-                              double* vals, int nslots, int threads, hipStream_t s) {
+                              double* vals, int nslots, int threads, hipStream_t s,
+                              const int32_t* skip) {
     hipLaunchKernelGGL(k_sp_refactor, dim3(1), dim3(threads), (size_t)(nslots + 1) * sizeof(double),
-                       s, sd, pd, ip, sip, words, ax, vals);
+                       s, sd, pd, ip, sip, words, ax, vals, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_sp_solve(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
                            const int32_t* sip, const SpOpWord* words, const double* ax,
                            const double* vals, const double* rhs, double* x, int nslots, int threads,
-                           hipStream_t s) {
+                           hipStream_t s, const int32_t* skip) {
     hipLaunchKernelGGL(k_sp_solve, dim3(1), dim3(threads), (size_t)(nslots + 1) * sizeof(double), s,
-                       sd, pd, ip, sip, words, ax, vals, rhs, x);
+                       sd, pd, ip, sip, words, ax, vals, rhs, x, skip);
     return hipGetLastError();
 }
 

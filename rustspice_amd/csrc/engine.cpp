@@ -32,10 +32,13 @@ hipError_t launch_mos_eval(const rs_bsim_model* models, const int32_t* mi, const
                            const double* in, int64_t n, double* out, hipStream_t s);
 hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
                            const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
-                           hipStream_t s);
+                           hipStream_t s, const int32_t* skip = nullptr);
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
-                            uint8_t* has_i, hipStream_t s);
+                            uint8_t* has_i, hipStream_t s, const int32_t* gate = nullptr);
+hipError_t launch_newton_begin(double* t_dev, double dt, int32_t* st, double damping, hipStream_t s);
+hipError_t launch_newton_update(int n, double* x, const double* xn, double abs_tol, double rel_tol,
+                                double damping_min, int max_iters, int32_t* st, hipStream_t s);
 hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* vals, const double* e,
                           const int32_t* lt_ptr, const int32_t* lt_cols, int n_lt, double* T,
                           double* lam, hipStream_t s);
@@ -96,6 +99,9 @@ struct rs_circuit {
         DevBuf<int32_t> rp, rc, rq;
     } adj[2];
     int64_t polish_iters = 0;
+    // fixed-step transient with each TRAP step as a captured hipGraph
+    bool use_graph = false;
+    int64_t graph_steps = 0, graph_launches = 0;
     DevBuf<double> d_e, d_t, d_lam, d_dp, d_r, d_del, d_nrm;
     int32_t adj_refine = 0;  // refinement steps of the last adjoint solve (largest over a TRAN)
     // BSIM devices: slot bases, model table, group masks of the last evaluation
@@ -648,6 +654,95 @@ int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, 
     return RS_OK;
 }
 
+// Steps of the fixed-step TRAP transient from time t as one captured graph
+// each: kGraphIters Newton iterations of device evaluation, assembly,
+// refactor, solve and k_newton_update (the host loop's convergence test and
+// damping), every kernel returning at once after the update has set done;
+// then the accepted-step capacitor history, gated on convergence. The host
+// reads the 32-byte state once per graph launch and relaunches while a step
+// has neither converged nor spent max_iters. Results equal the host-driven
+// loop's bit for bit. *done: steps completed (-1: not eligible, nothing run).
+int32_t tran_fixed_graph(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::vector<double>& x,
+                         double t, double dt, int64_t nsteps, int64_t* done, int32_t* iters, int32_t* status) {
+    constexpr int kGraphIters = 2;
+    *done = -1;
+    DevPattern* dp;
+    int32_t rc;
+    if (c->nbsim || o->gmin_steps != 0 || o->source_steps != 0 || !solver_graph_ready(sol)) return RS_OK;
+    if ((rc = get_pattern(c, MODE_TRAN_TRAP, dp))) return rc;
+    const i64 n = dp->p.n;
+    if ((i64)x.size() != n || sol->n != n || sol->last_ap != dp->p.ap || sol->last_ai != dp->p.ai)
+        return RS_OK;  // the solver holds another pattern
+    hipError_t e;
+    DevBuf<double> d_t;
+    DevBuf<int32_t> d_st;  // done, converged, iterations, pad, damping, previous |dx|
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    if ((e = d_t.ensure(1)) || (e = d_st.ensure(8)) || (e = hipStreamCreate(&s)) ||
+        (e = hipMemcpy(d_t.p, &t, 8, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(c->d_x.p, x.data(), n * 8, hipMemcpyHostToDevice))) {
+        cleanup();
+        return chip(c, e, "graph setup");
+    }
+    int32_t* st = d_st.p;
+    StampArgs a{0.0, dt, o->gmin, 1.0, MODE_TRAN_TRAP, 0, c->d_models.p, d_t.p, st};
+    if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) {
+        cleanup();
+        return chip(c, e, "hipStreamBeginCapture");
+    }
+    for (int m = 0; m < kGraphIters && !e; m++) {
+        if ((e = launch_eval(c->d_dev.p, (int)c->dev.size(), c->d_x.p, (int)n, a, c->d_cap_v.p, c->d_cap_i.p,
+                             c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s, nullptr, c->d_gmask.p)) ||
+            (e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p, (int)n,
+                                 c->d_sv.p, c->d_ax.p, c->d_rhs.p, s, st)) ||
+            (e = solver_graph_iteration(sol, c->d_ax.p, c->d_rhs.p, c->d_xnew.p, s, st)) ||
+            (e = launch_newton_update((int)n, c->d_x.p, c->d_xnew.p, o->abs_tol, o->rel_tol, o->damping_min,
+                                      o->max_iters, st, s)))
+            break;
+    }
+    if (!e)
+        e = launch_cap_state(c->d_dev.p, (int)c->dev.size(), c->d_x.p, (int)n, dt, 1, c->d_cap_v.p,
+                             c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p, s, st + 1);
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e || (e = e2) || (e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0))) {
+        cleanup();
+        return chip(c, e, "graph capture");
+    }
+    int64_t k = 0;
+    for (; k < nsteps; k++) {
+        int32_t h[8];
+        if ((e = launch_newton_begin(d_t.p, dt, st, o->damping, s))) break;
+        do {
+            if ((e = hipGraphLaunch(exec, s)) || (e = hipMemcpyAsync(h, st, sizeof h, hipMemcpyDeviceToHost, s)) ||
+                (e = hipStreamSynchronize(s)))
+                break;
+            c->graph_launches++;
+        } while (!h[0]);
+        if (e) break;
+        *iters += h[2];
+        c->newton_iters += h[2];
+        c->factor_calls += h[2];
+        c->stamp_calls += h[2];
+        sol->refactor_count += h[2];
+        c->graph_steps++;
+        if (!h[1]) {
+            *status = 2;
+            break;
+        }
+    }
+    if (!e) e = hipMemcpy(x.data(), c->d_x.p, n * 8, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e) return chip(c, e, "graph step");
+    *done = k;
+    return RS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1102,6 +1197,12 @@ int32_t rs_tran_fixed(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, do
         double t = 0.0;
         for (int64_t k = 0; k < nsteps; k++) {
             double tt = t + dt;
+            if (k >= 1 && c->use_graph) {  // the remaining steps as captured graphs
+                int64_t done = 0;
+                rc = tran_fixed_graph(c, sol, o, x, t, dt, nsteps - k, &done, &it, &status);
+                if (rc) return rc;
+                if (done >= 0) break;  // the graph path ran (or stopped at a failed step)
+            }
             rc = run_newton_stepping(c, sol, o, x, MODE_TRAN_TRAP, tt, dt, &r, &it);
             if (rc) return rc;
             if (!r.converged) {
@@ -1119,6 +1220,10 @@ int32_t rs_tran_fixed(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, do
     if (status_out) *status_out = status;
     return RS_OK;
 }
+
+void rs_circuit_set_tran_graph(rs_circuit* c, int32_t on) { c->use_graph = on != 0; }
+
+int64_t rs_circuit_graph_steps(const rs_circuit* c) { return c->graph_steps; }
 
 int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double dt,
                         int64_t nsteps, int64_t probe, int64_t every, double* dp_out,

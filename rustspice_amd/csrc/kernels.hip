@@ -61,8 +61,10 @@ __global__ __launch_bounds__(1024) void k_refactor(const PlanDev* __restrict__ d
                                                   const int32_t* __restrict__ ip,
                                                   const double* __restrict__ axpool,
                                                   double* __restrict__ vpool,
-                                                  double* __restrict__ wpool) {
+                                                  double* __restrict__ wpool,
+                                                  const int32_t* __restrict__ skip) {
     extern __shared__ double smem[];
+    if (skip && *skip) return;
     const PlanDev d = descs[blockIdx.x];
     const int n = d.n;
     const int lane = threadIdx.x & (kWave - 1);
@@ -140,8 +142,10 @@ template <bool kLdsV>
 __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict__ descs,
                                                      const int32_t* __restrict__ ip,
                                                      const double* __restrict__ axpool,
-                                                     double* __restrict__ vpool) {
+                                                     double* __restrict__ vpool,
+                                                     const int32_t* __restrict__ skip) {
     extern __shared__ double smem[];
+    if (skip && *skip) return;
     const PlanDev d = descs[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
@@ -361,8 +365,10 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
                                                const double* __restrict__ vpool,
                                                const double* __restrict__ rhspool,
                                                double* __restrict__ xpool,
-                                               double* __restrict__ wpool) {
+                                               double* __restrict__ wpool,
+                                               const int32_t* __restrict__ skip) {
     extern __shared__ double smem[];
+    if (skip && *skip) return;
     const PlanDev d = descs[blockIdx.x];
     const int n = d.n;
     const int nw = blockDim.x / kWave;
@@ -391,41 +397,41 @@ namespace rsgpu {
 // schedule), bit 1 right-looking schedule.
 hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                            double* vals, double* wpool, int max_n, int mode, int threads,
-                           hipStream_t stream, int rl_nv) {
+                           hipStream_t stream, int rl_nv, const int32_t* skip) {
     const bool lds = mode & 1;
     if (mode & 2) {
         size_t shm = (size_t)(threads / kWave) * kWave * sizeof(double);
         if (lds) {
             shm += (size_t)rl_nv * sizeof(double);
             hipLaunchKernelGGL(k_refactor_rl<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip,
-                               ax, vals);
+                               ax, vals, skip);
         } else {
             hipLaunchKernelGGL(k_refactor_rl<false>, dim3(nmat), dim3(threads), shm, stream, descs,
-                               ip, ax, vals);
+                               ip, ax, vals, skip);
         }
         return hipGetLastError();
     }
     size_t shm = lds ? (size_t)(threads / kWave) * max_n * sizeof(double) : 0;
     if (lds)
         hipLaunchKernelGGL(k_refactor<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax,
-                           vals, wpool);
+                           vals, wpool, skip);
     else
         hipLaunchKernelGGL(k_refactor<false>, dim3(nmat), dim3(threads), 0, stream, descs, ip, ax,
-                           vals, wpool);
+                           vals, wpool, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
-                        bool lds, int threads, hipStream_t stream) {
+                        bool lds, int threads, hipStream_t stream, const int32_t* skip) {
     size_t shm = (size_t)(threads / kWave) * kWave * sizeof(double) +
                  (lds ? (size_t)max_n * sizeof(double) : 0);
     if (lds)
         hipLaunchKernelGGL(k_solve<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax, vals,
-                           rhs, x, wpool);
+                           rhs, x, wpool, skip);
     else
         hipLaunchKernelGGL(k_solve<false>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax, vals,
-                           rhs, x, wpool);
+                           rhs, x, wpool, skip);
     return hipGetLastError();
 }
 

@@ -446,21 +446,22 @@ std::vector<Shape> solve_shapes(const rs_solver* h) {
     return v;
 }
 
-hipError_t refactor_launch(rs_solver* h, const double* ax_dev, int mode, int threads, hipStream_t s) {
+hipError_t refactor_launch(rs_solver* h, const double* ax_dev, int mode, int threads, hipStream_t s,
+                           const int32_t* skip = nullptr) {
     if (mode == 4)
         return launch_sp_refactor(h->d_spd.p, h->d_desc.p, h->d_ip.p, h->d_spi.p, h->d_spw.p, ax_dev,
-                                  h->d_vals.p, h->spd.nslots, threads, s);
+                                  h->d_vals.p, h->spd.nslots, threads, s, skip);
     return launch_refactor(h->d_desc.p, 1, h->d_ip.p, ax_dev, h->d_vals.p, h->d_work.p, (int)h->n,
-                           mode, threads, s, h->pd.rl_nv);
+                           mode, threads, s, h->pd.rl_nv, skip);
 }
 
 hipError_t solve_launch(rs_solver* h, const double* rhs_dev, double* x_dev, int mode, int threads,
-                        hipStream_t s) {
+                        hipStream_t s, const int32_t* skip = nullptr) {
     if (mode == 2)
         return launch_sp_solve(h->d_spd.p, h->d_desc.p, h->d_ip.p, h->d_spi.p, h->d_spw.p, h->d_ax.p,
-                               h->d_vals.p, rhs_dev, x_dev, h->spd.nslots, threads, s);
+                               h->d_vals.p, rhs_dev, x_dev, h->spd.nslots, threads, s, skip);
     return launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev, h->d_work.p,
-                        (int)h->n, mode == 1, threads, s);
+                        (int)h->n, mode == 1, threads, s, skip);
 }
 
 // Mean time of `reps` launches of f on stream s (after one warm-up), in us.
@@ -614,6 +615,26 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
     prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_solve launch");
     return RS_OK;
+}
+
+// The refactor and solve of one Newton iteration as plain launches on s (for
+// graph capture): the pattern must be factored with its launch shapes
+// settled, and nothing here reads back to the host. Kernels return at once
+// while *skip != 0.
+bool solver_graph_ready(const rs_solver* h) {
+    return !h->bbd && !h->dense && h->state == ST_FACTORED && h->plan_on_device &&
+           (h->forced || !h->autotune || (h->tuned_ref && h->tuned_sol));
+}
+
+hipError_t solver_graph_iteration(rs_solver* h, const double* ax_dev, const double* rhs_dev, double* x_dev,
+                                  hipStream_t s, const int32_t* skip) {
+    hipError_t e;
+    if (h->ph.nod > 0 && ax_dev != h->d_ax.p &&
+        (e = hipMemcpyAsync(h->d_ax.p, ax_dev, h->last_ap.back() * sizeof(double), hipMemcpyDeviceToDevice,
+                            s)) != hipSuccess)
+        return e;
+    if ((e = refactor_launch(h, ax_dev, h->ref_mode, h->ref_threads, s, skip)) != hipSuccess) return e;
+    return solve_launch(h, rhs_dev, x_dev, h->sol_mode, h->sol_threads, s, skip);
 }
 
 }  // namespace rsgpu

@@ -14,18 +14,19 @@ namespace rsgpu {
 // mode: bit 0 LDS, bit 1 right-looking schedule (rl_nv value slots)
 hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                            double* vals, double* wpool, int max_n, int mode, int threads,
-                           hipStream_t stream, int rl_nv = 0);
+                           hipStream_t stream, int rl_nv = 0, const int32_t* skip = nullptr);
 hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
-                        bool lds, int threads, hipStream_t stream);
+                        bool lds, int threads, hipStream_t stream, const int32_t* skip = nullptr);
 size_t device_max_lds();
 hipError_t launch_sp_refactor(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
                               const int32_t* sip, const SpOpWord* words, const double* ax,
-                              double* vals, int nslots, int threads, hipStream_t s);
+                              double* vals, int nslots, int threads, hipStream_t s,
+                              const int32_t* skip = nullptr);
 hipError_t launch_sp_solve(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
                            const int32_t* sip, const SpOpWord* words, const double* ax,
                            const double* vals, const double* rhs, double* x, int nslots, int threads,
-                           hipStream_t s);
+                           hipStream_t s, const int32_t* skip = nullptr);
 size_t sp_solver_static_lds();
 
 struct BbdImpl;
@@ -117,3 +118,8 @@ struct rs_solver {
     bool rcond_stale = false;
 };
 
+namespace rsgpu {
+bool solver_graph_ready(const rs_solver* h);
+hipError_t solver_graph_iteration(rs_solver* h, const double* ax_dev, const double* rhs_dev, double* x_dev,
+                                  hipStream_t s, const int32_t* skip);
+}  // namespace rsgpu

@@ -69,7 +69,9 @@ struct StampArgs {
     double t, dt, gmin, source_scale;
     int32_t mode;
     int32_t flags;
-    const rs_bsim_model* models;  // BSIM model table (device memory) or NULL
+    const rs_bsim_model* models = nullptr;  // BSIM model table (device memory) or NULL
+    const double* t_dev = nullptr;          // time read from device memory instead of t
+    const int32_t* skip = nullptr;          // evaluation skipped while *skip != 0
 };
 
 }  // namespace rsgpu

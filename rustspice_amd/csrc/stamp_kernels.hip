@@ -31,6 +31,8 @@ __global__ __launch_bounds__(256) void k_eval(const rs_device* __restrict__ dev,
                                               uint8_t* __restrict__ gmask) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndev) return;
+    if (a.skip && *a.skip) return;
+    if (a.t_dev) a.t = *a.t_dev;
     const rs_device v = dev[d];
     if (sbase && rsgpu::is_bsim(v)) {
         gmask[d] = rsgpu::eval_bsim(v, x, xl, a, sv + sbase[d]);
@@ -61,7 +63,9 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax
                                                   const int32_t* __restrict__ rhs_src, int n,
                                                   const double* __restrict__ sv,
                                                   double* __restrict__ ax,
-                                                  double* __restrict__ rhs) {
+                                                  double* __restrict__ rhs,
+                                                  const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nnz) {
         const int s = ax_src[t];
@@ -81,9 +85,11 @@ __global__ __launch_bounds__(256) void k_cap_state(const rs_device* __restrict__
                                                    int full, double* __restrict__ cap_v,
                                                    double* __restrict__ cap_i,
                                                    uint8_t* __restrict__ has_v,
-                                                   uint8_t* __restrict__ has_i) {
+                                                   uint8_t* __restrict__ has_i,
+                                                   const int32_t* __restrict__ gate) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndev || dev[d].kind != RS_DEV_C) return;
+    if (gate && !*gate) return;
     const double v = xget(x, xl, dev[d].node[0]) - xget(x, xl, dev[d].node[1]);
     if (full) {
         const double vp = has_v[d] ? cap_v[d] : v;
@@ -118,20 +124,98 @@ hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, 
 
 hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
                            const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
-                           hipStream_t s) {
+                           hipStream_t s, const int32_t* skip) {
     int m = nnz > n ? nnz : n;
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3((m + 255) / 256), dim3(256), 0, s, ax_src, nnz, rhs_ptr,
-                       rhs_src, n, sv, ax, rhs);
+                       rhs_src, n, sv, ax, rhs, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
-                            uint8_t* has_i, hipStream_t s) {
+                            uint8_t* has_i, hipStream_t s, const int32_t* gate) {
     if (ndev <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cap_state, dim3((ndev + 255) / 256), dim3(256), 0, s, dev, ndev, x, xl,
-                       dt, full, cap_v, cap_i, has_v, has_i);
+                       dt, full, cap_v, cap_i, has_v, has_i, gate);
+    return hipGetLastError();
+}
+
+// ---- one Newton iteration's update on the device (run_newton of the engine,
+// newton.rs:65-117) for graph-captured transient steps --------------------------
+// state: done, converged, iterations, pad; then damping, previous |dx|.
+__global__ __launch_bounds__(256) void k_newton_begin(double* __restrict__ t_dev, double dt,
+                                                      int32_t* __restrict__ st, double damping) {
+    if (threadIdx.x == 0) {
+        *t_dev = *t_dev + dt;
+        st[0] = 0;
+        st[1] = 0;
+        st[2] = 0;
+        double* sd = reinterpret_cast<double*>(st + 4);
+        sd[0] = damping;
+        sd[1] = 1.7976931348623157e308;
+    }
+}
+
+// x <- x_new when every |dx_i| <= abs_tol + rel_tol |x_new_i| (converged),
+// else the damped step x + damping (x_new - x), with damping halved (not
+// below damping_min) when |dx| grew; done after max_iters. One workgroup;
+// |dx| is summed in index order by one thread, as the host loop sums it.
+__global__ __launch_bounds__(256) void k_newton_update(int n, double* __restrict__ x,
+                                                       const double* __restrict__ xn, double abs_tol,
+                                                       double rel_tol, double damping_min,
+                                                       int max_iters, int32_t* __restrict__ st) {
+    __shared__ int bad;
+    __shared__ double dmp;
+    if (st[0]) return;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double d = xn[i] - x[i];
+        if (!(fabs(d) <= abs_tol + rel_tol * fabs(xn[i]))) bad = 1;
+    }
+    __syncthreads();
+    double* sd = reinterpret_cast<double*>(st + 4);
+    if (!bad) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xn[i];
+        if (threadIdx.x == 0) {
+            st[2] += 1;
+            st[1] = 1;
+            st[0] = 1;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double d = xn[i] - x[i];
+            s += d * d;
+        }
+        const double dxn = sqrt(s);
+        double damping = sd[0];
+        if (dxn > sd[1] && damping > damping_min) {
+            const double d2 = damping * 0.5;
+            damping = d2 >= damping_min ? d2 : damping_min;
+        }
+        sd[0] = damping;
+        sd[1] = dxn;
+        dmp = damping;
+        st[2] += 1;
+        if (st[2] >= max_iters) st[0] = 1;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = x[i] + dmp * (xn[i] - x[i]);
+}
+
+hipError_t launch_newton_begin(double* t_dev, double dt, int32_t* st, double damping, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_begin, dim3(1), dim3(64), 0, s, t_dev, dt, st, damping);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton_update(int n, double* x, const double* xn, double abs_tol, double rel_tol,
+                                double damping_min, int max_iters, int32_t* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_update, dim3(1), dim3(256), 0, s, n, x, xn, abs_tol, rel_tol,
+                       damping_min, max_iters, st);
     return hipGetLastError();
 }
 

@@ -99,6 +99,10 @@ def _bind():
     L.rs_dc_sweep.restype = C.c_int32
     L.rs_dc_sweep.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_int64, C.c_double, C.c_double,
                               C.c_double, C.c_int64, vp, vp, C.c_int64, C.POINTER(SweepResult)]
+    L.rs_circuit_set_tran_graph.restype = None
+    L.rs_circuit_set_tran_graph.argtypes = [vp, C.c_int32]
+    L.rs_circuit_graph_steps.restype = C.c_int64
+    L.rs_circuit_graph_steps.argtypes = [vp]
     L.rs_tran_adjoint.restype = C.c_int32
     L.rs_tran_adjoint.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, C.c_int64,
                                   C.c_int64, vp, vp, C.c_int64, C.POINTER(C.c_int64),
@@ -257,6 +261,13 @@ class GpuCircuit:
         return dict(status=res.status, iterations=res.iterations, factor_calls=res.factor_calls,
                     values=values[: res.npoints].copy(), sols=sols[: res.nsolved].copy(),
                     fail_value=res.fail_value)
+
+    def set_tran_graph(self, on=True):
+        """rs_circuit_set_tran_graph: tran_fixed's steps as captured hipGraphs."""
+        self._L.rs_circuit_set_tran_graph(self.h, int(bool(on)))
+
+    def graph_steps(self) -> int:
+        return int(self._L.rs_circuit_graph_steps(self.h))
 
     def tran_adjoint(self, solver: GpuSolver, dt, nsteps, probe: int, every: int = 25, opts=None):
         """rs_tran_adjoint: d x_N[probe] / dp for every device after the
