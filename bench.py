@@ -142,6 +142,7 @@ def adjoint_cost(E, NL, S, torch, reps=5):
     x, r = g.dc_op(s)
     probe = c.node_count // 2
     g.dc_adjoint(x, probe)  # first call: host analysis of the Jacobian pattern
+    g.dc_adjoint(x, probe)  # second: the first device refactor times the launch shapes
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -237,6 +238,28 @@ def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
                 max_abs_dp=dict(R=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_R]).max()),
                                 C=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_C]).max()),
                                 W=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_M]).max())))
+
+
+def tran_graph_cost(E, NL, S):
+    """Fixed-step TRAP transients host-driven against every step after the
+    first as a captured hipGraph (bit-identical results)."""
+    out = {}
+    for name, c, dt, n in (("rc_ladder30", NL.rc_ladder(30), 1e-10, 500),
+                           ("rcmesh16_taps8", NL.rc_mesh_inverters(16, 16), 2e-12, 200)):
+        row = {}
+        for graph in (False, True):
+            g = E.GpuCircuit(c)
+            g.set_tran_graph(graph)
+            s = S.GpuSolver(c.node_count)
+            g.tran_fixed(s, dt, 3)
+            t0 = time.perf_counter()
+            st, x, it = g.tran_fixed(s, dt, n)
+            row["graph_ms" if graph else "host_ms"] = 1e3 * (time.perf_counter() - t0)
+            row["newton_iters"] = int(it)
+            row["graph_steps" if graph else "status"] = g.graph_steps() if graph else int(st)
+        row["speedup"] = row["host_ms"] / row["graph_ms"]
+        out[name] = row
+    return out
 
 
 def cpu_baseline_mesh(mesh, np, reps=10):
@@ -381,6 +404,7 @@ def main():
     if not args.no_adjoint:
         line["adjoint"] = adjoint_cost(E, NL, S, torch)
         line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
+        line["tran_graph"] = tran_graph_cost(E, NL, S)
         line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
     if not args.no_cpu:
         cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])

@@ -541,7 +541,7 @@ int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double 
 // J^T lam = rhs with the factor of the last adj_factor of this mode (J still
 // in c->d_ax), refined against the exact J^T (a pivot below SparseLU's 1e-14
 // clamp would otherwise leave its rank-one error in lambda).
-int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_dev, int max_refine = 30) {
+int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_dev, int max_refine = 12) {
     rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
     rs_solver* a = A.s;
     const i64 n = a->n;
@@ -555,7 +555,7 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
     double prev = 1.7976931348623157e308;
     for (int it = 0; it < max_refine; ++it) {
         double nrm[2] = {0.0, 1.0};
-        const bool last = it + 1 == max_refine && max_refine < 30;  // a fixed count: no read-back
+        const bool last = it + 1 == max_refine && max_refine < 12;  // a fixed count: no read-back
         if ((e = launch_resid_t(A.jap.p, A.jai.p, c->d_ax.p, rhs_dev, lam_dev, c->d_r.p, (int)n, s)) ||
             (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_r.p, A.ltp.p, A.ltc.p, A.n_lt,
                                 c->d_t.p, c->d_del.p, s)) ||
@@ -568,7 +568,7 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
         }
         c->adj_refine = std::max(c->adj_refine, it + 1);
         // converged, or stalled at the round-off of the residual (also stops on NaN)
-        if (!(nrm[0] > 1e-15 * nrm[1]) || (nrm[0] <= 1e-10 * nrm[1] && nrm[0] >= 0.5 * prev)) break;
+        if (!(nrm[0] > 1e-14 * nrm[1]) || (nrm[0] <= 1e-9 * nrm[1] && nrm[0] >= 0.5 * prev)) break;
         prev = nrm[0];
     }
     return RS_OK;

@@ -431,16 +431,18 @@ struct Shape {
     int threads;
     int mode;  // refactor_mode_fits / solve_mode_fits
 };
+// Wide launches first: they are usually the fastest, which lets
+// time_launches stop timing the slow shapes after their warm-up launch.
 std::vector<Shape> refactor_shapes(const rs_solver* h) {
     std::vector<Shape> v;
-    for (int t = 64; t <= kMaxThreads; t *= 2)
+    for (int t = kMaxThreads; t >= 64; t /= 2)
         for (int m : {4, 3, 2, 1, 0})
             if (refactor_mode_fits(h, t, m)) v.push_back({t, m});
     return v;
 }
 std::vector<Shape> solve_shapes(const rs_solver* h) {
     std::vector<Shape> v;
-    for (int t = 64; t <= kMaxThreads; t *= 2)
+    for (int t = kMaxThreads; t >= 64; t /= 2)
         for (int m : {2, 1, 0})
             if (solve_mode_fits(h, t, m)) v.push_back({t, m});
     return v;
@@ -464,22 +466,35 @@ hipError_t solve_launch(rs_solver* h, const double* rhs_dev, double* x_dev, int 
                         (int)h->n, mode == 1, threads, s, skip);
 }
 
-// Mean time of `reps` launches of f on stream s (after one warm-up), in us.
+// Mean time of `reps` launches of f on stream s, in us, after one warm-up
+// launch that is timed too: a shape whose warm-up already takes more than
+// three times the best time so far (best_us > 0) is not timed further and
+// reports the warm-up time.
 template <class F>
-float time_launches(F&& f, hipStream_t s, int reps, hipError_t& err) {
+float time_launches(F&& f, hipStream_t s, int reps, hipError_t& err, float best_us = -1.0f) {
     hipEvent_t a, b;
     float ms = -1.0f;
     if ((err = hipEventCreate(&a)) != hipSuccess) return ms;
     if ((err = hipEventCreate(&b)) != hipSuccess) { (void)hipEventDestroy(a); return ms; }
-    if ((err = f()) == hipSuccess && (err = hipEventRecord(a, s)) == hipSuccess) {
-        for (int r = 0; r < reps && err == hipSuccess; r++) err = f();
-        if (err == hipSuccess && (err = hipEventRecord(b, s)) == hipSuccess &&
-            (err = hipEventSynchronize(b)) == hipSuccess)
-            err = hipEventElapsedTime(&ms, a, b);
+    float warm = -1.0f;
+    if ((err = hipEventRecord(a, s)) == hipSuccess && (err = f()) == hipSuccess &&
+        (err = hipEventRecord(b, s)) == hipSuccess && (err = hipEventSynchronize(b)) == hipSuccess &&
+        (err = hipEventElapsedTime(&warm, a, b)) == hipSuccess) {
+        if (best_us > 0.0f && warm * 1000.0f > 3.0f * best_us) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            return warm * 1000.0f;
+        }
+        if ((err = hipEventRecord(a, s)) == hipSuccess) {
+            for (int r = 0; r < reps && err == hipSuccess; r++) err = f();
+            if (err == hipSuccess && (err = hipEventRecord(b, s)) == hipSuccess &&
+                (err = hipEventSynchronize(b)) == hipSuccess)
+                err = hipEventElapsedTime(&ms, a, b);
+        }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    return err == hipSuccess ? ms * 1000.0f / reps : -1.0f;
+    return err == hipSuccess && ms >= 0.0f ? ms * 1000.0f / reps : -1.0f;
 }
 
 int32_t tune_refactor(rs_solver* h, const double* ax_dev, hipStream_t s) {
@@ -489,7 +504,7 @@ int32_t tune_refactor(rs_solver* h, const double* ax_dev, hipStream_t s) {
         hipError_t e;
         const float us = time_launches(
             [&] { return refactor_launch(h, ax_dev, sh.mode, sh.threads, s); },
-            s, 3, e);
+            s, 3, e, best);
         if (e != hipSuccess) return hip_fail(h, e, "k_refactor (autotune)");
         if (us >= 0.0f && (best < 0.0f || us < best)) {
             best = us;
@@ -592,7 +607,8 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
         Shape pick{h->sol_threads, h->sol_mode};
         for (const Shape& sh : solve_shapes(h)) {
             const float us = time_launches(
-                [&] { return solve_launch(h, rhs_dev, h->d_tune.p, sh.mode, sh.threads, s); }, s, 3, e);
+                [&] { return solve_launch(h, rhs_dev, h->d_tune.p, sh.mode, sh.threads, s); }, s, 3, e,
+                best);
             if (e != hipSuccess) return hip_fail(h, e, "k_solve (autotune)");
             if (us >= 0.0f && (best < 0.0f || us < best)) {
                 best = us;
