@@ -115,6 +115,16 @@ int32_t rs_bbd_info(const rs_solver *h, int64_t *out);
  * solve lds, best refactor ns, best solve ns}; returns bit 0 refactor tuned,
  * bit 1 solve tuned, bit 2 shapes fixed by the caller. */
 void rs_set_autotune(rs_solver *h, int32_t on);
+/* Pivot-growth check of device refactors (no reference counterpart; off by
+ * default, as the reference always reuses the first factor's pivots on an
+ * unchanged pattern, native_klu.rs:1244-1255). With limit > 0, every device
+ * refactor measures max|U| / max|A| on the device; above the limit, or with a
+ * non-finite U, the pattern is factored afresh on the host with new pivots
+ * (counted as a factor, not a refactor). rs_growth_fallbacks: refactors so
+ * replaced; rs_last_growth: the last measured ratio. */
+void rs_set_growth_check(rs_solver *h, double limit);
+int64_t rs_growth_fallbacks(const rs_solver *h);
+double rs_last_growth(const rs_solver *h);
 int32_t rs_set_launch_shape(rs_solver *h, int32_t ref_threads, int32_t ref_mode, int32_t sol_threads,
                             int32_t sol_lds);
 int32_t rs_launch_shape(const rs_solver *h, int64_t *out);

@@ -389,9 +389,42 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T[perm_out[i]];
 }
 
+// out[0] = max |ax|, out[1] = max |ux| (+inf when some ux is not finite):
+// the element growth of a refactor on reused pivots. One workgroup.
+__global__ __launch_bounds__(256) void k_growth(const double* __restrict__ ax, int nnz,
+                                                const double* __restrict__ ux, int nu,
+                                                double* __restrict__ out) {
+    __shared__ double sa[256], su[256];
+    double ma = 0.0, mu = 0.0;
+    for (int i = threadIdx.x; i < nnz; i += blockDim.x) ma = fmax(ma, fabs(ax[i]));
+    for (int i = threadIdx.x; i < nu; i += blockDim.x) {
+        const double v = fabs(ux[i]);
+        mu = (v <= 1.7976931348623157e308) ? fmax(mu, v) : __builtin_huge_val();
+    }
+    sa[threadIdx.x] = ma;
+    su[threadIdx.x] = mu;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + w]);
+            su[threadIdx.x] = fmax(su[threadIdx.x], su[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sa[0];
+        out[1] = su[0];
+    }
+}
+
 }  // namespace
 
 namespace rsgpu {
+
+hipError_t launch_growth(const double* ax, int nnz, const double* ux, int nu, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_growth, dim3(1), dim3(256), 0, s, ax, nnz, ux, nu, out);
+    return hipGetLastError();
+}
 
 // mode: bit 0 LDS (work vectors, or the values of the right-looking
 // schedule), bit 1 right-looking schedule.

@@ -539,6 +539,31 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
     e = refactor_launch(h, ax_dev, h->ref_mode, h->ref_threads, s);
     prof_end(ps, s);
     if (e != hipSuccess) return hip_fail(h, e, "k_refactor launch");
+    h->refactor_fell_back = false;
+    if (h->growth_limit > 0.0) {
+        // growth max|U| / max|A| of the reused pivots; above the limit (or a
+        // non-finite U) the pattern is factored afresh with new pivots
+        double g[2];
+        const i64 nnz = h->last_ap.back();
+        if ((e = h->d_growth.ensure(2)) != hipSuccess ||
+            (e = launch_growth(ax_dev, (int)nnz, h->d_vals.p + h->pd.o_ux, h->ph.nu, h->d_growth.p, s)) !=
+                hipSuccess ||
+            (e = hipMemcpyAsync(g, h->d_growth.p, sizeof g, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return hip_fail(h, e, "growth check");
+        h->last_growth = g[0] > 0.0 ? g[1] / g[0] : g[1];
+        if (!(h->last_growth <= h->growth_limit)) {
+            std::vector<double> ax(nnz);
+            if ((e = hipMemcpy(ax.data(), ax_dev, nnz * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+                return hip_fail(h, e, "hipMemcpy(ax)");
+            const std::vector<i64> ap = h->last_ap, ai = h->last_ai;
+            const int32_t rc = host_full_factor(h, ap.data(), ai.data(), ax.data());
+            if (rc) return rc;
+            h->growth_fallbacks++;
+            h->refactor_fell_back = true;
+            return RS_OK;  // upload_values computed rcond
+        }
+    }
     if (sync_rcond) {
         h->h_ud.resize(h->n);
         if ((e = hipMemcpyAsync(h->h_ud.data(), h->d_vals.p + h->pd.o_ud, h->n * 8,
@@ -580,8 +605,8 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
     if (h->state == ST_FACTORED && h->plan_on_device) {
         int32_t rc = gpu_refactor(h, ax_dev, s, false);
         if (rc) return rc;
-        h->refactor_count++;
-        h->last_parallel = true;
+        (h->refactor_fell_back ? h->factor_count : h->refactor_count)++;
+        h->last_parallel = !h->refactor_fell_back;
     } else {
         std::vector<double> ax(ap[h->n]);
         hipError_t e = hipMemcpyAsync(ax.data(), ax_dev, ax.size() * sizeof(double),
@@ -638,7 +663,7 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
 // settled, and nothing here reads back to the host. Kernels return at once
 // while *skip != 0.
 bool solver_graph_ready(const rs_solver* h) {
-    return !h->bbd && !h->dense && h->state == ST_FACTORED && h->plan_on_device &&
+    return !h->bbd && !h->dense && h->state == ST_FACTORED && h->plan_on_device && h->growth_limit <= 0.0 &&
            (h->forced || !h->autotune || (h->tuned_ref && h->tuned_sol));
 }
 
@@ -733,8 +758,8 @@ int32_t rs_factor(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t
         if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
         int32_t rc = gpu_refactor(h, h->d_ax.p, nullptr, true);
         if (rc) return rc;
-        h->refactor_count++;
-        h->last_parallel = true;
+        (h->refactor_fell_back ? h->factor_count : h->refactor_count)++;
+        h->last_parallel = !h->refactor_fell_back;
     } else {
         int32_t rc = host_full_factor(h, ap, ai, ax);
         if (rc) return rc;
@@ -793,6 +818,12 @@ int32_t rs_bbd_info(const rs_solver* h, int64_t* out) {
 }
 
 void rs_set_parallel_threads(rs_solver* h, int64_t t) { h->threads_hint = t; }
+
+void rs_set_growth_check(rs_solver* h, double limit) { h->growth_limit = limit > 0.0 ? limit : 0.0; }
+
+int64_t rs_growth_fallbacks(const rs_solver* h) { return h->growth_fallbacks; }
+
+double rs_last_growth(const rs_solver* h) { return h->last_growth; }
 
 void rs_set_autotune(rs_solver* h, int32_t on) {
     h->autotune = on != 0;

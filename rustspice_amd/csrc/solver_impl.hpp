@@ -28,6 +28,7 @@ hipError_t launch_sp_solve(const SpSolverDev* sd, const PlanDev* pd, const int32
                            const double* vals, const double* rhs, double* x, int nslots, int threads,
                            hipStream_t s, const int32_t* skip = nullptr);
 size_t sp_solver_static_lds();
+hipError_t launch_growth(const double* ax, int nnz, const double* ux, int nu, double* out, hipStream_t s);
 
 struct BbdImpl;
 struct DenseImpl;
@@ -116,10 +117,16 @@ struct rs_solver {
     rsgpu::DevBuf<double> d_vals, d_ax, d_rhs, d_x, d_work;
     std::vector<double> h_ud;
     bool rcond_stale = false;
+    // pivot-growth check of device refactors (0: off, the reference's behaviour)
+    double growth_limit = 0.0;
+    int64_t growth_fallbacks = 0;  // refactors replaced by a fresh pivoting factor
+    bool refactor_fell_back = false;
+    double last_growth = 0.0;
+    rsgpu::DevBuf<double> d_growth;
 };
 
 namespace rsgpu {
-bool solver_graph_ready(const rs_solver* h);
+bool solver_graph_ready(const rs_solver* h);  // false while the growth check is on
 hipError_t solver_graph_iteration(rs_solver* h, const double* ax_dev, const double* rhs_dev, double* x_dev,
                                   hipStream_t s, const int32_t* skip);
 }  // namespace rsgpu

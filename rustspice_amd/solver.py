@@ -113,6 +113,15 @@ class GpuSolver:
         pattern and keep the fastest (default on)."""
         self._L.rs_set_autotune(self.h, int(bool(on)))
 
+    def set_growth_check(self, limit: float):
+        """Re-factor with new pivots when a device refactor's growth
+        max|U|/max|A| exceeds `limit` (0 disables: the reference's behaviour)."""
+        self._L.rs_set_growth_check(self.h, float(limit))
+
+    def growth(self) -> dict:
+        return dict(fallbacks=int(self._L.rs_growth_fallbacks(self.h)),
+                    last=float(self._L.rs_last_growth(self.h)))
+
     def set_launch_shape(self, ref_threads, ref_mode, sol_threads, sol_mode):
         """ref_mode: 0/1 column kernel (1: LDS work vectors), 2/3 right-looking
         schedule (3: values in LDS), 4 step program; sol_mode: 0 solve vector

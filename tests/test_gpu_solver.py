@@ -250,3 +250,39 @@ def test_autotune_picks_a_shape_and_keeps_parity(gpu):
     # later refactors reuse the shape; a new pattern is tuned again
     g.factor(ap, ai, ax)
     assert g.launch_shape()["ref_threads"] == sh["ref_threads"]
+
+
+def test_growth_check_refactors_afresh_when_reused_pivots_grow(gpu):
+    """The reference reuses the first factor's pivots on every later factor of
+    the pattern (native_klu.rs:1244-1255). With the growth check on, a
+    refactor whose reused pivots blow up max|U| is replaced by a fresh
+    pivoting factor; off (the default), the result is the reference's."""
+    ap = np.array([0, 2, 4])
+    ai = np.array([0, 1, 0, 1])
+    ax0 = np.array([4.0, 1.0, 1.0, 3.0])     # no row swap in the first factor
+    ax1 = np.array([1e-12, 1.0, 1.0, 1.0])   # the reused pivot is now tiny
+    rhs = np.array([1.0, 2.0])
+    A1 = np.array([[1e-12, 1.0], [1.0, 1.0]])
+    want = np.linalg.solve(A1, rhs)
+    o = O.NativeKlu(2)
+    o.set_config(use_btf=False)
+    o.factor(ap, ai, ax0)
+    o.factor(ap, ai, ax1)
+    xo = rhs.copy()
+    o.solve(xo)
+    for limit in (0.0, 1e6):
+        g = S.GpuSolver(2)
+        g.set_config(use_btf=False)
+        g.set_growth_check(limit)
+        g.factor(ap, ai, ax0)
+        g.factor(ap, ai, ax1)
+        x = rhs.copy()
+        g.solve(x)
+        st, gr = g.stats(), g.growth()
+        if limit == 0.0:
+            assert same_bits(x, xo) and gr["fallbacks"] == 0 and st["refactor_count"] == 1
+        else:
+            assert gr["fallbacks"] == 1 and gr["last"] > 1e6
+            assert st["factor_count"] == 2 and st["refactor_count"] == 0
+            assert np.allclose(x, want, rtol=1e-12)
+        g.close()
