@@ -394,7 +394,14 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
                             RPI[i] = i;
                             RP[i] = i;
                         }
-                    // RHS rows: short ones from their records, long ones cooperatively
+                    // scatter-run sums go into the L/U slots for a step-program
+                    // refactor, else into rsum for the column-by-column factor
+                    const bool spr = sp && factored;
+                    if (spr) {
+                        for (int i = tid; i < P.nl + P.nu; i += kT) VL[i] = 0.0;
+                        bsync<kWv>();
+                    }
+                    // RHS rows and scatter runs from their short records ...
                     for (int base = 0; base < n; base += 4 * kT) {
                         int4 rr[4];
 #pragma unroll
@@ -409,20 +416,7 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
                             if (i < n && rr[j].x <= 3) XN[i] = v;
                         }
                     }
-                    for (int j = wave; j < P.n_lrow; j += kWv) {
-                        const int i = lrow[j];
-                        const int4 rr = rhsrec[i];
-                        const double v = coop_sum(rhs_src + rr.y, rr.x, sv, Sw, lane);
-                        if (lane == 0) XN[i] = v;
-                    }
                     PHASE(1)
-                    // scatter-run sums: into the L/U slots for a step-program
-                    // refactor, else into rsum for the column-by-column factor
-                    const bool spr = sp && factored;
-                    if (spr) {
-                        for (int i = tid; i < P.nl + P.nu; i += kT) VL[i] = 0.0;
-                        bsync<kWv>();
-                    }
                     for (int base = 0; base < nsc; base += 4 * kT) {
                         int4 rr[4];
                         int tt[4];
@@ -442,13 +436,21 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
                             }
                         }
                     }
-                    for (int j = wave; j < P.n_lrun; j += kWv) {
-                        const int q = lrun[j];
-                        const int4 rr = screc[q];
-                        const double v = coop_sum(ax_src + rr.y, rr.x, sv, Sw, lane);
-                        if (lane == 0) {
-                            if (!spr) rsum[q] = v;
-                            else if (ss[q] >= 0) VL[ss[q]] = v;
+                    // ... and the long ordered sums of both, one wavefront each
+                    for (int j = wave; j < P.n_lrow + P.n_lrun; j += kWv) {
+                        if (j < P.n_lrow) {
+                            const int i = lrow[j];
+                            const int4 rr = rhsrec[i];
+                            const double v = coop_sum(rhs_src + rr.y, rr.x, sv, Sw, lane);
+                            if (lane == 0) XN[i] = v;
+                        } else {
+                            const int q = lrun[j - P.n_lrow];
+                            const int4 rr = screc[q];
+                            const double v = coop_sum(ax_src + rr.y, rr.x, sv, Sw, lane);
+                            if (lane == 0) {
+                                if (!spr) rsum[q] = v;
+                                else if (ss[q] >= 0) VL[ss[q]] = v;
+                            }
                         }
                     }
                     if (spr) bsync<kWv>();
@@ -638,9 +640,12 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
                         __syncthreads();
                         for (int w = 0; w < kWv; ++w) conv = conv && SBi[1 + w] != 0;
                     }
+                    // |dx| steers the damping of the next iteration; |x_new| is only
+                    // reported, so it is summed on the iteration that ends the loop
+                    const bool need_d = !conv, need_f = conv || it + 1 >= o.max_iters;
                     double dxn_l = 0.0, fn_l = 0.0;
                     if (kWv > 1) {
-                        // the squares in parallel; the two sums stay in order on one lane
+                        // the squares in parallel; the sums stay in order on one lane
                         for (int i = tid; i < n; i += kT) {
                             const double dxi = XN[i] - X[i];
                             Q[i] = i < nd ? dxi * dxi : 0.0;
@@ -648,62 +653,76 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
                         }
                         __syncthreads();
                         if (tid == 0) {
-                            double sd = 0.0, sf = 0.0;
-                            int i = 0;
-                            for (; i + 16 <= n; i += 16) {
-                                double qd[16], qf[16];
+                            if (need_d) {
+                                double sd = 0.0;
+                                int i = 0;
+                                for (; i + 16 <= nd; i += 16) {
+                                    double qd[16];
 #pragma unroll
-                                for (int j = 0; j < 16; ++j) {
-                                    qd[j] = Q[i + j];
-                                    qf[j] = Q[nmax + i + j];
-                                }
+                                    for (int j = 0; j < 16; ++j) qd[j] = Q[i + j];
 #pragma unroll
-                                for (int j = 0; j < 16; ++j) {
-                                    if (i + j < nd) sd += qd[j];
-                                    sf += qf[j];
+                                    for (int j = 0; j < 16; ++j) sd += qd[j];
                                 }
+                                for (; i < nd; ++i) sd += Q[i];
+                                dxn_l = sqrt(sd);
                             }
-                            for (; i < n; ++i) {
-                                if (i < nd) sd += Q[i];
-                                sf += Q[nmax + i];
+                            if (need_f) {
+                                double sf = 0.0;
+                                int i = 0;
+                                for (; i + 16 <= n; i += 16) {
+                                    double qf[16];
+#pragma unroll
+                                    for (int j = 0; j < 16; ++j) qf[j] = Q[nmax + i + j];
+#pragma unroll
+                                    for (int j = 0; j < 16; ++j) sf += qf[j];
+                                }
+                                for (; i < n; ++i) sf += Q[nmax + i];
+                                fn_l = sqrt(sf);
                             }
-                            dxn_l = sqrt(sd);
-                            fn_l = sqrt(sf);
                         }
                     } else if (tid == 0) {
-                        // two ordered sums (newton.rs:98-101), reads batched 8 at a time
-                        double sd = 0.0, sf = 0.0;
-                        int i = 0;
-                        for (; i + 8 <= n; i += 8) {
-                            double xn[8], xo[8];
+                        // the ordered sums (newton.rs:98-101), reads batched 8 at a time
+                        if (need_d) {
+                            double sd = 0.0;
+                            int i = 0;
+                            for (; i + 8 <= nd; i += 8) {
+                                double xn[8], xo[8];
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) {
-                                xn[j] = XN[i + j];
-                                xo[j] = X[i + j];
-                            }
+                                for (int j = 0; j < 8; ++j) {
+                                    xn[j] = XN[i + j];
+                                    xo[j] = X[i + j];
+                                }
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) {
-                                if (i + j < nd) {
+                                for (int j = 0; j < 8; ++j) {
                                     const double dxi = xn[j] - xo[j];
                                     sd += dxi * dxi;
                                 }
-                                sf += xn[j] * xn[j];
                             }
-                        }
-                        for (; i < n; ++i) {
-                            if (i < nd) {
+                            for (; i < nd; ++i) {
                                 const double dxi = XN[i] - X[i];
                                 sd += dxi * dxi;
                             }
-                            sf += XN[i] * XN[i];
+                            dxn_l = sqrt(sd);
                         }
-                        dxn_l = sqrt(sd);
-                        fn_l = sqrt(sf);
+                        if (need_f) {
+                            double sf = 0.0;
+                            int i = 0;
+                            for (; i + 8 <= n; i += 8) {
+                                double xn[8];
+#pragma unroll
+                                for (int j = 0; j < 8; ++j) xn[j] = XN[i + j];
+#pragma unroll
+                                for (int j = 0; j < 8; ++j) sf += xn[j] * xn[j];
+                            }
+                            for (; i < n; ++i) sf += XN[i] * XN[i];
+                            fn_l = sqrt(sf);
+                        }
                     }
                     double dxn;
                     if (kWv == 1) {
                         dxn = __shfl(dxn_l, 0);
-                        r.final_norm = __shfl(fn_l, 0);
+                        const double fn = __shfl(fn_l, 0);
+                        if (need_f) r.final_norm = fn;
                     } else {
                         if (tid == 0) {
                             SB[0] = dxn_l;
@@ -711,7 +730,7 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
                         }
                         __syncthreads();
                         dxn = SB[0];
-                        r.final_norm = SB[1];
+                        if (need_f) r.final_norm = SB[1];
                     }
                     if (conv) {
                         for (int i = tid; i < n; i += kT) X[i] = XN[i];
