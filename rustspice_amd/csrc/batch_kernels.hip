@@ -770,12 +770,18 @@ size_t lds_bytes(int nmax, int lumax, int waves) {
 }
 
 // ---- one matrix by its step programs (single-matrix solver) ------------------
+// At most 512 threads: the program's two 12-step op-word buffers and operand
+// sets then keep their registers (a 1,024-thread bound leaves 128 VGPRs and
+// spills inside the chunk loop, whose reload waits for every op-word load in
+// flight). Shared memory: the nslots + 1 value slots from address 0, then 64
+// doubles per wavefront for the ordered sums of long scatter runs.
+constexpr int kSpMaxThreads = 512;
 // VL (dynamic LDS from address 0, as sp_run requires) holds the value slots:
 // lx | ux | T | off-diagonal ax | dummy. The refactor scatters A into the
 // slots (duplicates summed in CSC order from 0.0, fill entries 0), runs the
 // refactor program on wavefront 0 and writes the values back in the solver's
 // layout; the solve loads them, runs the solve program and writes x.
-__global__ __launch_bounds__(1024) void k_sp_refactor(const SpSolverDev* __restrict__ sdp,
+__global__ __launch_bounds__(kSpMaxThreads) void k_sp_refactor(const SpSolverDev* __restrict__ sdp,
                                                      const PlanDev* __restrict__ pdp,
                                                      const int32_t* __restrict__ ip,
                                                      const int32_t* __restrict__ sip,
@@ -795,11 +801,30 @@ __global__ __launch_bounds__(1024) void k_sp_refactor(const SpSolverDev* __restr
     const int32_t* ss = sip + sd.o_ss;
     for (int s = tid; s < sd.nsc; s += bs) {
         const int slot = ss[s];
-        if (slot < 0) continue;
+        const int cnt = sc_cnt[s];
+        if (slot < 0 || cnt > kSpShortRun) continue;
         double v = 0.0;
-        const int src = sc_src[s], cnt = sc_cnt[s];
+        const int src = sc_src[s];
         for (int q = 0; q < cnt; ++q) v += ax[src + q];
         VL[slot] = v;
+    }
+    {  // long runs: the wavefront loads 64 terms at a time, lane 0 adds them in order
+        double* S = VL + sd.nslots + 1 + wave * kW;
+        const int32_t* lrun = sip + sd.o_lrun;
+        for (int j = wave; j < sd.n_lrun; j += bs / kW) {
+            const int s = lrun[j];
+            const int src = sc_src[s], cnt = sc_cnt[s];
+            double v = 0.0;
+            for (int b = 0; b < cnt; b += kW) {
+                const int m = cnt - b < kW ? cnt - b : kW;
+                if (lane < m) S[lane] = ax[src + b + lane];
+                wsync();
+                if (lane == 0)
+                    for (int q = 0; q < m; ++q) v += S[q];
+                wsync();
+            }
+            if (lane == 0) VL[ss[s]] = v;
+        }
     }
     __syncthreads();
     if (wave == 0)
@@ -823,7 +848,7 @@ __global__ __launch_bounds__(1024) void k_sp_refactor(const SpSolverDev* __restr
     }
 }
 
-__global__ __launch_bounds__(1024) void k_sp_solve(const SpSolverDev* __restrict__ sdp,
+__global__ __launch_bounds__(kSpMaxThreads) void k_sp_solve(const SpSolverDev* __restrict__ sdp,
                                                   const PlanDev* __restrict__ pdp,
                                                   const int32_t* __restrict__ ip,
                                                   const int32_t* __restrict__ sip,
@@ -886,8 +911,9 @@ hipError_t launch_sp_refactor(const SpSolverDev* sd, const PlanDev* pd, const in
                               const int32_t* sip, const SpOpWord* words, const double* ax,
                               double* vals, int nslots, int threads, hipStream_t s,
                               const int32_t* skip) {
-    hipLaunchKernelGGL(k_sp_refactor, dim3(1), dim3(threads), (size_t)(nslots + 1) * sizeof(double),
-                       s, sd, pd, ip, sip, words, ax, vals, skip);
+    hipLaunchKernelGGL(k_sp_refactor, dim3(1), dim3(threads),
+                       (size_t)(nslots + 1 + threads) * sizeof(double), s, sd, pd, ip, sip, words, ax,
+                       vals, skip);
     return hipGetLastError();
 }
 

@@ -161,7 +161,7 @@ uint64_t pattern_key(const rs_solver* h) {
 // can run with t threads on this plan.
 bool refactor_mode_fits(const rs_solver* h, int t, int m) {
     const size_t maxlds = device_max_lds();
-    if (m == 4) return h->sp_ok;
+    if (m == 4) return h->sp_ok && t <= 512;  // k_sp_refactor's launch bound
     if (m & 2) {
         if (h->pd.rl_nph == 0 && h->n > 0) return false;
         return !(m & 1) || ((size_t)h->pd.rl_nv + t) * sizeof(double) <= maxlds;
@@ -175,7 +175,7 @@ int default_refactor_mode(const rs_solver* h, int t) {
 }
 // Solve mode m: 0 solve vector in global memory, 1 in LDS, 2 step program.
 bool solve_mode_fits(const rs_solver* h, int t, int m) {
-    if (m == 2) return h->sp_ok;
+    if (m == 2) return h->sp_ok && t <= 512;  // k_sp_solve's launch bound
     return m == 0 || ((size_t)h->n + t) * sizeof(double) <= device_max_lds();
 }
 int default_solve_mode(const rs_solver* h, int t) {
@@ -189,7 +189,8 @@ int default_solve_mode(const rs_solver* h, int t) {
 bool build_solver_sp(rs_solver* h) {
     static const bool static_ok = sp_solver_static_lds() == 0;
     const SpLayout L = sp_layout(h->ph);
-    if (!static_ok || L.nslots >= kSpMaxSlots || (size_t)(L.nslots + 1) * sizeof(double) > device_max_lds())
+    if (!static_ok || L.nslots >= kSpMaxSlots ||
+        (size_t)(L.nslots + 1 + 512) * sizeof(double) > device_max_lds())  // + long-run staging
         return false;
     SpProgram fac, sol;
     std::string why;
@@ -214,10 +215,16 @@ bool build_solver_sp(rs_solver* h) {
     sd.o_sol = (int64_t)fac.words.size();
     h->sp_words = std::move(fac.words);
     h->sp_words.insert(h->sp_words.end(), sol.words.begin(), sol.words.end());
+    std::vector<int32_t> lrun;
+    for (int32_t r = 0; r < (int32_t)ss.size(); r++)
+        if (ss[r] >= 0 && h->ph.sc_cnt[r] > kSpShortRun) lrun.push_back(r);
     sd.o_ss = 0;
     sd.o_rpi = (int32_t)ss.size();
+    sd.o_lrun = sd.o_rpi + (int32_t)rpi.size();
+    sd.n_lrun = (int32_t)lrun.size();
     h->sp_ints = std::move(ss);
     h->sp_ints.insert(h->sp_ints.end(), rpi.begin(), rpi.end());
+    h->sp_ints.insert(h->sp_ints.end(), lrun.begin(), lrun.end());
     h->sp_ints.push_back(0);
     return true;
 }

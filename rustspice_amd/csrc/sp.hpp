@@ -93,8 +93,10 @@ SpLayout sp_layout(const PlanHost& p);
 struct SpSolverDev {
     int32_t n, nl, nu, nod, nslots, t_base, od_base, nsc;
     int32_t fac_chunks, sol_chunks, o_ss, o_rpi;
+    int32_t o_lrun, n_lrun;  // scatter runs longer than kSpShortRun, summed by a wavefront
     int64_t o_fac, o_sol;
 };
+constexpr int32_t kSpShortRun = 16;
 
 // Builds the refactor and solve programs of a plan whose positions are
 // pre-pivot (the batched kernel's identity-pivot analysis). Returns false
