@@ -226,12 +226,20 @@ def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
     t_adj = time.perf_counter() - t0
     if info["status"] != 0:
         return dict(workload=f"{nx}x{nx} RC mesh adjoint", adjoint_info=info, forward_status=int(st))
+    # every state kept in host memory instead of checkpoints: no recomputation
+    t0 = time.perf_counter()
+    dp_all, _, info_all = g.tran_adjoint(s, dt, nsteps, probe, 0)
+    t_all = time.perf_counter() - t0
     kinds = [d["kind"] for d in c.devices]
     return dict(workload=f"{nx}x{nx} RC mesh, {nx * nx // 32} inverter taps, {nsteps} TRAP steps of 2 ps, "
                          f"checkpoint every {every}; dV(centre)/d(every R, C, W)",
                 n=int(len(xa)), forward_status=int(st), forward_ms=1e3 * t_fwd, forward_newton_iters=int(it),
                 adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd,
                 adjoint_over_exact_forward=t_adj / max(info["forward_ms"] * 1e-3, 1e-12), adjoint_info=info,
+                keep_all_states=dict(adjoint_ms=1e3 * t_all, adjoint_over_forward=t_all / t_fwd,
+                                     states=int(info_all["checkpoints"]),
+                                     recomputed_steps=int(info_all["recomputed_steps"]),
+                                     same_dp_as_checkpointed=bool(np.array_equal(dp_all, dp))),
                 sensitivities=dict(R=kinds.count(NL.RS_DEV_R), C=kinds.count(NL.RS_DEV_C),
                                    W=kinds.count(NL.RS_DEV_M)),
                 probe_v=float(xa[probe]),

@@ -209,7 +209,9 @@ int64_t rs_circuit_graph_steps(const rs_circuit *c);
  * between steps) on solver s, keeping a checkpoint (x_k, x_{k-1}, capacitor
  * history) every `every` steps; then, segment by segment from the last
  * checkpoint, recomputes the segment's steps and solves the adjoint steps
- * backward: J_k^T lambda_k = e_probe [k = nsteps] - sum over the two later
+ * backward. every <= 0 keeps every state x_0..x_N in host memory instead
+ * (nsteps x n doubles) and recomputes nothing; info.checkpoints then counts
+ * the stored states. The backward solves: J_k^T lambda_k = e_probe [k = nsteps] - sum over the two later
  * steps of (dF_j/dx_k)^T lambda_j (their capacitor currents read v_k), with
  * J_k the exact Jacobian of step k's stamped currents (SparseLU as above,
  * transposed solves with refinement). dp[ndev] receives dx_N[probe]/dp:

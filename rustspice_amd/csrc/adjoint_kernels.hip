@@ -76,30 +76,44 @@ __global__ __launch_bounds__(kT) void k_solve_t(const rsgpu::PlanDev* __restrict
 __device__ __forceinline__ double at(const double* v, int xl, int i) { return i < xl ? v[i] : 0.0; }
 
 // r = b - A x with A the unmerged CSC through its row map (rp: row pointers,
-// rq: CSC entry of each row entry, in column order), one row per thread;
-// x beyond xl reads 0. Deterministic: each row sums in a fixed order.
-__global__ void k_resid_n(const int32_t* __restrict__ rp, const int32_t* __restrict__ rc,
-                          const int32_t* __restrict__ rq, const double* __restrict__ ax,
-                          const double* __restrict__ b, const double* __restrict__ x, int xl,
-                          double* __restrict__ r, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// rq: CSC entry of each row entry, in column order); x beyond xl reads 0.
+// One wavefront per row, lanes over its entries and a butterfly sum, so a
+// dense row (the ground row of a large mesh) costs a few loads per lane
+// instead of a serial walk. Deterministic: the summation tree is fixed.
+__global__ __launch_bounds__(kT) void k_resid_n(const int32_t* __restrict__ rp,
+                                               const int32_t* __restrict__ rc,
+                                               const int32_t* __restrict__ rq,
+                                               const double* __restrict__ ax,
+                                               const double* __restrict__ b,
+                                               const double* __restrict__ x, int xl,
+                                               double* __restrict__ r, int n) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
     if (i >= n) return;
     double s = 0.0;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) s += ax[rq[e]] * at(x, xl, rc[e]);
-    r[i] = b[i] - s;
+    for (int e = rp[i] + lane; e < rp[i + 1]; e += kWave) s += ax[rq[e]] * at(x, xl, rc[e]);
+    s = wave_sum(s);
+    if (lane == 0) r[i] = b[i] - s;
 }
 
-// r = e - J^T lam with J the unmerged CSC (one column per thread): the
+// r = e - J^T lam with J the unmerged CSC (one wavefront per column): the
 // residual of the transposed system for iterative refinement.
-__global__ void k_resid_t(const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
-                          const double* __restrict__ ax, const double* __restrict__ e,
-                          const double* __restrict__ lam, double* __restrict__ r, int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kT) void k_resid_t(const int32_t* __restrict__ ap,
+                                               const int32_t* __restrict__ ai,
+                                               const double* __restrict__ ax,
+                                               const double* __restrict__ e,
+                                               const double* __restrict__ lam,
+                                               double* __restrict__ r, int n) {
+    const int j = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
     if (j >= n) return;
     double s = 0.0;
-    for (int q = ap[j]; q < ap[j + 1]; ++q)
-        if (ai[q] < n) s += ax[q] * lam[ai[q]];
-    r[j] = e[j] - s;
+    for (int q = ap[j] + lane; q < ap[j + 1]; q += kWave) {
+        const int row = ai[q];
+        if (row < n) s += ax[q] * lam[row];
+    }
+    s = wave_sum(s);
+    if (lane == 0) r[j] = e[j] - s;
 }
 
 // lam += delta; out[0] = max |delta|, out[1] = max |lam| (one workgroup)
@@ -199,17 +213,21 @@ __global__ void k_adjoint_contract(const rs_device* __restrict__ dev, int ndev,
 // k = 0): rhs = e_probe [k = N] - sum_j (dF_j/dx_k)^T lam_j over the two
 // later steps, whose capacitor currents read v_k:
 //   dI_{k+1}/dv_k = -2C/dt - C/dt [k >= 1],   dI_{k+2}/dv_k = C/dt.
-// One thread per unknown; the capacitors of a node (cap_ptr/cap_dev, with
-// the node's sign in bit 31) are summed in a fixed order.
-__global__ void k_tadj_rhs(const rs_device* __restrict__ dev, const int32_t* __restrict__ cap_ptr,
-                           const int32_t* __restrict__ cap_dev, int node_count,
-                           const double* __restrict__ lam1, const double* __restrict__ lam2,
-                           double dt, int k_ge1, int probe, int n, double* __restrict__ rhs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per unknown, lanes over the capacitors of its node
+// (cap_ptr/cap_dev, the node's sign in bit 31) and a fixed summation tree.
+__global__ __launch_bounds__(kT) void k_tadj_rhs(const rs_device* __restrict__ dev,
+                                                const int32_t* __restrict__ cap_ptr,
+                                                const int32_t* __restrict__ cap_dev,
+                                                int node_count, const double* __restrict__ lam1,
+                                                const double* __restrict__ lam2, double dt,
+                                                int k_ge1, int probe, int n,
+                                                double* __restrict__ rhs) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
     if (i >= n) return;
     double s = 0.0;
     if (i < node_count) {
-        for (int q = cap_ptr[i]; q < cap_ptr[i + 1]; ++q) {
+        for (int q = cap_ptr[i] + lane; q < cap_ptr[i + 1]; q += kWave) {
             const int w = cap_dev[q];
             const rs_device& v = dev[w & 0x7FFFFFFF];
             const int a = v.node[0], b = v.node[1];
@@ -220,13 +238,19 @@ __global__ void k_tadj_rhs(const rs_device* __restrict__ dev, const int32_t* __r
             s += (w < 0) ? -coef : coef;
         }
     }
-    if (i == probe) s += 1.0;
-    rhs[i] = s;
+    s = wave_sum(s);
+    if (lane == 0) rhs[i] = (i == probe) ? s + 1.0 : s;
 }
 
 }  // namespace
 
 namespace rsgpu {
+
+// workgroups of kT threads holding one wavefront per item
+static unsigned wave_blocks(int items) {
+    const long long waves_per_block = kT / kWave;
+    return (unsigned)((items + waves_per_block - 1) / waves_per_block);
+}
 
 hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* vals, const double* e,
                           const int32_t* lt_ptr, const int32_t* lt_cols, int n_lt, double* T,
@@ -238,14 +262,15 @@ hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* va
 
 hipError_t launch_resid_t(const int32_t* ap, const int32_t* ai, const double* ax, const double* e,
                           const double* lam, double* r, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_resid_t, dim3((n + kT - 1) / kT), dim3(kT), 0, s, ap, ai, ax, e, lam, r, n);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resid_t, dim3(wave_blocks(n)), dim3(kT), 0, s, ap, ai, ax, e, lam, r, n);
     return hipGetLastError();
 }
 
 hipError_t launch_resid_n(const int32_t* rp, const int32_t* rc, const int32_t* rq, const double* ax,
                           const double* b, const double* x, int xl, double* r, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_resid_n, dim3((n + kT - 1) / kT), dim3(kT), 0, s, rp, rc, rq, ax, b, x, xl, r, n);
+    hipLaunchKernelGGL(k_resid_n, dim3(wave_blocks(n)), dim3(kT), 0, s, rp, rc, rq, ax, b, x, xl, r, n);
     return hipGetLastError();
 }
 
@@ -268,7 +293,7 @@ hipError_t launch_tadj_rhs(const rs_device* dev, const int32_t* cap_ptr, const i
                            int node_count, const double* lam1, const double* lam2, double dt,
                            int k_ge1, int probe, int n, double* rhs, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tadj_rhs, dim3((n + kT - 1) / kT), dim3(kT), 0, s, dev, cap_ptr, cap_dev,
+    hipLaunchKernelGGL(k_tadj_rhs, dim3(wave_blocks(n)), dim3(kT), 0, s, dev, cap_ptr, cap_dev,
                        node_count, lam1, lam2, dt, k_ge1, probe, n, rhs);
     return hipGetLastError();
 }

@@ -1233,7 +1233,8 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
     if (probe < 0 || probe >= c->node_count)
         return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: probe must be a node");
     if (nsteps < 0 || !(dt > 0.0)) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: bad dt or step count");
-    if (every < 1) every = 1;
+    const bool keep_all = every < 1;  // every state kept: no recomputation
+    if (keep_all) every = nsteps + 1;
     const i64 nd = (i64)c->dev.size(), nc = c->node_count;
     hipError_t e;
     int32_t rc;
@@ -1292,6 +1293,11 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
     }
     if ((rc = cap_update(c, x, 0.0, 0, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p))) return rc;
     if ((rc = save(0, x, {}))) return rc;
+    std::vector<std::vector<double>> xall;  // keep_all: x_0 .. x_nsteps
+    if (keep_all) {
+        xall.reserve(nsteps + 1);
+        xall.push_back(x);
+    }
     // one TRAP step to its exact root: exact-Jacobian Newton from x_{k-1};
     // should that not converge, the reference loop (with its stepping) first.
     // The recomputation runs the same function, so it reproduces x_k exactly.
@@ -1338,7 +1344,9 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         t = tt;
         tk[k] = t;
         if (k % every == 0 && k < nsteps && (rc = save(k, x, xm1))) return rc;
+        if (keep_all) xall.push_back(x);
     }
+    if (keep_all) info->checkpoints = (int32_t)xall.size();
     info->forward_iters = it;
     info->forward_ms = ms_since(t_fwd);
     if (x_final) std::memcpy(x_final, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
@@ -1388,11 +1396,12 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         const int64_t k0 = p.k, k1 = ci + 1 < (int64_t)ck.size() ? ck[ci + 1].k : nsteps;
         std::vector<std::vector<double>> xs(k1 - k0 + 1);
         xs[0] = p.x;
+        if (keep_all) xs = std::move(xall);  // one segment, every state stored
         if ((rc = restore(p))) return rc;
         std::vector<double> xr = p.x;
         int32_t rit = 0;
         const clk::time_point t_rec = clk::now();
-        for (int64_t k = k0 + 1; k <= k1; k++) {  // recompute the segment
+        for (int64_t k = k0 + 1; k <= k1 && !keep_all; k++) {  // recompute the segment
             bool okr = false;
             if ((rc = tran_step(xr, k, tk[k], &rit, &okr))) return rc;
             if (!okr) return cfail(c, RS_ERR_SOLVE_FAILED, "recomputed step did not converge");

@@ -58,6 +58,19 @@ __global__ __launch_bounds__(256) void k_mos_eval(const rs_bsim_model* __restric
     o[12] = r.gigd;
 }
 
+__device__ __forceinline__ void lane_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// RHS rows with more contributions than this are summed by the whole
+// wavefront: lanes gather 64 values at a time, lane 0 adds them in order.
+constexpr int kLongRhs = 16;
+
+// ax[q] from its stamp slot (thread q); rhs[i] = 0.0 + its contributions in
+// insertion order (thread i, or the wavefront holding row i when the row is
+// long, e.g. the ground row of a large mesh). The additions are the same in
+// both cases, so the result does not depend on which path summed a row.
 __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax_src, int nnz,
                                                   const int32_t* __restrict__ rhs_ptr,
                                                   const int32_t* __restrict__ rhs_src, int n,
@@ -65,16 +78,40 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax
                                                   double* __restrict__ ax,
                                                   double* __restrict__ rhs,
                                                   const int32_t* __restrict__ skip) {
+    __shared__ double stage[256];
     if (skip && *skip) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     if (t < nnz) {
         const int s = ax_src[t];
         ax[t] = s < 0 ? 1.0 : sv[s];
     }
+    if (t - lane >= n) return;  // wavefront-uniform: no row in this wavefront
+    int q0 = 0, q1 = 0;
     if (t < n) {
+        q0 = rhs_ptr[t];
+        q1 = rhs_ptr[t + 1];
+    }
+    const bool lng = q1 - q0 > kLongRhs;
+    if (t < n && !lng) {
         double r = 0.0;
-        for (int q = rhs_ptr[t]; q < rhs_ptr[t + 1]; ++q) r += sv[rhs_src[q]];
+        for (int q = q0; q < q1; ++q) r += sv[rhs_src[q]];
         rhs[t] = r;
+    }
+    double* S = stage + (threadIdx.x - lane);
+    for (unsigned long long m = __ballot(lng); m; m &= m - 1) {
+        const int l = __ffsll((long long)m) - 1;
+        const int b0 = __shfl(q0, l), b1 = __shfl(q1, l);
+        double acc = 0.0;
+        for (int b = b0; b < b1; b += 64) {
+            const int c = min(64, b1 - b);
+            if (lane < c) S[lane] = sv[rhs_src[b + lane]];
+            lane_sync();
+            if (lane == 0)
+                for (int i = 0; i < c; ++i) acc += S[i];
+            lane_sync();
+        }
+        if (lane == 0) rhs[t - lane + l] = acc;
     }
 }
 

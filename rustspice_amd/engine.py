@@ -272,7 +272,8 @@ class GpuCircuit:
     def tran_adjoint(self, solver: GpuSolver, dt, nsteps, probe: int, every: int = 25, opts=None):
         """rs_tran_adjoint: d x_N[probe] / dp for every device after the
         fixed-step TRAP transient of tran_fixed, by the discrete adjoint with a
-        checkpoint every `every` steps. Returns (dp, x_N, info dict)."""
+        checkpoint every `every` steps (every <= 0: every state kept, no
+        recomputation). Returns (dp, x_N, info dict)."""
         opts = opts or default_opts()
         cap = self.circuit.node_count + self.circuit.ndev + 8
         x = np.zeros(cap)

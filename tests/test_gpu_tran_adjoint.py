@@ -82,10 +82,12 @@ def test_matches_central_differences(gpu):
 def test_checkpoint_interval_does_not_change_the_gradient(gpu):
     c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
     probe = NL.rc_mesh_probe(4, 4)
-    runs = [_gpu(c, 10, probe, every) for every in (1, 3, 1000)]
+    runs = [_gpu(c, 10, probe, every) for every in (1, 3, 1000, 0)]
     for dp, x, info in runs[1:]:
         assert np.array_equal(dp, runs[0][0]) and np.array_equal(x, runs[0][1])
-    assert [r[2]["checkpoints"] for r in runs] == [10, 4, 1]
+    # every = 0 keeps all 11 states and recomputes nothing
+    assert [r[2]["checkpoints"] for r in runs] == [10, 4, 1, 11]
+    assert [r[2]["recomputed_steps"] for r in runs] == [10, 10, 10, 0]
 
 
 def test_unloaded_instance_matches_differences_of_the_gpu_forward(gpu):
