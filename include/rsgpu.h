@@ -108,11 +108,15 @@ int32_t rs_bbd_info(const rs_solver *h, int64_t *out);
  * the first solve of each pattern, keeps the fastest and remembers it per
  * pattern for the process; results are bit-identical for every shape.
  * rs_set_launch_shape fixes the shapes instead: threads a power of 2 in
- * 64..1024; ref_mode bit 0 LDS, bit 1 right-looking schedule (checked against
- * the plan by the next factor when none is built yet, which then fails with
- * RS_ERR_INVALID_MATRIX if it does not fit); sol_lds only where it fits.
+ * 64..1024; ref_mode 0/1 column kernel (1: work vectors in LDS), 2/3
+ * right-looking schedule (3: values in LDS), 4 step program; sol_mode 0 level
+ * sweeps with the solve vector in global memory, 1 in LDS, 2 step program,
+ * 3 pipelined level sweeps (the next levels' row records and first entries
+ * loaded while a level computes; solve vector in LDS). A mode is checked
+ * against the plan by the next factor when none is built yet, which then
+ * fails with RS_ERR_INVALID_MATRIX if it does not fit.
  * rs_launch_shape: out[6] = {refactor threads, refactor mode, solve threads,
- * solve lds, best refactor ns, best solve ns}; returns bit 0 refactor tuned,
+ * solve mode, best refactor ns, best solve ns}; returns bit 0 refactor tuned,
  * bit 1 solve tuned, bit 2 shapes fixed by the caller. */
 void rs_set_autotune(rs_solver *h, int32_t on);
 /* Pivot-growth check of device refactors (no reference counterpart; off by
@@ -126,7 +130,7 @@ void rs_set_growth_check(rs_solver *h, double limit);
 int64_t rs_growth_fallbacks(const rs_solver *h);
 double rs_last_growth(const rs_solver *h);
 int32_t rs_set_launch_shape(rs_solver *h, int32_t ref_threads, int32_t ref_mode, int32_t sol_threads,
-                            int32_t sol_lds);
+                            int32_t sol_mode);
 int32_t rs_launch_shape(const rs_solver *h, int64_t *out);
 /* Text of the last error (SolverError Display, solver.rs:94-117). */
 int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);

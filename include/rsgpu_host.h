@@ -70,6 +70,16 @@ int32_t rs_host_sp_selftest(int32_t kind, int64_t n, const int64_t *ap, const in
 int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t *ap, const int64_t *ai,
                             const double *ax0, const double *ax1, double *lx, double *ux, double *ud,
                             int64_t *stats);
+/* Flat pipelined solve (solve mode 3) on the host (no reference
+ * counterpart): the first factor on ax (kind bits 8..15: the ordering of
+ * rs_solver_set_config), then the level records of the
+ * solver's plan run level by level, products and rows each in reverse order.
+ * x equals NativeKluSolver factor(ax); solve(rhs) bit for bit. Returns 6 when
+ * the plan has no level records (BTF off-diagonal entries) or a record is out
+ * of range. stats[5] = {levels, most entries in a level, entries, row
+ * records, levels of the row sweeps}. */
+int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t *ap, const int64_t *ai,
+                            const double *ax, const double *rhs, double *x, int64_t *stats);
 /* Unmerged CSC pattern of one stamp pass (mode 0 DC, else TRAN) as
  * SparseBuilder::finalize produces it (mna.rs:320-347); ap/ai may be NULL to
  * query n and nnz. Returns 0, or 6 for an unsupported device kind. */

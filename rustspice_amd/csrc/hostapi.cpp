@@ -86,7 +86,7 @@ rs_host_factor* rs_host_factor_create(int32_t kind, int64_t n, const int64_t* ap
     } else {
         f.sym.resize(1);
         f.num.resize(1);
-        block_symbolic(n, ap, ai, natural, f.sym[0]);
+        block_symbolic(n, ap, ai, natural, f.sym[0], ordering == 4);
         f.sparselu = sparselu;
         factor_block_host(f.sym[0], ap, ai, ax, nullptr, false, tol, f.num[0], f.sparselu);
     }
@@ -285,6 +285,63 @@ int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
             stats[5] = p.nl;
             stats[6] = p.nu;
             stats[7] = r.nv;
+        }
+        delete h;
+        return RS_OK;
+    } catch (const std::exception&) {
+        delete h;
+        return RS_ERR_INVALID_MATRIX;
+    }
+}
+
+int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t* ap, const int64_t* ai, const double* ax,
+                            const double* rhs, double* x, int64_t* stats) {
+    rs_host_factor* h = nullptr;
+    try {
+        // kind bits 8..15: the ordering (rs_solver_set_config)
+        h = rs_host_factor_create(kind & 0xFF, n, ap, ai, ax, 0.001, (kind >> 8) & 0xFF, 1);
+        const PlanHost& p = h->p;
+        std::vector<int32_t> pool;
+        PlanDev d{};
+        append_level_records(p, pool, d);
+        if (d.sq_nlev == 0) {
+            delete h;
+            return RS_ERR_INVALID_MATRIX;
+        }
+        // value layout of the solver: lx | pad | ux | pad, offsets from lx
+        std::vector<double> lx, ux, ud;
+        gather_values(h->f, lx, ux, ud);
+        std::vector<double> vals((size_t)p.nl + 1 + p.nu + 1, 0.0);
+        std::copy(lx.begin(), lx.end(), vals.begin());
+        std::copy(ux.begin(), ux.end(), vals.begin() + p.nl + 1);
+        auto at = [&](int32_t off, size_t i) { return pool.at((size_t)off + i); };
+        std::vector<double> T(n), P((size_t)d.sq_maxent);
+        for (int64_t i = 0; i < n; i++) T.at(p.perm_in[i]) = rhs[i];
+        int64_t slots = 0;
+        for (int32_t l = 0; l < d.sq_nlev; l++) {
+            const int32_t s0 = at(d.o_sq_lev, 4 * l), k0 = at(d.o_sq_lev, 4 * l + 1), div = at(d.o_sq_lev, 4 * l + 2);
+            const int32_t s1 = at(d.o_sq_lev, 4 * l + 4), k1 = at(d.o_sq_lev, 4 * l + 5);
+            if (s1 - s0 > d.sq_maxent || k1 <= k0) throw std::runtime_error("bad level");
+            // the products in reverse order, then the rows in reverse order:
+            // neither phase may depend on the order inside a level
+            for (int32_t sl = s1 - 1; sl >= s0; sl--)
+                P.at(sl - s0) = vals.at(at(d.o_sq_src, sl)) * T.at(at(d.o_sq_col, sl));
+            for (int32_t k = k1 - 1; k >= k0; k--) {
+                const int32_t r = at(d.o_sq_row, 4 * k), sb = at(d.o_sq_row, 4 * k + 1), se = at(d.o_sq_row, 4 * k + 2);
+                if (sb < s0 || se > s1 || sb > se) throw std::runtime_error("bad row record");
+                double t = T.at(r);
+                for (int32_t sl = sb; sl < se; sl++) t = t - P.at(sl - s0);
+                T.at(r) = div ? t / ud.at(r) : t;
+            }
+            slots += s1 - s0;
+        }
+        for (int64_t i = 0; i < n; i++) x[i] = T.at(p.perm_out[i]);
+        if (stats) {
+            stats[0] = d.sq_nlev;
+            stats[1] = d.sq_maxent;
+            stats[2] = slots;
+            stats[3] = at(d.o_sq_lev, 4 * d.sq_nlev + 1);  // row records
+            stats[4] = p.n_lsl + p.n_usl;                   // levels of the row sweeps
         }
         delete h;
         return RS_OK;

@@ -50,6 +50,43 @@ __device__ __forceinline__ void csr_copies(const PlanDev& d, const int32_t* __re
     for (int e = threadIdx.x; e < nur; e += blockDim.x) uxr[e] = ux[urv[e]];
 }
 
+// t - P[0] - P[1] - ... - P[c-1], subtracted in this order; the reads of P
+// (LDS) are issued sixteen at a time, so only the subtractions are serial.
+__device__ __forceinline__ double ordered_sub(double t, const double* P, int c) {
+    int i = 0;
+    for (; i + 16 <= c; i += 16) {
+        double p[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) p[j] = P[i + j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) t = t - p[j];
+    }
+    if (i + 8 <= c) {
+        double p[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p[j] = P[i + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t = t - p[j];
+        i += 8;
+    }
+    if (i + 4 <= c) {
+        double p[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[j] = P[i + j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t = t - p[j];
+        i += 4;
+    }
+    if (i + 2 <= c) {
+        const double p0 = P[i], p1 = P[i + 1];
+        t = t - p0;
+        t = t - p1;
+        i += 2;
+    }
+    if (i < c) t = t - P[i];
+    return t;
+}
+
 __device__ __forceinline__ double clamp_pivot(double diag, bool singleton) {
     if (singleton) return fabs(diag) < 1e-300 ? 1e-14 : diag;
     if (fabs(diag) < 1e-14) return diag >= 0.0 ? 1e-14 : -1e-14;
@@ -212,14 +249,10 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
                         act = true;
                     }
                 }
-                const unsigned long long m = __ballot(act);
-                S[lane] = pr;
+                // a skipped update stages +0.0, and v - (+0.0) is v bit for bit
+                S[lane] = act ? pr : 0.0;
                 wave_sync();
-                if (lane == 0) {
-                    const int c = min(kWave, e - base);
-                    for (int i = 0; i < c; ++i)
-                        if ((m >> i) & 1ull) v = v - S[i];
-                }
+                if (lane == 0) v = ordered_sub(v, S, min(kWave, e - base));
                 wave_sync();
             }
             if (lane == 0) V[tgt] = v;
@@ -288,10 +321,7 @@ __device__ __forceinline__ void sweep_level(double* T, double* S, const int32_t*
             if (e < e1) pr = vals[kCsr ? e : rv[e]] * T[rc[e]];
             S[lane] = pr;
             wave_sync();
-            if (lane == 0) {
-                const int c = min(kWave, e1 - base);
-                for (int i = 0; i < c; ++i) t = t - S[i];
-            }
+            if (lane == 0) t = ordered_sub(t, S, min(kWave, e1 - base));
             wave_sync();
         }
         if (lane == 0) T[r] = kDiv ? t / ud[r] : t;
@@ -389,6 +419,136 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T[perm_out[i]];
 }
 
+// ---- flat pipelined level solve (solve mode 3) ----
+// Each level in two phases: (a) every entry's product v * T[col] into the
+// LDS buffer P, one slot per thread; (b) one thread per row subtracts its
+// products from T[row] in entry order (then divides by the U diagonal): the
+// same operations in the same order as the row sweeps, so the result is
+// bit-identical. The static slot and row data of level l + 2 and the values
+// and diagonals of level l + 1 are loaded while level l computes; loads are
+// unconditional from in-bounds indices (slot 0, record 0 when there is
+// nothing to load) and loaded registers are never rewritten, so a level
+// waits for loads issued a level earlier only.
+constexpr int kFlatSlots = 2;  // slots per thread loaded ahead
+
+struct FlStatic {
+    int col[kFlatSlots], src[kFlatSlots];
+    int4 row;
+};
+struct FlDyn {
+    double v[kFlatSlots];
+    double dg;
+};
+
+// Barrier for data exchanged through LDS only: global loads issued before it
+// stay in flight.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+__device__ __forceinline__ FlStatic fl_static(const int4* LV, int l, int nlev, int tid, int bs,
+                                              const int32_t* __restrict__ col, const int32_t* __restrict__ src,
+                                              const int4* __restrict__ rows) {
+    FlStatic f;
+    const int lc = l < nlev ? l : nlev - 1;
+    const int4 a = LV[lc], b = LV[lc + 1];
+#pragma unroll
+    for (int i = 0; i < kFlatSlots; ++i) {
+        const int sl = a.x + tid + i * bs;
+        const int q = l < nlev && sl < b.x ? sl : 0;
+        f.col[i] = col[q];
+        f.src[i] = src[q];
+    }
+    const int k = a.y + tid;
+    f.row = rows[l < nlev && k < b.y ? k : 0];
+    return f;
+}
+
+__device__ __forceinline__ FlDyn fl_dyn(const FlStatic& f, const double* __restrict__ vals,
+                                        const double* __restrict__ ud) {
+    FlDyn g;
+#pragma unroll
+    for (int i = 0; i < kFlatSlots; ++i) g.v[i] = vals[f.src[i]];
+    g.dg = ud[f.row.x];
+    return g;
+}
+
+__device__ __forceinline__ void fl_level(double* T, double* P, const int4* LV, int l, const FlStatic& f,
+                                         const FlDyn& g, int tid, int bs, const int32_t* __restrict__ col,
+                                         const int32_t* __restrict__ src, const int4* __restrict__ rows,
+                                         const double* __restrict__ vals, const double* __restrict__ ud) {
+    const int4 a = LV[l], b = LV[l + 1];
+    const int ns = b.x - a.x;
+#pragma unroll
+    for (int i = 0; i < kFlatSlots; ++i)
+        if (tid + i * bs < ns) P[tid + i * bs] = g.v[i] * T[f.col[i]];
+    for (int t = tid + kFlatSlots * bs; t < ns; t += bs) P[t] = vals[src[a.x + t]] * T[col[a.x + t]];
+    lds_barrier();
+    const int div = a.z;
+    if (a.y + tid < b.y) {  // the thread's first row: record and diagonal loaded ahead
+        double t = T[f.row.x];
+        t = ordered_sub(t, P + (f.row.y - a.x), f.row.z - f.row.y);
+        T[f.row.x] = div ? t / g.dg : t;
+    }
+    for (int k = a.y + tid + bs; k < b.y; k += bs) {
+        const int4 rr = rows[k];
+        double t = T[rr.x];
+        t = ordered_sub(t, P + (rr.y - a.x), rr.z - rr.y);
+        T[rr.x] = div ? t / ud[rr.x] : t;
+    }
+    lds_barrier();
+}
+
+// Shared memory: T (n doubles, rounded up to an even count), P (sq_maxent
+// doubles, rounded up to even), the level table (sq_nlev + 1 int4).
+__global__ __launch_bounds__(1024) void k_solve_lp(const PlanDev* __restrict__ descs,
+                                                  const int32_t* __restrict__ ip,
+                                                  const double* __restrict__ vpool,
+                                                  const double* __restrict__ rhspool,
+                                                  double* __restrict__ xpool,
+                                                  const int32_t* __restrict__ skip) {
+    extern __shared__ double smem[];
+    if (skip && *skip) return;
+    const PlanDev d = descs[blockIdx.x];
+    const int n = d.n, nlev = d.sq_nlev;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    double* T = smem;
+    double* P = T + ((n + 1) & ~1);
+    int4* LV = reinterpret_cast<int4*>(P + ((d.sq_maxent + 1) & ~1));
+    const int4* glev = reinterpret_cast<const int4*>(ip + d.o_sq_lev);
+    const int4* rows = reinterpret_cast<const int4*>(ip + d.o_sq_row);
+    const int32_t* col = ip + d.o_sq_col;
+    const int32_t* src = ip + d.o_sq_src;
+    const int32_t* perm_in = ip + d.o_perm_in;
+    const int32_t* perm_out = ip + d.o_perm_out;
+    const double* vals = vpool + d.o_lx;
+    const double* ud = vpool + d.o_ud;
+    const double* rhs = rhspool + d.o_vec;
+    double* x = xpool + d.o_vec;
+    for (int i = tid; i < n; i += bs) T[perm_in[i]] = rhs[i];
+    for (int l = tid; l <= nlev; l += bs) LV[l] = glev[l];
+    __syncthreads();
+    // static data rotates over three register sets, values over two
+    FlStatic A0 = fl_static(LV, 0, nlev, tid, bs, col, src, rows), A1 = fl_static(LV, 1, nlev, tid, bs, col, src, rows),
+             A2;
+    FlDyn G0 = fl_dyn(A0, vals, ud), G1;
+#define FL_STEP(L, AC, AN, AF, GC, GN)                                         \
+    if ((L) < nlev) {                                                         \
+        GN = fl_dyn(AN, vals, ud);                                            \
+        AF = fl_static(LV, (L) + 2, nlev, tid, bs, col, src, rows);           \
+        fl_level(T, P, LV, (L), AC, GC, tid, bs, col, src, rows, vals, ud);   \
+    }
+    for (int l = 0; l < nlev; l += 6) {
+        FL_STEP(l, A0, A1, A2, G0, G1)
+        FL_STEP(l + 1, A1, A2, A0, G1, G0)
+        FL_STEP(l + 2, A2, A0, A1, G0, G1)
+        FL_STEP(l + 3, A0, A1, A2, G1, G0)
+        FL_STEP(l + 4, A1, A2, A0, G0, G1)
+        FL_STEP(l + 5, A2, A0, A1, G1, G0)
+    }
+#undef FL_STEP
+    __syncthreads();
+    for (int i = tid; i < n; i += bs) x[i] = T[perm_out[i]];
+}
+
 // out[0] = max |ax|, out[1] = max |ux| (+inf when some ux is not finite):
 // the element growth of a refactor on reused pivots. One workgroup.
 __global__ __launch_bounds__(256) void k_growth(const double* __restrict__ ax, int nnz,
@@ -451,6 +611,19 @@ hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, co
     else
         hipLaunchKernelGGL(k_refactor<false>, dim3(nmat), dim3(threads), 0, stream, descs, ip, ax,
                            vals, wpool, skip);
+    return hipGetLastError();
+}
+
+// Solve mode 3 (k_solve_lp) of one matrix whose plan has level records.
+size_t solve_lp_lds(int n, int maxent, int nlev) {
+    return (size_t)((n + 1) & ~1) * sizeof(double) + (size_t)((maxent + 1) & ~1) * sizeof(double) +
+           (size_t)(nlev + 1) * 4 * sizeof(int32_t);
+}
+hipError_t launch_solve_lp(const PlanDev* descs, const int32_t* ip, const double* vals, const double* rhs,
+                           double* x, int n, int maxent, int nlev, int threads, hipStream_t stream,
+                           const int32_t* skip) {
+    hipLaunchKernelGGL(k_solve_lp, dim3(1), dim3(threads), solve_lp_lds(n, maxent, nlev), stream, descs, ip,
+                       vals, rhs, x, skip);
     return hipGetLastError();
 }
 

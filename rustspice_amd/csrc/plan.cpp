@@ -348,6 +348,64 @@ void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
     d.o_usl_nlong = put(p.usl_nlong);
 }
 
+// Every solve level in execution order (blocks from the last; per block its
+// L levels, then its U levels) as int4 {s0, k0, div, 0} plus a sentinel: the
+// level's entries are the slots [s0, s0') and its rows the records [k0, k0').
+// Each row record is int4 {row, slot begin, slot end, 0}; a row's slots hold
+// its entries in the row-ordered sweep's order (lrp/urp), each with its
+// column (sq_col) and its value's offset from o_lx (sq_src: L position q, or
+// nl + 1 + U position q). L rows without entries leave T unchanged and get
+// no record; L levels left empty are dropped.
+void append_level_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
+    d.sq_nlev = d.sq_maxent = 0;
+    d.o_sq_lev = d.o_sq_row = d.o_sq_col = d.o_sq_src = 0;
+    for (int32_t v : p.blk_has_od)
+        if (v) return;
+    std::vector<int32_t> lev, row, col, src;
+    int32_t maxent = 0;
+    auto add = [&](const std::vector<int32_t>& ptr, const std::vector<int32_t>& rows,
+                   const std::vector<int32_t>& rp, const std::vector<int32_t>& rc,
+                   const std::vector<int32_t>& rv, int32_t l, int32_t div) {
+        const int32_t s0 = (int32_t)col.size(), k0 = (int32_t)(row.size() / 4);
+        for (int32_t q = ptr[l]; q < ptr[l + 1]; q++) {
+            const int32_t r = rows[q];
+            if (!div && rp[r] == rp[r + 1]) continue;
+            const int32_t sb = (int32_t)col.size();
+            for (int32_t e = rp[r]; e < rp[r + 1]; e++) {
+                col.push_back(rc[e]);
+                src.push_back(div ? p.nl + 1 + rv[e] : rv[e]);
+            }
+            row.insert(row.end(), {r, sb, (int32_t)col.size(), 0});
+        }
+        if ((int32_t)(row.size() / 4) == k0) return;
+        lev.insert(lev.end(), {s0, k0, div, 0});
+        maxent = std::max(maxent, (int32_t)col.size() - s0);
+    };
+    for (int32_t b = p.nblk - 1; b >= 0; b--) {
+        for (int32_t l = p.lsl_blk[b]; l < p.lsl_blk[b + 1]; l++)
+            add(p.lsl_ptr, p.lsl_rows, p.lrp, p.lrc, p.lrv, l, 0);
+        for (int32_t l = p.usl_blk[b]; l < p.usl_blk[b + 1]; l++)
+            add(p.usl_ptr, p.usl_rows, p.urp, p.urc, p.urv, l, 1);
+    }
+    if (lev.empty()) return;
+    const int32_t nlev = (int32_t)(lev.size() / 4);
+    lev.insert(lev.end(), {(int32_t)col.size(), (int32_t)(row.size() / 4), 0, 0});
+    auto put16 = [&](const std::vector<int32_t>& v) -> int32_t {
+        while (pool.size() & 3) pool.push_back(0);
+        const size_t off = pool.size();
+        if (off + v.size() + 4 > (size_t)INT32_MAX) throw std::runtime_error("plan pool too large");
+        pool.insert(pool.end(), v.begin(), v.end());
+        pool.insert(pool.end(), {0, 0, 0, 0});  // index 0 stays valid for empty arrays
+        return (int32_t)off;
+    };
+    d.o_sq_lev = put16(lev);
+    d.o_sq_row = put16(row);
+    d.o_sq_col = put16(col);
+    d.o_sq_src = put16(src);
+    d.sq_nlev = nlev;
+    d.sq_maxent = maxent;
+}
+
 // Pattern-level analysis: symbolic factor with identity pivots so that the
 // plan's scatter targets, solve permutation and off-diagonal row keys are the
 // pre-pivot positions the kernel maps through its own RPI.

@@ -26,6 +26,11 @@ struct PlanDev {
     int32_t o_blk_ptr, o_lsl_blk, o_lsl_ptr, o_lsl_rows, o_usl_blk, o_usl_ptr, o_usl_rows;
     int32_t o_odp, o_odax, o_odsrc, o_blk_has_od;
     int32_t o_lsl_nlong, o_usl_nlong;  // per solve level: its first rows are long (a wavefront each)
+    // flat pipelined solve (sq_nlev == 0: none): every solve level in
+    // execution order as int4 {s0, k0, div, 0} (nlev + 1 of them, the last a
+    // sentinel) over entry slots s (column sq_col, value offset sq_src from
+    // o_lx) and row records k (int4 {row, slot begin, slot end, 0})
+    int32_t sq_nlev, sq_maxent, o_sq_lev, o_sq_row, o_sq_col, o_sq_src;
     // right-looking refactor schedule (rl_nph == 0: none); value indices are
     // relative to o_lx with lx | pad | ux | pad | ud contiguous
     int32_t rl_nph, rl_nv;
@@ -73,6 +78,10 @@ struct PlanHost {
 };
 
 void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p);
+
+// Level records of the pipelined solve (PlanDev::sq_*), appended to pool at
+// 16-byte aligned offsets; none when a block has BTF off-diagonal entries.
+void append_level_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
 
 // Gather L/U values of all blocks into position-space arrays (lx[nl], ux[nu],
 // ud[n]) matching PlanHost's lri/uri order.

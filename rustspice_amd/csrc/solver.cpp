@@ -173,13 +173,17 @@ int default_refactor_mode(const rs_solver* h, int t) {
         if (refactor_mode_fits(h, t, m)) return m;
     return 0;
 }
-// Solve mode m: 0 solve vector in global memory, 1 in LDS, 2 step program.
+// Solve mode m: 0 solve vector in global memory, 1 in LDS, 2 step program,
+// 3 pipelined level sweeps over the level records (solve vector in LDS).
 bool solve_mode_fits(const rs_solver* h, int t, int m) {
     if (m == 2) return h->sp_ok && t <= 512;  // k_sp_solve's launch bound
+    if (m == 3)  // value offsets assume ux right after lx and its pad
+        return h->pd.sq_nlev > 0 && h->pd.o_ux == h->pd.o_lx + h->pd.nl + 1 &&
+               solve_lp_lds((int)h->n, h->pd.sq_maxent, h->pd.sq_nlev) <= device_max_lds();
     return m == 0 || ((size_t)h->n + t) * sizeof(double) <= device_max_lds();
 }
 int default_solve_mode(const rs_solver* h, int t) {
-    for (int m : {2, 1, 0})
+    for (int m : {2, 3, 1, 0})
         if (solve_mode_fits(h, t, m)) return m;
     return 0;
 }
@@ -259,6 +263,7 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
     std::vector<int32_t> pool;
     PlanDev d{};
     append_plan(h->ph, pool, d);
+    append_level_records(h->ph, pool, d);
     const i64 n = h->n;
     d.o_lx = 0;
     d.o_ux = h->ph.nl + 1;
@@ -271,10 +276,10 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
     d.o_lxr = d.o_ud + n + 1;
     d.o_uxr = d.o_lxr + h->ph.nl + 1;
     const int64_t nvals = d.o_uxr + (h->ph.nu - n) + 1;
+    const uint64_t key = plan_key(h->ph);
     {
         // the right-looking schedule and the step programs, rebuilt only when
         // the plan changes (a DC OP re-analyzes the same pattern after prepare())
-        const uint64_t key = plan_key(h->ph);
         if (key != h->rl_key || h->rl.nv == 0) {
             h->rl_key = key;
             h->rl_ok = h->ph.refactor_flops <= 2.0 * kRlMaxUpdates && build_rl_plan(h->ph, h->rl);
@@ -297,19 +302,21 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
         h->sol_mode = default_solve_mode(h, kThreads);
     }
     h->tuned_ref = h->tuned_sol = false;
-    h->tune_key = pattern_key(h);
+    // tuned shapes are kept per plan (the pattern alone does not fix the
+    // pivots, the ordering or BTF), and used only where they fit it
+    h->tune_key = pattern_key(h) ^ (key * 0x9E3779B97F4A7C15ull);
     if (!h->forced) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(h->tune_key);
         if (it != g_tuned.end()) {
             const TunedShapes& t = it->second;
-            if (t.have_ref) {
+            if (t.have_ref && refactor_mode_fits(h, t.ref_threads, t.ref_mode)) {
                 h->ref_threads = t.ref_threads;
                 h->ref_mode = t.ref_mode;
                 h->tune_ref_us = t.ref_us;
                 h->tuned_ref = true;
             }
-            if (t.have_sol) {
+            if (t.have_sol && solve_mode_fits(h, t.sol_threads, t.sol_mode)) {
                 h->sol_threads = t.sol_threads;
                 h->sol_mode = t.sol_mode;
                 h->tune_sol_us = t.sol_us;
@@ -450,13 +457,15 @@ std::vector<Shape> refactor_shapes(const rs_solver* h) {
 std::vector<Shape> solve_shapes(const rs_solver* h) {
     std::vector<Shape> v;
     for (int t = kMaxThreads; t >= 64; t /= 2)
-        for (int m : {2, 1, 0})
+        for (int m : {2, 3, 1, 0})
             if (solve_mode_fits(h, t, m)) v.push_back({t, m});
     return v;
 }
 
 hipError_t refactor_launch(rs_solver* h, const double* ax_dev, int mode, int threads, hipStream_t s,
                            const int32_t* skip = nullptr) {
+    if ((mode == 4 && !h->sp_ok) || ((mode & 2) && mode != 4 && h->pd.rl_nph == 0 && h->n > 0))
+        return hipErrorInvalidValue;
     if (mode == 4)
         return launch_sp_refactor(h->d_spd.p, h->d_desc.p, h->d_ip.p, h->d_spi.p, h->d_spw.p, ax_dev,
                                   h->d_vals.p, h->spd.nslots, threads, s, skip);
@@ -466,9 +475,14 @@ hipError_t refactor_launch(rs_solver* h, const double* ax_dev, int mode, int thr
 
 hipError_t solve_launch(rs_solver* h, const double* rhs_dev, double* x_dev, int mode, int threads,
                         hipStream_t s, const int32_t* skip = nullptr) {
+    // a mode without its plan data (step programs, level records) fails loudly
+    if ((mode == 2 && !h->sp_ok) || (mode == 3 && h->pd.sq_nlev == 0)) return hipErrorInvalidValue;
     if (mode == 2)
         return launch_sp_solve(h->d_spd.p, h->d_desc.p, h->d_ip.p, h->d_spi.p, h->d_spw.p, h->d_ax.p,
                                h->d_vals.p, rhs_dev, x_dev, h->spd.nslots, threads, s, skip);
+    if (mode == 3)
+        return launch_solve_lp(h->d_desc.p, h->d_ip.p, h->d_vals.p, rhs_dev, x_dev, (int)h->n, h->pd.sq_maxent,
+                               h->pd.sq_nlev, threads, s, skip);
     return launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev, h->d_work.p,
                         (int)h->n, mode == 1, threads, s, skip);
 }
@@ -841,15 +855,15 @@ int32_t rs_set_launch_shape(rs_solver* h, int32_t ref_threads, int32_t ref_mode,
                             int32_t sol_mode) {
     auto ok = [](int t) { return t >= 64 && t <= kMaxThreads && (t & (t - 1)) == 0; };
     if (!ok(ref_threads) || !ok(sol_threads) || ref_mode < 0 || ref_mode > 4 || sol_mode < 0 ||
-        sol_mode > 2)
+        sol_mode > 3)
         return fail(h, RS_ERR_INVALID_MATRIX,
                     "Invalid matrix: launch threads must be 64..1024, a power of 2; refactor mode 0..4, "
-                    "solve mode 0..2");
-    // the right-looking schedule and the step programs are known once a
-    // factor has built the plan; until then their fit is checked by the next
-    // factor
+                    "solve mode 0..3");
+    // the right-looking schedule, the step programs and the level records
+    // are known once a factor has built the plan; until then their fit is
+    // checked by the next factor
     const bool kref = h->plan_on_device || !(ref_mode & 6);
-    const bool ksol = h->plan_on_device || sol_mode != 2;
+    const bool ksol = h->plan_on_device || sol_mode < 2;
     if (h->n > 0 && ((kref && !refactor_mode_fits(h, ref_threads, ref_mode)) ||
                      (ksol && !solve_mode_fits(h, sol_threads, sol_mode))))
         return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: launch shape does not fit this plan or the LDS");

@@ -19,6 +19,11 @@ hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
                         bool lds, int threads, hipStream_t stream, const int32_t* skip = nullptr);
 size_t device_max_lds();
+// solve mode 3: flat pipelined level sweeps of one matrix (PlanDev::sq_nlev > 0)
+size_t solve_lp_lds(int n, int maxent, int nlev);
+hipError_t launch_solve_lp(const PlanDev* descs, const int32_t* ip, const double* vals, const double* rhs,
+                           double* x, int n, int maxent, int nlev, int threads, hipStream_t stream,
+                           const int32_t* skip = nullptr);
 hipError_t launch_sp_refactor(const SpSolverDev* sd, const PlanDev* pd, const int32_t* ip,
                               const int32_t* sip, const SpOpWord* words, const double* ax,
                               double* vals, int nslots, int threads, hipStream_t s,
@@ -93,7 +98,7 @@ struct rs_solver {
     std::string err;
     size_t plan_bytes = 0;
     int ref_mode = 0;  // bit 0 LDS, bit 1 right-looking schedule, 4 step program
-    int sol_mode = 0;  // 0 solve vector in global memory, 1 in LDS, 2 step program
+    int sol_mode = 0;  // 0 solve vector in global memory, 1 in LDS, 2 step program, 3 pipelined levels
     // launch shapes of k_refactor / k_solve: chosen by timing every candidate
     // on the first device refactor and solve of a pattern (autotune), or set
     int ref_threads = rsgpu::kThreads, sol_threads = rsgpu::kThreads;

@@ -107,8 +107,17 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax
             const int c = min(64, b1 - b);
             if (lane < c) S[lane] = sv[rhs_src[b + lane]];
             lane_sync();
-            if (lane == 0)
-                for (int i = 0; i < c; ++i) acc += S[i];
+            if (lane == 0) {  // reads issued eight at a time, the additions in order
+                int i = 0;
+                for (; i + 8 <= c; i += 8) {
+                    double p[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) p[j] = S[i + j];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc += p[j];
+                }
+                for (; i < c; ++i) acc += S[i];
+            }
             lane_sync();
         }
         if (lane == 0) rhs[t - lane + l] = acc;

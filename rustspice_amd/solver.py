@@ -125,7 +125,7 @@ class GpuSolver:
     def set_launch_shape(self, ref_threads, ref_mode, sol_threads, sol_mode):
         """ref_mode: 0/1 column kernel (1: LDS work vectors), 2/3 right-looking
         schedule (3: values in LDS), 4 step program; sol_mode: 0 solve vector
-        in global memory, 1 in LDS, 2 step program."""
+        in global memory, 1 in LDS, 2 step program, 3 pipelined level sweeps."""
         self._check(self._L.rs_set_launch_shape(self.h, ref_threads, int(ref_mode), sol_threads,
                                                 int(sol_mode)))
 
@@ -257,6 +257,21 @@ def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU, own_plan=False):
         raise SolverError(rc, "step program")
     keys = ["refactor_steps", "refactor_ops", "solve_steps", "solve_ops", "refactor_slow_chunks",
             "solve_slow_chunks", "refactor_forwarded", "solve_forwarded"]
+    return x, dict(zip(keys, map(int, st)))
+
+
+def host_lp_selftest(n, ap, ai, ax, rhs, kind=NATIVEKLU, ordering=0):
+    """Host model of the flat pipelined solve (solve mode 3): first factor on
+    ax (with the given ordering), then the level records run level by level.
+    Returns (x, stats), or raises SolverError when the plan has no level
+    records."""
+    L = _lib.lib()
+    x = np.zeros(n)
+    st = np.zeros(5, np.int64)
+    rc = L.rs_host_lp_selftest(kind | (ordering << 8), n, _i64(ap), _i64(ai), _f64(ax), _f64(rhs), x, st)
+    if rc:
+        raise SolverError(rc, "level records")
+    keys = ["levels", "max_level_entries", "entries", "rows", "sweep_levels"]
     return x, dict(zip(keys, map(int, st)))
 
 

@@ -231,6 +231,62 @@ def test_every_launch_shape_is_bit_identical(gpu, case):
     assert tried >= 6
 
 
+FLAT_CASES = {
+    "mesh64": lambda: NL.resistor_mesh(64, 64),
+    "mesh16": lambda: NL.resistor_mesh(16, 16),
+    "rcmesh8": lambda: NL.rc_mesh_inverters(8, 8, 4, load=1e4),
+    "ladder": lambda: NL.rc_ladder(30),
+}
+
+
+@pytest.mark.parametrize("case", sorted(FLAT_CASES))
+def test_flat_pipelined_solve_is_bit_identical(gpu, case):
+    """Solve mode 3 (k_solve_lp: per level, entry products in parallel, then
+    one thread per row subtracting them in entry order, the next levels'
+    records and values loaded ahead) gives the reference's solution bit for
+    bit for 1 to 16 wavefronts, for two right-hand sides after a refactor."""
+    c = FLAT_CASES[case]()
+    rs = np.random.RandomState(7)
+    ap, ai, ax, rhs, n = O.build_mna(c, x=rs.rand(c.node_count))
+    ax2 = ax * (1.0 + 0.25 * rs.rand(len(ax)))
+    rhs2 = rs.randn(n)
+    o = O.NativeKlu(n)
+    o.factor(ap, ai, ax)
+    o.factor(ap, ai, ax2)
+    xo, xo2 = rhs.copy(), rhs2.copy()
+    o.solve(xo)
+    o.solve(xo2)
+    for t in (64, 128, 256, 512, 1024):
+        print(f"flat solve {case} n={n} threads={t}", flush=True)
+        g = S.GpuSolver(n)
+        g.prepare(n)
+        g.analyze(ap, ai)
+        g.set_launch_shape(t, 0, t, 3)
+        _, xg = _refactor_solve(g, ap, ai, ax, ax2, rhs)
+        assert same_bits(xg, xo), t
+        x2 = rhs2.copy()
+        g.solve(x2)
+        assert same_bits(x2, xo2), t
+        sh = g.launch_shape()
+        assert sh["sol_mode"] == 3 and sh["sol_threads"] == t
+        g.close()
+
+
+def test_flat_solve_refused_with_btf_couplings(gpu):
+    """The chain's BTF plan couples blocks through off-diagonal entries, for
+    which there are no level records: solve mode 3 is refused."""
+    c = NL.inverter_chain(50)
+    ap, ai, ax, rhs, n = O.build_mna(c, x=np.zeros(c.node_count))
+    g = S.GpuSolver(n)
+    g.prepare(n)
+    g.analyze(ap, ai)
+    g.set_launch_shape(256, 0, 256, 3)
+    with pytest.raises(S.SolverError) as e:
+        _refactor_solve(g, ap, ai, ax, ax, rhs)
+    assert e.value.variant == "InvalidMatrix"
+    g.close()
+
+
 def test_autotune_picks_a_shape_and_keeps_parity(gpu):
     c = NL.resistor_mesh()
     ap, ai, ax, rhs, n = O.build_mna(c)
