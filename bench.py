@@ -201,6 +201,85 @@ def mesh_refactor_solve(E, NL, S, torch, np, reps=10):
     return res, (ap, ai, ax, rhs, n, d_x.cpu().numpy())
 
 
+# Per device kind: terminal voltages read, stamp values written, FP64 flops
+# of one evaluation (R: g = 1/R; C, TRAP: g = 2C/dt and the history current;
+# level-1 MOSFET: region, current and three conductances; diode: exp and g).
+_EVAL_COST = {1: (2, 4, 1), 2: (2, 6, 6), 3: (2, 5, 0), 4: (2, 2, 0), 5: (2, 6, 10), 6: (4, 10, 25)}
+
+
+def kernel_rooflines(E, NL, S, torch, np, reps=10, nsteps=20):
+    """Achieved rate of each single-matrix kernel against its roofline, HIP
+    events around every launch (rs_profile): k_refactor* and k_solve* on the
+    64x64 mesh DC matrix (B_f = 12 nnz(A) + 12 (nnz L + nnz U) + 8 n,
+    B_s = 12 (nnz L + nnz U) + 24 n bytes per launch); k_eval and k_assemble
+    in a host-driven fixed-step TRAP transient of the 32x32 RC mesh with 32
+    inverter taps (eval: device records, terminal voltages and capacitor
+    history read, stamp values written, and its FP64 flops; assemble: 20 B per
+    matrix and RHS contribution (slot index, value read and written) + 8 n)."""
+    from rustspice_amd import _lib
+
+    out = {}
+    c = NL.resistor_mesh(64, 64)
+    g = E.GpuCircuit(c)
+    ap, ai, ax, rhs, n = g.stamp(np.zeros(c.node_count))
+    s = S.GpuSolver(n, S.NATIVEKLU)
+    s.factor(ap, ai, ax)
+    d_ax = torch.tensor(ax, dtype=torch.float64, device="cuda")
+    d_b = torch.tensor(rhs, dtype=torch.float64, device="cuda")
+    d_x = torch.empty_like(d_b)
+    s.refactor_device(d_ax)
+    s.solve_device(d_b, d_x)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    for _ in range(reps):
+        s.refactor_device(d_ax)
+        s.solve_device(d_b, d_x)
+    torch.cuda.synchronize()
+    prof = _lib.profile_get()
+    _lib.profile_enable(False)
+    ps = S.HostFactor(n, ap, ai, ax).plan_stats()
+    b_ref, b_sol = _algorithmic_bytes(len(ai), ps["nnz_l"], ps["nnz_u"], n)
+    f_sol = 2 * (ps["nnz_l"] + ps["nnz_u"] - n) + n
+    sh = s.launch_shape()
+    for name, b, mode in (("refactor", b_ref, sh["ref_mode"]), ("solve", b_sol, sh["sol_mode"])):
+        p = prof[name]
+        gbs = b / (p["avg_us"] * 1e-6) / 1e9
+        out[f"{name}_mesh64"] = dict(avg_us=p["avg_us"], count=p["count"], bytes=b, gbs=gbs,
+                                     hbm_frac=gbs / HBM_PEAK_GBS, mode=mode, threads=sh[f"{name[:3]}_threads"])
+    out["solve_mesh64"]["gflops"] = f_sol / (prof["solve"]["avg_us"] * 1e-6) / 1e9
+    s.close()
+
+    c = NL.rc_mesh_inverters(32, 32)
+    g = E.GpuCircuit(c)
+    s = S.GpuSolver(c.node_count)
+    g.tran_fixed(s, NL.C5_TRAN["dt"], 2)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    g.tran_fixed(s, NL.C5_TRAN["dt"], nsteps)
+    torch.cuda.synchronize()
+    prof = _lib.profile_get()
+    _lib.profile_enable(False)
+    kinds = [int(k) for k in c.devices["kind"]]
+    reads = sum(c.devices.dtype.itemsize + 8 * _EVAL_COST[k][0] + (16 if k == 2 else 0) for k in kinds)
+    writes = sum(8 * _EVAL_COST[k][1] for k in kinds)
+    flops = sum(_EVAL_COST[k][2] for k in kinds)
+    pe = prof["eval"]
+    out["eval_rcmesh32"] = dict(avg_us=pe["avg_us"], count=pe["count"], bytes=reads + writes,
+                                gbs=(reads + writes) / (pe["avg_us"] * 1e-6) / 1e9,
+                                fp64_gflops=flops / (pe["avg_us"] * 1e-6) / 1e9,
+                                fp64_frac=flops / (pe["avg_us"] * 1e-6) / 78.6e12, devices=len(kinds))
+    apt, ait = g.pattern(E.MODE_TRAN_TRAP)
+    nt = len(apt) - 1
+    n_rhs = sum(_EVAL_COST[k][1] for k in kinds) - len(ait)  # stamp values not in the matrix
+    b_asm = 20 * (len(ait) + max(n_rhs, 0)) + 8 * nt
+    pa = prof["assemble"]
+    out["assemble_rcmesh32"] = dict(avg_us=pa["avg_us"], count=pa["count"], bytes=b_asm,
+                                    gbs=b_asm / (pa["avg_us"] * 1e-6) / 1e9,
+                                    hbm_frac=b_asm / (pa["avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS)
+    s.close()
+    return out
+
+
 def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
     """Transient adjoint on the RC mesh driven by level-1 inverters (the
     adjoint workload at nx x nx with nx*nx/32 taps): dV(centre at t =
@@ -414,6 +493,7 @@ def main():
         line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
         line["tran_graph"] = tran_graph_cost(E, NL, S)
         line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
+        line["kernel_rooflines"] = kernel_rooflines(E, NL, S, torch, np)
     if not args.no_cpu:
         cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])
         xg, rg = g.dc_op(s)

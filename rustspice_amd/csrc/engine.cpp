@@ -580,9 +580,12 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
 // currents with their exact derivative; x converges quadratically to the
 // step's exact root, and the residual form keeps the solve's round-off
 // relative to the shrinking residual. The factor is the mode's adjoint solver
-// (SparseLU, zero-diagonal unknowns last: no clamped pivot). Stops when every
-// |dx| <= 1e-12 + 1e-10 |x| (the update then leaves an error of order
-// |dx|^2), or when the steps stall at round-off below 1e-8 |x|.
+// (SparseLU, zero-diagonal unknowns last: no clamped pivot). After the first
+// step the factor is reused (chord steps: the residual is always the exact
+// one) while each correction is at most a tenth of the one before, and
+// refactored at x otherwise. Stops when every |dx| <= 1e-12 + 1e-10 |x| (the
+// update then leaves an error of a small fraction of |dx|), or when a step on
+// a fresh factor stalls at round-off below 1e-8 |x|.
 int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, double dt,
                      double gmin, double scale, bool* ok) {
     rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
@@ -590,6 +593,7 @@ int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, 
     int32_t rc;
     *ok = false;
     double prev = 1.7976931348623157e308;
+    bool refactor = true;
     for (int it = 0; it < 50; ++it) {
         const i64 xl = (i64)x.size();
         if ((e = c->d_x.ensure(std::max(c->d_x.n, x.size()))) ||
@@ -627,10 +631,13 @@ int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, 
             (e = launch_resid_n(A.rp.p, A.rc.p, A.rq.p, c->d_ax.p, c->d_rhs.p, c->d_x.p, (int)xl, c->d_r.p,
                                 (int)n, nullptr)))
             return chip(c, e, "exact Newton residual");
-        rs_prepare(A.s, n);
-        if ((rc = solver_factor_dev(A.s, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
-                                    c->d_ax.p, nullptr)))
-            return from_solver(c, A.s, rc);
+        const bool full = refactor || A.s->state != ST_FACTORED || A.s->n != n;
+        if (full) {
+            rs_prepare(A.s, n);
+            if ((rc = solver_factor_dev(A.s, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
+                                        c->d_ax.p, nullptr)))
+                return from_solver(c, A.s, rc);
+        }
         if ((rc = solver_solve_dev(A.s, c->d_r.p, c->d_del.p, nullptr))) return from_solver(c, A.s, rc);
         std::vector<double> dx(n);
         if ((e = hipMemcpy(dx.data(), c->d_del.p, n * 8, hipMemcpyDeviceToHost))) return chip(c, e, "exact Newton dx");
@@ -645,10 +652,13 @@ int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, 
         }
         c->polish_iters++;
         if (!(rel == rel)) return RS_OK;  // NaN: not converged
-        if (conv || (rel <= 1e-8 && rel >= 0.5 * prev)) {
+        // a stall counts only on a step with a fresh factor (a chord step
+        // that contracts slowly is followed by one)
+        if (conv || (full && rel <= 1e-8 && rel >= 0.5 * prev)) {
             *ok = true;
             return RS_OK;
         }
+        refactor = !(rel <= 0.1 * prev);
         prev = rel;
     }
     return RS_OK;

@@ -253,9 +253,12 @@ int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
             for (int32_t k = r.div_ptr[ph + 1] - 1; k >= r.div_ptr[ph]; k--)
                 V[r.div[2 * k]] = V[r.div[2 * k]] / clampd(V[r.div[2 * k + 1]], false);
             for (int32_t k = r.piv_ptr[ph + 1] - 1; k >= r.piv_ptr[ph]; k--) {
-                const int32_t j = r.piv[k];
-                const int32_t a = ux0 + p.ucp[j + 1] - 1;
-                V[a] = clampd(V[a], p.colflag[j] & COL_SINGLETON);
+                // the kernel's record: diagonal slot, singleton flag in bit 31
+                const int32_t j = r.piv[k], pa = r.piv_a[k];
+                const int32_t a = pa & 0x7FFFFFFF;
+                if (a != ux0 + p.ucp[j + 1] - 1 || (pa < 0) != ((p.colflag[j] & COL_SINGLETON) != 0))
+                    throw std::runtime_error("pivot record");
+                V[a] = clampd(V[a], pa < 0);
                 udv[j] = V[a];
             }
             nlong += r.task_nlong[ph];

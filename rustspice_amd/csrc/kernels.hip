@@ -200,6 +200,7 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
     const int2* div = reinterpret_cast<const int2*>(ip + d.o_rl_div);
     const int32_t* piv_ptr = ip + d.o_rl_piv_ptr;
     const int32_t* piv = ip + d.o_rl_piv;
+    const int32_t* piv_a = ip + d.o_rl_piv_a;
     const int32_t* task_ptr = ip + d.o_rl_task_ptr;
     const int32_t* task_nlong = ip + d.o_rl_task_nlong;
     const int32_t* task_tgt = ip + d.o_rl_task_tgt;
@@ -207,8 +208,6 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
     const int2* pair = reinterpret_cast<const int2*>(ip + d.o_rl_pair);
     const int32_t* sgl_ptr = ip + d.o_rl_sgl_ptr;
     const int4* sgl = reinterpret_cast<const int4*>(ip + d.o_rl_sgl);
-    const int32_t* ucp = ip + d.o_ucp;
-    const int32_t* colflag = ip + d.o_colflag;
 
     // A scattered into every slot: duplicates summed in CSC order from 0.0
     for (int a = tid; a < nv; a += bs) {
@@ -227,9 +226,9 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
         __syncthreads();
         // (b) the clamped diagonals, then the ordered updates of the phase
         for (int k = piv_ptr[p] + tid; k < piv_ptr[p + 1]; k += bs) {
-            const int j = piv[k];
-            const int a = ux0 + ucp[j + 1] - 1;
-            const double diag = clamp_pivot(V[a], colflag[j] & rsgpu::COL_SINGLETON);
+            const int j = piv[k], pa = piv_a[k];
+            const int a = pa & 0x7FFFFFFF;
+            const double diag = clamp_pivot(V[a], pa < 0);
             V[a] = diag;
             G[ud0 + j] = diag;
         }
@@ -279,13 +278,24 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
             for (int i = 0; i < kSglBatch; ++i)
                 if (rc[i].x >= 0 && vu[i] != 0.0) V[rc[i].x] = vt[i] - vl[i] * vu[i];
         }
+        // short tasks: pairs and values loaded eight at a time (every load
+        // in bounds: a task has at least one pair), updates in order
         for (int k = tl + tid; k < t1; k += bs) {
             const int tgt = task_tgt[k], b = task_beg[k], e = task_beg[k + 1];
             double v = V[tgt];
-            for (int q = b; q < e; ++q) {
-                const int2 pq = pair[q];
-                const double uu = V[pq.y];
-                if (uu != 0.0) v = v - V[pq.x] * uu;
+            for (int q0 = b; q0 < e; q0 += 8) {
+                int2 pq[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) pq[i] = pair[min(q0 + i, e - 1)];
+                double lv[8], uv[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    uv[i] = V[pq[i].y];
+                    lv[i] = V[pq[i].x];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (q0 + i < e && uv[i] != 0.0) v = v - lv[i] * uv[i];
             }
             V[tgt] = v;
         }

@@ -36,7 +36,7 @@ struct PlanDev {
     int32_t rl_nph, rl_nv;
     int32_t o_rl_ini_src, o_rl_ini_cnt, o_rl_div_ptr, o_rl_div, o_rl_piv_ptr, o_rl_piv;
     int32_t o_rl_task_ptr, o_rl_task_nlong, o_rl_task_tgt, o_rl_task_beg, o_rl_pair;
-    int32_t o_rl_sgl_ptr, o_rl_sgl;
+    int32_t o_rl_sgl_ptr, o_rl_sgl, o_rl_piv_a;
     int64_t o_lx, o_ux, o_ud;  // value pool
     int64_t o_lxr, o_uxr;      // row-ordered copies of lx / ux (flags & PLAN_CSR)
     int64_t o_ax;              // ax pool (unmerged CSC values as handed in)
@@ -111,6 +111,7 @@ struct RlHost {
     std::vector<int32_t> ini_src, ini_cnt;  // per value slot: run of ax entries summed into it
     std::vector<int32_t> div_ptr, div;      // per phase: (L slot, diagonal slot) pairs
     std::vector<int32_t> piv_ptr, piv;      // per phase: pivot columns
+    std::vector<int32_t> piv_a;             // per pivot: its diagonal slot, bit 31 a singleton column
     std::vector<int32_t> task_ptr, task_nlong, task_tgt, task_beg;  // long tasks first per phase
     std::vector<int32_t> pair;              // (L slot, U slot) pairs of each task, in order
     std::vector<int32_t> sgl_ptr, sgl;      // per phase: one-update tasks as (target, L, U, 0)

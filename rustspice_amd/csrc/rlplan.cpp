@@ -121,6 +121,7 @@ bool build_rl_plan(const PlanHost& p, RlHost& r) {
     }
     r.div.assign(2 * (size_t)r.div_ptr[nph], 0);
     r.piv.assign(r.piv_ptr[nph], 0);
+    r.piv_a.assign(r.piv_ptr[nph], 0);
     {
         std::vector<int32_t> fd(r.div_ptr.begin(), r.div_ptr.end() - 1);
         std::vector<int32_t> fp(r.piv_ptr.begin(), r.piv_ptr.end() - 1);
@@ -131,6 +132,7 @@ bool build_rl_plan(const PlanHost& p, RlHost& r) {
                 r.div[2 * (size_t)at] = l;
                 r.div[2 * (size_t)at + 1] = dslot;
             }
+            r.piv_a[fp[lev[j]]] = dslot | ((p.colflag[j] & COL_SINGLETON) ? (int32_t)0x80000000u : 0);
             r.piv[fp[lev[j]]++] = j;
         }
     }
@@ -211,6 +213,7 @@ void append_rl(const RlHost& r, std::vector<int32_t>& pool, PlanDev& d) {
     d.o_rl_div = put(r.div, true);
     d.o_rl_piv_ptr = put(r.piv_ptr, false);
     d.o_rl_piv = put(r.piv, false);
+    d.o_rl_piv_a = put(r.piv_a, false);
     d.o_rl_task_ptr = put(r.task_ptr, false);
     d.o_rl_task_nlong = put(r.task_nlong, false);
     d.o_rl_task_tgt = put(r.task_tgt, false);
