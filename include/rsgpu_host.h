@@ -72,7 +72,8 @@ int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t *ap, const in
                             int64_t *stats);
 /* Flat pipelined solve (solve mode 3) on the host (no reference
  * counterpart): the first factor on ax (kind bits 8..15: the ordering of
- * rs_solver_set_config), then the level records of the
+ * rs_solver_set_config; bit 16: the transposed records, x solving J^T x =
+ * rhs, for single-block plans), then the level records of the
  * solver's plan run level by level, products and rows each in reverse order.
  * x equals NativeKluSolver factor(ax); solve(rhs) bit for bit. Returns 6 when
  * the plan has no level records (BTF off-diagonal entries) or a record is out

@@ -95,6 +95,12 @@ struct rs_circuit {
         int64_t factors = -1;  // factor_count when the L^T levels were built
         int n_lt = 0;
         DevBuf<int32_t> aux, ltp, ltc, jap, jai;
+        // flat pipelined transposed solve (k_solve_lp on the transposed
+        // records) when they fit the LDS; k_solve_t otherwise
+        bool t_ok = false;
+        int t_nlev = 0, t_maxent = 0;
+        DevBuf<int32_t> tip;
+        DevBuf<PlanDev> tdesc;
         bool have_rows = false;        // row map of the stamped pattern (polish residuals)
         DevBuf<int32_t> rp, rc, rq;
     } adj[2];
@@ -533,6 +539,19 @@ int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double 
             (e = hipMemcpy(A.jai.p, jai.data(), jai.size() * 4, hipMemcpyHostToDevice)))
             return chip(c, e, "adjoint pattern");
         A.n_lt = nlev;
+        std::vector<int32_t> tpool;
+        PlanDev td = a->pd;
+        A.t_ok = build_transposed_records(a->ph, tpool, td) &&
+                 solve_lp_lds((int)n, td.sq_maxent, td.sq_nlev) <= device_max_lds();
+        if (A.t_ok) {
+            td.o_vec = 0;
+            A.t_nlev = td.sq_nlev;
+            A.t_maxent = td.sq_maxent;
+            if ((e = A.tip.ensure(tpool.size())) || (e = A.tdesc.ensure(1)) ||
+                (e = hipMemcpy(A.tip.p, tpool.data(), tpool.size() * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(A.tdesc.p, &td, sizeof td, hipMemcpyHostToDevice)))
+                return chip(c, e, "adjoint transposed records");
+        }
         A.factors = a->factor_count;
     }
     return RS_OK;
@@ -547,18 +566,20 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
     const i64 n = a->n;
     hipStream_t s = nullptr;
     hipError_t e;
+    auto tsolve = [&](const double* b, double* y) -> hipError_t {
+        if (A.t_ok)
+            return launch_solve_lp(A.tdesc.p, A.tip.p, a->d_vals.p, b, y, (int)n, A.t_maxent, A.t_nlev, 256, s);
+        return launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, b, A.ltp.p, A.ltc.p, A.n_lt, c->d_t.p, y, s);
+    };
     if ((e = c->d_t.ensure(n)) || (e = c->d_r.ensure(n)) ||
-        (e = c->d_del.ensure(n)) || (e = c->d_nrm.ensure(2)) ||
-        (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, rhs_dev, A.ltp.p, A.ltc.p, A.n_lt,
-                            c->d_t.p, lam_dev, s)))
+        (e = c->d_del.ensure(n)) || (e = c->d_nrm.ensure(2)) || (e = tsolve(rhs_dev, lam_dev)))
         return chip(c, e, "adjoint solve");
     double prev = 1.7976931348623157e308;
     for (int it = 0; it < max_refine; ++it) {
         double nrm[2] = {0.0, 1.0};
         const bool last = it + 1 == max_refine && max_refine < 12;  // a fixed count: no read-back
         if ((e = launch_resid_t(A.jap.p, A.jai.p, c->d_ax.p, rhs_dev, lam_dev, c->d_r.p, (int)n, s)) ||
-            (e = launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, c->d_r.p, A.ltp.p, A.ltc.p, A.n_lt,
-                                c->d_t.p, c->d_del.p, s)) ||
+            (e = tsolve(c->d_r.p, c->d_del.p)) ||
             (e = launch_refine_update(lam_dev, c->d_del.p, (int)n, c->d_nrm.p, s)) ||
             (!last && (e = hipMemcpy(nrm, c->d_nrm.p, sizeof nrm, hipMemcpyDeviceToHost))))
             return chip(c, e, "adjoint refinement");

@@ -301,12 +301,23 @@ int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
                             const double* rhs, double* x, int64_t* stats) {
     rs_host_factor* h = nullptr;
     try {
-        // kind bits 8..15: the ordering (rs_solver_set_config)
+        // kind bits 8..15: the ordering (rs_solver_set_config); bit 16: transposed
         h = rs_host_factor_create(kind & 0xFF, n, ap, ai, ax, 0.001, (kind >> 8) & 0xFF, 1);
         const PlanHost& p = h->p;
         std::vector<int32_t> pool;
         PlanDev d{};
-        append_level_records(p, pool, d);
+        const bool transposed = (kind >> 16) & 1;  // J^T x = rhs through the transposed records
+        std::vector<int32_t> pin = p.perm_in, pout = p.perm_out;
+        if (transposed) {
+            if (!build_transposed_records(p, pool, d)) {
+                delete h;
+                return RS_ERR_INVALID_MATRIX;
+            }
+            pin.assign(pool.begin() + d.o_perm_in, pool.begin() + d.o_perm_in + n);
+            pout.assign(pool.begin() + d.o_perm_out, pool.begin() + d.o_perm_out + n);
+        } else {
+            append_level_records(p, pool, d);
+        }
         if (d.sq_nlev == 0) {
             delete h;
             return RS_ERR_INVALID_MATRIX;
@@ -319,7 +330,7 @@ int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
         std::copy(ux.begin(), ux.end(), vals.begin() + p.nl + 1);
         auto at = [&](int32_t off, size_t i) { return pool.at((size_t)off + i); };
         std::vector<double> T(n), P((size_t)d.sq_maxent);
-        for (int64_t i = 0; i < n; i++) T.at(p.perm_in[i]) = rhs[i];
+        for (int64_t i = 0; i < n; i++) T.at(pin[i]) = rhs[i];
         int64_t slots = 0;
         for (int32_t l = 0; l < d.sq_nlev; l++) {
             const int32_t s0 = at(d.o_sq_lev, 4 * l), k0 = at(d.o_sq_lev, 4 * l + 1), div = at(d.o_sq_lev, 4 * l + 2);
@@ -338,7 +349,7 @@ int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
             }
             slots += s1 - s0;
         }
-        for (int64_t i = 0; i < n; i++) x[i] = T.at(p.perm_out[i]);
+        for (int64_t i = 0; i < n; i++) x[i] = T.at(pout[i]);
         if (stats) {
             stats[0] = d.sq_nlev;
             stats[1] = d.sq_maxent;

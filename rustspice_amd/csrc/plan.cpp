@@ -406,6 +406,76 @@ void append_level_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev
     d.sq_maxent = maxent;
 }
 
+bool build_transposed_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
+    if (p.nblk != 1) return false;
+    const int32_t n = p.n;
+    pool.clear();
+    auto put16 = [&](const std::vector<int32_t>& v) -> int32_t {
+        while (pool.size() & 3) pool.push_back(0);
+        const size_t off = pool.size();
+        if (off + v.size() + 4 > (size_t)INT32_MAX) throw std::runtime_error("plan pool too large");
+        pool.insert(pool.end(), v.begin(), v.end());
+        pool.insert(pool.end(), {0, 0, 0, 0});
+        return (int32_t)off;
+    };
+    std::vector<int32_t> lev, row, col, src;
+    int32_t maxent = 0;
+    auto close_level = [&](int32_t s0, int32_t k0, int32_t div) {
+        if ((int32_t)(row.size() / 4) == k0) return;
+        lev.insert(lev.end(), {s0, k0, div, 0});
+        maxent = std::max(maxent, (int32_t)col.size() - s0);
+    };
+    // U^T: y_k = (t_k - sum_j U(j,k) y_j) / U(k,k) on the column levels
+    for (int32_t l = 0; l < p.nflev; l++) {
+        const int32_t s0 = (int32_t)col.size(), k0 = (int32_t)(row.size() / 4);
+        for (int32_t q = p.flev_ptr[l]; q < p.flev_ptr[l + 1]; q++) {
+            const int32_t k = p.flev_cols[q], sb = (int32_t)col.size();
+            for (int32_t u = p.ucp[k]; u < p.ucp[k + 1] - 1; u++) {
+                col.push_back(p.uri[u]);
+                src.push_back(p.nl + 1 + u);
+            }
+            row.insert(row.end(), {k, sb, (int32_t)col.size(), 0});
+        }
+        close_level(s0, k0, 1);
+    }
+    // L^T: z_k = y_k - sum_i L(i,k) z_i, column k after every row of L(:,k)
+    std::vector<int32_t> llev(n, 0);
+    int32_t nl_lev = 0;
+    for (int32_t k = n - 1; k >= 0; k--) {
+        int32_t lv = 0;
+        for (int32_t q = p.lcp[k]; q < p.lcp[k + 1]; q++) lv = std::max(lv, llev[p.lri[q]] + 1);
+        llev[k] = lv;
+        nl_lev = std::max(nl_lev, lv + 1);
+    }
+    std::vector<std::vector<int32_t>> by(nl_lev);
+    for (int32_t k = 0; k < n; k++)
+        if (p.lcp[k + 1] > p.lcp[k]) by[llev[k]].push_back(k);
+    for (int32_t l = 0; l < nl_lev; l++) {
+        const int32_t s0 = (int32_t)col.size(), k0 = (int32_t)(row.size() / 4);
+        for (int32_t k : by[l]) {
+            const int32_t sb = (int32_t)col.size();
+            for (int32_t q = p.lcp[k]; q < p.lcp[k + 1]; q++) {
+                col.push_back(p.lri[q]);
+                src.push_back(q);
+            }
+            row.insert(row.end(), {k, sb, (int32_t)col.size(), 0});
+        }
+        close_level(s0, k0, 0);
+    }
+    if (lev.empty()) return false;
+    const int32_t nlev = (int32_t)(lev.size() / 4);
+    lev.insert(lev.end(), {(int32_t)col.size(), (int32_t)(row.size() / 4), 0, 0});
+    d.o_perm_in = put16(p.perm_out);  // T[perm_out[i]] = e[i]
+    d.o_perm_out = put16(p.perm_in);  // y[i] = T[perm_in[i]]
+    d.o_sq_lev = put16(lev);
+    d.o_sq_row = put16(row);
+    d.o_sq_col = put16(col);
+    d.o_sq_src = put16(src);
+    d.sq_nlev = nlev;
+    d.sq_maxent = maxent;
+    return true;
+}
+
 // Pattern-level analysis: symbolic factor with identity pivots so that the
 // plan's scatter targets, solve permutation and off-diagonal row keys are the
 // pre-pivot positions the kernel maps through its own RPI.

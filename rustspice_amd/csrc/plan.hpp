@@ -83,6 +83,13 @@ void build_plan(const HostFactor& f, const i64* ap, const i64* ai, PlanHost& p);
 // 16-byte aligned offsets; none when a block has BTF off-diagonal entries.
 void append_level_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
 
+// The same records for the transposed solve J^T y = e of a single-block plan,
+// in a pool of their own with the permutations swapped: U^T rows (the columns
+// of U above the diagonal, on the refactor's column levels, divided by the
+// diagonal), then L^T rows (the columns of L, on their own levels). d is the
+// plan's descriptor with perm and record offsets into that pool.
+bool build_transposed_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
+
 // Gather L/U values of all blocks into position-space arrays (lx[nl], ux[nu],
 // ud[n]) matching PlanHost's lri/uri order.
 void gather_values(const HostFactor& f, std::vector<double>& lx, std::vector<double>& ux,

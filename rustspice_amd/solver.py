@@ -260,15 +260,16 @@ def host_sp_selftest(n, ap, ai, ax0, ax1, rhs, kind=NATIVEKLU, own_plan=False):
     return x, dict(zip(keys, map(int, st)))
 
 
-def host_lp_selftest(n, ap, ai, ax, rhs, kind=NATIVEKLU, ordering=0):
+def host_lp_selftest(n, ap, ai, ax, rhs, kind=NATIVEKLU, ordering=0, transposed=False):
     """Host model of the flat pipelined solve (solve mode 3): first factor on
-    ax (with the given ordering), then the level records run level by level.
-    Returns (x, stats), or raises SolverError when the plan has no level
-    records."""
+    ax (with the given ordering), then the level records run level by level
+    (transposed: the records of J^T x = rhs used by the adjoint). Returns (x,
+    stats), or raises SolverError when the plan has no level records."""
     L = _lib.lib()
     x = np.zeros(n)
     st = np.zeros(5, np.int64)
-    rc = L.rs_host_lp_selftest(kind | (ordering << 8), n, _i64(ap), _i64(ai), _f64(ax), _f64(rhs), x, st)
+    k = kind | (ordering << 8) | (0x10000 if transposed else 0)
+    rc = L.rs_host_lp_selftest(k, n, _i64(ap), _i64(ai), _f64(ax), _f64(rhs), x, st)
     if rc:
         raise SolverError(rc, "level records")
     keys = ["levels", "max_level_entries", "entries", "rows", "sweep_levels"]

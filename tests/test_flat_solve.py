@@ -79,3 +79,32 @@ def test_random_matrices_with_pivoting():
             assert same_bits(x, want), (trial, kind)
             checked += 1
     assert checked > 60
+
+
+def test_transposed_records_solve_the_transposed_system():
+    """The adjoint's transposed records (U^T on the refactor's column levels,
+    then L^T) solve J^T y = e, SparseLU as the adjoint factors it (ordering 0
+    and 4: unknowns without a diagonal entry last), against a dense solve.
+    The matrices are diagonally dominant, so the factor is J to round-off (on
+    a circuit matrix SparseLU clamps the ground pivot and F is not J)."""
+    rs = np.random.RandomState(5)
+    checked = 0
+    for trial in range(40):
+        n = int(rs.randint(5, 80))
+        ap, ai = random_csc(rs, n, rs.uniform(0.05, 0.3))
+        A = np.zeros((n, n))
+        for j in range(n):
+            for q in range(ap[j], ap[j + 1]):
+                A[ai[q], j] += rs.randn()
+        A += 10.0 * np.eye(n)
+        cols = [np.nonzero(A[:, j])[0] for j in range(n)]
+        ap2 = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int64)
+        ai2 = np.concatenate(cols).astype(np.int64)
+        ax2 = np.concatenate([A[c, j] for j, c in enumerate(cols)])
+        e = rs.randn(n)
+        want = np.linalg.solve(A.T, e)
+        for ordering in (0, 4):
+            y, st = S.host_lp_selftest(n, ap2, ai2, ax2, e, S.SPARSELU, ordering, transposed=True)
+            assert np.abs(y - want).max() <= 1e-12 * np.abs(want).max(), (trial, ordering)
+            checked += 1
+    assert checked == 80
