@@ -381,6 +381,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the per-kernel rooflines (mesh refactor/solve, RC mesh eval/assemble)")
     args = ap.parse_args()
     if args.copy_mib:
         import torch
@@ -407,6 +409,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.kernels_only:
+        print(json.dumps({"kernel_rooflines": kernel_rooflines(E, NL, S, torch, np)}))
+        return
     circ = NL.inverter_chain(500)
     tran_args = dict(NL.C2_TRAN)
     g = E.GpuCircuit(circ)

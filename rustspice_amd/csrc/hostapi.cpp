@@ -233,6 +233,20 @@ int32_t rs_host_rl_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
             delete h;
             return RS_ERR_INVALID_MATRIX;
         }
+        {  // the device form of the one-update records decodes to the records
+            std::vector<int32_t> pool;
+            PlanDev d{};
+            append_rl(r, pool, d);
+            const uint64_t mask = (1ull << kRlSlotBits) - 1;
+            for (int32_t k = 0; k < r.nsgl; k++) {
+                const uint64_t w = (uint64_t)(uint32_t)pool.at(d.o_rl_sgl + 2 * (size_t)k) |
+                                   ((uint64_t)(uint32_t)pool.at(d.o_rl_sgl + 2 * (size_t)k + 1) << 32);
+                if ((int32_t)(w & mask) != r.sgl[4 * (size_t)k] ||
+                    (int32_t)((w >> kRlSlotBits) & mask) != r.sgl[4 * (size_t)k + 1] ||
+                    (int32_t)(w >> (2 * kRlSlotBits)) != r.sgl[4 * (size_t)k + 2])
+                    throw std::runtime_error("packed record");
+            }
+        }
         // the schedule, with every phase's independent work run in reverse
         // order: the values must not depend on the order inside a phase
         std::vector<double> V((size_t)r.nv, 0.0);

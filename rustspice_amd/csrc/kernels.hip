@@ -207,7 +207,7 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
     const int32_t* task_beg = ip + d.o_rl_task_beg;
     const int2* pair = reinterpret_cast<const int2*>(ip + d.o_rl_pair);
     const int32_t* sgl_ptr = ip + d.o_rl_sgl_ptr;
-    const int4* sgl = reinterpret_cast<const int4*>(ip + d.o_rl_sgl);
+    const unsigned long long* sgl = reinterpret_cast<const unsigned long long*>(ip + d.o_rl_sgl);
 
     // A scattered into every slot: duplicates summed in CSC order from 0.0
     for (int a = tid; a < nv; a += bs) {
@@ -258,25 +258,26 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
         }
         // one-update tasks, kSglBatch per thread with every load issued first
         const int g0 = sgl_ptr[p], g1 = sgl_ptr[p + 1];
+        // (records packed as target | L << 21 | U << 42; a batch slot past
+        // the phase's end reloads the record at base, which is in range)
+        constexpr unsigned long long kSlot = (1ull << rsgpu::kRlSlotBits) - 1;
         for (int base = g0 + tid; base < g1; base += kSglBatch * bs) {
-            int4 rc[kSglBatch];
+            unsigned long long rc[kSglBatch];
 #pragma unroll
             for (int i = 0; i < kSglBatch; ++i) {
                 const int k = base + i * bs;
-                rc[i] = k < g1 ? sgl[k] : make_int4(-1, 0, 0, 0);
+                rc[i] = sgl[k < g1 ? k : base];
             }
             double vt[kSglBatch], vl[kSglBatch], vu[kSglBatch];
 #pragma unroll
             for (int i = 0; i < kSglBatch; ++i) {
-                if (rc[i].x >= 0) {
-                    vt[i] = V[rc[i].x];
-                    vl[i] = V[rc[i].y];
-                    vu[i] = V[rc[i].z];
-                }
+                vt[i] = V[rc[i] & kSlot];
+                vl[i] = V[(rc[i] >> rsgpu::kRlSlotBits) & kSlot];
+                vu[i] = V[rc[i] >> (2 * rsgpu::kRlSlotBits)];
             }
 #pragma unroll
             for (int i = 0; i < kSglBatch; ++i)
-                if (rc[i].x >= 0 && vu[i] != 0.0) V[rc[i].x] = vt[i] - vl[i] * vu[i];
+                if (base + i * bs < g1 && vu[i] != 0.0) V[rc[i] & kSlot] = vt[i] - vl[i] * vu[i];
         }
         // short tasks: pairs and values loaded eight at a time (every load
         // in bounds: a task has at least one pair), updates in order

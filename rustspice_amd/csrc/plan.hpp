@@ -112,6 +112,9 @@ void append_plan(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
 // while the entries of one phase are updated in parallel. Value indices are
 // relative to lx with lx | pad | ux | pad | ud contiguous.
 constexpr int32_t kRlLongTask = 24;  // a task with this many updates takes a wavefront
+// One-update records go to the device packed in 8 bytes: target, L and U
+// value slots of kRlSlotBits each (the schedule is used only below 2^21 slots).
+constexpr int32_t kRlSlotBits = 21;
 
 struct RlHost {
     int32_t nph = 0, nv = 0, ntask = 0, npair = 0;
@@ -121,7 +124,7 @@ struct RlHost {
     std::vector<int32_t> piv_a;             // per pivot: its diagonal slot, bit 31 a singleton column
     std::vector<int32_t> task_ptr, task_nlong, task_tgt, task_beg;  // long tasks first per phase
     std::vector<int32_t> pair;              // (L slot, U slot) pairs of each task, in order
-    std::vector<int32_t> sgl_ptr, sgl;      // per phase: one-update tasks as (target, L, U, 0)
+    std::vector<int32_t> sgl_ptr, sgl;      // per phase: one-update tasks as (target, L, U, 0); packed on upload
     int32_t nsgl = 0;
 };
 

@@ -222,7 +222,17 @@ void append_rl(const RlHost& r, std::vector<int32_t>& pool, PlanDev& d) {
     if (pool.size() & 3) pool.resize((pool.size() + 3) & ~(size_t)3, 0);  // 16-byte records
     d.o_rl_sgl_ptr = put(r.sgl_ptr, false);
     if (pool.size() & 3) pool.resize((pool.size() + 3) & ~(size_t)3, 0);
-    d.o_rl_sgl = put(r.sgl, false);
+    // (target, L, U) in one little-endian 64-bit word, kRlSlotBits each
+    if (r.nv >= (1 << kRlSlotBits)) throw std::runtime_error("right-looking schedule: too many value slots");
+    std::vector<int32_t> packed(2 * (size_t)r.nsgl);
+    for (int32_t k = 0; k < r.nsgl; k++) {
+        const uint64_t w = (uint64_t)(uint32_t)r.sgl[4 * (size_t)k] |
+                           ((uint64_t)(uint32_t)r.sgl[4 * (size_t)k + 1] << kRlSlotBits) |
+                           ((uint64_t)(uint32_t)r.sgl[4 * (size_t)k + 2] << (2 * kRlSlotBits));
+        packed[2 * (size_t)k] = (int32_t)(uint32_t)w;
+        packed[2 * (size_t)k + 1] = (int32_t)(uint32_t)(w >> 32);
+    }
+    d.o_rl_sgl = put(packed, false);
 }
 
 }  // namespace rsgpu

@@ -282,7 +282,8 @@ int32_t upload_plan(rs_solver* h, i64 nnz) {
         // the plan changes (a DC OP re-analyzes the same pattern after prepare())
         if (key != h->rl_key || h->rl.nv == 0) {
             h->rl_key = key;
-            h->rl_ok = h->ph.refactor_flops <= 2.0 * kRlMaxUpdates && build_rl_plan(h->ph, h->rl);
+            h->rl_ok = h->ph.refactor_flops <= 2.0 * kRlMaxUpdates && build_rl_plan(h->ph, h->rl) &&
+                       h->rl.nv < (1 << kRlSlotBits);
             if (!h->rl_ok) h->rl = RlHost();
         }
         if (h->rl_ok) append_rl(h->rl, pool, d);
