@@ -258,22 +258,25 @@ __global__ __launch_bounds__(1024) void k_refactor_rl(const PlanDev* __restrict_
         }
         // one-update tasks, kSglBatch per thread with every load issued first
         const int g0 = sgl_ptr[p], g1 = sgl_ptr[p + 1];
-        // (records packed as target | L << 21 | U << 42; a batch slot past
-        // the phase's end reloads the record at base, which is in range)
+        // (records packed as target | L << 21 | U << 42; slots past the
+        // phase's end load nothing)
         constexpr unsigned long long kSlot = (1ull << rsgpu::kRlSlotBits) - 1;
         for (int base = g0 + tid; base < g1; base += kSglBatch * bs) {
             unsigned long long rc[kSglBatch];
 #pragma unroll
             for (int i = 0; i < kSglBatch; ++i) {
                 const int k = base + i * bs;
-                rc[i] = sgl[k < g1 ? k : base];
+                rc[i] = k < g1 ? sgl[k] : 0ull;
             }
             double vt[kSglBatch], vl[kSglBatch], vu[kSglBatch];
 #pragma unroll
             for (int i = 0; i < kSglBatch; ++i) {
-                vt[i] = V[rc[i] & kSlot];
-                vl[i] = V[(rc[i] >> rsgpu::kRlSlotBits) & kSlot];
-                vu[i] = V[rc[i] >> (2 * rsgpu::kRlSlotBits)];
+                vu[i] = 0.0;
+                if (base + i * bs < g1) {
+                    vt[i] = V[rc[i] & kSlot];
+                    vl[i] = V[(rc[i] >> rsgpu::kRlSlotBits) & kSlot];
+                    vu[i] = V[rc[i] >> (2 * rsgpu::kRlSlotBits)];
+                }
             }
 #pragma unroll
             for (int i = 0; i < kSglBatch; ++i)
