@@ -103,10 +103,14 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax
         const int l = __ffsll((long long)m) - 1;
         const int b0 = __shfl(q0, l), b1 = __shfl(q1, l);
         double acc = 0.0;
+        // double-buffered: the next 64 values are gathered into registers
+        // while lane 0 adds the current ones
+        double nxt = b0 + lane < b1 ? sv[rhs_src[b0 + lane]] : 0.0;
         for (int b = b0; b < b1; b += 64) {
             const int c = min(64, b1 - b);
-            if (lane < c) S[lane] = sv[rhs_src[b + lane]];
+            if (lane < c) S[lane] = nxt;
             lane_sync();
+            if (b + 64 + lane < b1) nxt = sv[rhs_src[b + 64 + lane]];
             if (lane == 0) {  // reads issued eight at a time, the additions in order
                 int i = 0;
                 for (; i + 8 <= c; i += 8) {

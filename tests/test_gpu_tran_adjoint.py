@@ -83,8 +83,14 @@ def test_checkpoint_interval_does_not_change_the_gradient(gpu):
     c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
     probe = NL.rc_mesh_probe(4, 4)
     runs = [_gpu(c, 10, probe, every) for every in (1, 3, 1000, 0)]
+    # the recomputed segments reproduce the forward states bit for bit, and
+    # the gradients agree to round-off: in a long process, sensitivities that
+    # are zero up to round-off (|dp| ~ 1e-41 against O(1) ones) have been
+    # seen to differ in their last bits between runs
+    scale = np.abs(runs[0][0]).max()
     for dp, x, info in runs[1:]:
-        assert np.array_equal(dp, runs[0][0]) and np.array_equal(x, runs[0][1])
+        assert np.array_equal(x, runs[0][1])
+        assert np.allclose(dp, runs[0][0], rtol=1e-13, atol=1e-30 * scale)
     # every = 0 keeps all 11 states and recomputes nothing
     assert [r[2]["checkpoints"] for r in runs] == [10, 4, 1, 11]
     assert [r[2]["recomputed_steps"] for r in runs] == [10, 10, 10, 0]
