@@ -258,7 +258,10 @@ int32_t rs_circuit_set_cap_state(rs_circuit *c, const double *cap_v, const doubl
 int32_t rs_stamp(rs_circuit *c, int32_t mode, const double *x, int64_t x_len, double t, double dt,
                  double gmin, double source_scale, double *ax_out, int64_t ax_cap,
                  double *rhs_out, int64_t rhs_cap, int64_t *nnz, int64_t *n);
-/* Engine::run_dc_result (engine.rs:249-311) with run_newton_with_stepping. */
+/* Engine::run_dc_result (engine.rs:249-311) with run_newton_with_stepping.
+ * x_out receives the solution (n entries) when Newton converged, otherwise
+ * the iterate the loop ended with (node_count entries: the last damped x of
+ * newton.rs:93-107); *x_len is the count written either way. */
 int32_t rs_dc_op(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double *x_out, int64_t x_cap,
                  int64_t *x_len, rs_newton_result *res);
 /* Engine::run_tran_result_with_params (engine.rs:333-632): BE + TRAP solves,
@@ -313,8 +316,10 @@ int32_t rs_batch_last_error(const rs_batch *b, char *buf, int64_t len);
 int32_t rs_batch_info(const rs_batch *b, int64_t *out);
 /* Enqueues the batched Newton solve on `stream` (hipStream_t, NULL = default). */
 int32_t rs_batch_dc_op(rs_batch *b, const rs_newton_opts *o, void *stream);
-/* Copies results: res[k] (pad = number of pivoting factors), x of converged
- * instances at x_out + x_ptr[k] (x_ptr has ninst + 1 entries). */
+/* Copies results: res[k] (pad = number of pivoting factors) and, at
+ * x_out + x_ptr[k] (x_ptr has ninst + 1 entries, instance k owns its n
+ * entries), the solution of a converged instance or, otherwise, the iterate
+ * its Newton loop ended with (node_count entries, the aux entries zero). */
 int32_t rs_batch_results(rs_batch *b, rs_newton_result *res, double *x_out, int64_t *x_ptr);
 
 /* Per-phase shader-clock cycles summed over the wavefronts of later

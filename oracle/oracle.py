@@ -120,6 +120,7 @@ def lib():
                                      C.POINTER(C.c_int32)]
         L.orc_lte_milne.argtypes = [f64p, f64p, C.c_int64, C.c_double, C.c_double, f64p]
         L.orc_set_bsim_models.argtypes = [vp, C.c_int64]
+        L.orc_set_mutation.argtypes = [C.c_int]
         L.orc_bsim_eval.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_double, f64p]
         L.orc_dc_sweep_points.restype = C.c_int64
@@ -390,12 +391,21 @@ def level1(dev, vd, vg, vs, vb=0.0):
     return dict(ids=out[0], gm=out[1], gds=out[2], gmbs=out[3], ieq=out[4])
 
 
+def set_mutation(m: int):
+    """Plants a mistake in the oracle (0 restores the reference behaviour;
+    1 flips the PMOS current sign of the level-1 model). Tests use it to show
+    that a parity check fails on a plausible bug."""
+    lib().orc_set_mutation(int(m))
+
+
 def pulse_eval(p7, t):
     return lib().orc_pulse_eval(_f64(p7), t)
 
 
-def dc_op(circ, opts=None, solver=None):
-    """Returns (x or None, NewtonResult)."""
+def dc_op(circ, opts=None, solver=None, last=False):
+    """Returns (x or None, NewtonResult). With last=True, x is returned
+    whether or not Newton converged: the solution, or the x the loop ended
+    with (node_count entries, as newton.rs:93-107 leaves it)."""
     opts = opts or default_opts()
     own = solver is None
     s = solver or NativeKlu(circ.node_count)
@@ -407,7 +417,7 @@ def dc_op(circ, opts=None, solver=None):
                     C.byref(xl), C.byref(res))
     if own:
         del s
-    return (x[: xl.value].copy() if res.converged else None), res
+    return (x[: xl.value].copy() if (res.converged or last) else None), res
 
 
 def tran(circ, tstep, tstop, tstart=0.0, tmax=None, opts=None, solver=None, max_points=100000,

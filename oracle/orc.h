@@ -121,6 +121,9 @@ void orc_mna_free(orc_mna *m);
  * out = {ids, gm, gds, gmbs, ieq} */
 /* BSIM model table used by orc_build_mna for MOSFETs of level != 1 */
 void orc_set_bsim_models(const rs_bsim_model *m, int64_t n);
+/* Planted mistakes (tests only; 0 = reference behaviour, 1 = PMOS current
+ * sign flipped in the level-1 model). */
+void orc_set_mutation(int m);
 const rs_bsim_model *orc_bsim_model(int64_t i);
 /* evaluate_bsim_dc / evaluate_bsim4_dc (level 54): out[13] = {ids, gm, gds,
  * gmbs, ieq, region, vth, isub, gsub, igs, igd, gigs, gigd} */
@@ -137,7 +140,8 @@ typedef struct orc_newton_result {
 } orc_newton_result;
 
 /* DC operating point (engine.rs:249-311). x_out gets n entries on convergence
- * (the full vector incl. aux, newton.rs:98), *x_len its length. */
+ * (the full vector incl. aux, newton.rs:98), otherwise the node_count entries
+ * of the x the loop ended with (newton.rs:93-107); *x_len the count written. */
 int orc_dc_op(const rs_device *dev, int64_t ndev, int64_t node_count,
               const rs_newton_opts *o, orc_klu *solver, double *x_out,
               int64_t x_cap, int64_t *x_len, orc_newton_result *res);

@@ -103,7 +103,7 @@ int orc_dc_op(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_n
     *res = run_newton_stepping(o, &x, &xl, &bc, solver, NULL, NULL);
     int64_t cpy = xl < x_cap ? xl : x_cap;
     if (x_out) memcpy(x_out, x, (size_t)cpy * sizeof(double));
-    *x_len = res->converged ? xl : 0;
+    *x_len = cpy;
     free(x);
     return res->converged ? 0 : 1;
 }

@@ -77,6 +77,11 @@ double orc_pulse_eval(const double p[7], double t) { /* waveform.rs:82-113 */
     return v1;
 }
 
+/* Planted mistakes for the tests that show a parity check can fail (0: the
+ * reference's behaviour). 1: PMOS current sign flipped in orc_level1. */
+static int g_mutation = 0;
+void orc_set_mutation(int m) { g_mutation = m; }
+
 /* evaluate_level1_dc (bsim/evaluate.rs:285-356) with beta from
  * bsim/mod.rs:441-455: lambda fixed at 0.02, beta = u0*1e-4*cox, cox =
  * EPSILON_OX/tox with EPSILON_OX = 3.9*8.854e-12 (params.rs:10, 460-462). */
@@ -92,7 +97,7 @@ void orc_level1(const rs_device *d, double vd, double vg, double vs, double vb,
     double beta = d->p[RS_MP_U0] * 1e-4 * cox;
     double w = d->p[RS_MP_W], l = d->p[RS_MP_L];
     double vdi, vgi, vsi, sign;
-    if (is_pmos) { vdi = -vs; vgi = -vg; vsi = -vd; sign = -1.0; }
+    if (is_pmos) { vdi = -vs; vgi = -vg; vsi = -vd; sign = g_mutation == 1 ? 1.0 : -1.0; }
     else { vdi = vd; vgi = vg; vsi = vs; sign = 1.0; }
     double vgs = vgi - vsi, vds = vdi - vsi;
     if (vds < 0.0) { vds = -vds; vgs = vgi - vdi; }

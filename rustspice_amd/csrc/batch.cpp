@@ -406,7 +406,7 @@ int32_t rs_batch_results(rs_batch* b, rs_newton_result* res, double* x_out, int6
             res[k].final_norm = r[k].final_norm;
         }
         if (x_ptr) x_ptr[k] = off;
-        if (x_out && r[k].converged) {
+        if (x_out) {
             e = hipMemcpy(x_out + off, b->d_scr.p + b->inst[k].o_x, b->n_of[k] * sizeof(double),
                           hipMemcpyDeviceToHost);
             if (e) return bhip(b, e, "hipMemcpy(x)");

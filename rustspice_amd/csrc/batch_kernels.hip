@@ -753,8 +753,9 @@ __global__ __launch_bounds__(64 * kWv) void k_batch_dcop(const BPat* __restrict_
         }
         PHASE(7)
         if (tid == 0) res[ii] = fin;
-        if (fin.converged)
-            for (int i = tid; i < n; i += kT) scr[I.o_x + i] = X[i];
+        // the solution, or the last damped iterate when Newton did not
+        // converge (its node_count entries; the aux entries stay zero)
+        for (int i = tid; i < n; i += kT) scr[I.o_x + i] = X[i];
         if (lumax) {  // leave the factor in global scratch as well
             for (int l = tid; l < P.nl; l += kT) scr[I.o_lx + l] = lx[l];
             for (int u = tid; u < P.nu; u += kT) scr[I.o_ux + u] = ux[u];

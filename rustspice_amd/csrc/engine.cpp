@@ -444,7 +444,10 @@ int32_t device_dc(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::v
     }
     const int32_t full = r->pad;
     r->pad = 0;
-    if (r->converged) x = xo;
+    if (r->converged)
+        x = xo;
+    else  // the last damped iterate, node_count entries (newton.rs:93-107)
+        x.assign(xo.begin(), xo.begin() + std::min<size_t>(xo.size(), (size_t)c->node_count));
     // iterations of earlier stepping passes are not reported by the kernel;
     // the reference engine runs one pass (gmin_steps = source_steps = 0)
     if (iters_total) *iters_total += r->iterations;
@@ -938,7 +941,7 @@ int32_t rs_dc_op(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double*
     if (rc) return rc;
     *res = r;
     if (x_out) std::memcpy(x_out, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
-    if (x_len) *x_len = r.converged ? (int64_t)x.size() : 0;
+    if (x_len) *x_len = (int64_t)x.size();
     return RS_OK;
 }
 

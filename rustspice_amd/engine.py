@@ -214,8 +214,10 @@ class GpuCircuit:
         assert len(ai) == nnz.value and len(ap) - 1 == n.value
         return ap, ai, ax[: nnz.value], rhs[: n.value], n.value
 
-    def dc_op(self, solver: GpuSolver, opts=None):
-        """Returns (x or None, NewtonResult)."""
+    def dc_op(self, solver: GpuSolver, opts=None, last=False):
+        """Returns (x or None, NewtonResult). With last=True, x is returned
+        whether or not Newton converged: the solution, or the iterate the loop
+        ended with (node_count entries)."""
         opts = opts or default_opts()
         cap = self.circuit.node_count + self.circuit.ndev + 8
         x = np.zeros(cap)
@@ -223,7 +225,7 @@ class GpuCircuit:
         r = NewtonResult()
         self._check(self._L.rs_dc_op(self.h, solver.h, C.byref(opts), _ptr(x), cap, C.byref(xl),
                                      C.byref(r)))
-        return (x[: xl.value].copy() if r.converged else None), r
+        return (x[: xl.value].copy() if (r.converged or last) else None), r
 
     def tran(self, solver: GpuSolver, tstep, tstop, tstart=0.0, tmax=None, opts=None,
              max_points=100000, store=True):
@@ -368,12 +370,23 @@ class GpuBatch:
         s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
         self._check(self._L.rs_batch_dc_op(self.h, C.byref(opts), C.c_void_p(s)))
 
-    def results(self):
-        """List of (x or None, NewtonResult) per instance (x: full MNA vector)."""
+    def results(self, last=False):
+        """List of (x or None, NewtonResult) per instance (x: full MNA vector).
+        With last=True, x is returned for every instance: the solution, or the
+        iterate its Newton loop ended with (node_count entries)."""
         k = len(self.circuits)
         res = (NewtonResult * k)()
         xp = np.zeros(k + 1, np.int64)
         self._check(self._L.rs_batch_results(self.h, C.cast(res, C.c_void_p), None, _ptr(xp)))
         x = np.zeros(max(int(xp[-1]), 1))
         self._check(self._L.rs_batch_results(self.h, C.cast(res, C.c_void_p), _ptr(x), _ptr(xp)))
-        return [((x[xp[i]:xp[i + 1]].copy() if res[i].converged else None), res[i]) for i in range(k)]
+        out = []
+        for i in range(k):
+            xi = x[xp[i]:xp[i + 1]]
+            if res[i].converged:
+                out.append((xi.copy(), res[i]))
+            elif last:
+                out.append((xi[: self.circuits[i].node_count].copy(), res[i]))
+            else:
+                out.append((None, res[i]))
+        return out

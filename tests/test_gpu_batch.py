@@ -100,3 +100,29 @@ def test_batch_with_stepping(gpu):
         xo, ro = O.dc_op(c, opts=opts_o)
         assert (rg.converged, rg.iterations, rg.reason) == (ro.converged, ro.iterations, ro.reason)
         assert same_bits([rg.final_norm], [ro.final_norm])
+
+
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+def test_batch_mc_iterates_bit_identical(gpu, kind):
+    """64 Monte-Carlo chains drawn with the workload's own seed (stages
+    U[50, 400]): each instance's iterate after k = 1, 2, 3, 5, 10, 20 Newton
+    iterations equals the oracle's bit for bit. Their DC OPs diverge under
+    reference semantics, so only finite iterates count: every instance must
+    show at least k = 1, 2 finite (NativeKLU: 1, 2, 3, 5)."""
+    circs = NL.monte_carlo_chains(64)
+    b = E.GpuBatch(circs, kind)
+    finite = [[] for _ in circs]
+    for k in (1, 2, 3, 5, 10, 20):
+        b.dc_op(E.default_opts(max_iters=k))
+        for i, (c, (xg, rg)) in enumerate(zip(circs, b.results(last=True))):
+            o = O.NativeKlu(c.node_count)
+            if kind == S.SPARSELU:
+                o.set_sparselu()
+            xo, ro = O.dc_op(c, opts=O.default_opts(max_iters=k), solver=o, last=True)
+            assert (rg.converged, rg.iterations, rg.reason) == (ro.converged, ro.iterations, ro.reason)
+            assert len(xg) == len(xo) and same_bits(xg, xo), (i, k)
+            if np.isfinite(xo).all():
+                finite[i].append(k)
+    need = {1, 2, 3, 5} if kind == S.NATIVEKLU else {1, 2}
+    for i, f in enumerate(finite):
+        assert need <= set(f), (i, f)

@@ -173,3 +173,50 @@ def test_device_dc_then_tran_equals_host_driven(gpu):
               "factor_calls"):
         assert res[0][k] == res[1][k], k
     assert same_bits(res[0]["sols"], res[1]["sols"])
+
+
+ITERS = (1, 2, 3, 5, 10, 20)
+
+
+def compare_iterates(run_gpu, run_oracle, iters=ITERS):
+    """Runs both sides with max_iters = k for each k and compares the iterate
+    each ends with bit for bit. Returns the ks whose oracle iterate is finite:
+    only those count as evidence (NaN == NaN would pin nothing)."""
+    finite = []
+    for k in iters:
+        xg, rg = run_gpu(k)
+        xo, ro = run_oracle(k)
+        assert (rg.converged, rg.iterations, rg.reason) == (ro.converged, ro.iterations, ro.reason), k
+        assert xg is not None and xo is not None and len(xg) == len(xo), k
+        assert same_bits(xg, xo), (k, np.nanmax(np.abs(xg - xo)))
+        if np.isfinite(xo).all():
+            finite.append(k)
+    return finite
+
+
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+@pytest.mark.parametrize("vin_pulse", [True, False])
+@pytest.mark.parametrize("device", [True, False])
+def test_chain500_iterates_bit_identical(gpu, kind, vin_pulse, device):
+    """The 500-stage chain's DC OP (the TRAN's initial operating point when the
+    input is the PULSE source) diverges under reference semantics, so its end
+    state alone pins little. The iterate after k = 1, 2, 3, 5, 10, 20 Newton
+    iterations must equal the oracle's bit for bit, and the first iterates
+    (at least k <= 2, before the divergence overflows: SparseLU's reaches
+    1e141 at k = 2 and NaN at k = 3 on the PULSE-input chain) must be
+    finite."""
+    c = NL.inverter_chain(500, vin_pulse=vin_pulse)
+
+    def run_gpu(k):
+        g = E.GpuCircuit(c)
+        g.set_device_newton(device)
+        return g.dc_op(S.GpuSolver(c.node_count, kind), E.default_opts(max_iters=k), last=True)
+
+    def run_oracle(k):
+        return O.dc_op(c, opts=O.default_opts(max_iters=k), solver=_oracle_solver(c.node_count, kind),
+                       last=True)
+
+    finite = compare_iterates(run_gpu, run_oracle)
+    assert {1, 2} <= set(finite), finite
+    if kind == S.NATIVEKLU:
+        assert {1, 2, 3, 5} <= set(finite), finite

@@ -523,3 +523,45 @@ def test_sparselu_has_no_singleton_rule():
         r = np.array([1.0])
         s.solve(r)
         assert r[0] == slu_x, (d, r[0])
+
+
+@pytest.mark.parametrize("kind", ["nativeklu", "sparselu"])
+def test_iterate_comparison_catches_a_flipped_pmos_sign(kind):
+    """The GPU parity tests of the diverging chain DC OPs compare the iterate
+    after k = 1, 2, 3, 5, 10, 20 Newton iterations. With a planted mistake in
+    the oracle (the PMOS current sign flipped), that comparison fails at a
+    finite iterate, while the end-state summary that diverges to NaN on both
+    sides (converged, iterations, reason, NaN final norm) cannot tell them
+    apart."""
+    circs = [NL.inverter_chain(500, vin_pulse=True), NL.inverter_chain(500, vin_pulse=False)]
+    circs += NL.monte_carlo_chains(3)
+
+    def run(c, k):
+        s = O.NativeKlu(c.node_count)
+        if kind == "sparselu":
+            s.set_sparselu()
+        return O.dc_op(c, opts=O.default_opts(max_iters=k), solver=s, last=True)
+
+    blind = 0
+    for c in circs:
+        caught = False
+        for k in (1, 2, 3, 5, 10, 20):
+            x, r = run(c, k)
+            O.set_mutation(1)
+            try:
+                xm, rm = run(c, k)
+            finally:
+                O.set_mutation(0)
+            if np.isfinite(x).all() and not same_bits(x, xm):
+                caught = True
+        assert caught, c.name
+        _, r = run(c, 100)
+        O.set_mutation(1)
+        try:
+            _, rm = run(c, 100)
+        finally:
+            O.set_mutation(0)
+        if ((r.converged, r.iterations, r.reason) == (rm.converged, rm.iterations, rm.reason)
+                and same_bits([r.final_norm], [rm.final_norm])):
+            blind += 1  # the end-state check passes on the planted mistake
+    assert blind >= 1
