@@ -247,7 +247,8 @@ int32_t rs_tran_adjoint(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, do
  * afterwards the solver keeps its counts but no factorization. 0: every
  * Newton iteration is driven from the host. */
 void rs_circuit_set_device_newton(rs_circuit *c, int32_t on);
-/* TransientState per device (stamp.rs:699-718), device-index keyed. */
+/* TransientState per device (stamp.rs:699-718), device-index keyed: each
+ * array holds the circuit's ndev entries. */
 int32_t rs_circuit_set_cap_state(rs_circuit *c, const double *cap_v, const double *cap_i,
                                  const uint8_t *has_v, const uint8_t *has_i);
 /* One build(x) of the engine closure on the device: ax (unmerged, nnz) and
@@ -337,6 +338,12 @@ int32_t rs_batch_phase_cycles(rs_batch *b, int64_t *out);
  * Enabling (or disabling) clears the totals. */
 void rs_profile_enable(int32_t on);
 void rs_profile_get(double *ms_total, int64_t *counts);
+/* Checking aid (no reference counterpart): with on != 0, every FP64 device
+ * buffer the library allocates afterwards starts filled with NaN, so a value
+ * read before it is written changes the result visibly instead of reading
+ * whatever the allocator's memory held. The environment variable
+ * RSGPU_POISON=1 sets it for the whole process. */
+void rs_set_poison(int32_t on);
 
 #ifdef __cplusplus
 }

@@ -33,7 +33,7 @@ EXPORTS = [
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
     "rs_circuit_set_cap_state", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
     "rs_dc_sweep_points", "rs_dc_sweep", "rs_circuit_set_bsim_models", "rs_mos_eval",
-    "rs_profile_enable", "rs_profile_get",
+    "rs_profile_enable", "rs_profile_get", "rs_set_poison",
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_info",
     "rs_batch_dc_op", "rs_batch_results", "rs_circuit_set_device_newton",
     "rs_batch_phase_profile", "rs_batch_phase_cycles", "rs_dc_adjoint", "rs_tran_adjoint",
@@ -131,6 +131,7 @@ def lib():
     L.rs_host_lp_selftest.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, f64p, f64p, i64p]
     L.rs_profile_enable.argtypes = [C.c_int32]
     L.rs_profile_get.argtypes = [f64p, i64p]
+    L.rs_set_poison.argtypes = [C.c_int32]
     _LIB = L
     return L
 
@@ -140,6 +141,12 @@ PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble", "batch")
 
 def profile_enable(on: bool = True):
     lib().rs_profile_enable(int(on))
+
+
+def set_poison(on: bool = True):
+    """Fresh FP64 device buffers start as NaN while on (a checking aid: a
+    value read before it is written then changes the result visibly)."""
+    lib().rs_set_poison(int(on))
 
 
 def profile_get() -> dict:

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -166,9 +167,17 @@ int32_t get_pattern(rs_circuit* c, int mode, DevPattern*& out) {
         size_t nmax = (size_t)dp.p.n;
         if ((e = c->d_ax.ensure(std::max(c->d_ax.n, dp.p.ai.size()))) ||
             (e = c->d_rhs.ensure(std::max(c->d_rhs.n, nmax))) ||
-            (e = c->d_x.ensure(std::max(c->d_x.n, nmax))) ||
             (e = c->d_xnew.ensure(std::max(c->d_xnew.n, nmax))))
             return chip(c, e, "hipMalloc(vectors)");
+        if (c->d_x.n < nmax) {
+            // x may already hold the caller's iterate: a larger buffer keeps it
+            DevBuf<double> nx;
+            if ((e = nx.ensure(nmax)) ||
+                (c->d_x.p && (e = hipMemcpy(nx.p, c->d_x.p, c->d_x.n * sizeof(double), hipMemcpyDeviceToDevice))))
+                return chip(c, e, "hipMalloc(x)");
+            std::swap(c->d_x.p, nx.p);
+            std::swap(c->d_x.n, nx.n);
+        }
         dp.built = true;
     }
     out = &dp;
@@ -488,7 +497,7 @@ int32_t cap_update(rs_circuit* c, const std::vector<double>& x, double dt, int f
 // refactor when the pattern is unchanged), and the L^T levels and J's int32
 // pattern of that factorization.
 int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t, double dt,
-                   double gmin, bool fresh) {
+                   double gmin) {
     DevPattern* dp;
     int32_t rc;
     if ((rc = stamp_dev(c, mode, x_dev, xl, t, dt, gmin, 1.0, &dp, nullptr, kStampJacobian)))
@@ -503,7 +512,9 @@ int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double 
         A.s = rs_solver_create(RS_SOLVER_SPARSELU, n);
         A.s->ordering = 4;
     }
-    if (fresh || A.s->n != n) rs_prepare(A.s, n);
+    // SparseLU picks no pivots (pivot_tol 0), so a device refactor on the
+    // stored symbolic analysis computes what a fresh factor would
+    rs_prepare(A.s, n);
     if ((rc = solver_factor_dev(A.s, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
                                 c->d_ax.p, nullptr)))
         return from_solver(c, A.s, rc);
@@ -618,6 +629,12 @@ int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, 
     *ok = false;
     double prev = 1.7976931348623157e308;
     bool refactor = true;
+    {
+        // the pattern's device vectors first: building them may grow d_x,
+        // which must not happen between the upload of x and its last read
+        DevPattern* dp0;
+        if ((rc = get_pattern(c, mode, dp0))) return rc;
+    }
     for (int it = 0; it < 50; ++it) {
         const i64 xl = (i64)x.size();
         if ((e = c->d_x.ensure(std::max(c->d_x.n, x.size()))) ||
@@ -960,7 +977,7 @@ int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t pro
     if ((e = hipMemcpy(c->d_x.p, x, xl * sizeof(double), hipMemcpyHostToDevice)))
         return chip(c, e, "hipMemcpy(x)");
     c->adj_refine = 0;
-    if ((rc = adj_factor(c, MODE_DC, c->d_x.p, xl, 0.0, 0.0, gmin, true)))
+    if ((rc = adj_factor(c, MODE_DC, c->d_x.p, xl, 0.0, 0.0, gmin)))
         return rc;
     std::vector<double> ev(n, 0.0);
     ev[probe] = 1.0;
@@ -1421,7 +1438,6 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
     const double* l1 = nullptr;  // lambda_{k+1}
     const double* l2 = nullptr;  // lambda_{k+2}
     int cur = 0;
-    bool fresh[2] = {true, true};
     auto up = [&](DevBuf<double>& b, const std::vector<double>& v) -> hipError_t {
         return v.empty() ? hipSuccess : hipMemcpy(b.p, v.data(), v.size() * 8, hipMemcpyHostToDevice);
     };
@@ -1456,9 +1472,7 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
             if (k >= 2 && !xv2) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint checkpoint lacks x_{k-2}");
             if ((e = up(xk, xv)) || (xv1 && (e = up(xk1, *xv1))) || (xv2 && (e = up(xk2, *xv2))))
                 return chip(c, e, "adjoint x");
-            if ((rc = adj_factor(c, mode, xk.p, (i64)xv.size(), tk[k], dt, gm[k], fresh[mode == MODE_DC ? 0 : 1])))
-                return rc;
-            fresh[mode == MODE_DC ? 0 : 1] = false;
+            if ((rc = adj_factor(c, mode, xk.p, (i64)xv.size(), tk[k], dt, gm[k]))) return rc;
             rs_circuit::AdjCtx& A = c->adj[mode == MODE_DC ? 0 : 1];
             const i64 n = A.s->n;
             if ((e = launch_tadj_rhs(c->d_dev.p, d_cptr.p, d_cdev.p, (int)nc, l1, l2, dt, k >= 1 ? 1 : 0,

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -603,6 +604,15 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
 }  // namespace
 
 namespace rsgpu {
+static int g_poison = -1;  // -1: not decided yet (RSGPU_POISON=1 turns it on)
+bool poison_fp64_buffers() {
+    if (g_poison < 0) {
+        const char* v = std::getenv("RSGPU_POISON");
+        g_poison = v && v[0] == '1' ? 1 : 0;
+    }
+    return g_poison == 1;
+}
+
 size_t device_max_lds() {
     static size_t v = 0;
     if (!v) {
@@ -846,6 +856,8 @@ void rs_set_growth_check(rs_solver* h, double limit) { h->growth_limit = limit >
 int64_t rs_growth_fallbacks(const rs_solver* h) { return h->growth_fallbacks; }
 
 double rs_last_growth(const rs_solver* h) { return h->last_growth; }
+
+void rs_set_poison(int32_t on) { rsgpu::g_poison = on != 0 ? 1 : 0; }
 
 void rs_set_autotune(rs_solver* h, int32_t on) {
     h->autotune = on != 0;

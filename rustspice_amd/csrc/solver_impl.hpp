@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/rsgpu.h"
@@ -42,6 +43,11 @@ enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
 constexpr int kThreads = 256;
 constexpr int kMaxThreads = 1024;  // largest launch the autotuner tries (16 wavefronts)
 
+// RSGPU_POISON=1 (a checking aid): fresh FP64 buffers are filled with NaN, so
+// a value read before it is written shows up instead of reading whatever the
+// allocator's memory held.
+bool poison_fp64_buffers();
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -52,6 +58,8 @@ struct DevBuf {
         p = nullptr;
         n = 0;
         hipError_t e = hipMalloc((void**)&p, (cnt ? cnt : 1) * sizeof(T));
+        if (e == hipSuccess && std::is_same<T, double>::value && poison_fp64_buffers())
+            e = hipMemset(p, 0xFF, (cnt ? cnt : 1) * sizeof(T));
         if (e == hipSuccess) n = cnt;
         return e;
     }

@@ -190,6 +190,9 @@ class GpuCircuit:
     def set_cap_state(self, cap_v, cap_i, has_v, has_i):
         arrs = [np.ascontiguousarray(cap_v, np.float64), np.ascontiguousarray(cap_i, np.float64),
                 np.ascontiguousarray(has_v, np.uint8), np.ascontiguousarray(has_i, np.uint8)]
+        for a in arrs:  # the C entry point reads ndev entries of each
+            if a.ndim != 1 or len(a) != self.circuit.ndev:
+                raise ValueError(f"capacitor state arrays need {self.circuit.ndev} entries, got {a.shape}")
         self._check(self._L.rs_circuit_set_cap_state(self.h, *[_ptr(a) for a in arrs]))
 
     def stamp(self, x=None, mode=MODE_DC, t=0.0, dt=0.0, gmin=1e-12, source_scale=1.0):

@@ -342,3 +342,50 @@ def test_growth_check_refactors_afresh_when_reused_pivots_grow(gpu):
             assert st["factor_count"] == 2 and st["refactor_count"] == 0
             assert np.allclose(x, want, rtol=1e-12)
         g.close()
+
+
+def test_every_launch_shape_is_bit_identical_sparselu_tran(gpu):
+    """SparseLU on a TRAP-step matrix of the adjoint workload (RC mesh with
+    inverter taps): every refactor schedule with every solve mode, and the
+    host first factor against the device refactor of the same values, give
+    the oracle's solution bit for bit."""
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    rs = np.random.RandomState(7)
+    x0 = rs.rand(c.node_count)
+    cv, ci = rs.rand(c.ndev), rs.randn(c.ndev) * 1e-4
+    hv, hi = np.ones(c.ndev, np.uint8), np.ones(c.ndev, np.uint8)
+    ap, ai, ax, rhs, n = O.build_mna(c, x=x0, mode=1, t=4e-12, dt=2e-12, tstate=(cv, ci, hv, hi, 1))
+    ax2 = ax * (1.0 + 0.25 * rs.rand(len(ax)))
+    o = O.NativeKlu(n)
+    o.set_sparselu()
+    o.factor(ap, ai, ax2)
+    xo = rhs.copy()
+    o.solve(xo)
+    g = S.GpuSolver(n, S.SPARSELU)
+    g.factor(ap, ai, ax2)  # host first factor
+    xh = rhs.copy()
+    g.solve(xh)
+    assert same_bits(xh, xo)
+    g.factor(ap, ai, ax2)  # device refactor of the same values
+    xd = rhs.copy()
+    g.solve(xd)
+    assert same_bits(xd, xo)
+    g.close()
+    tried = 0
+    for t in (64, 256, 1024):
+        for mode in (0, 1, 2, 3, 4):
+            for smode in (0, 1, 2, 3):
+                g = S.GpuSolver(n, S.SPARSELU)
+                g.prepare(n)
+                g.analyze(ap, ai)
+                try:
+                    g.set_launch_shape(t, mode, t, smode)
+                    _, xg = _refactor_solve(g, ap, ai, ax, ax2, rhs)
+                except S.SolverError as e:
+                    assert e.variant == "InvalidMatrix", e
+                    g.close()
+                    continue
+                assert same_bits(xg, xo), (t, mode, smode, np.abs(xg - xo).max())
+                g.close()
+                tried += 1
+    assert tried >= 12
