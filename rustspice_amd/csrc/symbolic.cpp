@@ -22,6 +22,18 @@ namespace rsgpu {
 // scanned, and a shortcut for a variable whose own live neighbours already
 // cover every other live variable (the ground node of an MNA matrix), whose
 // degree is then nlive-1 without a scan.
+//
+// The degree of a variable v next to the new element L_p is
+//   |L_p \ {v}| + |(A_v u the other elements of v) \ L_p \ {v}|,
+// so L_p is counted once instead of scanned for each of its members, and an
+// element whose variables all lie in L_p (|L_e \ L_p| = 0, counted once per
+// pivot and element) adds nothing and is not scanned. When only one other
+// element e1 reaches outside L_p, its contribution is |L_e1 \ L_p| and a
+// live neighbour of v outside L_p counts unless e1 is among its elements, so
+// nothing is scanned element-wide at all. Together these keep the large
+// separator cliques of a grid from being rescanned for every one of their
+// variables. Non-absorbed elements hold live variables only: eliminating a
+// variable absorbs every element that contains it.
 // ---------------------------------------------------------------------------
 AmdResult amd_order(i64 n, const i64* ap, const i64* ai) {
     AmdResult r;
@@ -52,7 +64,9 @@ AmdResult amd_order(i64 n, const i64* ap, const i64* ai) {
     std::vector<i64> live_nbrs(n);
     std::vector<std::vector<i64>> eadj(n), evars(n);
     std::vector<uint8_t> absorbed(n, 0);
-    std::vector<i64> marker(n, 0);
+    std::vector<i64> marker(n, 0);   // per-degree dedup stamps
+    std::vector<i64> in_lp(n, -1);   // == p: the variable belongs to L_p
+    std::vector<i64> ecnt_at(n, -1), ecnt(n, 0);  // |L_e \ L_p| of pivot ecnt_at[e]
     i64 cur = 0, nlive = n;
     using E = std::pair<i64, i64>;
     std::priority_queue<E, std::vector<E>, std::greater<E>> heap;
@@ -97,29 +111,63 @@ AmdResult amd_order(i64 n, const i64* ap, const i64* ai) {
         evars[p] = nvars;
         for (i64 q = vptr[p]; q < vptr[p + 1]; q++) live_nbrs[vadj[q]]--;
         for (i64 v : nvars) eadj[v].push_back(p);
+        // L_p membership, and |L_e \ L_p| once per element next to a member
+        // of L_p whose degree is computed below (the live neighbours of a
+        // variable adjacent to every live variable need no element lists)
+        for (i64 v : nvars) in_lp[v] = p;
+        for (i64 v : nvars) {
+            if (live_nbrs[v] == nlive - 1) continue;
+            auto& el = eadj[v];
+            size_t w = 0;
+            for (size_t q = 0; q < el.size(); q++) {
+                i64 e = el[q];
+                if (absorbed[e]) continue;
+                el[w++] = e;
+                if (e == p || ecnt_at[e] == p) continue;
+                ecnt_at[e] = p;
+                i64 outside = 0;
+                for (i64 a : evars[e]) outside += in_lp[a] != p;
+                ecnt[e] = outside;
+            }
+            el.resize(w);
+        }
+        const i64 lp_size = (i64)nvars.size();
         for (i64 v : nvars) {
             i64 deg;
             if (live_nbrs[v] == nlive - 1) {
                 deg = nlive - 1;
             } else {
-                i64 m2 = ++cur;
-                marker[v] = m2;
-                deg = 0;
-                for (i64 q = vptr[v]; q < vptr[v + 1]; q++) {
-                    i64 a = vadj[q];
-                    if (status[a] >= 0 && marker[a] != m2) { marker[a] = m2; deg++; }
+                deg = lp_size - 1;  // L_p without v
+                // the elements of v that reach outside L_p
+                i64 e1 = -1, nout = 0;
+                for (i64 e : eadj[v])  // pruned above
+                    if (e != p && ecnt[e] != 0) { e1 = e; nout++; }
+                if (nout <= 1) {
+                    // one such element: |L_e1 \ L_p| is known, and a live
+                    // neighbour outside L_p counts unless it is in L_e1
+                    if (nout == 1) deg += ecnt[e1];
+                    for (i64 q = vptr[v]; q < vptr[v + 1]; q++) {
+                        i64 a = vadj[q];
+                        if (status[a] < 0 || in_lp[a] == p) continue;
+                        bool in_e1 = false;
+                        if (nout == 1)
+                            for (i64 e : eadj[a])
+                                if (e == e1) { in_e1 = true; break; }
+                        deg += !in_e1;
+                    }
+                } else {
+                    i64 m2 = ++cur;
+                    marker[v] = m2;
+                    for (i64 q = vptr[v]; q < vptr[v + 1]; q++) {
+                        i64 a = vadj[q];
+                        if (status[a] >= 0 && in_lp[a] != p && marker[a] != m2) { marker[a] = m2; deg++; }
+                    }
+                    for (i64 e : eadj[v]) {
+                        if (e == p || ecnt[e] == 0) continue;  // L_p itself, or inside it
+                        for (i64 a : evars[e])
+                            if (in_lp[a] != p && marker[a] != m2) { marker[a] = m2; deg++; }
+                    }
                 }
-                auto& el = eadj[v];
-                size_t w = 0;
-                for (size_t q = 0; q < el.size(); q++) {
-                    i64 e = el[q];
-                    if (absorbed[e]) continue;
-                    el[w++] = e;
-                    auto& vs = evars[e];
-                    for (i64 a : vs)
-                        if (status[a] >= 0 && marker[a] != m2) { marker[a] = m2; deg++; }
-                }
-                el.resize(w);
             }
             status[v] = deg;
             heap.emplace(deg, v);

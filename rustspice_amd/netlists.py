@@ -397,6 +397,69 @@ def rc_mesh_inverters(nx: int = 256, ny: int = 256, taps: int | None = None,
     return b.build(3 + nx * ny, f"rcmesh{nx}x{ny}t{taps}")
 
 
+def power_grid(nx: int = 1024, ny: int = 1024, pitch: int = 64, loads: int | None = None,
+               seed: int = 0x52530000 + 3) -> Circuit:
+    """RC power grid: nx*ny nodes, segment R ~ U[0.05, 0.5] ohm, node C ~
+    U[1, 10] fF to ground, a 1.0 V Vdd pad every `pitch` nodes in both
+    directions, and `loads` current sources PULSE(0 10m td 0 0 20p) with td ~
+    U[0, 100] ps drawing current from distinct seeded nodes to ground
+    (default: one per 256 nodes, 4,096 at 1024x1024).
+
+    Nodes: 0 ground, grid node (i, j) = 1 + i*nx + j. Devices, in this order
+    (which makes first appearance row-major): the node capacitors (row-major),
+    the horizontal then the vertical segment resistors, the pads (row-major
+    over i, j multiples of `pitch`), then the loads in draw order. Draw order:
+    the capacitor values, the horizontal then vertical resistor values, then
+    per load one uniform for its node (a partial Fisher-Yates shuffle of the
+    grid nodes) and one for its delay. At 1024x1024 the MNA has n = 1,048,833
+    (ground, the nodes and one aux row per pad)."""
+    loads = max(1, nx * ny // 256) if loads is None else loads
+    rng = LcgStream(seed)
+    nn = nx * ny
+    nh, nv = ny * (nx - 1), (ny - 1) * nx
+    pads = [(i, j) for i in range(0, ny, pitch) for j in range(0, nx, pitch)]
+    arr = np.zeros(nn + nh + nv + len(pads) + loads, dtype=DEVICE_DTYPE)
+    node = np.arange(nn, dtype=np.int32).reshape(ny, nx) + 1
+    o = 0
+    cc = arr[o:o + nn]
+    cc["kind"] = RS_DEV_C
+    cc["node"][:, 0] = node.ravel()
+    cc["value"] = (1.0 + 9.0 * rng.uniforms(nn)) * F
+    o += nn
+    rh = arr[o:o + nh]
+    rh["kind"] = RS_DEV_R
+    rh["node"][:, 0] = node[:, :-1].ravel()
+    rh["node"][:, 1] = node[:, 1:].ravel()
+    rh["value"] = 0.05 + 0.45 * rng.uniforms(nh)
+    o += nh
+    rv = arr[o:o + nv]
+    rv["kind"] = RS_DEV_R
+    rv["node"][:, 0] = node[:-1, :].ravel()
+    rv["node"][:, 1] = node[1:, :].ravel()
+    rv["value"] = 0.05 + 0.45 * rng.uniforms(nv)
+    o += nv
+    for (i, j) in pads:
+        arr[o]["kind"] = RS_DEV_V
+        arr[o]["node"][0] = node[i, j]
+        arr[o]["value"] = 1.0
+        o += 1
+    pool = np.arange(nn, dtype=np.int64)
+    u = rng.uniforms(2 * loads).reshape(loads, 2)
+    for t in range(loads):
+        k = t + int(u[t, 0] * (nn - t))
+        pool[t], pool[k] = pool[k], pool[t]
+        arr[o]["kind"] = RS_DEV_I
+        arr[o]["flags"] = RS_F_WAVEFORM
+        arr[o]["node"][0] = 1 + pool[t]
+        arr[o]["p"][:7] = pulse_params(0.0, sfx(10, 1e-3), sfx(100, P) * u[t, 1], 0.0, 0.0, sfx(20, P), 0.0)
+        o += 1
+    return Circuit(arr, 1 + nn, f"pgrid{nx}x{ny}")
+
+
+#: 200 TRAP steps of 1 ps
+C3_TRAN = dict(dt=sfx(1, P), nsteps=200)
+
+
 #: 500 TRAP steps of 2 ps, checkpoint every 25, probe at the mesh centre
 C5_TRAN = dict(dt=sfx(2, P), nsteps=500, every=25)
 

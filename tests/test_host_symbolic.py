@@ -133,3 +133,40 @@ def test_stamp_pattern_matches_oracle_mna():
             ap, ai, _, _, n = O.build_mna(c, mode=mode, t=1e-12, dt=1e-12, tstate=ts)
             pa, pi = S.host_stamp_pattern(c, mode)
             assert np.array_equal(ap, pa) and np.array_equal(ai, pi), (c.name, mode)
+
+
+@pytest.mark.parametrize("nx", [32, 80])
+def test_amd_power_grid_matches_oracle(nx):
+    """The RC power grid's TRAN pattern (every node tied to ground through its
+    capacitor, pads with aux rows): the product's element shortcuts (an
+    element inside the new one skipped, a single outside element counted by
+    its size) give the literal restatement's order."""
+    c = NL.power_grid(nx, nx, pitch=16)
+    ap, ai = S.host_stamp_pattern(c, mode=1)
+    n = len(ap) - 1
+    pa, _ = O.amd_order(n, ap, ai)
+    pb, _ = S.host_amd_order(n, ap, ai)
+    assert np.array_equal(pa, pb)
+
+
+def test_amd_random_grids_with_cliques_match_oracle():
+    """Random sparse graphs plus a few dense rows and cliques, where elements
+    overlap L_p partly, wholly or not at all."""
+    rs = np.random.RandomState(11)
+    for _ in range(60):
+        n = int(rs.randint(20, 200))
+        M = rs.rand(n, n) < rs.rand() * 0.05
+        for _ in range(int(rs.randint(0, 4))):  # cliques
+            k = rs.choice(n, size=int(rs.randint(3, 12)), replace=False)
+            M[np.ix_(k, k)] = True
+        if rs.rand() < 0.5:  # a dense row/column, like ground
+            g = int(rs.randint(n))
+            M[g, :] = M[:, g] = True
+        M = M | M.T
+        np.fill_diagonal(M, True)
+        cols = [np.nonzero(M[:, j])[0] for j in range(n)]
+        ap = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int64)
+        ai = np.concatenate(cols).astype(np.int64)
+        pa, _ = O.amd_order(n, ap, ai)
+        pb, _ = S.host_amd_order(n, ap, ai)
+        assert np.array_equal(pa, pb)
