@@ -87,6 +87,19 @@ int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t *ap, const in
 int32_t rs_host_stamp_pattern(const rs_device *dev, int64_t ndev, int64_t node_count,
                               int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap, int64_t *ai);
 
+/* Supernodal schedule on the host (no reference counterpart): the SparseLU
+ * symbolic factor of (ap, ai) (sparse_lu.rs:302-437), its supernodal plan
+ * and, when ax is given, the factor of ax and the solve of rhs (x) executed
+ * in the order of the GPU kernels. lx/ux/ud receive the factor in the
+ * symbolic's CSC order. x, lx, ux, ud equal SparseLuSolver factor(ax);
+ * solve(rhs) bit for bit. Returns 6 when the pattern is not structurally
+ * symmetric. stats[10] = {supernodes, levels, contributions, panel doubles,
+ * widest supernode, longest R_J, multiply-subtracts of one factor, gather
+ * tiles, nnz(L), nnz(U)}. */
+int32_t rs_host_sn_selftest(int64_t n, const int64_t *ap, const int64_t *ai, const double *ax,
+                            const double *rhs, double *x, double *lx, double *ux, double *ud,
+                            int64_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

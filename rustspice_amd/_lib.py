@@ -28,7 +28,7 @@ EXPORTS = [
     "rs_solve_device", "rs_get_factor_values",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
-    "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest",
+    "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest",
     "rs_host_bbd_decompose", "rs_set_bbd_blocks", "rs_bbd_info",
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
     "rs_circuit_set_cap_state", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
@@ -129,6 +129,8 @@ def lib():
                                       i64p]
     L.rs_host_lp_selftest.restype = C.c_int32
     L.rs_host_lp_selftest.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, f64p, f64p, i64p]
+    L.rs_host_sn_selftest.restype = C.c_int32
+    L.rs_host_sn_selftest.argtypes = [C.c_int64, i64p, i64p, vp, vp, vp, vp, vp, vp, i64p]
     L.rs_profile_enable.argtypes = [C.c_int32]
     L.rs_profile_get.argtypes = [f64p, i64p]
     L.rs_set_poison.argtypes = [C.c_int32]

@@ -12,6 +12,7 @@
 #include "../../include/rsgpu_host.h"
 #include "bbd.hpp"
 #include "plan.hpp"
+#include "sn.hpp"
 #include "sp.hpp"
 #include "stamp.hpp"
 
@@ -375,6 +376,43 @@ int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
         return RS_OK;
     } catch (const std::exception&) {
         delete h;
+        return RS_ERR_INVALID_MATRIX;
+    }
+}
+
+int32_t rs_host_sn_selftest(int64_t n, const int64_t* ap, const int64_t* ai, const double* ax,
+                            const double* rhs, double* x, double* lx, double* ux, double* ud,
+                            int64_t* stats) {
+    try {
+        BlockSymbolic s;
+        block_symbolic(n, ap, ai, false, s);
+        SnHost h;
+        if (!build_sn_plan(s, ap, ai, h)) return RS_ERR_INVALID_MATRIX;
+        if (stats) {
+            stats[0] = h.ns;
+            stats[1] = h.nlev;
+            stats[2] = h.npairs;
+            stats[3] = h.nvals;
+            stats[4] = h.max_w;
+            stats[5] = h.max_h;
+            stats[6] = (int64_t)h.updates;
+            stats[7] = (int64_t)(h.gitems.size() / 4);
+            stats[8] = (int64_t)s.lri.size();
+            stats[9] = (int64_t)s.uri.size();
+        }
+        if (!ax) return RS_OK;  // plan statistics only
+        std::vector<double> v, d;
+        sn_host_factor(h, ax, v, d);
+        if (rhs && x) sn_host_solve(h, v, d, rhs, x);
+        if (lx && ux) {
+            std::vector<double> l, u;
+            sn_to_csc(h, s, v, l, u);
+            std::copy(l.begin(), l.end(), lx);
+            std::copy(u.begin(), u.end(), ux);
+        }
+        if (ud) std::copy(d.begin(), d.end(), ud);
+        return RS_OK;
+    } catch (const std::exception&) {
         return RS_ERR_INVALID_MATRIX;
     }
 }

@@ -1,0 +1,95 @@
+// sn.hpp - supernodal factor and solve of one large SparseLU matrix spread
+// over the whole GPU (the RC power grid's n = 1,048,833 TRAN matrix).
+//
+// The reference factor (sparse_lu.rs numeric_factor, restated in
+// native_klu.rs:1194-1382 with pivot_tol 0) computes every entry (i, k) of
+// L and U as
+//     x = 0.0 + A(i, k)  (duplicates in CSC order)
+//     x = x - L(i, j) * U(j, k)   for j ascending, skipped where U(j, k) == 0
+//     U(k, k) = x clamped to +-1e-14,  L(i, k) = x / U(k, k)
+// Any schedule that gives each entry the same terms in the same order
+// reproduces it bit for bit. This one groups the pivot columns into
+// fundamental supernodes J = [j0, j1) (consecutive columns whose L patterns
+// nest: L(:, c) = {c+1 .. j1-1} u R_J) and keeps each supernode's values as
+// dense panels:
+//   LP_J: (w + h) x w, column-major: rows J then R_J, columns J (the
+//         diagonal block holds L below and U on and above its diagonal);
+//   UP_J: w x h, column-major: U(J, R_J).
+// A pattern that is structurally symmetric (L(:, k) and U(k, :) hold the
+// same positions, as an MNA matrix does) makes every U row of J span R_J.
+//
+// Fan-in order per supernode J, level by level on the supernodal
+// elimination tree (J waits for every descendant):
+//   1. gather: for each descendant J' whose R_J' meets J, in ascending order,
+//      subtract L(rows, J') U(J', cols) from the entries of LP_J and UP_J it
+//      touches (each entry's j' ascending: the J' ranges are disjoint and
+//      increasing);
+//   2. factor: right-looking over J's own columns t = 0..w-1 (pivot clamp,
+//      L column divided by it, rank-one update of the later columns of LP_J
+//      and the later rows of UP_J, skipping zero multipliers).
+// Every entry thus receives its descendants' terms first (all smaller
+// indices), then J's own, each in ascending order.
+//
+// The solves follow the reference's column sweeps (solve_block_internal):
+// forward y[i] -= L(i, j) y[j] for j ascending; backward x[k] = y[k] / U(k,k),
+// then y[i] -= U(i, k) x[k] for k descending. Fan-in per supernode again:
+// forward, J's rows gather its descendants' columns (pairs ascending, then
+// J's own columns); backward, root first, J's rows gather R_J's x (columns
+// descending), then J's own columns descending.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "symbolic.hpp"
+
+namespace rsgpu {
+
+// Target tiles of the gather: kSnTile x kSnTile entries of LP_J or UP_J.
+constexpr int32_t kSnTile = 64;
+
+struct SnHost {
+    int32_t n = 0, ns = 0, nlev = 0;
+    int64_t nvals = 0;               // doubles in the panel pool
+    std::vector<int32_t> j0;         // ns + 1: supernode column ranges
+    std::vector<int32_t> rptr, rows; // R_J (sorted) of each supernode
+    std::vector<int64_t> oL, oU;     // panel offsets in the value pool
+    std::vector<int32_t> parent;     // supernodal elimination tree (-1: root)
+    std::vector<int32_t> lev_ptr, lev_sn;  // supernodes by level, ascending inside a level
+    // contributions to each target J, ascending J': J' (src), the range
+    // [a0, a1) of R_J' that falls in J, and rel[] (from rel_off) giving, for
+    // every t in [a0, |R_J'|), the row of LP_J holding R_J'[t]
+    std::vector<int32_t> cptr, csrc, ca0, ca1, crel;
+    std::vector<int64_t> crel_off;
+    std::vector<int32_t> rel;
+    // gather work items per level: {J, kind (0 LP, 1 UP), r0, c0} as int4
+    std::vector<int32_t> glev_ptr, gitems;
+    // scatter of A: every run of duplicate CSC entries -> panel slot
+    std::vector<int32_t> sc_src, sc_cnt;
+    std::vector<int64_t> sc_slot;
+    // position -> original index (the solve's permutations)
+    std::vector<int32_t> perm;       // perm[original] = position
+    // statistics
+    int64_t npairs = 0, max_w = 0, max_h = 0;
+    double updates = 0.0;            // multiply-subtracts of one factor (pattern count)
+};
+
+// Builds the supernodal plan of a single-block SparseLU symbolic factor
+// (positions = pivot steps, identity row pivots). Returns false, with a
+// reason, when the pattern is not structurally symmetric or exceeds the
+// 32-bit index ranges.
+bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost& h,
+                   std::string* why = nullptr);
+
+// Host execution of the plan in the device kernels' order: factor of ax,
+// then the solve of rhs into x; lx/ux/ud (optional) receive the factor in
+// the symbolic's CSC order. For checking the schedule against the reference
+// arithmetic on the CPU.
+void sn_host_factor(const SnHost& h, const double* ax, std::vector<double>& vals,
+                    std::vector<double>& ud);
+void sn_host_solve(const SnHost& h, const std::vector<double>& vals, const std::vector<double>& ud,
+                   const double* rhs, double* x);
+void sn_to_csc(const SnHost& h, const BlockSymbolic& s, const std::vector<double>& vals,
+               std::vector<double>& lx, std::vector<double>& ux);
+
+}  // namespace rsgpu

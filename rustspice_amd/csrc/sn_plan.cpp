@@ -1,0 +1,411 @@
+// sn_plan.cpp - host side of the supernodal factor (see sn.hpp): supernodes,
+// panel layout, the ordered descendant contributions of every supernode, the
+// levels of the supernodal elimination tree, the gather work items and the
+// scatter of A; plus a host execution of the same schedule for checking it
+// against the reference arithmetic on the CPU.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "sn.hpp"
+
+namespace rsgpu {
+
+namespace {
+
+// index of r in the sorted range [b, e), or -1
+int64_t find_sorted(const int32_t* b, const int32_t* e, int32_t r) {
+    const int32_t* it = std::lower_bound(b, e, r);
+    return (it != e && *it == r) ? (int64_t)(it - b) : -1;
+}
+
+bool fail(std::string* why, const char* m) {
+    if (why) *why = m;
+    return false;
+}
+
+}  // namespace
+
+bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost& h,
+                   std::string* why) {
+    h = SnHost();
+    const i64 n = s.n;
+    if (n <= 0) return fail(why, "empty matrix");
+    if (n >= (i64)std::numeric_limits<int32_t>::max() / 2) return fail(why, "n exceeds 32-bit indices");
+    h.n = (int32_t)n;
+    const std::vector<i64>& lcp = s.lcp;
+    const std::vector<i64>& lri = s.lri;
+    const std::vector<i64>& ucp = s.ucp;
+    const std::vector<i64>& uri = s.uri;
+
+    // 1. structural symmetry: the strictly upper part of U's row k must be
+    //    L(:, k)'s pattern (U's rows collected column by column come out
+    //    ascending, like lri)
+    {
+        std::vector<i64> cnt(n + 1, 0);
+        for (i64 c = 0; c < n; c++)
+            for (i64 q = ucp[c]; q < ucp[c + 1] - 1; q++) cnt[uri[q] + 1]++;
+        for (i64 k = 0; k < n; k++)
+            if (cnt[k + 1] != lcp[k + 1] - lcp[k]) return fail(why, "pattern is not structurally symmetric");
+        for (i64 k = 0; k < n; k++) cnt[k + 1] += cnt[k];
+        std::vector<i64> fill(cnt.begin(), cnt.end() - 1);
+        std::vector<int32_t> ucols((size_t)cnt[n]);
+        for (i64 c = 0; c < n; c++)
+            for (i64 q = ucp[c]; q < ucp[c + 1] - 1; q++) ucols[fill[uri[q]]++] = (int32_t)c;
+        for (i64 k = 0; k < n; k++)
+            for (i64 q = lcp[k], t = cnt[k]; q < lcp[k + 1]; q++, t++)
+                if (ucols[t] != (int32_t)lri[q]) return fail(why, "pattern is not structurally symmetric");
+    }
+
+    // 2. fundamental supernodes: column c + 1 continues c when L(:, c) is
+    //    {c + 1} u L(:, c + 1) and c is c + 1's only child
+    std::vector<int32_t> nchild(n, 0);
+    for (i64 c = 0; c < n; c++)
+        if (lcp[c + 1] > lcp[c]) nchild[lri[lcp[c]]]++;
+    h.j0.push_back(0);
+    for (i64 c = 1; c < n; c++) {
+        const i64 lp = lcp[c] - lcp[c - 1];
+        const bool cont = lp > 0 && lri[lcp[c - 1]] == c && lp == lcp[c + 1] - lcp[c] + 1 && nchild[c] == 1;
+        if (!cont) h.j0.push_back((int32_t)c);
+    }
+    h.j0.push_back((int32_t)n);
+    const int32_t ns = (int32_t)h.j0.size() - 1;
+    h.ns = ns;
+    std::vector<int32_t> sn_of(n);
+    for (int32_t J = 0; J < ns; J++)
+        for (int32_t c = h.j0[J]; c < h.j0[J + 1]; c++) sn_of[c] = J;
+
+    // 3. R_J (the L rows of J's last column) and the panel layout
+    h.rptr.assign(ns + 1, 0);
+    for (int32_t J = 0; J < ns; J++) {
+        const i64 lc = h.j0[J + 1] - 1;
+        h.rptr[J + 1] = h.rptr[J] + (int32_t)(lcp[lc + 1] - lcp[lc]);
+    }
+    h.rows.resize(h.rptr[ns]);
+    h.oL.resize(ns);
+    h.oU.resize(ns);
+    int64_t off = 0;
+    for (int32_t J = 0; J < ns; J++) {
+        const i64 lc = h.j0[J + 1] - 1;
+        for (i64 q = lcp[lc]; q < lcp[lc + 1]; q++) h.rows[h.rptr[J] + (q - lcp[lc])] = (int32_t)lri[q];
+        const int64_t w = h.j0[J + 1] - h.j0[J], hh = h.rptr[J + 1] - h.rptr[J];
+        h.max_w = std::max(h.max_w, w);
+        h.max_h = std::max(h.max_h, hh);
+        h.oL[J] = off;
+        off += (w + hh) * w;
+        h.oU[J] = off;
+        off += w * hh;
+    }
+    h.nvals = off;
+
+    // 4. supernodal elimination tree and its levels
+    h.parent.assign(ns, -1);
+    std::vector<int32_t> level(ns, 0);
+    for (int32_t J = 0; J < ns; J++) {
+        if (h.rptr[J + 1] > h.rptr[J]) {
+            const int32_t P = sn_of[h.rows[h.rptr[J]]];
+            if (P <= J) return fail(why, "supernodal tree is not ascending");
+            h.parent[J] = P;
+            level[P] = std::max(level[P], level[J] + 1);
+        }
+    }
+    h.nlev = ns ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+    h.lev_ptr.assign(h.nlev + 1, 0);
+    for (int32_t J = 0; J < ns; J++) h.lev_ptr[level[J] + 1]++;
+    for (int32_t l = 0; l < h.nlev; l++) h.lev_ptr[l + 1] += h.lev_ptr[l];
+    h.lev_sn.resize(ns);
+    {
+        std::vector<int32_t> fill(h.lev_ptr.begin(), h.lev_ptr.end() - 1);
+        for (int32_t J = 0; J < ns; J++) h.lev_sn[fill[level[J]]++] = J;
+    }
+
+    // 5. contributions: R_J' splits into runs of rows by the supernode they
+    //    fall in; each run makes a pair (J' -> J), listed per target in
+    //    ascending J' (J' is visited in ascending order)
+    std::vector<int32_t> npair(ns + 1, 0);
+    std::vector<int64_t> nrel(ns + 1, 0);
+    for (int32_t Jp = 0; Jp < ns; Jp++) {
+        const int32_t b = h.rptr[Jp], e = h.rptr[Jp + 1];
+        for (int32_t t = b; t < e;) {
+            const int32_t J = sn_of[h.rows[t]];
+            int32_t t2 = t + 1;
+            while (t2 < e && sn_of[h.rows[t2]] == J) t2++;
+            npair[J + 1]++;
+            nrel[J + 1] += e - t;
+            t = t2;
+        }
+    }
+    h.cptr.assign(ns + 1, 0);
+    std::vector<int64_t> rbase(ns + 1, 0);
+    for (int32_t J = 0; J < ns; J++) {
+        h.cptr[J + 1] = h.cptr[J] + npair[J + 1];
+        rbase[J + 1] = rbase[J] + nrel[J + 1];
+    }
+    h.npairs = h.cptr[ns];
+    if (rbase[ns] >= (int64_t)std::numeric_limits<int32_t>::max())
+        return fail(why, "relative row index table exceeds 32-bit offsets");
+    h.csrc.resize(h.npairs);
+    h.ca0.resize(h.npairs);
+    h.ca1.resize(h.npairs);
+    h.crel_off.resize(h.npairs);
+    h.rel.resize(rbase[ns]);
+    {
+        std::vector<int32_t> pf(h.cptr.begin(), h.cptr.end() - 1);
+        std::vector<int64_t> rf(rbase.begin(), rbase.end() - 1);
+        for (int32_t Jp = 0; Jp < ns; Jp++) {
+            const int32_t b = h.rptr[Jp], e = h.rptr[Jp + 1];
+            for (int32_t t = b; t < e;) {
+                const int32_t J = sn_of[h.rows[t]];
+                int32_t t2 = t + 1;
+                while (t2 < e && sn_of[h.rows[t2]] == J) t2++;
+                const int32_t pi = pf[J]++;
+                h.csrc[pi] = Jp;
+                h.ca0[pi] = t - b;
+                h.ca1[pi] = t2 - b;
+                h.crel_off[pi] = rf[J];
+                const int32_t jj0 = h.j0[J], jj1 = h.j0[J + 1], w = jj1 - jj0;
+                const int32_t* R = h.rows.data() + h.rptr[J];
+                const int32_t* RE = h.rows.data() + h.rptr[J + 1];
+                for (int32_t u = t; u < e; u++) {
+                    const int32_t r = h.rows[u];
+                    int64_t rr;
+                    if (r < jj1) {
+                        rr = r - jj0;
+                    } else {
+                        const int64_t x = find_sorted(R, RE, r);
+                        if (x < 0) return fail(why, "a descendant row lies outside its ancestor's panel");
+                        rr = w + x;
+                    }
+                    h.rel[rf[J]++] = (int32_t)rr;
+                }
+                t = t2;
+            }
+        }
+    }
+
+    // 6. gather work items: tiles of LP_J and UP_J of supernodes with
+    //    contributions, level by level
+    h.glev_ptr.assign(h.nlev + 1, 0);
+    for (int32_t l = 0; l < h.nlev; l++) {
+        for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
+            const int32_t J = h.lev_sn[q];
+            if (h.cptr[J + 1] == h.cptr[J]) continue;
+            const int32_t w = h.j0[J + 1] - h.j0[J], hh = h.rptr[J + 1] - h.rptr[J];
+            for (int32_t c0 = 0; c0 < w; c0 += kSnTile)
+                for (int32_t r0 = 0; r0 < w + hh; r0 += kSnTile) {
+                    h.gitems.push_back(J);
+                    h.gitems.push_back(0);
+                    h.gitems.push_back(r0);
+                    h.gitems.push_back(c0);
+                }
+            for (int32_t c0 = 0; c0 < hh; c0 += kSnTile)
+                for (int32_t r0 = 0; r0 < w; r0 += kSnTile) {
+                    h.gitems.push_back(J);
+                    h.gitems.push_back(1);
+                    h.gitems.push_back(r0);
+                    h.gitems.push_back(c0);
+                }
+        }
+        h.glev_ptr[l + 1] = (int32_t)(h.gitems.size() / 4);
+    }
+
+    // 7. scatter of A: runs of duplicate rows of each column, in the column
+    //    order of the pivot steps, onto the panel slot of (position, step)
+    auto slot_of = [&](int32_t p, int32_t k) -> int64_t {
+        const int32_t J = sn_of[k];
+        const int32_t jj0 = h.j0[J], jj1 = h.j0[J + 1], w = jj1 - jj0;
+        const int32_t hh = h.rptr[J + 1] - h.rptr[J];
+        if (p >= jj0) {
+            int64_t rr;
+            if (p < jj1) {
+                rr = p - jj0;
+            } else {
+                rr = find_sorted(h.rows.data() + h.rptr[J], h.rows.data() + h.rptr[J + 1], p);
+                if (rr < 0) return -1;
+                rr += w;
+            }
+            return h.oL[J] + (int64_t)(k - jj0) * (w + hh) + rr;
+        }
+        const int32_t Jp = sn_of[p];
+        const int32_t wp = h.j0[Jp + 1] - h.j0[Jp];
+        const int64_t c = find_sorted(h.rows.data() + h.rptr[Jp], h.rows.data() + h.rptr[Jp + 1], k);
+        if (c < 0) return -1;
+        return h.oU[Jp] + c * wp + (p - h.j0[Jp]);
+    };
+    for (i64 k = 0; k < n; k++) {
+        const i64 oc = s.col_perm_inv[k];
+        i64 q = ap[oc];
+        while (q < ap[oc + 1]) {
+            const i64 orow = ai[q];
+            i64 q2 = q + 1;
+            while (q2 < ap[oc + 1] && ai[q2] == orow) q2++;
+            if (orow < n) {
+                const int64_t sl = slot_of((int32_t)s.col_perm[orow], (int32_t)k);
+                if (sl >= 0) {
+                    if (q > std::numeric_limits<int32_t>::max() - 1) return fail(why, "nnz exceeds 32-bit offsets");
+                    h.sc_src.push_back((int32_t)q);
+                    h.sc_cnt.push_back((int32_t)(q2 - q));
+                    h.sc_slot.push_back(sl);
+                }
+            }
+            q = q2;
+        }
+    }
+    h.perm.resize(n);
+    for (i64 i = 0; i < n; i++) h.perm[i] = (int32_t)s.col_perm[i];
+
+    // 8. algorithmic work: every U(j, k), j < k, applies L(:, j)
+    double upd = 0.0;
+    for (i64 k = 0; k < n; k++)
+        for (i64 q = ucp[k]; q < ucp[k + 1] - 1; q++) upd += (double)(lcp[uri[q] + 1] - lcp[uri[q]]);
+    h.updates = upd;
+    return true;
+}
+
+// ---- host execution of the schedule ------------------------------------------
+
+namespace {
+
+struct Panel {
+    const SnHost& h;
+    int32_t J, jj0, w, hh;
+    int64_t oL, oU;
+    Panel(const SnHost& hs, int32_t j)
+        : h(hs), J(j), jj0(hs.j0[j]), w(hs.j0[j + 1] - hs.j0[j]), hh(hs.rptr[j + 1] - hs.rptr[j]),
+          oL(hs.oL[j]), oU(hs.oU[j]) {}
+    int64_t lp(int64_t r, int64_t c) const { return oL + c * (w + hh) + r; }
+    int64_t up(int64_t r, int64_t c) const { return oU + c * w + r; }
+};
+
+}  // namespace
+
+void sn_host_factor(const SnHost& h, const double* ax, std::vector<double>& v, std::vector<double>& ud) {
+    v.assign((size_t)h.nvals, 0.0);
+    ud.assign(h.n, 0.0);
+    for (size_t s = 0; s < h.sc_src.size(); s++) {
+        double x = 0.0;
+        for (int32_t q = 0; q < h.sc_cnt[s]; q++) x += ax[h.sc_src[s] + q];
+        v[h.sc_slot[s]] = x;
+    }
+    for (int32_t l = 0; l < h.nlev; l++) {
+        for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
+            const Panel P(h, h.lev_sn[q]);
+            // gather, contributions in ascending order
+            for (int32_t pi = h.cptr[P.J]; pi < h.cptr[P.J + 1]; pi++) {
+                const Panel S(h, h.csrc[pi]);
+                const int32_t a0 = h.ca0[pi], a1 = h.ca1[pi];
+                const int32_t* rel = h.rel.data() + h.crel_off[pi] - a0;  // rel[t] for t >= a0
+                for (int32_t ci = a0; ci < a1; ci++) {  // columns of LP_J
+                    const int32_t col = h.rows[h.rptr[S.J] + ci] - P.jj0;
+                    for (int32_t ri = a0; ri < S.hh; ri++) {
+                        double x = v[P.lp(rel[ri], col)];
+                        for (int32_t t = 0; t < S.w; t++) {
+                            const double u = v[S.up(t, ci)];
+                            if (u != 0.0) x -= v[S.lp(S.w + ri, t)] * u;
+                        }
+                        v[P.lp(rel[ri], col)] = x;
+                    }
+                }
+                for (int32_t ci = a0; ci < a1; ci++) {  // rows of UP_J
+                    const int32_t row = h.rows[h.rptr[S.J] + ci] - P.jj0;
+                    for (int32_t ri = a1; ri < S.hh; ri++) {
+                        const int32_t col = rel[ri] - P.w;
+                        double x = v[P.up(row, col)];
+                        for (int32_t t = 0; t < S.w; t++) {
+                            const double u = v[S.up(t, ri)];
+                            if (u != 0.0) x -= v[S.lp(S.w + ci, t)] * u;
+                        }
+                        v[P.up(row, col)] = x;
+                    }
+                }
+            }
+            // J's own columns, right-looking
+            const int32_t W = P.w + P.hh;
+            for (int32_t t = 0; t < P.w; t++) {
+                double d = v[P.lp(t, t)];
+                if (std::fabs(d) < 1e-14) d = d >= 0.0 ? 1e-14 : -1e-14;
+                v[P.lp(t, t)] = d;
+                ud[P.jj0 + t] = d;
+                for (int32_t r = t + 1; r < W; r++) v[P.lp(r, t)] = v[P.lp(r, t)] / d;
+                for (int32_t c = t + 1; c < P.w; c++) {
+                    const double u = v[P.lp(t, c)];
+                    if (u == 0.0) continue;
+                    for (int32_t r = t + 1; r < W; r++) v[P.lp(r, c)] -= v[P.lp(r, t)] * u;
+                }
+                for (int32_t c = 0; c < P.hh; c++) {
+                    const double u = v[P.up(t, c)];
+                    if (u == 0.0) continue;
+                    for (int32_t r = t + 1; r < P.w; r++) v[P.up(r, c)] -= v[P.lp(r, t)] * u;
+                }
+            }
+        }
+    }
+}
+
+void sn_host_solve(const SnHost& h, const std::vector<double>& v, const std::vector<double>& ud,
+                   const double* rhs, double* xout) {
+    std::vector<double> y(h.n);
+    for (int32_t i = 0; i < h.n; i++) y[h.perm[i]] = rhs[i];
+    for (int32_t l = 0; l < h.nlev; l++)
+        for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
+            const Panel P(h, h.lev_sn[q]);
+            for (int32_t pi = h.cptr[P.J]; pi < h.cptr[P.J + 1]; pi++) {
+                const Panel S(h, h.csrc[pi]);
+                for (int32_t ci = h.ca0[pi]; ci < h.ca1[pi]; ci++) {
+                    const int32_t row = h.rows[h.rptr[S.J] + ci];
+                    double x = y[row];
+                    for (int32_t t = 0; t < S.w; t++) x -= v[S.lp(S.w + ci, t)] * y[S.jj0 + t];
+                    y[row] = x;
+                }
+            }
+            for (int32_t t = 0; t < P.w; t++)
+                for (int32_t r = t + 1; r < P.w; r++) y[P.jj0 + r] -= v[P.lp(r, t)] * y[P.jj0 + t];
+        }
+    for (int32_t l = h.nlev - 1; l >= 0; l--)
+        for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
+            const Panel P(h, h.lev_sn[q]);
+            const int32_t* R = h.rows.data() + h.rptr[P.J];
+            for (int32_t r = 0; r < P.w; r++) {
+                double x = y[P.jj0 + r];
+                for (int32_t c = P.hh - 1; c >= 0; c--) x -= v[P.up(r, c)] * y[R[c]];
+                y[P.jj0 + r] = x;
+            }
+            for (int32_t t = P.w - 1; t >= 0; t--) {
+                y[P.jj0 + t] /= ud[P.jj0 + t];
+                const double xt = y[P.jj0 + t];
+                for (int32_t r = 0; r < t; r++) y[P.jj0 + r] -= v[P.lp(r, t)] * xt;
+            }
+        }
+    for (int32_t i = 0; i < h.n; i++) xout[i] = y[h.perm[i]];
+}
+
+void sn_to_csc(const SnHost& h, const BlockSymbolic& s, const std::vector<double>& v,
+               std::vector<double>& lx, std::vector<double>& ux) {
+    const i64 n = s.n;
+    std::vector<int32_t> sn_of(n);
+    for (int32_t J = 0; J < h.ns; J++)
+        for (int32_t c = h.j0[J]; c < h.j0[J + 1]; c++) sn_of[c] = J;
+    lx.assign(s.lri.size(), 0.0);
+    ux.assign(s.uri.size(), 0.0);
+    for (i64 k = 0; k < n; k++) {
+        const Panel P(h, sn_of[k]);
+        const int32_t* R = h.rows.data() + h.rptr[P.J];
+        for (i64 q = s.lcp[k]; q < s.lcp[k + 1]; q++) {
+            const int32_t r = (int32_t)s.lri[q];
+            const int64_t rr = r < P.jj0 + P.w ? r - P.jj0 : P.w + find_sorted(R, R + P.hh, r);
+            lx[q] = v[P.lp(rr, k - P.jj0)];
+        }
+        for (i64 q = s.ucp[k]; q < s.ucp[k + 1]; q++) {
+            const int32_t r = (int32_t)s.uri[q];
+            if (r >= P.jj0) {
+                ux[q] = v[P.lp(r - P.jj0, k - P.jj0)];
+            } else {
+                const Panel S(h, sn_of[r]);
+                const int32_t* RS = h.rows.data() + h.rptr[S.J];
+                ux[q] = v[S.up(r - S.jj0, find_sorted(RS, RS + S.hh, (int32_t)k))];
+            }
+        }
+    }
+}
+
+}  // namespace rsgpu

@@ -302,3 +302,31 @@ def host_bbd_decompose(n, ap, ai, num_blocks):
     nb = L.rs_host_bbd_decompose(n, _i64(ap), _i64(ai), num_blocks, perm, bp, C.byref(b0))
     return dict(num_blocks=int(nb), perm=perm[:n], block_ptr=bp[: nb + 1], border_start=b0.value,
                 border_size=n - b0.value if nb > 1 else 0)
+
+
+def host_sn_selftest(n, ap, ai, ax=None, rhs=None):
+    """Host execution of the supernodal schedule (the order of the GPU
+    kernels): SparseLU symbolic factor, supernodal plan and, with ax, the
+    factor of ax and the solve of rhs. Returns (x, lx, ux, ud, stats); the
+    arrays are None when ax is None."""
+    L = _lib.lib()
+    st = np.zeros(10, np.int64)
+    ap, ai = _i64(ap), _i64(ai)
+    rc = L.rs_host_sn_selftest(n, ap, ai, None, None, None, None, None, None, st)
+    if rc:
+        raise SolverError(rc, "no supernodal plan")
+    keys = ["supernodes", "levels", "contributions", "panel_doubles", "max_w", "max_h", "updates",
+            "gather_tiles", "nnz_l", "nnz_u"]
+    stats = dict(zip(keys, map(int, st)))
+    if ax is None:
+        return None, None, None, None, stats
+    ax = _f64(ax)
+    rhs = _f64(rhs if rhs is not None else np.zeros(n))
+    x = np.zeros(max(n, 1))
+    lx, ux = np.zeros(max(stats["nnz_l"], 1)), np.zeros(max(stats["nnz_u"], 1))
+    ud = np.zeros(max(n, 1))
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    rc = L.rs_host_sn_selftest(n, ap, ai, p(ax), p(rhs), p(x), p(lx), p(ux), p(ud), st)
+    if rc:
+        raise SolverError(rc, "no supernodal plan")
+    return x[:n], lx[: stats["nnz_l"]], ux[: stats["nnz_u"]], ud[:n], stats
