@@ -338,6 +338,20 @@ int32_t rs_batch_phase_cycles(rs_batch *b, int64_t *out);
  * Enabling (or disabling) clears the totals. */
 void rs_profile_enable(int32_t on);
 void rs_profile_get(double *ms_total, int64_t *counts);
+/* Supernodal factor and solve (no reference counterpart: a schedule of the
+ * same arithmetic). mode 0: off; 1 (default): SparseLU patterns (and a
+ * BbdSolver's SparseLU fallback) of n >= 20,000 whose pattern is
+ * structurally symmetric, as an MNA matrix without MOSFETs is; 2: every
+ * structurally symmetric SparseLU pattern. Such a pattern is factored on the
+ * device from its first factor on: every entry of L and U receives the
+ * reference's terms in the reference's order (sparse_lu.rs numeric_factor),
+ * grouped into dense supernode panels that the levels of the supernodal
+ * elimination tree gather and factor in parallel; the solves run the same
+ * levels. Takes effect at the next analysis. rs_supernodal_info: out[8] =
+ * {active, supernodes, levels, contributions, panel doubles, device bytes,
+ * multiply-subtracts per factor, widest supernode}. */
+void rs_set_supernodal(rs_solver *h, int32_t mode);
+int32_t rs_supernodal_info(const rs_solver *h, int64_t *out);
 /* Checking aid (no reference counterpart): with on != 0, every FP64 device
  * buffer the library allocates afterwards starts filled with NaN, so a value
  * read before it is written changes the result visibly instead of reading

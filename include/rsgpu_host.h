@@ -93,12 +93,14 @@ int32_t rs_host_stamp_pattern(const rs_device *dev, int64_t ndev, int64_t node_c
  * in the order of the GPU kernels. lx/ux/ud receive the factor in the
  * symbolic's CSC order. x, lx, ux, ud equal SparseLuSolver factor(ax);
  * solve(rhs) bit for bit. Returns 6 when the pattern is not structurally
- * symmetric. stats[10] = {supernodes, levels, contributions, panel doubles,
- * widest supernode, longest R_J, multiply-subtracts of one factor, gather
- * tiles, nnz(L), nnz(U)}. */
+ * symmetric. hot_min > 0: the count of supernodes whose R must hold a position
+ * for its diagonal entry to be summed on its own (default 256). stats[12] =
+ * {supernodes, levels, contributions, panel doubles, widest supernode,
+ * longest R_J, multiply-subtracts of one factor, gather tiles, nnz(L),
+ * nnz(U), entries summed on their own, tile records}. */
 int32_t rs_host_sn_selftest(int64_t n, const int64_t *ap, const int64_t *ai, const double *ax,
                             const double *rhs, double *x, double *lx, double *ux, double *ud,
-                            int64_t *stats);
+                            int64_t hot_min, int64_t *stats);
 
 #ifdef __cplusplus
 }

@@ -36,6 +36,10 @@ void orc_btf_decompose(int64_t n, const int64_t *ap, const int64_t *ai, orc_btf 
 void orc_btf_free(orc_btf *b);
 
 /* ---- NativeKLU (crates/sim-core/src/native_klu.rs) --------------------- */
+/* Both solver structs start with a uint32_t tag, so the Newton drivers
+ * (engine.rs with its Box<dyn LinearSolver>) take either as a void pointer. */
+#define ORC_MAGIC_KLU 0x4B4C5530u
+#define ORC_MAGIC_BBD 0x42424430u
 typedef struct orc_klu orc_klu;
 orc_klu *orc_klu_new(int64_t n);
 void orc_klu_free(orc_klu *s);
@@ -143,7 +147,7 @@ typedef struct orc_newton_result {
  * (the full vector incl. aux, newton.rs:98), otherwise the node_count entries
  * of the x the loop ended with (newton.rs:93-107); *x_len the count written. */
 int orc_dc_op(const rs_device *dev, int64_t ndev, int64_t node_count,
-              const rs_newton_opts *o, orc_klu *solver, double *x_out,
+              const rs_newton_opts *o, void *solver, double *x_out,
               int64_t x_cap, int64_t *x_len, orc_newton_result *res);
 
 typedef struct orc_tran_result {
@@ -156,14 +160,14 @@ typedef struct orc_tran_result {
 /* TRAN (engine.rs:333-632). times[npoints], sols[npoints*x_stride] */
 int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count,
              const rs_newton_opts *o, double tstep, double tstop, double tstart,
-             double tmax, orc_klu *solver, int64_t max_points, double *times,
+             double tmax, void *solver, int64_t max_points, double *times,
              double *sols, int64_t x_stride, orc_tran_result *res);
 
 /* Fixed-step trapezoidal transient (no reference counterpart): one TRAP
  * Newton solve per step of dt, state updated as on an accepted step. */
 int orc_tran_fixed(const rs_device *dev, int64_t ndev, int64_t node_count,
                    const rs_newton_opts *o, double dt, int64_t nsteps,
-                   orc_klu *solver, double *x_final, int64_t x_cap,
+                   void *solver, double *x_final, int64_t x_cap,
                    int64_t *x_len, int32_t *newton_iters_total);
 
 /* Sweep points of engine.rs:715-741; returns the count (values may be NULL). */
@@ -178,7 +182,7 @@ typedef struct {
  * sols[nsolved * x_stride]. */
 int orc_dc_sweep(const rs_device *dev, int64_t ndev, int64_t node_count,
                  const rs_newton_opts *o, int64_t source, double start, double stop,
-                 double step, orc_klu *solver, int64_t max_points, double *sols,
+                 double step, void *solver, int64_t max_points, double *sols,
                  int64_t x_stride, orc_sweep_result *res);
 
 /* analysis.rs:147-200: out = {error_norm, max_node, accept, factor, raw_lte} */

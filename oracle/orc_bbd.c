@@ -323,6 +323,7 @@ typedef struct {
 } sblk;
 
 struct orc_bbd {
+    uint32_t magic; /* ORC_MAGIC_BBD: the engine dispatches on it */
     int64_t n, target;
     int analyzed, factored, has_decomp;
     orc_bbd_decomp d;
@@ -367,6 +368,7 @@ static void bbd_clear(orc_bbd *s) {
 
 orc_bbd *orc_bbd_new(int64_t n, int64_t num_blocks) {
     orc_bbd *s = (orc_bbd *)orc_xcalloc(1, sizeof(orc_bbd));
+    s->magic = ORC_MAGIC_BBD;
     s->n = n;
     s->target = num_blocks > 2 ? num_blocks : 2;
     return s;

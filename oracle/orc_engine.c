@@ -17,6 +17,24 @@ typedef struct {
     const orc_tstate *st;
 } build_ctx;
 
+/* LinearSolver dispatch (solver.rs:141-166) over the two oracle solvers */
+static int sv_bbd(const void *s) { return *(const uint32_t *)s == ORC_MAGIC_BBD; }
+static void sv_prepare(void *s, int64_t n) {
+    if (sv_bbd(s)) orc_bbd_prepare((orc_bbd *)s, n); else orc_klu_prepare((orc_klu *)s, n);
+}
+static int sv_analyze(void *s, int64_t apl, const int64_t *ap, int64_t ail, const int64_t *ai) {
+    return sv_bbd(s) ? orc_bbd_analyze((orc_bbd *)s, apl, ap, ail, ai)
+                     : orc_klu_analyze((orc_klu *)s, apl, ap, ail, ai);
+}
+static int sv_factor(void *s, int64_t apl, const int64_t *ap, int64_t ail, const int64_t *ai,
+                     const double *ax) {
+    return sv_bbd(s) ? orc_bbd_factor((orc_bbd *)s, apl, ap, ail, ai, ax)
+                     : orc_klu_factor((orc_klu *)s, apl, ap, ail, ai, ax);
+}
+static int sv_solve(void *s, int64_t n, double *rhs) {
+    return sv_bbd(s) ? orc_bbd_solve((orc_bbd *)s, n, rhs) : orc_klu_solve((orc_klu *)s, n, rhs);
+}
+
 static double norm2(const double *v, int64_t n) {
     double s = 0.0;
     for (int64_t i = 0; i < n; i++) s += v[i] * v[i];
@@ -27,7 +45,7 @@ static double norm2(const double *v, int64_t n) {
  * convergence it is replaced by the full solution (newton.rs:98). */
 static orc_newton_result run_newton(const rs_newton_opts *o, double **x, int64_t *xl,
                                     const build_ctx *bc, double gmin, double source_scale,
-                                    orc_klu *solver, int32_t *factor_calls) {
+                                    void *solver, int32_t *factor_calls) {
     orc_newton_result r = {0, 0, 1, 0.0};
     double damping = o->damping;
     double prev_dx_norm = 1.7976931348623157e308;
@@ -36,10 +54,10 @@ static orc_newton_result run_newton(const rs_newton_opts *o, double **x, int64_t
         orc_mna m;
         orc_build_mna(bc->dev, bc->ndev, bc->node_count, *x, *xl, bc->mode, bc->t, bc->dt,
                       bc->st, gmin, source_scale, &m);
-        orc_klu_prepare(solver, m.n);
-        int err = orc_klu_analyze(solver, m.n + 1, m.ap, m.nnz, m.ai);
-        if (!err) { err = orc_klu_factor(solver, m.n + 1, m.ap, m.nnz, m.ai, m.ax); if (factor_calls) (*factor_calls)++; }
-        if (!err) err = orc_klu_solve(solver, m.n, m.rhs);
+        sv_prepare(solver, m.n);
+        int err = sv_analyze(solver, m.n + 1, m.ap, m.nnz, m.ai);
+        if (!err) { err = sv_factor(solver, m.n + 1, m.ap, m.nnz, m.ai, m.ax); if (factor_calls) (*factor_calls)++; }
+        if (!err) err = sv_solve(solver, m.n, m.rhs);
         if (err) { r.reason = 2; orc_mna_free(&m); break; }
         double *xn = m.rhs;
         int64_t nn = m.n;
@@ -75,7 +93,7 @@ static orc_newton_result run_newton(const rs_newton_opts *o, double **x, int64_t
 
 /* run_newton_with_stepping (newton.rs:142-185) */
 static orc_newton_result run_newton_stepping(const rs_newton_opts *o, double **x, int64_t *xl,
-                                             const build_ctx *bc, orc_klu *solver,
+                                             const build_ctx *bc, void *solver,
                                              int32_t *factor_calls, int32_t *iters_total) {
     double gstart = o->gmin * 1e3; if (gstart < 1e-6) gstart = 1e-6;
     orc_newton_result last = {0, 0, 1, 0.0};
@@ -94,11 +112,11 @@ static orc_newton_result run_newton_stepping(const rs_newton_opts *o, double **x
 }
 
 int orc_dc_op(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_newton_opts *o,
-              orc_klu *solver, double *x_out, int64_t x_cap, int64_t *x_len,
+              void *solver, double *x_out, int64_t x_cap, int64_t *x_len,
               orc_newton_result *res) {
     int64_t xl = node_count;
     double *x = (double *)orc_xcalloc((size_t)(xl ? xl : 1), sizeof(double));
-    orc_klu_prepare(solver, node_count);
+    sv_prepare(solver, node_count);
     build_ctx bc = {dev, ndev, node_count, ORC_MODE_DC, 0.0, 0.0, NULL};
     *res = run_newton_stepping(o, &x, &xl, &bc, solver, NULL, NULL);
     int64_t cpy = xl < x_cap ? xl : x_cap;
@@ -224,7 +242,7 @@ static void tstate_update_full(const rs_device *dev, int64_t ndev, const double 
 }
 
 int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_newton_opts *o,
-             double tstep, double tstop, double tstart, double tmax, orc_klu *solver,
+             double tstep, double tstop, double tstart, double tmax, void *solver,
              int64_t max_points, double *times, double *sols, int64_t x_stride,
              orc_tran_result *res) {
     memset(res, 0, sizeof(*res));
@@ -233,7 +251,7 @@ int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_ne
     orc_tstate sbe, str;
     tstate_init(&sbe, ndev, ORC_BE);
     tstate_init(&str, ndev, ORC_TRAP);
-    orc_klu_prepare(solver, node_count);
+    sv_prepare(solver, node_count);
     double min_dt = tstep * 1e-6; if (min_dt < 1e-15) min_dt = 1e-15;
     double max_dt = tmax < tstop / 10.0 ? tmax : tstop / 10.0;
     double abs_tol = o->abs_tol, rel_tol = o->rel_tol;
@@ -362,13 +380,13 @@ int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_ne
 }
 
 int orc_tran_fixed(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_newton_opts *o,
-                   double dt, int64_t nsteps, orc_klu *solver, double *x_final, int64_t x_cap,
+                   double dt, int64_t nsteps, void *solver, double *x_final, int64_t x_cap,
                    int64_t *x_len, int32_t *newton_iters_total) {
     int64_t xl = node_count;
     double *x = (double *)orc_xcalloc((size_t)(xl ? xl : 1), sizeof(double));
     orc_tstate str;
     tstate_init(&str, ndev, ORC_TRAP);
-    orc_klu_prepare(solver, node_count);
+    sv_prepare(solver, node_count);
     build_ctx bc = {dev, ndev, node_count, ORC_MODE_DC, 0.0, 0.0, NULL};
     int32_t it = 0;
     orc_newton_result r = run_newton_stepping(o, &x, &xl, &bc, solver, NULL, &it);
@@ -423,7 +441,7 @@ int64_t orc_dc_sweep_points(double start, double stop, double step, double *valu
  * parses as a plain number afterwards), stop at the first failure. */
 int orc_dc_sweep(const rs_device *dev, int64_t ndev, int64_t node_count,
                  const rs_newton_opts *o, int64_t source, double start, double stop,
-                 double step, orc_klu *solver, int64_t max_points, double *sols,
+                 double step, void *solver, int64_t max_points, double *sols,
                  int64_t x_stride, orc_sweep_result *res) {
     memset(res, 0, sizeof(*res));
     if (source < 0 || source >= ndev) { res->status = 3; return 3; }
@@ -436,7 +454,7 @@ int orc_dc_sweep(const rs_device *dev, int64_t ndev, int64_t node_count,
     d[source].flags &= ~RS_F_WAVEFORM;
     int64_t xl = node_count;
     double *x = (double *)orc_xcalloc((size_t)(xl ? xl : 1), sizeof(double));
-    orc_klu_prepare(solver, node_count);
+    sv_prepare(solver, node_count);
     build_ctx bc = {d, ndev, node_count, ORC_MODE_DC, 0.0, 0.0, NULL};
     for (int64_t k = 0; k < np; k++) {
         d[source].value = pts[k];

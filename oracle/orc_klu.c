@@ -29,6 +29,7 @@ typedef struct {
 enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
 
 struct orc_klu {
+    uint32_t magic; /* ORC_MAGIC_KLU: the engine dispatches on it */
     int64_t n;
     double pivot_tol;
     int ordering, use_btf;
@@ -71,6 +72,7 @@ static void clear_factors(orc_klu *s) {
 
 orc_klu *orc_klu_new(int64_t n) {
     orc_klu *s = (orc_klu *)orc_xcalloc(1, sizeof(orc_klu));
+    s->magic = ORC_MAGIC_KLU;
     s->n = n;
     s->pivot_tol = 0.001; s->ordering = 0; s->use_btf = 1; s->btf_min_blocks = 2;
     s->state = ST_EMPTY;

@@ -23,7 +23,7 @@ vp = C.c_void_p
 EXPORTS = [
     "rs_solver_create", "rs_solver_destroy", "rs_prepare", "rs_analyze", "rs_factor",
     "rs_solve", "rs_reset_pattern", "rs_solver_name", "rs_set_parallel_threads",
-    "rs_set_config", "rs_get_stats", "rs_set_autotune", "rs_set_growth_check", "rs_growth_fallbacks",
+    "rs_set_config", "rs_get_stats", "rs_set_autotune", "rs_set_supernodal", "rs_supernodal_info", "rs_set_growth_check", "rs_growth_fallbacks",
     "rs_last_growth", "rs_set_launch_shape", "rs_launch_shape", "rs_last_error", "rs_refactor_device",
     "rs_solve_device", "rs_get_factor_values",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
@@ -81,6 +81,8 @@ def lib():
     L.rs_set_parallel_threads.argtypes = [vp, C.c_int64]
     L.rs_set_config.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
     L.rs_set_autotune.argtypes = [vp, C.c_int32]
+    L.rs_set_supernodal.argtypes = [vp, C.c_int32]
+    L.rs_supernodal_info.argtypes = [vp, i64p]
     L.rs_set_growth_check.restype = None
     L.rs_set_growth_check.argtypes = [vp, C.c_double]
     L.rs_growth_fallbacks.restype = C.c_int64
@@ -130,7 +132,7 @@ def lib():
     L.rs_host_lp_selftest.restype = C.c_int32
     L.rs_host_lp_selftest.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, f64p, f64p, i64p]
     L.rs_host_sn_selftest.restype = C.c_int32
-    L.rs_host_sn_selftest.argtypes = [C.c_int64, i64p, i64p, vp, vp, vp, vp, vp, vp, i64p]
+    L.rs_host_sn_selftest.argtypes = [C.c_int64, i64p, i64p, vp, vp, vp, vp, vp, vp, C.c_int64, i64p]
     L.rs_profile_enable.argtypes = [C.c_int32]
     L.rs_profile_get.argtypes = [f64p, i64p]
     L.rs_set_poison.argtypes = [C.c_int32]

@@ -217,6 +217,7 @@ void bbd_decompose(int64_t n, const int64_t* ap, const int64_t* ai, int64_t num_
 // ---------------------------------------------------------------------------
 struct BbdImpl {
     int64_t target = 4;  // SolverType::Bbd hard-wires 4 blocks (solver.rs:169-200)
+    int sn_mode = 1;     // supernodal mode of the SparseLU fallback
     bool analyzed = false, factored = false, has_decomp = false, dense = true;
     BbdDecomp d;
     rs_solver* fallback = nullptr;  // SparseLU on the whole matrix
@@ -296,6 +297,7 @@ int32_t setup_fallback(rs_solver* h, const int64_t* ap, int64_t apl, const int64
     BbdImpl* b = h->bbd;
     clear_impl(b);
     b->fallback = rs_solver_create(RS_SOLVER_SPARSELU, h->n);
+    rs_set_supernodal(b->fallback, b->sn_mode);
     rs_prepare(b->fallback, h->n);
     int32_t rc = rs_analyze(b->fallback, ap, apl, ai, ail);
     if (rc) {
@@ -648,6 +650,55 @@ int32_t bbd_solve(rs_solver* h, double* rhs) {
         (e = hipMemcpy(rhs, b->d_rhs.p, n * sizeof(double), hipMemcpyDeviceToHost)))
         return bhip(h, e, "BBD solve (back-substitution)");
     return RS_OK;
+}
+
+int32_t bbd_factor_dev(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* ai, int64_t ail,
+                       const double* ax_dev, hipStream_t s) {
+    BbdImpl* b = h->bbd;
+    int32_t rc = bbd_analyze(h, ap, apl, ai, ail);
+    if (rc) return rc;
+    if (b->has_decomp) {  // the decomposition's launches read host-handed values
+        std::vector<double> ax((size_t)ap[h->n]);
+        hipError_t e = ax.empty() ? hipSuccess
+                                  : hipMemcpyAsync(ax.data(), ax_dev, ax.size() * sizeof(double),
+                                                   hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return bhip(h, e, "hipMemcpy(ax)");
+        return bbd_factor(h, ap, apl, ai, ail, ax.data());
+    }
+    rc = solver_factor_dev(b->fallback, ap, apl, ai, ail, ax_dev, s);
+    if (rc) return bfail(h, rc, b->fallback->err);
+    b->factored = true;
+    h->factor_count++;
+    h->state = ST_FACTORED;
+    return RS_OK;
+}
+
+int32_t bbd_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
+    BbdImpl* b = h->bbd;
+    if (!b->factored) return bfail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    if (!b->has_decomp) {
+        int32_t rc = solver_solve_dev(b->fallback, rhs_dev, x_dev, s);
+        return rc ? bfail(h, rc, b->fallback->err) : RS_OK;
+    }
+    std::vector<double> r(h->n);
+    hipError_t e = hipMemcpyAsync(r.data(), rhs_dev, h->n * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return bhip(h, e, "hipMemcpy(rhs)");
+    int32_t rc = bbd_solve(h, r.data());
+    if (rc) return rc;
+    e = hipMemcpyAsync(x_dev, r.data(), h->n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? RS_OK : bhip(h, e, "hipMemcpy(x)");
+}
+
+void bbd_set_supernodal(rs_solver* h, int mode) {
+    h->bbd->sn_mode = mode;
+    if (h->bbd->fallback) rs_set_supernodal(h->bbd->fallback, mode);
+}
+
+const rs_solver* bbd_fallback_solver(const rs_solver* h) {
+    return h->bbd && !h->bbd->has_decomp ? h->bbd->fallback : nullptr;
 }
 
 void bbd_reset(rs_solver* h) {

@@ -59,6 +59,15 @@ int32_t bbd_solve(::rs_solver* h, double* rhs);
 void bbd_reset(::rs_solver* h);
 void bbd_set_target(::rs_solver* h, int64_t num_blocks);
 int32_t bbd_info(const ::rs_solver* h, int64_t* out);
+// The same on device-resident values: a SparseLU fallback factors and solves
+// them on the device; a real decomposition copies them to the host first.
+int32_t bbd_factor_dev(::rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* ai, int64_t ail,
+                       const double* ax_dev, hipStream_t s);
+int32_t bbd_solve_dev(::rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s);
+// the supernodal mode handed to the SparseLU fallback (rs_set_supernodal)
+void bbd_set_supernodal(::rs_solver* h, int mode);
+// the SparseLU fallback solver, when the decomposition collapsed (else null)
+const ::rs_solver* bbd_fallback_solver(const ::rs_solver* h);
 int32_t dense_factor(::rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* ai, int64_t ail,
                      const double* ax);
 int32_t dense_solve(::rs_solver* h, double* rhs);

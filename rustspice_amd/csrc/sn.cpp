@@ -1,0 +1,170 @@
+// sn.cpp - device state and launch sequence of the supernodal factor and
+// solve (plan and schedule in sn.hpp / sn_plan.cpp, kernels in
+// sn_kernels.hip).
+#include <algorithm>
+#include <cstring>
+
+#include "sn.hpp"
+#include "solver_impl.hpp"
+
+namespace rsgpu {
+
+hipError_t sn_launch_zero(double* v, int64_t n, hipStream_t s);
+hipError_t sn_launch_scatter(const int32_t* src, const int32_t* cnt, const int64_t* slot, int64_t nruns,
+                             const double* ax, double* v, hipStream_t s);
+hipError_t sn_launch_gather(const SnPtrs& p, int item0, int nitems, hipStream_t s);
+hipError_t sn_launch_hot(const SnPtrs& p, const int64_t* hot_ptr, const int64_t* terms, int e, hipStream_t s);
+hipError_t sn_launch_factor(const SnPtrs& p, const int32_t* list, int n, hipStream_t s);
+hipError_t sn_launch_big(const SnPtrs& p, int J, int w, int hh, hipStream_t s);
+hipError_t sn_launch_perm(const int32_t* perm, const double* in, double* out, int n, bool to_pos, hipStream_t s);
+hipError_t sn_launch_fwd(const SnPtrs& p, const int32_t* rows, int nrows, const int32_t* inner, int ninner,
+                         double* y, hipStream_t s);
+hipError_t sn_launch_bwd(const SnPtrs& p, const int32_t* rows, const int32_t* sn_of, int nrows,
+                         const int32_t* inner, int ninner, double* y, hipStream_t s);
+
+struct SnImpl {
+    SnHost h;
+    BlockSymbolic sym;
+    DevBuf<int32_t> j0, rptr, rows, csrc, ca0, ca1, rel, perm, lev_sn, rec, lrow_sn, lrow_t, sc_src, sc_cnt,
+        sn_of, lev_rows, lists;
+    DevBuf<int4> gitems;
+    DevBuf<int64_t> oL, oU, crel_off, tile_rec, hot_slot, hot_ptr, hot_terms, lrow_ptr, sc_slot;
+    DevBuf<double> v, ud, y;
+    SnPtrs ptr{};
+    // per level: rows [lr[l], lr[l+1]) of lev_rows; small-factor list
+    // [fs[l], fs[l+1]) and inner list [in[l], in[l+1]) of lists; wide supernodes
+    std::vector<int32_t> lr, fs, in;
+    std::vector<std::vector<int32_t>> big;
+    std::vector<std::vector<int32_t>> hot_at;  // hot entries per level
+    int64_t bytes = 0;
+};
+
+namespace {
+
+template <class T>
+hipError_t up(SnImpl* p, DevBuf<T>& b, const std::vector<T>& v) {
+    hipError_t e = b.ensure(std::max<size_t>(v.size(), 1));
+    if (e || v.empty()) return e;
+    p->bytes += (int64_t)(v.size() * sizeof(T));
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why) {
+    SnImpl* p = new SnImpl();
+    if (!build_sn_plan(s, ap, ai, p->h, why)) {
+        delete p;
+        return nullptr;
+    }
+    p->sym = s;
+    SnHost& h = p->h;
+    // launch lists
+    std::vector<int32_t> sn_of(h.n), lev_rows, lists_small, lists_inner;
+    for (int32_t J = 0; J < h.ns; J++)
+        for (int32_t c = h.j0[J]; c < h.j0[J + 1]; c++) sn_of[c] = J;
+    p->lr.assign(1, 0);
+    p->fs.assign(1, 0);
+    p->in.assign(1, 0);
+    p->big.resize(h.nlev);
+    p->hot_at.resize(h.nlev);
+    for (size_t e = 0; e < h.hot_slot.size(); e++) p->hot_at[h.hot_lev[e]].push_back((int32_t)e);
+    for (int32_t l = 0; l < h.nlev; l++) {
+        for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
+            const int32_t J = h.lev_sn[q], w = h.j0[J + 1] - h.j0[J];
+            for (int32_t c = h.j0[J]; c < h.j0[J + 1]; c++) lev_rows.push_back(c);
+            if (w >= kSnBigW) p->big[l].push_back(J);
+            else lists_small.push_back(J);
+            if (w > 1) lists_inner.push_back(J);
+        }
+        p->lr.push_back((int32_t)lev_rows.size());
+        p->fs.push_back((int32_t)lists_small.size());
+        p->in.push_back((int32_t)lists_inner.size());
+    }
+    const int32_t inner_base = (int32_t)lists_small.size();
+    std::vector<int32_t> lists = lists_small;
+    lists.insert(lists.end(), lists_inner.begin(), lists_inner.end());
+    for (auto& x : p->in) x += inner_base;
+    std::vector<int4> gi(h.gitems.size() / 4);
+    std::memcpy(gi.data(), h.gitems.data(), gi.size() * sizeof(int4));
+    hipError_t e;
+    if ((e = up(p, p->j0, h.j0)) || (e = up(p, p->rptr, h.rptr)) || (e = up(p, p->rows, h.rows)) ||
+        (e = up(p, p->csrc, h.csrc)) || (e = up(p, p->ca0, h.ca0)) || (e = up(p, p->ca1, h.ca1)) ||
+        (e = up(p, p->rel, h.rel)) || (e = up(p, p->perm, h.perm)) || (e = up(p, p->lev_sn, h.lev_sn)) ||
+        (e = up(p, p->rec, h.rec)) || (e = up(p, p->lrow_sn, h.lrow_sn)) || (e = up(p, p->lrow_t, h.lrow_t)) ||
+        (e = up(p, p->sc_src, h.sc_src)) || (e = up(p, p->sc_cnt, h.sc_cnt)) || (e = up(p, p->sn_of, sn_of)) ||
+        (e = up(p, p->lev_rows, lev_rows)) || (e = up(p, p->lists, lists)) || (e = up(p, p->gitems, gi)) ||
+        (e = up(p, p->oL, h.oL)) || (e = up(p, p->oU, h.oU)) || (e = up(p, p->crel_off, h.crel_off)) ||
+        (e = up(p, p->tile_rec, h.tile_rec)) || (e = up(p, p->hot_slot, h.hot_slot)) ||
+        (e = up(p, p->hot_ptr, h.hot_ptr)) || (e = up(p, p->hot_terms, h.hot_terms)) ||
+        (e = up(p, p->lrow_ptr, h.lrow_ptr)) || (e = up(p, p->sc_slot, h.sc_slot)) ||
+        (e = p->v.ensure((size_t)std::max<int64_t>(h.nvals, 1))) || (e = p->ud.ensure(h.n)) ||
+        (e = p->y.ensure(h.n))) {
+        if (why) *why = std::string("device upload: ") + hipGetErrorString(e);
+        delete p;
+        return nullptr;
+    }
+    p->bytes += (h.nvals + 2 * (int64_t)h.n) * (int64_t)sizeof(double);
+    SnPtrs& q = p->ptr;
+    q.j0 = p->j0.p; q.rptr = p->rptr.p; q.rows = p->rows.p; q.csrc = p->csrc.p; q.ca0 = p->ca0.p;
+    q.ca1 = p->ca1.p; q.rel = p->rel.p; q.perm = p->perm.p; q.lev_sn = p->lev_sn.p; q.rec = p->rec.p;
+    q.gitems = p->gitems.p; q.oL = p->oL.p; q.oU = p->oU.p; q.crel_off = p->crel_off.p;
+    q.tile_rec = p->tile_rec.p; q.hot_slot = p->hot_slot.p; q.lrow_ptr = p->lrow_ptr.p;
+    q.lrow_sn = p->lrow_sn.p; q.lrow_t = p->lrow_t.p; q.nhot = (int32_t)h.hot_slot.size(); q.n = h.n;
+    q.v = p->v.p; q.ud = p->ud.p;
+    return p;
+}
+
+void sn_destroy(SnImpl* p) { delete p; }
+
+hipError_t sn_factor(SnImpl* p, const double* ax, hipStream_t s) {
+    const SnHost& h = p->h;
+    hipError_t e;
+    if ((e = sn_launch_zero(p->v.p, h.nvals, s)) ||
+        (e = sn_launch_scatter(p->sc_src.p, p->sc_cnt.p, p->sc_slot.p, (int64_t)h.sc_src.size(), ax, p->v.p, s)))
+        return e;
+    for (int32_t l = 0; l < h.nlev; l++) {
+        if ((e = sn_launch_gather(p->ptr, h.glev_ptr[l], h.glev_ptr[l + 1] - h.glev_ptr[l], s))) return e;
+        for (int32_t he : p->hot_at[l])
+            if ((e = sn_launch_hot(p->ptr, p->hot_ptr.p, p->hot_terms.p, he, s))) return e;
+        if ((e = sn_launch_factor(p->ptr, p->lists.p + p->fs[l], p->fs[l + 1] - p->fs[l], s))) return e;
+        for (int32_t J : p->big[l])
+            if ((e = sn_launch_big(p->ptr, J, h.j0[J + 1] - h.j0[J], h.rptr[J + 1] - h.rptr[J], s))) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t sn_solve(SnImpl* p, const double* rhs, double* x, hipStream_t s) {
+    const SnHost& h = p->h;
+    hipError_t e;
+    if ((e = sn_launch_perm(p->perm.p, rhs, p->y.p, h.n, true, s))) return e;
+    for (int32_t l = 0; l < h.nlev; l++)
+        if ((e = sn_launch_fwd(p->ptr, p->lev_rows.p + p->lr[l], p->lr[l + 1] - p->lr[l], p->lists.p + p->in[l],
+                               p->in[l + 1] - p->in[l], p->y.p, s)))
+            return e;
+    for (int32_t l = h.nlev - 1; l >= 0; l--)
+        if ((e = sn_launch_bwd(p->ptr, p->lev_rows.p + p->lr[l], p->sn_of.p, p->lr[l + 1] - p->lr[l],
+                               p->lists.p + p->in[l], p->in[l + 1] - p->in[l], p->y.p, s)))
+            return e;
+    return sn_launch_perm(p->perm.p, p->y.p, x, h.n, false, s);
+}
+
+hipError_t sn_values(SnImpl* p, double* lx, double* ux, double* ud) {
+    std::vector<double> v((size_t)p->h.nvals);
+    hipError_t e;
+    if ((e = hipDeviceSynchronize()) ||
+        (!v.empty() && (e = hipMemcpy(v.data(), p->v.p, v.size() * sizeof(double), hipMemcpyDeviceToHost))) ||
+        (ud && (e = hipMemcpy(ud, p->ud.p, p->h.n * sizeof(double), hipMemcpyDeviceToHost))))
+        return e;
+    std::vector<double> l, u;
+    sn_to_csc(p->h, p->sym, v, l, u);
+    if (lx) std::copy(l.begin(), l.end(), lx);
+    if (ux) std::copy(u.begin(), u.end(), ux);
+    return hipSuccess;
+}
+
+const double* sn_pivots(const SnImpl* p) { return p->ud.p; }
+const SnHost& sn_plan(const SnImpl* p) { return p->h; }
+int64_t sn_device_bytes(const SnImpl* p) { return p->bytes; }
+
+}  // namespace rsgpu

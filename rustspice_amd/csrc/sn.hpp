@@ -37,6 +37,8 @@
 // J's own columns); backward, root first, J's rows gather R_J's x (columns
 // descending), then J's own columns descending.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -47,6 +49,8 @@ namespace rsgpu {
 
 // Target tiles of the gather: kSnTile x kSnTile entries of LP_J or UP_J.
 constexpr int32_t kSnTile = 64;
+// An entry that this many contributions touch is summed on its own.
+constexpr int32_t kSnHotRecords = 256;
 
 struct SnHost {
     int32_t n = 0, ns = 0, nlev = 0;
@@ -62,8 +66,24 @@ struct SnHost {
     std::vector<int32_t> cptr, csrc, ca0, ca1, crel;
     std::vector<int64_t> crel_off;
     std::vector<int32_t> rel;
-    // gather work items per level: {J, kind (0 LP, 1 UP), r0, c0} as int4
+    // gather tiles per level: {J, kind (0 LP, 1 UP), r0, c0} as int4, each
+    // with its records (tile_rec[t] .. tile_rec[t+1]): the contributions that
+    // touch the tile, ascending, as {pair, t0, t1, u0, u1}: the pair's rows
+    // t in [t0, t1) of R_J' and columns u in [u0, u1) of R_J' that land in
+    // the tile (LP: rows rel[t], columns R_J'[u] - j0; UP: rows R_J'[u] - j0,
+    // columns rel[t] - w)
     std::vector<int32_t> glev_ptr, gitems;
+    std::vector<int64_t> tile_rec;
+    std::vector<int32_t> rec;  // 5 per record
+    // entries with at least kSnHotRecords contributions (the ground node's
+    // diagonal): taken out of their tiles and summed by a wavefront each from
+    // a flat list of (L slot, U slot) terms in the reference's order
+    std::vector<int64_t> hot_slot, hot_ptr, hot_terms;  // hot_terms: 2 per term
+    std::vector<int32_t> hot_lev;                       // level of each hot entry
+    // forward-solve rows: for every position p, the supernodes S (ascending)
+    // with p in R_S and p's index t in R_S
+    std::vector<int64_t> lrow_ptr;
+    std::vector<int32_t> lrow_sn, lrow_t;
     // scatter of A: every run of duplicate CSC entries -> panel slot
     std::vector<int32_t> sc_src, sc_cnt;
     std::vector<int64_t> sc_slot;
@@ -74,12 +94,30 @@ struct SnHost {
     double updates = 0.0;            // multiply-subtracts of one factor (pattern count)
 };
 
+// Supernodes at least this wide are factored by the blocked multi-workgroup
+// path (panel of kSnPanel columns, U block row, trailing update tiles).
+constexpr int32_t kSnBigW = 256;
+constexpr int32_t kSnPanel = 32;
+
+// Device view of a plan (plain pointers, passed by value to the kernels).
+struct SnPtrs {
+    const int32_t *j0, *rptr, *rows, *csrc, *ca0, *ca1, *rel, *perm, *lev_sn;
+    const int32_t* rec;               // tile records, 5 ints each
+    const int4* gitems;
+    const int64_t *oL, *oU, *crel_off, *tile_rec, *hot_slot;
+    const int64_t *lrow_ptr;          // forward-solve row lists: per position
+    const int32_t *lrow_sn, *lrow_t;  // (S, t): L(p, S) = LP_S row w_S + t
+    int32_t nhot, n;
+    double* v;   // panels
+    double* ud;  // pivots (n)
+};
+
 // Builds the supernodal plan of a single-block SparseLU symbolic factor
 // (positions = pivot steps, identity row pivots). Returns false, with a
 // reason, when the pattern is not structurally symmetric or exceeds the
 // 32-bit index ranges.
 bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost& h,
-                   std::string* why = nullptr);
+                   std::string* why = nullptr, int32_t hot_min = kSnHotRecords);
 
 // Host execution of the plan in the device kernels' order: factor of ax,
 // then the solve of rhs into x; lx/ux/ud (optional) receive the factor in
@@ -91,5 +129,25 @@ void sn_host_solve(const SnHost& h, const std::vector<double>& vals, const std::
                    const double* rhs, double* x);
 void sn_to_csc(const SnHost& h, const BlockSymbolic& s, const std::vector<double>& vals,
                std::vector<double>& lx, std::vector<double>& ux);
+
+}  // namespace rsgpu
+
+namespace rsgpu {
+
+// Device state of one supernodal plan: the plan arrays, the panels, the
+// pivots and the solve vector, with the per-level launch lists.
+struct SnImpl;
+SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why);
+void sn_destroy(SnImpl* p);
+// factor of the unmerged CSC values ax_dev (device memory, nnz entries)
+hipError_t sn_factor(SnImpl* p, const double* ax_dev, hipStream_t s);
+// x = A^-1 rhs (device vectors of n entries; rhs and x may alias)
+hipError_t sn_solve(SnImpl* p, const double* rhs_dev, double* x_dev, hipStream_t s);
+// factor values in the symbolic's CSC order (host arrays)
+hipError_t sn_values(SnImpl* p, double* lx, double* ux, double* ud);
+// pivots (device, n entries)
+const double* sn_pivots(const SnImpl* p);
+const SnHost& sn_plan(const SnImpl* p);
+int64_t sn_device_bytes(const SnImpl* p);
 
 }  // namespace rsgpu

@@ -27,7 +27,7 @@ bool fail(std::string* why, const char* m) {
 }  // namespace
 
 bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost& h,
-                   std::string* why) {
+                   std::string* why, int32_t hot_min) {
     h = SnHost();
     const i64 n = s.n;
     if (n <= 0) return fail(why, "empty matrix");
@@ -183,31 +183,142 @@ bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost&
         }
     }
 
-    // 6. gather work items: tiles of LP_J and UP_J of supernodes with
-    //    contributions, level by level
+    // 6. gather tiles and their records. A position that lies in the R of at
+    //    least hot_min supernodes (the ground node) has its diagonal
+    //    entry summed on its own: its records are dropped from the tile
+    //    unless they touch another entry of it, and its terms listed flat.
+    std::vector<int32_t> occ(n, 0);
+    for (int32_t r : h.rows) occ[r]++;
+    auto hot_pos = [&](int32_t p) { return occ[p] >= hot_min; };
+    std::vector<int64_t> tile_base(ns + 1, 0);
+    std::vector<int32_t> nrt(ns), nct(ns), nur(ns), nuc(ns);
+    for (int32_t J = 0; J < ns; J++) {
+        const int32_t w = h.j0[J + 1] - h.j0[J], hh = h.rptr[J + 1] - h.rptr[J];
+        nrt[J] = (w + hh + kSnTile - 1) / kSnTile;
+        nct[J] = (w + kSnTile - 1) / kSnTile;
+        nur[J] = (w + kSnTile - 1) / kSnTile;
+        nuc[J] = (hh + kSnTile - 1) / kSnTile;
+        const bool any = h.cptr[J + 1] > h.cptr[J];
+        tile_base[J + 1] = tile_base[J] + (any ? (int64_t)nrt[J] * nct[J] + (int64_t)nur[J] * nuc[J] : 0);
+    }
+    const int64_t ntiles = tile_base[ns];
+    if (ntiles >= (int64_t)std::numeric_limits<int32_t>::max()) return fail(why, "too many gather tiles");
+    // enumerate the records of one target J, calling emit(tile, pair, t0, t1, u0, u1)
+    auto records = [&](int32_t J, auto&& emit) {
+        const int32_t jj0 = h.j0[J], w = h.j0[J + 1] - jj0;
+        for (int32_t pi = h.cptr[J]; pi < h.cptr[J + 1]; pi++) {
+            const int32_t S = h.csrc[pi], a0 = h.ca0[pi], a1 = h.ca1[pi];
+            const int32_t hS = h.rptr[S + 1] - h.rptr[S];
+            const int32_t* RS = h.rows.data() + h.rptr[S];
+            const int32_t* rl = h.rel.data() + h.crel_off[pi] - a0;  // rl[t], t in [a0, hS)
+            // LP: rows t in [a0, hS) by row tile, columns u in [a0, a1) by column tile
+            for (int32_t u0 = a0; u0 < a1;) {
+                const int32_t ct = (RS[u0] - jj0) / kSnTile;
+                int32_t u1 = u0 + 1;
+                while (u1 < a1 && (RS[u1] - jj0) / kSnTile == ct) u1++;
+                for (int32_t t0 = a0; t0 < hS;) {
+                    const int32_t rt = rl[t0] / kSnTile;
+                    int32_t t1 = t0 + 1;
+                    while (t1 < hS && rl[t1] / kSnTile == rt) t1++;
+                    // a record whose only entry is a hot diagonal is summed elsewhere
+                    const bool only_hot = t1 - t0 == 1 && u1 - u0 == 1 && RS[t0] == RS[u0] && hot_pos(RS[t0]);
+                    if (!only_hot) emit(tile_base[J] + (int64_t)ct * nrt[J] + rt, pi, t0, t1, u0, u1);
+                    t0 = t1;
+                }
+                u0 = u1;
+            }
+            // UP: rows u in [a0, a1) by row tile, columns t in [a1, hS) by column tile
+            for (int32_t u0 = a0; u0 < a1;) {
+                const int32_t rt = (RS[u0] - jj0) / kSnTile;
+                int32_t u1 = u0 + 1;
+                while (u1 < a1 && (RS[u1] - jj0) / kSnTile == rt) u1++;
+                for (int32_t t0 = a1; t0 < hS;) {
+                    const int32_t ct = (rl[t0] - w) / kSnTile;
+                    int32_t t1 = t0 + 1;
+                    while (t1 < hS && (rl[t1] - w) / kSnTile == ct) t1++;
+                    emit(tile_base[J] + (int64_t)nrt[J] * nct[J] + (int64_t)ct * nur[J] + rt, pi, t0, t1, u0, u1);
+                    t0 = t1;
+                }
+                u0 = u1;
+            }
+        }
+    };
+    h.tile_rec.assign(ntiles + 1, 0);
+    for (int32_t J = 0; J < ns; J++)
+        records(J, [&](int64_t tile, int32_t, int32_t, int32_t, int32_t, int32_t) { h.tile_rec[tile + 1]++; });
+    for (int64_t t = 0; t < ntiles; t++) h.tile_rec[t + 1] += h.tile_rec[t];
+    h.rec.resize((size_t)h.tile_rec[ntiles] * 5);
+    {
+        std::vector<int64_t> fill(h.tile_rec.begin(), h.tile_rec.end() - 1);
+        for (int32_t J = 0; J < ns; J++)
+            records(J, [&](int64_t tile, int32_t pi, int32_t t0, int32_t t1, int32_t u0, int32_t u1) {
+                int32_t* r = h.rec.data() + 5 * fill[tile]++;
+                r[0] = pi; r[1] = t0; r[2] = t1; r[3] = u0; r[4] = u1;
+            });
+    }
+    // tiles in level order (a tile is {J, kind, r0, c0}); tile ids of J are
+    // consecutive, so the level's tiles are the ids of its supernodes
     h.glev_ptr.assign(h.nlev + 1, 0);
+    h.gitems.clear();
+    std::vector<int64_t> order;  // tile id of each gather item
     for (int32_t l = 0; l < h.nlev; l++) {
         for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
             const int32_t J = h.lev_sn[q];
-            if (h.cptr[J + 1] == h.cptr[J]) continue;
-            const int32_t w = h.j0[J + 1] - h.j0[J], hh = h.rptr[J + 1] - h.rptr[J];
-            for (int32_t c0 = 0; c0 < w; c0 += kSnTile)
-                for (int32_t r0 = 0; r0 < w + hh; r0 += kSnTile) {
-                    h.gitems.push_back(J);
-                    h.gitems.push_back(0);
-                    h.gitems.push_back(r0);
-                    h.gitems.push_back(c0);
+            for (int64_t t = tile_base[J]; t < tile_base[J + 1]; t++) {
+                const int64_t loc = t - tile_base[J];
+                const int64_t nlp = (int64_t)nrt[J] * nct[J];
+                int32_t kind, r0, c0;
+                if (loc < nlp) {
+                    kind = 0; c0 = (int32_t)(loc / nrt[J]) * kSnTile; r0 = (int32_t)(loc % nrt[J]) * kSnTile;
+                } else {
+                    const int64_t u = loc - nlp;
+                    kind = 1; c0 = (int32_t)(u / nur[J]) * kSnTile; r0 = (int32_t)(u % nur[J]) * kSnTile;
                 }
-            for (int32_t c0 = 0; c0 < hh; c0 += kSnTile)
-                for (int32_t r0 = 0; r0 < w; r0 += kSnTile) {
-                    h.gitems.push_back(J);
-                    h.gitems.push_back(1);
-                    h.gitems.push_back(r0);
-                    h.gitems.push_back(c0);
-                }
+                h.gitems.push_back(J);
+                h.gitems.push_back(kind);
+                h.gitems.push_back(r0);
+                h.gitems.push_back(c0);
+                order.push_back(t);
+            }
         }
         h.glev_ptr[l + 1] = (int32_t)(h.gitems.size() / 4);
     }
+    // records re-indexed in gather-item order
+    {
+        std::vector<int64_t> tr(order.size() + 1, 0);
+        std::vector<int32_t> rec2;
+        rec2.reserve(h.rec.size());
+        for (size_t i = 0; i < order.size(); i++) {
+            const int64_t t = order[i];
+            rec2.insert(rec2.end(), h.rec.begin() + 5 * h.tile_rec[t], h.rec.begin() + 5 * h.tile_rec[t + 1]);
+            tr[i + 1] = tr[i] + (h.tile_rec[t + 1] - h.tile_rec[t]);
+        }
+        h.rec.swap(rec2);
+        h.tile_rec.swap(tr);
+    }
+    // hot diagonal entries: every contribution (S ascending, s ascending) as
+    // the slots of L_S(p, s) and U_S(s, p)
+    for (int32_t p = 0; p < n; p++) {
+        if (!hot_pos(p)) continue;
+        const int32_t J = sn_of[p];
+        const int32_t jj0 = h.j0[J], w = h.j0[J + 1] - jj0, hh = h.rptr[J + 1] - h.rptr[J];
+        h.hot_slot.push_back(h.oL[J] + (int64_t)(p - jj0) * (w + hh) + (p - jj0));
+        h.hot_lev.push_back(level[J]);
+        h.hot_ptr.push_back((int64_t)h.hot_terms.size() / 2);
+        for (int32_t pi = h.cptr[J]; pi < h.cptr[J + 1]; pi++) {
+            const int32_t S = h.csrc[pi], a0 = h.ca0[pi], a1 = h.ca1[pi];
+            const int32_t* RS = h.rows.data() + h.rptr[S];
+            const int64_t x = find_sorted(RS + a0, RS + a1, p);
+            if (x < 0) continue;
+            const int32_t t = a0 + (int32_t)x;
+            const int32_t wS = h.j0[S + 1] - h.j0[S], hS = h.rptr[S + 1] - h.rptr[S];
+            for (int32_t s2 = 0; s2 < wS; s2++) {
+                h.hot_terms.push_back(h.oL[S] + (int64_t)s2 * (wS + hS) + wS + t);  // L_S(p, s)
+                h.hot_terms.push_back(h.oU[S] + (int64_t)t * wS + s2);              // U_S(s, p)
+            }
+        }
+    }
+    h.hot_ptr.push_back((int64_t)h.hot_terms.size() / 2);
 
     // 7. scatter of A: runs of duplicate rows of each column, in the column
     //    order of the pivot steps, onto the panel slot of (position, step)
@@ -253,6 +364,20 @@ bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost&
     }
     h.perm.resize(n);
     for (i64 i = 0; i < n; i++) h.perm[i] = (int32_t)s.col_perm[i];
+    h.lrow_ptr.assign(n + 1, 0);
+    for (int32_t r : h.rows) h.lrow_ptr[r + 1]++;
+    for (i64 i = 0; i < n; i++) h.lrow_ptr[i + 1] += h.lrow_ptr[i];
+    h.lrow_sn.resize(h.rows.size());
+    h.lrow_t.resize(h.rows.size());
+    {
+        std::vector<int64_t> fill(h.lrow_ptr.begin(), h.lrow_ptr.end() - 1);
+        for (int32_t S = 0; S < ns; S++)
+            for (int32_t t = 0; t < h.rptr[S + 1] - h.rptr[S]; t++) {
+                const int64_t at = fill[h.rows[h.rptr[S] + t]]++;
+                h.lrow_sn[at] = S;
+                h.lrow_t[at] = t;
+            }
+    }
 
     // 8. algorithmic work: every U(j, k), j < k, applies L(:, j)
     double upd = 0.0;
@@ -287,38 +412,49 @@ void sn_host_factor(const SnHost& h, const double* ax, std::vector<double>& v, s
         for (int32_t q = 0; q < h.sc_cnt[s]; q++) x += ax[h.sc_src[s] + q];
         v[h.sc_slot[s]] = x;
     }
+    std::vector<char> is_hot;
+    is_hot.assign((size_t)h.nvals, 0);
+    for (int64_t sl : h.hot_slot) is_hot[sl] = 1;
     for (int32_t l = 0; l < h.nlev; l++) {
+        // gather: every tile applies its records in order; each record is a
+        // block of entries, each entry's terms in ascending order
+        for (int32_t it = h.glev_ptr[l]; it < h.glev_ptr[l + 1]; it++) {
+            const Panel P(h, h.gitems[4 * it]);
+            const int32_t kind = h.gitems[4 * it + 1];
+            for (int64_t ri = h.tile_rec[it]; ri < h.tile_rec[it + 1]; ri++) {
+                const int32_t* r = h.rec.data() + 5 * ri;
+                const int32_t pi = r[0];
+                const Panel S(h, h.csrc[pi]);
+                const int32_t a0 = h.ca0[pi];
+                const int32_t* rl = h.rel.data() + h.crel_off[pi] - a0;
+                const int32_t* RS = h.rows.data() + h.rptr[S.J];
+                for (int32_t t = r[1]; t < r[2]; t++)
+                    for (int32_t u = r[3]; u < r[4]; u++) {
+                        // LP: row rl[t], column RS[u] - j0, L_S row t, U_S column u
+                        // UP: row RS[u] - j0, column rl[t] - w, L_S row u, U_S column t
+                        const int64_t slot = kind == 0 ? P.lp(rl[t], RS[u] - P.jj0) : P.up(RS[u] - P.jj0, rl[t] - P.w);
+                        if (is_hot[slot]) continue;
+                        const int32_t lrow = kind == 0 ? t : u, ucol = kind == 0 ? u : t;
+                        double x = v[slot];
+                        for (int32_t s2 = 0; s2 < S.w; s2++) {
+                            const double uu = v[S.up(s2, ucol)];
+                            if (uu != 0.0) x -= v[S.lp(S.w + lrow, s2)] * uu;
+                        }
+                        v[slot] = x;
+                    }
+            }
+        }
+        for (size_t e = 0; e < h.hot_slot.size(); e++) {
+            if (h.hot_lev[e] != l) continue;
+            double x = v[h.hot_slot[e]];
+            for (int64_t q = h.hot_ptr[e]; q < h.hot_ptr[e + 1]; q++) {
+                const double uu = v[h.hot_terms[2 * q + 1]];
+                if (uu != 0.0) x -= v[h.hot_terms[2 * q]] * uu;
+            }
+            v[h.hot_slot[e]] = x;
+        }
         for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
             const Panel P(h, h.lev_sn[q]);
-            // gather, contributions in ascending order
-            for (int32_t pi = h.cptr[P.J]; pi < h.cptr[P.J + 1]; pi++) {
-                const Panel S(h, h.csrc[pi]);
-                const int32_t a0 = h.ca0[pi], a1 = h.ca1[pi];
-                const int32_t* rel = h.rel.data() + h.crel_off[pi] - a0;  // rel[t] for t >= a0
-                for (int32_t ci = a0; ci < a1; ci++) {  // columns of LP_J
-                    const int32_t col = h.rows[h.rptr[S.J] + ci] - P.jj0;
-                    for (int32_t ri = a0; ri < S.hh; ri++) {
-                        double x = v[P.lp(rel[ri], col)];
-                        for (int32_t t = 0; t < S.w; t++) {
-                            const double u = v[S.up(t, ci)];
-                            if (u != 0.0) x -= v[S.lp(S.w + ri, t)] * u;
-                        }
-                        v[P.lp(rel[ri], col)] = x;
-                    }
-                }
-                for (int32_t ci = a0; ci < a1; ci++) {  // rows of UP_J
-                    const int32_t row = h.rows[h.rptr[S.J] + ci] - P.jj0;
-                    for (int32_t ri = a1; ri < S.hh; ri++) {
-                        const int32_t col = rel[ri] - P.w;
-                        double x = v[P.up(row, col)];
-                        for (int32_t t = 0; t < S.w; t++) {
-                            const double u = v[S.up(t, ri)];
-                            if (u != 0.0) x -= v[S.lp(S.w + ci, t)] * u;
-                        }
-                        v[P.up(row, col)] = x;
-                    }
-                }
-            }
             // J's own columns, right-looking
             const int32_t W = P.w + P.hh;
             for (int32_t t = 0; t < P.w; t++) {

@@ -19,6 +19,7 @@
 #include "../../include/rsgpu.h"
 #include "plan.hpp"
 #include "bbd.hpp"
+#include "sn.hpp"
 #include "solver_impl.hpp"
 #include "profile.hpp"
 
@@ -45,6 +46,8 @@ bool pattern_matches(const rs_solver* h, const i64* ap, i64 apl, const i64* ai, 
 }
 
 void reset(rs_solver* h) {
+    sn_destroy(h->sn);
+    h->sn = nullptr;
     h->state = ST_EMPTY;
     h->hf = HostFactor();
     h->ph = PlanHost();
@@ -110,6 +113,19 @@ int32_t do_analyze(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail)
     for (auto& s : f.sym) {
         f.nnz_l += (i64)s.lri.size();
         f.nnz_u += (i64)s.uri.size();
+    }
+    sn_destroy(h->sn);
+    h->sn = nullptr;
+    if (h->kind == RS_SOLVER_SPARSELU && f.sym.size() == 1 &&
+        (h->sn_mode == 2 || (h->sn_mode == 1 && n >= kSnAutoN))) {
+        // supernodal factor and solve on the device for every factor of this
+        // pattern (no host first factor: SparseLU picks no pivots); a pattern
+        // that does not admit it (not structurally symmetric) keeps the
+        // level-scheduled kernels
+        std::string why;
+        h->sn = sn_create(f.sym[0], ap, ai, &why);
+        if (!h->sn && why.rfind("device upload", 0) == 0)
+            return fail(h, RS_ERR_DEVICE, "HIP error in supernodal plan " + why);
     }
     h->state = ST_ANALYZED;
     h->plan_on_device = false;
@@ -603,6 +619,22 @@ int32_t gpu_refactor(rs_solver* h, const double* ax_dev, hipStream_t s, bool syn
 
 }  // namespace
 
+namespace {
+// One supernodal factor of device values: the first of a pattern counts as a
+// factor, later ones as refactors (the reference's counts).
+int32_t sn_factor_counted(rs_solver* h, const double* ax_dev, hipStream_t s) {
+    const int ps = prof_begin(PK_REFACTOR, s);
+    hipError_t e = sn_factor(h->sn, ax_dev, s);
+    prof_end(ps, s);
+    if (e != hipSuccess) return hip_fail(h, e, "supernodal factor");
+    (h->state == ST_FACTORED ? h->refactor_count : h->factor_count)++;
+    h->state = ST_FACTORED;
+    h->last_parallel = true;
+    h->rcond_stale = true;
+    return RS_OK;
+}
+}  // namespace
+
 namespace rsgpu {
 static int g_poison = -1;  // -1: not decided yet (RSGPU_POISON=1 turns it on)
 bool poison_fp64_buffers() {
@@ -629,11 +661,21 @@ size_t device_max_lds() {
 
 int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail,
                           const double* ax_dev, hipStream_t s) {
+    if (h->bbd) return bbd_factor_dev(h, ap, apl, ai, ail, ax_dev, s);
+    if (h->dense) {
+        std::vector<double> ax(apl == h->n + 1 ? (size_t)ap[h->n] : 0);
+        hipError_t e = hipSuccess;
+        if (!ax.empty()) e = hipMemcpyAsync(ax.data(), ax_dev, ax.size() * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
+        return dense_factor(h, ap, apl, ai, ail, ax.data());
+    }
     if (h->state == ST_EMPTY || !pattern_matches(h, ap, apl, ai, ail)) {
         if (apl != h->n + 1) return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: bad ap length");
         int32_t rc = do_analyze(h, ap, apl, ai, ail);
         if (rc) return rc;
     }
+    if (h->sn) return sn_factor_counted(h, ax_dev, s);
     if (h->state == ST_FACTORED && h->plan_on_device) {
         int32_t rc = gpu_refactor(h, ax_dev, s, false);
         if (rc) return rc;
@@ -655,6 +697,26 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
 }
 
 int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
+    if (h->bbd) return bbd_solve_dev(h, rhs_dev, x_dev, s);
+    if (h->dense) {
+        if (h->state != ST_FACTORED) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+        std::vector<double> b(h->n);
+        hipError_t e = hipMemcpyAsync(b.data(), rhs_dev, h->n * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(rhs)");
+        int32_t rc = dense_solve(h, b.data());
+        if (rc) return rc;
+        e = hipMemcpyAsync(x_dev, b.data(), h->n * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e == hipSuccess ? RS_OK : hip_fail(h, e, "hipMemcpy(x)");
+    }
+    if (h->sn) {
+        if (h->state != ST_FACTORED) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+        const int ps = prof_begin(PK_SOLVE, s);
+        hipError_t e = sn_solve(h->sn, rhs_dev, x_dev, s);
+        prof_end(ps, s);
+        return e == hipSuccess ? RS_OK : hip_fail(h, e, "supernodal solve");
+    }
     if (h->state != ST_FACTORED || !h->plan_on_device)
         return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
     if (h->autotune && !h->forced && !h->tuned_sol) {  // time the shapes into scratch
@@ -695,7 +757,7 @@ int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hip
 // settled, and nothing here reads back to the host. Kernels return at once
 // while *skip != 0.
 bool solver_graph_ready(const rs_solver* h) {
-    return !h->bbd && !h->dense && h->state == ST_FACTORED && h->plan_on_device && h->growth_limit <= 0.0 &&
+    return !h->bbd && !h->dense && !h->sn && h->state == ST_FACTORED && h->plan_on_device && h->growth_limit <= 0.0 &&
            (h->forced || !h->autotune || (h->tuned_ref && h->tuned_sol));
 }
 
@@ -784,6 +846,12 @@ int32_t rs_factor(rs_solver* h, const int64_t* ap, int64_t ap_len, const int64_t
         if (rc) return rc;
     }
     if (ax_len < ap[h->n]) return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: ax shorter than nnz");
+    if (h->sn) {
+        hipError_t e = h->d_ax.ensure(std::max<size_t>(h->d_ax.n, (size_t)ap[h->n]));
+        if (e == hipSuccess) e = hipMemcpy(h->d_ax.p, ax, ap[h->n] * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(ax)");
+        return sn_factor_counted(h, h->d_ax.p, nullptr);
+    }
     const bool pattern_same = h->state == ST_FACTORED;
     if (pattern_same && h->plan_on_device) {
         hipError_t e = hipMemcpy(h->d_ax.p, ax, ap[h->n] * sizeof(double), hipMemcpyHostToDevice);
@@ -816,7 +884,11 @@ int32_t rs_solve(rs_solver* h, double* rhs, int64_t n) {
         return fail(h, RS_ERR_INVALID_MATRIX, b);
     }
     if (h->bbd) return bbd_solve(h, rhs);
-    hipError_t e = hipMemcpy(h->d_rhs.p, rhs, n * 8, hipMemcpyHostToDevice);
+    hipError_t e = hipSuccess;
+    if (h->sn && ((e = h->d_rhs.ensure(std::max<size_t>(h->d_rhs.n, n))) != hipSuccess ||
+                  (e = h->d_x.ensure(std::max<size_t>(h->d_x.n, n))) != hipSuccess))
+        return hip_fail(h, e, "hipMalloc(solve vectors)");
+    e = hipMemcpy(h->d_rhs.p, rhs, n * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy(rhs)");
     const int32_t rc = solver_solve_dev(h, h->d_rhs.p, h->d_x.p, nullptr);
     if (rc) return rc;
@@ -858,6 +930,27 @@ int64_t rs_growth_fallbacks(const rs_solver* h) { return h->growth_fallbacks; }
 double rs_last_growth(const rs_solver* h) { return h->last_growth; }
 
 void rs_set_poison(int32_t on) { rsgpu::g_poison = on != 0 ? 1 : 0; }
+
+void rs_set_supernodal(rs_solver* h, int32_t mode) {
+    h->sn_mode = mode < 0 ? 0 : (mode > 2 ? 2 : mode);
+    if (h->bbd) bbd_set_supernodal(h, h->sn_mode);
+}
+
+int32_t rs_supernodal_info(const rs_solver* h, int64_t* out) {
+    std::memset(out, 0, 8 * sizeof(int64_t));
+    const rs_solver* s = h->bbd ? bbd_fallback_solver(h) : h;
+    if (!s || !s->sn) return RS_OK;
+    const SnHost& q = sn_plan(s->sn);
+    out[0] = 1;
+    out[1] = q.ns;
+    out[2] = q.nlev;
+    out[3] = q.npairs;
+    out[4] = q.nvals;
+    out[5] = sn_device_bytes(s->sn);
+    out[6] = (int64_t)q.updates;
+    out[7] = q.max_w;
+    return RS_OK;
+}
 
 void rs_set_autotune(rs_solver* h, int32_t on) {
     h->autotune = on != 0;
@@ -907,6 +1000,13 @@ void rs_set_config(rs_solver* h, double pivot_tol, int32_t ordering, int32_t use
 
 int32_t rs_get_stats(const rs_solver* hc, rs_solver_stats* o) {
     rs_solver* h = const_cast<rs_solver*>(hc);
+    if (h->rcond_stale && h->sn) {
+        h->h_ud.resize(h->n);
+        if (hipMemcpy(h->h_ud.data(), sn_pivots(h->sn), h->n * 8, hipMemcpyDeviceToHost) == hipSuccess) {
+            compute_rcond(h, h->h_ud.data(), h->n);
+            h->rcond_stale = false;
+        }
+    }
     if (h->rcond_stale && h->plan_on_device) {  // refactors on the device defer rcond
         h->h_ud.resize(h->n);
         if (hipMemcpy(h->h_ud.data(), h->d_vals.p + h->pd.o_ud, h->n * 8, hipMemcpyDeviceToHost) ==
@@ -931,6 +1031,14 @@ int32_t rs_get_stats(const rs_solver* hc, rs_solver_stats* o) {
     o->refactor_flops = h->ph.refactor_flops;
     o->solve_flops = h->ph.solve_flops;
     o->plan_bytes = (int64_t)h->plan_bytes;
+    if (h->sn) {
+        const SnHost& q = sn_plan(h->sn);
+        o->n_factor_levels = q.nlev;
+        o->n_lsolve_levels = o->n_usolve_levels = q.nlev;
+        o->refactor_flops = 2.0 * q.updates + (double)h->hf.nnz_l;
+        o->solve_flops = 2.0 * (double)(h->hf.nnz_l + h->hf.nnz_u - h->n) + (double)h->n;
+        o->plan_bytes = sn_device_bytes(h->sn);
+    }
     return RS_OK;
 }
 
@@ -942,6 +1050,7 @@ int32_t rs_last_error(const rs_solver* h, char* buf, int64_t len) {
 }
 
 int32_t rs_refactor_device(rs_solver* h, const double* ax_dev, void* stream) {
+    if (h->sn && h->state == ST_FACTORED) return sn_factor_counted(h, ax_dev, (hipStream_t)stream);
     if (h->state != ST_FACTORED || !h->plan_on_device)
         return fail(h, RS_ERR_FACTOR_FAILED, "Numerical factorization failed: no factored pattern");
     int32_t rc = gpu_refactor(h, ax_dev, (hipStream_t)stream, false);
@@ -957,6 +1066,10 @@ int32_t rs_solve_device(rs_solver* h, const double* rhs_dev, double* x_dev, void
 }
 
 int32_t rs_get_factor_values(rs_solver* h, double* lx, double* ux, double* udiag) {
+    if (h->sn && h->state == ST_FACTORED) {
+        hipError_t e = sn_values(h->sn, lx, ux, udiag);
+        return e == hipSuccess ? RS_OK : hip_fail(h, e, "supernodal values");
+    }
     if (h->state != ST_FACTORED || !h->plan_on_device)
         return fail(h, RS_ERR_FACTOR_FAILED, "Numerical factorization failed: not factored");
     hipError_t e;

@@ -38,6 +38,10 @@ hipError_t launch_growth(const double* ax, int nnz, const double* ux, int nu, do
 
 struct BbdImpl;
 struct DenseImpl;
+struct SnImpl;
+// SparseLU matrices at least this large with a structurally symmetric
+// pattern use the supernodal factor and solve (rs_set_supernodal mode 1).
+constexpr int64_t kSnAutoN = 20000;
 
 enum { ST_EMPTY = 0, ST_ANALYZED = 1, ST_FACTORED = 2 };
 constexpr int kThreads = 256;
@@ -88,6 +92,8 @@ struct rs_solver {
     int kind = RS_SOLVER_NATIVEKLU;
     rsgpu::BbdImpl* bbd = nullptr;      // kind RS_SOLVER_BBD
     rsgpu::DenseImpl* dense = nullptr;  // kind RS_SOLVER_DENSE
+    rsgpu::SnImpl* sn = nullptr;        // supernodal factor and solve of the analyzed pattern
+    int sn_mode = 1;                    // 0 off, 1 large SparseLU patterns, 2 every SparseLU pattern
     i64 n = 0;
     double pivot_tol = 0.001;
     int ordering = 0;

@@ -113,6 +113,19 @@ class GpuSolver:
         pattern and keep the fastest (default on)."""
         self._L.rs_set_autotune(self.h, int(bool(on)))
 
+    def set_supernodal(self, mode: int = 1):
+        """0 off, 1 SparseLU patterns of n >= 20,000 (default), 2 every
+        SparseLU pattern: factor and solve by supernodal panels on the
+        device when the pattern is structurally symmetric."""
+        self._L.rs_set_supernodal(self.h, int(mode))
+
+    def supernodal_info(self) -> dict:
+        out = np.zeros(8, np.int64)
+        self._L.rs_supernodal_info(self.h, out)
+        keys = ["active", "supernodes", "levels", "contributions", "panel_doubles", "device_bytes",
+                "updates", "max_w"]
+        return dict(zip(keys, map(int, out)))
+
     def set_growth_check(self, limit: float):
         """Re-factor with new pivots when a device refactor's growth
         max|U|/max|A| exceeds `limit` (0 disables: the reference's behaviour)."""
@@ -304,19 +317,19 @@ def host_bbd_decompose(n, ap, ai, num_blocks):
                 border_size=n - b0.value if nb > 1 else 0)
 
 
-def host_sn_selftest(n, ap, ai, ax=None, rhs=None):
+def host_sn_selftest(n, ap, ai, ax=None, rhs=None, hot_min=0):
     """Host execution of the supernodal schedule (the order of the GPU
     kernels): SparseLU symbolic factor, supernodal plan and, with ax, the
     factor of ax and the solve of rhs. Returns (x, lx, ux, ud, stats); the
     arrays are None when ax is None."""
     L = _lib.lib()
-    st = np.zeros(10, np.int64)
+    st = np.zeros(12, np.int64)
     ap, ai = _i64(ap), _i64(ai)
-    rc = L.rs_host_sn_selftest(n, ap, ai, None, None, None, None, None, None, st)
+    rc = L.rs_host_sn_selftest(n, ap, ai, None, None, None, None, None, None, hot_min, st)
     if rc:
         raise SolverError(rc, "no supernodal plan")
     keys = ["supernodes", "levels", "contributions", "panel_doubles", "max_w", "max_h", "updates",
-            "gather_tiles", "nnz_l", "nnz_u"]
+            "gather_tiles", "nnz_l", "nnz_u", "hot_entries", "tile_records"]
     stats = dict(zip(keys, map(int, st)))
     if ax is None:
         return None, None, None, None, stats
@@ -326,7 +339,7 @@ def host_sn_selftest(n, ap, ai, ax=None, rhs=None):
     lx, ux = np.zeros(max(stats["nnz_l"], 1)), np.zeros(max(stats["nnz_u"], 1))
     ud = np.zeros(max(n, 1))
     p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    rc = L.rs_host_sn_selftest(n, ap, ai, p(ax), p(rhs), p(x), p(lx), p(ux), p(ud), st)
+    rc = L.rs_host_sn_selftest(n, ap, ai, p(ax), p(rhs), p(x), p(lx), p(ux), p(ud), hot_min, st)
     if rc:
         raise SolverError(rc, "no supernodal plan")
     return x[:n], lx[: stats["nnz_l"]], ux[: stats["nnz_u"]], ud[:n], stats

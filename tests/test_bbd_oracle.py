@@ -198,3 +198,38 @@ def test_dense_solver_matches_numpy_and_fails_like_reference():
     rc, x = O.dense_factor_solve(1, [0, 1], [0], [1e-31], [1.0])
     assert rc == 0 and x[0] == 1e31
     assert O.dense_factor_solve(1, [0, 1], [0], [1e-31], [1.0], schur=True)[0] == 4
+
+
+def test_engine_runs_bbd_solver():
+    """The oracle's Newton drivers take a BbdSolver like engine.rs's
+    Box<dyn LinearSolver>: on the power grid the DC pattern (capacitors open)
+    decomposes into blocks and a border while the TRAP patterns collapse to
+    the SparseLU fallback, and the run differs from an all-SparseLU one."""
+    from rustspice_amd import netlists as NL
+    from rustspice_amd import solver as S
+    c = NL.power_grid(64, 64, pitch=32)
+    ap, ai = S.host_stamp_pattern(c, mode=0)
+    assert S.host_bbd_decompose(len(ap) - 1, ap, ai, 64)["num_blocks"] > 1
+    b = O.Bbd(c.node_count, 64)
+    rc, xb, itb = O.tran_fixed(c, 1e-12, 20, solver=b)
+    assert rc == 0 and b.info()["blocks"] == 1  # the last pattern (TRAP) collapsed
+    o = O.NativeKlu(c.node_count)
+    o.set_sparselu()
+    rc2, xs, its = O.tran_fixed(c, 1e-12, 20, solver=o)
+    assert rc2 == 0 and (itb != its or not np.array_equal(xb, xs))
+    # with the decomposition collapsed throughout (the grid's TRAN matrix), the
+    # BBD solver is SparseLU bit for bit
+    ap, ai = S.host_stamp_pattern(c, mode=1)
+    ax = np.random.RandomState(1).rand(len(ai)) + 0.5
+    n = len(ap) - 1
+    rhs = np.random.RandomState(2).randn(n)
+    b = O.Bbd(n, 64)
+    assert b.factor(ap, ai, ax) == 0
+    xb = rhs.copy()
+    assert b.solve(xb) == 0
+    o = O.NativeKlu(n)
+    o.set_sparselu()
+    assert o.factor(ap, ai, ax) == 0
+    xs = rhs.copy()
+    assert o.solve(xs) == 0
+    assert np.array_equal(xb, xs)

@@ -10,14 +10,14 @@ from rustspice_amd import solver as S
 from tests.helpers import same_bits
 
 
-def _check(ap, ai, ax, rhs, n):
+def _check(ap, ai, ax, rhs, n, hot_min=0):
     o = O.NativeKlu(n)
     o.set_sparselu()
     assert o.factor(ap, ai, ax) == 0
     xo = rhs.copy()
     assert o.solve(xo) == 0
     want = o.block(0)
-    x, lx, ux, ud, st = S.host_sn_selftest(n, ap, ai, ax, rhs)
+    x, lx, ux, ud, st = S.host_sn_selftest(n, ap, ai, ax, rhs, hot_min)
     assert same_bits(lx, want["l_values"])
     assert same_bits(ux, want["u_values"])
     assert same_bits(ud, want["u_diag"])
@@ -33,21 +33,24 @@ def _tran_mna(c, seed):
                        tstate=(cv, ci, hv, hi, 1))
 
 
-@pytest.mark.parametrize("nx,pitch", [(8, 4), (16, 8), (24, 8), (40, 16)])
-def test_power_grid_tran_matrix(nx, pitch):
+@pytest.mark.parametrize("nx,pitch", [(8, 4), (16, 8), (24, 8), (40, 16), (72, 32)])
+@pytest.mark.parametrize("hot_min", [0, 8, 40])
+def test_power_grid_tran_matrix(nx, pitch, hot_min):
+    """hot_min sets how many supernodes must hold a position for its diagonal
+    to be summed on its own: small values exercise that path (the ground
+    node's diagonal at every size, separator nodes at the smallest)."""
     c = NL.power_grid(nx, nx, pitch=pitch)
     ap, ai, ax, rhs, n = _tran_mna(c, nx)
-    st = _check(ap, ai, ax, rhs, n)
+    st = _check(ap, ai, ax, rhs, n, hot_min)
     assert st["levels"] < st["supernodes"]
+    if hot_min and nx >= 40:
+        assert st["hot_entries"] >= 1
 
 
-def test_resistor_mesh_and_rc_mesh_inverters():
-    for c, mode in ((NL.resistor_mesh(20, 20), 0), (NL.rc_mesh_inverters(12, 12, 9), 1)):
-        if mode == 0:
-            ap, ai, ax, rhs, n = O.build_mna(c)
-        else:
-            ap, ai, ax, rhs, n = _tran_mna(c, 3)
-        _check(ap, ai, ax, rhs, n)
+def test_resistor_mesh_dc_matrix():
+    c = NL.resistor_mesh(20, 20)
+    ap, ai, ax, rhs, n = O.build_mna(c)
+    _check(ap, ai, ax, rhs, n, hot_min=6)
 
 
 def test_random_symmetric_patterns_with_zeros_and_clamps():
@@ -70,7 +73,13 @@ def test_random_symmetric_patterns_with_zeros_and_clamps():
 
 
 def test_unsymmetric_pattern_refused():
+    """MOSFET stamps (gm at (drain, gate) without (gate, drain)) make the
+    inverter-tapped mesh structurally unsymmetric: no supernodal plan."""
     ap = np.array([0, 2, 3], np.int64)
     ai = np.array([0, 1, 1], np.int64)
     with pytest.raises(S.SolverError):
         S.host_sn_selftest(2, ap, ai)
+    c = NL.rc_mesh_inverters(12, 12, 9)
+    ap, ai, ax, rhs, n = _tran_mna(c, 3)
+    with pytest.raises(S.SolverError):
+        S.host_sn_selftest(n, ap, ai)
