@@ -301,6 +301,18 @@ int32_t rs_tran_fixed(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, doub
                       int64_t nsteps, double *x_final, int64_t x_cap, int64_t *x_len,
                       int32_t *newton_iters, int32_t *status);
 
+/* Continues a fixed-step trapezoidal transient (no reference counterpart):
+ * nsteps TRAP steps of dt from time t0, Newton from x_io (*x_len entries,
+ * node_count .. x_cap) and the circuit's current capacitor history (left by
+ * rs_tran_fixed or a previous call), with the accepted-step update after
+ * each step. A run split into calls takes the same steps, iterations and
+ * values as one rs_tran_fixed call, on any solver (a BbdSolver's SparseLU
+ * fallback included). x_io and *x_len receive the last state. status: 0 ok,
+ * 2 a step failed. */
+int32_t rs_tran_steps(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, double t0, double dt,
+                      int64_t nsteps, double *x_io, int64_t x_cap, int64_t *x_len,
+                      int32_t *newton_iters, int32_t *status);
+
 /* ---- batched DC operating points (no reference counterpart: the reference
  * runs one Engine per circuit). Instance k owns devices
  * dev[dev_ptr[k] .. dev_ptr[k+1]) over node_count[k] nodes. One launch runs

@@ -37,7 +37,7 @@ EXPORTS = [
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_info",
     "rs_batch_dc_op", "rs_batch_results", "rs_circuit_set_device_newton",
     "rs_batch_phase_profile", "rs_batch_phase_cycles", "rs_dc_adjoint", "rs_tran_adjoint",
-    "rs_circuit_set_tran_graph", "rs_circuit_graph_steps",
+    "rs_circuit_set_tran_graph", "rs_circuit_graph_steps", "rs_tran_steps",
 ]
 
 

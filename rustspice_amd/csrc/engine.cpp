@@ -1272,6 +1272,34 @@ int32_t rs_tran_fixed(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, do
     return RS_OK;
 }
 
+int32_t rs_tran_steps(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double t0, double dt,
+                      int64_t nsteps, double* x_io, int64_t x_cap, int64_t* x_len, int32_t* newton_iters,
+                      int32_t* status_out) {
+    if (!x_len || *x_len < c->node_count || *x_len > x_cap)
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: x must hold node_count..x_cap entries");
+    std::vector<double> x(x_io, x_io + *x_len);
+    int32_t it = 0, status = 0;
+    rs_newton_result r{};
+    double t = t0;
+    for (int64_t k = 0; k < nsteps; k++) {
+        const double tt = t + dt;
+        int32_t rc = run_newton_stepping(c, sol, o, x, MODE_TRAN_TRAP, tt, dt, &r, &it);
+        if (rc) return rc;
+        if (!r.converged) {
+            status = 2;
+            break;
+        }
+        rc = cap_update(c, x, dt, 1, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p);
+        if (rc) return rc;
+        t = tt;
+    }
+    std::memcpy(x_io, x.data(), std::min<size_t>(x.size(), (size_t)x_cap) * 8);
+    *x_len = (int64_t)std::min<size_t>(x.size(), (size_t)x_cap);
+    if (newton_iters) *newton_iters = it;
+    if (status_out) *status_out = status;
+    return RS_OK;
+}
+
 void rs_circuit_set_tran_graph(rs_circuit* c, int32_t on) { c->use_graph = on != 0; }
 
 int64_t rs_circuit_graph_steps(const rs_circuit* c) { return c->graph_steps; }

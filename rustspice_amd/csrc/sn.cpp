@@ -2,6 +2,9 @@
 // solve (plan and schedule in sn.hpp / sn_plan.cpp, kernels in
 // sn_kernels.hip).
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "sn.hpp"
@@ -14,17 +17,36 @@ hipError_t sn_launch_scatter(const int32_t* src, const int32_t* cnt, const int64
                              const double* ax, double* v, hipStream_t s);
 hipError_t sn_launch_gather(const SnPtrs& p, int item0, int nitems, hipStream_t s);
 hipError_t sn_launch_hot(const SnPtrs& p, const int64_t* hot_ptr, const int64_t* terms, int e, hipStream_t s);
+hipError_t sn_launch_scatter_long(const int32_t* src, const int32_t* cnt, const int64_t* slot, int nruns,
+                                  const double* ax, double* v, hipStream_t s);
+hipError_t sn_launch_fwd_long(const SnPtrs& p, const int32_t* rows, int nrows, const int64_t* tptr,
+                              const int64_t* lslot, const int32_t* yidx, double* y, hipStream_t s);
 hipError_t sn_launch_factor(const SnPtrs& p, const int32_t* list, int n, hipStream_t s);
-hipError_t sn_launch_big(const SnPtrs& p, int J, int w, int hh, hipStream_t s);
+hipError_t sn_launch_big(const SnPtrs& p, const int32_t* list, const int32_t* w, const int32_t* hh, int nbig,
+                         hipStream_t s);
 hipError_t sn_launch_perm(const int32_t* perm, const double* in, double* out, int n, bool to_pos, hipStream_t s);
 hipError_t sn_launch_fwd(const SnPtrs& p, const int32_t* rows, int nrows, const int32_t* inner, int ninner,
                          double* y, hipStream_t s);
 hipError_t sn_launch_bwd(const SnPtrs& p, const int32_t* rows, const int32_t* sn_of, int nrows,
                          const int32_t* inner, int ninner, double* y, hipStream_t s);
 
+// Runs of duplicates and forward-solve rows with more terms than these are
+// folded by a workgroup each (ordered long sums).
+constexpr int32_t kSnLongRun = 64;
+constexpr int64_t kSnLongRow = 256;
+
 struct SnImpl {
     SnHost h;
     BlockSymbolic sym;
+    // scatter: short runs (a thread each) and long runs (a workgroup each)
+    DevBuf<int32_t> ss_src, ss_cnt, sl_src, sl_cnt;
+    DevBuf<int64_t> ss_slot, sl_slot;
+    int64_t n_short = 0, n_long = 0;
+    // forward solve: per level the short rows [fr[l], fr[l+1]) of fwd_rows and
+    // the long rows [fl[l], fl[l+1]) of long_rows with their flat terms
+    DevBuf<int32_t> fwd_rows, long_rows, yidx;
+    DevBuf<int64_t> long_tptr, lslot;
+    std::vector<int32_t> fr, fl;
     DevBuf<int32_t> j0, rptr, rows, csrc, ca0, ca1, rel, perm, lev_sn, rec, lrow_sn, lrow_t, sc_src, sc_cnt,
         sn_of, lev_rows, lists;
     DevBuf<int4> gitems;
@@ -34,7 +56,8 @@ struct SnImpl {
     // per level: rows [lr[l], lr[l+1]) of lev_rows; small-factor list
     // [fs[l], fs[l+1]) and inner list [in[l], in[l+1]) of lists; wide supernodes
     std::vector<int32_t> lr, fs, in;
-    std::vector<std::vector<int32_t>> big;
+    std::vector<std::vector<int32_t>> big, big_w, big_h;  // per level, widest first
+    std::vector<int32_t> big_off;                        // their lists' offsets in `lists`
     std::vector<std::vector<int32_t>> hot_at;  // hot entries per level
     int64_t bytes = 0;
 };
@@ -53,10 +76,17 @@ hipError_t up(SnImpl* p, DevBuf<T>& b, const std::vector<T>& v) {
 
 SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why) {
     SnImpl* p = new SnImpl();
+    const char* vb = std::getenv("RSGPU_VERBOSE");
+    const bool verbose = vb && vb[0] == '1';
+    const auto t0 = std::chrono::steady_clock::now();
     if (!build_sn_plan(s, ap, ai, p->h, why)) {
         delete p;
         return nullptr;
     }
+    if (verbose)
+        std::fprintf(stderr, "rsgpu: supernodal plan n=%d supernodes=%d levels=%d records=%lld in %.1f s\n", p->h.n,
+                     p->h.ns, p->h.nlev, (long long)(p->h.tile_rec.empty() ? 0 : p->h.tile_rec.back()),
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     p->sym = s;
     SnHost& h = p->h;
     // launch lists
@@ -81,10 +111,65 @@ SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::str
         p->fs.push_back((int32_t)lists_small.size());
         p->in.push_back((int32_t)lists_inner.size());
     }
+    // scatter runs by length
+    std::vector<int32_t> ss_src, ss_cnt, sl_src, sl_cnt;
+    std::vector<int64_t> ss_slot, sl_slot;
+    for (size_t r = 0; r < h.sc_src.size(); r++) {
+        const bool lng = h.sc_cnt[r] > kSnLongRun;
+        (lng ? sl_src : ss_src).push_back(h.sc_src[r]);
+        (lng ? sl_cnt : ss_cnt).push_back(h.sc_cnt[r]);
+        (lng ? sl_slot : ss_slot).push_back(h.sc_slot[r]);
+    }
+    p->n_short = (int64_t)ss_src.size();
+    p->n_long = (int64_t)sl_src.size();
+    // forward rows by the length of their term lists
+    std::vector<int32_t> fwd_rows, long_rows, yidx;
+    std::vector<int64_t> long_tptr(1, 0), lslot;
+    p->fr.assign(1, 0);
+    p->fl.assign(1, 0);
+    for (int32_t l = 0; l < h.nlev; l++) {
+        for (int32_t q = p->lr[l]; q < p->lr[l + 1]; q++) {
+            const int32_t pos = lev_rows[q];
+            int64_t terms = 0;
+            for (int64_t e2 = h.lrow_ptr[pos]; e2 < h.lrow_ptr[pos + 1]; e2++)
+                terms += h.j0[h.lrow_sn[e2] + 1] - h.j0[h.lrow_sn[e2]];
+            if (terms <= kSnLongRow) {
+                fwd_rows.push_back(pos);
+                continue;
+            }
+            long_rows.push_back(pos);
+            for (int64_t e2 = h.lrow_ptr[pos]; e2 < h.lrow_ptr[pos + 1]; e2++) {
+                const int32_t S = h.lrow_sn[e2], t = h.lrow_t[e2];
+                const int32_t wS = h.j0[S + 1] - h.j0[S], hS = h.rptr[S + 1] - h.rptr[S];
+                for (int32_t s2 = 0; s2 < wS; s2++) {
+                    lslot.push_back(h.oL[S] + (int64_t)s2 * (wS + hS) + wS + t);
+                    yidx.push_back(h.j0[S] + s2);
+                }
+            }
+            long_tptr.push_back((int64_t)lslot.size());
+        }
+        p->fr.push_back((int32_t)fwd_rows.size());
+        p->fl.push_back((int32_t)long_rows.size());
+    }
     const int32_t inner_base = (int32_t)lists_small.size();
     std::vector<int32_t> lists = lists_small;
     lists.insert(lists.end(), lists_inner.begin(), lists_inner.end());
     for (auto& x : p->in) x += inner_base;
+    p->big_w.resize(h.nlev);
+    p->big_h.resize(h.nlev);
+    p->big_off.assign(h.nlev, 0);
+    for (int32_t l = 0; l < h.nlev; l++) {
+        auto& b = p->big[l];
+        std::stable_sort(b.begin(), b.end(), [&](int32_t a, int32_t c) {
+            return h.j0[a + 1] - h.j0[a] > h.j0[c + 1] - h.j0[c];
+        });
+        p->big_off[l] = (int32_t)lists.size();
+        for (int32_t J : b) {
+            lists.push_back(J);
+            p->big_w[l].push_back(h.j0[J + 1] - h.j0[J]);
+            p->big_h[l].push_back(h.rptr[J + 1] - h.rptr[J]);
+        }
+    }
     std::vector<int4> gi(h.gitems.size() / 4);
     std::memcpy(gi.data(), h.gitems.data(), gi.size() * sizeof(int4));
     hipError_t e;
@@ -98,6 +183,10 @@ SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::str
         (e = up(p, p->tile_rec, h.tile_rec)) || (e = up(p, p->hot_slot, h.hot_slot)) ||
         (e = up(p, p->hot_ptr, h.hot_ptr)) || (e = up(p, p->hot_terms, h.hot_terms)) ||
         (e = up(p, p->lrow_ptr, h.lrow_ptr)) || (e = up(p, p->sc_slot, h.sc_slot)) ||
+        (e = up(p, p->ss_src, ss_src)) || (e = up(p, p->ss_cnt, ss_cnt)) || (e = up(p, p->ss_slot, ss_slot)) ||
+        (e = up(p, p->sl_src, sl_src)) || (e = up(p, p->sl_cnt, sl_cnt)) || (e = up(p, p->sl_slot, sl_slot)) ||
+        (e = up(p, p->fwd_rows, fwd_rows)) || (e = up(p, p->long_rows, long_rows)) || (e = up(p, p->yidx, yidx)) ||
+        (e = up(p, p->long_tptr, long_tptr)) || (e = up(p, p->lslot, lslot)) ||
         (e = p->v.ensure((size_t)std::max<int64_t>(h.nvals, 1))) || (e = p->ud.ensure(h.n)) ||
         (e = p->y.ensure(h.n))) {
         if (why) *why = std::string("device upload: ") + hipGetErrorString(e);
@@ -121,15 +210,18 @@ hipError_t sn_factor(SnImpl* p, const double* ax, hipStream_t s) {
     const SnHost& h = p->h;
     hipError_t e;
     if ((e = sn_launch_zero(p->v.p, h.nvals, s)) ||
-        (e = sn_launch_scatter(p->sc_src.p, p->sc_cnt.p, p->sc_slot.p, (int64_t)h.sc_src.size(), ax, p->v.p, s)))
+        (e = sn_launch_scatter(p->ss_src.p, p->ss_cnt.p, p->ss_slot.p, p->n_short, ax, p->v.p, s)) ||
+        (e = sn_launch_scatter_long(p->sl_src.p, p->sl_cnt.p, p->sl_slot.p, (int)p->n_long, ax, p->v.p, s)))
         return e;
     for (int32_t l = 0; l < h.nlev; l++) {
         if ((e = sn_launch_gather(p->ptr, h.glev_ptr[l], h.glev_ptr[l + 1] - h.glev_ptr[l], s))) return e;
         for (int32_t he : p->hot_at[l])
             if ((e = sn_launch_hot(p->ptr, p->hot_ptr.p, p->hot_terms.p, he, s))) return e;
         if ((e = sn_launch_factor(p->ptr, p->lists.p + p->fs[l], p->fs[l + 1] - p->fs[l], s))) return e;
-        for (int32_t J : p->big[l])
-            if ((e = sn_launch_big(p->ptr, J, h.j0[J + 1] - h.j0[J], h.rptr[J + 1] - h.rptr[J], s))) return e;
+        if (!p->big[l].empty() &&
+            (e = sn_launch_big(p->ptr, p->lists.p + p->big_off[l], p->big_w[l].data(), p->big_h[l].data(),
+                               (int)p->big[l].size(), s)))
+            return e;
     }
     return hipSuccess;
 }
@@ -139,7 +231,9 @@ hipError_t sn_solve(SnImpl* p, const double* rhs, double* x, hipStream_t s) {
     hipError_t e;
     if ((e = sn_launch_perm(p->perm.p, rhs, p->y.p, h.n, true, s))) return e;
     for (int32_t l = 0; l < h.nlev; l++)
-        if ((e = sn_launch_fwd(p->ptr, p->lev_rows.p + p->lr[l], p->lr[l + 1] - p->lr[l], p->lists.p + p->in[l],
+        if ((e = sn_launch_fwd_long(p->ptr, p->long_rows.p + p->fl[l], p->fl[l + 1] - p->fl[l],
+                                    p->long_tptr.p + p->fl[l], p->lslot.p, p->yidx.p, p->y.p, s)) ||
+            (e = sn_launch_fwd(p->ptr, p->fwd_rows.p + p->fr[l], p->fr[l + 1] - p->fr[l], p->lists.p + p->in[l],
                                p->in[l + 1] - p->in[l], p->y.p, s)))
             return e;
     for (int32_t l = h.nlev - 1; l >= 0; l--)

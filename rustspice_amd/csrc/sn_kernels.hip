@@ -43,8 +43,12 @@ __global__ __launch_bounds__(kT) void k_sn_scatter(const int32_t* __restrict__ s
 // One workgroup per gather tile: the tile is held in LDS, its records are
 // applied in order (a barrier between records), each entry of a record by one
 // thread, its terms ascending with zero multipliers skipped.
+constexpr int kStageMin = 256;  // record entries from which L and U pass through LDS
+
 __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
     __shared__ double T[kTile * kTile];
+    __shared__ double A[kPanel * kTile];  // A[q * kTile + i]: L_S(row i, s0 + q)
+    __shared__ double B[kTile * kPanel];  // B[j * kPanel + q]: U_S(s0 + q, column j)
     const int it = item0 + blockIdx.x;
     const int4 g = p.gitems[it];
     const int J = g.x, kind = g.y, r0 = g.z, c0 = g.w;
@@ -71,7 +75,45 @@ __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
         const int32_t* RS = p.rows + p.rptr[S];
         const double* LS = p.v + p.oL[S] + wS;  // LS[s * ldS + row]: L_S(R_S[row], s)
         const double* US = p.v + p.oU[S];       // US[col * wS + s]:  U_S(s, R_S[col])
-        const int mt = t1 - t0, cnt = mt * (u1 - u0);
+        const int mt = t1 - t0, mu = u1 - u0, cnt = mt * mu;
+        if (cnt >= kStageMin && wS >= kPanel / 4) {
+            // many entries: the record's L rows and U columns pass through LDS
+            // kPanel terms at a time; each thread keeps its entries across the
+            // chunks, so every entry still takes its terms in order
+            const int nli = kind == 0 ? mt : mu, nuj = kind == 0 ? mu : mt;
+            const int l0 = kind == 0 ? t0 : u0, j0c = kind == 0 ? u0 : t0;
+            for (int s0 = 0; s0 < wS; s0 += kPanel) {
+                const int sb = min(kPanel, wS - s0);
+                for (int e = threadIdx.x; e < kPanel * kTile; e += kT) {
+                    const int i = e % kTile, q = e / kTile;
+                    A[q * kTile + i] = (i < nli && q < sb) ? LS[(int64_t)(s0 + q) * ldS + l0 + i] : 0.0;
+                    const int jj = e / kPanel, qq = e % kPanel;
+                    B[jj * kPanel + qq] = (jj < nuj && qq < sb) ? US[(int64_t)(j0c + jj) * wS + s0 + qq] : 0.0;
+                }
+                __syncthreads();
+                for (int e = threadIdx.x; e < cnt; e += kT) {
+                    const int t = t0 + e % mt, u = u0 + e / mt;
+                    int tr, tc;
+                    if (kind == 0) {
+                        tr = rl[t] - r0; tc = RS[u] - jj0 - c0;
+                    } else {
+                        tr = RS[u] - jj0 - r0; tc = rl[t] - w - c0;
+                    }
+                    bool hot = false;
+                    for (int k = 0; k < p.nhot; k++) hot |= p.hot_slot[k] == base_off + (int64_t)tc * ld + tr;
+                    if (hot) continue;
+                    const int li = kind == 0 ? t - t0 : u - u0, uj = kind == 0 ? u - u0 : t - t0;
+                    double x = T[tc * kTile + tr];
+                    for (int q = 0; q < sb; q++) {
+                        const double uu = B[uj * kPanel + q];
+                        if (uu != 0.0) x -= A[q * kTile + li] * uu;
+                    }
+                    T[tc * kTile + tr] = x;
+                }
+                __syncthreads();
+            }
+            continue;
+        }
         for (int e = threadIdx.x; e < cnt; e += kT) {
             const int t = t0 + e % mt, u = u0 + e / mt;
             int tr, tc, lrow, ucol;
@@ -101,38 +143,74 @@ __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
     }
 }
 
-// One wavefront per entry summed on its own: lanes load 64 terms at a time
-// (the next 64 while lane 0 subtracts the current ones in order).
-__global__ __launch_bounds__(64) void k_sn_hot(SnPtrs p, const int64_t* __restrict__ hot_ptr,
-                                               const int64_t* __restrict__ terms, int e) {
-    __shared__ double SL[64], SU[64];
-    const int lane = threadIdx.x;
-    const int64_t q0 = hot_ptr[e], q1 = hot_ptr[e + 1];
-    double x = lane == 0 ? p.v[p.hot_slot[e]] : 0.0;
-    double nl = 0.0, nu = 0.0;
-    if (q0 + lane < q1) {
-        nl = p.v[terms[2 * (q0 + lane)]];
-        nu = p.v[terms[2 * (q0 + lane) + 1]];
-    }
-    for (int64_t b = q0; b < q1; b += 64) {
-        const int c = (int)min((int64_t)64, q1 - b);
-        SL[lane] = nl;
-        SU[lane] = nu;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        if (b + 64 + lane < q1) {
-            nl = p.v[terms[2 * (b + 64 + lane)]];
-            nu = p.v[terms[2 * (b + 64 + lane) + 1]];
-        }
-        if (lane == 0)
-            for (int i = 0; i < c; i++) {
-                const double uu = SU[i];
-                if (uu != 0.0) x -= SL[i] * uu;
+// ---- ordered long sums -----------------------------------------------------
+// One workgroup folds a long list of terms into one value in the list's
+// order: waves 1..3 compute the next kChunk terms into LDS while thread 0
+// folds the current chunk (reads eight at a time, the additions serial).
+constexpr int kChunk = 1024;
+
+template <bool kSub, class Term>
+__device__ double fold_long(double x, int64_t q0, int64_t q1, Term term) {
+    __shared__ double buf[2][kChunk];
+    const int tid = threadIdx.x, wave = tid / 64;
+    for (int i = tid; i < kChunk && q0 + i < q1; i += kT) buf[0][i] = term(q0 + i);
+    __syncthreads();
+    int k = 0;
+    for (int64_t b = q0; b < q1; b += kChunk, k++) {
+        const int c = (int)min((int64_t)kChunk, q1 - b), cur = k & 1;
+        if (wave > 0) {
+            const int64_t nb = b + kChunk;
+            for (int i = tid - 64; i < kChunk && nb + i < q1; i += kT - 64) buf[cur ^ 1][i] = term(nb + i);
+        } else if (tid == 0) {
+            const double* B = buf[cur];
+            int i = 0;
+            for (; i + 8 <= c; i += 8) {
+                double t[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) t[j] = B[i + j];
+#pragma unroll
+                for (int j = 0; j < 8; j++) x = kSub ? x - t[j] : x + t[j];
             }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
+            for (; i < c; i++) x = kSub ? x - B[i] : x + B[i];
+        }
+        __syncthreads();
     }
-    if (lane == 0) p.v[p.hot_slot[e]] = x;
+    return x;
+}
+
+// a run of many duplicate CSC entries (the ground diagonal): 0.0 + the run
+__global__ __launch_bounds__(kT) void k_sn_scatter_long(const int32_t* __restrict__ src,
+                                                        const int32_t* __restrict__ cnt,
+                                                        const int64_t* __restrict__ slot,
+                                                        const double* __restrict__ ax, double* __restrict__ v) {
+    const int r = blockIdx.x;
+    const int64_t s0 = src[r];
+    const double x = fold_long<false>(0.0, s0, s0 + cnt[r], [&](int64_t q) { return ax[q]; });
+    if (threadIdx.x == 0) v[slot[r]] = x;
+}
+
+// an entry summed on its own: x - L * U over its flat term list, zero
+// multipliers skipped (a skipped term is +0.0, which leaves x unchanged)
+__global__ __launch_bounds__(kT) void k_sn_hot(SnPtrs p, const int64_t* __restrict__ hot_ptr,
+                                               const int64_t* __restrict__ terms, int e) {
+    const double* v = p.v;
+    const double x = fold_long<true>(v[p.hot_slot[e]], hot_ptr[e], hot_ptr[e + 1], [&](int64_t q) {
+        const double u = v[terms[2 * q + 1]];
+        return u != 0.0 ? v[terms[2 * q]] * u : 0.0;
+    });
+    if (threadIdx.x == 0) p.v[p.hot_slot[e]] = x;
+}
+
+// forward solve of a row with a long list: y[p] - L(p, j) y[j], j ascending
+__global__ __launch_bounds__(kT) void k_sn_fwd_long(SnPtrs p, const int32_t* __restrict__ rows,
+                                                    const int64_t* __restrict__ tptr,
+                                                    const int64_t* __restrict__ lslot,
+                                                    const int32_t* __restrict__ yidx, double* __restrict__ y) {
+    const int r = blockIdx.x;
+    const double* v = p.v;
+    const double x = fold_long<true>(y[rows[r]], tptr[r], tptr[r + 1],
+                                     [&](int64_t q) { return v[lslot[q]] * y[yidx[q]]; });
+    if (threadIdx.x == 0) y[rows[r]] = x;
 }
 
 // One workgroup per supernode narrower than kSnBigW: right-looking over its
@@ -172,61 +250,125 @@ __global__ __launch_bounds__(kT) void k_sn_factor(SnPtrs p, const int32_t* __res
 }
 
 // ---- wide supernodes: blocked right-looking over panels of kPanel columns --
-// (i) the panel [t0, t1) over all rows [t0, W), one workgroup
-__global__ __launch_bounds__(kT) void k_sn_panel(SnPtrs p, int J, int t0) {
-    __shared__ double sd;
-    const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
-    const int W = w + hh, t1 = min(t0 + kPanel, w);
-    double* LP = p.v + p.oL[J];
-    for (int t = t0; t < t1; t++) {
-        if (threadIdx.x == 0) {
-            const double d = clamp_pivot(LP[(int64_t)t * W + t]);
-            LP[(int64_t)t * W + t] = d;
-            p.ud[jj0 + t] = d;
-            sd = d;
-        }
+// (i + ii) one launch per panel step. Every workgroup loads the panel's
+// diagonal block D = LP([t0, t1), [t0, t1)) into LDS and factors it there
+// (the same operations in every workgroup, so the same bits); workgroup 0
+// writes D and the pivots back. Then each workgroup takes a chunk of
+//  - rows r >= t1 of the panel's columns (one thread per row): for t in order,
+//    L(r, t) = x[t] / d_t, then x[c] -= L(r, t) U(t, c) for the later panel
+//    columns c, skipping zero U(t, c) - the terms each entry receives in the
+//    column-by-column factor, in the same order;
+//  - columns right of the panel in LP and every UP column (one thread per
+//    column): for t in order, x[r] -= L(r, t) x[t] for the later panel rows r,
+//    skipping zero x[t].
+constexpr int kRowsPerBlock = kT;
+
+__device__ void panel_diag(double* D, const double* LP, int W, int t0, int nb) {
+    // D[c * kPanel + r], factored right-looking with the pivot clamp
+    for (int e = threadIdx.x; e < kPanel * kPanel; e += kT) {
+        const int r = e % kPanel, c = e / kPanel;
+        D[e] = (r < nb && c < nb) ? LP[(int64_t)(t0 + c) * W + t0 + r] : 0.0;
+    }
+    __syncthreads();
+    for (int t = 0; t < nb; t++) {
+        if (threadIdx.x == 0) D[t * kPanel + t] = clamp_pivot(D[t * kPanel + t]);
         __syncthreads();
-        const double d = sd;
-        for (int r = t + 1 + threadIdx.x; r < W; r += kT) LP[(int64_t)t * W + r] = LP[(int64_t)t * W + r] / d;
+        const double d = D[t * kPanel + t];
+        if (threadIdx.x > t && threadIdx.x < nb) D[t * kPanel + threadIdx.x] = D[t * kPanel + threadIdx.x] / d;
         __syncthreads();
-        const int m = W - t - 1, mw = t1 - t - 1;
-        for (int e = threadIdx.x; e < m * mw; e += kT) {
+        const int m = nb - t - 1;
+        for (int e = threadIdx.x; e < m * m; e += kT) {
             const int r = t + 1 + e % m, c = t + 1 + e / m;
-            const double u = LP[(int64_t)c * W + t];
-            if (u != 0.0) LP[(int64_t)c * W + r] -= LP[(int64_t)t * W + r] * u;
+            const double u = D[c * kPanel + t];
+            if (u != 0.0) D[c * kPanel + r] -= D[t * kPanel + r] * u;
         }
         __syncthreads();
     }
 }
 
-// (ii) the U block row: rows [t0, t1) of the later LP columns and of every
-// UP column, one thread per column, the panel's pivots in order
-__global__ __launch_bounds__(kT) void k_sn_urow(SnPtrs p, int J, int t0) {
+__global__ __launch_bounds__(kT) void k_sn_panel(SnPtrs p, const int32_t* __restrict__ list, int t0) {
+    __shared__ double D[kPanel * kPanel];
+    const int J = list[blockIdx.y];
     const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
-    const int W = w + hh, t1 = min(t0 + kPanel, w);
-    const int k = blockIdx.x * kT + threadIdx.x;
-    const int nlp = w - t1;
-    if (k >= nlp + hh) return;
-    const double* LP = p.v + p.oL[J];
-    double* U = k < nlp ? p.v + p.oL[J] + (int64_t)(t1 + k) * W : p.v + p.oU[J] + (int64_t)(k - nlp) * w;
-    for (int t = t0; t < t1; t++) {
-        const double u = U[t];
-        if (u == 0.0) continue;
-        for (int r = t + 1; r < t1; r++) U[r] -= LP[(int64_t)t * W + r] * u;
+    const int W = w + hh, t1 = min(t0 + kPanel, w), nb = t1 - t0;
+    const int nrow = W - t1, ncol = (w - t1) + hh;
+    const int nrb = (nrow + kRowsPerBlock - 1) / kRowsPerBlock, ncb = (ncol + kT - 1) / kT;
+    if ((int)blockIdx.x >= max(nrb + ncb, 1)) return;  // the launch covers the widest supernode
+    double* LP = p.v + p.oL[J];
+    panel_diag(D, LP, W, t0, nb);  // D stays unwritten here: other workgroups still read it
+    const int b = blockIdx.x;
+    if (b < nrb) {  // rows below the diagonal block
+        const int r = t1 + b * kRowsPerBlock + threadIdx.x;
+        if (r >= W) return;
+        double x[kPanel];
+#pragma unroll
+        for (int c = 0; c < kPanel; c++) x[c] = c < nb ? LP[(int64_t)(t0 + c) * W + r] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kPanel; t++) {
+            if (t < nb) {
+                const double l = x[t] / D[t * kPanel + t];
+                x[t] = l;
+#pragma unroll
+                for (int c = t + 1; c < kPanel; c++) {
+                    const double u = D[c * kPanel + t];
+                    if (c < nb && u != 0.0) x[c] -= l * u;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kPanel; c++)
+            if (c < nb) LP[(int64_t)(t0 + c) * W + r] = x[c];
+    } else if (b < nrb + ncb) {  // columns right of the panel (U block row)
+        const int k = (b - nrb) * kT + threadIdx.x;
+        if (k >= ncol) return;
+        const int nlp = w - t1;
+        double* U = k < nlp ? LP + (int64_t)(t1 + k) * W : p.v + p.oU[J] + (int64_t)(k - nlp) * w;
+        double x[kPanel];
+#pragma unroll
+        for (int r = 0; r < kPanel; r++) x[r] = r < nb ? U[t0 + r] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kPanel; t++) {
+            const double u = x[t];
+            if (t >= nb || u == 0.0) continue;
+#pragma unroll
+            for (int r = t + 1; r < kPanel; r++)
+                if (r < nb) x[r] -= D[t * kPanel + r] * u;
+        }
+#pragma unroll
+        for (int r = 0; r < kPanel; r++)
+            if (r < nb) U[t0 + r] = x[r];
     }
+}
+
+// the factored diagonal block and its pivots, written after every workgroup
+// of the panel step has read the original block
+__global__ __launch_bounds__(kT) void k_sn_pdiag(SnPtrs p, const int32_t* __restrict__ list, int t0) {
+    __shared__ double D[kPanel * kPanel];
+    const int J = list[blockIdx.x];
+    const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
+    const int W = w + hh, t1 = min(t0 + kPanel, w), nb = t1 - t0;
+    double* LP = p.v + p.oL[J];
+    panel_diag(D, LP, W, t0, nb);
+    for (int e = threadIdx.x; e < nb * nb; e += kT) {
+        const int r = e % nb, c = e / nb;
+        LP[(int64_t)(t0 + c) * W + t0 + r] = D[c * kPanel + r];
+    }
+    if (threadIdx.x < nb) p.ud[jj0 + t0 + threadIdx.x] = D[threadIdx.x * kPanel + threadIdx.x];
 }
 
 // (iii) trailing update of the tiles right of and below the panel: LP rows
 // [t1, W) x columns [t1, w) and UP rows [t1, w) x columns [0, h); the panel's
 // L rows and U columns are staged in LDS, each thread owns 16 entries and
 // subtracts their kPanel terms in order.
-__global__ __launch_bounds__(kT) void k_sn_trail(SnPtrs p, int J, int t0) {
+__global__ __launch_bounds__(kT) void k_sn_trail(SnPtrs p, const int32_t* __restrict__ list, int t0) {
     __shared__ double A[kTile * kPanel];  // A[t * kTile + r]
     __shared__ double B[kTile * kPanel];  // B[c * kPanel + t]
+    const int J = list[blockIdx.y];
     const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
     const int W = w + hh, t1 = min(t0 + kPanel, w), nb = t1 - t0;
     const int nrl = (W - t1 + kTile - 1) / kTile, ncl = (w - t1 + kTile - 1) / kTile;
-    const int nru = (w - t1 + kTile - 1) / kTile;
+    const int nru = (w - t1 + kTile - 1) / kTile, ncu = (hh + kTile - 1) / kTile;
+    if ((int)blockIdx.x >= nrl * ncl + nru * ncu) return;  // the launch covers the widest supernode
     int b = blockIdx.x, kind, r0, c0;
     if (b < nrl * ncl) {
         kind = 0; r0 = t1 + (b % nrl) * kTile; c0 = t1 + (b / nrl) * kTile;
@@ -367,7 +509,19 @@ hipError_t sn_launch_gather(const SnPtrs& p, int item0, int nitems, hipStream_t 
     return hipGetLastError();
 }
 hipError_t sn_launch_hot(const SnPtrs& p, const int64_t* hot_ptr, const int64_t* terms, int e, hipStream_t s) {
-    hipLaunchKernelGGL(k_sn_hot, dim3(1), dim3(64), 0, s, p, hot_ptr, terms, e);
+    hipLaunchKernelGGL(k_sn_hot, dim3(1), dim3(kT), 0, s, p, hot_ptr, terms, e);
+    return hipGetLastError();
+}
+hipError_t sn_launch_scatter_long(const int32_t* src, const int32_t* cnt, const int64_t* slot, int nruns,
+                                  const double* ax, double* v, hipStream_t s) {
+    if (nruns <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sn_scatter_long, dim3(nruns), dim3(kT), 0, s, src, cnt, slot, ax, v);
+    return hipGetLastError();
+}
+hipError_t sn_launch_fwd_long(const SnPtrs& p, const int32_t* rows, int nrows, const int64_t* tptr,
+                              const int64_t* lslot, const int32_t* yidx, double* y, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sn_fwd_long, dim3(nrows), dim3(kT), 0, s, p, rows, tptr, lslot, yidx, y);
     return hipGetLastError();
 }
 hipError_t sn_launch_factor(const SnPtrs& p, const int32_t* list, int n, hipStream_t s) {
@@ -375,17 +529,28 @@ hipError_t sn_launch_factor(const SnPtrs& p, const int32_t* list, int n, hipStre
     hipLaunchKernelGGL(k_sn_factor, dim3(n), dim3(kT), 0, s, p, list);
     return hipGetLastError();
 }
-hipError_t sn_launch_big(const SnPtrs& p, int J, int w, int hh, hipStream_t s) {
-    const int W = w + hh;
-    for (int t0 = 0; t0 < w; t0 += kPanel) {
-        const int t1 = std::min(t0 + kPanel, w);
-        hipLaunchKernelGGL(k_sn_panel, dim3(1), dim3(kT), 0, s, p, J, t0);
-        const int ncol = (w - t1) + hh;
-        if (ncol > 0) hipLaunchKernelGGL(k_sn_urow, dim3(blocks_for(ncol)), dim3(kT), 0, s, p, J, t0);
-        const int nrl = (W - t1 + kTile - 1) / kTile, ncl = (w - t1 + kTile - 1) / kTile;
-        const int nru = (w - t1 + kTile - 1) / kTile, ncu = (hh + kTile - 1) / kTile;
-        const int nt = nrl * ncl + nru * ncu;
-        if (nt > 0) hipLaunchKernelGGL(k_sn_trail, dim3(nt), dim3(kT), 0, s, p, J, t0);
+// Wide supernodes of one level, widest first (list on the device, w and h on
+// the host in the same order): panel step t0 runs for the first k of them,
+// those wider than t0; a launch's grid covers its widest member.
+hipError_t sn_launch_big(const SnPtrs& p, const int32_t* list, const int32_t* w, const int32_t* hh, int nbig,
+                         hipStream_t s) {
+    for (int t0 = 0; nbig > 0 && t0 < w[0]; t0 += kPanel) {
+        int k = 0, nt = 0;
+        for (; k < nbig && w[k] > t0; k++) {
+            const int t1 = std::min(t0 + kPanel, w[k]), W = w[k] + hh[k];
+            const int nrl = (W - t1 + kTile - 1) / kTile, ncl = (w[k] - t1 + kTile - 1) / kTile;
+            const int nru = (w[k] - t1 + kTile - 1) / kTile, ncu = (hh[k] + kTile - 1) / kTile;
+            nt = std::max(nt, nrl * ncl + nru * ncu);
+        }
+        int nblk = 1;
+        for (int j = 0; j < k; j++) {
+            const int t1 = std::min(t0 + kPanel, w[j]);
+            const int nrow = w[j] + hh[j] - t1, nc = (w[j] - t1) + hh[j];
+            nblk = std::max(nblk, (nrow + kRowsPerBlock - 1) / kRowsPerBlock + (nc + kT - 1) / kT);
+        }
+        hipLaunchKernelGGL(k_sn_panel, dim3(nblk, k), dim3(kT), 0, s, p, list, t0);
+        hipLaunchKernelGGL(k_sn_pdiag, dim3(k), dim3(kT), 0, s, p, list, t0);
+        if (nt > 0) hipLaunchKernelGGL(k_sn_trail, dim3(nt, k), dim3(kT), 0, s, p, list, t0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

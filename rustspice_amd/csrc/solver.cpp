@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cmath>
 #include <map>
@@ -108,7 +109,13 @@ int32_t do_analyze(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail)
         f.noffdiag = od;
     } else {
         f.sym.resize(1);
+        const auto t0 = std::chrono::steady_clock::now();
         block_symbolic(n, ap, ai, natural, f.sym[0], h->ordering == 4);
+        const char* vb = std::getenv("RSGPU_VERBOSE");
+        if (vb && vb[0] == '1' && n >= 100000)
+            std::fprintf(stderr, "rsgpu: symbolic factor n=%lld nnz(L+U)=%lld in %.1f s\n", (long long)n,
+                         (long long)(f.sym[0].lri.size() + f.sym[0].uri.size()),
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     }
     for (auto& s : f.sym) {
         f.nnz_l += (i64)s.lri.size();

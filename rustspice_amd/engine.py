@@ -89,6 +89,9 @@ def _bind():
     L.rs_tran_fixed.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, vp,
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32)]
+    L.rs_tran_steps.restype = C.c_int32
+    L.rs_tran_steps.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_double, C.c_int64, vp,
+                                C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.rs_circuit_set_device_newton.argtypes = [vp, C.c_int32]
     L.rs_circuit_set_bsim_models.restype = C.c_int32
     L.rs_circuit_set_bsim_models.argtypes = [vp, vp, C.c_int64]
@@ -298,6 +301,19 @@ class GpuCircuit:
         self._check(self._L.rs_tran_fixed(self.h, solver.h, C.byref(opts), dt, nsteps, _ptr(x), cap,
                                           C.byref(xl), C.byref(it), C.byref(st)))
         return st.value, x[: xl.value].copy(), it.value
+
+    def tran_steps(self, solver: GpuSolver, t0, dt, nsteps, x, opts=None):
+        """Continues a fixed-step TRAP transient from time t0 and state x with
+        the circuit's current capacitor history (after tran_fixed or a
+        previous call). Returns (status, x, newton_iters)."""
+        opts = opts or default_opts()
+        cap = max(len(x), self.circuit.node_count + self.circuit.ndev + 8)
+        xb = np.zeros(cap)
+        xb[: len(x)] = x
+        xl, it, st = C.c_int64(len(x)), C.c_int32(), C.c_int32()
+        self._check(self._L.rs_tran_steps(self.h, solver.h, C.byref(opts), t0, dt, nsteps, _ptr(xb), cap,
+                                          C.byref(xl), C.byref(it), C.byref(st)))
+        return st.value, xb[: xl.value].copy(), it.value
 
 
 def mos_eval(models, model_idx, level, bias):
