@@ -1,24 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark: refactor+solve/s of the GPU LinearSolver drop-in on the
-500-stage level-1 CMOS inverter chain transient (.tran 5p 10n, max step 5p).
+"""Benchmark: refactor+solve/s of the GPU LinearSolver drop-in on the RC
+power-grid transient (1024x1024 grid, n = 1,048,833: segment R U[0.05, 0.5]
+ohm, node C U[1, 10] fF, 1.0 V pads every 64 nodes, 4,096 pulsed 10 mA loads;
+trapezoidal steps of 1 ps through BbdSolver with 64 partitions).
 
-One step = one Engine TRAN run of that circuit through the GPU backend
-(crates/sim-core/src/engine.rs:333-632): device stamping, refactor and solve
-in every Newton iteration. Under the reference's own semantics the TRAN's
-initial DC operating point ends MaxIters after 100 Newton iterations (the
-level-1 PMOS frame of bsim/evaluate.rs:298-321 gives the chain no DC
-solution), so a step is 100 stamp+refactor+solve iterations on the n=504 DC
-matrix followed by the reference's "DC operating point failed" exit.
+One step = one trapezoidal timestep of that circuit (crates/sim-core/src/
+engine.rs:333-632 with a fixed step): device stamping, then in every Newton
+iteration a refactor and a solve of the whole matrix. BbdSolver(64) on this
+pattern finds a single block (every node reaches ground through its
+capacitor), so it factors the whole matrix with its SparseLU fallback
+(bbd_solver.rs:414-504, sparse_lu.rs:440-618); here that is the supernodal
+GPU factor, bit-identical to SparseLU. The initial DC operating point runs
+through SparseLU directly: BbdSolver's DC pattern has a 225k-node border.
 
 value = refactor+solve calls per second over the timed steps (inputs resident
-in device memory). The DC OP runs device-resident: stamping, factor, solve and
-the convergence test of all 100 iterations in one k_batch_dcop launch. The
-host-driven loop (one refactor and one solve launch per iteration, the
-convergence test on the host) is measured beside it.
-roofline = the dominant kernel's algorithmic bytes per launch over its average
-launch time from HIP events on its launch stream during the timed region.
-cpu_baseline = the C restatement of NativeKluSolver + Engine (oracle/) on one
-host core, same circuit, same semantics, checked against the GPU result.
+in device memory; the Newton loop is driven from the host).
+roofline = the dominant kernel (k_sn_gather, the descendant contributions of
+the supernodal factor): its multiply-subtract flops per launch over its
+average launch time from HIP events on its launch stream, against the FP64
+peak (the factor does far more flops per compulsory byte than the FP64/HBM
+balance of about 10: compute-bound).
+cpu_baseline = the oracle's restatement of the reference's column-parallel
+factor (native_klu.rs:402-576) and solve on the same matrix, on the host's
+cores, checked value for value against the GPU factor.
+
+The earlier workloads (500-stage inverter chain, 4,096 batched Monte-Carlo
+DC-OPs, the adjoints, the 64x64 mesh) follow as secondary sections.
 """
 import argparse
 import json
@@ -43,6 +50,202 @@ def _algorithmic_bytes(nnz_a, nnz_l, nnz_u, n):
     refactor = 12 * nnz_a + 12 * (nnz_l + nnz_u) + 8 * n
     solve = 12 * (nnz_l + nnz_u) + 24 * n
     return refactor, solve
+
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _log(msg):
+    """Progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def _cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def _traffic_from_profile(name, key="traffic_bytes_per_launch:"):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split(":")[1])
+    return None
+
+
+def power_grid_tran(args, E, NL, S, torch, np):
+    """The headline: DC operating point, then warm-up and timed trapezoidal
+    steps of the power grid through BbdSolver(64), with HIP-event kernel
+    timing over the timed steps and the KCL residual of one more step."""
+    import scipy.sparse as sp
+    from rustspice_amd import _lib
+
+    nx = args.grid
+    dt = NL.C3_TRAN["dt"]
+    t0 = time.perf_counter()
+    c = NL.power_grid(nx, nx)
+    gc = E.GpuCircuit(c)
+    _log(f"power grid {nx}x{nx}: {c.ndev} devices, {c.node_count} nodes ({time.perf_counter() - t0:.1f} s)")
+    dcs = S.GpuSolver(c.node_count, S.SPARSELU)
+    t1 = time.perf_counter()
+    st_dc, x, it_dc = gc.tran_fixed(dcs, dt, 0)  # DC operating point, no steps
+    dc_s = time.perf_counter() - t1
+    dc_info = dcs.supernodal_info()
+    _log(f"DC operating point (SparseLU): status {st_dc}, {it_dc} Newton iterations, {dc_s:.1f} s")
+    dcs.close()
+    g = S.GpuSolver(c.node_count, S.BBD)
+    g.set_bbd_blocks(64)
+    t = 0.0
+    t1 = time.perf_counter()
+    st_w, x, it_w = gc.tran_steps(g, t, dt, args.warmup, x)
+    t += args.warmup * dt
+    warm_s = time.perf_counter() - t1
+    info = g.supernodal_info()
+    bbd = g.bbd_info()
+    _log(f"{args.warmup} warm-up steps (first: analysis of the TRAN pattern): status {st_w}, "
+         f"{it_w} iterations, {warm_s:.1f} s")
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    t1 = time.perf_counter()
+    st, x, iters = gc.tran_steps(g, t, dt, args.steps, x)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    prof = _lib.profile_get()
+    _lib.profile_enable(False)
+    t += args.steps * dt
+    _log(f"{args.steps} timed steps: status {st}, {iters} Newton iterations, {elapsed:.2f} s")
+    # KCL residual of one more step: its matrix and right-hand side from the
+    # device stamp before the step, its solution from the step itself
+    ap, ai, ax, rhs, n = gc.stamp(x, mode=E.MODE_TRAN_TRAP, t=t + dt, dt=dt)
+    st_k, xk, it_k = gc.tran_steps(g, t, dt, 1, x)
+    A = sp.csc_matrix((ax, ai, ap), shape=(n, n))
+    r = A @ xk[:n] - rhs
+    scale = np.abs(A).max(axis=1).toarray().ravel() * np.abs(xk[:n]).max() + np.abs(rhs)
+    rel = np.abs(r) / np.maximum(scale, 1e-300)
+    nc = c.node_count
+    v = xk[1:nc] - xk[0]
+    kcl = dict(step_status=int(st_k), newton_iters=int(it_k),
+               max_rel_residual_nonground_rows=float(rel[1:].max()),
+               ground_row_rel_residual=float(rel[0]), ground_potential_x0=float(xk[0]),
+               node_v_minus_ground_min=float(v.min()), node_v_minus_ground_max=float(v.max()))
+    _log(f"KCL residual (rows other than ground): {kcl['max_rel_residual_nonground_rows']:.2e}")
+    calls = int(iters)
+    value = calls / elapsed
+    pat_sizes = np.zeros(3, np.int64)
+    g._L.rs_get_factor_pattern(g.h, pat_sizes, None, None, None, None, None)
+    nnz_l, nnz_u = int(pat_sizes[1]), int(pat_sizes[2])
+    kern = {}
+    for name in ("refactor", "solve", "sn_gather"):
+        p = prof[name]
+        kern[name] = dict(avg_us=p["avg_us"], count=p["count"], total_ms=p["ms_total"])
+    nfac = max(kern["refactor"]["count"], 1)
+    g_flops = 2.0 * info["gather_updates"]  # per factor
+    f_flops = 2.0 * info["updates"] + nnz_l
+    launches = kern["sn_gather"]["count"] / nfac
+    gather_tflops = g_flops * nfac / (kern["sn_gather"]["total_ms"] * 1e-3) / 1e12
+    kern["sn_gather"].update(flops_per_launch=g_flops / max(launches, 1), launches_per_factor=launches,
+                             tflops=gather_tflops)
+    kern["refactor"].update(flops=f_flops, tflops=f_flops / (kern["refactor"]["avg_us"] * 1e-6) / 1e12,
+                            compulsory_bytes=12 * (len(ai) + nnz_l + nnz_u) + 8 * n)
+    kern["refactor"]["flops_per_compulsory_byte"] = f_flops / kern["refactor"]["compulsory_bytes"]
+    kern["solve"].update(bytes=12 * (nnz_l + nnz_u) + 24 * n)
+    kern["solve"]["gbs"] = kern["solve"]["bytes"] / (kern["solve"]["avg_us"] * 1e-6) / 1e9
+    roof = dict(bound="mfma", achieved=gather_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=gather_tflops / FP64_PEAK_TFLOPS,
+                traffic=_traffic_from_profile("k_sn_gather_power_grid1024_traffic.md"),
+                kernel="k_sn_gather", flops_per_launch=kern["sn_gather"]["flops_per_launch"],
+                avg_launch_us=kern["sn_gather"]["avg_us"],
+                note="FP64 vector ALU (multiply and subtract rounded separately, as the reference "
+                     "computes them); peak = MI355X FP64 78.6 TFLOP/s")
+    res = dict(value=value, elapsed=elapsed, calls=calls, steps_per_s=args.steps / elapsed,
+               status=int(st), kernels=kern, roofline=roof, kcl=kcl,
+               dc=dict(status=int(st_dc), newton_iters=int(it_dc), seconds=dc_s,
+                       solver="SparseLU (supernodal GPU factor)", supernodes=dc_info["supernodes"]),
+               warmup=dict(steps=args.warmup, seconds=warm_s, newton_iters=int(it_w)),
+               n=int(n), nnz_unmerged=int(len(ai)), nnz_l=nnz_l, nnz_u=nnz_u,
+               supernodal=info, bbd=bbd, devices=int(c.ndev))
+    return res, (g, gc, ap, ai, ax, rhs, n)
+
+
+def cpu_baseline_power_grid(state, budget_s, np):
+    """The oracle's column-parallel factor (native_klu.rs:402-576, identity
+    pivots = SparseLU's arithmetic) and solve on the grid's TRAN matrix and
+    the GPU factor's own pattern. 1 thread: a systematic sample of every m-th
+    column, each recomputed from the GPU's L values of the columns before it
+    (the per-column work is independent given those), scaled by m. All host
+    threads (at most 16): the whole factor. Both checked value for value
+    against the GPU factor of the same matrix."""
+    from oracle import oracle as O
+
+    g, gc, ap, ai, ax, rhs, n = state
+    model, ncpu = _cpu_info()
+    nthreads = max(1, min(16, ncpu))
+    t0 = time.perf_counter()
+    g.factor(ap, ai, ax)  # the GPU factor of exactly this matrix
+    pat = g.factor_pattern()
+    lx_g, ux_g, ud_g = g.factor_values()
+    _log(f"CPU baseline: GPU factor and pattern fetched ({time.perf_counter() - t0:.1f} s)")
+    cp = O.ColPar(pat)
+    cp.lx[: len(lx_g)] = lx_g
+    cp.ux[: len(ux_g)] = ux_g
+    cp.ud[:n] = ud_g
+    # calibrate the stride on a small sample, then time one that fills ~budget/2
+    m = 4096
+    cols = np.arange(0, n, m, dtype=np.int64)
+    t_cal = cp.columns(ap, ai, ax, cols)
+    est_full = t_cal * m
+    m = int(max(1, min(4096, est_full / max(budget_s * 0.5, 1e-3))))
+    cols = np.arange(m // 2, n, m, dtype=np.int64)
+    t_s = cp.columns(ap, ai, ax, cols)
+    t1_factor = t_s * m
+    lx, ux, ud = cp.values()
+    sample_ok = bool(np.array_equal(lx.view(np.int64), lx_g.view(np.int64)) and
+                     np.array_equal(ux.view(np.int64), ux_g.view(np.int64)) and
+                     np.array_equal(ud.view(np.int64), ud_g.view(np.int64)))
+    _log(f"CPU baseline 1 thread: {len(cols)} sampled columns (every {m}-th) in {t_s:.1f} s "
+         f"-> {t1_factor:.1f} s per factor")
+    t0 = time.perf_counter()
+    xs = cp.solve(rhs)
+    t_solve = time.perf_counter() - t0
+    xg = rhs.copy()
+    g.solve(xg)
+    res = dict(unit="refactor+solve/s", cores=1, kind="port",
+               value=1.0 / (t1_factor + t_solve),
+               sample=(f"oracle column-parallel factor restatement, 1 thread: {len(cols)} of {n} columns "
+                       f"(every {m}-th), each from the GPU's L values of earlier columns, {t_s:.1f} s, "
+                       f"scaled x{m} to {t1_factor:.1f} s per factor; full solve {t_solve:.2f} s"),
+               factor_s_est=t1_factor, solve_s=t_solve, sample_values_identical=sample_ok,
+               solution_identical_to_gpu=bool(np.array_equal(xs.view(np.int64), xg.view(np.int64))),
+               cpu_model=model, nproc=ncpu)
+    par = None
+    if t1_factor / nthreads < 3.0 * budget_s:
+        cp2 = O.ColPar(pat)
+        t0 = time.perf_counter()
+        cp2.factor(ap, ai, ax, nthreads=nthreads)
+        t_par = time.perf_counter() - t0
+        lx2, ux2, ud2 = cp2.values()
+        same = bool(np.array_equal(lx2.view(np.int64), lx_g.view(np.int64)) and
+                    np.array_equal(ux2.view(np.int64), ux_g.view(np.int64)) and
+                    np.array_equal(ud2.view(np.int64), ud_g.view(np.int64)))
+        x2 = cp2.solve(rhs)
+        par = dict(value=1.0 / (t_par + t_solve), unit="refactor+solve/s", cores=nthreads,
+                   kind="port", factor_s=t_par, solve_s=t_solve, values_identical=same,
+                   solution_identical=bool(np.array_equal(x2.view(np.int64), xs.view(np.int64))),
+                   sample=f"whole factor on {nthreads} threads ({t_par:.1f} s) + 1-thread solve")
+        _log(f"CPU baseline {nthreads} threads: whole factor {t_par:.1f} s, identical={same}")
+    else:
+        par = dict(skipped=f"estimated {t1_factor / nthreads:.0f} s on {nthreads} threads")
+    res["column_parallel"] = par
+    return res
 
 
 def _measured_traffic(kernel):
@@ -367,13 +570,89 @@ def cpu_baseline_mesh(mesh, np, reps=10):
                 bit_identical=bool(np.array_equal(xg.view(np.int64), bb.view(np.int64))))
 
 
+def chain_tran(args, E, NL, S, torch, np):
+    """The 500-stage level-1 inverter chain, .tran 5p 10n max step 5p, through
+    the NativeKLU-semantics GPU backend. Under the reference's semantics the
+    TRAN's initial DC operating point ends MaxIters after 100 Newton
+    iterations (the level-1 PMOS frame of bsim/evaluate.rs:298-321 gives the
+    chain no DC solution), so a run is 100 stamp+refactor+solve iterations
+    on the n=504 DC matrix, device-resident in one k_batch_dcop launch."""
+    from rustspice_amd import _lib
+
+    circ = NL.inverter_chain(500)
+    tran_args = dict(NL.C2_TRAN)
+    g = E.GpuCircuit(circ)
+    s = S.GpuSolver(circ.node_count, S.NATIVEKLU)
+    for _ in range(2):
+        g.tran(s, store=False, **tran_args)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    t0 = time.perf_counter()
+    res = [g.tran(s, store=False, **tran_args) for _ in range(args.chain_runs)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = _lib.profile_get()
+    _lib.profile_enable(False)
+    calls = sum(r["factor_calls"] for r in res)
+    ap_dc, ai_dc = g.pattern(E.MODE_DC)
+    n_dc = len(ap_dc) - 1
+    _, _, ax0, _, _ = g.stamp(np.zeros(circ.node_count), mode=E.MODE_DC)
+    ps = S.HostFactor(n_dc, ap_dc, ai_dc, ax0).plan_stats()
+    b_ref, b_sol = _algorithmic_bytes(len(ai_dc), ps["nnz_l"], ps["nnz_u"], n_dc)
+    kern = {}
+    b_batch = (b_ref + b_sol) * res[-1]["dc_iters"]  # dc_iters refactor+solve per launch
+    for name, b in (("refactor", b_ref), ("solve", b_sol), ("batch", b_batch)):
+        p = prof[name]
+        if p["count"]:
+            kern[name] = dict(avg_us=p["avg_us"], count=p["count"], bytes=b,
+                              gbs=b / (p["avg_us"] * 1e-6) / 1e9, total_ms=p["ms_total"])
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    kname = {"batch": "k_batch_dcop"}.get(dom, f"k_{dom}")
+    gh = E.GpuCircuit(circ)
+    gh.set_device_newton(False)
+    sh = S.GpuSolver(circ.node_count, S.NATIVEKLU)
+    gh.tran(sh, store=False, **tran_args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hres = [gh.tran(sh, store=False, **tran_args) for _ in range(args.host_steps)]
+    torch.cuda.synchronize()
+    h_el = time.perf_counter() - t0
+    out = dict(
+        workload="500-stage level-1 CMOS inverter chain, .tran 5p 10n max step 5p; reference "
+                 "semantics: the initial DC OP ends MaxIters at 100 Newton iterations",
+        solver="NativeKLU semantics, device-resident Newton (one launch per DC OP)",
+        refactor_solve_per_s=calls / elapsed, ms_per_run=1e3 * elapsed / args.chain_runs,
+        runs=args.chain_runs, n=n_dc, nnz_unmerged=len(ai_dc), nnz_l=ps["nnz_l"], nnz_u=ps["nnz_u"],
+        tran_status=res[-1]["status"], dc_iters_per_run=res[-1]["dc_iters"], kernels=kern,
+        roofline=dict(bound="hbm", achieved=kern[dom]["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=kern[dom]["gbs"] / HBM_PEAK_GBS, traffic=_measured_traffic(kname),
+                      kernel=kname, bytes_per_launch=kern[dom]["bytes"], avg_launch_us=kern[dom]["avg_us"]),
+        host_driven=dict(refactor_solve_per_s=sum(r["factor_calls"] for r in hres) / h_el,
+                         ms_per_run=1e3 * h_el / args.host_steps, runs=args.host_steps,
+                         same_result=bool(hres[-1]["status"] == res[-1]["status"] and
+                                          hres[-1]["dc_iters"] == res[-1]["dc_iters"] and
+                                          hres[-1]["factor_calls"] == res[-1]["factor_calls"])))
+    if not args.no_cpu:
+        cb, par = cpu_baseline(circ, tran_args, min(args.cpu_seconds, 10.0), res[-1])
+        xg, rg = g.dc_op(s)
+        par["gpu_dc_final_norm"] = rg.final_norm
+        par["bit_identical_trajectory"] = (_same(rg.final_norm, par["oracle_dc_final_norm"]) and
+                                           rg.iterations == par["oracle_dc_iters"])
+        out["cpu_baseline"] = cb
+        out["parity"] = par
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--steps", type=int, default=40, help="timed trapezoidal steps of the power grid")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps (the first analyzes the pattern)")
+    ap.add_argument("--grid", type=int, default=1024, help="power grid side (1024: the full workload)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the secondary sections")
+    ap.add_argument("--chain-runs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096, help="Monte-Carlo instances (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--loop-sample", type=int, default=16)
@@ -402,7 +681,6 @@ def main():
     import numpy as np
     import torch
 
-    from rustspice_amd import _lib
     from rustspice_amd import engine as E
     from rustspice_amd import netlists as NL
     from rustspice_amd import solver as S
@@ -412,103 +690,41 @@ def main():
     if args.kernels_only:
         print(json.dumps({"kernel_rooflines": kernel_rooflines(E, NL, S, torch, np)}))
         return
-    circ = NL.inverter_chain(500)
-    tran_args = dict(NL.C2_TRAN)
-    g = E.GpuCircuit(circ)
-    s = S.GpuSolver(circ.node_count, S.NATIVEKLU)
-
-    for _ in range(args.warmup):
-        g.tran(s, store=False, **tran_args)
-    torch.cuda.synchronize()
-    _lib.profile_enable(True)
-    t0 = time.perf_counter()
-    res = []
-    for _ in range(args.steps):
-        res.append(g.tran(s, store=False, **tran_args))
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = _lib.profile_get()
-    _lib.profile_enable(False)
-    elapsed = t1 - t0
-    calls = sum(r["factor_calls"] for r in res)
-    value = calls / elapsed
-    st = s.stats()
-    ap_dc, ai_dc = g.pattern(E.MODE_DC)
-    n_dc = len(ap_dc) - 1
-    # L/U sizes of the DC matrix from the host analysis (the device-resident
-    # DC OP leaves no factorization on the solver handle)
-    _, _, ax0, _, _ = g.stamp(np.zeros(circ.node_count), mode=E.MODE_DC)
-    ps = S.HostFactor(n_dc, ap_dc, ai_dc, ax0).plan_stats()
-    st.update(nnz_l=ps["nnz_l"], nnz_u=ps["nnz_u"], n_factor_levels=ps["factor_levels"],
-              n_lsolve_levels=ps["lsolve_levels"], n_usolve_levels=ps["usolve_levels"])
-    b_ref, b_sol = _algorithmic_bytes(len(ai_dc), st["nnz_l"], st["nnz_u"], n_dc)
-    kern = {}
-    # k_batch_dcop runs dc_iters refactor+solve iterations per launch
-    b_batch = (b_ref + b_sol) * res[-1]["dc_iters"]
-    for name, b in (("refactor", b_ref), ("solve", b_sol), ("batch", b_batch)):
-        p = prof[name]
-        if p["count"]:
-            avg_s = p["avg_us"] * 1e-6
-            kern[name] = dict(avg_us=p["avg_us"], count=p["count"], bytes=b,
-                              gbs=b / avg_s / 1e9, total_ms=p["ms_total"])
-    for name in ("eval", "assemble"):
-        p = prof[name]
-        kern[name] = dict(avg_us=p["avg_us"], count=p["count"], total_ms=p["ms_total"])
-    dom = max(("refactor", "solve", "batch"), key=lambda k: kern.get(k, {}).get("total_ms", 0.0))
-    kname = {"batch": "k_batch_dcop"}.get(dom, f"k_{dom}")
-
-    # the same TRAN with every Newton iteration driven from the host
-    gh = E.GpuCircuit(circ)
-    gh.set_device_newton(False)
-    sh = S.GpuSolver(circ.node_count, S.NATIVEKLU)
-    gh.tran(sh, store=False, **tran_args)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    hres = [gh.tran(sh, store=False, **tran_args) for _ in range(args.host_steps)]
-    torch.cuda.synchronize()
-    h_el = time.perf_counter() - t0
-    host_driven = dict(refactor_solve_per_s=sum(r["factor_calls"] for r in hres) / h_el,
-                       ms_per_step=1e3 * h_el / args.host_steps, steps=args.host_steps,
-                       same_result=bool(hres[-1]["status"] == res[-1]["status"] and
-                                        hres[-1]["dc_iters"] == res[-1]["dc_iters"] and
-                                        hres[-1]["factor_calls"] == res[-1]["factor_calls"]))
-    roof = dict(bound="hbm", achieved=kern[dom]["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=kern[dom]["gbs"] / HBM_PEAK_GBS, traffic=_measured_traffic(kname), kernel=kname,
-                bytes_per_launch=kern[dom]["bytes"], avg_launch_us=kern[dom]["avg_us"])
-
+    hl, state = power_grid_tran(args, E, NL, S, torch, np)
     line = dict(
-        metric=METRIC, value=value, unit="refactor+solve/s", n_gpus=1, steps=args.steps,
-        warmup=args.warmup, ms_per_step=1e3 * elapsed / args.steps, higher_is_better=True,
+        metric=METRIC, value=hl["value"], unit="refactor+solve/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=1e3 * hl["elapsed"] / args.steps, higher_is_better=True,
         scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
-        config=dict(workload="500-stage level-1 CMOS inverter chain, .tran 5p 10n max step 5p "
-                             "(configs[1] circuit); reference semantics: initial DC OP ends "
-                             "MaxIters at 100 Newton iterations",
-                    solver="NativeKLU semantics, device-resident Newton (one launch per DC OP)",
-                    n=n_dc,
-                    nnz_unmerged=len(ai_dc), nnz_l=st["nnz_l"], nnz_u=st["nnz_u"],
-                    factor_levels=st["n_factor_levels"], lsolve_levels=st["n_lsolve_levels"],
-                    usolve_levels=st["n_usolve_levels"]),
-        tran_status=res[-1]["status"], dc_iters_per_step=res[-1]["dc_iters"],
-        refactor_solve_calls=calls, kernels=kern, roofline=roof, host_driven=host_driven,
-    )
-    if args.batch > 0:
-        line["batched"] = batched_dc_ops(args, E, NL, S, torch)
-    if not args.no_adjoint:
-        line["adjoint"] = adjoint_cost(E, NL, S, torch)
-        line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
-        line["tran_graph"] = tran_graph_cost(E, NL, S)
-        line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
-        line["kernel_rooflines"] = kernel_rooflines(E, NL, S, torch, np)
+        config=dict(workload=f"RC power-grid transient, {args.grid}x{args.grid} grid (n={hl['n']}), "
+                             "segment R U[0.05,0.5] ohm, node C U[1,10] fF, 1.0 V pads every 64 nodes, "
+                             f"{args.grid * args.grid // 256} pulsed 10 mA loads; trapezoidal steps of 1 ps "
+                             "through BbdSolver(64) (whole-matrix SparseLU fallback); DC OP via SparseLU",
+                    solver="BbdSolver(64) -> SparseLU semantics, supernodal GPU factor and solve",
+                    n=hl["n"], nnz_unmerged=hl["nnz_unmerged"], nnz_l=hl["nnz_l"], nnz_u=hl["nnz_u"],
+                    devices=hl["devices"], supernodes=hl["supernodal"]["supernodes"],
+                    levels=hl["supernodal"]["levels"]),
+        timesteps_per_s=hl["steps_per_s"], newton_iters=hl["calls"], tran_status=hl["status"],
+        kernels=hl["kernels"], roofline=hl["roofline"], kcl=hl["kcl"], dc_op=hl["dc"],
+        warmup_detail=hl["warmup"], supernodal=hl["supernodal"], bbd=hl["bbd"])
     if not args.no_cpu:
-        cb, par = cpu_baseline(circ, tran_args, args.cpu_seconds, res[-1])
-        xg, rg = g.dc_op(s)
-        par["gpu_dc_final_norm"] = rg.final_norm
-        par["bit_identical_trajectory"] = (_same(rg.final_norm, par["oracle_dc_final_norm"]) and
-                                           rg.iterations == par["oracle_dc_iters"])
+        cb = cpu_baseline_power_grid(state, args.cpu_seconds, np)
         line["cpu_baseline"] = cb
-        line["parity"] = par
+    del state
+    if not args.headline_only:
+        _log("secondary: 500-stage inverter chain")
+        line["chain500"] = chain_tran(args, E, NL, S, torch, np)
+        if args.batch > 0:
+            _log("secondary: batched Monte-Carlo DC-OPs")
+            line["batched"] = batched_dc_ops(args, E, NL, S, torch)
         if not args.no_adjoint:
-            line["mesh_refactor_solve"]["cpu_baseline"] = cpu_baseline_mesh(mesh, np)
+            _log("secondary: adjoints, graphs, mesh refactor+solve")
+            line["adjoint"] = adjoint_cost(E, NL, S, torch)
+            line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
+            line["tran_graph"] = tran_graph_cost(E, NL, S)
+            line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
+            line["kernel_rooflines"] = kernel_rooflines(E, NL, S, torch, np)
+            if not args.no_cpu:
+                line["mesh_refactor_solve"]["cpu_baseline"] = cpu_baseline_mesh(mesh, np)
     print(json.dumps(line))
 
 

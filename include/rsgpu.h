@@ -140,9 +140,21 @@ int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);
  * are device memory; `stream` is a hipStream_t (NULL = default stream). ---- */
 int32_t rs_refactor_device(rs_solver *h, const double *ax_dev, void *stream);
 int32_t rs_solve_device(rs_solver *h, const double *rhs_dev, double *x_dev, void *stream);
-/* Copies the current factor values to host arrays sized by rs_get_stats
- * (lx[nnz_l], ux[nnz_u], udiag[n]); position-space order. */
+/* Copies the current factor values to host arrays sized by rs_get_stats or
+ * rs_get_factor_pattern (lx[nnz_l], ux[nnz_u], udiag[n]); position-space
+ * order. A NULL array is skipped. */
 int32_t rs_get_factor_values(rs_solver *h, double *lx, double *ux, double *udiag);
+/* The pattern behind rs_get_factor_values for a one-block factor (the
+ * reference's SymbolicFactor, native_klu.rs:92-107, or SparseLuSolver's
+ * perm and L/U pointers, sparse_lu.rs:146-159): sizes[3] = {n, nnz(L),
+ * nnz(U)}; col_perm[original column] = position; L
+ * strictly lower with rows ascending, U with rows ascending and the diagonal
+ * last, both column pointers n + 1 long. Array pointers may be NULL (sizes
+ * only). A BbdSolver handle answers for its whole-matrix fallback; several
+ * BTF or BBD blocks give RS_ERR_FACTOR_FAILED (with several BTF blocks,
+ * sizes still holds the array lengths rs_get_factor_values fills). */
+int32_t rs_get_factor_pattern(rs_solver *h, int64_t *sizes, int64_t *col_perm, int64_t *lcp,
+                              int64_t *lri, int64_t *ucp, int64_t *uri);
 
 /* ---- circuit-level boundary: DeviceStamp (stamp.rs:94-121) for a whole
  * instance list, and the Newton drivers of engine.rs / newton.rs ---------- */
@@ -346,7 +358,8 @@ int32_t rs_batch_phase_cycles(rs_batch *b, int64_t *out);
 /* ---- measurement (no reference counterpart) -------------------------------
  * Kernel timing with HIP events recorded on each kernel's launch stream.
  * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly,
- * 4 batched Newton (k_batch_dcop). ms_total and counts hold 5 entries.
+ * 4 batched Newton (k_batch_dcop), 5 each k_sn_gather launch of a
+ * supernodal factor (also inside index 0). ms_total and counts hold 6 entries.
  * Enabling (or disabling) clears the totals. */
 void rs_profile_enable(int32_t on);
 void rs_profile_get(double *ms_total, int64_t *counts);
@@ -359,9 +372,10 @@ void rs_profile_get(double *ms_total, int64_t *counts);
  * reference's terms in the reference's order (sparse_lu.rs numeric_factor),
  * grouped into dense supernode panels that the levels of the supernodal
  * elimination tree gather and factor in parallel; the solves run the same
- * levels. Takes effect at the next analysis. rs_supernodal_info: out[8] =
+ * levels. Takes effect at the next analysis. rs_supernodal_info: out[9] =
  * {active, supernodes, levels, contributions, panel doubles, device bytes,
- * multiply-subtracts per factor, widest supernode}. */
+ * multiply-subtracts per factor, widest supernode, the multiply-subtracts
+ * of descendant contributions (the k_sn_gather launches)}. */
 void rs_set_supernodal(rs_solver *h, int32_t mode);
 int32_t rs_supernodal_info(const rs_solver *h, int64_t *out);
 /* Checking aid (no reference counterpart): with on != 0, every FP64 device

@@ -144,6 +144,13 @@ def lib():
         L.orc_dense_build.argtypes = [C.c_int64, C.c_int64, i64p, i64p, f64p, f64p]
         L.orc_dense_factor.argtypes = [C.c_int64, f64p, i64p, C.c_int]
         L.orc_dense_solve.argtypes = [C.c_int64, f64p, i64p, f64p, C.c_int]
+        L.orc_colpar_factor.restype = C.c_int
+        L.orc_colpar_factor.argtypes = [C.c_int64, i64p, i64p, vp, i64p, i64p, i64p, i64p, i64p, i64p,
+                                        f64p, f64p, f64p, f64p, C.c_int]
+        L.orc_colpar_columns.restype = C.c_double
+        L.orc_colpar_columns.argtypes = [C.c_int64, i64p, i64p, vp, i64p, i64p, i64p, i64p, i64p, i64p,
+                                         f64p, f64p, f64p, f64p, i64p, C.c_int64]
+        L.orc_lu_solve.argtypes = [C.c_int64, i64p, vp, i64p, i64p, f64p, i64p, i64p, f64p, f64p]
         _LIB = L
     return _LIB
 
@@ -476,6 +483,49 @@ def dc_sweep(circ, source, start, stop, step, opts=None, solver=None):
                        C.byref(res))
     return dict(status=res.status, iterations=res.iterations, values=values,
                 nsolved=int(res.nsolved), sols=sols[: res.nsolved].copy())
+
+
+class ColPar:
+    """The reference's column-parallel factor (native_klu.rs:402-576) and the
+    matching triangular solve on a given one-block pattern: col_perm[original]
+    = position, L/U column pointers and row indices in position space (U
+    diagonal last). row_perm_inv None: identity pivots (SparseLU's factor)."""
+
+    def __init__(self, pat, row_perm_inv=None):
+        self.n = int(pat["n"])
+        self.cp = _i64(pat["col_perm"])
+        self.cpi = np.empty(self.n, np.int64)
+        self.cpi[self.cp] = np.arange(self.n, dtype=np.int64)
+        self.lcp, self.lri = _i64(pat["l_col_ptr"]), _i64(pat["l_row_idx"])
+        self.ucp, self.uri = _i64(pat["u_col_ptr"]), _i64(pat["u_row_idx"])
+        self.rpi = None if row_perm_inv is None else _i64(row_perm_inv)
+        self.lx = np.zeros(max(len(self.lri), 1))
+        self.ux = np.zeros(max(len(self.uri), 1))
+        self.ud = np.zeros(max(self.n, 1))
+
+    def _rpi(self):
+        return None if self.rpi is None else self.rpi.ctypes.data_as(C.c_void_p)
+
+    def factor(self, ap, ai, ax, nthreads=1):
+        lib().orc_colpar_factor(self.n, self.cp, self.cpi, self._rpi(), self.lcp, self.lri, self.ucp,
+                                self.uri, _i64(ap), _i64(ai), _f64(ax), self.lx, self.ux, self.ud,
+                                int(nthreads))
+
+    def columns(self, ap, ai, ax, cols):
+        """Recompute the listed columns on one thread from the L values
+        already held (a full factor first); returns the seconds taken."""
+        return lib().orc_colpar_columns(self.n, self.cp, self.cpi, self._rpi(), self.lcp, self.lri,
+                                        self.ucp, self.uri, _i64(ap), _i64(ai), _f64(ax), self.lx,
+                                        self.ux, self.ud, _i64(cols), len(cols))
+
+    def solve(self, rhs):
+        x = _f64(rhs).copy()
+        lib().orc_lu_solve(self.n, self.cp, self._rpi(), self.lcp, self.lri, self.lx, self.ucp,
+                           self.uri, self.ux, x)
+        return x
+
+    def values(self):
+        return self.lx[: len(self.lri)], self.ux[: len(self.uri)], self.ud[: self.n]
 
 
 def lte_milne(xb, xt, abs_tol=1e-6, rel_tol=1e-3):

@@ -185,6 +185,24 @@ int orc_dc_sweep(const rs_device *dev, int64_t ndev, int64_t node_count,
                  double step, void *solver, int64_t max_points, double *sols,
                  int64_t x_stride, orc_sweep_result *res);
 
+/* ---- column-parallel factor and solve (orc_colpar.c) -------------------
+ * native_klu.rs:402-576 with nthreads workers; row_perm_inv NULL means
+ * identity pivots (SparseLU). orc_colpar_columns recomputes the listed
+ * columns on one thread from the L values already in lx and returns the
+ * seconds it took. */
+int orc_colpar_factor(int64_t n, const int64_t *col_perm, const int64_t *col_perm_inv,
+                      const int64_t *row_perm_inv, const int64_t *lcp, const int64_t *lri,
+                      const int64_t *ucp, const int64_t *uri, const int64_t *ap, const int64_t *ai,
+                      const double *ax, double *lx, double *ux, double *ud, int nthreads);
+double orc_colpar_columns(int64_t n, const int64_t *col_perm, const int64_t *col_perm_inv,
+                          const int64_t *row_perm_inv, const int64_t *lcp, const int64_t *lri,
+                          const int64_t *ucp, const int64_t *uri, const int64_t *ap,
+                          const int64_t *ai, const double *ax, double *lx, double *ux, double *ud,
+                          const int64_t *cols, int64_t ncols);
+void orc_lu_solve(int64_t n, const int64_t *col_perm, const int64_t *row_perm_inv,
+                  const int64_t *lcp, const int64_t *lri, const double *lx, const int64_t *ucp,
+                  const int64_t *uri, const double *ux, double *rhs);
+
 /* analysis.rs:147-200: out = {error_norm, max_node, accept, factor, raw_lte} */
 void orc_lte_milne(const double *xb, const double *xt, int64_t n, double abs_tol,
                    double rel_tol, double out[5]);

@@ -17,6 +17,17 @@ _LIB = None
 
 i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def _nullable(base):
+    """An ndpointer argument type that also passes None as NULL."""
+    def from_param(cls, obj):
+        return None if obj is None else base.from_param(obj)
+    return type(base.__name__ + "_or_null", (base,), {"from_param": classmethod(from_param)})
+
+
+i64p_n = _nullable(i64p)
+f64p_n = _nullable(f64p)
 vp = C.c_void_p
 
 #: symbols declared in include/rsgpu.h and include/rsgpu_host.h
@@ -25,7 +36,7 @@ EXPORTS = [
     "rs_solve", "rs_reset_pattern", "rs_solver_name", "rs_set_parallel_threads",
     "rs_set_config", "rs_get_stats", "rs_set_autotune", "rs_set_supernodal", "rs_supernodal_info", "rs_set_growth_check", "rs_growth_fallbacks",
     "rs_last_growth", "rs_set_launch_shape", "rs_launch_shape", "rs_last_error", "rs_refactor_device",
-    "rs_solve_device", "rs_get_factor_values",
+    "rs_solve_device", "rs_get_factor_values", "rs_get_factor_pattern",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest",
@@ -102,7 +113,9 @@ def lib():
     L.rs_solve_device.restype = C.c_int32
     L.rs_solve_device.argtypes = [vp, vp, vp, vp]
     L.rs_get_factor_values.restype = C.c_int32
-    L.rs_get_factor_values.argtypes = [vp, f64p, f64p, f64p]
+    L.rs_get_factor_values.argtypes = [vp, f64p_n, f64p_n, f64p_n]
+    L.rs_get_factor_pattern.restype = C.c_int32
+    L.rs_get_factor_pattern.argtypes = [vp, i64p, i64p_n, i64p_n, i64p_n, i64p_n, i64p_n]
     L.rs_host_amd_order.argtypes = [C.c_int64, i64p, i64p, i64p, i64p]
     L.rs_host_btf.restype = C.c_int64
     L.rs_host_btf.argtypes = [C.c_int64, i64p, i64p, i64p, i64p, i64p, C.POINTER(C.c_int64)]
@@ -140,7 +153,7 @@ def lib():
     return L
 
 
-PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble", "batch")
+PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble", "batch", "sn_gather")
 
 
 def profile_enable(on: bool = True):

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "profile.hpp"
 #include "sn.hpp"
 #include "solver_impl.hpp"
 
@@ -214,7 +215,11 @@ hipError_t sn_factor(SnImpl* p, const double* ax, hipStream_t s) {
         (e = sn_launch_scatter_long(p->sl_src.p, p->sl_cnt.p, p->sl_slot.p, (int)p->n_long, ax, p->v.p, s)))
         return e;
     for (int32_t l = 0; l < h.nlev; l++) {
-        if ((e = sn_launch_gather(p->ptr, h.glev_ptr[l], h.glev_ptr[l + 1] - h.glev_ptr[l], s))) return e;
+        const int ng = h.glev_ptr[l + 1] - h.glev_ptr[l];
+        const int ps = ng > 0 ? prof_begin(PK_SN_GATHER, s) : -1;
+        e = sn_launch_gather(p->ptr, h.glev_ptr[l], ng, s);
+        prof_end(ps, s);
+        if (e) return e;
         for (int32_t he : p->hot_at[l])
             if ((e = sn_launch_hot(p->ptr, p->hot_ptr.p, p->hot_terms.p, he, s))) return e;
         if ((e = sn_launch_factor(p->ptr, p->lists.p + p->fs[l], p->fs[l + 1] - p->fs[l], s))) return e;
@@ -259,6 +264,7 @@ hipError_t sn_values(SnImpl* p, double* lx, double* ux, double* ud) {
 
 const double* sn_pivots(const SnImpl* p) { return p->ud.p; }
 const SnHost& sn_plan(const SnImpl* p) { return p->h; }
+const BlockSymbolic& sn_symbolic(const SnImpl* p) { return p->sym; }
 int64_t sn_device_bytes(const SnImpl* p) { return p->bytes; }
 
 }  // namespace rsgpu

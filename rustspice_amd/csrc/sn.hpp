@@ -92,6 +92,7 @@ struct SnHost {
     // statistics
     int64_t npairs = 0, max_w = 0, max_h = 0;
     double updates = 0.0;            // multiply-subtracts of one factor (pattern count)
+    double gather_updates = 0.0;     // the part applied by descendant contributions
 };
 
 // Supernodes at least this wide are factored by the blocked multi-workgroup
@@ -148,6 +149,7 @@ hipError_t sn_values(SnImpl* p, double* lx, double* ux, double* ud);
 // pivots (device, n entries)
 const double* sn_pivots(const SnImpl* p);
 const SnHost& sn_plan(const SnImpl* p);
+const BlockSymbolic& sn_symbolic(const SnImpl* p);
 int64_t sn_device_bytes(const SnImpl* p);
 
 }  // namespace rsgpu

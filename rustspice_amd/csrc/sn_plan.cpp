@@ -379,11 +379,22 @@ bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost&
             }
     }
 
-    // 8. algorithmic work: every U(j, k), j < k, applies L(:, j)
-    double upd = 0.0;
-    for (i64 k = 0; k < n; k++)
-        for (i64 q = ucp[k]; q < ucp[k + 1] - 1; q++) upd += (double)(lcp[uri[q] + 1] - lcp[uri[q]]);
+    // 8. algorithmic work: every U(j, k), j < k, applies L(:, j); those with
+    //    j and k in different supernodes are the gather's
+    double upd = 0.0, gupd = 0.0;
+    {
+        std::vector<int32_t> sn_of(n);
+        for (int32_t S = 0; S < ns; S++)
+            for (i64 j = h.j0[S]; j < h.j0[S + 1]; j++) sn_of[j] = S;
+        for (i64 k = 0; k < n; k++)
+            for (i64 q = ucp[k]; q < ucp[k + 1] - 1; q++) {
+                const double c = (double)(lcp[uri[q] + 1] - lcp[uri[q]]);
+                upd += c;
+                if (sn_of[uri[q]] != sn_of[k]) gupd += c;
+            }
+    }
     h.updates = upd;
+    h.gather_updates = gupd;
     return true;
 }
 

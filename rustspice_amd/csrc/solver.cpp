@@ -944,7 +944,7 @@ void rs_set_supernodal(rs_solver* h, int32_t mode) {
 }
 
 int32_t rs_supernodal_info(const rs_solver* h, int64_t* out) {
-    std::memset(out, 0, 8 * sizeof(int64_t));
+    std::memset(out, 0, 9 * sizeof(int64_t));
     const rs_solver* s = h->bbd ? bbd_fallback_solver(h) : h;
     if (!s || !s->sn) return RS_OK;
     const SnHost& q = sn_plan(s->sn);
@@ -956,6 +956,7 @@ int32_t rs_supernodal_info(const rs_solver* h, int64_t* out) {
     out[5] = sn_device_bytes(s->sn);
     out[6] = (int64_t)q.updates;
     out[7] = q.max_w;
+    out[8] = (int64_t)q.gather_updates;
     return RS_OK;
 }
 
@@ -1072,7 +1073,39 @@ int32_t rs_solve_device(rs_solver* h, const double* rhs_dev, double* x_dev, void
     return solver_solve_dev(h, rhs_dev, x_dev, (hipStream_t)stream);
 }
 
+int32_t rs_get_factor_pattern(rs_solver* h, int64_t* sizes, int64_t* col_perm, int64_t* lcp, int64_t* lri,
+                              int64_t* ucp, int64_t* uri) {
+    if (h->bbd) {
+        rs_solver* f = const_cast<rs_solver*>(bbd_fallback_solver(h));
+        if (!f) return fail(h, RS_ERR_FACTOR_FAILED, "factor pattern: the BBD decomposition has several blocks");
+        return rs_get_factor_pattern(f, sizes, col_perm, lcp, lri, ucp, uri);
+    }
+    const BlockSymbolic* b = nullptr;
+    if (h->sn) b = &sn_symbolic(h->sn);
+    else if (h->state != ST_EMPTY && h->hf.sym.size() == 1) b = &h->hf.sym[0];
+    if (!b) {  // the sizes rs_get_factor_values fills all the same
+        sizes[0] = h->n;
+        sizes[1] = h->hf.nnz_l;
+        sizes[2] = h->hf.nnz_u;
+        return fail(h, RS_ERR_FACTOR_FAILED, "factor pattern: no single-block factor");
+    }
+    sizes[0] = b->n;
+    sizes[1] = (int64_t)b->lri.size();
+    sizes[2] = (int64_t)b->uri.size();
+    if (col_perm) std::copy(b->col_perm.begin(), b->col_perm.end(), col_perm);
+    if (lcp) std::copy(b->lcp.begin(), b->lcp.end(), lcp);
+    if (lri) std::copy(b->lri.begin(), b->lri.end(), lri);
+    if (ucp) std::copy(b->ucp.begin(), b->ucp.end(), ucp);
+    if (uri) std::copy(b->uri.begin(), b->uri.end(), uri);
+    return RS_OK;
+}
+
 int32_t rs_get_factor_values(rs_solver* h, double* lx, double* ux, double* udiag) {
+    if (h->bbd) {
+        rs_solver* f = const_cast<rs_solver*>(bbd_fallback_solver(h));
+        if (!f) return fail(h, RS_ERR_FACTOR_FAILED, "factor values: the BBD decomposition has several blocks");
+        return rs_get_factor_values(f, lx, ux, udiag);
+    }
     if (h->sn && h->state == ST_FACTORED) {
         hipError_t e = sn_values(h->sn, lx, ux, udiag);
         return e == hipSuccess ? RS_OK : hip_fail(h, e, "supernodal values");
@@ -1080,11 +1113,11 @@ int32_t rs_get_factor_values(rs_solver* h, double* lx, double* ux, double* udiag
     if (h->state != ST_FACTORED || !h->plan_on_device)
         return fail(h, RS_ERR_FACTOR_FAILED, "Numerical factorization failed: not factored");
     hipError_t e;
-    if (h->ph.nl && (e = hipMemcpy(lx, h->d_vals.p + h->pd.o_lx, h->ph.nl * 8, hipMemcpyDeviceToHost)))
+    if (lx && h->ph.nl && (e = hipMemcpy(lx, h->d_vals.p + h->pd.o_lx, h->ph.nl * 8, hipMemcpyDeviceToHost)))
         return hip_fail(h, e, "hipMemcpy(lx)");
-    if ((e = hipMemcpy(ux, h->d_vals.p + h->pd.o_ux, h->ph.nu * 8, hipMemcpyDeviceToHost)))
+    if (ux && (e = hipMemcpy(ux, h->d_vals.p + h->pd.o_ux, h->ph.nu * 8, hipMemcpyDeviceToHost)))
         return hip_fail(h, e, "hipMemcpy(ux)");
-    if ((e = hipMemcpy(udiag, h->d_vals.p + h->pd.o_ud, h->n * 8, hipMemcpyDeviceToHost)))
+    if (udiag && (e = hipMemcpy(udiag, h->d_vals.p + h->pd.o_ud, h->n * 8, hipMemcpyDeviceToHost)))
         return hip_fail(h, e, "hipMemcpy(ud)");
     return RS_OK;
 }

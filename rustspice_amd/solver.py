@@ -120,10 +120,10 @@ class GpuSolver:
         self._L.rs_set_supernodal(self.h, int(mode))
 
     def supernodal_info(self) -> dict:
-        out = np.zeros(8, np.int64)
+        out = np.zeros(9, np.int64)
         self._L.rs_supernodal_info(self.h, out)
         keys = ["active", "supernodes", "levels", "contributions", "panel_doubles", "device_bytes",
-                "updates", "max_w"]
+                "updates", "max_w", "gather_updates"]
         return dict(zip(keys, map(int, out)))
 
     def set_growth_check(self, limit: float):
@@ -167,14 +167,32 @@ class GpuSolver:
                                             C.c_void_p(x_dev.data_ptr()),
                                             C.c_void_p(_stream_ptr(stream))))
 
+    def factor_pattern(self):
+        """One-block factor pattern: col_perm[original] = position, L/U
+        column pointers and row indices (position space)."""
+        sz = np.zeros(3, np.int64)
+        self._check(self._L.rs_get_factor_pattern(self.h, sz, None, None, None, None, None))
+        n, nl, nu = map(int, sz)
+        d = dict(col_perm=np.zeros(n, np.int64), l_col_ptr=np.zeros(n + 1, np.int64),
+                 l_row_idx=np.zeros(max(nl, 1), np.int64), u_col_ptr=np.zeros(n + 1, np.int64),
+                 u_row_idx=np.zeros(max(nu, 1), np.int64))
+        self._check(self._L.rs_get_factor_pattern(self.h, sz, d["col_perm"], d["l_col_ptr"], d["l_row_idx"],
+                                                  d["u_col_ptr"], d["u_row_idx"]))
+        d["l_row_idx"], d["u_row_idx"] = d["l_row_idx"][:nl], d["u_row_idx"][:nu]
+        d["n"] = n
+        return d
+
     def factor_values(self):
-        st = self.stats()
-        n = st["nnz_l"], st["nnz_u"]
+        sz = np.full(3, -1, np.int64)
+        self._L.rs_get_factor_pattern(self.h, sz, None, None, None, None, None)
+        if sz[0] < 0:  # a BbdSolver with several blocks: no factor values here
+            self._check(self._L.rs_get_factor_values(self.h, None, None, None))
+        dim, n = int(sz[0]), (int(sz[1]), int(sz[2]))
         lx = np.zeros(max(n[0], 1))
         ux = np.zeros(max(n[1], 1))
-        ud = np.zeros(max(self.n, 1))
+        ud = np.zeros(max(dim, 1))
         self._check(self._L.rs_get_factor_values(self.h, lx, ux, ud))
-        return lx[: n[0]], ux[: n[1]], ud[: self.n]
+        return lx[: n[0]], ux[: n[1]], ud[:dim]
 
 
 def _stream_ptr(stream) -> int:

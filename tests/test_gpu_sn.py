@@ -53,6 +53,10 @@ def test_factor_and_solve_bit_exact(gpu, nx, pitch):
     info = g.supernodal_info()
     st = g.stats()
     assert info["active"] == 1 and info["levels"] < info["supernodes"]
+    assert 0 < info["gather_updates"] < info["updates"]
+    pat = g.factor_pattern()
+    for k in ("col_perm", "l_col_ptr", "l_row_idx", "u_col_ptr", "u_row_idx"):
+        assert np.array_equal(pat[k], want[k]), k
     assert (st["factor_count"], st["refactor_count"]) == (1, 1)
     assert st["rcond"] == O_rcond(want)
 
@@ -116,6 +120,12 @@ def test_tran_fixed_bbd_fallback_bit_exact(gpu, nx, pitch):
     assert rco == st == 0 and ito == itg
     assert same_bits(xg, xo)
     assert g.bbd_info()["blocks"] == 1 and g.supernodal_info()["active"] == 1
+    # the fallback's factor through the BbdSolver handle: pattern and values
+    pat = g.factor_pattern()
+    assert pat["n"] == len(xg) and pat["u_col_ptr"][-1] == len(pat["u_row_idx"])
+    lx, ux, ud = g.factor_values()
+    assert len(lx) == len(pat["l_row_idx"]) and len(ux) == len(pat["u_row_idx"])
+    assert same_bits(ud, ux[pat["u_col_ptr"][1:] - 1])
 
 
 def test_large_grid_matches_host_schedule(gpu):
