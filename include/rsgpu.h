@@ -140,6 +140,17 @@ int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);
  * are device memory; `stream` is a hipStream_t (NULL = default stream). ---- */
 int32_t rs_refactor_device(rs_solver *h, const double *ax_dev, void *stream);
 int32_t rs_solve_device(rs_solver *h, const double *rhs_dev, double *x_dev, void *stream);
+/* rs_solve_device with no device allocation (no reference counterpart): for
+ * callers that cannot allocate, such as code under HIP graph capture or a
+ * serving loop that forbids hipMalloc. Same kernels and bit-identical
+ * results; the solve vector they use besides rhs and x is a static device
+ * array of rs_noalloc_max_n() = 2,097,152 doubles (16 MiB per GPU, held
+ * until the process ends) instead of the handle's own, and the launch shape
+ * is the one the handle holds (no tuning). n above that size, BbdSolver and
+ * Dense handles give RS_ERR_SOLVE_FAILED. Every call on one GPU shares that
+ * array: calls that overlap on one GPU (any handles, any streams) race. */
+int32_t rs_solve_device_noalloc(rs_solver *h, const double *rhs_dev, double *x_dev, void *stream);
+int64_t rs_noalloc_max_n(void);
 /* Copies the current factor values to host arrays sized by rs_get_stats or
  * rs_get_factor_pattern (lx[nnz_l], ux[nnz_u], udiag[n]); position-space
  * order. A NULL array is skipped. */

@@ -37,6 +37,7 @@ EXPORTS = [
     "rs_set_config", "rs_get_stats", "rs_set_autotune", "rs_set_supernodal", "rs_supernodal_info", "rs_set_growth_check", "rs_growth_fallbacks",
     "rs_last_growth", "rs_set_launch_shape", "rs_launch_shape", "rs_last_error", "rs_refactor_device",
     "rs_solve_device", "rs_get_factor_values", "rs_get_factor_pattern",
+    "rs_solve_device_noalloc", "rs_noalloc_max_n",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest",
@@ -114,6 +115,10 @@ def lib():
     L.rs_solve_device.argtypes = [vp, vp, vp, vp]
     L.rs_get_factor_values.restype = C.c_int32
     L.rs_get_factor_values.argtypes = [vp, f64p_n, f64p_n, f64p_n]
+    L.rs_solve_device_noalloc.restype = C.c_int32
+    L.rs_solve_device_noalloc.argtypes = [vp, vp, vp, vp]
+    L.rs_noalloc_max_n.restype = C.c_int64
+    L.rs_noalloc_max_n.argtypes = []
     L.rs_get_factor_pattern.restype = C.c_int32
     L.rs_get_factor_pattern.argtypes = [vp, i64p, i64p_n, i64p_n, i64p_n, i64p_n, i64p_n]
     L.rs_host_amd_order.argtypes = [C.c_int64, i64p, i64p, i64p, i64p]

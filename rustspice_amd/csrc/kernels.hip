@@ -410,14 +410,15 @@ __global__ __launch_bounds__(1024) void k_solve(const PlanDev* __restrict__ desc
                                                const double* __restrict__ rhspool,
                                                double* __restrict__ xpool,
                                                double* __restrict__ wpool,
-                                               const int32_t* __restrict__ skip) {
+                                               const int32_t* __restrict__ skip,
+                                               double* __restrict__ tvec) {
     extern __shared__ double smem[];
     if (skip && *skip) return;
     const PlanDev d = descs[blockIdx.x];
     const int n = d.n;
     const int nw = blockDim.x / kWave;
     double* S = smem + (threadIdx.x / kWave) * kWave;
-    double* T = kLds ? smem + nw * kWave : wpool + d.o_tvec;
+    double* T = kLds ? smem + nw * kWave : (tvec ? tvec : wpool + d.o_tvec);
     const int32_t* perm_in = ip + d.o_perm_in;
     const int32_t* perm_out = ip + d.o_perm_out;
     const double* ax = axpool + d.o_ax;
@@ -643,15 +644,16 @@ hipError_t launch_solve_lp(const PlanDev* descs, const int32_t* ip, const double
 
 hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
-                        bool lds, int threads, hipStream_t stream, const int32_t* skip) {
+                        bool lds, int threads, hipStream_t stream, const int32_t* skip, double* tvec) {
+    if (tvec && nmat != 1) return hipErrorInvalidValue;
     size_t shm = (size_t)(threads / kWave) * kWave * sizeof(double) +
                  (lds ? (size_t)max_n * sizeof(double) : 0);
     if (lds)
         hipLaunchKernelGGL(k_solve<true>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax, vals,
-                           rhs, x, wpool, skip);
+                           rhs, x, wpool, skip, tvec);
     else
         hipLaunchKernelGGL(k_solve<false>, dim3(nmat), dim3(threads), shm, stream, descs, ip, ax, vals,
-                           rhs, x, wpool, skip);
+                           rhs, x, wpool, skip, tvec);
     return hipGetLastError();
 }
 

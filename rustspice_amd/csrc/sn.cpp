@@ -231,21 +231,22 @@ hipError_t sn_factor(SnImpl* p, const double* ax, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t sn_solve(SnImpl* p, const double* rhs, double* x, hipStream_t s) {
+hipError_t sn_solve(SnImpl* p, const double* rhs, double* x, hipStream_t s, double* y) {
     const SnHost& h = p->h;
+    if (!y) y = p->y.p;
     hipError_t e;
-    if ((e = sn_launch_perm(p->perm.p, rhs, p->y.p, h.n, true, s))) return e;
+    if ((e = sn_launch_perm(p->perm.p, rhs, y, h.n, true, s))) return e;
     for (int32_t l = 0; l < h.nlev; l++)
         if ((e = sn_launch_fwd_long(p->ptr, p->long_rows.p + p->fl[l], p->fl[l + 1] - p->fl[l],
-                                    p->long_tptr.p + p->fl[l], p->lslot.p, p->yidx.p, p->y.p, s)) ||
+                                    p->long_tptr.p + p->fl[l], p->lslot.p, p->yidx.p, y, s)) ||
             (e = sn_launch_fwd(p->ptr, p->fwd_rows.p + p->fr[l], p->fr[l + 1] - p->fr[l], p->lists.p + p->in[l],
-                               p->in[l + 1] - p->in[l], p->y.p, s)))
+                               p->in[l + 1] - p->in[l], y, s)))
             return e;
     for (int32_t l = h.nlev - 1; l >= 0; l--)
         if ((e = sn_launch_bwd(p->ptr, p->lev_rows.p + p->lr[l], p->sn_of.p, p->lr[l + 1] - p->lr[l],
-                               p->lists.p + p->in[l], p->in[l + 1] - p->in[l], p->y.p, s)))
+                               p->lists.p + p->in[l], p->in[l + 1] - p->in[l], y, s)))
             return e;
-    return sn_launch_perm(p->perm.p, p->y.p, x, h.n, false, s);
+    return sn_launch_perm(p->perm.p, y, x, h.n, false, s);
 }
 
 hipError_t sn_values(SnImpl* p, double* lx, double* ux, double* ud) {

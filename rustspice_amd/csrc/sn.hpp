@@ -142,8 +142,10 @@ SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::str
 void sn_destroy(SnImpl* p);
 // factor of the unmerged CSC values ax_dev (device memory, nnz entries)
 hipError_t sn_factor(SnImpl* p, const double* ax_dev, hipStream_t s);
-// x = A^-1 rhs (device vectors of n entries; rhs and x may alias)
-hipError_t sn_solve(SnImpl* p, const double* rhs_dev, double* x_dev, hipStream_t s);
+// x = A^-1 rhs (device vectors of n entries; rhs and x may alias). y: a
+// device vector of n entries for the permuted solution in place of the
+// handle's own (nullptr: the handle's).
+hipError_t sn_solve(SnImpl* p, const double* rhs_dev, double* x_dev, hipStream_t s, double* y = nullptr);
 // factor values in the symbolic's CSC order (host arrays)
 hipError_t sn_values(SnImpl* p, double* lx, double* ux, double* ud);
 // pivots (device, n entries)

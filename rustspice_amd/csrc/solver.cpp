@@ -703,6 +703,30 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
     return RS_OK;
 }
 
+int32_t solver_solve_noalloc(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s, double* tmp,
+                             int64_t tmp_len, hipError_t tmp_err) {
+    if (h->bbd || h->dense)
+        return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed: no allocation-free solve for this solver kind");
+    if (h->state != ST_FACTORED || (!h->sn && !h->plan_on_device))
+        return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    if (h->n > tmp_len)
+        return fail(h, RS_ERR_SOLVE_FAILED,
+                    "Solve step failed: n = " + std::to_string(h->n) + " exceeds the allocation-free solve's " +
+                        std::to_string(tmp_len) + "-entry scratch");
+    if (tmp_err != hipSuccess) return hip_fail(h, tmp_err, "hipGetSymbolAddress(scratch)");
+    const int ps = prof_begin(PK_SOLVE, s);
+    hipError_t e;
+    if (h->sn)
+        e = sn_solve(h->sn, rhs_dev, x_dev, s, tmp);
+    else if (h->sol_mode == 0)  // the global solve vector is tmp
+        e = launch_solve(h->d_desc.p, 1, h->d_ip.p, h->d_ax.p, h->d_vals.p, rhs_dev, x_dev, h->d_work.p,
+                         (int)h->n, false, h->sol_threads, s, nullptr, tmp);
+    else  // the solve vector lives in LDS or registers: the handle's shape as it stands
+        e = solve_launch(h, rhs_dev, x_dev, h->sol_mode, h->sol_threads, s);
+    prof_end(ps, s);
+    return e == hipSuccess ? RS_OK : hip_fail(h, e, "allocation-free solve launch");
+}
+
 int32_t solver_solve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
     if (h->bbd) return bbd_solve_dev(h, rhs_dev, x_dev, s);
     if (h->dense) {

@@ -16,9 +16,12 @@ namespace rsgpu {
 hipError_t launch_refactor(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                            double* vals, double* wpool, int max_n, int mode, int threads,
                            hipStream_t stream, int rl_nv = 0, const int32_t* skip = nullptr);
+// tvec (one matrix, global path only): the solve vector in place of
+// wpool + o_tvec
 hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const double* ax,
                         const double* vals, const double* rhs, double* x, double* wpool, int max_n,
-                        bool lds, int threads, hipStream_t stream, const int32_t* skip = nullptr);
+                        bool lds, int threads, hipStream_t stream, const int32_t* skip = nullptr,
+                        double* tvec = nullptr);
 size_t device_max_lds();
 // solve mode 3: flat pipelined level sweeps of one matrix (PlanDev::sq_nlev > 0)
 size_t solve_lp_lds(int n, int maxent, int nlev);
@@ -145,6 +148,11 @@ struct rs_solver {
 };
 
 namespace rsgpu {
+// The device solve of rs_solve_device without allocation or tuning: the
+// kernels' solve vector is tmp (a device array of tmp_len doubles), the
+// launch shape the handle holds. tmp_err: the error of finding tmp.
+int32_t solver_solve_noalloc(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s, double* tmp,
+                             int64_t tmp_len, hipError_t tmp_err);
 bool solver_graph_ready(const rs_solver* h);  // false while the growth check is on
 hipError_t solver_graph_iteration(rs_solver* h, const double* ax_dev, const double* rhs_dev, double* x_dev,
                                   hipStream_t s, const int32_t* skip);

@@ -167,6 +167,12 @@ class GpuSolver:
                                             C.c_void_p(x_dev.data_ptr()),
                                             C.c_void_p(_stream_ptr(stream))))
 
+    def solve_device_noalloc(self, rhs_dev, x_dev, stream=None):
+        """rs_solve_device_noalloc: the device solve with no allocation."""
+        self._check(self._L.rs_solve_device_noalloc(self.h, C.c_void_p(rhs_dev.data_ptr()),
+                                                    C.c_void_p(x_dev.data_ptr()),
+                                                    C.c_void_p(_stream_ptr(stream))))
+
     def factor_pattern(self):
         """One-block factor pattern: col_perm[original] = position, L/U
         column pointers and row indices (position space)."""
