@@ -140,6 +140,15 @@ int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);
  * are device memory; `stream` is a hipStream_t (NULL = default stream). ---- */
 int32_t rs_refactor_device(rs_solver *h, const double *ax_dev, void *stream);
 int32_t rs_solve_device(rs_solver *h, const double *rhs_dev, double *x_dev, void *stream);
+/* A^T x = rhs with the stored L/U (no reference counterpart: the adjoint's
+ * transposed solve, A^T = Q^T U^T L^T P Q, reusing the factor of the last
+ * rs_factor / rs_refactor_device). Host (in place, like rs_solve) and device
+ * forms. Needs a single-block level-set factor: BbdSolver, Dense, the
+ * supernodal path and BTF-split matrices give RS_ERR_SOLVE_FAILED. Not the
+ * reference's arithmetic order (its sums are reduced across a wavefront):
+ * an FP64 solve, checked against a dense transposed solve. */
+int32_t rs_tsolve(rs_solver *h, double *rhs, int64_t n);
+int32_t rs_tsolve_device(rs_solver *h, const double *rhs_dev, double *x_dev, void *stream);
 /* rs_solve_device with no device allocation (no reference counterpart): for
  * callers that cannot allocate, such as code under HIP graph capture or a
  * serving loop that forbids hipMalloc. Same kernels and bit-identical
@@ -166,6 +175,36 @@ int32_t rs_get_factor_values(rs_solver *h, double *lx, double *ux, double *udiag
  * sizes still holds the array lengths rs_get_factor_values fills). */
 int32_t rs_get_factor_pattern(rs_solver *h, int64_t *sizes, int64_t *col_perm, int64_t *lcp,
                               int64_t *lri, int64_t *ucp, int64_t *uri);
+
+/* ---- many matrices per launch (no reference counterpart: N independent
+ * LinearSolver refactor/solve calls, native_klu.rs:1632-1657, as one launch
+ * each) ----------------------------------------------------------------------
+ * rs_solver_batch_create takes `count` CSC matrices concatenated: matrix m
+ * has n[m] columns, n[m] + 1 column pointers starting at 0 in ap_cat, and
+ * its nnz = ap[n] row indices and values next in ai_cat / ax_cat. Each is
+ * analyzed and factored on the host as rs_factor's first factor would
+ * (kind NativeKLU: AMD, BTF, threshold pivots; SparseLU: no pivots).
+ * rs_solver_batch_refactor then refactors all of them in one launch from
+ * device values in the same concatenated order (each NativeKLU matrix reuses
+ * its pivots), rs_solver_batch_solve solves all, one workgroup per matrix,
+ * with device vectors concatenated by matrix (rs_solver_batch_layout: sizes
+ * = {count, total nnz, total n}, the offsets of each matrix). Results equal
+ * a separate solver per matrix bit for bit. Errors are kept for
+ * rs_solver_batch_last_error (a failed create still returns a handle). */
+typedef struct rs_solver_batch rs_solver_batch;
+rs_solver_batch *rs_solver_batch_create(int32_t kind, int64_t count, const int64_t *n,
+                                        const int64_t *ap_cat, const int64_t *ai_cat,
+                                        const double *ax_cat);
+void rs_solver_batch_destroy(rs_solver_batch *b);
+int32_t rs_solver_batch_last_error(const rs_solver_batch *b, char *buf, int64_t len);
+int32_t rs_solver_batch_layout(const rs_solver_batch *b, int64_t *sizes, int64_t *ax_off,
+                               int64_t *vec_off);
+int32_t rs_solver_batch_refactor(rs_solver_batch *b, const double *ax_dev, void *stream);
+int32_t rs_solver_batch_solve(rs_solver_batch *b, const double *rhs_dev, double *x_dev, void *stream);
+/* Factor values of matrix m (lx[nnz_l], ux[nnz_u], udiag[n], position order,
+ * as rs_get_factor_values gives them for a single solver). */
+int32_t rs_solver_batch_factor_values(rs_solver_batch *b, int64_t m, double *lx, double *ux,
+                                      double *udiag);
 
 /* ---- circuit-level boundary: DeviceStamp (stamp.rs:94-121) for a whole
  * instance list, and the Newton drivers of engine.rs / newton.rs ---------- */
@@ -202,6 +241,13 @@ int32_t rs_circuit_set_bsim_models(rs_circuit *c, const rs_bsim_model *models, i
  * 2 saturation), vth, isub, gsub, igs, igd, gigs, gigd}. Host arrays. */
 int32_t rs_mos_eval(const rs_bsim_model *models, int64_t nmodels, const int32_t *model,
                     const int32_t *level, const double *bias, int64_t n, double *out);
+/* rs_stamp with x in device memory and ax / rhs written to device memory on
+ * `stream` (no host round trip; same values bit for bit). The pattern is the
+ * one rs_circuit_pattern reports afterwards. */
+int32_t rs_stamp_device(rs_circuit *c, int32_t mode, const double *x_dev, int64_t x_len, double t,
+                        double dt, double gmin, double source_scale, double *ax_dev, int64_t ax_cap,
+                        double *rhs_dev, int64_t rhs_cap, int64_t *nnz_out, int64_t *n_out,
+                        void *stream);
 /* Unmerged CSC pattern the stamp loop produces in `mode` (the engine closure
  * of engine.rs:262-283 / :485-501): call with ap = ai = NULL for the sizes. */
 int32_t rs_circuit_pattern(rs_circuit *c, int32_t mode, int64_t *n, int64_t *nnz, int64_t *ap,

@@ -37,7 +37,7 @@ EXPORTS = [
     "rs_set_config", "rs_get_stats", "rs_set_autotune", "rs_set_supernodal", "rs_supernodal_info", "rs_set_growth_check", "rs_growth_fallbacks",
     "rs_last_growth", "rs_set_launch_shape", "rs_launch_shape", "rs_last_error", "rs_refactor_device",
     "rs_solve_device", "rs_get_factor_values", "rs_get_factor_pattern",
-    "rs_solve_device_noalloc", "rs_noalloc_max_n",
+    "rs_solve_device_noalloc", "rs_noalloc_max_n", "rs_tsolve", "rs_tsolve_device",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest",
@@ -49,7 +49,10 @@ EXPORTS = [
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_info",
     "rs_batch_dc_op", "rs_batch_results", "rs_circuit_set_device_newton",
     "rs_batch_phase_profile", "rs_batch_phase_cycles", "rs_dc_adjoint", "rs_tran_adjoint",
-    "rs_circuit_set_tran_graph", "rs_circuit_graph_steps", "rs_tran_steps",
+    "rs_circuit_set_tran_graph", "rs_circuit_graph_steps", "rs_tran_steps", "rs_stamp_device",
+    "rs_solver_batch_create", "rs_solver_batch_destroy", "rs_solver_batch_last_error",
+    "rs_solver_batch_layout", "rs_solver_batch_refactor", "rs_solver_batch_solve",
+    "rs_solver_batch_factor_values",
 ]
 
 
@@ -115,6 +118,23 @@ def lib():
     L.rs_solve_device.argtypes = [vp, vp, vp, vp]
     L.rs_get_factor_values.restype = C.c_int32
     L.rs_get_factor_values.argtypes = [vp, f64p_n, f64p_n, f64p_n]
+    L.rs_tsolve.restype = C.c_int32
+    L.rs_tsolve.argtypes = [vp, f64p, C.c_int64]
+    L.rs_tsolve_device.restype = C.c_int32
+    L.rs_tsolve_device.argtypes = [vp, vp, vp, vp]
+    L.rs_solver_batch_create.restype = vp
+    L.rs_solver_batch_create.argtypes = [C.c_int32, C.c_int64, i64p, i64p, i64p, f64p]
+    L.rs_solver_batch_destroy.argtypes = [vp]
+    L.rs_solver_batch_last_error.restype = C.c_int32
+    L.rs_solver_batch_last_error.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.rs_solver_batch_layout.restype = C.c_int32
+    L.rs_solver_batch_layout.argtypes = [vp, i64p, i64p_n, i64p_n]
+    L.rs_solver_batch_refactor.restype = C.c_int32
+    L.rs_solver_batch_refactor.argtypes = [vp, vp, vp]
+    L.rs_solver_batch_solve.restype = C.c_int32
+    L.rs_solver_batch_solve.argtypes = [vp, vp, vp, vp]
+    L.rs_solver_batch_factor_values.restype = C.c_int32
+    L.rs_solver_batch_factor_values.argtypes = [vp, C.c_int64, f64p_n, f64p_n, f64p_n]
     L.rs_solve_device_noalloc.restype = C.c_int32
     L.rs_solve_device_noalloc.argtypes = [vp, vp, vp, vp]
     L.rs_noalloc_max_n.restype = C.c_int64

@@ -40,9 +40,6 @@ hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int
 hipError_t launch_newton_begin(double* t_dev, double dt, int32_t* st, double damping, hipStream_t s);
 hipError_t launch_newton_update(int n, double* x, const double* xn, double abs_tol, double rel_tol,
                                 double damping_min, int max_iters, int32_t* st, hipStream_t s);
-hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* vals, const double* e,
-                          const int32_t* lt_ptr, const int32_t* lt_cols, int n_lt, double* T,
-                          double* lam, hipStream_t s);
 hipError_t launch_adjoint_contract(const rs_device* dev, int ndev, const int32_t* aux_of,
                                    const double* x, int xl, const double* lam, double gmin,
                                    double scale, int mode, const double* xp, const double* xp2,
@@ -522,21 +519,9 @@ int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double 
     if (a->pd.nblk != 1) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint needs a single-block factor");
     hipError_t e;
     if (A.factors != a->factor_count || !A.ltp.p) {  // L^T levels of this factorization
-        const PlanHost& ph = a->ph;
-        std::vector<int32_t> lev(n, 0), ptr, cols;
-        int32_t nlev = 0;
-        for (i64 k = n - 1; k >= 0; k--) {
-            int32_t lv = 0;
-            for (int32_t q = ph.lcp[k]; q < ph.lcp[k + 1]; q++) lv = std::max(lv, lev[ph.lri[q]] + 1);
-            lev[k] = lv;
-            nlev = std::max(nlev, lv + 1);
-        }
-        ptr.assign(nlev + 1, 0);
-        for (i64 k = 0; k < n; k++) ptr[lev[k] + 1]++;
-        for (int32_t l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
-        cols.resize(n);
-        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
-        for (i64 k = 0; k < n; k++) cols[fill[lev[k]]++] = (int32_t)k;
+        std::vector<int32_t> ptr, cols;
+        lt_levels(a->ph, n, ptr, cols);
+        const int32_t nlev = (int32_t)ptr.size() - 1;
         std::vector<int32_t> aux(nd);
         for (i64 k = 0; k < nd; k++) aux[k] = dp->p.aux_of[k];
         if ((e = A.ltp.ensure(ptr.size())) || (e = A.ltc.ensure(std::max<size_t>(cols.size(), 1))) ||
@@ -943,6 +928,26 @@ int32_t rs_stamp(rs_circuit* c, int32_t mode, const double* x, int64_t x_len, do
         return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: stamp output buffers too small");
     if ((e = hipMemcpy(ax_out, c->d_ax.p, dp->p.ai.size() * 8, hipMemcpyDeviceToHost)) ||
         (e = hipMemcpy(rhs_out, c->d_rhs.p, dp->p.n * 8, hipMemcpyDeviceToHost)))
+        return chip(c, e, "hipMemcpy(out)");
+    return RS_OK;
+}
+
+int32_t rs_stamp_device(rs_circuit* c, int32_t mode, const double* x_dev, int64_t x_len, double t, double dt,
+                        double gmin, double source_scale, double* ax_dev, int64_t ax_cap, double* rhs_dev,
+                        int64_t rhs_cap, int64_t* nnz_out, int64_t* n_out, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    DevPattern* dp;
+    int32_t rc = get_pattern(c, mode, dp);
+    if (rc) return rc;
+    rc = stamp_dev(c, mode, x_dev, x_len, t, dt, gmin, source_scale, &dp, s);
+    if (rc) return rc;
+    if (nnz_out) *nnz_out = (int64_t)dp->p.ai.size();
+    if (n_out) *n_out = dp->p.n;
+    if ((int64_t)dp->p.ai.size() > ax_cap || dp->p.n > rhs_cap)
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: stamp output buffers too small");
+    hipError_t e;
+    if ((e = hipMemcpyAsync(ax_dev, c->d_ax.p, dp->p.ai.size() * 8, hipMemcpyDeviceToDevice, s)) ||
+        (e = hipMemcpyAsync(rhs_dev, c->d_rhs.p, dp->p.n * 8, hipMemcpyDeviceToDevice, s)))
         return chip(c, e, "hipMemcpy(out)");
     return RS_OK;
 }

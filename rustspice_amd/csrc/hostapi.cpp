@@ -18,10 +18,6 @@
 
 using namespace rsgpu;
 
-struct rs_host_factor {
-    HostFactor f;
-    PlanHost p;
-};
 
 extern "C" {
 

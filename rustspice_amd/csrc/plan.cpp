@@ -406,6 +406,23 @@ void append_level_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev
     d.sq_maxent = maxent;
 }
 
+void lt_levels(const PlanHost& p, int64_t n, std::vector<int32_t>& ptr, std::vector<int32_t>& cols) {
+    std::vector<int32_t> lev(n, 0);
+    int32_t nlev = 0;
+    for (int64_t k = n - 1; k >= 0; k--) {
+        int32_t lv = 0;
+        for (int32_t q = p.lcp[k]; q < p.lcp[k + 1]; q++) lv = std::max(lv, lev[p.lri[q]] + 1);
+        lev[k] = lv;
+        nlev = std::max(nlev, lv + 1);
+    }
+    ptr.assign(nlev + 1, 0);
+    for (int64_t k = 0; k < n; k++) ptr[lev[k] + 1]++;
+    for (int32_t l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
+    cols.resize(n);
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < n; k++) cols[fill[lev[k]]++] = (int32_t)k;
+}
+
 bool build_transposed_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d) {
     if (p.nblk != 1) return false;
     const int32_t n = p.n;

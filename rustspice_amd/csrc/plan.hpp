@@ -90,6 +90,11 @@ void append_level_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev
 // plan's descriptor with perm and record offsets into that pool.
 bool build_transposed_records(const PlanHost& p, std::vector<int32_t>& pool, PlanDev& d);
 
+// Levels of L^T (unit upper triangular) for the level-set transposed solve of
+// a single-block plan: column k waits for every row of L(:, k). ptr holds
+// nlev + 1 entries, cols the columns level by level.
+void lt_levels(const PlanHost& p, int64_t n, std::vector<int32_t>& ptr, std::vector<int32_t>& cols);
+
 // Gather L/U values of all blocks into position-space arrays (lx[nl], ux[nu],
 // ud[n]) matching PlanHost's lri/uri order.
 void gather_values(const HostFactor& f, std::vector<double>& lx, std::vector<double>& ux,
@@ -135,3 +140,9 @@ bool build_rl_plan(const PlanHost& p, RlHost& r);
 void append_rl(const RlHost& r, std::vector<int32_t>& pool, PlanDev& d);
 
 }  // namespace rsgpu
+
+// Host analysis and first factor with its plan (rs_host_factor_create).
+struct rs_host_factor {
+    rsgpu::HostFactor f;
+    rsgpu::PlanHost p;
+};

@@ -57,6 +57,7 @@ void reset(rs_solver* h) {
     h->last_ai.clear();
     h->factor_count = h->refactor_count = 0;
     h->rcond = 0.0;
+    h->ts_factors = -1;
 }
 
 // do_analyze (native_klu.rs:704-761)
@@ -703,6 +704,31 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
     return RS_OK;
 }
 
+int32_t solver_tsolve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s) {
+    if (h->bbd || h->dense || h->sn)
+        return fail(h, RS_ERR_SOLVE_FAILED,
+                    "Solve step failed: the transposed solve needs a single-block level-set factor");
+    if (h->state != ST_FACTORED || !h->plan_on_device) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    if (h->pd.nblk != 1 || h->ph.nod > 0)
+        return fail(h, RS_ERR_SOLVE_FAILED,
+                    "Solve step failed: the transposed solve needs a single-block factor (no BTF blocks)");
+    hipError_t e;
+    if (h->ts_factors != h->factor_count) {
+        std::vector<int32_t> ptr, cols;
+        lt_levels(h->ph, h->n, ptr, cols);
+        if ((e = h->ts_ptr.ensure(ptr.size())) || (e = h->ts_cols.ensure(std::max<size_t>(cols.size(), 1))) ||
+            (e = h->ts_t.ensure(std::max<int64_t>(h->n, 1))) ||
+            (e = hipMemcpy(h->ts_ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice)) ||
+            (!cols.empty() && (e = hipMemcpy(h->ts_cols.p, cols.data(), cols.size() * 4, hipMemcpyHostToDevice))))
+            return hip_fail(h, e, "transposed solve levels");
+        h->ts_nlev = (int)ptr.size() - 1;
+        h->ts_factors = h->factor_count;
+    }
+    e = launch_solve_t(h->d_desc.p, h->d_ip.p, h->d_vals.p, rhs_dev, h->ts_ptr.p, h->ts_cols.p, h->ts_nlev,
+                       h->ts_t.p, x_dev, s);
+    return e == hipSuccess ? RS_OK : hip_fail(h, e, "transposed solve launch");
+}
+
 int32_t solver_solve_noalloc(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s, double* tmp,
                              int64_t tmp_len, hipError_t tmp_err) {
     if (h->bbd || h->dense)
@@ -1095,6 +1121,31 @@ int32_t rs_refactor_device(rs_solver* h, const double* ax_dev, void* stream) {
 
 int32_t rs_solve_device(rs_solver* h, const double* rhs_dev, double* x_dev, void* stream) {
     return solver_solve_dev(h, rhs_dev, x_dev, (hipStream_t)stream);
+}
+
+int32_t rs_tsolve_device(rs_solver* h, const double* rhs_dev, double* x_dev, void* stream) {
+    return solver_tsolve_dev(h, rhs_dev, x_dev, (hipStream_t)stream);
+}
+
+int32_t rs_tsolve(rs_solver* h, double* rhs, int64_t n) {
+    if (h->state != ST_FACTORED) return fail(h, RS_ERR_SOLVE_FAILED, "Solve step failed");
+    if (n != h->n) {
+        char b[160];
+        std::snprintf(b, sizeof b, "Invalid matrix: RHS length %lld != matrix dimension %lld", (long long)n,
+                      (long long)h->n);
+        return fail(h, RS_ERR_INVALID_MATRIX, b);
+    }
+    hipError_t e;
+    if ((e = h->d_rhs.ensure(std::max<size_t>(h->d_rhs.n, n))) != hipSuccess ||
+        (e = h->d_x.ensure(std::max<size_t>(h->d_x.n, n))) != hipSuccess)
+        return hip_fail(h, e, "hipMalloc(solve vectors)");
+    if ((e = hipMemcpy(h->d_rhs.p, rhs, n * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(rhs)");
+    const int32_t rc = solver_tsolve_dev(h, h->d_rhs.p, h->d_x.p, nullptr);
+    if (rc) return rc;
+    if ((e = hipMemcpy(rhs, h->d_x.p, n * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpy(x)");
+    return RS_OK;
 }
 
 int32_t rs_get_factor_pattern(rs_solver* h, int64_t* sizes, int64_t* col_perm, int64_t* lcp, int64_t* lri,

@@ -23,6 +23,10 @@ hipError_t launch_solve(const PlanDev* descs, int nmat, const int32_t* ip, const
                         bool lds, int threads, hipStream_t stream, const int32_t* skip = nullptr,
                         double* tvec = nullptr);
 size_t device_max_lds();
+// A^T y = e of a single-block plan on its L^T levels (adjoint_kernels.hip)
+hipError_t launch_solve_t(const PlanDev* pd, const int32_t* ip, const double* vals, const double* e,
+                          const int32_t* lt_ptr, const int32_t* lt_cols, int n_lt, double* T,
+                          double* lam, hipStream_t s);
 // solve mode 3: flat pipelined level sweeps of one matrix (PlanDev::sq_nlev > 0)
 size_t solve_lp_lds(int n, int maxent, int nlev);
 hipError_t launch_solve_lp(const PlanDev* descs, const int32_t* ip, const double* vals, const double* rhs,
@@ -145,12 +149,19 @@ struct rs_solver {
     bool refactor_fell_back = false;
     double last_growth = 0.0;
     rsgpu::DevBuf<double> d_growth;
+    // transposed solve: L^T levels of the plan (built for factor_count ts_factors)
+    int64_t ts_factors = -1;
+    int ts_nlev = 0;
+    rsgpu::DevBuf<int32_t> ts_ptr, ts_cols;
+    rsgpu::DevBuf<double> ts_t;
 };
 
 namespace rsgpu {
 // The device solve of rs_solve_device without allocation or tuning: the
 // kernels' solve vector is tmp (a device array of tmp_len doubles), the
 // launch shape the handle holds. tmp_err: the error of finding tmp.
+// A^T x = rhs with the stored factor (single-block level-set plans).
+int32_t solver_tsolve_dev(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s);
 int32_t solver_solve_noalloc(rs_solver* h, const double* rhs_dev, double* x_dev, hipStream_t s, double* tmp,
                              int64_t tmp_len, hipError_t tmp_err);
 bool solver_graph_ready(const rs_solver* h);  // false while the growth check is on

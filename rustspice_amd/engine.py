@@ -75,6 +75,10 @@ def _bind():
                                      vp, vp]
     L.rs_circuit_set_cap_state.restype = C.c_int32
     L.rs_circuit_set_cap_state.argtypes = [vp] * 5
+    L.rs_stamp_device.restype = C.c_int32
+    L.rs_stamp_device.argtypes = [vp, C.c_int32, vp, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), vp]
     L.rs_stamp.restype = C.c_int32
     L.rs_stamp.argtypes = [vp, C.c_int32, vp, C.c_int64, C.c_double, C.c_double, C.c_double,
                            C.c_double, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64),
@@ -219,6 +223,18 @@ class GpuCircuit:
         ap, ai = self.pattern(mode)
         assert len(ai) == nnz.value and len(ap) - 1 == n.value
         return ap, ai, ax[: nnz.value], rhs[: n.value], n.value
+
+    def stamp_device(self, x_dev, ax_dev, rhs_dev, mode=MODE_DC, t=0.0, dt=0.0, gmin=1e-12,
+                     source_scale=1.0, stream=None):
+        """rs_stamp_device: x, ax and rhs are float64 device tensors (ax and
+        rhs at least as long as the pattern's nnz and n). Returns (nnz, n)."""
+        nnz, n = C.c_int64(), C.c_int64()
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        self._check(self._L.rs_stamp_device(self.h, mode, C.c_void_p(x_dev.data_ptr()), x_dev.numel(), t, dt,
+                                            gmin, source_scale, C.c_void_p(ax_dev.data_ptr()), ax_dev.numel(),
+                                            C.c_void_p(rhs_dev.data_ptr()), rhs_dev.numel(), C.byref(nnz),
+                                            C.byref(n), C.c_void_p(s)))
+        return nnz.value, n.value
 
     def dc_op(self, solver: GpuSolver, opts=None, last=False):
         """Returns (x or None, NewtonResult). With last=True, x is returned

@@ -85,6 +85,17 @@ class GpuSolver:
             raise TypeError("rhs must be a contiguous float64 array")
         self._check(self._L.rs_solve(self.h, rhs, len(rhs)))
 
+    def tsolve(self, rhs: np.ndarray):
+        """rs_tsolve: overwrites ``rhs`` with x of A^T x = rhs (the stored L/U)."""
+        if rhs.dtype != np.float64 or not rhs.flags.c_contiguous:
+            raise TypeError("rhs must be a contiguous float64 array")
+        self._check(self._L.rs_tsolve(self.h, rhs, len(rhs)))
+
+    def tsolve_device(self, rhs_dev, x_dev, stream=None):
+        self._check(self._L.rs_tsolve_device(self.h, C.c_void_p(rhs_dev.data_ptr()),
+                                             C.c_void_p(x_dev.data_ptr()),
+                                             C.c_void_p(_stream_ptr(stream))))
+
     def reset_pattern(self):
         self._L.rs_reset_pattern(self.h)
 
@@ -220,6 +231,67 @@ def host_btf(n, ap, ai):
     nb = _lib.lib().rs_host_btf(n, _i64(ap), _i64(ai), rp, cp, bp, C.byref(rank))
     return dict(row_perm=rp[:n], col_perm=cp[:n], block_ptr=bp[: nb + 1], num_blocks=int(nb),
                 structural_rank=rank.value)
+
+
+class GpuSolverBatch:
+    """rs_solver_batch_*: many CSC matrices (patterns may differ), each with
+    its first factor (pivots) on the host, then refactored in one launch and
+    solved in another from device vectors concatenated by matrix."""
+
+    def __init__(self, mats, kind=NATIVEKLU):
+        """mats: list of (ap, ai, ax) CSC triples."""
+        self._L = _lib.lib()
+        self.h = None
+        if not mats:
+            raise SolverError(6, "Invalid matrix: empty batch")
+        n = np.array([len(ap) - 1 for ap, _, _ in mats], np.int64)
+        ap_cat = np.concatenate([_i64(ap) for ap, _, _ in mats])
+        ai_cat = np.concatenate([_i64(ai) for _, ai, _ in mats])
+        ax_cat = np.concatenate([_f64(ax) for _, _, ax in mats])
+        self.h = self._L.rs_solver_batch_create(kind, len(mats), n, ap_cat, ai_cat, ax_cat)
+        err = self.last_error()
+        if err:
+            raise SolverError(6, err)
+        sz = np.zeros(3, np.int64)
+        self.ax_off = np.zeros(len(mats), np.int64)
+        self.vec_off = np.zeros(len(mats), np.int64)
+        self._L.rs_solver_batch_layout(self.h, sz, self.ax_off, self.vec_off)
+        self.count, self.nnz_total, self.n_total = map(int, sz)
+        self.n = n
+
+    def last_error(self):
+        buf = C.create_string_buffer(512)
+        self._L.rs_solver_batch_last_error(self.h, buf, 512)
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc:
+            raise SolverError(rc, self.last_error())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.rs_solver_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def refactor_device(self, ax_dev, stream=None):
+        self._check(self._L.rs_solver_batch_refactor(self.h, C.c_void_p(ax_dev.data_ptr()),
+                                                     C.c_void_p(_stream_ptr(stream))))
+
+    def solve_device(self, rhs_dev, x_dev, stream=None):
+        self._check(self._L.rs_solver_batch_solve(self.h, C.c_void_p(rhs_dev.data_ptr()),
+                                                  C.c_void_p(x_dev.data_ptr()),
+                                                  C.c_void_p(_stream_ptr(stream))))
+
+    def factor_values(self, m, nnz_l, nnz_u):
+        lx, ux, ud = np.zeros(max(nnz_l, 1)), np.zeros(max(nnz_u, 1)), np.zeros(int(self.n[m]))
+        self._check(self._L.rs_solver_batch_factor_values(self.h, m, lx, ux, ud))
+        return lx[:nnz_l], ux[:nnz_u], ud
 
 
 class HostFactor:

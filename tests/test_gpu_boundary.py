@@ -1,0 +1,181 @@
+"""Boundary entry points beside the LinearSolver trait: the transposed solve
+on the stored factor (rs_tsolve), stamping from device x (rs_stamp_device)
+and many matrices refactored and solved per launch (rs_solver_batch_*)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from rustspice_amd import engine as E
+from rustspice_amd import netlists as NL
+from rustspice_amd import solver as S
+from tests.helpers import same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_csc(n, density, seed, symmetric=False):
+    """A well-conditioned (diagonally dominant) random matrix, unsymmetric
+    pattern unless asked, some duplicate entries."""
+    rs = np.random.RandomState(seed)
+    pat = rs.rand(n, n) < density
+    if symmetric:
+        pat = pat | pat.T
+    np.fill_diagonal(pat, True)
+    dense = np.where(pat, rs.randn(n, n), 0.0)
+    np.fill_diagonal(dense, np.abs(dense).sum(1) + 1.0)
+    ap, ai, ax = [0], [], []
+    for j in range(n):
+        for i in np.nonzero(pat[:, j])[0]:
+            if rs.rand() < 0.1:  # a duplicate: two entries summing to the value
+                h = rs.rand()
+                ai += [i, i]
+                ax += [dense[i, j] * h, dense[i, j] * (1 - h)]
+            else:
+                ai.append(i)
+                ax.append(dense[i, j])
+        ap.append(len(ai))
+    A = np.zeros((n, n))
+    for j in range(n):
+        for q in range(ap[j], ap[j + 1]):
+            A[ai[q], j] += ax[q]
+    return np.array(ap, np.int64), np.array(ai, np.int64), np.array(ax), A
+
+
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+@pytest.mark.parametrize("n,density,seed", [(1, 1.0, 1), (7, 0.4, 2), (60, 0.08, 3), (300, 0.02, 4)])
+def test_tsolve_solves_the_transposed_system(gpu, kind, n, density, seed):
+    ap, ai, ax, A = _random_csc(n, density, seed)
+    g = S.GpuSolver(n, kind)
+    g.set_config(use_btf=False)
+    g.factor(ap, ai, ax)
+    g.factor(ap, ai, ax)  # the device refactor
+    b = np.random.RandomState(seed + 7).randn(n)
+    x = b.copy()
+    g.tsolve(x)
+    want = np.linalg.solve(A.T, b)
+    assert np.abs(x - want).max() <= 1e-11 * max(1.0, np.abs(want).max())
+    if n > 1:  # it is the transposed system, not the plain one
+        y = b.copy()
+        g.solve(y)
+        assert np.abs(y - want).max() > 1e-6 * np.abs(want).max()
+    d_b = torch.tensor(b, dtype=torch.float64, device="cuda")
+    d_x = torch.full_like(d_b, float("nan"))
+    g.tsolve_device(d_b, d_x)
+    torch.cuda.synchronize()
+    assert same_bits(d_x.cpu().numpy(), x)
+    # new values on the same pattern: the transposed solve follows the refactor
+    ax2 = ax * (1.0 + 0.3 * np.random.RandomState(seed + 9).rand(len(ax)))
+    A2 = np.zeros((n, n))
+    for j in range(n):
+        for q in range(ap[j], ap[j + 1]):
+            A2[ai[q], j] += ax2[q]
+    g.factor(ap, ai, ax2)
+    x2 = b.copy()
+    g.tsolve(x2)
+    want2 = np.linalg.solve(A2.T, b)
+    assert np.abs(x2 - want2).max() <= 1e-11 * max(1.0, np.abs(want2).max())
+
+
+def test_tsolve_refuses_what_it_cannot_do(gpu):
+    ap, ai, ax, A = _random_csc(40, 0.1, 5, symmetric=True)
+    for kind, setup in ((S.SPARSELU, lambda g: g.set_supernodal(2)), (S.BBD, lambda g: None)):
+        g = S.GpuSolver(40, kind)
+        setup(g)
+        g.factor(ap, ai, ax)
+        with pytest.raises(S.SolverError):
+            g.tsolve(np.ones(40))
+    g = S.GpuSolver(40, S.NATIVEKLU)
+    with pytest.raises(S.SolverError):  # nothing factored yet
+        g.tsolve(np.ones(40))
+
+
+@pytest.mark.parametrize("name", ["chain30", "rcmesh8"])
+def test_stamp_device_equals_host_stamp(gpu, name):
+    c = NL.inverter_chain(30) if name == "chain30" else NL.rc_mesh_inverters(8, 8)
+    g = E.GpuCircuit(c)
+    if name == "rcmesh8":  # capacitor history for the TRAN stamps
+        g.tran_fixed(S.GpuSolver(c.node_count), NL.C5_TRAN["dt"], 3)
+    rs = np.random.RandomState(3)
+    for mode, t, dt in ((E.MODE_DC, 0.0, 0.0), (E.MODE_TRAN_TRAP, 7e-12, 2e-12), (E.MODE_TRAN_BE, 5e-12, 1e-12)):
+        x = rs.rand(c.node_count + 4)
+        ap, ai, ax, rhs, n = g.stamp(x, mode=mode, t=t, dt=dt, gmin=1e-12, source_scale=0.75)
+        d_x = torch.tensor(x, dtype=torch.float64, device="cuda")
+        d_ax = torch.full((len(ax) + 3,), float("nan"), dtype=torch.float64, device="cuda")
+        d_rhs = torch.full((n + 2,), float("nan"), dtype=torch.float64, device="cuda")
+        nnz, nn = g.stamp_device(d_x, d_ax, d_rhs, mode=mode, t=t, dt=dt, gmin=1e-12, source_scale=0.75)
+        torch.cuda.synchronize()
+        assert (nnz, nn) == (len(ax), n)
+        assert same_bits(d_ax[:nnz].cpu().numpy(), ax)
+        assert same_bits(d_rhs[:nn].cpu().numpy(), rhs)
+    with pytest.raises(S.SolverError):  # output buffers too small
+        g.stamp_device(d_x, d_ax[:5], d_rhs, mode=E.MODE_DC)
+
+
+def _batch_mats():
+    rs = np.random.RandomState(8)
+    mats = []
+    ap, ai, ax, _, _ = O.build_mna(NL.resistor_mesh(8, 8))
+    mats.append((ap, ai, ax))
+    c = NL.inverter_chain(20)
+    ap, ai, ax, _, _ = O.build_mna(c, x=rs.rand(c.node_count))
+    mats.append((ap, ai, ax))
+    c = NL.power_grid(12, 12, pitch=4)
+    cv, ci = rs.rand(c.ndev), rs.randn(c.ndev) * 1e-4
+    hv, hi = np.ones(c.ndev, np.uint8), np.ones(c.ndev, np.uint8)
+    ap, ai, ax, _, _ = O.build_mna(c, x=rs.rand(c.node_count), mode=1, t=3e-12, dt=1e-12,
+                                   tstate=(cv, ci, hv, hi, 1))
+    mats.append((ap, ai, ax))
+    ap, ai, ax, _ = _random_csc(50, 0.08, 9)
+    mats.append((ap, ai, ax))
+    ap, ai, ax, _ = _random_csc(1, 1.0, 10)
+    mats.append((ap, ai, ax))
+    return mats
+
+
+@pytest.mark.parametrize("kind", [S.NATIVEKLU, S.SPARSELU])
+def test_batch_refactor_and_solve_equal_single_solvers(gpu, kind):
+    """Five matrices of different patterns and sizes in one launch: factor
+    values and solutions after a refactor on new values equal one solver per
+    matrix, and the oracle, bit for bit."""
+    mats = _batch_mats()
+    b = S.GpuSolverBatch(mats, kind)
+    assert b.count == len(mats)
+    rs = np.random.RandomState(4)
+    ax2 = [ax * (1.0 + 0.4 * rs.rand(len(ax))) for _, _, ax in mats]
+    rhs = [rs.randn(len(ap) - 1) for ap, _, _ in mats]
+    d_ax = torch.tensor(np.concatenate(ax2), dtype=torch.float64, device="cuda")
+    d_b = torch.tensor(np.concatenate(rhs), dtype=torch.float64, device="cuda")
+    d_x = torch.full_like(d_b, float("nan"))
+    b.refactor_device(d_ax)
+    b.solve_device(d_b, d_x)
+    torch.cuda.synchronize()
+    xb = d_x.cpu().numpy()
+    for m, ((ap, ai, ax), a2, r) in enumerate(zip(mats, ax2, rhs)):
+        n = len(ap) - 1
+        g = S.GpuSolver(n, kind)
+        g.factor(ap, ai, ax)
+        g.factor(ap, ai, a2)  # refactor with the first factor's pivots
+        x = r.copy()
+        g.solve(x)
+        o = O.NativeKlu(n)
+        if kind == S.SPARSELU:
+            o.set_sparselu()
+        o.factor(ap, ai, ax)
+        o.factor(ap, ai, a2)
+        xo = r.copy()
+        o.solve(xo)
+        got = xb[b.vec_off[m]:b.vec_off[m] + n]
+        assert same_bits(got, x), m
+        assert same_bits(got, xo), m
+        st = g.stats()
+        lx, ux, ud = b.factor_values(m, st["nnz_l"], st["nnz_u"])
+        glx, gux, gud = g.factor_values()
+        assert same_bits(lx, glx) and same_bits(ux, gux) and same_bits(ud, gud), m
+
+
+def test_batch_rejects_bad_input(gpu):
+    with pytest.raises(S.SolverError):
+        S.GpuSolverBatch([(np.array([0, 2, 1]), np.array([0, 1]), np.array([1.0, 2.0]))])
+    with pytest.raises(S.SolverError):
+        S.GpuSolverBatch([], S.NATIVEKLU)
