@@ -19,6 +19,7 @@ that the first appearance order equals the ids documented below.
 from __future__ import annotations
 
 import math
+import re
 from dataclasses import dataclass
 
 import numpy as np
@@ -161,6 +162,75 @@ def bsim_model(pmos: bool = False, tnom_c: float | None = None, **overrides) -> 
         rec[k] = v
     rec["pmos"] = 1 if pmos else 0
     return rec
+
+
+_RUST_F64 = re.compile(r"[+-]?((\d+\.?\d*|\.\d+)(e[+-]?\d+)?|inf|infinity|nan)")
+
+
+def _rust_f64(s: str):
+    """str::parse::<f64> on a lower-case string: an optional sign, then
+    digits with an optional point and exponent, or inf/infinity/nan; no
+    whitespace, underscores or hex (which Python's float() would take)."""
+    return float(s) if _RUST_F64.fullmatch(s) else None
+
+
+def parse_number(s: str):
+    """parse_number (crates/sim-devices/src/bsim/mod.rs:390-420): a number
+    with an optional SI suffix (meg, mil, f p n u m k g t; lower-cased,
+    trimmed); when the part before a one-letter suffix does not parse, the
+    whole string is tried as a plain number. None when nothing parses."""
+    t = s.lower().strip()
+    if t.endswith("meg"):
+        num, mult = t[:-3], 1e6
+    elif t.endswith("mil"):
+        num, mult = t[:-3], 25.4e-6
+    else:
+        sfx = {"f": 1e-15, "p": 1e-12, "n": 1e-9, "u": 1e-6, "m": 1e-3, "k": 1e3, "g": 1e9, "t": 1e12}
+        cut = max(len(t) - 1, 0)
+        num, mult = (t[:cut], sfx[t[cut:]]) if t[cut:] in sfx else (t, 1.0)
+    v = _rust_f64(num)
+    return v * mult if v is not None else _rust_f64(t)
+
+
+#: build_bsim_params (bsim/mod.rs:83-388): (BsimParams field, netlist keys
+#: tried in order); tnom is given in Celsius (+273.15) and pemod is cast to
+#: an unsigned integer
+BSIM_PARAM_KEYS = (
+    [("vth0", ("vth0", "vto", "vth"))] +
+    [(k, (k,)) for k in "k1 k2 dvt0 dvt1 dvt2 eta0 dsub nlx nfactor".split()] +
+    [("u0", ("u0", "uo"))] +
+    [(k, (k,)) for k in ("ua ub uc vsat a0 ags pclm pdiblc1 pdiblc2 pdiblcb drout tox lint wint rdsw "
+                         "rsh tnom ute kt1 kt1l kt2 cgso cgdo cgbo dvt0w dvt1w dvt2w voff voffl minv "
+                         "k3 k3b w0 lpe0 lpeb vfb ute0 ute1 pemod up lp ud ud1 eu vs vsattemp lambda "
+                         "vtl lc alpha0 alpha1 beta0 beta1 saref sbref wlod ku0 kvth0 ku0mult tku0 "
+                         "jtss jtsd nsti vtss vtsd pvag fprout pdits delta").split()])
+
+
+def build_bsim_params(params: dict, level: int, is_pmos: bool):
+    """build_bsim_params (bsim/mod.rs:83-388): the type's defaults, then every
+    parameter the map gives (keys as stored in the map; a key's value that
+    does not parse falls through to the next alias). Returns (record, parsed):
+    the rs_bsim_model record the device evaluation reads (level and type
+    apart) and every BsimParams field the map set, including the ones the
+    evaluation does not read."""
+    parsed = {}
+    for field, keys in BSIM_PARAM_KEYS:
+        for k in keys:
+            v = params.get(k.lower())
+            v = parse_number(v) if v is not None else None
+            if v is not None:
+                break
+        if v is None:
+            continue
+        if field == "tnom":
+            v = v + 273.15
+        elif field == "pemod":  # `v as u32`: truncation, saturating, NaN -> 0
+            v = 0 if v != v else int(min(max(v, 0.0), 4294967295.0))
+        parsed[field] = v
+    rec = bsim_model(pmos=is_pmos, **{k: v for k, v in parsed.items() if k in BSIM_FIELDS})
+    parsed["level"] = int(level)
+    parsed["pmos"] = bool(is_pmos)
+    return rec, parsed
 
 
 def bsim_params(level: int, model: int, w: float, l: float, temp: float = 300.15):

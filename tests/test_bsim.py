@@ -280,3 +280,92 @@ def test_gpu_bsim_dc_op(gpu, case, kind):
     assert (sg["factor_count"], sg["refactor_count"]) == (so["factor_count"], so["refactor_count"])
     if ro.converged:
         np.testing.assert_allclose(xg, xo, rtol=1e-9, atol=1e-12)
+
+
+# ---- conductances against central differences of ids --------------------------
+# evaluate_bsim_dc (evaluate.rs:132-243) forms gm, gds and gmbs as partial
+# derivatives that leave out some dependencies by design: mobility
+# degradation (ua, ub, uc), DIBL in vth (eta0, dsub; gds adds gm * eta0
+# instead), the series resistance (gds / (1 + rds gds)) and, in saturation,
+# the CLM factor's dependence on Vdsat. With those effects off the model's
+# current depends on the voltages only through the terms the conductances
+# differentiate, so there gm, gds and gmbs must be the derivatives of ids: a
+# dropped term or a wrong sign in either the current or a conductance fails.
+CLEAN = dict(ua=0.0, ub=0.0, uc=0.0, eta0=0.0, dsub=0.0, rdsw=0.0)
+GMIN = 1e-12
+
+
+def _fd(m, level, w, l, v, k, h=1e-6):
+    """Central difference of ids in terminal k (0 d, 1 g, 2 s, 3 b)."""
+    a, b = list(v), list(v)
+    a[k] += h
+    b[k] -= h
+    return (ev(m, level, w, l, *a)["ids"] - ev(m, level, w, l, *b)["ids"]) / (2 * h)
+
+
+def _close(got, want, rel):
+    return abs(got - want) <= rel * abs(want) or abs(got - want) <= 1.01 * GMIN
+
+
+@pytest.mark.parametrize("level", [49, 54])
+@pytest.mark.parametrize("region,nfactor,w,l,v", [
+    (1, 1.0, 1e-6, 1e-6, (0.1, 1.5, 0.0, 0.0)),
+    (1, 1.0, 1e-6, 1e-6, (0.2, 1.2, 0.0, -0.4)),
+    (1, 1.0, 4e-6, 2e-6, (0.2, 1.3, 0.1, -0.2)),
+    (0, 1.5, 1e-6, 1e-6, (0.5, 0.3, 0.0, 0.0)),
+    (0, 2.0, 1e-6, 1e-6, (0.05, 0.4, 0.0, -0.3)),
+    (2, 1.0, 1e-6, 1e-6, (1.8, 1.5, 0.0, 0.0)),
+])
+def test_nmos_conductances_are_derivatives(level, region, nfactor, w, l, v):
+    m = NL.bsim_model(nfactor=nfactor, **CLEAN)
+    r = ev(m, level, w, l, *v)
+    assert r["region"] == region and r["ids"] > 0.0
+    assert _close(r["gds"], _fd(m, level, w, l, v, 0), 1e-6)
+    if region == 2:  # gm and gmbs leave out d(CLM)/dVgs: checked on a long channel below
+        return
+    assert _close(r["gm"], _fd(m, level, w, l, v, 1), 1e-6)
+    assert _close(r["gmbs"], _fd(m, level, w, l, v, 3), 1e-6)
+    assert abs(r["gmbs"]) > 1e-3 * abs(r["gm"])  # the body term is really exercised
+
+
+@pytest.mark.parametrize("level", [49, 54])
+def test_saturation_gm_on_a_long_channel(level):
+    """Saturation: gm = beta Vdsat dVdsat/dVgs CLM leaves out the CLM factor's
+    Vdsat dependence, a term of order (Vds - Vdsat) / VA; on a 100 um channel
+    (VA ~ Esat L / PCLM large) that is below 2e-3 of gm."""
+    m = NL.bsim_model(**CLEAN)
+    v = (1.8, 1.5, 0.0, -0.3)
+    r = ev(m, level, 2e-6, 1e-4, *v)
+    assert r["region"] == 2
+    assert abs(r["gm"] - _fd(m, level, 2e-6, 1e-4, v, 1)) <= 2e-3 * r["gm"]
+    assert abs(r["gmbs"] - _fd(m, level, 2e-6, 1e-4, v, 3)) <= 2e-3 * abs(r["gmbs"])
+
+
+@pytest.mark.parametrize("level", [49, 54])
+def test_pmos_conductances_in_the_reference_frame(level):
+    """The PMOS is evaluated as the NMOS at (-vs, -vg, -vd, -vb) (evaluate.rs:
+    87-91); with vd < vs that internal Vds is negative, so the reverse-mode
+    swap (:98-102) runs too: Vgs = vs - vg and Vds = vs - vd, while Vbs stays
+    vd - vb (measured from the internal source, the PMOS drain). The
+    conductances are the derivatives in those variables, so
+    dids/dvg = gm, dids/dvb = gmbs, dids/dvd = gds - gmbs and
+    dids/dvs = -(gm + gds)."""
+    m = NL.bsim_model(pmos=True, **CLEAN)
+    for v in ((-0.1, -1.5, 0.0, 0.0), (-0.2, -1.2, 0.0, 0.4), (-0.6, -1.1, -0.1, 0.2)):
+        r = ev(m, level, 1e-6, 1e-6, *v)
+        assert r["region"] == 1 and r["ids"] < 0.0
+        assert _close(r["gm"], _fd(m, level, 1e-6, 1e-6, v, 1), 1e-6)
+        assert _close(r["gmbs"], _fd(m, level, 1e-6, 1e-6, v, 3), 1e-6)
+        assert _close(r["gds"] - r["gmbs"], _fd(m, level, 1e-6, 1e-6, v, 0), 1e-6)
+        assert _close(-(r["gm"] + r["gds"]), _fd(m, level, 1e-6, 1e-6, v, 2), 1e-6)
+        assert abs(r["gmbs"]) > 1e-3 * abs(r["gds"])
+
+
+def test_default_model_conductances_are_the_reference_approximations():
+    """With the default parameters (mobility degradation and DIBL on) the
+    conductances are not the derivatives of ids: gds in saturation carries
+    gm * eta0 and misses the CLM slope's other terms. Recorded so that a
+    restatement which 'fixed' them would be noticed."""
+    r = ev(NM, 49, 1e-6, 1e-6, 1.8, 1.5, 0.0, 0.0)
+    fd = _fd(NM, 49, 1e-6, 1e-6, (1.8, 1.5, 0.0, 0.0), 0)
+    assert r["region"] == 2 and abs(r["gds"] - fd) > 0.2 * fd
