@@ -325,6 +325,19 @@ def batched_dc_ops(args, E, NL, S, torch):
         same &= (r.iterations == out[i][1].iterations and _same(r.final_norm, out[i][1].final_norm))
     torch.cuda.synchronize()
     loop_per = (time.perf_counter() - t0) / k
+    # the same sample through the device-resident plain backend (each DC OP
+    # one k_batch_dcop launch of its own)
+    gds = [(E.GpuCircuit(circs[i]), S.GpuSolver(circs[i].node_count)) for i in idx]
+    for g, s in gds:
+        g.dc_op(s)  # warm-up: patterns and plans
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    same_dev = True
+    for (g, s), i in zip(gds, idx):
+        _, r = g.dc_op(s)
+        same_dev &= (r.iterations == out[i][1].iterations and _same(r.final_norm, out[i][1].final_norm))
+    torch.cuda.synchronize()
+    dev_per = (time.perf_counter() - t0) / k
     return dict(
         workload=f"{args.batch} Monte-Carlo level-1 inverter chains, stages U[50,400], DC OP each "
                  "(reference semantics: NativeKLU, no gmin/source stepping)",
@@ -333,7 +346,10 @@ def batched_dc_ops(args, E, NL, S, torch):
         instances=args.batch, patterns=info["patterns"], grid=info["grid"], nmax=info["nmax"],
         converged=conv, newton_iters_total=int(iters.sum()), setup_s=setup_s,
         loop_plain_backend_dc_ops_per_s=1.0 / loop_per, loop_sample=k,
-        speedup_vs_loop_plain=dcops * loop_per, loop_results_identical=bool(same))
+        speedup_vs_loop_plain=dcops * loop_per, loop_results_identical=bool(same),
+        loop_device_resident_dc_ops_per_s=1.0 / dev_per,
+        speedup_vs_loop_device_resident=dcops * dev_per,
+        loop_device_resident_results_identical=bool(same_dev))
 
 
 def adjoint_cost(E, NL, S, torch, reps=5):
@@ -553,8 +569,13 @@ def tran_graph_cost(E, NL, S):
 
 
 def cpu_baseline_mesh(mesh, np, reps=10):
-    """The oracle's refactor + solve of the same mesh matrix on one host core,
-    and whether the GPU result is bit-identical to it."""
+    """The oracle's refactor + solve of the same mesh matrix on one host core
+    and the column-parallel refactor (native_klu.rs:402-576) on all host
+    threads, whether the GPU result is bit-identical to them, and how far the
+    reference-semantics answer lies from an exact FP64 solve of the matrix
+    (ground row and column removed, node voltages against ground)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
     from oracle import oracle as O
 
     ap, ai, ax, rhs, n, xg = mesh
@@ -566,8 +587,35 @@ def cpu_baseline_mesh(mesh, np, reps=10):
         bb = rhs.copy()
         o.solve(bb)
     t_cpu = (time.perf_counter() - t0) / reps
-    return dict(cpu_oracle_ms=1e3 * t_cpu, cores=1, kind="port",
-                bit_identical=bool(np.array_equal(xg.view(np.int64), bb.view(np.int64))))
+    model, ncpu = _cpu_info()
+    out = dict(cpu_oracle_ms=1e3 * t_cpu, cores=1, kind="port", cpu_model=model, nproc=ncpu,
+               bit_identical=bool(np.array_equal(xg.view(np.int64), bb.view(np.int64))))
+    blocks = [o.block(b) for b in range(o.stats()["nblocks"])]
+    if len(blocks) == 1:  # one BTF block: the column-parallel refactor applies
+        blk = blocks[0]
+        rp = blk["row_perm"]
+        rpi = np.empty_like(rp)
+        rpi[rp] = np.arange(len(rp))
+        for th in (1, max(1, min(16, ncpu))):
+            cp = O.ColPar(blk, row_perm_inv=rpi)
+            cp.factor(ap, ai, ax, nthreads=th)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                cp.factor(ap, ai, ax, nthreads=th)
+                xc = cp.solve(rhs)
+            out[f"column_parallel_{th}t_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+            out[f"column_parallel_{th}t_bit_identical"] = bool(np.array_equal(xc.view(np.int64),
+                                                                              xg.view(np.int64)))
+    A = sp.csc_matrix((ax, ai, ap), shape=(n, n))
+    xe = np.zeros(n)
+    xe[1:] = spl.spsolve(A[1:, 1:].tocsc(), rhs[1:])
+    v_ref, v_ex = xg[1:] - xg[0], xe[1:]
+    out["distance_from_exact"] = dict(
+        max_rel_node_voltages=float(np.abs(v_ref - v_ex).max() / np.abs(v_ex).max()),
+        max_rel_unreferenced=float(np.abs(xg[1:] - v_ex).max() / np.abs(v_ex).max()),
+        ground_potential_x0=float(xg[0]),
+        note="exact: scipy spsolve with the ground row and column removed")
+    return out
 
 
 def chain_tran(args, E, NL, S, torch, np):
