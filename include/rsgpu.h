@@ -132,6 +132,10 @@ double rs_last_growth(const rs_solver *h);
 int32_t rs_set_launch_shape(rs_solver *h, int32_t ref_threads, int32_t ref_mode, int32_t sol_threads,
                             int32_t sol_mode);
 int32_t rs_launch_shape(const rs_solver *h, int64_t *out);
+/* The payload of the last RS_ERR_SINGULAR: SolverError::SingularMatrix's
+ * pivot (solver.rs:84, raised by the dense Schur factor, bbd_solver.rs:138-140);
+ * -1 if none. RS_ERR_ILL_CONDITIONED's rcond is rs_get_stats' rcond. */
+int64_t rs_error_pivot(const rs_solver *h);
 /* Text of the last error (SolverError Display, solver.rs:94-117). */
 int32_t rs_last_error(const rs_solver *h, char *buf, int64_t len);
 

@@ -37,7 +37,7 @@ EXPORTS = [
     "rs_set_config", "rs_get_stats", "rs_set_autotune", "rs_set_supernodal", "rs_supernodal_info", "rs_set_growth_check", "rs_growth_fallbacks",
     "rs_last_growth", "rs_set_launch_shape", "rs_launch_shape", "rs_last_error", "rs_refactor_device",
     "rs_solve_device", "rs_get_factor_values", "rs_get_factor_pattern",
-    "rs_solve_device_noalloc", "rs_noalloc_max_n", "rs_tsolve", "rs_tsolve_device",
+    "rs_solve_device_noalloc", "rs_noalloc_max_n", "rs_error_pivot", "rs_tsolve", "rs_tsolve_device",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest",
@@ -135,6 +135,8 @@ def lib():
     L.rs_solver_batch_solve.argtypes = [vp, vp, vp, vp]
     L.rs_solver_batch_factor_values.restype = C.c_int32
     L.rs_solver_batch_factor_values.argtypes = [vp, C.c_int64, f64p_n, f64p_n, f64p_n]
+    L.rs_error_pivot.restype = C.c_int64
+    L.rs_error_pivot.argtypes = [vp]
     L.rs_solve_device_noalloc.restype = C.c_int32
     L.rs_solve_device_noalloc.argtypes = [vp, vp, vp, vp]
     L.rs_noalloc_max_n.restype = C.c_int64

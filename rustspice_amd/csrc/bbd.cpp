@@ -509,7 +509,7 @@ int32_t build_bbd(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* a
         (e = b->d_work.ensure(n + 1)) || (e = b->d_z.ensure(n + 1)) || (e = b->d_w.ensure(n + 1)) ||
         (e = b->d_wz.ensure(n + 1)) || (e = b->d_bb.ensure(bs + 1)) || (e = b->d_xt.ensure(bs + 1)) ||
         (e = b->d_tmp.ensure(bs + 1)) || (e = b->d_piv.ensure(bs + 1)) ||
-        (e = b->d_status.ensure(1)))
+        (e = b->d_status.ensure(2)))
         return bhip(h, e, "BBD buffers");
     if ((e = hipMemset(b->d_wpool.p, 0, (wp + 1) * sizeof(double))))
         return bhip(h, e, "BBD work");
@@ -589,13 +589,15 @@ int32_t bbd_factor(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* 
                                   b->d_y2.p, bs, b->ncells, s)))
             return bhip(h, e, "BBD Schur assembly");
         if (b->dense) {
-            int32_t st = 0;
+            int32_t st[2] = {0, -1};
             if ((e = launch_dense_lu(b->d_s.p, b->d_piv.p, b->d_status.p, bs, 1, s)) ||
-                (e = hipMemcpy(&st, b->d_status.p, sizeof st, hipMemcpyDeviceToHost)))
+                (e = hipMemcpy(st, b->d_status.p, sizeof st, hipMemcpyDeviceToHost)))
                 return bhip(h, e, "BBD dense Schur LU");
-            if (st) {
+            if (st[0]) {  // DenseSchurSolver::factor (bbd_solver.rs:138-140)
                 b->factored = false;
-                return bfail(h, RS_ERR_SINGULAR, "Singular matrix at Schur pivot");
+                h->err_pivot = st[1];
+                return bfail(h, RS_ERR_SINGULAR,
+                             "Singular matrix: zero pivot at row/column " + std::to_string(st[1]));
             }
         } else {
             rc = solver_factor_dev(b->schur, b->s_ap.data(), bs + 1, b->s_ai.data(),
@@ -755,7 +757,7 @@ int32_t dense_factor(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t
     int32_t st = 0;
     if ((e = put(d->d_ap, ap32)) || (e = put(d->d_ai, ai32)) ||
         (e = d->d_ax.ensure(nnz + 1)) || (e = d->d_lu.ensure(n * n + 1)) ||
-        (e = d->d_piv.ensure(n + 1)) || (e = d->d_status.ensure(1)) ||
+        (e = d->d_piv.ensure(n + 1)) || (e = d->d_status.ensure(2)) ||
         (e = d->d_rhs.ensure(n + 1)) || (e = d->d_b.ensure(n + 1)) ||
         (e = hipMemcpy(d->d_ax.p, ax, nnz * sizeof(double), hipMemcpyHostToDevice)) ||
         (e = hipMemset(d->d_lu.p, 0, (n * n + 1) * sizeof(double))) ||

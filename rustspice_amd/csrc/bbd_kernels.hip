@@ -135,7 +135,7 @@ __global__ void k_dense_build(double* __restrict__ lu, const int32_t* __restrict
 __global__ __launch_bounds__(kT) void k_dense_lu(double* __restrict__ lu, int32_t* __restrict__ piv,
                                                  int32_t* __restrict__ status, int64_t n,
                                                  int schur) {
-    __shared__ int64_t s_p;
+    __shared__ int64_t s_p, s_kfail;
     __shared__ int s_fail;
     const int t = threadIdx.x;
     for (int64_t i = t; i < n; i += kT) piv[i] = (int32_t)i;
@@ -152,7 +152,10 @@ __global__ __launch_bounds__(kT) void k_dense_lu(double* __restrict__ lu, int32_
                     p = i;
                 }
             }
-            if (schur ? (mx < 1e-30) : (mx == 0.0)) s_fail = schur ? 4 : 2;
+            if (schur ? (mx < 1e-30) : (mx == 0.0)) {
+                s_fail = schur ? 4 : 2;
+                s_kfail = k;
+            }
             s_p = p;
         }
         __syncthreads();
@@ -181,7 +184,10 @@ __global__ __launch_bounds__(kT) void k_dense_lu(double* __restrict__ lu, int32_
         }
         __syncthreads();
     }
-    if (t == 0) *status = s_fail;
+    if (t == 0) {  // status[0]: 0, 2 (FactorFailed) or 4 (SingularMatrix); status[1]: the pivot step
+        status[0] = s_fail;
+        status[1] = s_fail ? (int32_t)s_kfail : -1;
+    }
 }
 
 // Forward and backward substitution of the dense factor (solver.rs:698-721,

@@ -1100,6 +1100,8 @@ int32_t rs_get_stats(const rs_solver* hc, rs_solver_stats* o) {
     return RS_OK;
 }
 
+int64_t rs_error_pivot(const rs_solver* h) { return h->err_pivot; }
+
 int32_t rs_last_error(const rs_solver* h, char* buf, int64_t len) {
     if (len > 0) {
         std::snprintf(buf, (size_t)len, "%s", h->err.c_str());

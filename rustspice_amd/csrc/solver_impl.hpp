@@ -150,6 +150,7 @@ struct rs_solver {
     double last_growth = 0.0;
     rsgpu::DevBuf<double> d_growth;
     // transposed solve: L^T levels of the plan (built for factor_count ts_factors)
+    int64_t err_pivot = -1;  // the pivot of the last RS_ERR_SINGULAR
     int64_t ts_factors = -1;
     int ts_nlev = 0;
     rsgpu::DevBuf<int32_t> ts_ptr, ts_cols;

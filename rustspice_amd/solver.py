@@ -23,9 +23,13 @@ _ERR_NAMES = {
 
 
 class SolverError(RuntimeError):
-    def __init__(self, code: int, message: str):
+    """A SolverError variant (solver.rs:76-92) with its payload: `pivot` for
+    SingularMatrix, `rcond` for IllConditioned."""
+
+    def __init__(self, code: int, message: str, pivot=None, rcond=None):
         self.code = code
         self.variant = _ERR_NAMES.get(code, f"Unknown({code})")
+        self.pivot, self.rcond = pivot, rcond
         super().__init__(f"{self.variant}: {message}")
 
 
@@ -64,7 +68,9 @@ class GpuSolver:
         if rc != 0:
             buf = C.create_string_buffer(512)
             self._L.rs_last_error(self.h, buf, 512)
-            raise SolverError(rc, buf.value.decode(errors="replace"))
+            pivot = int(self._L.rs_error_pivot(self.h)) if rc == 4 else None
+            rcond = self.stats()["rcond"] if rc == 5 else None
+            raise SolverError(rc, buf.value.decode(errors="replace"), pivot=pivot, rcond=rcond)
 
     # --- LinearSolver ---------------------------------------------------------
     def prepare(self, n: int):

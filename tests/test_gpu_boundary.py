@@ -179,3 +179,34 @@ def test_batch_rejects_bad_input(gpu):
         S.GpuSolverBatch([(np.array([0, 2, 1]), np.array([0, 1]), np.array([1.0, 2.0]))])
     with pytest.raises(S.SolverError):
         S.GpuSolverBatch([], S.NATIVEKLU)
+
+
+def test_singular_schur_reports_the_pivot(gpu):
+    """BbdSolver whose dense Schur complement has a zero first column:
+    SolverError::SingularMatrix { pivot: 0 } (bbd_solver.rs:138-140), the
+    payload carried by rs_error_pivot, as the oracle reports it."""
+    n = 5
+    A = np.zeros((n, n))
+    for i, j in [(0, 1), (1, 4), (4, 2), (2, 3)]:  # the chain 0-1-4-2-3
+        A[i, j] = A[j, i] = -1.0
+    np.fill_diagonal(A, 2.0)
+    A[1, 4] = A[4, 1] = 0.0  # kept as explicit zeros
+    A[1, 1] = 0.5  # Schur(1, 1) = 0.5 - 1 * 1 / 2 = 0 with block {0}
+    ap, ai, ax = [0], [], []
+    for j in range(n):
+        for i in range(n):
+            if A[i, j] != 0 or (i, j) in ((1, 4), (4, 1)):
+                ai.append(i)
+                ax.append(A[i, j])
+        ap.append(len(ai))
+    ap, ai, ax = np.array(ap, np.int64), np.array(ai, np.int64), np.array(ax)
+    d = O.bbd_decompose(n, ap, ai, 2)
+    assert d["num_blocks"] == 2 and d["border_size"] == 2
+    o = O.Bbd(n, 2)
+    assert o.factor(ap, ai, ax) == 4
+    g = S.GpuSolver(n, S.BBD)
+    g.set_bbd_blocks(2)
+    with pytest.raises(S.SolverError) as ei:
+        g.factor(ap, ai, ax)
+    assert ei.value.code == 4 and ei.value.variant and ei.value.pivot == 0
+    assert "zero pivot at row/column 0" in str(ei.value)
