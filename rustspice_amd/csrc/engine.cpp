@@ -22,6 +22,7 @@
 #include "../../include/rsgpu.h"
 #include "solver_impl.hpp"
 #include "stamp.hpp"
+#include "fold.hpp"
 #include "profile.hpp"
 
 namespace rsgpu {
@@ -33,7 +34,7 @@ hipError_t launch_mos_eval(const rs_bsim_model* models, const int32_t* mi, const
                            const double* in, int64_t n, double* out, hipStream_t s);
 hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
                            const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
-                           hipStream_t s, const int32_t* skip = nullptr);
+                           const int32_t* long_rows, int nlong, hipStream_t s, const int32_t* skip = nullptr);
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
                             uint8_t* has_i, hipStream_t s, const int32_t* gate = nullptr);
@@ -71,6 +72,8 @@ struct DevPattern {
     std::vector<uint8_t> gmask;  // BSIM group masks the pattern was built for
     StampPattern p;
     DevBuf<int32_t> ax_src, rhs_ptr, rhs_src;
+    DevBuf<int32_t> long_rows;  // RHS rows assembled by a workgroup each (kFoldRhs)
+    int nlong = 0;
 };
 }  // namespace
 
@@ -161,6 +164,15 @@ int32_t get_pattern(rs_circuit* c, int mode, DevPattern*& out) {
             (!dp.p.rhs_src.empty() &&
              (e = hipMemcpy(dp.rhs_src.p, dp.p.rhs_src.data(), dp.p.rhs_src.size() * 4, hipMemcpyHostToDevice))))
             return chip(c, e, "hipMemcpy(pattern)");
+        {
+            std::vector<int32_t> lr;
+            for (size_t r = 0; r + 1 < dp.p.rhs_ptr.size(); r++)
+                if (dp.p.rhs_ptr[r + 1] - dp.p.rhs_ptr[r] > kFoldRhs) lr.push_back((int32_t)r);
+            dp.nlong = (int)lr.size();
+            if ((e = dp.long_rows.ensure(std::max<size_t>(lr.size(), 1))) ||
+                (!lr.empty() && (e = hipMemcpy(dp.long_rows.p, lr.data(), lr.size() * 4, hipMemcpyHostToDevice))))
+                return chip(c, e, "hipMemcpy(long rows)");
+        }
         size_t nmax = (size_t)dp.p.n;
         if ((e = c->d_ax.ensure(std::max(c->d_ax.n, dp.p.ai.size()))) ||
             (e = c->d_rhs.ensure(std::max(c->d_rhs.n, nmax))) ||
@@ -205,7 +217,7 @@ int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t
     if ((rc = get_pattern(c, mode, dp))) return rc;
     ps = prof_begin(PK_ASSEMBLE, s);
     e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p,
-                        (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, s);
+                        (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, dp->long_rows.p, dp->nlong, s);
     prof_end(ps, s);
     if (e) return chip(c, e, "k_assemble");
     c->stamp_calls++;
@@ -736,7 +748,7 @@ int32_t tran_fixed_graph(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o,
         if ((e = launch_eval(c->d_dev.p, (int)c->dev.size(), c->d_x.p, (int)n, a, c->d_cap_v.p, c->d_cap_i.p,
                              c->d_has_v.p, c->d_has_i.p, c->d_sv.p, s, nullptr, c->d_gmask.p)) ||
             (e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p, (int)n,
-                                 c->d_sv.p, c->d_ax.p, c->d_rhs.p, s, st)) ||
+                                 c->d_sv.p, c->d_ax.p, c->d_rhs.p, dp->long_rows.p, dp->nlong, s, st)) ||
             (e = solver_graph_iteration(sol, c->d_ax.p, c->d_rhs.p, c->d_xnew.p, s, st)) ||
             (e = launch_newton_update((int)n, c->d_x.p, c->d_xnew.p, o->abs_tol, o->rel_tol, o->damping_min,
                                       o->max_iters, st, s)))
@@ -830,6 +842,7 @@ void rs_circuit_destroy(rs_circuit* c) {
         p.ax_src.release();
         p.rhs_ptr.release();
         p.rhs_src.release();
+        p.long_rows.release();
     }
     c->d_dev.release();
     c->d_sv.release();

@@ -52,6 +52,24 @@ constexpr int32_t kSnTile = 64;
 // An entry that this many contributions touch is summed on its own.
 constexpr int32_t kSnHotRecords = 256;
 
+// A gather record as the kernel reads it: the entries it touches in its
+// tile as a set of rows and a set of columns (bit r of rowmask: tile row r;
+// the k-th set bit is the record's k-th row, in order), and where its
+// operands start: L(row k, s) = v[lbase + s * lds + k], U(s, column k) =
+// v[ubase + k * ws + s], s in [0, ws).
+struct SnRec {
+    int64_t lbase, ubase;
+    uint64_t rowmask, colmask;
+    int32_t ws, lds;
+};
+
+// A stretch [q0, q1) of one long ordered sum (a hot entry's terms, a long
+// forward row's terms), folded on a side stream once its terms are final.
+struct SnPart {
+    int64_t q0, q1;
+    int32_t idx, pad;  // hot entry, or y position
+};
+
 struct SnHost {
     int32_t n = 0, ns = 0, nlev = 0;
     int64_t nvals = 0;               // doubles in the panel pool
@@ -80,6 +98,15 @@ struct SnHost {
     // a flat list of (L slot, U slot) terms in the reference's order
     std::vector<int64_t> hot_slot, hot_ptr, hot_terms;  // hot_terms: 2 per term
     std::vector<int32_t> hot_lev;                       // level of each hot entry
+    std::vector<int32_t> hot_tlev;                      // level of each term's source supernode
+    // the records in the kernel's form (rec's order), and per gather item
+    // the hot entries inside its tile as row | column << 8
+    std::vector<SnRec> frec;
+    std::vector<int32_t> tile_hot_ptr, tile_hot;
+    // per gather item and quarter of its columns (a wavefront's share), the
+    // records that touch the quarter, in order: qrec[qrec_ptr[4 it + q] ..]
+    std::vector<int64_t> qrec_ptr;
+    std::vector<int32_t> qrec;
     // forward-solve rows: for every position p, the supernodes S (ascending)
     // with p in R_S and p's index t in R_S
     std::vector<int64_t> lrow_ptr;
@@ -103,7 +130,10 @@ constexpr int32_t kSnPanel = 32;
 // Device view of a plan (plain pointers, passed by value to the kernels).
 struct SnPtrs {
     const int32_t *j0, *rptr, *rows, *csrc, *ca0, *ca1, *rel, *perm, *lev_sn;
-    const int32_t* rec;               // tile records, 5 ints each
+    const SnRec* frec;                // tile records
+    const int32_t *tile_hot_ptr, *tile_hot;
+    const int64_t* qrec_ptr;
+    const int32_t* qrec;
     const int4* gitems;
     const int64_t *oL, *oU, *crel_off, *tile_rec, *hot_slot;
     const int64_t *lrow_ptr;          // forward-solve row lists: per position

@@ -9,6 +9,7 @@
 // a rounded difference, as in the reference.
 #include <hip/hip_runtime.h>
 
+#include "fold.hpp"
 #include "sn.hpp"
 
 namespace {
@@ -40,15 +41,79 @@ __global__ __launch_bounds__(kT) void k_sn_scatter(const int32_t* __restrict__ s
     v[slot[i]] = x;
 }
 
-// One workgroup per gather tile: the tile is held in LDS, its records are
-// applied in order (a barrier between records), each entry of a record by one
-// thread, its terms ascending with zero multipliers skipped.
-constexpr int kStageMin = 256;  // record entries from which L and U pass through LDS
+// One workgroup per gather tile, no barrier: wavefront w owns the tile's
+// columns 16w .. 16w + 15 and lane r its row r, so the workgroup's 256 lanes
+// hold the 64 x 64 tile in registers, 16 entries each. Each wavefront runs
+// through the records that touch its columns, in order, and applies the
+// terms of its own entries: an entry therefore takes its records in order
+// and, inside a record, its terms s = 0 .. ws - 1 in order, skipping zero
+// multipliers, as the reference does. The record's row and column masks give
+// each lane its L row and each column its U column by rank.
+//
+// The work is a stream of chunks (a record's terms s0 .. s0 + 7): the record
+// fields of 64 records at a time sit one per lane (read out with readlane),
+// and the operands of the next chunk are loaded while the current one is
+// applied, L one value per lane and term, U (the same for every row) one
+// value per lane for 8 columns x 8 terms.
+constexpr int kCols = kTile / 4;  // columns per wavefront
+constexpr int kSb = 8;            // terms per chunk
+
+__device__ __forceinline__ uint64_t bits_below(int b) { return b == 0 ? 0ull : (~0ull >> (64 - b)); }
+
+__device__ __forceinline__ int64_t rl64(int64_t v, int k) {
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, k);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), k);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ double rlf64(double v, int k) {
+    return __longlong_as_double((long long)rl64((int64_t)__double_as_longlong(v), k));
+}
+
+struct GRec {  // one record, wavefront-uniform
+    int64_t lbase, ubase;
+    uint64_t rowmask;
+    uint32_t cm;  // this wavefront's 16 columns of colmask
+    int crank0, ws, lds;
+};
+
+struct GOps {  // one chunk's operands
+    double l[kSb];
+    double u0, u1;  // U(s0 + (lane & 7), column (lane >> 3) [+ 8])
+};
+
+__device__ __forceinline__ void gather_load(const double* __restrict__ v, const GRec& R, int s0, int lane,
+                                            uint64_t below, GOps& o) {
+    const bool mine = (R.rowmask >> lane) & 1;
+    const double* L = v + R.lbase + __popcll(R.rowmask & below);
+#pragma unroll
+    for (int j = 0; j < kSb; j++) o.l[j] = (mine && s0 + j < R.ws) ? L[(int64_t)(s0 + j) * R.lds] : 0.0;
+    const int s = s0 + (lane & 7), c = lane >> 3;
+    const bool a0 = ((R.cm >> c) & 1) && s < R.ws, a1 = ((R.cm >> (c + 8)) & 1) && s < R.ws;
+    const int k0 = R.crank0 + __popc(R.cm & ((1u << c) - 1u));
+    const int k1 = R.crank0 + __popc(R.cm & ((1u << (c + 8)) - 1u));
+    o.u0 = a0 ? v[R.ubase + (int64_t)k0 * R.ws + s] : 0.0;
+    o.u1 = a1 ? v[R.ubase + (int64_t)k1 * R.ws + s] : 0.0;
+}
+
+__device__ __forceinline__ void gather_apply(const GRec& R, int s0, const GOps& o, bool mine, uint32_t skip,
+                                             double (&acc)[kCols]) {
+    const int sb = min(kSb, R.ws - s0);
+#pragma unroll
+    for (int k = 0; k < kCols; k++) {
+        if (!((R.cm >> k) & 1)) continue;
+        const bool upd = mine && !((skip >> k) & 1);
+#pragma unroll
+        for (int j = 0; j < kSb; j++) {
+            if (j < sb) {
+                const double uu = rlf64(k < 8 ? o.u0 : o.u1, (k & 7) * 8 + j);
+                if (upd && uu != 0.0) acc[k] = acc[k] - o.l[j] * uu;
+            }
+        }
+    }
+}
 
 __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
-    __shared__ double T[kTile * kTile];
-    __shared__ double A[kPanel * kTile];  // A[q * kTile + i]: L_S(row i, s0 + q)
-    __shared__ double B[kTile * kPanel];  // B[j * kPanel + q]: U_S(s0 + q, column j)
     const int it = item0 + blockIdx.x;
     const int4 g = p.gitems[it];
     const int J = g.x, kind = g.y, r0 = g.z, c0 = g.w;
@@ -58,125 +123,80 @@ __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
     const int nc = kind == 0 ? min(kTile, w - c0) : min(kTile, hh - c0);
     const int ld = kind == 0 ? W : w;
     double* base = p.v + (kind == 0 ? p.oL[J] : p.oU[J]) + (int64_t)c0 * ld + r0;
-    for (int e = threadIdx.x; e < kTile * kTile; e += kT) {
-        const int r = e & (kTile - 1), c = e / kTile;
-        T[e] = (r < nr && c < nc) ? base[(int64_t)c * ld + r] : 0.0;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int cb = wv * kCols;
+    double acc[kCols];
+    // entries summed on their own elsewhere (the ground node's diagonal),
+    // which this kernel neither reads nor writes: another stream folds them
+    uint32_t skip = 0;
+    for (int q = p.tile_hot_ptr[it]; q < p.tile_hot_ptr[it + 1]; q++) {
+        const int e = p.tile_hot[q], hr = e & 255, hc = e >> 8;
+        if (hr == lane && hc >= cb && hc < cb + kCols) skip |= 1u << (hc - cb);
     }
-    __syncthreads();
-    const int64_t base_off = base - p.v;
-    const int64_t rb = p.tile_rec[it], re = p.tile_rec[it + 1];
-    for (int64_t ri = rb; ri < re; ri++) {
-        const int32_t* rc = p.rec + 5 * ri;
-        const int pi = rc[0], t0 = rc[1], t1 = rc[2], u0 = rc[3], u1 = rc[4];
-        const int S = p.csrc[pi], a0 = p.ca0[pi];
-        const int wS = p.j0[S + 1] - p.j0[S], hS = p.rptr[S + 1] - p.rptr[S];
-        const int64_t ldS = wS + hS;
-        const int32_t* rl = p.rel + p.crel_off[pi] - a0;
-        const int32_t* RS = p.rows + p.rptr[S];
-        const double* LS = p.v + p.oL[S] + wS;  // LS[s * ldS + row]: L_S(R_S[row], s)
-        const double* US = p.v + p.oU[S];       // US[col * wS + s]:  U_S(s, R_S[col])
-        const int mt = t1 - t0, mu = u1 - u0, cnt = mt * mu;
-        if (cnt >= kStageMin && wS >= kPanel / 4) {
-            // many entries: the record's L rows and U columns pass through LDS
-            // kPanel terms at a time; each thread keeps its entries across the
-            // chunks, so every entry still takes its terms in order
-            const int nli = kind == 0 ? mt : mu, nuj = kind == 0 ? mu : mt;
-            const int l0 = kind == 0 ? t0 : u0, j0c = kind == 0 ? u0 : t0;
-            for (int s0 = 0; s0 < wS; s0 += kPanel) {
-                const int sb = min(kPanel, wS - s0);
-                for (int e = threadIdx.x; e < kPanel * kTile; e += kT) {
-                    const int i = e % kTile, q = e / kTile;
-                    A[q * kTile + i] = (i < nli && q < sb) ? LS[(int64_t)(s0 + q) * ldS + l0 + i] : 0.0;
-                    const int jj = e / kPanel, qq = e % kPanel;
-                    B[jj * kPanel + qq] = (jj < nuj && qq < sb) ? US[(int64_t)(j0c + jj) * wS + s0 + qq] : 0.0;
-                }
-                __syncthreads();
-                for (int e = threadIdx.x; e < cnt; e += kT) {
-                    const int t = t0 + e % mt, u = u0 + e / mt;
-                    int tr, tc;
-                    if (kind == 0) {
-                        tr = rl[t] - r0; tc = RS[u] - jj0 - c0;
-                    } else {
-                        tr = RS[u] - jj0 - r0; tc = rl[t] - w - c0;
-                    }
-                    bool hot = false;
-                    for (int k = 0; k < p.nhot; k++) hot |= p.hot_slot[k] == base_off + (int64_t)tc * ld + tr;
-                    if (hot) continue;
-                    const int li = kind == 0 ? t - t0 : u - u0, uj = kind == 0 ? u - u0 : t - t0;
-                    double x = T[tc * kTile + tr];
-                    for (int q = 0; q < sb; q++) {
-                        const double uu = B[uj * kPanel + q];
-                        if (uu != 0.0) x -= A[q * kTile + li] * uu;
-                    }
-                    T[tc * kTile + tr] = x;
-                }
-                __syncthreads();
-            }
-            continue;
+#pragma unroll
+    for (int k = 0; k < kCols; k++)
+        acc[k] = (lane < nr && cb + k < nc && !((skip >> k) & 1)) ? base[(int64_t)(cb + k) * ld + lane] : 0.0;
+    const uint64_t below = bits_below(lane), cbelow = bits_below(cb);
+    const int64_t qb = p.qrec_ptr[4 * it + wv], qe = p.qrec_ptr[4 * it + wv + 1];
+    for (int64_t b0 = qb; b0 < qe; b0 += 64) {
+        // fields of up to 64 records, one per lane
+        const int nb = (int)min((int64_t)64, qe - b0);
+        int64_t f_lb = 0, f_ub = 0;
+        uint64_t f_rm = 0, f_cm = 0;
+        int f_ws = 0, f_lds = 0;
+        if (lane < nb) {
+            const rsgpu::SnRec& r = p.frec[p.qrec[b0 + lane]];
+            f_lb = r.lbase;
+            f_ub = r.ubase;
+            f_rm = r.rowmask;
+            f_cm = r.colmask;
+            f_ws = r.ws;
+            f_lds = r.lds;
         }
-        for (int e = threadIdx.x; e < cnt; e += kT) {
-            const int t = t0 + e % mt, u = u0 + e / mt;
-            int tr, tc, lrow, ucol;
-            if (kind == 0) {
-                tr = rl[t] - r0; tc = RS[u] - jj0 - c0; lrow = t; ucol = u;
-            } else {
-                tr = RS[u] - jj0 - r0; tc = rl[t] - w - c0; lrow = u; ucol = t;
+        auto rec = [&](int k) {
+            GRec R;
+            R.lbase = rl64(f_lb, k);
+            R.ubase = rl64(f_ub, k);
+            R.rowmask = (uint64_t)rl64((int64_t)f_rm, k);
+            const uint64_t cmk = (uint64_t)rl64((int64_t)f_cm, k);
+            R.cm = (uint32_t)(cmk >> cb) & 0xFFFFu;
+            R.crank0 = __popcll(cmk & cbelow);
+            R.ws = __builtin_amdgcn_readlane(f_ws, k);
+            R.lds = __builtin_amdgcn_readlane(f_lds, k);
+            return R;
+        };
+        GRec cur = rec(0);
+        GOps oc, on;
+        gather_load(p.v, cur, 0, lane, below, oc);
+        int k = 0, s0 = 0;
+        while (true) {
+            // the next chunk: the rest of this record, else the next record
+            GRec nxt = cur;
+            int nk = k, ns0 = s0 + kSb;
+            const bool more = ns0 < cur.ws;
+            if (!more) {
+                nk = k + 1;
+                ns0 = 0;
+                if (nk < nb) nxt = rec(nk);
             }
-            // entries summed on their own elsewhere (the ground node's diagonal)
-            bool hot = false;
-            for (int k = 0; k < p.nhot; k++) hot |= p.hot_slot[k] == base_off + (int64_t)tc * ld + tr;
-            if (hot) continue;
-            double x = T[tc * kTile + tr];
-            const double* Lr = LS + lrow;
-            const double* Uc = US + (int64_t)ucol * wS;
-            for (int s = 0; s < wS; s++) {
-                const double uu = Uc[s];
-                if (uu != 0.0) x -= Lr[(int64_t)s * ldS] * uu;
-            }
-            T[tc * kTile + tr] = x;
+            const bool have = more || nk < nb;
+            if (have) gather_load(p.v, nxt, ns0, lane, below, on);
+            gather_apply(cur, s0, oc, (cur.rowmask >> lane) & 1, skip, acc);
+            if (!have) break;
+            cur = nxt;
+            k = nk;
+            s0 = ns0;
+            oc = on;
         }
-        __syncthreads();
     }
-    for (int e = threadIdx.x; e < kTile * kTile; e += kT) {
-        const int r = e & (kTile - 1), c = e / kTile;
-        if (r < nr && c < nc) base[(int64_t)c * ld + r] = T[e];
-    }
+#pragma unroll
+    for (int k = 0; k < kCols; k++)
+        if (lane < nr && cb + k < nc && !((skip >> k) & 1)) base[(int64_t)(cb + k) * ld + lane] = acc[k];
 }
 
-// ---- ordered long sums -----------------------------------------------------
-// One workgroup folds a long list of terms into one value in the list's
-// order: waves 1..3 compute the next kChunk terms into LDS while thread 0
-// folds the current chunk (reads eight at a time, the additions serial).
-constexpr int kChunk = 1024;
-
-template <bool kSub, class Term>
-__device__ double fold_long(double x, int64_t q0, int64_t q1, Term term) {
-    __shared__ double buf[2][kChunk];
-    const int tid = threadIdx.x, wave = tid / 64;
-    for (int i = tid; i < kChunk && q0 + i < q1; i += kT) buf[0][i] = term(q0 + i);
-    __syncthreads();
-    int k = 0;
-    for (int64_t b = q0; b < q1; b += kChunk, k++) {
-        const int c = (int)min((int64_t)kChunk, q1 - b), cur = k & 1;
-        if (wave > 0) {
-            const int64_t nb = b + kChunk;
-            for (int i = tid - 64; i < kChunk && nb + i < q1; i += kT - 64) buf[cur ^ 1][i] = term(nb + i);
-        } else if (tid == 0) {
-            const double* B = buf[cur];
-            int i = 0;
-            for (; i + 8 <= c; i += 8) {
-                double t[8];
-#pragma unroll
-                for (int j = 0; j < 8; j++) t[j] = B[i + j];
-#pragma unroll
-                for (int j = 0; j < 8; j++) x = kSub ? x - t[j] : x + t[j];
-            }
-            for (; i < c; i++) x = kSub ? x - B[i] : x + B[i];
-        }
-        __syncthreads();
-    }
-    return x;
-}
+// ---- ordered long sums (fold_long, fold.hpp) ------------------------------
+using rsgpu::fold_long;
 
 // a run of many duplicate CSC entries (the ground diagonal): 0.0 + the run
 __global__ __launch_bounds__(kT) void k_sn_scatter_long(const int32_t* __restrict__ src,
@@ -189,16 +209,29 @@ __global__ __launch_bounds__(kT) void k_sn_scatter_long(const int32_t* __restric
     if (threadIdx.x == 0) v[slot[r]] = x;
 }
 
-// an entry summed on its own: x - L * U over its flat term list, zero
+// an entry summed on its own: x - L * U over a stretch [q0, q1) of its flat
+// term list, continuing from the value the previous stretch left; zero
 // multipliers skipped (a skipped term is +0.0, which leaves x unchanged)
-__global__ __launch_bounds__(kT) void k_sn_hot(SnPtrs p, const int64_t* __restrict__ hot_ptr,
-                                               const int64_t* __restrict__ terms, int e) {
-    const double* v = p.v;
-    const double x = fold_long<true>(v[p.hot_slot[e]], hot_ptr[e], hot_ptr[e + 1], [&](int64_t q) {
+__global__ __launch_bounds__(kT) void k_sn_hot_parts(SnPtrs p, const rsgpu::SnPart* __restrict__ parts,
+                                                     const int64_t* __restrict__ terms) {
+    const rsgpu::SnPart pt = parts[blockIdx.x];
+    double* v = p.v;
+    const int64_t slot = p.hot_slot[pt.idx];
+    const double x = fold_long<true>(v[slot], pt.q0, pt.q1, [&](int64_t q) {
         const double u = v[terms[2 * q + 1]];
         return u != 0.0 ? v[terms[2 * q]] * u : 0.0;
     });
-    if (threadIdx.x == 0) p.v[p.hot_slot[e]] = x;
+    if (threadIdx.x == 0) v[slot] = x;
+}
+
+// a stretch [q0, q1) of a long forward row's terms: y[row] - L(row, j) y[j]
+__global__ __launch_bounds__(kT) void k_sn_fwd_parts(SnPtrs p, const rsgpu::SnPart* __restrict__ parts,
+                                                     const int64_t* __restrict__ lslot,
+                                                     const int32_t* __restrict__ yidx, double* __restrict__ y) {
+    const rsgpu::SnPart pt = parts[blockIdx.x];
+    const double* v = p.v;
+    const double x = fold_long<true>(y[pt.idx], pt.q0, pt.q1, [&](int64_t q) { return v[lslot[q]] * y[yidx[q]]; });
+    if (threadIdx.x == 0) y[pt.idx] = x;
 }
 
 // forward solve of a row with a long list: y[p] - L(p, j) y[j], j ascending
@@ -508,8 +541,16 @@ hipError_t sn_launch_gather(const SnPtrs& p, int item0, int nitems, hipStream_t 
     hipLaunchKernelGGL(k_sn_gather, dim3(nitems), dim3(kT), 0, s, p, item0);
     return hipGetLastError();
 }
-hipError_t sn_launch_hot(const SnPtrs& p, const int64_t* hot_ptr, const int64_t* terms, int e, hipStream_t s) {
-    hipLaunchKernelGGL(k_sn_hot, dim3(1), dim3(kT), 0, s, p, hot_ptr, terms, e);
+hipError_t sn_launch_hot_parts(const SnPtrs& p, const rsgpu::SnPart* parts, int nparts, const int64_t* terms,
+                               hipStream_t s) {
+    if (nparts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sn_hot_parts, dim3(nparts), dim3(kT), 0, s, p, parts, terms);
+    return hipGetLastError();
+}
+hipError_t sn_launch_fwd_parts(const SnPtrs& p, const rsgpu::SnPart* parts, int nparts, const int64_t* lslot,
+                               const int32_t* yidx, double* y, hipStream_t s) {
+    if (nparts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sn_fwd_parts, dim3(nparts), dim3(kT), 0, s, p, parts, lslot, yidx, y);
     return hipGetLastError();
 }
 hipError_t sn_launch_scatter_long(const int32_t* src, const int32_t* cnt, const int64_t* slot, int nruns,

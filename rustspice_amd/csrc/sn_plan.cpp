@@ -4,6 +4,7 @@
 // scatter of A; plus a host execution of the same schedule for checking it
 // against the reference arithmetic on the CPU.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <limits>
 
@@ -315,10 +316,72 @@ bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost&
             for (int32_t s2 = 0; s2 < wS; s2++) {
                 h.hot_terms.push_back(h.oL[S] + (int64_t)s2 * (wS + hS) + wS + t);  // L_S(p, s)
                 h.hot_terms.push_back(h.oU[S] + (int64_t)t * wS + s2);              // U_S(s, p)
+                h.hot_tlev.push_back(level[S]);
             }
         }
     }
     h.hot_ptr.push_back((int64_t)h.hot_terms.size() / 2);
+    // records in the kernel's form, and the hot entries of every tile
+    {
+        const int64_t nrec = h.tile_rec.back();
+        h.frec.resize((size_t)nrec);
+        h.tile_hot_ptr.assign(h.gitems.size() / 4 + 1, 0);
+        // hot slot -> (J, row, column) of LP_J
+        std::vector<std::array<int32_t, 3>> hot_at;
+        for (int64_t sl : h.hot_slot)
+            for (int32_t J = 0; J < ns; J++) {  // few hot entries: a scan is fine
+                const int32_t W = h.j0[J + 1] - h.j0[J] + h.rptr[J + 1] - h.rptr[J];
+                const int64_t d = sl - h.oL[J];
+                if (d >= 0 && d < (int64_t)W * (h.j0[J + 1] - h.j0[J])) {
+                    hot_at.push_back({J, (int32_t)(d % W), (int32_t)(d / W)});
+                    break;
+                }
+            }
+        for (size_t it = 0; it < h.gitems.size() / 4; it++) {
+            const int32_t J = h.gitems[4 * it], kind = h.gitems[4 * it + 1], r0 = h.gitems[4 * it + 2],
+                          c0 = h.gitems[4 * it + 3];
+            const int32_t jj0 = h.j0[J], w = h.j0[J + 1] - jj0;
+            for (int64_t ri = h.tile_rec[it]; ri < h.tile_rec[it + 1]; ri++) {
+                const int32_t* r = h.rec.data() + 5 * ri;
+                const int32_t pi = r[0], t0 = r[1], t1 = r[2], u0 = r[3], u1 = r[4];
+                const int32_t S = h.csrc[pi], a0 = h.ca0[pi];
+                const int32_t wS = h.j0[S + 1] - h.j0[S], hS = h.rptr[S + 1] - h.rptr[S];
+                const int32_t* RS = h.rows.data() + h.rptr[S];
+                const int32_t* rl = h.rel.data() + h.crel_off[pi] - a0;
+                SnRec& q = h.frec[(size_t)ri];
+                q.rowmask = q.colmask = 0;
+                // LP tiles: rows from R_S(t), columns from R_S(u); UP tiles the other way round
+                const int32_t lfirst = kind == 0 ? t0 : u0, ufirst = kind == 0 ? u0 : t0;
+                for (int32_t t = t0; t < t1; t++) {
+                    const int32_t b = kind == 0 ? rl[t] - r0 : rl[t] - w - c0;
+                    (kind == 0 ? q.rowmask : q.colmask) |= 1ull << b;
+                }
+                for (int32_t u = u0; u < u1; u++) {
+                    const int32_t b = kind == 0 ? RS[u] - jj0 - c0 : RS[u] - jj0 - r0;
+                    (kind == 0 ? q.colmask : q.rowmask) |= 1ull << b;
+                }
+                q.lbase = h.oL[S] + wS + lfirst;
+                q.ubase = h.oU[S] + (int64_t)ufirst * wS;
+                q.ws = wS;
+                q.lds = wS + hS;
+            }
+            for (const auto& a : hot_at)
+                if (kind == 0 && a[0] == J && a[1] >= r0 && a[1] < r0 + kSnTile && a[2] >= c0 &&
+                    a[2] < c0 + kSnTile)
+                    h.tile_hot.push_back((a[1] - r0) | ((a[2] - c0) << 8));
+            h.tile_hot_ptr[it + 1] = (int32_t)h.tile_hot.size();
+        }
+        h.qrec_ptr.assign(h.gitems.size() + 1, 0);  // 4 quarters per item
+        for (size_t it = 0; it < h.gitems.size() / 4; it++)
+            for (int32_t qq = 0; qq < 4; qq++) {
+                for (int64_t ri = h.tile_rec[it]; ri < h.tile_rec[it + 1]; ri++)
+                    if ((h.frec[(size_t)ri].colmask >> (16 * qq)) & 0xFFFFull) {
+                        if (ri > std::numeric_limits<int32_t>::max()) return fail(why, "too many gather records");
+                        h.qrec.push_back((int32_t)ri);
+                    }
+                h.qrec_ptr[4 * it + qq + 1] = (int64_t)h.qrec.size();
+            }
+    }
 
     // 7. scatter of A: runs of duplicate rows of each column, in the column
     //    order of the pivot steps, onto the panel slot of (position, step)
@@ -423,36 +486,38 @@ void sn_host_factor(const SnHost& h, const double* ax, std::vector<double>& v, s
         for (int32_t q = 0; q < h.sc_cnt[s]; q++) x += ax[h.sc_src[s] + q];
         v[h.sc_slot[s]] = x;
     }
-    std::vector<char> is_hot;
-    is_hot.assign((size_t)h.nvals, 0);
-    for (int64_t sl : h.hot_slot) is_hot[sl] = 1;
     for (int32_t l = 0; l < h.nlev; l++) {
         // gather: every tile applies its records in order; each record is a
         // block of entries, each entry's terms in ascending order
         for (int32_t it = h.glev_ptr[l]; it < h.glev_ptr[l + 1]; it++) {
             const Panel P(h, h.gitems[4 * it]);
-            const int32_t kind = h.gitems[4 * it + 1];
+            const int32_t kind = h.gitems[4 * it + 1], r0 = h.gitems[4 * it + 2], c0 = h.gitems[4 * it + 3];
+            const int64_t base = kind == 0 ? P.lp(r0, c0) : P.up(r0, c0);
+            const int64_t ld = kind == 0 ? P.w + P.hh : P.w;
             for (int64_t ri = h.tile_rec[it]; ri < h.tile_rec[it + 1]; ri++) {
-                const int32_t* r = h.rec.data() + 5 * ri;
-                const int32_t pi = r[0];
-                const Panel S(h, h.csrc[pi]);
-                const int32_t a0 = h.ca0[pi];
-                const int32_t* rl = h.rel.data() + h.crel_off[pi] - a0;
-                const int32_t* RS = h.rows.data() + h.rptr[S.J];
-                for (int32_t t = r[1]; t < r[2]; t++)
-                    for (int32_t u = r[3]; u < r[4]; u++) {
-                        // LP: row rl[t], column RS[u] - j0, L_S row t, U_S column u
-                        // UP: row RS[u] - j0, column rl[t] - w, L_S row u, U_S column t
-                        const int64_t slot = kind == 0 ? P.lp(rl[t], RS[u] - P.jj0) : P.up(RS[u] - P.jj0, rl[t] - P.w);
-                        if (is_hot[slot]) continue;
-                        const int32_t lrow = kind == 0 ? t : u, ucol = kind == 0 ? u : t;
-                        double x = v[slot];
-                        for (int32_t s2 = 0; s2 < S.w; s2++) {
-                            const double uu = v[S.up(s2, ucol)];
-                            if (uu != 0.0) x -= v[S.lp(S.w + lrow, s2)] * uu;
+                const SnRec& q = h.frec[(size_t)ri];
+                int32_t i = 0;
+                for (int32_t r = 0; r < kSnTile; r++) {
+                    if (!((q.rowmask >> r) & 1)) continue;
+                    int32_t k = 0;
+                    for (int32_t c = 0; c < kSnTile; c++) {
+                        if (!((q.colmask >> c) & 1)) continue;
+                        bool hot = false;
+                        for (int32_t e = h.tile_hot_ptr[it]; e < h.tile_hot_ptr[it + 1]; e++)
+                            hot |= h.tile_hot[e] == (r | (c << 8));
+                        const int64_t slot = base + (int64_t)c * ld + r;
+                        if (!hot) {
+                            double x = v[slot];
+                            for (int32_t s2 = 0; s2 < q.ws; s2++) {
+                                const double uu = v[q.ubase + (int64_t)k * q.ws + s2];
+                                if (uu != 0.0) x -= v[q.lbase + (int64_t)s2 * q.lds + i] * uu;
+                            }
+                            v[slot] = x;
                         }
-                        v[slot] = x;
+                        k++;
                     }
+                    i++;
+                }
             }
         }
         for (size_t e = 0; e < h.hot_slot.size(); e++) {

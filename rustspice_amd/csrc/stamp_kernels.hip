@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_eval.hpp"
+#include "fold.hpp"
 #include "stamp.hpp"
 
 namespace {
@@ -92,8 +93,9 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax
         q0 = rhs_ptr[t];
         q1 = rhs_ptr[t + 1];
     }
+    if (q1 - q0 > rsgpu::kFoldRhs) q1 = q0 - 1;  // a row for k_assemble_long: no work here
     const bool lng = q1 - q0 > kLongRhs;
-    if (t < n && !lng) {
+    if (t < n && !lng && q1 >= q0) {
         double r = 0.0;
         for (int q = q0; q < q1; ++q) r += sv[rhs_src[q]];
         rhs[t] = r;
@@ -126,6 +128,22 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ ax
         }
         if (lane == 0) rhs[t - lane + l] = acc;
     }
+}
+
+// RHS rows with more than kFoldRhs contributions (the ground row: one per
+// capacitor and source), one workgroup each: 0.0 + the contributions in
+// insertion order through the workgroup's ordered fold.
+__global__ __launch_bounds__(256) void k_assemble_long(const int32_t* __restrict__ rows,
+                                                       const int32_t* __restrict__ rhs_ptr,
+                                                       const int32_t* __restrict__ rhs_src,
+                                                       const double* __restrict__ sv,
+                                                       double* __restrict__ rhs,
+                                                       const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int r = rows[blockIdx.x];
+    const double x = rsgpu::fold_long<false>(0.0, rhs_ptr[r], rhs_ptr[r + 1],
+                                             [&](int64_t q) { return sv[rhs_src[q]]; });
+    if (threadIdx.x == 0) rhs[r] = x;
 }
 
 // full = 1: update_transient_state_full (v and i = C (v - v_prev) / dt);
@@ -174,11 +192,14 @@ hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, 
 
 hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
                            const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
-                           hipStream_t s, const int32_t* skip) {
+                           const int32_t* long_rows, int nlong, hipStream_t s, const int32_t* skip) {
     int m = nnz > n ? nnz : n;
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3((m + 255) / 256), dim3(256), 0, s, ax_src, nnz, rhs_ptr,
                        rhs_src, n, sv, ax, rhs, skip);
+    if (nlong > 0)
+        hipLaunchKernelGGL(k_assemble_long, dim3(nlong), dim3(256), 0, s, long_rows, rhs_ptr, rhs_src, sv, rhs,
+                           skip);
     return hipGetLastError();
 }
 
