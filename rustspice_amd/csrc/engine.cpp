@@ -35,6 +35,8 @@ hipError_t launch_mos_eval(const rs_bsim_model* models, const int32_t* mi, const
 hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_ptr,
                            const int32_t* rhs_src, int n, const double* sv, double* ax, double* rhs,
                            const int32_t* long_rows, int nlong, hipStream_t s, const int32_t* skip = nullptr);
+hipError_t launch_assemble_long(const int32_t* rhs_ptr, const int32_t* rhs_src, const double* sv, double* rhs,
+                                const int32_t* long_rows, int nlong, hipStream_t s);
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
                             uint8_t* has_i, hipStream_t s, const int32_t* gate = nullptr);
@@ -84,6 +86,10 @@ struct rs_circuit {
     DevPattern pat[2];  // 0: DC, 1: TRAN (BE and TRAP share the pattern)
     DevBuf<double> d_sv, d_ax, d_rhs, d_x, d_xnew, d_cap_v, d_cap_i;
     DevBuf<uint8_t> d_has_v, d_has_i;
+    // side stream for the long RHS rows of a deferred stamp (default priority)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_sv = nullptr, ev_rhs = nullptr;
+    bool rhs_pending = false;
     std::string err;
     int64_t stamp_calls = 0, newton_iters = 0, factor_calls = 0;
     bool device_newton = true;  // whole DC OP in one launch when equivalent
@@ -202,6 +208,8 @@ int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t
         return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: BSIM devices without a model table");
     if (c->nbsim && (flags & kStampJacobian))
         return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: no exact Jacobian for BSIM devices");
+    const bool defer = (flags & kStampDeferLongRhs) != 0;
+    flags &= ~kStampDeferLongRhs;
     StampArgs a{t, dt, gmin, scale, mode, flags, c->d_models.p};
     int ps = prof_begin(PK_EVAL, s);
     hipError_t e = launch_eval(c->d_dev.p, (int)c->dev.size(), x_dev, (int)xl, a, c->d_cap_v.p,
@@ -216,13 +224,35 @@ int32_t stamp_dev(rs_circuit* c, int mode, const double* x_dev, i64 xl, double t
     }
     if ((rc = get_pattern(c, mode, dp))) return rc;
     ps = prof_begin(PK_ASSEMBLE, s);
+    const bool side = defer && dp->nlong > 0;
     e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p,
-                        (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, dp->long_rows.p, dp->nlong, s);
+                        (int)dp->p.n, c->d_sv.p, c->d_ax.p, c->d_rhs.p, dp->long_rows.p, side ? 0 : dp->nlong, s);
     prof_end(ps, s);
     if (e) return chip(c, e, "k_assemble");
+    if (side) {  // the long rows overlap the factor; rhs_join() before the solve
+        if (!c->side &&
+            ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) ||
+             (e = hipEventCreateWithFlags(&c->ev_sv, hipEventDisableTiming)) ||
+             (e = hipEventCreateWithFlags(&c->ev_rhs, hipEventDisableTiming))))
+            return chip(c, e, "side stream");
+        if ((e = hipEventRecord(c->ev_sv, s)) || (e = hipStreamWaitEvent(c->side, c->ev_sv, 0)) ||
+            (e = launch_assemble_long(dp->rhs_ptr.p, dp->rhs_src.p, c->d_sv.p, c->d_rhs.p, dp->long_rows.p,
+                                      dp->nlong, c->side)) ||
+            (e = hipEventRecord(c->ev_rhs, c->side)))
+            return chip(c, e, "k_assemble_long");
+        c->rhs_pending = true;
+    }
     c->stamp_calls++;
     if (pout) *pout = dp;
     return RS_OK;
+}
+
+// the stream waits for the RHS rows a deferred stamp left on the side stream
+int32_t rhs_join(rs_circuit* c, hipStream_t s) {
+    if (!c->rhs_pending) return RS_OK;
+    c->rhs_pending = false;
+    const hipError_t e = hipStreamWaitEvent(s, c->ev_rhs, 0);
+    return e ? chip(c, e, "rhs join") : RS_OK;
 }
 
 double norm2(const double* v, i64 n) {
@@ -256,13 +286,16 @@ int32_t run_newton(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::
         hipError_t e = hipMemcpyAsync(c->d_x.p, x.data(), x.size() * 8, hipMemcpyHostToDevice, s);
         if (e) return chip(c, e, "hipMemcpy(x)");
         DevPattern* dp;
-        int32_t rc = stamp_dev(c, mode, c->d_x.p, (i64)x.size(), t, dt, gmin, scale, &dp, s, stamp_flags);
+        int32_t rc = stamp_dev(c, mode, c->d_x.p, (i64)x.size(), t, dt, gmin, scale, &dp, s,
+                               stamp_flags | kStampDeferLongRhs);
         if (rc) return rc;
         const i64 n = dp->p.n;
         rs_prepare(sol, n);
         rc = solver_factor_dev(sol, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
                                c->d_ax.p, s);
         c->factor_calls++;
+        const int32_t rj = rhs_join(c, s);  // also on a failed factor: nothing is left behind
+        if (rj) return rj;
         if (rc == RS_OK) rc = solver_solve_dev(sol, c->d_rhs.p, c->d_xnew.p, s);
         if (rc != RS_OK) {
             if (rc == RS_ERR_DEVICE) return from_solver(c, sol, rc);
@@ -834,6 +867,12 @@ rs_circuit* rs_circuit_create(const rs_device* dev, int64_t ndev, int64_t node_c
 
 void rs_circuit_destroy(rs_circuit* c) {
     if (!c) return;
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipEventDestroy(c->ev_sv);
+        (void)hipEventDestroy(c->ev_rhs);
+        (void)hipStreamDestroy(c->side);
+    }
     for (auto& a : c->adj)
         if (a.s) rs_solver_destroy(a.s);
     for (auto*& b : c->dcb)

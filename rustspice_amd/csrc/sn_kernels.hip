@@ -374,13 +374,9 @@ __global__ __launch_bounds__(kT) void k_sn_panel(SnPtrs p, const int32_t* __rest
 }
 
 // the factored diagonal block and its pivots, written after every workgroup
-// of the panel step has read the original block
-__global__ __launch_bounds__(kT) void k_sn_pdiag(SnPtrs p, const int32_t* __restrict__ list, int t0) {
-    __shared__ double D[kPanel * kPanel];
-    const int J = list[blockIdx.x];
-    const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
-    const int W = w + hh, t1 = min(t0 + kPanel, w), nb = t1 - t0;
-    double* LP = p.v + p.oL[J];
+// of the panel step has read the original block (by one extra workgroup of
+// the trailing update, which does not read the diagonal block)
+__device__ void panel_diag_store(double* D, SnPtrs& p, double* LP, int W, int jj0, int t0, int nb) {
     panel_diag(D, LP, W, t0, nb);
     for (int e = threadIdx.x; e < nb * nb; e += kT) {
         const int r = e % nb, c = e / nb;
@@ -401,7 +397,12 @@ __global__ __launch_bounds__(kT) void k_sn_trail(SnPtrs p, const int32_t* __rest
     const int W = w + hh, t1 = min(t0 + kPanel, w), nb = t1 - t0;
     const int nrl = (W - t1 + kTile - 1) / kTile, ncl = (w - t1 + kTile - 1) / kTile;
     const int nru = (w - t1 + kTile - 1) / kTile, ncu = (hh + kTile - 1) / kTile;
-    if ((int)blockIdx.x >= nrl * ncl + nru * ncu) return;  // the launch covers the widest supernode
+    const int ntiles = nrl * ncl + nru * ncu;
+    if ((int)blockIdx.x == ntiles) {  // the extra workgroup: the diagonal block and pivots
+        panel_diag_store(A, p, p.v + p.oL[J], W, jj0, t0, nb);
+        return;
+    }
+    if ((int)blockIdx.x > ntiles) return;  // the launch covers the widest supernode
     int b = blockIdx.x, kind, r0, c0;
     if (b < nrl * ncl) {
         kind = 0; r0 = t1 + (b % nrl) * kTile; c0 = t1 + (b / nrl) * kTile;
@@ -590,8 +591,7 @@ hipError_t sn_launch_big(const SnPtrs& p, const int32_t* list, const int32_t* w,
             nblk = std::max(nblk, (nrow + kRowsPerBlock - 1) / kRowsPerBlock + (nc + kT - 1) / kT);
         }
         hipLaunchKernelGGL(k_sn_panel, dim3(nblk, k), dim3(kT), 0, s, p, list, t0);
-        hipLaunchKernelGGL(k_sn_pdiag, dim3(k), dim3(kT), 0, s, p, list, t0);
-        if (nt > 0) hipLaunchKernelGGL(k_sn_trail, dim3(nt, k), dim3(kT), 0, s, p, list, t0);
+        hipLaunchKernelGGL(k_sn_trail, dim3(nt + 1, k), dim3(kT), 0, s, p, list, t0);  // + the diagonal block
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

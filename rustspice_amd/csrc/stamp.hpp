@@ -65,6 +65,9 @@ i64 stamp_slot_bases(const rs_device* dev, i64 ndev, std::vector<int32_t>& sbase
 // matrix values are the exact partial derivatives of the stamped current
 // (used by the adjoint), not the Newton linearisation of stamp.rs.
 constexpr int32_t kStampJacobian = 1;
+// stamp_dev only: the RHS rows assembled by a workgroup run on the circuit's
+// side stream; rhs_join() makes the stream wait for them before d_rhs is read
+constexpr int32_t kStampDeferLongRhs = 2;
 struct StampArgs {
     double t, dt, gmin, source_scale;
     int32_t mode;

@@ -203,6 +203,14 @@ hipError_t launch_assemble(const int32_t* ax_src, int nnz, const int32_t* rhs_pt
     return hipGetLastError();
 }
 
+hipError_t launch_assemble_long(const int32_t* rhs_ptr, const int32_t* rhs_src, const double* sv, double* rhs,
+                                const int32_t* long_rows, int nlong, hipStream_t s) {
+    if (nlong <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_assemble_long, dim3(nlong), dim3(256), 0, s, long_rows, rhs_ptr, rhs_src, sv, rhs,
+                       (const int32_t*)nullptr);
+    return hipGetLastError();
+}
+
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
                             uint8_t* has_i, hipStream_t s, const int32_t* gate) {
