@@ -320,6 +320,10 @@ int32_t rs_tran_adjoint(rs_circuit *c, rs_solver *s, const rs_newton_opts *o, do
  * afterwards the solver keeps its counts but no factorization. 0: every
  * Newton iteration is driven from the host. */
 void rs_circuit_set_device_newton(rs_circuit *c, int32_t on);
+/* Initial guess of the next rs_tran's DC operating point: x[0..n) over the
+ * zeros engine.rs:340-348 starts from (the netlist's .ic values by node);
+ * n <= node_count, n = 0 clears it. */
+int32_t rs_circuit_set_initial_guess(rs_circuit *c, const double *x, int64_t n);
 /* TransientState per device (stamp.rs:699-718), device-index keyed: each
  * array holds the circuit's ndev entries. */
 int32_t rs_circuit_set_cap_state(rs_circuit *c, const double *cap_v, const double *cap_i,

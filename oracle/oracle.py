@@ -112,6 +112,8 @@ def lib():
         L.orc_pulse_eval.argtypes = [f64p, C.c_double]
         L.orc_dc_op.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(NewtonOpts), vp, f64p,
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(NewtonResult)]
+        L.orc_set_initial_guess.argtypes = [vp, C.c_int64]
+        L.orc_set_initial_guess.restype = None
         L.orc_tran.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(NewtonOpts), C.c_double,
                                C.c_double, C.c_double, C.c_double, vp, C.c_int64, vp, vp,
                                C.c_int64, C.POINTER(TranResult)]
@@ -428,8 +430,10 @@ def dc_op(circ, opts=None, solver=None, last=False):
 
 
 def tran(circ, tstep, tstop, tstart=0.0, tmax=None, opts=None, solver=None, max_points=100000,
-         store=True):
+         store=True, x0=None):
     opts = opts or default_opts()
+    ic = np.ascontiguousarray(np.zeros(0) if x0 is None else x0, np.float64)
+    lib().orc_set_initial_guess(ic.ctypes.data_as(C.c_void_p), len(ic))
     s = solver or NativeKlu(circ.node_count)
     tmax = tstop if tmax is None else tmax
     stride = circ.node_count + circ.ndev + 8
@@ -443,6 +447,7 @@ def tran(circ, tstep, tstop, tstart=0.0, tmax=None, opts=None, solver=None, max_
     out = dict(status=res.status, accepted=res.accepted, rejected=res.rejected,
                newton_iters_total=res.newton_iters_total, factor_calls=res.factor_calls,
                dc_iters=res.dc_iters, npoints=int(res.npoints))
+    lib().orc_set_initial_guess(None, 0)
     if store:
         k = min(int(res.npoints), max_points)
         out["times"] = times[:k].copy()

@@ -157,6 +157,9 @@ typedef struct orc_tran_result {
     int64_t npoints;      /* stored points incl. t=tstart */
 } orc_tran_result;
 
+/* Initial guess of the next orc_tran (x[0..n) over zeros; n = 0 clears). */
+void orc_set_initial_guess(const double *x, int64_t n);
+
 /* TRAN (engine.rs:333-632). times[npoints], sols[npoints*x_stride] */
 int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count,
              const rs_newton_opts *o, double tstep, double tstop, double tstart,

@@ -241,6 +241,21 @@ static void tstate_update_full(const rs_device *dev, int64_t ndev, const double 
     }
 }
 
+/* .ic values applied to x before the initial DC OP (engine.rs:343-348) */
+static double *g_ic = NULL;
+static int64_t g_nic = 0;
+
+void orc_set_initial_guess(const double *x, int64_t n) {
+    free(g_ic);
+    g_ic = NULL;
+    g_nic = 0;
+    if (n > 0) {
+        g_ic = (double *)orc_xcalloc((size_t)n, sizeof(double));
+        memcpy(g_ic, x, (size_t)n * sizeof(double));
+        g_nic = n;
+    }
+}
+
 int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_newton_opts *o,
              double tstep, double tstop, double tstart, double tmax, void *solver,
              int64_t max_points, double *times, double *sols, int64_t x_stride,
@@ -248,6 +263,7 @@ int orc_tran(const rs_device *dev, int64_t ndev, int64_t node_count, const rs_ne
     memset(res, 0, sizeof(*res));
     int64_t xl = node_count;
     double *x = (double *)orc_xcalloc((size_t)(xl ? xl : 1), sizeof(double));
+    for (int64_t q = 0; q < g_nic && q < node_count; q++) x[q] = g_ic[q];
     orc_tstate sbe, str;
     tstate_init(&sbe, ndev, ORC_BE);
     tstate_init(&str, ndev, ORC_TRAP);

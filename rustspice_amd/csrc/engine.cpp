@@ -126,6 +126,7 @@ struct rs_circuit {
     DevBuf<rs_bsim_model> d_models;
     int64_t nmodels = 0;
     int64_t pattern_switches = 0;  // patterns rebuilt because a BSIM group switched
+    std::vector<double> ic;        // rs_tran's initial guess (.ic), empty: zeros
 };
 
 namespace {
@@ -1051,12 +1052,20 @@ int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t pro
     return RS_OK;
 }
 
+int32_t rs_circuit_set_initial_guess(rs_circuit* c, const double* x, int64_t n) {
+    if (n < 0 || n > c->node_count || (n > 0 && !x))
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: initial guess longer than the node count");
+    c->ic.assign(x, x + n);
+    return RS_OK;
+}
+
 int32_t rs_tran(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double tstep, double tstop,
                 double tstart, double tmax, int64_t max_points, double* times, double* sols,
                 int64_t x_stride, rs_tran_result* res) {
     std::memset(res, 0, sizeof(*res));
     const i64 nd = (i64)c->dev.size();
     std::vector<double> x(c->node_count, 0.0);
+    for (size_t q = 0; q < c->ic.size() && (i64)q < c->node_count; q++) x[q] = c->ic[q];
     int32_t rc = reset_cap_state(c);
     if (rc) return rc;
     // BE and TRAP keep separate histories (engine.rs:351-355)
@@ -1085,7 +1094,8 @@ int32_t rs_tran(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double t
     int32_t iters = 0;
     rs_newton_result dcr{};
     int64_t fc0 = c->factor_calls;
-    bool dev_dc = can_run_device_dc(c, sol) && o->gmin_steps == 0 && o->source_steps == 0;
+    // the one-launch DC OP starts from zeros: not with an initial guess
+    bool dev_dc = can_run_device_dc(c, sol) && o->gmin_steps == 0 && o->source_steps == 0 && c->ic.empty();
     if (dev_dc) {  // the TRAN pattern must differ from DC's, else TRAN would refactor on DC pivots
         DevPattern *pd, *pt;
         if ((rc = get_pattern(c, MODE_DC, pd)) || (rc = get_pattern(c, MODE_TRAN_BE, pt))) return rc;

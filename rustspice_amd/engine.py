@@ -73,6 +73,8 @@ def _bind():
     L.rs_circuit_pattern.restype = C.c_int32
     L.rs_circuit_pattern.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      vp, vp]
+    L.rs_circuit_set_initial_guess.restype = C.c_int32
+    L.rs_circuit_set_initial_guess.argtypes = [vp, vp, C.c_int64]
     L.rs_circuit_set_cap_state.restype = C.c_int32
     L.rs_circuit_set_cap_state.argtypes = [vp] * 5
     L.rs_stamp_device.restype = C.c_int32
@@ -201,6 +203,15 @@ class GpuCircuit:
             if a.ndim != 1 or len(a) != self.circuit.ndev:
                 raise ValueError(f"capacitor state arrays need {self.circuit.ndev} entries, got {a.shape}")
         self._check(self._L.rs_circuit_set_cap_state(self.h, *[_ptr(a) for a in arrs]))
+
+    def set_initial_guess(self, x=None):
+        """The next tran()'s starting point for its DC operating point (the
+        netlist's .ic values by node); None clears it."""
+        if x is None:
+            self._check(self._L.rs_circuit_set_initial_guess(self.h, None, 0))
+            return
+        self._ic = np.ascontiguousarray(x, np.float64)
+        self._check(self._L.rs_circuit_set_initial_guess(self.h, _ptr(self._ic), len(self._ic)))
 
     def stamp(self, x=None, mode=MODE_DC, t=0.0, dt=0.0, gmin=1e-12, source_scale=1.0):
         """One MNA build: returns (ap, ai, ax, rhs, n) like the engine closure.
