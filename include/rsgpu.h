@@ -450,6 +450,36 @@ int32_t rs_supernodal_info(const rs_solver *h, int64_t *out);
  * RSGPU_POISON=1 sets it for the whole process. */
 void rs_set_poison(int32_t on);
 
+/* ---- AC small-signal sweep (Engine::run_ac_result, engine.rs:829-970) ------
+ * The reference solves one frequency after another with ComplexDenseSolver
+ * (complex_solver.rs:50-136); here every frequency of the sweep is one
+ * workgroup of one launch (chunked so the dense matrices stay within 512 MiB).
+ * rs_ac_frequencies: generate_frequency_sweep (engine.rs:1002-1044), sweep 0
+ * dec, 1 oct, 2 lin; returns the count (-1: unknown sweep), writes up to cap.
+ * rs_ac_create: the devices the AC stamps read (stamp.rs:1525-1747: R, C, V,
+ * I, diode, MOSFET; a V or I source whose value does not parse is still
+ * stamped here), node_count as in rs_circuit_create, the BSIM model table
+ * (sa and sb are taken as 0, as evaluate_mos passes them), and each device's
+ * AC magnitude and phase in degrees (0 for none; read for V and I only).
+ * rs_ac_run: the sweep at the DC solution x_dc (nx entries); x_out[f][i]
+ * (cap_n complex entries per frequency, (re, im) pairs) and magph[f][i] =
+ * (20 log10|v| or -600 below 1e-30, arg v in degrees) for the node_count
+ * nodes (engine.rs:932-946); *n_out = the complex MNA size; *nsolved = the
+ * frequencies before the first whose solve fails (the reference stops there
+ * and reports Failed). rs_ac_timing: out[5] = {ms of the last sweep's solve
+ * launches, launches, n, dense entries, contributions}. */
+typedef struct rs_ac rs_ac;
+int64_t rs_ac_frequencies(int32_t sweep, int64_t points, double fstart, double fstop, double *out,
+                          int64_t cap);
+rs_ac *rs_ac_create(const rs_device *dev, int64_t ndev, int64_t node_count,
+                    const rs_bsim_model *models, int64_t nmodels, const double *ac_mag,
+                    const double *ac_phase);
+int32_t rs_ac_run(rs_ac *h, const double *x_dc, int64_t nx, const double *freqs, int64_t nfreq,
+                  double *x_out, double *magph, int64_t cap_n, int64_t *n_out, int64_t *nsolved);
+int32_t rs_ac_timing(const rs_ac *h, double *out);
+int32_t rs_ac_last_error(const rs_ac *h, char *buf, int64_t len);
+void rs_ac_destroy(rs_ac *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,11 @@ def lib():
         L.orc_dc_op.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(NewtonOpts), vp, f64p,
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(NewtonResult)]
         L.orc_set_initial_guess.argtypes = [vp, C.c_int64]
+        L.orc_ac_frequencies.restype = C.c_int64
+        L.orc_ac_frequencies.argtypes = [C.c_int, C.c_int64, C.c_double, C.c_double, vp, C.c_int64]
+        L.orc_ac.restype = C.c_int64
+        L.orc_ac.argtypes = [vp, C.c_int64, C.c_int64, vp, vp, vp, C.c_int64, vp, C.c_int64, vp, vp,
+                             C.c_int64, C.POINTER(C.c_int64)]
         L.orc_set_initial_guess.restype = None
         L.orc_tran.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(NewtonOpts), C.c_double,
                                C.c_double, C.c_double, C.c_double, vp, C.c_int64, vp, vp,
@@ -539,3 +544,39 @@ def lte_milne(xb, xt, abs_tol=1e-6, rel_tol=1e-3):
     lib().orc_lte_milne(xb, xt, min(len(xb), len(xt)), abs_tol, rel_tol, out)
     return dict(error_norm=out[0], max_node=int(out[1]), accept=bool(out[2]),
                 suggested_factor=out[3], raw_lte=out[4])
+
+
+AC_SWEEPS = {"dec": 0, "oct": 1, "lin": 2}
+
+
+def ac_frequencies(sweep, points, fstart, fstop):
+    """generate_frequency_sweep (engine.rs:1002-1044)."""
+    k = AC_SWEEPS[sweep]
+    n = lib().orc_ac_frequencies(k, points, fstart, fstop, None, 0)
+    out = np.zeros(max(n, 1))
+    lib().orc_ac_frequencies(k, points, fstart, fstop, out.ctypes.data_as(C.c_void_p), n)
+    return out[:n].copy()
+
+
+def ac(circ, ac_mag, ac_phase, x_dc, freqs):
+    """The AC sweep of engine.rs:864-949 at the DC solution x_dc. Returns
+    (x[nsolved, n] complex, magph[nsolved, node_count, 2] (dB, degrees), n)."""
+    if circ.models is not None and len(circ.models):
+        set_bsim_models(circ.models)
+    freqs = np.ascontiguousarray(freqs, np.float64)
+    ndev = circ.ndev
+    nv = int(np.count_nonzero(circ.devices["kind"] == 3)) if ndev else 0
+    cap_n = circ.node_count + nv
+    nf = len(freqs)
+    xo = np.zeros(max(nf * cap_n * 2, 1))
+    mp = np.zeros(max(nf * circ.node_count * 2, 1))
+    nout = C.c_int64()
+    am = np.ascontiguousarray(ac_mag, np.float64)
+    ph = np.ascontiguousarray(ac_phase, np.float64)
+    xd = np.ascontiguousarray(x_dc, np.float64)
+    k = lib().orc_ac(_devptr(circ), ndev, circ.node_count, am.ctypes.data_as(C.c_void_p),
+                     ph.ctypes.data_as(C.c_void_p), xd.ctypes.data_as(C.c_void_p), len(xd),
+                     freqs.ctypes.data_as(C.c_void_p), nf, xo.ctypes.data_as(C.c_void_p),
+                     mp.ctypes.data_as(C.c_void_p), cap_n, C.byref(nout))
+    x = xo[: nf * cap_n * 2].reshape(nf, cap_n, 2)[:k]
+    return x[..., 0] + 1j * x[..., 1], mp[: nf * circ.node_count * 2].reshape(nf, circ.node_count, 2)[:k], nout.value

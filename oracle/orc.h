@@ -157,6 +157,14 @@ typedef struct orc_tran_result {
     int64_t npoints;      /* stored points incl. t=tstart */
 } orc_tran_result;
 
+/* ---- AC analysis (engine.rs:829-1044, stamp.rs:1525-1747, complex_mna.rs,
+ * complex_solver.rs) ------------------------------------------------------ */
+int64_t orc_ac_frequencies(int sweep, int64_t points, double fstart, double fstop, double *out,
+                           int64_t cap);
+int64_t orc_ac(const rs_device *dev, int64_t ndev, int64_t node_count, const double *ac_mag,
+               const double *ac_phase, const double *x_dc, int64_t nx, const double *freqs,
+               int64_t nfreq, double *xout, double *magph, int64_t cap_n, int64_t *n_out);
+
 /* Initial guess of the next orc_tran (x[0..n) over zeros; n = 0 clears). */
 void orc_set_initial_guess(const double *x, int64_t n);
 

@@ -53,6 +53,7 @@ EXPORTS = [
     "rs_solver_batch_create", "rs_solver_batch_destroy", "rs_solver_batch_last_error",
     "rs_solver_batch_layout", "rs_solver_batch_refactor", "rs_solver_batch_solve",
     "rs_solver_batch_factor_values",
+    "rs_ac_frequencies", "rs_ac_create", "rs_ac_run", "rs_ac_timing", "rs_ac_last_error", "rs_ac_destroy",
 ]
 
 

@@ -1205,6 +1205,9 @@ class Lowered:
     source_index: dict  # V/I instance name -> record index
     ac_sources: dict  # record index -> (ac_mag, ac_phase) of V/I sources
     record_of: dict  # instance index -> record index (instances that stamp)
+    ac_circuit: NL.Circuit = None  # the records the AC stamps read (stamp.rs:1525-1747)
+    ac_mag: np.ndarray = None  # per AC record: ac_mag of a V/I source (0 when none)
+    ac_phase: np.ndarray = None  # per AC record: ac_phase in degrees (0 when none)
 
 
 def lower(circ: SpiceCircuit) -> Lowered:
@@ -1213,10 +1216,13 @@ def lower(circ: SpiceCircuit) -> Lowered:
     node count: stamp.rs:242-349) stamps nothing; the engine ignores the error
     (engine.rs:262-269), so the device is listed in ``skipped``."""
     b = NL._Builder()
+    ba = NL._Builder()  # AC: a V/I source stamps whether or not its value parses
+    amag, aph = [], []
     skipped, src_index, ac, rec_of = [], {}, {}, {}
     models: list = []
     model_key: dict = {}
     seen_named = set()
+    ac_v_names: list = []
 
     def one(inst):
         k = inst.kind
@@ -1231,9 +1237,21 @@ def lower(circ: SpiceCircuit) -> Lowered:
                 skipped.append((inst.name, "needs 2 nodes"))
                 return
             v = parse_number_with_suffix(inst.value) if inst.value is not None else None
+            if k in "VI":
+                if k == "V" and any(nm == inst.name for nm in ac_v_names):
+                    raise UnsupportedDevice(f"{inst.name}: a repeated V name shares its aux row in the reference")
+                if k == "V":
+                    ac_v_names.append(inst.name)
+                ba.add(NL.RS_DEV_V if k == "V" else NL.RS_DEV_I, nd, 0.0 if v is None else v)
+                amag.append(0.0 if inst.ac_mag is None else inst.ac_mag)
+                aph.append(0.0 if inst.ac_phase is None else inst.ac_phase)
             if v is None:
                 skipped.append((inst.name, "no numeric value (waveforms are not passed on by the parser)"))
                 return
+            if k in "RC":
+                ba.add(NL.RS_DEV_R if k == "R" else NL.RS_DEV_C, nd, v)
+                amag.append(0.0)
+                aph.append(0.0)
             if k in "CV":
                 # state (capacitor history) and aux rows are keyed by name in the reference
                 if (k, inst.name) in seen_named:
@@ -1252,6 +1270,9 @@ def lower(circ: SpiceCircuit) -> Lowered:
             isat = _param_value(inst.params, ["is"])
             em = _param_value(inst.params, ["n", "nj"])
             b.add(NL.RS_DEV_D, nd, 0.0, [1e-14 if isat is None else isat, 1.0 if em is None else em])
+            ba.rows.append(b.rows[-1])
+            amag.append(0.0)
+            aph.append(0.0)
         elif k == "M":
             if len(nd) < 4:
                 skipped.append((inst.name, "needs 4 nodes"))
@@ -1268,6 +1289,9 @@ def lower(circ: SpiceCircuit) -> Lowered:
             if level == 1:
                 b.add(NL.RS_DEV_M, nd[:4], 0.0,
                       [1.0, float(rec["vth0"]), float(rec["u0"]), float(rec["tox"]), w, ln], flags)
+                ba.rows.append(b.rows[-1])
+                amag.append(0.0)
+                aph.append(0.0)
             else:
                 temp = _param_value(inst.params, ["temp"])
                 sa = _param_value(inst.params, ["sa"])
@@ -1282,6 +1306,9 @@ def lower(circ: SpiceCircuit) -> Lowered:
                     models.append(rec)
                 b.add(NL.RS_DEV_M, nd[:4], 0.0,
                       NL.bsim_params(level, idx, w, ln, 300.15 if temp is None else temp), flags)
+                ba.rows.append(b.rows[-1])
+                amag.append(0.0)
+                aph.append(0.0)
         else:
             skipped.append((inst.name, f"unknown device kind {k}"))
 
@@ -1291,7 +1318,9 @@ def lower(circ: SpiceCircuit) -> Lowered:
         if len(b.rows) > before:
             rec_of[ii] = before
     c = b.build(circ.node_count, circ.title or "netlist", models if models else None)
-    return Lowered(c, list(circ.node_names), skipped, src_index, ac, rec_of)
+    ca = ba.build(circ.node_count, (circ.title or "netlist") + " (ac)", models if models else None)
+    return Lowered(c, list(circ.node_names), skipped, src_index, ac, rec_of, ca,
+                   np.array(amag, dtype=np.float64), np.array(aph, dtype=np.float64))
 
 
 #: the solver kinds of rustspice_amd.solver

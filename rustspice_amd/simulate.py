@@ -93,13 +93,17 @@ class Simulator:
                              max_points=max_points)
         finally:
             self.gc.set_initial_guess(None)
+        if d["npoints"] == 0:  # the initial DC operating point failed (engine.rs:431-447)
+            return R.RunResult("Tran", "Failed", 0, self.node_names,
+                               message="DC operating point failed to converge")
         status = "Converged" if d["status"] == 0 else "Failed"
         n_tr = self._tran_n()
         rows = [list(s[: (self._dc_n() if i == 0 else n_tr)]) for i, s in enumerate(d.get("sols", []))]
-        return R.RunResult("Tran", status, int(d["newton_iters_total"]), self.node_names,
+        # RunResult.iterations of a TRAN is its accepted step count (engine.rs:618)
+        return R.RunResult("Tran", status, int(d["accepted"]), self.node_names,
                            solution=rows[-1] if rows else [], tran_times=list(d.get("times", [])),
                            tran_solutions=rows,
-                           message=None if status == "Converged" else "transient analysis failed")
+                           message=f"Adaptive: {d['accepted']} accepted, {d['rejected']} rejected")
 
     def ac(self, sweep: str, points: int, fstart: float, fstop: float) -> R.RunResult:
         from . import ac as A
