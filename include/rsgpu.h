@@ -277,9 +277,21 @@ int32_t rs_dc_adjoint(rs_circuit *c, const double *x, int64_t x_len, int64_t pro
  * to the host-driven loop. Used when the circuit has no BSIM devices, no
  * gmin or source stepping is asked for and the solver is NativeKLU or
  * SparseLU; otherwise the host-driven loop runs. 0 (default): host-driven.
- * rs_circuit_graph_steps: steps run through graphs so far. */
+ * rs_circuit_graph_steps: steps run through graphs so far.
+ * The same switch runs rs_tran's adaptive loop (engine.rs:463-602) on the
+ * device after its first accepted step: graphs of 16 Newton-iteration slots
+ * with the step controller on the device (Milne LTE, PI controller and its
+ * emergency reductions, breakpoints and settling, BE then TRAP solve per
+ * attempt, the accepted points into a ring the host drains once per launch);
+ * the controller's powers are evaluated in double-double, and the ~4 % of
+ * them within 0.02 ulp of a rounding midpoint are handed to the host's libm
+ * (the graph stops until the host writes them back). Results are
+ * bit-identical to the host-driven loop. rs_circuit_tran_graph_info: out[3]
+ * = {rs_tran runs that used the device controller, graph launches, powers
+ * evaluated by the host}. */
 void rs_circuit_set_tran_graph(rs_circuit *c, int32_t on);
 int64_t rs_circuit_graph_steps(const rs_circuit *c);
+int32_t rs_circuit_tran_graph_info(const rs_circuit *c, int64_t *out);
 /* Transient sensitivities by the discrete adjoint (no reference
  * counterpart). Runs rs_tran_fixed's transient (DC OP, then nsteps TRAP
  * steps of dt, the accepted-step capacitor history of stamp.rs:615-668

@@ -101,6 +101,13 @@ int32_t rs_host_stamp_pattern(const rs_device *dev, int64_t ndev, int64_t node_c
 int32_t rs_host_sn_selftest(int64_t n, const int64_t *ap, const int64_t *ai, const double *ax,
                             const double *rhs, double *x, double *lx, double *ux, double *ud,
                             int64_t hot_min, int64_t *stats);
+/* The adaptive transient controller's power x^y (double-double, rounded
+ * once; ddpow.hpp) evaluated on the host for n pairs, with near[i] = 1 where
+ * the exact power lies within zone ulp of a rounding midpoint (where the
+ * device hands the power to the host's libm). No reference counterpart: the
+ * checking aid for f64::powf's role in analysis.rs:173, 530-536. */
+int32_t rs_host_pow_dd(const double *x, const double *y, int64_t n, double zone, double *out,
+                       uint8_t *near);
 
 #ifdef __cplusplus
 }

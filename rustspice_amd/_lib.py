@@ -40,7 +40,7 @@ EXPORTS = [
     "rs_solve_device_noalloc", "rs_noalloc_max_n", "rs_error_pivot", "rs_tsolve", "rs_tsolve_device",
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
-    "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest",
+    "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest", "rs_host_pow_dd",
     "rs_host_bbd_decompose", "rs_set_bbd_blocks", "rs_bbd_info",
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
     "rs_circuit_set_cap_state", "rs_circuit_set_initial_guess", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
@@ -53,7 +53,7 @@ EXPORTS = [
     "rs_solver_batch_create", "rs_solver_batch_destroy", "rs_solver_batch_last_error",
     "rs_solver_batch_layout", "rs_solver_batch_refactor", "rs_solver_batch_solve",
     "rs_solver_batch_factor_values",
-    "rs_ac_frequencies", "rs_ac_create", "rs_ac_run", "rs_ac_timing", "rs_ac_last_error", "rs_ac_destroy",
+    "rs_circuit_tran_graph_info", "rs_ac_frequencies", "rs_ac_create", "rs_ac_run", "rs_ac_timing", "rs_ac_last_error", "rs_ac_destroy",
 ]
 
 

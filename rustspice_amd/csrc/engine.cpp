@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/rsgpu.h"
@@ -24,6 +25,7 @@
 #include "stamp.hpp"
 #include "fold.hpp"
 #include "profile.hpp"
+#include "tran_ctl.hpp"
 
 namespace rsgpu {
 hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, StampArgs a,
@@ -39,7 +41,8 @@ hipError_t launch_assemble_long(const int32_t* rhs_ptr, const int32_t* rhs_src, 
                                 const int32_t* long_rows, int nlong, hipStream_t s);
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
-                            uint8_t* has_i, hipStream_t s, const int32_t* gate = nullptr);
+                            uint8_t* has_i, hipStream_t s, const int32_t* gate = nullptr,
+                            const double* dt_dev = nullptr);
 hipError_t launch_newton_begin(double* t_dev, double dt, int32_t* st, double damping, hipStream_t s);
 hipError_t launch_newton_update(int n, double* x, const double* xn, double abs_tol, double rel_tol,
                                 double damping_min, int max_iters, int32_t* st, hipStream_t s);
@@ -115,6 +118,7 @@ struct rs_circuit {
     // fixed-step transient with each TRAP step as a captured hipGraph
     bool use_graph = false;
     int64_t graph_steps = 0, graph_launches = 0;
+    int64_t tran_graph_runs = 0, tran_host_pows = 0;  // adaptive TRAN on the device controller
     DevBuf<double> d_e, d_t, d_lam, d_dp, d_r, d_del, d_nrm;
     int32_t adj_refine = 0;  // refinement steps of the last adjoint solve (largest over a TRAN)
     // BSIM devices: slot bases, model table, group masks of the last evaluation
@@ -825,6 +829,165 @@ int32_t tran_fixed_graph(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o,
     return RS_OK;
 }
 
+// Loop state of rs_tran handed to and back from the device controller.
+struct TranLoop {
+    double t, dt, prev_err;
+    int32_t has_prev, ctl_consec, consec, settle, status;
+    int32_t accepted, rejected, iters;
+    int64_t np;
+};
+
+// The rest of rs_tran's adaptive loop (engine.rs:463-602) as captured graphs
+// of kTranSlots Newton-iteration slots with the step controller on the device
+// (tran_ctl.hpp). Starts from an accepted step: x holds the TRAN solution, the
+// capacitor histories are current (TRAP in the circuit's arrays, BE in be_*)
+// and the solver holds the TRAN pattern's factor. The host reads the state
+// block and the new points once per graph launch, and evaluates the
+// controller's powers with libm when the device finds them within 0.02 ulp of
+// a rounding midpoint. *ran = false (nothing changed) when the solver or the
+// circuit cannot run this way.
+int32_t tran_adaptive_graph(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, std::vector<double>& x,
+                            TranLoop& L, const std::vector<double>& bps, double tstop, double min_dt,
+                            double max_dt, double* be_v, double* be_i, uint8_t* be_hv, uint8_t* be_hi,
+                            const std::function<void(double, const double*)>& store, bool* ran) {
+    *ran = false;
+    DevPattern* dp;
+    int32_t rc;
+    if (c->nbsim || o->gmin_steps != 0 || o->source_steps != 0 || !solver_graph_ready(sol)) return RS_OK;
+    if ((rc = get_pattern(c, MODE_TRAN_TRAP, dp))) return rc;
+    const i64 n = dp->p.n;
+    if ((i64)x.size() != n || sol->n != n || sol->last_ap != dp->p.ap || sol->last_ai != dp->p.ai)
+        return RS_OK;
+    hipError_t e;
+    DevBuf<TranCtl> d_ctl;
+    DevBuf<double> d_xa, d_xbe, d_bp, d_rt, d_rx;
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    TranCtl h{};
+    h.t = L.t;
+    h.dt = L.dt;
+    h.prev_err = L.prev_err;
+    h.has_prev = L.has_prev;
+    h.ctl_consec = L.ctl_consec;
+    h.consec = L.consec;
+    h.settle = L.settle;
+    h.accepted = L.accepted;
+    h.rejected = L.rejected;
+    h.npoints = L.np;
+    h.nw_done = 1;
+    h.phase = L.t < tstop ? kPhaseBegin : kPhaseFinished;
+    h.mode = MODE_TRAN_BE;
+    if ((e = d_ctl.ensure(1)) || (e = d_xa.ensure(n)) || (e = d_xbe.ensure(n)) ||
+        (e = d_bp.ensure(std::max<size_t>(bps.size(), 1))) || (e = d_rt.ensure(kTranRing)) ||
+        (e = d_rx.ensure((size_t)kTranRing * n)) || (e = c->d_x.ensure(std::max<size_t>(c->d_x.n, n))) ||
+        (e = hipStreamCreate(&s)) ||
+        (e = hipMemcpy(d_ctl.p, &h, sizeof h, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(d_xa.p, x.data(), n * 8, hipMemcpyHostToDevice)) ||
+        (!bps.empty() && (e = hipMemcpy(d_bp.p, bps.data(), bps.size() * 8, hipMemcpyHostToDevice)))) {
+        cleanup();
+        return chip(c, e, "adaptive graph setup");
+    }
+    TranParams P{tstop, min_dt, max_dt, o->abs_tol, o->rel_tol, o->damping, d_bp.p, (int32_t)bps.size(), (int32_t)n};
+    TranCtl* ctl = d_ctl.p;
+    int32_t* st = reinterpret_cast<int32_t*>(ctl);  // the Newton state block leads the struct
+    StampArgs a{0.0, 0.0, o->gmin, 1.0, MODE_TRAN_TRAP, 0, c->d_models.p, &ctl->t_target, st};
+    a.mode_dev = &ctl->mode;
+    a.dt_dev = &ctl->dt_step;
+    a.be_cap_v = be_v;
+    a.be_cap_i = be_i;
+    a.be_has_v = be_hv;
+    a.be_has_i = be_hi;
+    if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) {
+        cleanup();
+        return chip(c, e, "hipStreamBeginCapture");
+    }
+    const int nd = (int)c->dev.size();
+    for (int m = 0; m < kTranSlots && !e; m++) {
+        if ((e = launch_tctl_pre(ctl, P, s)) || (e = launch_gated_copy(c->d_x.p, d_xa.p, n, &ctl->cp_begin, s)) ||
+            (e = launch_eval(c->d_dev.p, nd, c->d_x.p, (int)n, a, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p,
+                             c->d_has_i.p, c->d_sv.p, s, nullptr, c->d_gmask.p)) ||
+            (e = launch_assemble(dp->ax_src.p, (int)dp->p.ai.size(), dp->rhs_ptr.p, dp->rhs_src.p, (int)n,
+                                 c->d_sv.p, c->d_ax.p, c->d_rhs.p, dp->long_rows.p, dp->nlong, s, st)) ||
+            (e = solver_graph_iteration(sol, c->d_ax.p, c->d_rhs.p, c->d_xnew.p, s, st)) ||
+            (e = launch_newton_update((int)n, c->d_x.p, c->d_xnew.p, o->abs_tol, o->rel_tol, o->damping_min,
+                                      o->max_iters, st, s)) ||
+            (e = launch_tctl_post(ctl, P, d_xbe.p, c->d_x.p, s)) ||
+            (e = launch_gated_copy(d_xbe.p, c->d_x.p, n, &ctl->cp_be, s)) ||
+            (e = launch_gated_copy(c->d_x.p, d_xa.p, n, &ctl->cp_begin, s)) ||
+            (e = launch_tstore(ctl, c->d_x.p, d_xa.p, d_rt.p, d_rx.p, n, s)) ||
+            (e = launch_cap_state(c->d_dev.p, nd, d_xa.p, (int)n, 0.0, 1, c->d_cap_v.p, c->d_cap_i.p,
+                                  c->d_has_v.p, c->d_has_i.p, s, &ctl->accept_now, &ctl->dt_step)) ||
+            (e = launch_cap_state(c->d_dev.p, nd, d_xa.p, (int)n, 0.0, 0, be_v, be_i, be_hv, be_hi, s,
+                                  &ctl->accept_now, &ctl->dt_step)))
+            break;
+    }
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e || (e = e2) || (e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0))) {
+        cleanup();
+        return chip(c, e, "adaptive graph capture");
+    }
+    *ran = true;
+    std::vector<double> rx((size_t)n);
+    int64_t np_host = L.np;
+    const int64_t fc0 = c->factor_calls;
+    int32_t it0 = 0;
+    for (;;) {
+        if ((e = hipGraphLaunch(exec, s)) || (e = hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, s)) ||
+            (e = hipStreamSynchronize(s)))
+            break;
+        c->graph_launches++;
+        for (; np_host < h.npoints; np_host++) {  // new points, oldest first
+            const int64_t slot = np_host % kTranRing;
+            double tt;
+            if ((e = hipMemcpy(&tt, d_rt.p + slot, 8, hipMemcpyDeviceToHost)) ||
+                (e = hipMemcpy(rx.data(), d_rx.p + slot * n, n * 8, hipMemcpyDeviceToHost)))
+                break;
+            store(tt, rx.data());
+        }
+        if (e) break;
+        if (h.phase == kPhaseHostPow) {  // libm's pow where the device's power is near a midpoint
+            const double ex[3] = {0.7 / 2.0, 0.4 / 2.0, 0.5};
+            for (int j = 0; j < 3; j++)
+                if (h.pw_need & (1 << j)) h.pw_out[j] = std::pow(h.pw_in[j], ex[j]);
+            h.phase = kPhaseResume;
+            c->tran_host_pows++;
+            if ((e = hipMemcpy(&ctl->pw_out[0], h.pw_out, sizeof h.pw_out, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(&ctl->phase, &h.phase, 4, hipMemcpyHostToDevice)))
+                break;
+            continue;
+        }
+        if (h.phase == kPhaseFinished) break;
+    }
+    if (!e) e = hipMemcpy(x.data(), d_xa.p, n * 8, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e) return chip(c, e, "adaptive graph step");
+    (void)it0;
+    const int32_t its = h.iters_total;
+    c->newton_iters += its;
+    c->stamp_calls += its;
+    sol->refactor_count += its;
+    c->factor_calls = fc0 + its;
+    L.t = h.t;
+    L.dt = h.dt;
+    L.prev_err = h.prev_err;
+    L.has_prev = h.has_prev;
+    L.ctl_consec = h.ctl_consec;
+    L.consec = h.consec;
+    L.settle = h.settle;
+    L.status = h.status;
+    L.accepted = h.accepted;
+    L.rejected = h.rejected;
+    L.iters += its;
+    L.np = h.npoints;
+    return RS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1120,12 +1283,42 @@ int32_t rs_tran(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double t
         np++;
     };
     store(tstart);
+    auto store_ptr = [&](double tt, const double* xp) {
+        if (np < max_points) {
+            if (times) times[np] = tt;
+            if (sols)
+                for (i64 q = 0; q < x_stride; q++) sols[np * x_stride + q] = q < (i64)x.size() ? xp[q] : 0.0;
+        }
+        np++;
+    };
     // update_transient_state into both histories (engine.rs:448-449)
     if ((rc = cap_update(c, x, 0.0, 0, bv.p, bi.p, bhv.p, bhi.p)) ||
         (rc = cap_update(c, x, 0.0, 0, c->d_cap_v.p, c->d_cap_i.p, c->d_has_v.p, c->d_has_i.p)))
         return rc;
     int status = 0;
+    bool graph_tried = false;
     while (t < tstop) {
+        // after the first accepted step (the solver then holds the TRAN
+        // pattern's factor), the rest of the loop on the device controller
+        if (c->use_graph && !graph_tried && res->accepted > 0) {
+            graph_tried = true;
+            TranLoop L{t, dt, prev_err, has_prev ? 1 : 0, ctl_consec, consec, bp.settle, 0,
+                       res->accepted, res->rejected, 0, np};
+            bool ran = false;
+            rc = tran_adaptive_graph(c, sol, o, x, L, bp.t, tstop, min_dt, max_dt, bv.p, bi.p, bhv.p, bhi.p,
+                                     store_ptr, &ran);
+            if (rc) return rc;
+            if (ran) {
+                c->tran_graph_runs++;
+                t = L.t;
+                dt = L.dt;
+                status = L.status;
+                iters += L.iters;
+                res->accepted = L.accepted;
+                res->rejected = L.rejected;
+                break;
+            }
+        }
         dt = bp.limit_dt(t, dt, min_dt);
         if (bp.settle > 0) dt = std::max(dt * bp.factor, min_dt);
         if (t + dt > tstop) dt = tstop - t;
@@ -1370,6 +1563,13 @@ int32_t rs_tran_steps(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, do
 void rs_circuit_set_tran_graph(rs_circuit* c, int32_t on) { c->use_graph = on != 0; }
 
 int64_t rs_circuit_graph_steps(const rs_circuit* c) { return c->graph_steps; }
+
+int32_t rs_circuit_tran_graph_info(const rs_circuit* c, int64_t* out) {
+    out[0] = c->tran_graph_runs;
+    out[1] = c->graph_launches;
+    out[2] = c->tran_host_pows;
+    return RS_OK;
+}
 
 int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, double dt,
                         int64_t nsteps, int64_t probe, int64_t every, double* dp_out,

@@ -11,6 +11,7 @@
 #include "../../include/rsgpu.h"
 #include "../../include/rsgpu_host.h"
 #include "bbd.hpp"
+#include "ddpow.hpp"
 #include "plan.hpp"
 #include "sn.hpp"
 #include "sp.hpp"
@@ -424,6 +425,15 @@ int64_t rs_host_bbd_decompose(int64_t n, const int64_t* ap, const int64_t* ai, i
     std::memcpy(block_ptr, d.block_ptr.data(), d.block_ptr.size() * sizeof(int64_t));
     if (border_start) *border_start = d.border_start;
     return d.num_blocks;
+}
+
+int32_t rs_host_pow_dd(const double* x, const double* y, int64_t n, double zone, double* out, uint8_t* near) {
+    for (int64_t i = 0; i < n; i++) {
+        bool nr = false;
+        out[i] = rsgpu::pow_dd(x[i], y[i], &nr, zone);
+        if (near) near[i] = nr ? 1 : 0;
+    }
+    return RS_OK;
 }
 
 }  // extern "C"

@@ -75,6 +75,16 @@ struct StampArgs {
     const rs_bsim_model* models = nullptr;  // BSIM model table (device memory) or NULL
     const double* t_dev = nullptr;          // time read from device memory instead of t
     const int32_t* skip = nullptr;          // evaluation skipped while *skip != 0
+    // adaptive transient graphs: mode and dt read from device memory; in BE
+    // mode the capacitors read the BE history arrays below instead of the
+    // kernel's own (the reference keeps separate BE and TRAP states,
+    // engine.rs:351-355)
+    const int32_t* mode_dev = nullptr;
+    const double* dt_dev = nullptr;
+    const double* be_cap_v = nullptr;
+    const double* be_cap_i = nullptr;
+    const uint8_t* be_has_v = nullptr;
+    const uint8_t* be_has_i = nullptr;
 };
 
 }  // namespace rsgpu

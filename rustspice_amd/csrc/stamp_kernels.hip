@@ -34,6 +34,16 @@ __global__ __launch_bounds__(256) void k_eval(const rs_device* __restrict__ dev,
     if (d >= ndev) return;
     if (a.skip && *a.skip) return;
     if (a.t_dev) a.t = *a.t_dev;
+    if (a.dt_dev) a.dt = *a.dt_dev;
+    if (a.mode_dev) {
+        a.mode = *a.mode_dev;
+        if (a.mode == rsgpu::MODE_TRAN_BE && a.be_cap_v) {
+            cap_v = a.be_cap_v;
+            cap_i = a.be_cap_i;
+            has_v = a.be_has_v;
+            has_i = a.be_has_i;
+        }
+    }
     const rs_device v = dev[d];
     if (sbase && rsgpu::is_bsim(v)) {
         gmask[d] = rsgpu::eval_bsim(v, x, xl, a, sv + sbase[d]);
@@ -154,10 +164,12 @@ __global__ __launch_bounds__(256) void k_cap_state(const rs_device* __restrict__
                                                    double* __restrict__ cap_i,
                                                    uint8_t* __restrict__ has_v,
                                                    uint8_t* __restrict__ has_i,
-                                                   const int32_t* __restrict__ gate) {
+                                                   const int32_t* __restrict__ gate,
+                                                   const double* __restrict__ dt_dev) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndev || dev[d].kind != RS_DEV_C) return;
     if (gate && !*gate) return;
+    if (dt_dev) dt = *dt_dev;
     const double v = xget(x, xl, dev[d].node[0]) - xget(x, xl, dev[d].node[1]);
     if (full) {
         const double vp = has_v[d] ? cap_v[d] : v;
@@ -213,10 +225,11 @@ hipError_t launch_assemble_long(const int32_t* rhs_ptr, const int32_t* rhs_src, 
 
 hipError_t launch_cap_state(const rs_device* dev, int ndev, const double* x, int xl, double dt,
                             int full, double* cap_v, double* cap_i, uint8_t* has_v,
-                            uint8_t* has_i, hipStream_t s, const int32_t* gate) {
+                            uint8_t* has_i, hipStream_t s, const int32_t* gate,
+                            const double* dt_dev) {
     if (ndev <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cap_state, dim3((ndev + 255) / 256), dim3(256), 0, s, dev, ndev, x, xl,
-                       dt, full, cap_v, cap_i, has_v, has_i, gate);
+                       dt, full, cap_v, cap_i, has_v, has_i, gate, dt_dev);
     return hipGetLastError();
 }
 

@@ -112,6 +112,8 @@ def _bind():
     L.rs_circuit_set_tran_graph.argtypes = [vp, C.c_int32]
     L.rs_circuit_graph_steps.restype = C.c_int64
     L.rs_circuit_graph_steps.argtypes = [vp]
+    L.rs_circuit_tran_graph_info.restype = C.c_int32
+    L.rs_circuit_tran_graph_info.argtypes = [vp, vp]
     L.rs_tran_adjoint.restype = C.c_int32
     L.rs_tran_adjoint.argtypes = [vp, vp, C.POINTER(NewtonOpts), C.c_double, C.c_int64, C.c_int64,
                                   C.c_int64, vp, vp, C.c_int64, C.POINTER(C.c_int64),
@@ -303,6 +305,13 @@ class GpuCircuit:
 
     def graph_steps(self) -> int:
         return int(self._L.rs_circuit_graph_steps(self.h))
+
+    def tran_graph_info(self) -> dict:
+        """Adaptive rs_tran runs on the device controller, graph launches,
+        controller powers evaluated by the host's libm."""
+        out = np.zeros(3, np.int64)
+        self._L.rs_circuit_tran_graph_info(self.h, _ptr(out))
+        return dict(runs=int(out[0]), launches=int(out[1]), host_pows=int(out[2]))
 
     def tran_adjoint(self, solver: GpuSolver, dt, nsteps, probe: int, every: int = 25, opts=None):
         """rs_tran_adjoint: d x_N[probe] / dp for every device after the
