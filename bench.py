@@ -45,6 +45,14 @@ def _same(a, b):
     return (a != a and b != b) or a == b
 
 
+def _same_arr(a, b):
+    """Equal arrays of doubles, NaN == NaN."""
+    import numpy as np
+
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and bool(np.array_equal(a, b, equal_nan=True))
+
+
 def _algorithmic_bytes(nnz_a, nnz_l, nnz_u, n):
     """Compulsory traffic per launch (FP64 values 8 B, int32 indices 4 B)."""
     refactor = 12 * nnz_a + 12 * (nnz_l + nnz_u) + 8 * n
@@ -568,6 +576,76 @@ def tran_graph_cost(E, NL, S):
     return out
 
 
+def tran_device_cost(E, NL, S):
+    """The reference's adaptive TRAN (rs_tran) host-driven against its loop on
+    the device controller after the first accepted step (bit-identical
+    results), on a PULSE-driven RC ladder."""
+    b = NL._Builder()
+    b.add(NL.RS_DEV_V, (1, 0), 0.0, NL.pulse_params(0.0, 1.0, 1e-10, 5e-11, 5e-11, 3e-10, 1e-9),
+          flags=NL.RS_F_WAVEFORM)
+    for i in range(30):
+        b.add(NL.RS_DEV_R, (1 + i, 2 + i), 1e3)
+        b.add(NL.RS_DEV_C, (2 + i, 0), 1e-13)
+    c = b.build(32, "pulse_ladder30")
+    args = dict(tstep=1e-11, tstop=3e-9, tmax=5e-11)
+    row = dict(workload="30-section RC ladder, PULSE input, .tran 10p 3n max 50p (NativeKLU semantics)")
+    res = {}
+    for dev in (False, True):
+        best = None
+        for _ in range(3):
+            g = E.GpuCircuit(c)
+            g.set_tran_graph(dev)
+            s = S.GpuSolver(c.node_count)
+            t0 = time.perf_counter()
+            r = g.tran(s, **args)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        res[dev] = r
+        row["device_ms" if dev else "host_ms"] = 1e3 * best
+        if dev:
+            row["device_info"] = g.tran_graph_info()
+    row["accepted"], row["rejected"] = res[True]["accepted"], res[True]["rejected"]
+    row["newton_iters"] = res[True]["newton_iters_total"]
+    row["same_result"] = _same_arr(res[True]["sols"], res[False]["sols"]) and \
+        _same_arr(res[True]["times"], res[False]["times"])
+    row["speedup"] = row["host_ms"] / row["device_ms"]
+    return row
+
+
+def ac_sweep_cost(NL, np, nodes=200, per_decade=100):
+    """An AC sweep (engine.rs:829-970) of a seeded 200-node R/C/V/I/diode/
+    CMOS network over 1 Hz .. 10 GHz, all frequencies in one launch, against
+    the oracle on one host core (a sample of frequencies, scaled)."""
+    from oracle import oracle as O
+    from rustspice_amd import ac as A
+
+    circ, mag, ph = NL.ac_network(2024, nodes=nodes, with_nonlinear=True)
+    x = np.random.RandomState(7).uniform(0.0, 1.0, circ.node_count + 1)
+    freqs = A.frequencies("dec", per_decade, 1.0, 1e10)
+    g = A.GpuAc(circ, mag, ph)
+    g.run(x, freqs)  # warm-up (allocates the scratch of this sweep)
+    t0 = time.perf_counter()
+    xg, mg, n, k = g.run(x, freqs)
+    wall = time.perf_counter() - t0
+    t = g.timing()
+    sample = freqs[:: max(1, len(freqs) // 16)]
+    t1 = time.perf_counter()
+    xo, mo, _ = O.ac(circ, mag, ph, x, sample)
+    cpu_s = (time.perf_counter() - t1) * len(freqs) / len(sample)
+    idx = np.searchsorted(freqs, sample)
+    flops = 8.0 * n ** 3 / 3.0 * len(freqs)
+    return dict(workload=f"{nodes}-node R/C/V/I/diode/CMOS network, .ac dec {per_decade} 1 10g "
+                         f"({len(freqs)} frequencies), complex MNA n={n}",
+                frequencies=int(len(freqs)), solved=int(k), n=int(n),
+                gpu_ms=1e3 * wall, kernel_ms=t["ms"], launches=t["launches"],
+                frequencies_per_s=len(freqs) / wall, kernel_gflops=flops / (t["ms"] * 1e-3) / 1e9,
+                cpu_baseline=dict(frequencies_per_s=len(freqs) / cpu_s, cores=1, kind="port",
+                                  sample=f"{len(sample)} of the frequencies through oracle/orc_ac.c, scaled"),
+                speedup_vs_one_core=cpu_s / wall,
+                sample_bit_identical=_same_arr(xg[idx].real, xo.real) and _same_arr(xg[idx].imag, xo.imag)
+                and _same_arr(mg[idx], mo))
+
+
 def cpu_baseline_mesh(mesh, np, reps=10):
     """The oracle's refactor + solve of the same mesh matrix on one host core
     and the column-parallel refactor (native_klu.rs:402-576) on all host
@@ -769,6 +847,8 @@ def main():
             line["adjoint"] = adjoint_cost(E, NL, S, torch)
             line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
             line["tran_graph"] = tran_graph_cost(E, NL, S)
+            line["tran_device_controller"] = tran_device_cost(E, NL, S)
+            line["ac_sweep"] = ac_sweep_cost(NL, np)
             line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
             line["kernel_rooflines"] = kernel_rooflines(E, NL, S, torch, np)
             if not args.no_cpu:

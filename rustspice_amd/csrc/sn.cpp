@@ -30,9 +30,9 @@ hipError_t sn_launch_big(const SnPtrs& p, const int32_t* list, const int32_t* w,
                          hipStream_t s);
 hipError_t sn_launch_perm(const int32_t* perm, const double* in, double* out, int n, bool to_pos, hipStream_t s);
 hipError_t sn_launch_fwd(const SnPtrs& p, const int32_t* rows, int nrows, const int32_t* inner, int ninner,
-                         double* y, hipStream_t s);
+                         int maxw, double* y, hipStream_t s);
 hipError_t sn_launch_bwd(const SnPtrs& p, const int32_t* rows, const int32_t* sn_of, int nrows,
-                         const int32_t* inner, int ninner, double* y, hipStream_t s);
+                         const int32_t* inner, int ninner, int maxw, double* y, hipStream_t s);
 
 // Runs of duplicates and forward-solve rows with more terms than these are
 // folded by a workgroup each (ordered long sums).
@@ -63,6 +63,7 @@ struct SnImpl {
     // per level: rows [lr[l], lr[l+1]) of lev_rows; small-factor list
     // [fs[l], fs[l+1]) and inner list [in[l], in[l+1]) of lists; wide supernodes
     std::vector<int32_t> lr, fs, in;
+    std::vector<int32_t> inw;  // per level: the widest supernode of its inner list
     std::vector<std::vector<int32_t>> big, big_w, big_h;  // per level, widest first
     std::vector<int32_t> big_off;                        // their lists' offsets in `lists`
     int64_t bytes = 0;
@@ -122,6 +123,7 @@ SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::str
     p->lr.assign(1, 0);
     p->fs.assign(1, 0);
     p->in.assign(1, 0);
+    p->inw.assign(h.nlev, 0);
     p->big.resize(h.nlev);
     for (int32_t l = 0; l < h.nlev; l++) {
         for (int32_t q = h.lev_ptr[l]; q < h.lev_ptr[l + 1]; q++) {
@@ -129,7 +131,10 @@ SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::str
             for (int32_t c = h.j0[J]; c < h.j0[J + 1]; c++) lev_rows.push_back(c);
             if (w >= kSnBigW) p->big[l].push_back(J);
             else lists_small.push_back(J);
-            if (w > 1) lists_inner.push_back(J);
+            if (w > 1) {
+                lists_inner.push_back(J);
+                p->inw[l] = std::max(p->inw[l], w);
+            }
         }
         p->lr.push_back((int32_t)lev_rows.size());
         p->fs.push_back((int32_t)lists_small.size());
@@ -369,7 +374,7 @@ hipError_t sn_solve(SnImpl* p, const double* rhs, double* x, hipStream_t s, doub
         if ((e = sn_launch_fwd_long(p->ptr, p->long_rows.p + p->fl[l], p->fl[l + 1] - p->fl[l],
                                     p->long_tptr.p + p->fl[l], p->lslot.p, p->yidx.p, y, s)) ||
             (e = sn_launch_fwd(p->ptr, p->fwd_rows.p + p->fr[l], p->fr[l + 1] - p->fr[l], p->lists.p + p->in[l],
-                               p->in[l + 1] - p->in[l], y, s)))
+                               p->in[l + 1] - p->in[l], p->inw[l], y, s)))
             return e;
         if (st) {
             const int np = p->spp[l + 1] - p->spp[l];
@@ -382,7 +387,7 @@ hipError_t sn_solve(SnImpl* p, const double* rhs, double* x, hipStream_t s, doub
     }
     for (int32_t l = h.nlev - 1; l >= 0; l--)
         if ((e = sn_launch_bwd(p->ptr, p->lev_rows.p + p->lr[l], p->sn_of.p, p->lr[l + 1] - p->lr[l],
-                               p->lists.p + p->in[l], p->in[l + 1] - p->in[l], y, s)))
+                               p->lists.p + p->in[l], p->in[l + 1] - p->in[l], p->inw[l], y, s)))
             return e;
     return sn_launch_perm(p->perm.p, y, x, h.n, false, s);
 }

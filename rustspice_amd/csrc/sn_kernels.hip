@@ -57,6 +57,7 @@ __global__ __launch_bounds__(kT) void k_sn_scatter(const int32_t* __restrict__ s
 // value per lane for 8 columns x 8 terms.
 constexpr int kCols = kTile / 4;  // columns per wavefront
 constexpr int kSb = 8;            // terms per chunk
+constexpr int kDepth = 3;         // chunks in flight per wavefront
 
 __device__ __forceinline__ uint64_t bits_below(int b) { return b == 0 ? 0ull : (~0ull >> (64 - b)); }
 
@@ -166,28 +167,50 @@ __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
             R.lds = __builtin_amdgcn_readlane(f_lds, k);
             return R;
         };
-        GRec cur = rec(0);
-        GOps oc, on;
-        gather_load(p.v, cur, 0, lane, below, oc);
-        int k = 0, s0 = 0;
-        while (true) {
-            // the next chunk: the rest of this record, else the next record
-            GRec nxt = cur;
-            int nk = k, ns0 = s0 + kSb;
-            const bool more = ns0 < cur.ws;
-            if (!more) {
-                nk = k + 1;
-                ns0 = 0;
-                if (nk < nb) nxt = rec(nk);
+        // a ring of kDepth chunks in flight: slot i holds chunk (kDepth * round + i);
+        // a chunk is applied, then its slot is refilled with the chunk kDepth
+        // ahead, so the loads of the next kDepth - 1 chunks overlap each apply
+        int lk = 0, ls0 = 0;  // the last chunk issued: record lk, terms ls0 ..
+        GRec lr = rec(0);
+        auto advance = [&]() -> bool {
+            if (lk >= nb) return false;
+            ls0 += kSb;
+            if (ls0 < lr.ws) return true;
+            lk++;
+            ls0 = 0;
+            if (lk >= nb) return false;
+            lr = rec(lk);
+            return true;
+        };
+        GRec rq[kDepth];
+        GOps oq[kDepth];
+        int sq[kDepth];
+        bool vq[kDepth];
+#pragma unroll
+        for (int i = 0; i < kDepth; i++) {
+            vq[i] = i == 0 ? true : advance();
+            if (vq[i]) {
+                rq[i] = lr;
+                sq[i] = ls0;
+                gather_load(p.v, rq[i], sq[i], lane, below, oq[i]);
             }
-            const bool have = more || nk < nb;
-            if (have) gather_load(p.v, nxt, ns0, lane, below, on);
-            gather_apply(cur, s0, oc, (cur.rowmask >> lane) & 1, skip, acc);
-            if (!have) break;
-            cur = nxt;
-            k = nk;
-            s0 = ns0;
-            oc = on;
+        }
+        bool go = true;
+        while (go) {
+#pragma unroll
+            for (int i = 0; i < kDepth; i++) {
+                if (!vq[i]) {
+                    go = false;
+                    break;
+                }
+                gather_apply(rq[i], sq[i], oq[i], (rq[i].rowmask >> lane) & 1, skip, acc);
+                vq[i] = advance();
+                if (vq[i]) {
+                    rq[i] = lr;
+                    sq[i] = ls0;
+                    gather_load(p.v, rq[i], sq[i], lane, below, oq[i]);
+                }
+            }
         }
     }
 #pragma unroll
@@ -422,18 +445,27 @@ __global__ __launch_bounds__(kT) void k_sn_trail(SnPtrs p, const int32_t* __rest
         B[e] = (c < nc && tt < nb) ? C[(int64_t)(c0 + c) * ldc + t0 + tt] : 0.0;
     }
     __syncthreads();
+    // lane r holds row r of the wavefront's 16 columns in registers; the
+    // panel's terms t go in order, each applied to all 16 entries (16
+    // independent chains, one LDS read of L per term)
     const int r = threadIdx.x & (kTile - 1), cg = threadIdx.x / kTile;
     if (r >= nr) return;
-    for (int m = 0; m < kTile / 4; m++) {
-        const int c = cg * (kTile / 4) + m;
-        if (c >= nc) break;
-        double x = C[(int64_t)(c0 + c) * ldc + r0 + r];
-        for (int t = 0; t < nb; t++) {
-            const double u = B[c * kPanel + t];
-            if (u != 0.0) x -= A[t * kTile + r] * u;
+    constexpr int kC = kTile / 4;
+    const int cb = cg * kC;
+    double x[kC];
+#pragma unroll
+    for (int m = 0; m < kC; m++) x[m] = cb + m < nc ? C[(int64_t)(c0 + cb + m) * ldc + r0 + r] : 0.0;
+    for (int t = 0; t < nb; t++) {
+        const double a = A[t * kTile + r];
+#pragma unroll
+        for (int m = 0; m < kC; m++) {
+            const double u = B[(cb + m) * kPanel + t];  // zero beyond nc: skipped
+            if (u != 0.0) x[m] -= a * u;
         }
-        C[(int64_t)(c0 + c) * ldc + r0 + r] = x;
     }
+#pragma unroll
+    for (int m = 0; m < kC; m++)
+        if (cb + m < nc) C[(int64_t)(c0 + cb + m) * ldc + r0 + r] = x[m];
 }
 
 // ---- solve ----------------------------------------------------------------
@@ -461,7 +493,18 @@ __global__ __launch_bounds__(kT) void k_sn_fwd_gather(SnPtrs p, const int32_t* _
         const int jS = p.j0[S], wS = p.j0[S + 1] - jS, hS = p.rptr[S + 1] - p.rptr[S];
         const double* L = p.v + p.oL[S] + wS + t;
         const int64_t ld = wS + hS;
-        for (int s = 0; s < wS; s++) x -= L[(int64_t)s * ld] * y[jS + s];
+        int s = 0;
+        for (; s + 8 <= wS; s += 8) {  // the operands of 8 terms in flight, then the ordered subtractions
+            double l[8], yv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                l[k] = L[(int64_t)(s + k) * ld];
+                yv[k] = y[jS + s + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) x -= l[k] * yv[k];
+        }
+        for (; s < wS; s++) x -= L[(int64_t)s * ld] * y[jS + s];
     }
     y[pos] = x;
 }
@@ -492,7 +535,21 @@ __global__ __launch_bounds__(kT) void k_sn_bwd_gather(SnPtrs p, const int32_t* _
     const double* U = p.v + p.oU[J] + (pos - jj0);
     const int32_t* R = p.rows + p.rptr[J];
     double x = y[pos];
-    for (int c = hh - 1; c >= 0; c--) x -= U[(int64_t)c * w] * y[R[c]];
+    int c = hh - 1;
+    for (; c >= 7; c -= 8) {  // the operands of 8 terms in flight, then the ordered subtractions
+        int idx[8];
+        double u[8], yv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            idx[k] = R[c - k];
+            u[k] = U[(int64_t)(c - k) * w];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) yv[k] = y[idx[k]];
+#pragma unroll
+        for (int k = 0; k < 8; k++) x -= u[k] * yv[k];
+    }
+    for (; c >= 0; c--) x -= U[(int64_t)c * w] * y[R[c]];
     if (w == 1) x /= p.ud[pos];
     y[pos] = x;
 }
@@ -517,6 +574,78 @@ __global__ __launch_bounds__(kT) void k_sn_bwd_inner(SnPtrs p, const int32_t* __
         for (int r = threadIdx.x; r < t; r += kT) y[jj0 + r] -= LP[(int64_t)t * W + r] * xt;
         __syncthreads();
     }
+}
+
+// The same two sweeps with the supernode's y in LDS and kIB columns per
+// barrier pair: wavefront 0 finishes the block's own rows (lane r - t0 holds
+// row r, the finished value of a row passed to the others by shuffle), then
+// every thread applies the block's columns to its rows outside the block.
+// Each row still takes its terms one column at a time in the order above.
+constexpr int kIB = 16;
+
+__global__ __launch_bounds__(kT) void k_sn_fwd_inner_lds(SnPtrs p, const int32_t* __restrict__ list,
+                                                         double* __restrict__ y) {
+    extern __shared__ double ys[];
+    const int tid = threadIdx.x;
+    const int J = list[blockIdx.x];
+    const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
+    const int W = w + hh;
+    const double* LP = p.v + p.oL[J];
+    for (int r = tid; r < w; r += kT) ys[r] = y[jj0 + r];
+    __syncthreads();
+    for (int t0 = 0; t0 < w; t0 += kIB) {
+        const int t1 = min(t0 + kIB, w);
+        if (tid < 64) {
+            const int r = t0 + tid;
+            double yr = r < t1 ? ys[r] : 0.0;
+            for (int t = t0; t < t1 - 1; t++) {
+                const double yt = __shfl(yr, t - t0);
+                if (r > t && r < t1) yr -= LP[(int64_t)t * W + r] * yt;
+            }
+            if (r < t1) ys[r] = yr;
+        }
+        __syncthreads();
+        for (int r = t1 + tid; r < w; r += kT) {
+            double yr = ys[r];
+            for (int t = t0; t < t1; t++) yr -= LP[(int64_t)t * W + r] * ys[t];
+            ys[r] = yr;
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < w; r += kT) y[jj0 + r] = ys[r];
+}
+
+__global__ __launch_bounds__(kT) void k_sn_bwd_inner_lds(SnPtrs p, const int32_t* __restrict__ list,
+                                                         double* __restrict__ y) {
+    extern __shared__ double ys[];
+    const int tid = threadIdx.x;
+    const int J = list[blockIdx.x];
+    const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
+    const int W = w + hh;
+    const double* LP = p.v + p.oL[J];
+    for (int r = tid; r < w; r += kT) ys[r] = y[jj0 + r];
+    __syncthreads();
+    for (int t1 = w; t1 > 0; t1 -= kIB) {
+        const int t0 = max(t1 - kIB, 0);
+        if (tid < 64) {
+            const int r = t0 + tid;
+            double yr = r < t1 ? ys[r] : 0.0;
+            for (int t = t1 - 1; t >= t0; t--) {
+                if (r == t) yr = yr / p.ud[jj0 + t];
+                const double xt = __shfl(yr, t - t0);
+                if (r < t) yr -= LP[(int64_t)t * W + r] * xt;
+            }
+            if (r < t1) ys[r] = yr;
+        }
+        __syncthreads();
+        for (int r = tid; r < t0; r += kT) {
+            double yr = ys[r];
+            for (int t = t1 - 1; t >= t0; t--) yr -= LP[(int64_t)t * W + r] * ys[t];
+            ys[r] = yr;
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < w; r += kT) y[jj0 + r] = ys[r];
 }
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
@@ -605,17 +734,32 @@ hipError_t sn_launch_perm(const int32_t* perm, const double* in, double* out, in
         hipLaunchKernelGGL(k_sn_perm_out, dim3(blocks_for(n)), dim3(kT), 0, s, perm, in, out, n);
     return hipGetLastError();
 }
+// inner sweeps keep the supernode's y in LDS up to this width
+constexpr int kInnerLdsMax = 8192;
+
 hipError_t sn_launch_fwd(const SnPtrs& p, const int32_t* rows, int nrows, const int32_t* inner, int ninner,
-                         double* y, hipStream_t s) {
+                         int maxw, double* y, hipStream_t s) {
     if (nrows > 0) hipLaunchKernelGGL(k_sn_fwd_gather, dim3(blocks_for(nrows)), dim3(kT), 0, s, p, rows, nrows, y);
-    if (ninner > 0) hipLaunchKernelGGL(k_sn_fwd_inner, dim3(ninner), dim3(kT), 0, s, p, inner, y);
+    if (ninner > 0) {
+        if (maxw <= kInnerLdsMax)
+            hipLaunchKernelGGL(k_sn_fwd_inner_lds, dim3(ninner), dim3(kT), (size_t)maxw * sizeof(double), s, p,
+                               inner, y);
+        else
+            hipLaunchKernelGGL(k_sn_fwd_inner, dim3(ninner), dim3(kT), 0, s, p, inner, y);
+    }
     return hipGetLastError();
 }
 hipError_t sn_launch_bwd(const SnPtrs& p, const int32_t* rows, const int32_t* sn_of, int nrows,
-                         const int32_t* inner, int ninner, double* y, hipStream_t s) {
+                         const int32_t* inner, int ninner, int maxw, double* y, hipStream_t s) {
     if (nrows > 0)
         hipLaunchKernelGGL(k_sn_bwd_gather, dim3(blocks_for(nrows)), dim3(kT), 0, s, p, rows, sn_of, nrows, y);
-    if (ninner > 0) hipLaunchKernelGGL(k_sn_bwd_inner, dim3(ninner), dim3(kT), 0, s, p, inner, y);
+    if (ninner > 0) {
+        if (maxw <= kInnerLdsMax)
+            hipLaunchKernelGGL(k_sn_bwd_inner_lds, dim3(ninner), dim3(kT), (size_t)maxw * sizeof(double), s, p,
+                               inner, y);
+        else
+            hipLaunchKernelGGL(k_sn_bwd_inner, dim3(ninner), dim3(kT), 0, s, p, inner, y);
+    }
     return hipGetLastError();
 }
 

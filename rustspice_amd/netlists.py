@@ -555,3 +555,41 @@ def diode_circuit(v: float = 1.0, r: float = 1e3) -> Circuit:
     b.add(RS_DEV_R, (1, 2), r)
     b.add(RS_DEV_D, (2, 0), 0.0, [1e-14, 1.0])
     return b.build(3, "diode")
+
+
+# --------------------------------------------------------------------------
+# AC sweeps
+# --------------------------------------------------------------------------
+def ac_network(seed: int, nodes: int = 12, with_nonlinear: bool = True):
+    """A seeded small-signal test network: ground, nodes 1..nodes; a 1 V
+    source with AC 1 at node 1, a resistor chain plus random R and C, an AC
+    current source, and (optionally) a diode and a level-1 CMOS stage.
+    Returns (circuit, ac_mag, ac_phase), one magnitude and phase per device."""
+    rng = np.random.RandomState(seed)
+    b = _Builder()
+    mag, ph = [], []
+
+    def add(kind, nodes_, value, p=None, flags=0, m=0.0, phase=0.0):
+        b.add(kind, nodes_, value, p, flags)
+        mag.append(m)
+        ph.append(phase)
+
+    add(RS_DEV_V, (1, 0), 1.0, m=1.0, phase=0.0)
+    for i in range(1, nodes):
+        add(RS_DEV_R, (i, i + 1), float(10 ** rng.uniform(2, 4)))
+    for _ in range(nodes):
+        a, c = rng.randint(0, nodes + 1, 2)
+        if a != c:
+            add(RS_DEV_R, (int(a), int(c)), float(10 ** rng.uniform(2, 5)))
+    for i in range(2, nodes + 1):
+        add(RS_DEV_C, (i, 0), float(10 ** rng.uniform(-13, -10)))
+        add(RS_DEV_R, (i, 0), 1e6)
+    for _ in range(3):
+        a, c = rng.randint(1, nodes + 1, 2)
+        add(RS_DEV_C, (int(a), int(c)), float(10 ** rng.uniform(-13, -11)))
+    add(RS_DEV_I, (0, nodes), 1e-4, m=float(rng.uniform(0.1, 1.0)), phase=float(rng.uniform(-90, 90)))
+    if with_nonlinear:
+        add(RS_DEV_D, (nodes // 2, 0), 0.0, [1e-14, 1.0])
+        add(RS_DEV_M, (nodes - 1, 2, 0, 0), 0.0, mos_params(0.4, 500.0, 1e-6, 1e-7))
+        add(RS_DEV_M, (nodes - 1, 2, 1, 1), 0.0, mos_params(-0.4, 250.0, 2e-6, 1e-7), flags=RS_F_PMOS)
+    return b.build(nodes + 1, f"ac{seed}"), np.array(mag), np.array(ph)
