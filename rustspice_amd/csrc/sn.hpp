@@ -124,7 +124,7 @@ struct SnHost {
 
 // Supernodes at least this wide are factored by the blocked multi-workgroup
 // path (panel of kSnPanel columns, U block row, trailing update tiles).
-constexpr int32_t kSnBigW = 48;
+constexpr int32_t kSnBigW = 16;
 constexpr int32_t kSnPanel = 32;
 
 // Device view of a plan (plain pointers, passed by value to the kernels).

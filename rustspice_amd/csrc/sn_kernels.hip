@@ -57,7 +57,6 @@ __global__ __launch_bounds__(kT) void k_sn_scatter(const int32_t* __restrict__ s
 // value per lane for 8 columns x 8 terms.
 constexpr int kCols = kTile / 4;  // columns per wavefront
 constexpr int kSb = 8;            // terms per chunk
-constexpr int kDepth = 3;         // chunks in flight per wavefront
 
 __device__ __forceinline__ uint64_t bits_below(int b) { return b == 0 ? 0ull : (~0ull >> (64 - b)); }
 
@@ -167,50 +166,28 @@ __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
             R.lds = __builtin_amdgcn_readlane(f_lds, k);
             return R;
         };
-        // a ring of kDepth chunks in flight: slot i holds chunk (kDepth * round + i);
-        // a chunk is applied, then its slot is refilled with the chunk kDepth
-        // ahead, so the loads of the next kDepth - 1 chunks overlap each apply
-        int lk = 0, ls0 = 0;  // the last chunk issued: record lk, terms ls0 ..
-        GRec lr = rec(0);
-        auto advance = [&]() -> bool {
-            if (lk >= nb) return false;
-            ls0 += kSb;
-            if (ls0 < lr.ws) return true;
-            lk++;
-            ls0 = 0;
-            if (lk >= nb) return false;
-            lr = rec(lk);
-            return true;
-        };
-        GRec rq[kDepth];
-        GOps oq[kDepth];
-        int sq[kDepth];
-        bool vq[kDepth];
-#pragma unroll
-        for (int i = 0; i < kDepth; i++) {
-            vq[i] = i == 0 ? true : advance();
-            if (vq[i]) {
-                rq[i] = lr;
-                sq[i] = ls0;
-                gather_load(p.v, rq[i], sq[i], lane, below, oq[i]);
+        GRec cur = rec(0);
+        GOps oc, on;
+        gather_load(p.v, cur, 0, lane, below, oc);
+        int k = 0, s0 = 0;
+        while (true) {
+            // the next chunk: the rest of this record, else the next record
+            GRec nxt = cur;
+            int nk = k, ns0 = s0 + kSb;
+            const bool more = ns0 < cur.ws;
+            if (!more) {
+                nk = k + 1;
+                ns0 = 0;
+                if (nk < nb) nxt = rec(nk);
             }
-        }
-        bool go = true;
-        while (go) {
-#pragma unroll
-            for (int i = 0; i < kDepth; i++) {
-                if (!vq[i]) {
-                    go = false;
-                    break;
-                }
-                gather_apply(rq[i], sq[i], oq[i], (rq[i].rowmask >> lane) & 1, skip, acc);
-                vq[i] = advance();
-                if (vq[i]) {
-                    rq[i] = lr;
-                    sq[i] = ls0;
-                    gather_load(p.v, rq[i], sq[i], lane, below, oq[i]);
-                }
-            }
+            const bool have = more || nk < nb;
+            if (have) gather_load(p.v, nxt, ns0, lane, below, on);
+            gather_apply(cur, s0, oc, (cur.rowmask >> lane) & 1, skip, acc);
+            if (!have) break;
+            cur = nxt;
+            k = nk;
+            s0 = ns0;
+            oc = on;
         }
     }
 #pragma unroll
