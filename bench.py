@@ -507,51 +507,66 @@ def kernel_rooflines(E, NL, S, torch, np, reps=10, nsteps=20):
     return out
 
 
-def tran_adjoint_cost(E, NL, S, nx=32, nsteps=500, every=25):
+def tran_adjoint_cost(E, NL, S, nx=256, nsteps=50, every=25):
     """Transient adjoint on the RC mesh driven by level-1 inverters (the
-    adjoint workload at nx x nx with nx*nx/32 taps): dV(centre at t =
-    nsteps*2 ps)/dp for every R, C and MOSFET W, checkpoint every `every`
-    steps, against one forward fixed-step TRAP transient through the same
-    plain GPU backend (adjoint_over_forward) and against the adjoint's own
-    forward pass, which solves every step to its exact root
-    (adjoint_over_exact_forward)."""
+    adjoint workload: nx x nx with nx*nx/32 taps, 2,048 at 256x256):
+    dV(centre at t = nsteps*2 ps)/dp for every R, C and MOSFET W with a
+    checkpoint every `every` steps, against one plain fixed-step TRAP
+    transient of the same steps through SparseLuSolver (adjoint_over_forward;
+    at n >= 20,000 both factor supernodally, the plain one in the reference's
+    arithmetic) and against the adjoint's own forward pass, which solves every
+    step to its exact root (adjoint_over_exact_forward). The first call of
+    each (the symbolic analyses of the DC and TRAN patterns and their plans)
+    is timed apart as setup."""
     import numpy as np
 
     c = NL.rc_mesh_inverters(nx, nx)
     probe = NL.rc_mesh_probe(nx, nx)
     dt = NL.C5_TRAN["dt"]
     g = E.GpuCircuit(c)
-    s = S.GpuSolver(c.node_count)
-    st, _, _ = g.tran_fixed(s, dt, 2)  # warm-up: patterns, first factors, launch shapes
+    s = S.GpuSolver(c.node_count, S.SPARSELU)
+    t0 = time.perf_counter()
+    st, _, _ = g.tran_fixed(s, dt, 1)  # setup: patterns, analyses, plans
+    setup_fwd = time.perf_counter() - t0
+    _log(f"  adjoint workload {nx}x{nx}: plain forward setup {setup_fwd:.1f} s")
     t0 = time.perf_counter()
     st, xf, it = g.tran_fixed(s, dt, nsteps)
     t_fwd = time.perf_counter() - t0
-    g.tran_adjoint(s, dt, 2, probe, every)  # warm-up of the adjoint's solvers
+    _log(f"  plain forward {nsteps} steps {t_fwd:.1f} s, status {st}, {it} Newton iterations")
+    t0 = time.perf_counter()
+    g.tran_adjoint(s, dt, 1, probe, every)  # setup of the adjoint's solvers and plans
+    setup_adj = time.perf_counter() - t0
     t0 = time.perf_counter()
     dp, xa, info = g.tran_adjoint(s, dt, nsteps, probe, every)
     t_adj = time.perf_counter() - t0
+    _log(f"  adjoint {nsteps} steps every {every}: {t_adj:.1f} s, status {info['status']}")
     if info["status"] != 0:
         return dict(workload=f"{nx}x{nx} RC mesh adjoint", adjoint_info=info, forward_status=int(st))
     # every state kept in host memory instead of checkpoints: no recomputation
     t0 = time.perf_counter()
     dp_all, _, info_all = g.tran_adjoint(s, dt, nsteps, probe, 0)
     t_all = time.perf_counter() - t0
-    kinds = [d["kind"] for d in c.devices]
+    kinds = np.array([d["kind"] for d in c.devices])
+    sn = s.supernodal_info()
     return dict(workload=f"{nx}x{nx} RC mesh, {nx * nx // 32} inverter taps, {nsteps} TRAP steps of 2 ps, "
                          f"checkpoint every {every}; dV(centre)/d(every R, C, W)",
-                n=int(len(xa)), forward_status=int(st), forward_ms=1e3 * t_fwd, forward_newton_iters=int(it),
+                n=int(len(xa)), devices=int(c.ndev), solver="SparseLU (supernodal at n >= 20,000)",
+                supernodal=dict(active=int(sn["active"]), supernodes=int(sn["supernodes"]),
+                                levels=int(sn["levels"])),
+                setup_s=dict(plain_forward=setup_fwd, adjoint=setup_adj),
+                forward_status=int(st), forward_ms=1e3 * t_fwd, forward_newton_iters=int(it),
                 adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd,
                 adjoint_over_exact_forward=t_adj / max(info["forward_ms"] * 1e-3, 1e-12), adjoint_info=info,
                 keep_all_states=dict(adjoint_ms=1e3 * t_all, adjoint_over_forward=t_all / t_fwd,
                                      states=int(info_all["checkpoints"]),
                                      recomputed_steps=int(info_all["recomputed_steps"]),
                                      same_dp_as_checkpointed=bool(np.array_equal(dp_all, dp))),
-                sensitivities=dict(R=kinds.count(NL.RS_DEV_R), C=kinds.count(NL.RS_DEV_C),
-                                   W=kinds.count(NL.RS_DEV_M)),
+                sensitivities=dict(R=int((kinds == NL.RS_DEV_R).sum()), C=int((kinds == NL.RS_DEV_C).sum()),
+                                   W=int((kinds == NL.RS_DEV_M).sum())),
                 probe_v=float(xa[probe]),
-                max_abs_dp=dict(R=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_R]).max()),
-                                C=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_C]).max()),
-                                W=float(np.abs(dp[np.array(kinds) == NL.RS_DEV_M]).max())))
+                max_abs_dp=dict(R=float(np.abs(dp[kinds == NL.RS_DEV_R]).max()),
+                                C=float(np.abs(dp[kinds == NL.RS_DEV_C]).max()),
+                                W=float(np.abs(dp[kinds == NL.RS_DEV_M]).max())))
 
 
 def tran_graph_cost(E, NL, S):
@@ -784,8 +799,13 @@ def main():
     ap.add_argument("--loop-sample", type=int, default=16)
     ap.add_argument("--host-steps", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-grid", type=int, default=256, help="RC mesh side of the transient adjoint")
+    ap.add_argument("--adjoint-steps", type=int, default=50,
+                    help="TRAP steps of the transient adjoint (500: the workload's 1 ns)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the transient adjoint section (with --adjoint-steps 500: the full workload)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel rooflines (mesh refactor/solve, RC mesh eval/assemble)")
     args = ap.parse_args()
@@ -816,6 +836,10 @@ def main():
     if args.kernels_only:
         print(json.dumps({"kernel_rooflines": kernel_rooflines(E, NL, S, torch, np)}))
         return
+    if args.adjoint_only:
+        print(json.dumps({"tran_adjoint": tran_adjoint_cost(E, NL, S, nx=args.adjoint_grid,
+                                                            nsteps=args.adjoint_steps)}))
+        return
     hl, state = power_grid_tran(args, E, NL, S, torch, np)
     line = dict(
         metric=METRIC, value=hl["value"], unit="refactor+solve/s", n_gpus=1, steps=args.steps,
@@ -845,7 +869,7 @@ def main():
         if not args.no_adjoint:
             _log("secondary: adjoints, graphs, mesh refactor+solve")
             line["adjoint"] = adjoint_cost(E, NL, S, torch)
-            line["tran_adjoint"] = tran_adjoint_cost(E, NL, S)
+            line["tran_adjoint"] = tran_adjoint_cost(E, NL, S, nx=args.adjoint_grid, nsteps=args.adjoint_steps)
             line["tran_graph"] = tran_graph_cost(E, NL, S)
             line["tran_device_controller"] = tran_device_cost(E, NL, S)
             line["ac_sweep"] = ac_sweep_cost(NL, np)

@@ -230,6 +230,13 @@ typedef struct rs_tran_result {
 
 /* Copies the instance list to the device (node numbers as in rs_circuit.h). */
 rs_circuit *rs_circuit_create(const rs_device *dev, int64_t ndev, int64_t node_count);
+/* No reference counterpart (the reference rebuilds its circuit): replaces
+ * device k's values (value, p[], for a level-1 MOSFET its W, L and model
+ * parameters) in place, keeping its kind, nodes and flags, so the stamp
+ * pattern, the solvers' analyses and the adjoint's plans stay valid: finite
+ * differences and parameter sweeps without a new analysis. BSIM devices are
+ * refused (their model index selects a pattern-changing table). */
+int32_t rs_circuit_set_device(rs_circuit *c, int64_t k, const rs_device *d);
 void rs_circuit_destroy(rs_circuit *c);
 int32_t rs_circuit_last_error(const rs_circuit *c, char *buf, int64_t len);
 /* Model table of the circuit's BSIM MOSFETs (rs_device.p[RS_MP_LEVEL] != 1,
@@ -442,14 +449,16 @@ void rs_profile_enable(int32_t on);
 void rs_profile_get(double *ms_total, int64_t *counts);
 /* Supernodal factor and solve (no reference counterpart: a schedule of the
  * same arithmetic). mode 0: off; 1 (default): SparseLU patterns (and a
- * BbdSolver's SparseLU fallback) of n >= 20,000 whose pattern is
- * structurally symmetric, as an MNA matrix without MOSFETs is; 2: every
- * structurally symmetric SparseLU pattern. Such a pattern is factored on the
- * device from its first factor on: every entry of L and U receives the
- * reference's terms in the reference's order (sparse_lu.rs numeric_factor),
- * grouped into dense supernode panels that the levels of the supernodal
- * elimination tree gather and factor in parallel; the solves run the same
- * levels. Takes effect at the next analysis. rs_supernodal_info: out[9] =
+ * BbdSolver's SparseLU fallback) of n >= 20,000; 2: every SparseLU pattern.
+ * Such a pattern is factored on the device from its first factor on: every
+ * entry of L and U receives the reference's terms in the reference's order
+ * (sparse_lu.rs numeric_factor), grouped into dense supernode panels that
+ * the levels of the supernodal elimination tree gather and factor in
+ * parallel; the solves run the same levels. A pattern that is not
+ * structurally symmetric (MOSFET stamps) is planned on the symmetric closure
+ * of its fill, whose extra entries stay exact zeros and change no result.
+ * rs_tran_adjoint's own solvers follow the mode of its solver argument.
+ * Takes effect at the next analysis. rs_supernodal_info: out[9] =
  * {active, supernodes, levels, contributions, panel doubles, device bytes,
  * multiply-subtracts per factor, widest supernode, the multiply-subtracts
  * of descendant contributions (the k_sn_gather launches)}. */

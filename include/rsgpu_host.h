@@ -92,15 +92,17 @@ int32_t rs_host_stamp_pattern(const rs_device *dev, int64_t ndev, int64_t node_c
  * and, when ax is given, the factor of ax and the solve of rhs (x) executed
  * in the order of the GPU kernels. lx/ux/ud receive the factor in the
  * symbolic's CSC order. x, lx, ux, ud equal SparseLuSolver factor(ax);
- * solve(rhs) bit for bit. Returns 6 when the pattern is not structurally
- * symmetric. hot_min > 0: the count of supernodes whose R must hold a position
- * for its diagonal entry to be summed on its own (default 256). stats[12] =
- * {supernodes, levels, contributions, panel doubles, widest supernode,
- * longest R_J, multiply-subtracts of one factor, gather tiles, nnz(L),
- * nnz(U), entries summed on their own, tile records}. */
+ * solve(rhs) bit for bit. An unsymmetric pattern is planned on its symmetric
+ * closure. hot_min > 0: the count of supernodes whose R must hold a position
+ * for its diagonal entry to be summed on its own (default 256). flags bit 0:
+ * the plan of A^T on A's symbolic factor (x solves A^T x = rhs; lx and ux
+ * are not written). stats[13] = {supernodes, levels, contributions, panel
+ * doubles, widest supernode, longest R_J, multiply-subtracts of one factor,
+ * gather tiles, nnz(L), nnz(U), entries summed on their own, tile records,
+ * 1 when the pattern is structurally symmetric}. */
 int32_t rs_host_sn_selftest(int64_t n, const int64_t *ap, const int64_t *ai, const double *ax,
                             const double *rhs, double *x, double *lx, double *ux, double *ud,
-                            int64_t hot_min, int64_t *stats);
+                            int64_t hot_min, int64_t flags, int64_t *stats);
 /* The adaptive transient controller's power x^y (double-double, rounded
  * once; ddpow.hpp) evaluated on the host for n pairs, with near[i] = 1 where
  * the exact power lies within zone ulp of a rounding midpoint (where the

@@ -43,7 +43,7 @@ EXPORTS = [
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest", "rs_host_pow_dd",
     "rs_host_bbd_decompose", "rs_set_bbd_blocks", "rs_bbd_info",
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
-    "rs_circuit_set_cap_state", "rs_circuit_set_initial_guess", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
+    "rs_circuit_set_cap_state", "rs_circuit_set_initial_guess", "rs_circuit_set_device", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
     "rs_dc_sweep_points", "rs_dc_sweep", "rs_circuit_set_bsim_models", "rs_mos_eval",
     "rs_profile_enable", "rs_profile_get", "rs_set_poison",
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_info",
@@ -173,7 +173,7 @@ def lib():
     L.rs_host_lp_selftest.restype = C.c_int32
     L.rs_host_lp_selftest.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, f64p, f64p, i64p]
     L.rs_host_sn_selftest.restype = C.c_int32
-    L.rs_host_sn_selftest.argtypes = [C.c_int64, i64p, i64p, vp, vp, vp, vp, vp, vp, C.c_int64, i64p]
+    L.rs_host_sn_selftest.argtypes = [C.c_int64, i64p, i64p, vp, vp, vp, vp, vp, vp, C.c_int64, C.c_int64, i64p]
     L.rs_profile_enable.argtypes = [C.c_int32]
     L.rs_profile_get.argtypes = [f64p, i64p]
     L.rs_set_poison.argtypes = [C.c_int32]

@@ -26,6 +26,7 @@
 #include "fold.hpp"
 #include "profile.hpp"
 #include "tran_ctl.hpp"
+#include "sn.hpp"
 
 namespace rsgpu {
 hipError_t launch_eval(const rs_device* dev, int ndev, const double* x, int xl, StampArgs a,
@@ -113,7 +114,13 @@ struct rs_circuit {
         DevBuf<PlanDev> tdesc;
         bool have_rows = false;        // row map of the stamped pattern (polish residuals)
         DevBuf<int32_t> rp, rc, rq;
+        // a pattern the solver factors supernodally: J^T factored on its own
+        // (the plan of A^T on the solver's symbolic factor) for the solves
+        // with J^T, rebuilt when the solver analyzes a new pattern
+        SnImpl* snT = nullptr;
+        int64_t snT_analysis = -1;
     } adj[2];
+    int adj_sn_mode = 1;  // supernodal setting of the adjoint's solvers (the TRAN adjoint's caller's)
     int64_t polish_iters = 0;
     // fixed-step transient with each TRAP step as a captured hipGraph
     bool use_graph = false;
@@ -558,16 +565,44 @@ int32_t adj_factor(rs_circuit* c, int mode, const double* x_dev, i64 xl, double 
         // instead of a structural zero clamped to 1e-14
         A.s = rs_solver_create(RS_SOLVER_SPARSELU, n);
         A.s->ordering = 4;
+        rs_set_supernodal(A.s, c->adj_sn_mode);
     }
     // SparseLU picks no pivots (pivot_tol 0), so a device refactor on the
     // stored symbolic analysis computes what a fresh factor would
     rs_prepare(A.s, n);
-    if ((rc = solver_factor_dev(A.s, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
-                                c->d_ax.p, nullptr)))
-        return from_solver(c, A.s, rc);
     rs_solver* a = A.s;
-    if (a->pd.nblk != 1) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint needs a single-block factor");
     hipError_t e;
+    if ((rc = solver_analyze_dev(a, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size())))
+        return from_solver(c, a, rc);
+    if (a->sn) {  // J^T factored supernodally from J's values; J's own factor is not needed
+        if (!A.snT || A.snT_analysis != a->analyses) {
+            sn_destroy(A.snT);
+            std::string why;
+            A.snT = sn_create(sn_symbolic(a->sn), dp->p.ap.data(), dp->p.ai.data(), &why, true);
+            if (!A.snT) return cfail(c, RS_ERR_DEVICE, "adjoint transposed supernodal plan: " + why);
+            A.snT_analysis = a->analyses;
+            // J's int32 pattern for the refinement's residual, and every
+            // device's aux unknown for the contraction
+            std::vector<int32_t> jap(n + 1), jai(std::max<size_t>(dp->p.ai.size(), 1)), aux(std::max<i64>(nd, 1));
+            for (i64 i = 0; i <= n; i++) jap[i] = (int32_t)dp->p.ap[i];
+            for (size_t q = 0; q < dp->p.ai.size(); q++) jai[q] = (int32_t)dp->p.ai[q];
+            for (i64 k = 0; k < nd; k++) aux[k] = dp->p.aux_of[k];
+            if ((e = A.jap.ensure(n + 1)) || (e = A.jai.ensure(jai.size())) || (e = A.aux.ensure(aux.size())) ||
+                (e = hipMemcpy(A.jap.p, jap.data(), jap.size() * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(A.jai.p, jai.data(), jai.size() * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(A.aux.p, aux.data(), aux.size() * 4, hipMemcpyHostToDevice)))
+                return chip(c, e, "adjoint pattern");
+        }
+        const int ps = prof_begin(PK_REFACTOR, nullptr);
+        e = sn_factor(A.snT, c->d_ax.p, nullptr);
+        prof_end(ps, nullptr);
+        if (e) return chip(c, e, "adjoint transposed supernodal factor");
+        return RS_OK;
+    }
+    if ((rc = solver_factor_dev(a, dp->p.ap.data(), n + 1, dp->p.ai.data(), (i64)dp->p.ai.size(),
+                                c->d_ax.p, nullptr)))
+        return from_solver(c, a, rc);
+    if (a->pd.nblk != 1) return cfail(c, RS_ERR_SOLVE_FAILED, "adjoint needs a single-block factor");
     if (A.factors != a->factor_count || !A.ltp.p) {  // L^T levels of this factorization
         std::vector<int32_t> ptr, cols;
         lt_levels(a->ph, n, ptr, cols);
@@ -616,6 +651,12 @@ int32_t adj_solve(rs_circuit* c, int mode, const double* rhs_dev, double* lam_de
     hipStream_t s = nullptr;
     hipError_t e;
     auto tsolve = [&](const double* b, double* y) -> hipError_t {
+        if (a->sn && A.snT) {
+            const int ps = prof_begin(PK_SOLVE, s);
+            hipError_t er = sn_solve(A.snT, b, y, s);
+            prof_end(ps, s);
+            return er;
+        }
         if (A.t_ok)
             return launch_solve_lp(A.tdesc.p, A.tip.p, a->d_vals.p, b, y, (int)n, A.t_maxent, A.t_nlev, 256, s);
         return launch_solve_t(a->d_desc.p, a->d_ip.p, a->d_vals.p, b, A.ltp.p, A.ltc.p, A.n_lt, c->d_t.p, y, s);
@@ -682,6 +723,7 @@ int32_t exact_newton(rs_circuit* c, std::vector<double>& x, int mode, double t, 
         if (!A.s) {
             A.s = rs_solver_create(RS_SOLVER_SPARSELU, n);
             A.s->ordering = 4;
+            rs_set_supernodal(A.s, c->adj_sn_mode);
         }
         if (!A.have_rows) {  // row map of the pattern, columns ascending within a row
             std::vector<int32_t> rp(n + 1, 0), rcol(std::max<size_t>(dp->p.ai.size(), 1)), rq(rcol.size());
@@ -1037,8 +1079,10 @@ void rs_circuit_destroy(rs_circuit* c) {
         (void)hipEventDestroy(c->ev_rhs);
         (void)hipStreamDestroy(c->side);
     }
-    for (auto& a : c->adj)
+    for (auto& a : c->adj) {
+        sn_destroy(a.snT);
         if (a.s) rs_solver_destroy(a.s);
+    }
     for (auto*& b : c->dcb)
         if (b) rs_batch_destroy(b);
     for (auto& p : c->pat) {
@@ -1207,11 +1251,33 @@ int32_t rs_dc_adjoint(rs_circuit* c, const double* x, int64_t x_len, int64_t pro
         return chip(c, e, "adjoint rhs");
     if ((rc = adj_solve(c, MODE_DC, c->d_e.p, c->d_lam.p)))
         return rc;
+    if (nd && !c->adj[0].aux.p) return cfail(c, RS_ERR_DEVICE, "adjoint: no aux map for the contraction");
     if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, c->adj[0].aux.p, c->d_x.p, (int)xl, c->d_lam.p,
                                      gmin, 1.0, MODE_DC, nullptr, nullptr, 0.0, 0, c->d_dp.p, s)) ||
         (nd && (e = hipMemcpy(dp_out, c->d_dp.p, nd * sizeof(double), hipMemcpyDeviceToHost))) ||
         (lambda_out && (e = hipMemcpy(lambda_out, c->d_lam.p, n * sizeof(double), hipMemcpyDeviceToHost))))
         return chip(c, e, "adjoint");
+    return RS_OK;
+}
+
+int32_t rs_circuit_set_device(rs_circuit* c, int64_t k, const rs_device* d) {
+    if (k < 0 || k >= (int64_t)c->dev.size() || !d)
+        return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: device index out of range");
+    const rs_device& o = c->dev[k];
+    bool same = d->kind == o.kind && d->flags == o.flags;
+    for (int q = 0; q < 4; q++) same = same && d->node[q] == o.node[q];
+    if (is_bsim(o) || is_bsim(*d)) same = false;
+    if (!same)
+        return cfail(c, RS_ERR_INVALID_MATRIX,
+                     "Invalid matrix: a device update must keep its kind, nodes, flags and level-1 model");
+    c->dev[k] = *d;
+    hipError_t e = hipMemcpy(c->d_dev.p + k, d, sizeof(rs_device), hipMemcpyHostToDevice);
+    if (e) return chip(c, e, "hipMemcpy(device)");
+    for (auto*& b : c->dcb)  // the one-instance DC batches hold copies of the devices
+        if (b) {
+            rs_batch_destroy(b);
+            b = nullptr;
+        }
     return RS_OK;
 }
 
@@ -1581,6 +1647,13 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
     if (nsteps < 0 || !(dt > 0.0)) return cfail(c, RS_ERR_INVALID_MATRIX, "Invalid matrix: bad dt or step count");
     const bool keep_all = every < 1;  // every state kept: no recomputation
     if (keep_all) every = nsteps + 1;
+    // the adjoint's own solvers follow the caller's supernodal setting
+    c->adj_sn_mode = sol->sn_mode;
+    for (auto& a : c->adj)
+        if (a.s && a.s->sn_mode != sol->sn_mode) {
+            rs_set_supernodal(a.s, sol->sn_mode);
+            rs_reset_pattern(a.s);
+        }
     const i64 nd = (i64)c->dev.size(), nc = c->node_count;
     hipError_t e;
     int32_t rc;
@@ -1776,6 +1849,7 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
             double* lam = lamb[cur].p;
             // the adjoint factor has no clamped pivot: one refinement step
             if ((rc = adj_solve(c, mode, c->d_e.p, lam, 1))) return rc;
+            if (nd && !A.aux.p) return cfail(c, RS_ERR_DEVICE, "adjoint: no aux map for the contraction");
             if ((e = launch_adjoint_contract(c->d_dev.p, (int)nd, A.aux.p, xk.p, (int)xv.size(), lam, gm[k],
                                              sc[k], mode, xv1 ? xk1.p : nullptr, xv2 ? xk2.p : nullptr, dt, 1,
                                              dpa.p, nullptr)))

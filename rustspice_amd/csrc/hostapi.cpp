@@ -379,12 +379,13 @@ int32_t rs_host_lp_selftest(int32_t kind, int64_t n, const int64_t* ap, const in
 
 int32_t rs_host_sn_selftest(int64_t n, const int64_t* ap, const int64_t* ai, const double* ax,
                             const double* rhs, double* x, double* lx, double* ux, double* ud,
-                            int64_t hot_min, int64_t* stats) {
+                            int64_t hot_min, int64_t flags, int64_t* stats) {
     try {
         BlockSymbolic s;
         block_symbolic(n, ap, ai, false, s);
         SnHost h;
-        if (!build_sn_plan(s, ap, ai, h, nullptr, hot_min > 0 ? (int32_t)hot_min : kSnHotRecords))
+        const bool transpose = (flags & 1) != 0;
+        if (!build_sn_plan(s, ap, ai, h, nullptr, hot_min > 0 ? (int32_t)hot_min : kSnHotRecords, transpose))
             return RS_ERR_INVALID_MATRIX;
         if (stats) {
             stats[0] = h.ns;
@@ -399,12 +400,13 @@ int32_t rs_host_sn_selftest(int64_t n, const int64_t* ap, const int64_t* ai, con
             stats[9] = (int64_t)s.uri.size();
             stats[10] = (int64_t)h.hot_slot.size();
             stats[11] = h.tile_rec.empty() ? 0 : h.tile_rec.back();
+            stats[12] = h.symmetric ? 1 : 0;
         }
         if (!ax) return RS_OK;  // plan statistics only
         std::vector<double> v, d;
         sn_host_factor(h, ax, v, d);
         if (rhs && x) sn_host_solve(h, v, d, rhs, x);
-        if (lx && ux) {
+        if (lx && ux && !transpose) {
             std::vector<double> l, u;
             sn_to_csc(h, s, v, l, u);
             std::copy(l.begin(), l.end(), lx);

@@ -101,12 +101,12 @@ hipError_t up(SnImpl* p, DevBuf<T>& b, const std::vector<T>& v) {
 
 }  // namespace
 
-SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why) {
+SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why, bool transpose) {
     SnImpl* p = new SnImpl();
     const char* vb = std::getenv("RSGPU_VERBOSE");
     const bool verbose = vb && vb[0] == '1';
     const auto t0 = std::chrono::steady_clock::now();
-    if (!build_sn_plan(s, ap, ai, p->h, why)) {
+    if (!build_sn_plan(s, ap, ai, p->h, why, kSnHotRecords, transpose)) {
         delete p;
         return nullptr;
     }

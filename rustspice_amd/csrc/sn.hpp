@@ -117,6 +117,7 @@ struct SnHost {
     // position -> original index (the solve's permutations)
     std::vector<int32_t> perm;       // perm[original] = position
     // statistics
+    bool symmetric = true;           // false: the plan holds the pattern's symmetric closure
     int64_t npairs = 0, max_w = 0, max_h = 0;
     double updates = 0.0;            // multiply-subtracts of one factor (pattern count)
     double gather_updates = 0.0;     // the part applied by descendant contributions
@@ -144,11 +145,13 @@ struct SnPtrs {
 };
 
 // Builds the supernodal plan of a single-block SparseLU symbolic factor
-// (positions = pivot steps, identity row pivots). Returns false, with a
-// reason, when the pattern is not structurally symmetric or exceeds the
-// 32-bit index ranges.
+// (positions = pivot steps, identity row pivots). An unsymmetric pattern is
+// planned on its symmetric closure (explicit zeros that change no value).
+// transpose: the plan factors A^T from A's values (the same closure and
+// order; the scatter swaps every entry's row and column). Returns false,
+// with a reason, when the pattern exceeds the 32-bit index ranges.
 bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost& h,
-                   std::string* why = nullptr, int32_t hot_min = kSnHotRecords);
+                   std::string* why = nullptr, int32_t hot_min = kSnHotRecords, bool transpose = false);
 
 // Host execution of the plan in the device kernels' order: factor of ax,
 // then the solve of rhs into x; lx/ux/ud (optional) receive the factor in
@@ -168,7 +171,8 @@ namespace rsgpu {
 // Device state of one supernodal plan: the plan arrays, the panels, the
 // pivots and the solve vector, with the per-level launch lists.
 struct SnImpl;
-SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why);
+SnImpl* sn_create(const BlockSymbolic& s, const i64* ap, const i64* ai, std::string* why,
+                  bool transpose = false);
 void sn_destroy(SnImpl* p);
 // factor of the unmerged CSC values ax_dev (device memory, nnz entries)
 hipError_t sn_factor(SnImpl* p, const double* ax_dev, hipStream_t s);

@@ -25,38 +25,108 @@ bool fail(std::string* why, const char* m) {
     return false;
 }
 
+// Whether the strictly upper part of U's row k is L(:, k)'s pattern for
+// every k (U's rows collected column by column come out ascending, like lri).
+bool structurally_symmetric(const BlockSymbolic& s) {
+    const i64 n = s.n;
+    std::vector<i64> cnt(n + 1, 0);
+    for (i64 c = 0; c < n; c++)
+        for (i64 q = s.ucp[c]; q < s.ucp[c + 1] - 1; q++) cnt[s.uri[q] + 1]++;
+    for (i64 k = 0; k < n; k++)
+        if (cnt[k + 1] != s.lcp[k + 1] - s.lcp[k]) return false;
+    for (i64 k = 0; k < n; k++) cnt[k + 1] += cnt[k];
+    std::vector<i64> fill(cnt.begin(), cnt.end() - 1);
+    std::vector<int32_t> ucols((size_t)cnt[n]);
+    for (i64 c = 0; c < n; c++)
+        for (i64 q = s.ucp[c]; q < s.ucp[c + 1] - 1; q++) ucols[fill[s.uri[q]]++] = (int32_t)c;
+    for (i64 k = 0; k < n; k++)
+        for (i64 q = s.lcp[k], t = cnt[k]; q < s.lcp[k + 1]; q++, t++)
+            if (ucols[t] != (int32_t)s.lri[q]) return false;
+    return true;
+}
+
+// The symmetric closure of an unsymmetric fill pattern: the Cholesky fill of
+// pattern(L + U^T) in the same order, column by column (a column's rows are
+// its own entries and those of its elimination-tree children, less itself).
+// It holds L and U^T (both lie in the Cholesky fill of A + A^T), so the
+// plan's panels hold every entry of the factor plus explicit zeros. Those
+// change no value: an entry outside the factor's pattern stays exactly +0.0
+// through the fan-in (0.0 - (+-0) = +0.0) and contributes +-0 terms, and the
+// reference's running sum is never -0.0 (it starts as 0.0 + A(i, k), and
+// an exact cancellation rounds to +0.0), so subtracting +-0 leaves it
+// unchanged; a zero U(j, k) is skipped as the reference skips it.
+void symmetric_closure(const BlockSymbolic& s, BlockSymbolic& o) {
+    const i64 n = s.n;
+    // lower part of L + U^T by column
+    std::vector<i64> bp(n + 1, 0);
+    for (i64 k = 0; k < n; k++) bp[k + 1] += s.lcp[k + 1] - s.lcp[k];
+    for (i64 c = 0; c < n; c++)
+        for (i64 q = s.ucp[c]; q < s.ucp[c + 1] - 1; q++) bp[s.uri[q] + 1]++;
+    for (i64 k = 0; k < n; k++) bp[k + 1] += bp[k];
+    std::vector<int32_t> bi((size_t)bp[n]);
+    {
+        std::vector<i64> f(bp.begin(), bp.end() - 1);
+        for (i64 k = 0; k < n; k++)
+            for (i64 q = s.lcp[k]; q < s.lcp[k + 1]; q++) bi[f[k]++] = (int32_t)s.lri[q];
+        for (i64 c = 0; c < n; c++)
+            for (i64 q = s.ucp[c]; q < s.ucp[c + 1] - 1; q++) bi[f[s.uri[q]]++] = (int32_t)c;
+    }
+    o.n = n;
+    o.col_perm = s.col_perm;
+    o.col_perm_inv = s.col_perm_inv;
+    o.lcp.assign(n + 1, 0);
+    o.lri.clear();
+    std::vector<int32_t> head(n, -1), next(n, -1), mark(n, -1), col;
+    for (i64 k = 0; k < n; k++) {
+        col.clear();
+        auto add = [&](int32_t r) {
+            if (r > k && mark[r] != k) {
+                mark[r] = (int32_t)k;
+                col.push_back(r);
+            }
+        };
+        for (i64 q = bp[k]; q < bp[k + 1]; q++) add(bi[q]);
+        for (int32_t c = head[k]; c >= 0; c = next[c])
+            for (i64 q = o.lcp[c]; q < o.lcp[c + 1]; q++) add((int32_t)o.lri[q]);
+        std::sort(col.begin(), col.end());
+        for (int32_t r : col) o.lri.push_back(r);
+        o.lcp[k + 1] = (i64)o.lri.size();
+        if (!col.empty()) {  // elimination-tree parent: the first row below k
+            next[k] = head[col[0]];
+            head[col[0]] = (int32_t)k;
+        }
+    }
+    // U = L^T with the diagonal last; rows come out ascending
+    o.ucp.assign(n + 1, 0);
+    for (int32_t r : o.lri) o.ucp[r + 1]++;
+    for (i64 c = 0; c < n; c++) o.ucp[c + 1] += o.ucp[c] + 1;
+    o.uri.assign((size_t)o.ucp[n], 0);
+    std::vector<i64> f(o.ucp.begin(), o.ucp.end() - 1);
+    for (i64 k = 0; k < n; k++)
+        for (i64 q = o.lcp[k]; q < o.lcp[k + 1]; q++) o.uri[f[o.lri[q]]++] = k;
+    for (i64 c = 0; c < n; c++) o.uri[f[c]] = c;
+}
+
 }  // namespace
 
-bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost& h,
-                   std::string* why, int32_t hot_min) {
+bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost& h,
+                   std::string* why, int32_t hot_min, bool transpose) {
     h = SnHost();
-    const i64 n = s.n;
+    const i64 n = s0.n;
     if (n <= 0) return fail(why, "empty matrix");
     if (n >= (i64)std::numeric_limits<int32_t>::max() / 2) return fail(why, "n exceeds 32-bit indices");
     h.n = (int32_t)n;
+
+    // 1. a structurally symmetric pattern (every U row of a supernode spans
+    //    its R_J): the symbolic's own, or its symmetric closure
+    BlockSymbolic closure;
+    h.symmetric = structurally_symmetric(s0);
+    if (!h.symmetric) symmetric_closure(s0, closure);
+    const BlockSymbolic& s = h.symmetric ? s0 : closure;
     const std::vector<i64>& lcp = s.lcp;
     const std::vector<i64>& lri = s.lri;
     const std::vector<i64>& ucp = s.ucp;
     const std::vector<i64>& uri = s.uri;
-
-    // 1. structural symmetry: the strictly upper part of U's row k must be
-    //    L(:, k)'s pattern (U's rows collected column by column come out
-    //    ascending, like lri)
-    {
-        std::vector<i64> cnt(n + 1, 0);
-        for (i64 c = 0; c < n; c++)
-            for (i64 q = ucp[c]; q < ucp[c + 1] - 1; q++) cnt[uri[q] + 1]++;
-        for (i64 k = 0; k < n; k++)
-            if (cnt[k + 1] != lcp[k + 1] - lcp[k]) return fail(why, "pattern is not structurally symmetric");
-        for (i64 k = 0; k < n; k++) cnt[k + 1] += cnt[k];
-        std::vector<i64> fill(cnt.begin(), cnt.end() - 1);
-        std::vector<int32_t> ucols((size_t)cnt[n]);
-        for (i64 c = 0; c < n; c++)
-            for (i64 q = ucp[c]; q < ucp[c + 1] - 1; q++) ucols[fill[uri[q]]++] = (int32_t)c;
-        for (i64 k = 0; k < n; k++)
-            for (i64 q = lcp[k], t = cnt[k]; q < lcp[k + 1]; q++, t++)
-                if (ucols[t] != (int32_t)lri[q]) return fail(why, "pattern is not structurally symmetric");
-    }
 
     // 2. fundamental supernodes: column c + 1 continues c when L(:, c) is
     //    {c + 1} u L(:, c + 1) and c is c + 1's only child
@@ -414,7 +484,9 @@ bool build_sn_plan(const BlockSymbolic& s, const i64* ap, const i64* ai, SnHost&
             i64 q2 = q + 1;
             while (q2 < ap[oc + 1] && ai[q2] == orow) q2++;
             if (orow < n) {
-                const int64_t sl = slot_of((int32_t)s.col_perm[orow], (int32_t)k);
+                // A(orow, oc) at (position of orow, k); for A^T at (k, position of orow)
+                const int32_t pr = (int32_t)s.col_perm[orow];
+                const int64_t sl = transpose ? slot_of((int32_t)k, pr) : slot_of(pr, (int32_t)k);
                 if (sl >= 0) {
                     if (q > std::numeric_limits<int32_t>::max() - 1) return fail(why, "nnz exceeds 32-bit offsets");
                     h.sc_src.push_back((int32_t)q);

@@ -63,6 +63,7 @@ void reset(rs_solver* h) {
 // do_analyze (native_klu.rs:704-761)
 int32_t do_analyze(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail) {
     const i64 n = h->n;
+    h->analyses++;
     for (i64 c = 0; c < n; c++)
         if (ap[c + 1] < ap[c] || ap[c + 1] > ail)
             return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: column pointers are not monotone");
@@ -701,6 +702,15 @@ int32_t solver_factor_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i
         h->last_parallel = false;
     }
     h->state = ST_FACTORED;
+    return RS_OK;
+}
+
+int32_t solver_analyze_dev(rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail) {
+    if (h->bbd || h->dense) return RS_OK;
+    if (h->state == ST_EMPTY || !pattern_matches(h, ap, apl, ai, ail)) {
+        if (apl != h->n + 1) return fail(h, RS_ERR_INVALID_MATRIX, "Invalid matrix: bad ap length");
+        return do_analyze(h, ap, apl, ai, ail);
+    }
     return RS_OK;
 }
 

@@ -91,6 +91,9 @@ int32_t solver_factor_dev(struct ::rs_solver* h, const i64* ap, i64 apl, const i
                           const double* ax_dev, hipStream_t s);
 int32_t solver_solve_dev(struct ::rs_solver* h, const double* rhs_dev, double* x_dev,
                          hipStream_t s);
+// The symbolic analysis of a pattern when it is not the analyzed one (no
+// factor); each analysis advances rs_solver::analyses.
+int32_t solver_analyze_dev(struct ::rs_solver* h, const i64* ap, i64 apl, const i64* ai, i64 ail);
 }  // namespace rsgpu
 
 using rsgpu::i64;
@@ -113,6 +116,7 @@ struct rs_solver {
     rsgpu::PlanDev pd{};
     bool plan_on_device = false;
     i64 factor_count = 0, refactor_count = 0;
+    int64_t analyses = 0;  // symbolic analyses run (a new one replaces sn)
     double rcond = 0.0;
     bool last_parallel = false;
     i64 threads_hint = 0;

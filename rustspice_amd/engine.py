@@ -75,6 +75,8 @@ def _bind():
                                      vp, vp]
     L.rs_circuit_set_initial_guess.restype = C.c_int32
     L.rs_circuit_set_initial_guess.argtypes = [vp, vp, C.c_int64]
+    L.rs_circuit_set_device.restype = C.c_int32
+    L.rs_circuit_set_device.argtypes = [vp, C.c_int64, vp]
     L.rs_circuit_set_cap_state.restype = C.c_int32
     L.rs_circuit_set_cap_state.argtypes = [vp] * 5
     L.rs_stamp_device.restype = C.c_int32
@@ -157,6 +159,12 @@ class GpuCircuit:
             self._models = np.ascontiguousarray(models)
             self._check(self._L.rs_circuit_set_bsim_models(self.h, _ptr(self._models),
                                                            len(self._models)))
+
+    def set_device(self, k: int):
+        """rs_circuit_set_device: uploads device k's record from the host
+        circuit again after its values changed (same kind, nodes and flags)."""
+        rec = np.ascontiguousarray(self.circuit.devices[k: k + 1])
+        self._check(self._L.rs_circuit_set_device(self.h, int(k), _ptr(rec)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -133,7 +133,8 @@ class GpuSolver:
     def set_supernodal(self, mode: int = 1):
         """0 off, 1 SparseLU patterns of n >= 20,000 (default), 2 every
         SparseLU pattern: factor and solve by supernodal panels on the
-        device when the pattern is structurally symmetric."""
+        device (an unsymmetric pattern on its symmetric closure). The
+        transient adjoint's own solvers follow this setting."""
         self._L.rs_set_supernodal(self.h, int(mode))
 
     def supernodal_info(self) -> dict:
@@ -419,19 +420,22 @@ def host_bbd_decompose(n, ap, ai, num_blocks):
                 border_size=n - b0.value if nb > 1 else 0)
 
 
-def host_sn_selftest(n, ap, ai, ax=None, rhs=None, hot_min=0):
+def host_sn_selftest(n, ap, ai, ax=None, rhs=None, hot_min=0, transpose=False):
     """Host execution of the supernodal schedule (the order of the GPU
     kernels): SparseLU symbolic factor, supernodal plan and, with ax, the
-    factor of ax and the solve of rhs. Returns (x, lx, ux, ud, stats); the
-    arrays are None when ax is None."""
+    factor of ax and the solve of rhs (transpose: the plan of A^T on A's
+    symbolic factor, x solving A^T x = rhs; lx and ux are then not
+    computed). Returns (x, lx, ux, ud, stats); the arrays are None when ax
+    is None."""
     L = _lib.lib()
-    st = np.zeros(12, np.int64)
+    st = np.zeros(13, np.int64)
     ap, ai = _i64(ap), _i64(ai)
-    rc = L.rs_host_sn_selftest(n, ap, ai, None, None, None, None, None, None, hot_min, st)
+    fl = 1 if transpose else 0
+    rc = L.rs_host_sn_selftest(n, ap, ai, None, None, None, None, None, None, hot_min, fl, st)
     if rc:
         raise SolverError(rc, "no supernodal plan")
     keys = ["supernodes", "levels", "contributions", "panel_doubles", "max_w", "max_h", "updates",
-            "gather_tiles", "nnz_l", "nnz_u", "hot_entries", "tile_records"]
+            "gather_tiles", "nnz_l", "nnz_u", "hot_entries", "tile_records", "symmetric"]
     stats = dict(zip(keys, map(int, st)))
     if ax is None:
         return None, None, None, None, stats
@@ -441,7 +445,7 @@ def host_sn_selftest(n, ap, ai, ax=None, rhs=None, hot_min=0):
     lx, ux = np.zeros(max(stats["nnz_l"], 1)), np.zeros(max(stats["nnz_u"], 1))
     ud = np.zeros(max(n, 1))
     p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    rc = L.rs_host_sn_selftest(n, ap, ai, p(ax), p(rhs), p(x), p(lx), p(ux), p(ud), hot_min, st)
+    rc = L.rs_host_sn_selftest(n, ap, ai, p(ax), p(rhs), p(x), p(lx), p(ux), p(ud), hot_min, fl, st)
     if rc:
         raise SolverError(rc, "no supernodal plan")
     return x[:n], lx[: stats["nnz_l"]], ux[: stats["nnz_u"]], ud[:n], stats

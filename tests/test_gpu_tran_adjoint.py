@@ -21,9 +21,10 @@ pytestmark = pytest.mark.gpu
 DT = 2e-12
 
 
-def _gpu(c, nsteps, probe, every):
+def _gpu(c, nsteps, probe, every, sn=1):
     g = E.GpuCircuit(c)
     s = S.GpuSolver(c.node_count)
+    s.set_supernodal(sn)
     dp, x, info = g.tran_adjoint(s, DT, nsteps, probe, every)
     assert info["status"] == 0 and info["inexact_steps"] == 0, info
     return dp, x, info
@@ -141,3 +142,103 @@ def test_no_value_is_read_before_it_is_written(gpu):
         _lib.set_poison(False)
     for (dp, x, _), (dpp, xp, _) in zip(base, pois):
         assert np.array_equal(x, xp) and np.array_equal(dp, dpp)
+
+
+@pytest.mark.parametrize("nx,taps,load,N", [(4, 2, 1e4, 12), (8, 4, None, 20)])
+def test_supernodal_factors_agree_with_the_level_kernels(gpu, nx, taps, load, N):
+    """With the adjoint's solvers on the supernodal path (forced at this size;
+    the default above 20,000 unknowns), the exact-root forward is the same
+    SparseLU arithmetic, so its states are bit-identical; the backward factors
+    J^T on its own instead of solving with J's factor transposed, so the
+    gradient agrees to round-off. The checkpoint interval still leaves the
+    gradient bit-identical."""
+    c = NL.rc_mesh_inverters(nx, nx, taps, load=load)
+    probe = NL.rc_mesh_probe(nx, nx)
+    dp0, x0, _ = _gpu(c, N, probe, 5, sn=0)
+    runs = [_gpu(c, N, probe, every, sn=2) for every in (5, 3, 0)]
+    dp, x, info = runs[0]
+    assert np.array_equal(x, x0)
+    for kind in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M):
+        idx = np.array([k for k in range(c.ndev) if c.devices[k]["kind"] == kind])
+        scale = np.abs(dp0[idx]).max()
+        assert scale > 0.0
+        assert np.allclose(dp[idx], dp0[idx], rtol=1e-9, atol=1e-9 * scale), kind
+    for dpe, xe, _ in runs[1:]:
+        assert np.array_equal(xe, x) and np.array_equal(dpe, dp)
+
+
+def test_supernodal_adjoint_matches_dense_cpu_adjoint(gpu):
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    probe = NL.rc_mesh_probe(4, 4)
+    N = 12
+    dp, _, _ = _gpu(c, N, probe, 5, sn=2)
+    params = [(k, *R.param(c, k)) for k in range(c.ndev)
+              if c.devices[k]["kind"] in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M)]
+    ref, _, _ = R.adjoint(c, DT, N, probe, params)
+    for kind in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M):
+        idx = [i for i, (k, *_) in enumerate(params) if c.devices[k]["kind"] == kind]
+        want = ref[idx]
+        got = dp[[params[i][0] for i in idx]]
+        scale = np.abs(want).max()
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-6 * scale), (kind, got, want)
+
+
+def test_full_size_workload_matches_differences_of_the_gpu_forward(gpu):
+    """The adjoint workload at its full size (256x256 RC mesh, 2,048 inverter
+    taps, no loads, n = 65,541): the adjoint's solvers factor supernodally
+    (n >= 20,000) on the symmetric closure of the MOSFET-unsymmetric pattern,
+    J^T on a plan of its own. Over 25 steps (50 ps, one checkpoint segment),
+    central differences (h = 1e-6 relative) of the GPU's exact-root forward,
+    each parameter updated in place (rs_circuit_set_device), on seeded R, C
+    and W next to the probe, where the sensitivities stand well above the
+    difference quotient's round-off floor; checkpoints every 5 steps leave
+    the gradient bit-identical."""
+    nx = 256
+    c = NL.rc_mesh_inverters(nx, nx)
+    probe = NL.rc_mesh_probe(nx, nx)
+    N = 25
+    g = E.GpuCircuit(c)
+    s = S.GpuSolver(c.node_count, S.SPARSELU)  # its DC OP starts the forward (NativeKLU: no plan at this size)
+    dp, x, info = g.tran_adjoint(s, DT, N, probe, 0)
+    print(f"full-size adjoint: n={len(x)}, forward {info['forward_ms']:.0f} ms", flush=True)
+    assert info["status"] == 0 and info["inexact_steps"] == 0, info
+    dp1, x1, _ = g.tran_adjoint(s, DT, N, probe, 5)
+    assert np.array_equal(x1, x) and np.array_equal(dp1, dp)
+
+    def dist(k):
+        nodes = [int(v) - 3 for v in c.devices[k]["node"][:2] if int(v) >= 3]
+        pi, pj = divmod(probe - 3, nx)
+        return min((abs(v // nx - pi) + abs(v % nx - pj) for v in nodes), default=10 ** 9)
+
+    rng = np.random.RandomState(7)
+    picks = []
+    for kind, radius, count in ((NL.RS_DEV_R, 1, 2), (NL.RS_DEV_C, 1, 3), (NL.RS_DEV_M, 12, 3)):
+        cand = [k for k in range(c.ndev) if c.devices[k]["kind"] == kind and dist(k) <= radius]
+        assert len(cand) >= count
+        picks += [int(k) for k in rng.choice(cand, count, replace=False)]
+    checked = 0
+    for k in picks:
+        get, put = R.param(c, k)
+        p0 = get()
+        h = 1e-6 * abs(p0)
+        vals = []
+        for p in (p0 + h, p0 - h):
+            put(p)
+            g.set_device(k)
+            dpx, xx, inf = g.tran_adjoint(s, DT, N, probe, 0)
+            assert inf["status"] == 0
+            vals.append(xx[probe])
+        put(p0)
+        g.set_device(k)
+        fd = (vals[0] - vals[1]) / (2.0 * h)
+        print(f"device {k}: adjoint {dp[k]:.6e} difference {fd:.6e}", flush=True)
+        # the exact-root forward stops once |dx| <= 1e-12 + 1e-10 |x| and its
+        # chord steps contract by 10x at least, so two perturbed forwards
+        # carry a few 1e-12 V of solver noise at this size: the quotient's floor
+        floor = 5e-12 / h
+        assert abs(dp[k] - fd) <= 1e-6 * abs(fd) + floor, (k, dp[k], fd)
+        checked += abs(fd) >= 3.0 * floor
+    assert checked >= 4
+    # the restored parameters give the first gradient back
+    dp2, x2, _ = g.tran_adjoint(s, DT, N, probe, 0)
+    assert np.array_equal(x2, x) and np.array_equal(dp2, dp)

@@ -72,14 +72,92 @@ def test_random_symmetric_patterns_with_zeros_and_clamps():
         _check(ap, ai, ax, rhs, n)
 
 
-def test_unsymmetric_pattern_refused():
+def _csc(M, rs):
+    cols = [np.nonzero(M[:, j])[0] for j in range(M.shape[0])]
+    ap = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int64)
+    ai = np.concatenate(cols + [np.zeros(0, np.int64)]).astype(np.int64)
+    ax = rs.randn(len(ai))
+    ax[rs.rand(len(ai)) < 0.15] = 0.0
+    ax[rs.rand(len(ai)) < 0.05] = 1e-16
+    return ap, ai, ax
+
+
+def _transpose(ap, ai, ax, n):
+    """CSC of A^T with duplicates kept in A's CSC order."""
+    cols = np.repeat(np.arange(n), np.diff(ap))
+    order = np.lexsort((np.arange(len(ai)), ai))  # by row of A, stable
+    bp = np.concatenate([[0], np.cumsum(np.bincount(ai, minlength=n))]).astype(np.int64)
+    return bp, cols[order].astype(np.int64), ax[order]
+
+
+def test_random_unsymmetric_patterns_on_the_symmetric_closure():
+    """Random structurally unsymmetric matrices: the plan holds the symmetric
+    closure of the fill (explicit zeros) and still reproduces SparseLU's
+    factor and solve bit for bit, zero values and clamped pivots included."""
+    rs = np.random.RandomState(11)
+    seen = 0
+    for _ in range(120):
+        n = int(rs.randint(2, 70))
+        M = rs.rand(n, n) < rs.rand() * 0.2
+        np.fill_diagonal(M, rs.rand(n) < 0.9)
+        ap, ai, ax = _csc(M, rs)
+        rhs = rs.randn(n)
+        st = _check(ap, ai, ax, rhs, n)
+        seen += st["symmetric"] == 0
+    assert seen > 60
+
+
+def test_inverter_tapped_mesh_tran_matrix():
     """MOSFET stamps (gm at (drain, gate) without (gate, drain)) make the
-    inverter-tapped mesh structurally unsymmetric: no supernodal plan."""
+    inverter-tapped RC mesh structurally unsymmetric."""
+    for nx, taps, seed in [(12, 9, 3), (20, 12, 4)]:
+        c = NL.rc_mesh_inverters(nx, nx, taps)
+        ap, ai, ax, rhs, n = _tran_mna(c, seed)
+        st = _check(ap, ai, ax, rhs, n, hot_min=8)
+        assert st["symmetric"] == 0
     ap = np.array([0, 2, 3], np.int64)
     ai = np.array([0, 1, 1], np.int64)
-    with pytest.raises(S.SolverError):
-        S.host_sn_selftest(2, ap, ai)
-    c = NL.rc_mesh_inverters(12, 12, 9)
-    ap, ai, ax, rhs, n = _tran_mna(c, 3)
-    with pytest.raises(S.SolverError):
-        S.host_sn_selftest(n, ap, ai)
+    _check(ap, ai, np.array([2.0, -1.0, 3.0]), np.array([1.0, 2.0]), 2)
+
+
+def test_transposed_plan_solves_the_transposed_system():
+    """The plan of A^T on A's symbolic factor (the adjoint's factor): where
+    A^T's own SparseLU symbolic picks the same order, its solution equals
+    SparseLU on the explicit transpose bit for bit; always within round-off
+    of a dense solve."""
+    rs = np.random.RandomState(12)
+    same_order = 0
+    cases = []
+    for _ in range(60):
+        n = int(rs.randint(2, 60))
+        M = rs.rand(n, n) < rs.rand() * 0.2
+        np.fill_diagonal(M, True)
+        ap, ai, ax = _csc(M, rs)
+        ax[np.repeat(np.arange(n), np.diff(ap)) == ai] += 4.0  # a dominant diagonal: a tight dense check
+        cases.append((ap, ai, ax, n, True))
+    c = NL.rc_mesh_inverters(12, 12, 9)  # ground kept: singular but for the clamped pivot
+    ap, ai, ax, _, n = _tran_mna(c, 5)
+    cases.append((ap, ai, ax, n, False))
+    for ap, ai, ax, n, dense in cases:
+        rhs = rs.randn(n)
+        x, _, _, _, _ = S.host_sn_selftest(n, ap, ai, ax, rhs, transpose=True)
+        if dense:
+            A = np.zeros((n, n))
+            np.add.at(A, (ai, np.repeat(np.arange(n), np.diff(ap))), ax)
+            xd = np.linalg.solve(A.T, rhs)
+            assert np.allclose(x, xd, rtol=1e-8, atol=1e-8 * np.abs(xd).max())
+        bp, bi, bx = _transpose(ap, ai, ax, n)
+        o = O.NativeKlu(n)
+        o.set_sparselu()
+        assert o.factor(bp, bi, bx) == 0
+        oa = O.NativeKlu(n)
+        oa.set_sparselu()
+        assert oa.factor(ap, ai, ax) == 0
+        if np.array_equal(o.block(0)["col_perm"], oa.block(0)["col_perm"]):
+            same_order += 1
+            xo = rhs.copy()
+            assert o.solve(xo) == 0
+            assert same_bits(x, xo)
+        else:
+            assert dense  # the mesh's order is checked bit for bit
+    assert same_order > 30
