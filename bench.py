@@ -555,9 +555,15 @@ def tran_adjoint_cost(E, NL, S, nx=256, nsteps=50, every=25):
                                 levels=int(sn["levels"])),
                 setup_s=dict(plain_forward=setup_fwd, adjoint=setup_adj),
                 forward_status=int(st), forward_ms=1e3 * t_fwd, forward_newton_iters=int(it),
-                adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd,
+                forward_note=None if st == 0 else
+                "the reference-semantics TRAP loop through SparseLuSolver stops (MaxIters): the ground "
+                "node's row is kept and its pivot clamped to 1e-14, so the node voltages drift with "
+                "the ground potential; the adjoint's own forward (exact roots, no clamped pivot) is "
+                "the forward it is measured against",
+                adjoint_ms=1e3 * t_adj, adjoint_over_forward=t_adj / t_fwd if st == 0 else None,
                 adjoint_over_exact_forward=t_adj / max(info["forward_ms"] * 1e-3, 1e-12), adjoint_info=info,
-                keep_all_states=dict(adjoint_ms=1e3 * t_all, adjoint_over_forward=t_all / t_fwd,
+                keep_all_states=dict(adjoint_ms=1e3 * t_all,
+                                     adjoint_over_exact_forward=t_all / max(info_all["forward_ms"] * 1e-3, 1e-12),
                                      states=int(info_all["checkpoints"]),
                                      recomputed_steps=int(info_all["recomputed_steps"]),
                                      same_dp_as_checkpointed=bool(np.array_equal(dp_all, dp))),

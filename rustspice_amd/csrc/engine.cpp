@@ -1718,8 +1718,11 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         xall.push_back(x);
     }
     // one TRAP step to its exact root: exact-Jacobian Newton from x_{k-1};
-    // should that not converge, the reference loop (with its stepping) first.
-    // The recomputation runs the same function, so it reproduces x_k exactly.
+    // should that not converge, the reference loop (with its stepping) first,
+    // on the adjoint's own factor (SparseLU with the unknowns that have no
+    // diagonal entry last: no pivot clamped to 1e-14, which on large meshes
+    // keeps the caller's SparseLU from converging at all). The recomputation
+    // runs the same function, so it reproduces x_k exactly.
     int64_t inexact = 0;
     auto tran_step = [&](std::vector<double>& xv, int64_t k, double tt, int32_t* iters,
                          bool* conv) -> int32_t {
@@ -1730,7 +1733,8 @@ int32_t rs_tran_adjoint(rs_circuit* c, rs_solver* sol, const rs_newton_opts* o, 
         if (!okk) {
             xv = x0;
             rs_newton_result rr{};
-            if ((rcc = run_newton_stepping(c, sol, o, xv, MODE_TRAN_TRAP, tt, dt, &rr, iters, &gm[k], &sc[k])))
+            rs_solver* fb = c->adj[1].s ? c->adj[1].s : sol;
+            if ((rcc = run_newton_stepping(c, fb, o, xv, MODE_TRAN_TRAP, tt, dt, &rr, iters, &gm[k], &sc[k])))
                 return rcc;
             if (!rr.converged) {
                 *conv = false;

@@ -242,3 +242,30 @@ def test_full_size_workload_matches_differences_of_the_gpu_forward(gpu):
     # the restored parameters give the first gradient back
     dp2, x2, _ = g.tran_adjoint(s, DT, N, probe, 0)
     assert np.array_equal(x2, x) and np.array_equal(dp2, dp)
+
+
+def test_32x32_instance_matches_dense_cpu_adjoint(gpu):
+    """The adjoint workload's 32x32 instance (n = 1,031, 32 taps, each with a
+    load to ground so that no MOSFET sits on the level-1 kink, where a
+    central-difference Jacobian is not the model's derivative) over 50 steps
+    with a checkpoint every 25, against the dense FP64 CPU adjoint (colour
+    group Jacobians) on 64 seeded parameters: 24 R, 24 C and 16 W."""
+    nx, N = 32, 50
+    c = NL.rc_mesh_inverters(nx, nx, load=1e4)
+    probe = NL.rc_mesh_probe(nx, nx)
+    dp, _, _ = _gpu(c, N, probe, 25)
+    rng = np.random.RandomState(3)
+    ks = []
+    for kind, count in ((NL.RS_DEV_R, 24), (NL.RS_DEV_C, 24), (NL.RS_DEV_M, 16)):
+        cand = [k for k in range(c.ndev) if c.devices[k]["kind"] == kind]
+        ks += [int(k) for k in rng.choice(cand, count, replace=False)]
+    ks.sort()
+    ref, _, _ = R.adjoint(c, DT, N, probe, [(k, *R.param(c, k)) for k in ks], colored=True)
+    kinds = np.array([c.devices[k]["kind"] for k in ks])
+    for kind in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M):
+        sel = kinds == kind
+        assert sel.sum() >= 1
+        want, got = ref[sel], dp[np.array(ks)[sel]]
+        # the kind's largest over all devices sets the absolute scale
+        scale = np.abs(dp[[k for k in range(c.ndev) if c.devices[k]["kind"] == kind]]).max()
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-6 * scale), (kind, got, want)

@@ -49,3 +49,24 @@ def test_adjoint_matches_its_finite_differences(kind):
         assert abs(g - fd) <= 1e-5 * max(abs(fd), abs(g)) + floor, (kind, k, g, fd, floor)
         resolved += abs(fd) > 1e3 * floor
     assert resolved >= 1
+
+
+def test_colored_jacobians_match_column_differences():
+    """The colour-group Jacobians (the 32x32 check's) against the
+    column-by-column ones at every state of a short run, for J_k and the
+    cross-step terms; and the gradients of the two routes agree."""
+    c = NL.rc_mesh_inverters(4, 4, 2, load=1e4)
+    dt, N = 2e-12, 5
+    probe = NL.rc_mesh_probe(4, 4)
+    xs, hist = R.forward(c, dt, N)
+    pat = R._pattern(c, R.TRAN, xs[1:], len(xs[1]))
+    assert pat[1].max() + 1 < len(xs[1]) // 2  # fewer colours than columns
+    for k in range(1, N + 1):
+        f = lambda v, k=k: R._F(c, v, R.TRAN, k * dt, dt, hist[k])[0]  # noqa: E731
+        Jd, Jc = R._jac(f, xs[k]), R._jac_colored(f, xs[k], *pat)
+        assert np.allclose(Jc, Jd, rtol=1e-6, atol=1e-6 * np.abs(Jd).max()), k
+    params = [(k, *R.param(c, k)) for k in range(c.ndev)
+              if c.devices[k]["kind"] in (NL.RS_DEV_R, NL.RS_DEV_C, NL.RS_DEV_M)]
+    d0, _, _ = R.adjoint(c, dt, N, probe, params)
+    d1, _, _ = R.adjoint(c, dt, N, probe, params, colored=True)
+    assert np.allclose(d1, d0, rtol=1e-6, atol=1e-6 * np.abs(d0).max())
