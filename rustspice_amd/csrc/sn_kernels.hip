@@ -113,8 +113,8 @@ __device__ __forceinline__ void gather_apply(const GRec& R, int s0, const GOps& 
     }
 }
 
-__global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
-    const int it = item0 + blockIdx.x;
+template <int kDepth>
+__device__ __forceinline__ void gather_tile(const SnPtrs& p, int it) {
     const int4 g = p.gitems[it];
     const int J = g.x, kind = g.y, r0 = g.z, c0 = g.w;
     const int jj0 = p.j0[J], w = p.j0[J + 1] - jj0, hh = p.rptr[J + 1] - p.rptr[J];
@@ -166,33 +166,91 @@ __global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) {
             R.lds = __builtin_amdgcn_readlane(f_lds, k);
             return R;
         };
-        GRec cur = rec(0);
-        GOps oc, on;
-        gather_load(p.v, cur, 0, lane, below, oc);
-        int k = 0, s0 = 0;
-        while (true) {
-            // the next chunk: the rest of this record, else the next record
-            GRec nxt = cur;
-            int nk = k, ns0 = s0 + kSb;
-            const bool more = ns0 < cur.ws;
-            if (!more) {
-                nk = k + 1;
-                ns0 = 0;
-                if (nk < nb) nxt = rec(nk);
+        if constexpr (kDepth <= 2) {
+            // the next chunk's operands load while the current one is applied
+            GRec cur = rec(0);
+            GOps oc, on;
+            gather_load(p.v, cur, 0, lane, below, oc);
+            int k = 0, s0 = 0;
+            while (true) {
+                // the next chunk: the rest of this record, else the next record
+                GRec nxt = cur;
+                int nk = k, ns0 = s0 + kSb;
+                const bool more = ns0 < cur.ws;
+                if (!more) {
+                    nk = k + 1;
+                    ns0 = 0;
+                    if (nk < nb) nxt = rec(nk);
+                }
+                const bool have = more || nk < nb;
+                if (have) gather_load(p.v, nxt, ns0, lane, below, on);
+                gather_apply(cur, s0, oc, (cur.rowmask >> lane) & 1, skip, acc);
+                if (!have) break;
+                cur = nxt;
+                k = nk;
+                s0 = ns0;
+                oc = on;
             }
-            const bool have = more || nk < nb;
-            if (have) gather_load(p.v, nxt, ns0, lane, below, on);
-            gather_apply(cur, s0, oc, (cur.rowmask >> lane) & 1, skip, acc);
-            if (!have) break;
-            cur = nxt;
-            k = nk;
-            s0 = ns0;
-            oc = on;
+        } else {
+            // a ring of kDepth chunks in flight: slot i holds chunk kDepth *
+            // round + i; a chunk is applied, then its slot is refilled with
+            // the chunk kDepth ahead, so the loads of the next kDepth - 1
+            // chunks overlap each apply (for chains of short records, where
+            // one chunk ahead leaves the load latency exposed)
+            int lk = 0, ls0 = 0;  // the last chunk issued: record lk, terms ls0 ..
+            GRec lr = rec(0);
+            auto advance = [&]() -> bool {
+                if (lk >= nb) return false;
+                ls0 += kSb;
+                if (ls0 < lr.ws) return true;
+                lk++;
+                ls0 = 0;
+                if (lk >= nb) return false;
+                lr = rec(lk);
+                return true;
+            };
+            GRec rq[kDepth];
+            GOps oq[kDepth];
+            int sq[kDepth];
+            bool vq[kDepth];
+#pragma unroll
+            for (int i = 0; i < kDepth; i++) {
+                vq[i] = i == 0 ? true : advance();
+                if (vq[i]) {
+                    rq[i] = lr;
+                    sq[i] = ls0;
+                    gather_load(p.v, rq[i], sq[i], lane, below, oq[i]);
+                }
+            }
+            bool go = true;
+            while (go) {
+#pragma unroll
+                for (int i = 0; i < kDepth; i++) {
+                    if (!vq[i]) {
+                        go = false;
+                        break;
+                    }
+                    gather_apply(rq[i], sq[i], oq[i], (rq[i].rowmask >> lane) & 1, skip, acc);
+                    vq[i] = advance();
+                    if (vq[i]) {
+                        rq[i] = lr;
+                        sq[i] = ls0;
+                        gather_load(p.v, rq[i], sq[i], lane, below, oq[i]);
+                    }
+                }
+            }
         }
     }
 #pragma unroll
     for (int k = 0; k < kCols; k++)
         if (lane < nr && cb + k < nc && !((skip >> k) & 1)) base[(int64_t)(cb + k) * ld + lane] = acc[k];
+}
+
+__global__ __launch_bounds__(kT) void k_sn_gather(SnPtrs p, int item0) { gather_tile<2>(p, item0 + blockIdx.x); }
+// levels with few tiles (the top of the tree): long chains of short records
+// on few wavefronts, so occupancy matters less than chunks in flight
+__global__ __launch_bounds__(kT) void k_sn_gather_deep(SnPtrs p, int item0) {
+    gather_tile<4>(p, item0 + blockIdx.x);
 }
 
 // ---- ordered long sums (fold_long, fold.hpp) ------------------------------
@@ -643,9 +701,15 @@ hipError_t sn_launch_scatter(const int32_t* src, const int32_t* cnt, const int64
     hipLaunchKernelGGL(k_sn_scatter, dim3(blocks_for(nruns)), dim3(kT), 0, s, src, cnt, slot, nruns, ax, v);
     return hipGetLastError();
 }
+// levels with fewer gather tiles than this take the deep-ring gather
+constexpr int kDeepItems = 512;
+
 hipError_t sn_launch_gather(const SnPtrs& p, int item0, int nitems, hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sn_gather, dim3(nitems), dim3(kT), 0, s, p, item0);
+    if (nitems < kDeepItems)
+        hipLaunchKernelGGL(k_sn_gather_deep, dim3(nitems), dim3(kT), 0, s, p, item0);
+    else
+        hipLaunchKernelGGL(k_sn_gather, dim3(nitems), dim3(kT), 0, s, p, item0);
     return hipGetLastError();
 }
 hipError_t sn_launch_hot_parts(const SnPtrs& p, const rsgpu::SnPart* parts, int nparts, const int64_t* terms,

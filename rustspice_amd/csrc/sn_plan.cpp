@@ -258,9 +258,26 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
     //    least hot_min supernodes (the ground node) has its diagonal
     //    entry summed on its own: its records are dropped from the tile
     //    unless they touch another entry of it, and its terms listed flat.
+    //    In a supernode that at least 4 hot_min descendants update (the
+    //    root), every entry in the row or the column of such a position
+    //    lying in its panels is summed on its own too: all those
+    //    descendants hold the position in their R, so the tiles holding its
+    //    row or column would otherwise chain all of their records.
     std::vector<int32_t> occ(n, 0);
     for (int32_t r : h.rows) occ[r]++;
     auto hot_pos = [&](int32_t p) { return occ[p] >= hot_min; };
+    std::vector<std::vector<int32_t>> hlines(ns);  // hot positions whose lines are hot in J
+    for (int32_t J = 0; J < ns; J++) {
+        if (h.cptr[J + 1] - h.cptr[J] < 4 * (int64_t)hot_min) continue;
+        for (int32_t c = h.j0[J]; c < h.j0[J + 1]; c++)
+            if (hot_pos(c)) hlines[J].push_back(c);
+        for (int32_t q = h.rptr[J]; q < h.rptr[J + 1]; q++)
+            if (hot_pos(h.rows[q])) hlines[J].push_back(h.rows[q]);
+        std::sort(hlines[J].begin(), hlines[J].end());
+    }
+    auto in_lines = [&](int32_t J, int32_t pos) {
+        return std::binary_search(hlines[J].begin(), hlines[J].end(), pos);
+    };
     std::vector<int64_t> tile_base(ns + 1, 0);
     std::vector<int32_t> nrt(ns), nct(ns), nur(ns), nuc(ns);
     for (int32_t J = 0; J < ns; J++) {
@@ -274,6 +291,17 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
     }
     const int64_t ntiles = tile_base[ns];
     if (ntiles >= (int64_t)std::numeric_limits<int32_t>::max()) return fail(why, "too many gather tiles");
+    // whether every entry (R_S[i], R_S[k]), i in [i0, i1), k in [k0, k1), of
+    // J's panels is summed elsewhere
+    auto all_hot = [&](int32_t J, const int32_t* RS, int32_t i0, int32_t i1, int32_t k0, int32_t k1) {
+        if (hlines[J].empty())  // only the diagonal of a hot position of J's own columns
+            return i1 - i0 == 1 && k1 - k0 == 1 && RS[i0] == RS[k0] && hot_pos(RS[i0]) &&
+                   RS[i0] >= h.j0[J] && RS[i0] < h.j0[J + 1];
+        bool rows = true, cols = true;
+        for (int32_t i = i0; i < i1 && rows; i++) rows = in_lines(J, RS[i]);
+        for (int32_t k = k0; k < k1 && cols; k++) cols = in_lines(J, RS[k]);
+        return rows || cols;
+    };
     // enumerate the records of one target J, calling emit(tile, pair, t0, t1, u0, u1)
     auto records = [&](int32_t J, auto&& emit) {
         const int32_t jj0 = h.j0[J], w = h.j0[J + 1] - jj0;
@@ -291,8 +319,9 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
                     const int32_t rt = rl[t0] / kSnTile;
                     int32_t t1 = t0 + 1;
                     while (t1 < hS && rl[t1] / kSnTile == rt) t1++;
-                    // a record whose only entry is a hot diagonal is summed elsewhere
-                    const bool only_hot = t1 - t0 == 1 && u1 - u0 == 1 && RS[t0] == RS[u0] && hot_pos(RS[t0]);
+                    // a record whose entries are all summed elsewhere (a hot
+                    // diagonal, or lines of hot positions) is dropped
+                    const bool only_hot = all_hot(J, RS, t0, t1, u0, u1);
                     if (!only_hot) emit(tile_base[J] + (int64_t)ct * nrt[J] + rt, pi, t0, t1, u0, u1);
                     t0 = t1;
                 }
@@ -307,7 +336,8 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
                     const int32_t ct = (rl[t0] - w) / kSnTile;
                     int32_t t1 = t0 + 1;
                     while (t1 < hS && (rl[t1] - w) / kSnTile == ct) t1++;
-                    emit(tile_base[J] + (int64_t)nrt[J] * nct[J] + (int64_t)ct * nur[J] + rt, pi, t0, t1, u0, u1);
+                    if (!all_hot(J, RS, u0, u1, t0, t1))  // rows R_S(u), columns R_S(t)
+                        emit(tile_base[J] + (int64_t)nrt[J] * nct[J] + (int64_t)ct * nur[J] + rt, pi, t0, t1, u0, u1);
                     t0 = t1;
                 }
                 u0 = u1;
@@ -367,27 +397,73 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
         h.rec.swap(rec2);
         h.tile_rec.swap(tr);
     }
-    // hot diagonal entries: every contribution (S ascending, s ascending) as
-    // the slots of L_S(p, s) and U_S(s, p)
-    for (int32_t p = 0; p < n; p++) {
-        if (!hot_pos(p)) continue;
-        const int32_t J = sn_of[p];
+    // hot entries: every contribution (S ascending, s ascending) as the
+    // slots of L_S(row, s) and U_S(s, column); hot_at: (J, kind, row, col)
+    // of each in J's panels (kind 0 LP, 1 UP)
+    std::vector<std::array<int32_t, 4>> hot_at;
+    auto add_hot = [&](int32_t J, int32_t rowpos, int32_t colpos) {
         const int32_t jj0 = h.j0[J], w = h.j0[J + 1] - jj0, hh = h.rptr[J + 1] - h.rptr[J];
-        h.hot_slot.push_back(h.oL[J] + (int64_t)(p - jj0) * (w + hh) + (p - jj0));
-        h.hot_lev.push_back(level[J]);
-        h.hot_ptr.push_back((int64_t)h.hot_terms.size() / 2);
+        const int32_t* RJ = h.rows.data() + h.rptr[J];
+        const bool rin = rowpos >= jj0 && rowpos < jj0 + w, cin = colpos >= jj0 && colpos < jj0 + w;
+        int64_t slot;
+        std::array<int32_t, 4> at;
+        if (cin) {  // LP(row, column)
+            const int32_t rr = rin ? rowpos - jj0 : w + (int32_t)find_sorted(RJ, RJ + hh, rowpos);
+            slot = h.oL[J] + (int64_t)(colpos - jj0) * (w + hh) + rr;
+            at = {J, 0, rr, colpos - jj0};
+        } else {  // UP(row, index of the column in R_J)
+            const int32_t tc = (int32_t)find_sorted(RJ, RJ + hh, colpos);
+            slot = h.oU[J] + (int64_t)tc * w + (rowpos - jj0);
+            at = {J, 1, rowpos - jj0, tc};
+        }
+        const size_t t0 = h.hot_terms.size();
+        const int64_t p0 = (int64_t)t0 / 2;
         for (int32_t pi = h.cptr[J]; pi < h.cptr[J + 1]; pi++) {
-            const int32_t S = h.csrc[pi], a0 = h.ca0[pi], a1 = h.ca1[pi];
-            const int32_t* RS = h.rows.data() + h.rptr[S];
-            const int64_t x = find_sorted(RS + a0, RS + a1, p);
-            if (x < 0) continue;
-            const int32_t t = a0 + (int32_t)x;
+            const int32_t S = h.csrc[pi];
             const int32_t wS = h.j0[S + 1] - h.j0[S], hS = h.rptr[S + 1] - h.rptr[S];
+            const int32_t* RS = h.rows.data() + h.rptr[S];
+            const int64_t tr = find_sorted(RS, RS + hS, rowpos), tcS = find_sorted(RS, RS + hS, colpos);
+            if (tr < 0 || tcS < 0) continue;
             for (int32_t s2 = 0; s2 < wS; s2++) {
-                h.hot_terms.push_back(h.oL[S] + (int64_t)s2 * (wS + hS) + wS + t);  // L_S(p, s)
-                h.hot_terms.push_back(h.oU[S] + (int64_t)t * wS + s2);              // U_S(s, p)
+                h.hot_terms.push_back(h.oL[S] + (int64_t)s2 * (wS + hS) + wS + tr);  // L_S(row, s)
+                h.hot_terms.push_back(h.oU[S] + tcS * wS + s2);                      // U_S(s, column)
                 h.hot_tlev.push_back(level[S]);
             }
+        }
+        if (h.hot_terms.size() == t0 && !(rowpos == colpos && hot_pos(rowpos))) return;  // no terms: not hot
+        h.hot_slot.push_back(slot);
+        h.hot_lev.push_back(level[J]);
+        h.hot_ptr.push_back(p0);
+        hot_at.push_back(at);
+    };
+    for (int32_t J = 0; J < ns; J++) {
+        const int32_t jj0 = h.j0[J], w = h.j0[J + 1] - jj0, hh = h.rptr[J + 1] - h.rptr[J];
+        const int32_t* RJ = h.rows.data() + h.rptr[J];
+        if (hlines[J].empty()) {  // the diagonals of hot positions among J's columns
+            for (int32_t p = jj0; p < jj0 + w; p++)
+                if (hot_pos(p)) add_hot(J, p, p);
+            continue;
+        }
+        // every entry in a row or column of a line position, once
+        std::vector<std::pair<int32_t, int32_t>> ents;
+        for (int32_t p : hlines[J]) {
+            const bool pin = p >= jj0 && p < jj0 + w;
+            for (int32_t c = jj0; c < jj0 + w; c++) {  // row p and column p within J's columns
+                ents.push_back({p, c});
+                ents.push_back({c, p});
+            }
+            if (pin)
+                for (int32_t q = 0; q < hh; q++) {  // p's U row and L column over R_J
+                    ents.push_back({p, RJ[q]});
+                    ents.push_back({RJ[q], p});
+                }
+        }
+        std::sort(ents.begin(), ents.end());
+        ents.erase(std::unique(ents.begin(), ents.end()), ents.end());
+        for (const auto& e : ents) {
+            const bool rin = e.first >= jj0 && e.first < jj0 + w, cin = e.second >= jj0 && e.second < jj0 + w;
+            if (!rin && !cin) continue;  // an ancestor's entry
+            add_hot(J, e.first, e.second);
         }
     }
     h.hot_ptr.push_back((int64_t)h.hot_terms.size() / 2);
@@ -396,17 +472,9 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
         const int64_t nrec = h.tile_rec.back();
         h.frec.resize((size_t)nrec);
         h.tile_hot_ptr.assign(h.gitems.size() / 4 + 1, 0);
-        // hot slot -> (J, row, column) of LP_J
-        std::vector<std::array<int32_t, 3>> hot_at;
-        for (int64_t sl : h.hot_slot)
-            for (int32_t J = 0; J < ns; J++) {  // few hot entries: a scan is fine
-                const int32_t W = h.j0[J + 1] - h.j0[J] + h.rptr[J + 1] - h.rptr[J];
-                const int64_t d = sl - h.oL[J];
-                if (d >= 0 && d < (int64_t)W * (h.j0[J + 1] - h.j0[J])) {
-                    hot_at.push_back({J, (int32_t)(d % W), (int32_t)(d / W)});
-                    break;
-                }
-            }
+        // hot entries by supernode
+        std::vector<std::vector<int32_t>> hot_of(ns);
+        for (size_t e = 0; e < hot_at.size(); e++) hot_of[hot_at[e][0]].push_back((int32_t)e);
         for (size_t it = 0; it < h.gitems.size() / 4; it++) {
             const int32_t J = h.gitems[4 * it], kind = h.gitems[4 * it + 1], r0 = h.gitems[4 * it + 2],
                           c0 = h.gitems[4 * it + 3];
@@ -435,10 +503,11 @@ bool build_sn_plan(const BlockSymbolic& s0, const i64* ap, const i64* ai, SnHost
                 q.ws = wS;
                 q.lds = wS + hS;
             }
-            for (const auto& a : hot_at)
-                if (kind == 0 && a[0] == J && a[1] >= r0 && a[1] < r0 + kSnTile && a[2] >= c0 &&
-                    a[2] < c0 + kSnTile)
-                    h.tile_hot.push_back((a[1] - r0) | ((a[2] - c0) << 8));
+            for (int32_t e : hot_of[J]) {
+                const auto& a = hot_at[e];
+                if (a[1] == kind && a[2] >= r0 && a[2] < r0 + kSnTile && a[3] >= c0 && a[3] < c0 + kSnTile)
+                    h.tile_hot.push_back((a[2] - r0) | ((a[3] - c0) << 8));
+            }
             h.tile_hot_ptr[it + 1] = (int32_t)h.tile_hot.size();
         }
         h.qrec_ptr.assign(h.gitems.size() + 1, 0);  // 4 quarters per item
