@@ -667,6 +667,60 @@ def ac_sweep_cost(NL, np, nodes=200, per_decade=100):
                 and _same_arr(mg[idx], mo))
 
 
+def bbd_schur_cost(NL, S, np, width=2048, depth=4, rails=32, reps=20):
+    """BbdSolver(2)'s dense Schur product S = B_T - sum_k C_k B_kk^-1 R_k
+    (bbd_solver.rs:744-810) on two tap banks of `width` chains joined by a
+    bus of 2 x `rails` border rows that tap every edge cell (border 64, the
+    dense Schur limit; K = 2 x width coupling columns, C and Y half dense):
+    the reference's term order (k_bbd_schur, one cell per thread) against the
+    FP64 MFMA tiles (packing + v_mfma_f64_16x16x4f64 split-K tiles + ordered
+    chunk sum), HIP events around each launch; x of both against the oracle."""
+    from oracle import oracle as O
+    from rustspice_amd import _lib
+
+    n, ap, ai, ax = NL.tap_bank_matrix(width, depth, rails)
+    b = np.random.RandomState(5).randn(n)
+    o = O.Bbd(n, 2)
+    o.prepare(n)
+    o.factor(ap, ai, ax)
+    xo = b.copy()
+    o.solve(xo)
+    out = {}
+    for mode, name in ((0, "reference_order"), (2, "mfma_tiles")):
+        g = S.GpuSolver(n, S.BBD)
+        g.set_bbd_blocks(2)
+        g.set_bbd_mfma(mode)
+        g.factor(ap, ai, ax)
+        _lib.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.factor(ap, ai, ax)
+        wall = (time.perf_counter() - t0) / reps
+        prof = _lib.profile_get()["bbd_schur"]
+        _lib.profile_enable(False)
+        info, binfo = g.bbd_mfma_info(), g.bbd_info()
+        xg = b.copy()
+        g.solve(xg)
+        bs, k = binfo["border"], info["k"]
+        bsp = (bs + 15) // 16 * 16
+        tile_flops = 2.0 * bsp * bsp * ((k + 15) // 16 * 16)
+        out[name] = dict(schur_us=prof["avg_us"], launches=prof["count"], factor_ms=1e3 * wall,
+                         used_tiles=info["used"],
+                         rel_err_vs_oracle=float(np.max(np.abs(xg - xo)) / np.max(np.abs(xo))),
+                         bit_identical_to_oracle=_same_arr(xg, xo))
+        if info["used"]:
+            out[name]["tile_tflops"] = tile_flops / (prof["avg_us"] * 1e-6) / 1e12
+            out[name]["roofline"] = dict(bound="mfma", achieved=out[name]["tile_tflops"], peak=78.6,
+                                         unit="TFLOP/s", frac=out[name]["tile_tflops"] / 78.6)
+    useful = 2.0 * 2 * rails * width * rails  # reference terms with nonzero y, one multiply-subtract each
+    out.update(workload=f"two tap banks of {width} chains x {depth} cells joined by 2x{rails} rails "
+                        f"(n={n}), BbdSolver(2), border {bs}, K={k} coupling columns, "
+                        f"packed C {info['c_density']:.2f} dense",
+               n=int(n), border=int(bs), k=int(k), tile_flops=tile_flops, useful_flops=useful,
+               speedup=out["reference_order"]["schur_us"] / out["mfma_tiles"]["schur_us"])
+    return out
+
+
 def cpu_baseline_mesh(mesh, np, reps=10):
     """The oracle's refactor + solve of the same mesh matrix on one host core
     and the column-parallel refactor (native_klu.rs:402-576) on all host
@@ -812,6 +866,8 @@ def main():
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the transient adjoint section (with --adjoint-steps 500: the full workload)")
+    ap.add_argument("--bbd-only", action="store_true",
+                    help="only the BbdSolver dense Schur product (reference order against MFMA tiles)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel rooflines (mesh refactor/solve, RC mesh eval/assemble)")
     args = ap.parse_args()
@@ -841,6 +897,9 @@ def main():
     torch.cuda.set_device(0)
     if args.kernels_only:
         print(json.dumps({"kernel_rooflines": kernel_rooflines(E, NL, S, torch, np)}))
+        return
+    if args.bbd_only:
+        print(json.dumps({"bbd_schur": bbd_schur_cost(NL, S, np)}))
         return
     if args.adjoint_only:
         print(json.dumps({"tran_adjoint": tran_adjoint_cost(E, NL, S, nx=args.adjoint_grid,
@@ -879,6 +938,7 @@ def main():
             line["tran_graph"] = tran_graph_cost(E, NL, S)
             line["tran_device_controller"] = tran_device_cost(E, NL, S)
             line["ac_sweep"] = ac_sweep_cost(NL, np)
+            line["bbd_schur"] = bbd_schur_cost(NL, S, np)
             line["mesh_refactor_solve"], mesh = mesh_refactor_solve(E, NL, S, torch, np)
             line["kernel_rooflines"] = kernel_rooflines(E, NL, S, torch, np)
             if not args.no_cpu:

@@ -101,6 +101,20 @@ void rs_set_bbd_blocks(rs_solver *h, int64_t num_blocks);
 /* out[4] = {diagonal blocks (1: SparseLU fallback), border size, Schur kind
  * (1 dense, 0 sparse, -1 none), factor count} of an RS_SOLVER_BBD solver. */
 int32_t rs_bbd_info(const rs_solver *h, int64_t *out);
+/* Dense Schur product of an RS_SOLVER_BBD solver on FP64 MFMA tiles (no
+ * reference counterpart: bbd_solver.rs:790-806 accumulates
+ * S = B_T - sum_k C_k B_kk^-1 R_k one term at a time). The coupling columns
+ * (a block's columns that hold C entries) are packed into K x bs operands and
+ * multiplied on v_mfma_f64_16x16x4f64 tiles, split over K and summed in a fixed
+ * order: deterministic, but the terms are not added in the reference's order,
+ * so results agree with it to rounding (1e-9 relative is the bar), not bit
+ * for bit. mode 0 (default): off; 1: when the dense Schur path (border <= 64)
+ * has K >= 64 and both packed operands are at least a quarter nonzero;
+ * 2: whenever the dense Schur path has K > 0. Takes effect at the next factor.
+ * rs_bbd_mfma_info: out[4] = {mode, 1 if the last factor used the tiles,
+ * K, nonzero fraction of the packed C operand in parts per thousand}. */
+void rs_set_bbd_mfma(rs_solver *h, int32_t mode);
+int32_t rs_bbd_mfma_info(const rs_solver *h, int64_t *out);
 /* Launch shapes of the refactor and solve kernels (no reference
  * counterpart). By default the library times every shape (1 to 16
  * wavefronts; the column or the right-looking refactor schedule; work
@@ -443,7 +457,9 @@ int32_t rs_batch_phase_cycles(rs_batch *b, int64_t *out);
  * Kernel timing with HIP events recorded on each kernel's launch stream.
  * Index: 0 refactor, 1 solve, 2 device evaluation, 3 MNA assembly,
  * 4 batched Newton (k_batch_dcop), 5 each k_sn_gather launch of a
- * supernodal factor (also inside index 0). ms_total and counts hold 6 entries.
+ * supernodal factor (also inside index 0), 6 a BbdSolver's Schur product
+ * (k_bbd_schur, or the MFMA tiles with packing and chunk sum). ms_total and
+ * counts hold 7 entries.
  * Enabling (or disabling) clears the totals. */
 void rs_profile_enable(int32_t on);
 void rs_profile_get(double *ms_total, int64_t *counts);

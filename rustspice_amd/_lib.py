@@ -41,7 +41,7 @@ EXPORTS = [
     "rs_host_amd_order", "rs_host_btf", "rs_host_factor_create", "rs_host_factor_destroy",
     "rs_host_factor_nblocks", "rs_host_factor_block_sizes", "rs_host_factor_block_get",
     "rs_host_factor_plan_stats", "rs_host_stamp_pattern", "rs_host_sp_selftest", "rs_host_rl_selftest", "rs_host_lp_selftest", "rs_host_sn_selftest", "rs_host_pow_dd",
-    "rs_host_bbd_decompose", "rs_set_bbd_blocks", "rs_bbd_info",
+    "rs_host_bbd_decompose", "rs_set_bbd_blocks", "rs_bbd_info", "rs_set_bbd_mfma", "rs_bbd_mfma_info",
     "rs_circuit_create", "rs_circuit_destroy", "rs_circuit_last_error", "rs_circuit_pattern",
     "rs_circuit_set_cap_state", "rs_circuit_set_initial_guess", "rs_circuit_set_device", "rs_stamp", "rs_dc_op", "rs_tran", "rs_tran_fixed",
     "rs_dc_sweep_points", "rs_dc_sweep", "rs_circuit_set_bsim_models", "rs_mos_eval",
@@ -165,6 +165,9 @@ def lib():
     L.rs_set_bbd_blocks.argtypes = [C.c_void_p, C.c_int64]
     L.rs_bbd_info.restype = C.c_int32
     L.rs_bbd_info.argtypes = [C.c_void_p, i64p]
+    L.rs_set_bbd_mfma.argtypes = [C.c_void_p, C.c_int32]
+    L.rs_bbd_mfma_info.restype = C.c_int32
+    L.rs_bbd_mfma_info.argtypes = [C.c_void_p, i64p]
     L.rs_host_sp_selftest.restype = C.c_int32
     L.rs_host_sp_selftest.argtypes = [C.c_int32, C.c_int64, i64p, i64p, f64p, f64p, f64p, f64p, i64p]
     L.rs_host_rl_selftest.restype = C.c_int32
@@ -181,7 +184,7 @@ def lib():
     return L
 
 
-PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble", "batch", "sn_gather")
+PROFILE_KERNELS = ("refactor", "solve", "eval", "assemble", "batch", "sn_gather", "bbd_schur")
 
 
 def profile_enable(on: bool = True):

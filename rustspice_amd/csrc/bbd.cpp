@@ -11,6 +11,7 @@
 #include <stdexcept>
 
 #include "../../include/rsgpu.h"
+#include "profile.hpp"
 #include "solver_impl.hpp"
 
 namespace rsgpu {
@@ -226,6 +227,13 @@ struct BbdImpl {
     int64_t n_bmap = 0, n_cidx = 0, n_ridx = 0, n_pidx = 0;  // gather lengths
     std::vector<int64_t> s_ap, s_ai;
     bool lds_ref = false, lds_sol = false;
+    // dense Schur product on MFMA tiles: packed K x bsp operands
+    int32_t mfma_mode = 0;
+    bool mfma_dense = false, mfma_used = false;
+    int64_t mfma_k = 0, mfma_kp = 0, mfma_kc = 0, mfma_permille = 0;
+    int32_t mfma_chunks = 0;
+    DevBuf<int32_t> d_mcptr, d_mcidx, d_mymap;
+    DevBuf<double> d_mcp, d_myp, d_mpart;
     DevBuf<PlanDev> d_fac, d_blk, d_pair;
     DevBuf<int32_t> d_ip, d_bmap, d_perm, d_iperm, d_blkstart;
     DevBuf<int32_t> d_crow, d_cblk, d_ccol, d_cidx, d_rrow, d_rcol, d_ridx;
@@ -475,6 +483,56 @@ int32_t build_bbd(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* a
             }
         }
     }
+    // ---- packed operands of the MFMA Schur product (dense Schur only) ------
+    std::vector<int32_t> mcptr, mcidx, mymap;
+    b->mfma_k = b->mfma_kp = b->mfma_kc = b->mfma_permille = 0;
+    b->mfma_chunks = 0;
+    b->mfma_dense = false;
+    if (b->dense && bs > 0) {
+        std::vector<std::vector<int32_t>> cid(nb);  // coupling column id of (k, col)
+        for (int64_t k = 0; k < nb; k++) cid[k].assign(d.block_ptr[k + 1] - d.block_ptr[k], -1);
+        for (size_t e = 0; e < cblk.size(); e++) cid[cblk[e]][ccol[e]] = 0;
+        std::vector<std::array<int32_t, 2>> ccols;  // (k, col) in (k, col) order
+        for (int64_t k = 0; k < nb; k++)
+            for (size_t c = 0; c < cid[k].size(); c++)
+                if (cid[k][c] == 0) {
+                    cid[k][c] = c32((int64_t)ccols.size());
+                    ccols.push_back({c32(k), c32((int64_t)c)});
+                }
+        const int64_t K = (int64_t)ccols.size(), bsp = (bs + 15) / 16 * 16;
+        if (K > 0) {
+            const int64_t kp = (K + 15) / 16 * 16, m = kp * bsp;
+            std::vector<std::vector<int32_t>> cell(m);  // C entries' ax indices per packed cell
+            for (int64_t r = 0; r < bs; r++)
+                for (int32_t e = crow[r]; e < crow[r + 1]; e++)
+                    cell[(int64_t)cid[cblk[e]][ccol[e]] * bsp + r].push_back(cidx[e]);
+            mcptr.assign(m + 1, 0);
+            int64_t cnz = 0, ynz = 0;
+            for (int64_t t = 0; t < m; t++) {
+                cnz += !cell[t].empty();
+                mcidx.insert(mcidx.end(), cell[t].begin(), cell[t].end());
+                mcptr[t + 1] = c32((int64_t)mcidx.size());
+            }
+            mymap.assign(m, -1);
+            for (int64_t c = 0; c < K; c++)
+                for (int64_t j = 0; j < bs; j++) {
+                    const int32_t p = pof[ccols[c][0] * bs + j];
+                    if (p < 0) continue;
+                    mymap[c * bsp + j] = c32((int64_t)poff[p] + ccols[c][1]);
+                    ynz++;
+                }
+            const double dc = (double)cnz / (double)(K * bs), dy = (double)ynz / (double)(K * bs);
+            b->mfma_k = K;
+            b->mfma_kp = kp;
+            b->mfma_permille = (int64_t)(1000.0 * dc);
+            b->mfma_dense = K >= 64 && dc >= 0.25 && dy >= 0.25;
+            // split K so the tiles fill the chip: about 256 workgroups in all
+            const int64_t nt2 = (bsp / 16) * (bsp / 16);
+            int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(kp / 64, (256 + nt2 - 1) / nt2));
+            b->mfma_kc = (kp / 16 + chunks - 1) / chunks * 16;
+            b->mfma_chunks = (int32_t)((kp + b->mfma_kc - 1) / b->mfma_kc);
+        }
+    }
     b->ncells = (int64_t)cellr.size();
     b->n_bmap = (int64_t)bmap.size();
     b->n_cidx = (int64_t)cidx.size();
@@ -498,8 +556,15 @@ int32_t build_bbd(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* a
         (e = put(b->d_ridx, ridx)) || (e = put(b->d_poff, poff)) || (e = put(b->d_pptr, pptr)) ||
         (e = put(b->d_prow, prow)) || (e = put(b->d_pidx, pidx)) || (e = put(b->d_pof, pof)) ||
         (e = put(b->d_cellr, cellr)) || (e = put(b->d_cellc, cellc)) || (e = put(b->d_btp, btp)) ||
-        (e = put(b->d_bti, bti)))
+        (e = put(b->d_bti, bti)) || (e = put(b->d_mcptr, mcptr)) ||
+        (e = put(b->d_mcidx, mcidx)) || (e = put(b->d_mymap, mymap)))
         return bhip(h, e, "BBD plan upload");
+    if (b->mfma_kp > 0) {
+        const int64_t bsp = (bs + 15) / 16 * 16, nt = bsp / 16;
+        if ((e = b->d_mcp.ensure(b->mfma_kp * bsp)) || (e = b->d_myp.ensure(b->mfma_kp * bsp)) ||
+            (e = b->d_mpart.ensure((size_t)b->mfma_chunks * nt * nt * 256)))
+            return bhip(h, e, "BBD MFMA operands");
+    }
     if ((e = b->d_ax.ensure(b->nnz + 1)) || (e = b->d_bax.ensure(bax + 1)) ||
         (e = b->d_vals.ensure(vals + 1)) || (e = b->d_wpool.ensure(wp + 1)) ||
         (e = b->d_cval.ensure(cidx.size() + 1)) || (e = b->d_rval.ensure(ridx.size() + 1)) ||
@@ -583,11 +648,22 @@ int32_t bbd_factor(rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* 
                                   kThreads, s)))
                 return bhip(h, e, "BBD Schur solves");
         }
-        if ((e = launch_gather_sum(b->d_base.p, b->d_ax.p, b->d_btp.p, b->d_bti.p, b->ncells, s)) ||
-            (e = launch_bbd_schur(b->d_s.p, b->d_cellr.p, b->d_cellc.p, b->d_base.p, b->d_crow.p,
-                                  b->d_cblk.p, b->d_ccol.p, b->d_cval.p, b->d_pof.p, b->d_poff.p,
-                                  b->d_y2.p, bs, b->ncells, s)))
+        b->mfma_used = b->dense && b->mfma_kp > 0 &&
+                       (b->mfma_mode == 2 || (b->mfma_mode == 1 && b->mfma_dense));
+        if ((e = launch_gather_sum(b->d_base.p, b->d_ax.p, b->d_btp.p, b->d_bti.p, b->ncells, s)))
             return bhip(h, e, "BBD Schur assembly");
+        const int ps = prof_begin(PK_BBD_SCHUR, s);
+        if (b->mfma_used)
+            e = launch_bbd_schur_mfma(b->d_s.p, b->d_base.p, b->d_mcp.p, b->d_myp.p, b->d_mpart.p,
+                                      b->d_ax.p, b->d_mcptr.p, b->d_mcidx.p, b->d_mymap.p,
+                                      b->d_y2.p, (int32_t)bs, b->mfma_kp, b->mfma_chunks,
+                                      b->mfma_kc, s);
+        else
+            e = launch_bbd_schur(b->d_s.p, b->d_cellr.p, b->d_cellc.p, b->d_base.p, b->d_crow.p,
+                                 b->d_cblk.p, b->d_ccol.p, b->d_cval.p, b->d_pof.p, b->d_poff.p,
+                                 b->d_y2.p, bs, b->ncells, s);
+        prof_end(ps, s);
+        if (e) return bhip(h, e, "BBD Schur assembly");
         if (b->dense) {
             int32_t st[2] = {0, -1};
             if ((e = launch_dense_lu(b->d_s.p, b->d_piv.p, b->d_status.p, bs, 1, s)) ||
@@ -722,6 +798,17 @@ int32_t bbd_info(const rs_solver* h, int64_t* out) {
     out[1] = b->has_decomp ? b->d.border_size : 0;
     out[2] = b->has_decomp ? (b->dense ? 1 : 0) : -1;
     out[3] = h->factor_count;
+    return RS_OK;
+}
+
+void bbd_set_mfma(rs_solver* h, int32_t mode) { h->bbd->mfma_mode = mode < 0 ? 0 : mode > 2 ? 2 : mode; }
+
+int32_t bbd_mfma_info(const rs_solver* h, int64_t* out) {
+    const BbdImpl* b = h->bbd;
+    out[0] = b->mfma_mode;
+    out[1] = b->has_decomp && b->mfma_used ? 1 : 0;
+    out[2] = b->has_decomp ? b->mfma_k : 0;
+    out[3] = b->has_decomp ? b->mfma_permille : 0;
     return RS_OK;
 }
 

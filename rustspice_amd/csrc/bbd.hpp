@@ -59,6 +59,10 @@ int32_t bbd_solve(::rs_solver* h, double* rhs);
 void bbd_reset(::rs_solver* h);
 void bbd_set_target(::rs_solver* h, int64_t num_blocks);
 int32_t bbd_info(const ::rs_solver* h, int64_t* out);
+// Dense Schur product on FP64 MFMA tiles: mode 0 off, 1 where the packed
+// operands are dense enough, 2 always (dense Schur only).
+void bbd_set_mfma(::rs_solver* h, int32_t mode);
+int32_t bbd_mfma_info(const ::rs_solver* h, int64_t* out);
 // The same on device-resident values: a SparseLU fallback factors and solves
 // them on the device; a real decomposition copies them to the host first.
 int32_t bbd_factor_dev(::rs_solver* h, const int64_t* ap, int64_t apl, const int64_t* ai, int64_t ail,
@@ -84,6 +88,13 @@ hipError_t launch_bbd_schur(double* out, const int32_t* cell_row, const int32_t*
                             const int32_t* c_col, const double* c_val, const int32_t* pair_of,
                             const int32_t* pair_off, const double* y, int64_t bs, int64_t ncells,
                             hipStream_t s);
+// S = base - Cp Yp on FP64 MFMA tiles (packing, split-K tiles, ordered sum
+// of the chunks); kp and kc multiples of 16, nchunk * kc >= kp.
+hipError_t launch_bbd_schur_mfma(double* out, const double* base, double* cp, double* yp,
+                                 double* part, const double* ax, const int32_t* cptr,
+                                 const int32_t* cidx, const int32_t* ymap, const double* y,
+                                 int32_t bs, int64_t kp, int32_t nchunk, int64_t kc,
+                                 hipStream_t s);
 hipError_t launch_bbd_border_rhs(double* bb, const double* work, int64_t b0,
                                  const int32_t* crow_ptr, const int32_t* c_blk,
                                  const int32_t* c_col, const double* c_val,

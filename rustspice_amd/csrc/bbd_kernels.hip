@@ -1,6 +1,7 @@
 // bbd_kernels.hip - device kernels of the BbdSolver and DenseSolver back ends
 // (bbd.hpp). All sums keep the reference's operand order per output entry;
-// parallelism is across entries only.
+// parallelism is across entries only. The one exception is the opt-in dense
+// Schur product on MFMA tiles (k_bbd_schur_mfma), which sums per tile.
 #include <hip/hip_runtime.h>
 
 #include "bbd.hpp"
@@ -63,6 +64,75 @@ __global__ void k_bbd_schur(double* __restrict__ out, const int32_t* __restrict_
         acc = acc - c_val[e] * yv;
     }
     out[i] = acc;
+}
+
+// ---- Dense Schur product on FP64 MFMA tiles --------------------------------
+// S = B_T - Cp * Yp, where the K "coupling columns" are the (block, local
+// column) pairs that hold a C entry. Cp (K x bsp) holds, at (c, r), the sum
+// of border row r's C entries in coupling column c; Yp (K x bsp) holds, at
+// (c, j), Y_kj(col) of that column (0 where the pair (k, j) has no R entry).
+// Both are k-major with a row stride of bsp = border size rounded up to 16,
+// zero in the padding, and K is padded to a multiple of 16. The product's
+// terms are summed per 16x16 tile and K chunk by the matrix cores, then the
+// chunks in chunk order: deterministic, but not the reference's term order.
+__global__ void k_bbd_pack(double* __restrict__ cp, double* __restrict__ yp,
+                           const double* __restrict__ ax, const int32_t* __restrict__ cptr,
+                           const int32_t* __restrict__ cidx, const int32_t* __restrict__ ymap,
+                           const double* __restrict__ y, int64_t m) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    double s = 0.0;
+    for (int32_t q = cptr[t]; q < cptr[t + 1]; ++q) s += ax[cidx[q]];
+    cp[t] = s;
+    const int32_t o = ymap[t];
+    yp[t] = o >= 0 ? y[o] : 0.0;
+}
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+// Grid (tiles, chunks), 256 threads. Workgroup (t, s) multiplies tile t =
+// (tr, tc) of the output over K rows [s*kc, min(kp, (s+1)*kc)); wavefront w
+// takes the 4-row steps w*4, w*4+16, ... of that range. kc and kp are
+// multiples of 16, so every step's 4 rows lie below kp. v_mfma_f64_16x16x4f64
+// operands: lane l holds A(i = l%16, k = l/16) and B(k = l/16, j = l%16); the
+// result's element v of lane l is D(l/16 + 4*v, l%16) (rows interleaved by
+// four, unlike the f32 forms). The four
+// wavefronts' tiles are added in wavefront order into part[s][t][16x16].
+__global__ __launch_bounds__(256) void k_bbd_schur_mfma(double* __restrict__ part,
+                                                        const double* __restrict__ cp,
+                                                        const double* __restrict__ yp, int32_t nt,
+                                                        int32_t bsp, int64_t kp, int64_t kc) {
+    __shared__ double red[4][256];
+    const int32_t t = blockIdx.x, tr = t / nt, tc = t % nt;
+    const int32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t kb = (int64_t)blockIdx.y * kc;
+    const int64_t ke = kb + kc < kp ? kb + kc : kp;
+    const int32_t ri = tr * 16 + (l & 15), cj = tc * 16 + (l & 15);
+    dx4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = kb + w * 4; k0 < ke; k0 += 16) {
+        const int64_t k = k0 + (l >> 4);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cp[k * bsp + ri], yp[k * bsp + cj], acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; ++v) red[w][((l >> 4) + 4 * v) * 16 + (l & 15)] = acc[v];
+    __syncthreads();
+    const int32_t e = threadIdx.x;
+    const double s = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+    part[((int64_t)blockIdx.y * gridDim.x + t) * 256 + e] = s;
+}
+
+// out[r * bs + j] = base - sum over chunks (in chunk order) of the partial
+// products of cell (r, j).
+__global__ void k_bbd_schur_mfma_out(double* __restrict__ out, const double* __restrict__ base,
+                                     const double* __restrict__ part, int32_t bs, int32_t nt,
+                                     int32_t nchunk) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bs * bs) return;
+    const int32_t r = i / bs, j = i % bs;
+    const int32_t t = (r >> 4) * nt + (j >> 4), e = (r & 15) * 16 + (j & 15);
+    const int64_t stride = (int64_t)nt * nt * 256;
+    double acc = 0.0;
+    for (int32_t c = 0; c < nchunk; ++c) acc += part[c * stride + (int64_t)t * 256 + e];
+    out[i] = base[i] - acc;
 }
 
 // Border right-hand side b_T - sum_k C_k z_k (matvec_add_neg,
@@ -251,6 +321,24 @@ hipError_t launch_bbd_schur(double* out, const int32_t* cell_row, const int32_t*
     if (ncells <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bbd_schur, grid_for(ncells), dim3(kT), 0, s, out, cell_row, cell_col, base,
                        crow_ptr, c_blk, c_col, c_val, pair_of, pair_off, y, bs, ncells);
+    return hipGetLastError();
+}
+
+hipError_t launch_bbd_schur_mfma(double* out, const double* base, double* cp, double* yp,
+                                 double* part, const double* ax, const int32_t* cptr,
+                                 const int32_t* cidx, const int32_t* ymap, const double* y,
+                                 int32_t bs, int64_t kp, int32_t nchunk, int64_t kc,
+                                 hipStream_t s) {
+    if (bs <= 0) return hipSuccess;
+    const int32_t bsp = (bs + 15) / 16 * 16, nt = bsp / 16;
+    if (kp <= 0 || kp % 16 || kc % 16 || nchunk < 1 || (int64_t)nchunk * kc < kp)
+        return hipErrorInvalidValue;
+    const int64_t m = kp * bsp;
+    hipLaunchKernelGGL(k_bbd_pack, grid_for(m), dim3(kT), 0, s, cp, yp, ax, cptr, cidx, ymap, y, m);
+    hipLaunchKernelGGL(k_bbd_schur_mfma, dim3(nt * nt, nchunk), dim3(256), 0, s, part, cp, yp, nt,
+                       bsp, kp, kc);
+    hipLaunchKernelGGL(k_bbd_schur_mfma_out, grid_for((int64_t)bs * bs), dim3(kT), 0, s, out, base,
+                       part, bs, nt, nchunk);
     return hipGetLastError();
 }
 

@@ -988,6 +988,15 @@ int32_t rs_bbd_info(const rs_solver* h, int64_t* out) {
     return bbd_info(h, out);
 }
 
+void rs_set_bbd_mfma(rs_solver* h, int32_t mode) {
+    if (h->bbd) bbd_set_mfma(h, mode);
+}
+
+int32_t rs_bbd_mfma_info(const rs_solver* h, int64_t* out) {
+    if (!h->bbd) return RS_ERR_INVALID_MATRIX;
+    return bbd_mfma_info(h, out);
+}
+
 void rs_set_parallel_threads(rs_solver* h, int64_t t) { h->threads_hint = t; }
 
 void rs_set_growth_check(rs_solver* h, double limit) { h->growth_limit = limit > 0.0 ? limit : 0.0; }

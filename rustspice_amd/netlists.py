@@ -593,3 +593,55 @@ def ac_network(seed: int, nodes: int = 12, with_nonlinear: bool = True):
         add(RS_DEV_M, (nodes - 1, 2, 0, 0), 0.0, mos_params(0.4, 500.0, 1e-6, 1e-7))
         add(RS_DEV_M, (nodes - 1, 2, 1, 1), 0.0, mos_params(-0.4, 250.0, 2e-6, 1e-7), flags=RS_F_PMOS)
     return b.build(nodes + 1, f"ac{seed}"), np.array(mag), np.array(ph)
+
+
+def tap_bank_matrix(width: int = 2048, depth: int = 4, rails: int = 32, seed: int = 0x52530000 + 6):
+    """Symmetric conductance matrix of two tap banks joined by a rail bus,
+    shaped so that BbdSolver(2) (bbd.rs bfs_bisect) cuts exactly the bus:
+    border = 2 * rails rows, each rail tapping every cell of its bank's edge
+    row, so the border's coupling blocks C_k and Y_k = B_kk^-1 R_k are half
+    dense (the dense Schur case, border <= 64, for rails <= 32).
+
+    Nodes: 0 a lead of degree 1 (the search starts there), 1 the bank's feed,
+    then bank A as `width` chains of `depth` cells (layer t is one search
+    level), rails H_A (each tapping all of A's last layer), rails H_B (H_A[i]
+    to H_B[i]), bank B (H_B taps its first layer), its feed and its lead.
+    The half split lands between H_A and H_B because each side holds
+    width * depth + 2 nodes. Branch conductances are uniform in [0.5, 2] S and
+    every node has 0.01 S to ground. Returns (n, ap, ai, ax) in CSC with
+    sorted rows and the diagonal present."""
+    rs = np.random.RandomState(seed)
+    W, D, w = int(width), int(depth), int(rails)
+    a_lead, a_feed = 0, 1
+    a0 = 2
+    ha = a0 + W * D
+    hb = ha + w
+    b0 = hb + w
+    b_feed = b0 + W * D
+    b_lead = b_feed + 1
+    n = b_lead + 1
+    edges = [(a_lead, a_feed)]
+    edges += [(a_feed, a0 + i) for i in range(W)]
+    for t in range(D - 1):
+        edges += [(a0 + t * W + i, a0 + (t + 1) * W + i) for i in range(W)]
+    for r in range(w):
+        edges += [(a0 + (D - 1) * W + i, ha + r) for i in range(W)]
+        edges.append((ha + r, hb + r))
+        edges += [(hb + r, b0 + i) for i in range(W)]
+    for t in range(D - 1):
+        edges += [(b0 + t * W + i, b0 + (t + 1) * W + i) for i in range(W)]
+    edges += [(b0 + (D - 1) * W + i, b_feed) for i in range(W)]
+    edges.append((b_feed, b_lead))
+    e = np.array(edges, np.int64)
+    g = rs.uniform(0.5, 2.0, len(e))
+    diag = np.full(n, 0.01)
+    np.add.at(diag, e[:, 0], g)
+    np.add.at(diag, e[:, 1], g)
+    rows = np.concatenate([e[:, 0], e[:, 1], np.arange(n)])
+    cols = np.concatenate([e[:, 1], e[:, 0], np.arange(n)])
+    vals = np.concatenate([-g, -g, diag])
+    order = np.lexsort((rows, cols))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    ap = np.zeros(n + 1, np.int64)
+    np.add.at(ap, cols + 1, 1)
+    return n, np.cumsum(ap), rows, vals

@@ -115,6 +115,18 @@ class GpuSolver:
         return dict(blocks=int(out[0]), border=int(out[1]), schur=int(out[2]),
                     factor_count=int(out[3]))
 
+    def set_bbd_mfma(self, mode: int):
+        """Dense Schur product on FP64 MFMA tiles (BBD kind): 0 off (the
+        reference's term order, bit-exact), 1 where the packed operands are
+        dense enough, 2 always. Agrees with the reference to rounding only."""
+        self._L.rs_set_bbd_mfma(self.h, mode)
+
+    def bbd_mfma_info(self):
+        out = np.zeros(4, np.int64)
+        self._check(self._L.rs_bbd_mfma_info(self.h, out))
+        return dict(mode=int(out[0]), used=bool(out[1]), k=int(out[2]),
+                    c_density=out[3] / 1000.0)
+
     def name(self) -> str:
         return self._L.rs_solver_name(self.h).decode()
 

@@ -107,3 +107,93 @@ def test_gpu_dense_solver_bit_exact(gpu):
     g = S.GpuSolver(2, S.DENSE)  # a zero column fails like the reference (FactorFailed)
     with pytest.raises(S.SolverError):
         g.factor(np.array([0, 2, 2]), np.array([0, 1]), np.array([1.0, 2.0]))
+
+
+# ---- dense Schur product on FP64 MFMA tiles ---------------------------------
+TAP_BANKS = [(256, 3, 20), (64, 2, 8), (512, 2, 32)]  # border 40 (padded to 48), 16, 64
+
+
+@pytest.mark.parametrize("shape", TAP_BANKS)
+def test_tap_bank_border_is_the_rail_bus(shape):
+    """The generator's claim the MFMA tests rest on: BbdSolver(2)'s bisection
+    (the oracle's and the library's alike) cuts exactly the two rail rows."""
+    W, D, w = shape
+    n, ap, ai, ax = NL.tap_bank_matrix(W, D, w)
+    d = S.host_bbd_decompose(n, ap, ai, 2)
+    o = O.bbd_decompose(n, ap, ai, 2)
+    assert np.array_equal(d["perm"], o["perm"]) and d["border_start"] == o["border_start"]
+    assert d["num_blocks"] == 2 and n - d["border_start"] == 2 * w
+    hub = set(range(2 + W * D, 2 + W * D + 2 * w))
+    assert set(int(v) for v in d["perm"][d["border_start"]:]) == hub
+
+
+def _rel_inf(x, ref):
+    return float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+
+
+MFMA_TOL = 1e-9  # relative (inf-norm) to the reference's term order
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", TAP_BANKS)
+def test_gpu_bbd_mfma_schur_matches_oracle(gpu, shape):
+    """Tiles on (mode 2, and mode 1's density rule, which takes them here):
+    x within 1e-9 of the oracle's BbdSolver, through a refactor with new
+    values; the default (mode 0) stays bit-exact on the same solver."""
+    W, D, w = shape
+    n, ap, ai, ax = NL.tap_bank_matrix(W, D, w)
+    rs = np.random.RandomState(31)
+    o = O.Bbd(n, 2)
+    o.prepare(n)
+    for mode in (2, 1, 0):
+        g = S.GpuSolver(n, S.BBD)
+        g.set_bbd_blocks(2)
+        g.set_bbd_mfma(mode)
+        for vals in (ax, ax * (1.0 + 0.01 * rs.randn(len(ax)))):
+            assert o.factor(ap, ai, vals) == 0
+            g.factor(ap, ai, vals)
+            info = g.bbd_mfma_info()
+            assert info["k"] == 2 * W and abs(info["c_density"] - 0.5) < 1e-3
+            big = W >= 64  # mode 1 needs K >= 64
+            assert info["used"] == (mode == 2 or (mode == 1 and big)), info
+            b = rs.randn(n)
+            xo, xg = b.copy(), b.copy()
+            assert o.solve(xo) == 0
+            g.solve(xg)
+            if info["used"]:
+                assert _rel_inf(xg, xo) <= MFMA_TOL, (mode, _rel_inf(xg, xo))
+            else:
+                assert same_bits(xg, xo)
+
+
+@pytest.mark.gpu
+def test_gpu_bbd_mfma_forced_on_every_dense_schur_case(gpu):
+    """Mode 2 on every dense-Schur case of the bit-exact suite (tiny K, one
+    padded tile, random patterns): within 1e-9 of the oracle; mode 1 takes
+    the tiles only where its density rule says so, else stays bit-exact."""
+    rs = np.random.RandomState(37)
+    forced = 0
+    for nb in (2, 8):
+        for n, ap, ai, ax in _cases():
+            o = O.Bbd(n, nb)
+            o.prepare(n)
+            assert o.factor(ap, ai, ax) == 0
+            if o.info()["schur"] != 1:
+                continue
+            for mode in (2, 1):
+                g = S.GpuSolver(n, S.BBD)
+                g.set_bbd_blocks(nb)
+                g.set_bbd_mfma(mode)
+                g.factor(ap, ai, ax)
+                info = g.bbd_mfma_info()
+                b = rs.randn(n)
+                xo, xg = b.copy(), b.copy()
+                assert o.solve(xo) == 0
+                g.solve(xg)
+                if mode == 1 and not info["used"]:
+                    assert same_bits(xg, xo), (n, nb, info)
+                elif info["k"] > 0:
+                    assert info["used"]
+                    forced += 1
+                    assert _rel_inf(xg, xo) <= MFMA_TOL, (n, nb, _rel_inf(xg, xo))
+    assert forced >= 4
