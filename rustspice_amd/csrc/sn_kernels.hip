@@ -615,7 +615,9 @@ __global__ __launch_bounds__(kT) void k_sn_bwd_inner(SnPtrs p, const int32_t* __
 // barrier pair: wavefront 0 finishes the block's own rows (lane r - t0 holds
 // row r, the finished value of a row passed to the others by shuffle), then
 // every thread applies the block's columns to its rows outside the block.
-// Each row still takes its terms one column at a time in the order above.
+// Each row still takes its terms one column at a time in the order above;
+// a row's kIB multipliers of the block are loaded into registers before its
+// first subtraction, so the loads overlap instead of waiting one by one.
 constexpr int kIB = 16;
 
 __global__ __launch_bounds__(kT) void k_sn_fwd_inner_lds(SnPtrs p, const int32_t* __restrict__ list,
@@ -632,18 +634,37 @@ __global__ __launch_bounds__(kT) void k_sn_fwd_inner_lds(SnPtrs p, const int32_t
         const int t1 = min(t0 + kIB, w);
         if (tid < 64) {
             const int r = t0 + tid;
+            double l[kIB];
+#pragma unroll
+            for (int k = 0; k < kIB; k++) l[k] = (t0 + k < r && r < t1) ? LP[(int64_t)(t0 + k) * W + r] : 0.0;
             double yr = r < t1 ? ys[r] : 0.0;
-            for (int t = t0; t < t1 - 1; t++) {
-                const double yt = __shfl(yr, t - t0);
-                if (r > t && r < t1) yr -= LP[(int64_t)t * W + r] * yt;
+#pragma unroll
+            for (int k = 0; k < kIB - 1; k++) {  // t = t0 + k < t1 - 1 is uniform
+                const int t = t0 + k;
+                if (t < t1 - 1) {
+                    const double yt = __shfl(yr, k);
+                    if (r > t && r < t1) yr -= l[k] * yt;
+                }
             }
             if (r < t1) ys[r] = yr;
         }
         __syncthreads();
-        for (int r = t1 + tid; r < w; r += kT) {
-            double yr = ys[r];
-            for (int t = t0; t < t1; t++) yr -= LP[(int64_t)t * W + r] * ys[t];
-            ys[r] = yr;
+        if (t1 - t0 == kIB) {
+            for (int r = t1 + tid; r < w; r += kT) {
+                double l[kIB];
+#pragma unroll
+                for (int k = 0; k < kIB; k++) l[k] = LP[(int64_t)(t0 + k) * W + r];
+                double yr = ys[r];
+#pragma unroll
+                for (int k = 0; k < kIB; k++) yr -= l[k] * ys[t0 + k];
+                ys[r] = yr;
+            }
+        } else {
+            for (int r = t1 + tid; r < w; r += kT) {
+                double yr = ys[r];
+                for (int t = t0; t < t1; t++) yr -= LP[(int64_t)t * W + r] * ys[t];
+                ys[r] = yr;
+            }
         }
         __syncthreads();
     }
@@ -664,19 +685,39 @@ __global__ __launch_bounds__(kT) void k_sn_bwd_inner_lds(SnPtrs p, const int32_t
         const int t0 = max(t1 - kIB, 0);
         if (tid < 64) {
             const int r = t0 + tid;
+            double l[kIB];
+#pragma unroll
+            for (int k = 0; k < kIB; k++) l[k] = (r < t0 + k && t0 + k < t1) ? LP[(int64_t)(t0 + k) * W + r] : 0.0;
+            const double d = r < t1 ? p.ud[jj0 + r] : 1.0;
             double yr = r < t1 ? ys[r] : 0.0;
-            for (int t = t1 - 1; t >= t0; t--) {
-                if (r == t) yr = yr / p.ud[jj0 + t];
-                const double xt = __shfl(yr, t - t0);
-                if (r < t) yr -= LP[(int64_t)t * W + r] * xt;
+#pragma unroll
+            for (int k = kIB - 1; k >= 0; k--) {  // t = t0 + k < t1 is uniform
+                const int t = t0 + k;
+                if (t < t1) {
+                    if (r == t) yr = yr / d;
+                    const double xt = __shfl(yr, k);
+                    if (r < t) yr -= l[k] * xt;
+                }
             }
             if (r < t1) ys[r] = yr;
         }
         __syncthreads();
-        for (int r = tid; r < t0; r += kT) {
-            double yr = ys[r];
-            for (int t = t1 - 1; t >= t0; t--) yr -= LP[(int64_t)t * W + r] * ys[t];
-            ys[r] = yr;
+        if (t1 - t0 == kIB) {
+            for (int r = tid; r < t0; r += kT) {
+                double l[kIB];
+#pragma unroll
+                for (int k = 0; k < kIB; k++) l[k] = LP[(int64_t)(t0 + k) * W + r];
+                double yr = ys[r];
+#pragma unroll
+                for (int k = kIB - 1; k >= 0; k--) yr -= l[k] * ys[t0 + k];
+                ys[r] = yr;
+            }
+        } else {
+            for (int r = tid; r < t0; r += kT) {
+                double yr = ys[r];
+                for (int t = t1 - 1; t >= t0; t--) yr -= LP[(int64_t)t * W + r] * ys[t];
+                ys[r] = yr;
+            }
         }
         __syncthreads();
     }
